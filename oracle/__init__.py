@@ -1,0 +1,160 @@
+"""ORACLE — test infrastructure only.
+
+ctypes binding of the CPU restatement in ``oracle/*.cpp``.  Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` use
+this module, and only as the checker: the product library never loads it.
+
+The library is built by ``oracle/Makefile`` into ``oracle/_build/liboracle.so``
+(``__graft_entry__.build()`` runs the recipe).  If it is missing, ``load()``
+builds it once with ``make``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    lib.orc_last_error.restype = C.c_char_p
+    vp = C.c_void_p
+    lib.orc2d_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _ip, _ip, C.c_int, _dp, _dp, _dp,
+                                 C.c_int, _ip, _ip, C.c_int, C.c_double, C.POINTER(vp)]
+    lib.orc2d_destroy.argtypes = [vp]
+    lib.orc2d_nglob.argtypes = [vp]
+    lib.orc2d_numbering.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.orc2d_geometry.argtypes = [vp] + [C.c_void_p] * 6
+    lib.orc2d_mass_inverse.argtypes = [vp, _dp]
+    lib.orc2d_quadrature.argtypes = [vp, _dp, _dp, _dp]
+    lib.orc2d_locate.argtypes = [vp, C.c_double, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double)]
+    lib.orc2d_forward.argtypes = [vp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_double, C.c_int, C.c_int, _ip,
+                                  _dp, C.POINTER(C.c_double), C.c_int, C.c_void_p]
+    lib.orc2d_apply_stiffness.argtypes = [vp, _dp, _dp, _dp]
+    for name in ("orc3d_create", "orc3d_destroy"):
+        if hasattr(lib, name):
+            pass
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise RuntimeError(load().orc_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+SEISMO_TYPES = {"displacement": 0, "velocity": 1, "acceleration": 2}
+
+
+class Oracle2D:
+    """Assembly + forward solver of the restatement for one mesh.
+
+    ``mesh`` is a :class:`specfempp_amd.io.fortran_db.Mesh2D` or any object
+    with the same fields (``nspec, ngnod, npgeo, coorg (2, npgeo), knods
+    (ngnod, nspec), kmato, materials[i].{density,cp,cs}, abs_ispec,
+    abs_type``).  ``precision`` is 4 (type_real = float) or 8 (double).
+    """
+
+    def __init__(self, mesh, precision: int = 4, chunk: int = 1, dt: float = 0.0):
+        lib = load()
+        self.mesh = mesh
+        self.precision = precision
+        coorg = np.ascontiguousarray(np.asarray(mesh.coorg, dtype=np.float64).T)
+        knods = np.ascontiguousarray(np.asarray(mesh.knods, dtype=np.int32).T)
+        kmato = np.ascontiguousarray(mesh.kmato, dtype=np.int32)
+        rho = np.array([m.density for m in mesh.materials], dtype=np.float64)
+        cp = np.array([m.cp for m in mesh.materials], dtype=np.float64)
+        cs = np.array([m.cs for m in mesh.materials], dtype=np.float64)
+        abs_ispec = np.ascontiguousarray(mesh.abs_ispec, dtype=np.int32)
+        abs_type = np.ascontiguousarray(mesh.abs_type, dtype=np.int32)
+        h = C.c_void_p()
+        _check(lib.orc2d_create(precision, mesh.nspec, mesh.ngnod, mesh.npgeo, coorg, knods, kmato, len(rho), rho, cp,
+                                cs, len(abs_ispec), abs_ispec, abs_type, chunk, dt, C.byref(h)))
+        self.h = h
+        self.nspec = mesh.nspec
+        self.nglob = lib.orc2d_nglob(h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().orc2d_destroy(self.h)
+            self.h = None
+
+    def numbering(self):
+        ibool = np.zeros((self.nspec, 5, 5), dtype=np.int32)
+        midx = np.zeros(self.nglob, dtype=np.int32)
+        c2m = np.zeros(self.nspec, dtype=np.int32)
+        load().orc2d_numbering(self.h, _ptr(ibool), _ptr(midx), _ptr(c2m))
+        return ibool, midx, c2m
+
+    def geometry(self):
+        coord = np.zeros((self.nspec, 5, 5, 2))
+        arrs = [np.zeros((self.nspec, 5, 5)) for _ in range(5)]
+        load().orc2d_geometry(self.h, _ptr(coord), *[_ptr(a) for a in arrs])
+        return coord, arrs  # xix, xiz, gammax, gammaz, jacobian
+
+    def mass_inverse(self):
+        out = np.zeros((2, self.nglob))
+        load().orc2d_mass_inverse(self.h, out)
+        return out
+
+    def quadrature(self):
+        xi, w, hp = np.zeros(5), np.zeros(5), np.zeros(25)
+        load().orc2d_quadrature(self.h, xi, w, hp)
+        return xi, w, hp.reshape(5, 5)
+
+    def locate(self, x: float, z: float):
+        i, xi, g = C.c_int(), C.c_double(), C.c_double()
+        _check(load().orc2d_locate(self.h, x, z, C.byref(i), C.byref(xi), C.byref(g)))
+        return i.value, xi.value, g.value
+
+    def forward(self, sources, receivers, nstep: int, dt: float, nsb: int = 1,
+                types=("displacement",), nthreads: int = 1, return_fields: bool = False):
+        """Run the forward loop; returns (seismograms (nsig, ntypes, nrec, 2), t0[, fields])."""
+        src = np.zeros((len(sources), 10))
+        for i, s in enumerate(sources):
+            kind = 1 if s.get("type", "force") == "moment_tensor" else 0
+            src[i] = [kind, s["x"], s["z"], s.get("angle", 0.0), s.get("mxx", 0.0), s.get("mzz", 0.0),
+                      s.get("mxz", 0.0), s["f0"], s.get("tshift", 0.0), s["factor"]]
+        rec = np.zeros((len(receivers), 3))
+        for i, r in enumerate(receivers):
+            rec[i] = [r["x"], r["z"], r.get("angle", 0.0)]
+        tcodes = np.array([SEISMO_TYPES[t] for t in types], dtype=np.int32)
+        nsig = nstep // nsb if nstep % nsb == 0 else nstep // nsb + 1
+        seis = np.zeros((nsig, len(types), len(receivers), 2))
+        t0 = C.c_double()
+        fields = np.zeros((3, 2, self.nglob)) if return_fields else None
+        _check(load().orc2d_forward(self.h, len(sources), np.ascontiguousarray(src), len(receivers),
+                                    np.ascontiguousarray(rec), nstep, dt, nsb, len(types), tcodes, seis,
+                                    C.byref(t0), nthreads, _ptr(fields)))
+        if return_fields:
+            return seis, t0.value, fields
+        return seis, t0.value
+
+    def apply_stiffness(self, u, v):
+        out = np.zeros((2, self.nglob))
+        load().orc2d_apply_stiffness(self.h, np.ascontiguousarray(u, dtype=np.float64),
+                                     np.ascontiguousarray(v, dtype=np.float64), out)
+        return out

@@ -1,0 +1,1118 @@
+// ORACLE — test infrastructure only.
+//
+// CPU restatement of the SPECFEM++ 2-D elastic P-SV (isotropic) Newmark path,
+// used by tests/ and by bench.py's cpu_baseline leg as the checker.  It is
+// never linked into, loaded by or called from the product library.
+//
+// Every function follows the reference file:line it names.  The arithmetic
+// type T is the reference's `type_real` (float for the FP32 oracle, double for
+// the FP64 oracle, as with ENABLE_DOUBLE_PRECISION).  With one thread the loop
+// order is the reference's Serial-backend order (chunk size 1), including the
+// order in which contributions are summed into a global point.
+//
+// Pinning (checks that hold this restatement to the reference):
+//   - GLL nodes/weights/hprime: known answers of tests/unit-tests/gll and
+//     tests/unit-tests/lagrange (tests/test_oracle_quadrature.py).
+//   - mass matrix: SPECFEM2D rmass_inverse fixture, rel. L1 <= 1e-5
+//     (tests/unit-tests/domain/rmass_inverse_tests.cpp:229-234).
+//   - full forward run: SPECFEM2D traces, rel. L1 <= 1e-3
+//     (tests/unit-tests/displacement_tests/Newmark/newmark_tests.cpp:325-346),
+//     free-surface and Stacey fixtures.
+//   - numbering: invariant one-iglob-one-coordinate
+//     (tests/unit-tests/compute/index/compute_tests.cpp:70-129) and a
+//     brute-force coordinate grouping on small meshes.
+//   - stiffness: dense brute-force weak-form assembly on small meshes
+//     (symmetry, rigid-body null space, match of K u).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+constexpr int NGLL = 5;
+
+// ---------------------------------------------------------------------------
+// Quadrature: src/quadrature/gll/gll_library.cpp:11-33 (pnleg),
+// src/quadrature/gll/gll.cpp:56-62, lagrange_poly.cpp:12-43 (interpolants),
+// lagrange_poly.cpp:161-191 (hprime(i,j) = l_j'(xi_i)).
+// ---------------------------------------------------------------------------
+template <class T> T pnleg(T z, int n) {
+  T p1 = 1.0, p2 = z, p3 = p2;
+  for (int k = 1; k < n; k++) {
+    double dk = static_cast<double>(k);
+    p3 = ((2.0 * dk + 1.0) * z * p2 - dk * p1) / (dk + 1.0);
+    p1 = p2;
+    p2 = p3;
+  }
+  return p3;
+}
+
+template <class T> T pndleg(T z, int n) {
+  // derivative of the Legendre polynomial (gll_library.cpp:53-78 restated by
+  // the recurrence P'_{k+1} = (2k+1) P_k + P'_{k-1}).
+  T p1 = 1.0, p2 = z, p1d = 0.0, p2d = 1.0, p3d = 1.0;
+  for (int k = 1; k < n; k++) {
+    double dk = static_cast<double>(k);
+    T p3 = ((2.0 * dk + 1.0) * z * p2 - dk * p1) / (dk + 1.0);
+    p3d = ((2.0 * dk + 1.0) * p2 + (2.0 * dk + 1.0) * z * p2d - dk * p1d) / (dk + 1.0);
+    p1 = p2;
+    p2 = p3;
+    p1d = p2d;
+    p2d = p3d;
+  }
+  return p3d;
+}
+
+template <class T> struct Quadrature {
+  T xi[NGLL], w[NGLL], hprime[NGLL][NGLL];  // hprime[i][j] = l_j'(xi_i)
+  T hprime_wgll[NGLL][NGLL];                // [a][b] = hprime[b][a] * w[b]
+};
+
+// GLL nodes and weights for alpha = beta = 0: interior nodes are the roots of
+// P'_{N-1}; weights 2 / (N (N-1) P_{N-1}(x)^2).  Computed in double by Newton
+// iteration and rounded to T (the reference's zwgljd, gll_library.cpp:102-148,
+// converges to the same roots).
+template <class T> Quadrature<T> make_quadrature() {
+  Quadrature<T> q;
+  const int n = NGLL - 1;
+  double x[NGLL];
+  x[0] = -1.0;
+  x[n] = 1.0;
+  for (int i = 1; i < n; i++) {
+    double z = -std::cos(M_PI * i / n);  // Chebyshev-Gauss-Lobatto guess
+    for (int it = 0; it < 100; it++) {
+      // Newton on P'_n using P''_n from the Legendre ODE:
+      // (1 - z^2) P'' = 2 z P' - n (n+1) P
+      double p = pnleg<double>(z, n), dp = pndleg<double>(z, n);
+      double d2p = (2.0 * z * dp - n * (n + 1.0) * p) / (1.0 - z * z);
+      double dz = dp / d2p;
+      z -= dz;
+      if (std::abs(dz) < 1e-17) break;
+    }
+    x[i] = z;
+  }
+  if (NGLL % 2) x[n / 2] = 0.0;
+  for (int i = 0; i < NGLL; i++) {
+    double p = pnleg<double>(x[i], n);
+    q.xi[i] = static_cast<T>(x[i]);
+    q.w[i] = static_cast<T>(2.0 / (n * (n + 1.0) * p * p));
+  }
+  const int degpoly = n;
+  for (int i = 0; i < NGLL; i++)
+    for (int j = 0; j < NGLL; j++) {
+      T v;
+      if (i == 0 && j == 0)
+        v = -1.0 * static_cast<T>(degpoly) * (static_cast<T>(degpoly) + 1.0) * 0.25;
+      else if (i == degpoly && j == degpoly)
+        v = 1.0 * static_cast<T>(degpoly) * (static_cast<T>(degpoly) + 1.0) * 0.25;
+      else if (i == j)
+        v = 0.0;
+      else
+        v = pnleg<T>(q.xi[i], degpoly) /
+                (pnleg<T>(q.xi[j], degpoly) * (q.xi[i] - q.xi[j])) +
+            (1.0 - q.xi[i] * q.xi[i]) * pndleg<T>(q.xi[i], degpoly) /
+                (static_cast<T>(degpoly) * (static_cast<T>(degpoly) + 1.0) *
+                 pnleg<T>(q.xi[j], degpoly) * (q.xi[i] - q.xi[j]) * (q.xi[i] - q.xi[j]));
+      q.hprime[i][j] = v;
+    }
+  // include/compute/compute_mesh.hpp:266-272
+  for (int iz = 0; iz < NGLL; iz++)
+    for (int ix = 0; ix < NGLL; ix++) q.hprime_wgll[ix][iz] = q.hprime[iz][ix] * q.w[iz];
+  return q;
+}
+
+// lagrange_poly.cpp:12-43
+template <class T> void lagrange_interpolants(T xi, const T *xigll, T *h, T *hprime) {
+  for (int dgr = 0; dgr < NGLL; dgr++) {
+    T prod1 = 1.0, prod2 = 1.0, prod3, sum = 0.0;
+    T x0 = xigll[dgr];
+    for (int i = 0; i < NGLL; i++) {
+      if (i != dgr) {
+        T x = xigll[i];
+        prod1 = prod1 * (xi - x);
+        prod2 = prod2 * (x0 - x);
+        prod3 = 1.0;
+        for (int j = 0; j < NGLL; j++)
+          if (j != dgr && j != i) prod3 = prod3 * (xi - xigll[j]);
+        sum = sum + prod3;
+      }
+    }
+    T prod2_inv = 1.0 / prod2;
+    h[dgr] = prod1 * prod2_inv;
+    if (hprime) hprime[dgr] = sum * prod2_inv;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Shape functions: src/jacobian/shape_functions.cpp:17-152
+// ---------------------------------------------------------------------------
+template <class T> void shape2d(T xi, T gamma, int ngnod, T *s) {
+  const T one = 1.0, two = 2.0, quarter = 0.25, half = 0.5;
+  const T sp = xi + one, sm = xi - one, tp = gamma + one, tm = gamma - one;
+  if (ngnod == 4) {
+    s[0] = quarter * sm * tm;
+    s[1] = -quarter * sp * tm;
+    s[2] = quarter * sp * tp;
+    s[3] = -quarter * sm * tp;
+  } else if (ngnod == 9) {
+    const T ss = xi * xi, tt = gamma * gamma, st = xi * gamma;
+    (void)two;
+    s[0] = quarter * sm * st * tm;
+    s[1] = quarter * sp * st * tm;
+    s[2] = quarter * sp * st * tp;
+    s[3] = quarter * sm * st * tp;
+    s[4] = half * tm * gamma * (one - ss);
+    s[5] = half * sp * xi * (one - tt);
+    s[6] = half * tp * gamma * (one - ss);
+    s[7] = half * sm * xi * (one - tt);
+    s[8] = (one - ss) * (one - tt);
+  } else {
+    throw std::invalid_argument("wrong number of control nodes");
+  }
+}
+
+template <class T> void dshape2d(T xi, T gamma, int ngnod, T *d0, T *d1) {
+  const T one = 1.0, two = 2.0, quarter = 0.25, half = 0.5;
+  const T sp = xi + one, sm = xi - one, tp = gamma + one, tm = gamma - one;
+  if (ngnod == 4) {
+    d0[0] = quarter * tm;
+    d0[1] = -quarter * tm;
+    d0[2] = quarter * tp;
+    d0[3] = -quarter * tp;
+    d1[0] = quarter * sm;
+    d1[1] = -quarter * sp;
+    d1[2] = quarter * sp;
+    d1[3] = -quarter * sm;
+  } else if (ngnod == 9) {
+    const T s2 = xi * two, t2 = gamma * two, ss = xi * xi, tt = gamma * gamma, st = xi * gamma;
+    d0[0] = quarter * tm * gamma * (s2 - one);
+    d0[1] = quarter * tm * gamma * (s2 + one);
+    d0[2] = quarter * tp * gamma * (s2 + one);
+    d0[3] = quarter * tp * gamma * (s2 - one);
+    d1[0] = quarter * sm * xi * (t2 - one);
+    d1[1] = quarter * sp * xi * (t2 - one);
+    d1[2] = quarter * sp * xi * (t2 + one);
+    d1[3] = quarter * sm * xi * (t2 + one);
+    d0[4] = -one * st * tm;
+    d0[5] = half * (one - tt) * (s2 + one);
+    d0[6] = -one * st * tp;
+    d0[7] = half * (one - tt) * (s2 - one);
+    d1[4] = half * (one - ss) * (t2 - one);
+    d1[5] = -one * st * sp;
+    d1[6] = half * (one - ss) * (t2 + one);
+    d1[7] = -one * st * sm;
+    d0[8] = -one * s2 * (one - tt);
+    d1[8] = -one * t2 * (one - ss);
+  } else {
+    throw std::invalid_argument("wrong number of control nodes");
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Mesh input (mesh order) and assembled state (compute order)
+// ---------------------------------------------------------------------------
+enum Edge { BOTTOM = 0, RIGHT = 1, TOP = 2, LEFT = 3 };
+
+struct MeshIn {
+  int nspec, ngnod, npgeo;
+  const double *coorg;  // (npgeo, 2)
+  const int *knods;     // (nspec, ngnod)
+  const int *kmato;     // (nspec)
+  int nmat;
+  const double *rho, *cp, *cs;  // per material
+  int nabs;
+  const int *abs_ispec, *abs_type;  // mesh element and Edge of each absorbing edge
+};
+
+template <class T> struct Assembly {
+  int nspec = 0, ngnod = 0, nglob = 0, chunk = 1;
+  Quadrature<T> q;
+  std::vector<int> compute_to_mesh, mesh_to_compute;
+  std::vector<int> is_stacey;        // per compute element
+  int nspec_none = 0;                // elements [0, nspec_none) have no BC
+  std::vector<T> ctrl;               // (ispec, in, 2) control node coordinates
+  std::vector<int> ibool;            // (ispec, iz, ix) global index
+  std::vector<int> medium_index;     // iglob -> per-medium index
+  std::vector<T> coord;              // (ispec, iz, ix, 2) point coordinates
+  std::vector<T> xix, xiz, gammax, gammaz, jac;  // (ispec, iz, ix)
+  std::vector<T> kappa, mu, rho;     // (ispec, iz, ix)
+  // Stacey: per point of a Stacey element; tag 1 if the point is on an
+  // absorbing edge (compute/boundaries/impl/stacey.cpp:140-180)
+  std::vector<int> bnd_tag;          // (ispec, iz, ix), 0 for non-Stacey elements
+  std::vector<T> bnd_nx, bnd_nz, bnd_w;
+  std::vector<T> mass_inverse;       // (comp, nglob) LayoutLeft on medium index
+};
+
+// src/compute/compute_mesh.cpp:19-43 (note: evaluated on the *sorted* points,
+// and numeric_limits::min() is the smallest positive value, both as in the
+// reference).
+template <class T> struct QP {
+  T x, z;
+  int iloc, iglob;
+};
+
+template <class T>
+T get_tolerance(const std::vector<QP<T>> &c, int nspec, int ngllxz) {
+  T xtypdist = std::numeric_limits<T>::max();
+  for (int ispec = 0; ispec < nspec; ispec++) {
+    T xmax = std::numeric_limits<T>::min(), xmin = std::numeric_limits<T>::max();
+    T ymax = std::numeric_limits<T>::min(), ymin = std::numeric_limits<T>::max();
+    for (int xz = 0; xz < ngllxz; xz++) {
+      int iloc = ispec * ngllxz + xz;
+      xmax = std::max(xmax, c[iloc].x);
+      xmin = std::min(xmin, c[iloc].x);
+      ymax = std::max(ymax, c[iloc].z);
+      ymin = std::min(ymin, c[iloc].z);
+    }
+    xtypdist = std::min(xtypdist, xmax - xmin);
+    xtypdist = std::min(xtypdist, ymax - ymin);
+  }
+  return 1e-6 * xtypdist;
+}
+
+// src/compute/compute_mesh.cpp:45-170 (assign_numbering) with a selectable
+// storage chunk size (include/parallel_configuration/chunk_config.hpp:12-28).
+template <class T>
+void assign_numbering(Assembly<T> &a, const std::vector<double> &gcoord /*(ispec,iz,ix,2)*/) {
+  const int nspec = a.nspec, ngll = NGLL, ngllxz = NGLL * NGLL, chunk = a.chunk;
+  std::vector<QP<T>> cc(static_cast<size_t>(nspec) * ngllxz);
+  int iloc = 0;
+  for (int ichunk = 0; ichunk < nspec; ichunk += chunk)
+    for (int iz = 0; iz < ngll; iz++)
+      for (int ix = 0; ix < ngll; ix++)
+        for (int ie = 0; ie < chunk; ie++) {
+          int ispec = ichunk + ie;
+          if (ispec >= nspec) break;
+          size_t g = ((static_cast<size_t>(ispec) * ngll + iz) * ngll + ix) * 2;
+          cc[iloc].x = static_cast<T>(gcoord[g]);
+          cc[iloc].z = static_cast<T>(gcoord[g + 1]);
+          cc[iloc].iloc = iloc;
+          cc[iloc].iglob = 0;
+          iloc++;
+        }
+  std::sort(cc.begin(), cc.end(), [](const QP<T> &p, const QP<T> &q) {
+    if (p.x != q.x) return p.x < q.x;
+    return p.z < q.z;
+  });
+  int ig = 0;
+  cc[0].iglob = ig;
+  T xtol = get_tolerance(cc, nspec, ngllxz);
+  for (size_t i = 1; i < cc.size(); i++) {
+    if ((std::abs(cc[i].x - cc[i - 1].x) > xtol) || (std::abs(cc[i].z - cc[i - 1].z) > xtol)) ig++;
+    cc[i].iglob = ig;
+  }
+  std::vector<QP<T>> orig(cc.size());
+  for (size_t i = 0; i < cc.size(); i++) orig[cc[i].iloc] = cc[i];
+  const int nglob = ig + 1;
+  std::vector<int> counted(nglob, -1);
+  a.ibool.assign(static_cast<size_t>(nspec) * ngllxz, -1);
+  a.coord.assign(static_cast<size_t>(nspec) * ngllxz * 2, 0);
+  iloc = 0;
+  int inum = 0;
+  for (int ichunk = 0; ichunk < nspec; ichunk += chunk)
+    for (int iz = 0; iz < ngll; iz++)
+      for (int ix = 0; ix < ngll; ix++)
+        for (int ie = 0; ie < chunk; ie++) {
+          int ispec = ichunk + ie;
+          if (ispec >= nspec) break;
+          size_t p = (static_cast<size_t>(ispec) * ngll + iz) * ngll + ix;
+          if (counted[orig[iloc].iglob] == -1) {
+            counted[orig[iloc].iglob] = inum;
+            a.ibool[p] = inum++;
+          } else {
+            a.ibool[p] = counted[orig[iloc].iglob];
+          }
+          a.coord[2 * p] = orig[iloc].x;
+          a.coord[2 * p + 1] = orig[iloc].z;
+          iloc++;
+        }
+  if (inum != nglob) throw std::runtime_error("numbering: count mismatch");
+  a.nglob = nglob;
+}
+
+// Per-medium numbering, include/compute/fields/impl/field_impl.tpp:39-62
+// (loop order chunk, ix, iz, element).
+template <class T> void medium_numbering(Assembly<T> &a) {
+  a.medium_index.assign(a.nglob, -1);
+  int count = 0;
+  for (int ichunk = 0; ichunk < a.nspec; ichunk += a.chunk)
+    for (int ix = 0; ix < NGLL; ix++)
+      for (int iz = 0; iz < NGLL; iz++)
+        for (int ie = 0; ie < a.chunk; ie++) {
+          int ispec = ichunk + ie;
+          if (ispec >= a.nspec) break;
+          int g = a.ibool[(static_cast<size_t>(ispec) * NGLL + iz) * NGLL + ix];
+          if (a.medium_index[g] == -1) a.medium_index[g] = count++;
+        }
+  if (count != a.nglob) throw std::runtime_error("medium numbering: count mismatch");
+}
+
+template <class T> inline size_t pidx(int ispec, int iz, int ix) {
+  return (static_cast<size_t>(ispec) * NGLL + iz) * NGLL + ix;
+}
+
+// Stacey point term, include/boundary_conditions/stacey/stacey.tpp:36-72.
+template <class T>
+inline void stacey_traction(T nx, T nz, T w, T kappa, T mu, T rho, T vx, T vz, T &tx, T &tz) {
+  const T vn = vx * nx + vz * nz;
+  const T jacobian1d = std::sqrt(nx * nx + nz * nz);
+  const T lp2m = kappa + static_cast<T>(4.0 / 3.0) * mu;
+  const T rho_vp = std::sqrt(lp2m / rho) * rho;
+  const T rho_vs = std::sqrt(rho * mu);
+  const T f0 = ((vn * nx / (jacobian1d * jacobian1d)) * (rho_vp - rho_vs)) + vx * rho_vs;
+  const T f1 = ((vn * nz / (jacobian1d * jacobian1d)) * (rho_vp - rho_vs)) + vz * rho_vs;
+  tx += static_cast<T>(-1.0) * f0 * jacobian1d * w;
+  tz += static_cast<T>(-1.0) * f1 * jacobian1d * w;
+}
+
+template <class T> Assembly<T> *build_assembly(const MeshIn &m, int chunk, double dt) {
+  auto *a = new Assembly<T>();
+  a->nspec = m.nspec;
+  a->ngnod = m.ngnod;
+  a->chunk = chunk;
+  a->q = make_quadrature<T>();
+  const int nspec = m.nspec;
+  for (int i = 0; i < m.nmat; i++)
+    if (!(m.cs[i] > 0.0)) throw std::runtime_error("oracle: only elastic P-SV isotropic materials");
+
+  // Element boundary tags (src/mesh/dim2/tags/tags.cpp) and compute ordering
+  // (src/compute/compute_mesh.cpp:240-279): elastic/isotropic/none first, then
+  // elastic/isotropic/stacey, each in mesh order.
+  std::vector<int> stacey_mesh(nspec, 0);
+  for (int i = 0; i < m.nabs; i++) stacey_mesh[m.abs_ispec[i]] = 1;
+  a->compute_to_mesh.clear();
+  for (int pass = 0; pass < 2; pass++)
+    for (int e = 0; e < nspec; e++)
+      if (stacey_mesh[e] == pass) a->compute_to_mesh.push_back(e);
+  a->mesh_to_compute.assign(nspec, 0);
+  a->is_stacey.assign(nspec, 0);
+  for (int c = 0; c < nspec; c++) {
+    a->mesh_to_compute[a->compute_to_mesh[c]] = c;
+    a->is_stacey[c] = stacey_mesh[a->compute_to_mesh[c]];
+  }
+  a->nspec_none = 0;
+  for (int c = 0; c < nspec; c++) a->nspec_none += a->is_stacey[c] ? 0 : 1;
+
+  // Control nodes in compute order (compute_mesh.cpp:172-197), as type_real.
+  a->ctrl.resize(static_cast<size_t>(nspec) * m.ngnod * 2);
+  for (int c = 0; c < nspec; c++)
+    for (int in = 0; in < m.ngnod; in++) {
+      int node = m.knods[static_cast<size_t>(a->compute_to_mesh[c]) * m.ngnod + in];
+      a->ctrl[(static_cast<size_t>(c) * m.ngnod + in) * 2] = static_cast<T>(m.coorg[2 * node]);
+      a->ctrl[(static_cast<size_t>(c) * m.ngnod + in) * 2 + 1] = static_cast<T>(m.coorg[2 * node + 1]);
+    }
+
+  // GLL point coordinates (compute_mesh.cpp:299-333): float products summed in
+  // double.
+  std::vector<double> gcoord(static_cast<size_t>(nspec) * NGLL * NGLL * 2);
+  std::vector<T> sh(m.ngnod);
+  for (int c = 0; c < nspec; c++)
+    for (int iz = 0; iz < NGLL; iz++)
+      for (int ix = 0; ix < NGLL; ix++) {
+        shape2d<T>(a->q.xi[ix], a->q.xi[iz], m.ngnod, sh.data());
+        double xc = 0.0, zc = 0.0;
+        for (int in = 0; in < m.ngnod; in++) {
+          xc += a->ctrl[(static_cast<size_t>(c) * m.ngnod + in) * 2] * sh[in];
+          zc += a->ctrl[(static_cast<size_t>(c) * m.ngnod + in) * 2 + 1] * sh[in];
+        }
+        gcoord[pidx<T>(c, iz, ix) * 2] = xc;
+        gcoord[pidx<T>(c, iz, ix) * 2 + 1] = zc;
+      }
+  assign_numbering(*a, gcoord);
+  medium_numbering(*a);
+
+  // Partial derivatives (src/compute/compute_partial_derivatives.cpp:46-91,
+  // src/jacobian/jacobian.cpp:232-257).
+  const size_t np = static_cast<size_t>(nspec) * NGLL * NGLL;
+  a->xix.resize(np); a->xiz.resize(np); a->gammax.resize(np); a->gammaz.resize(np); a->jac.resize(np);
+  std::vector<T> d0(m.ngnod), d1(m.ngnod);
+  for (int c = 0; c < nspec; c++)
+    for (int iz = 0; iz < NGLL; iz++)
+      for (int ix = 0; ix < NGLL; ix++) {
+        dshape2d<T>(a->q.xi[ix], a->q.xi[iz], m.ngnod, d0.data(), d1.data());
+        T xxi = 0.0, zxi = 0.0, xgamma = 0.0, zgamma = 0.0;
+        for (int in = 0; in < m.ngnod; in++) {
+          const T cx = a->ctrl[(static_cast<size_t>(c) * m.ngnod + in) * 2];
+          const T cz = a->ctrl[(static_cast<size_t>(c) * m.ngnod + in) * 2 + 1];
+          xxi += d0[in] * cx;
+          zxi += d0[in] * cz;
+          xgamma += d1[in] * cx;
+          zgamma += d1[in] * cz;
+        }
+        const T jacobian = xxi * zgamma - xgamma * zxi;
+        if (jacobian < static_cast<T>(1e-10))
+          throw std::runtime_error("small jacobian");  // compute_partial_derivatives.cpp:114-150
+        size_t p = pidx<T>(c, iz, ix);
+        a->xix[p] = zgamma / jacobian;
+        a->gammax[p] = -zxi / jacobian;
+        a->xiz[p] = -xgamma / jacobian;
+        a->gammaz[p] = xxi / jacobian;
+        a->jac[p] = jacobian;
+      }
+
+  // Properties per point (include/medium/dim2/elastic/isotropic/material.hpp:62-77,126).
+  a->kappa.resize(np); a->mu.resize(np); a->rho.resize(np);
+  for (int c = 0; c < nspec; c++) {
+    const int mat = m.kmato[a->compute_to_mesh[c]];
+    const T density = static_cast<T>(m.rho[mat]);
+    const T cs = static_cast<T>(m.cs[mat]);
+    const T cp = static_cast<T>(m.cp[mat]);
+    const T mu = density * cs * cs;
+    const T kappa = density * (cp * cp - (4.0 / 3.0) * cs * cs);
+    for (int k = 0; k < NGLL * NGLL; k++) {
+      a->kappa[static_cast<size_t>(c) * NGLL * NGLL + k] = kappa;
+      a->mu[static_cast<size_t>(c) * NGLL * NGLL + k] = mu;
+      a->rho[static_cast<size_t>(c) * NGLL * NGLL + k] = density;
+    }
+  }
+
+  // Stacey edge data (src/compute/boundaries/impl/stacey.cpp:64-181).  The
+  // absorbing-edge list of the mesh never contains corners
+  // (src/io/mesh/impl/fortran/dim2/read_boundaries.cpp:10-96 finds none), and
+  // at a point shared by two listed edges of one element the last edge wins.
+  a->bnd_tag.assign(np, 0);
+  a->bnd_nx.assign(np, 0);
+  a->bnd_nz.assign(np, 0);
+  a->bnd_w.assign(np, 0);
+  std::map<int, std::vector<int>> by_elem;
+  for (int i = 0; i < m.nabs; i++) by_elem[a->mesh_to_compute[m.abs_ispec[i]]].push_back(i);
+  for (auto &kv : by_elem) {
+    const int c = kv.first;
+    for (int i : kv.second) {
+      const int type = m.abs_type[i];
+      for (int iz = 0; iz < NGLL; iz++)
+        for (int ix = 0; ix < NGLL; ix++) {
+          bool on = (type == TOP && iz == NGLL - 1) || (type == BOTTOM && iz == 0) ||
+                    (type == LEFT && ix == 0) || (type == RIGHT && ix == NGLL - 1);
+          if (!on) continue;
+          size_t p = pidx<T>(c, iz, ix);
+          a->bnd_tag[p] = 1;
+          const T J = a->jac[p];
+          T nx, nz, w;
+          if (type == LEFT) {
+            nx = static_cast<T>(-1.0) * a->xix[p] * J;
+            nz = static_cast<T>(-1.0) * a->xiz[p] * J;
+            w = a->q.w[iz];
+          } else if (type == RIGHT) {
+            nx = a->xix[p] * J;
+            nz = a->xiz[p] * J;
+            w = a->q.w[iz];
+          } else if (type == TOP) {
+            nx = a->gammax[p] * J;
+            nz = a->gammaz[p] * J;
+            w = a->q.w[ix];
+          } else {
+            nx = static_cast<T>(-1.0) * a->gammax[p] * J;
+            nz = static_cast<T>(-1.0) * a->gammaz[p] * J;
+            w = a->q.w[ix];
+          }
+          a->bnd_nx[p] = nx;
+          a->bnd_nz[p] = nz;
+          a->bnd_w[p] = w;
+        }
+    }
+  }
+
+  // Mass matrix (include/kokkos_kernels/impl/compute_mass_matrix.tpp:80-114,
+  // include/medium/dim2/elastic/isotropic/mass_matrix.tpp:5-16, Stacey term
+  // stacey.tpp:641-677), then inversion (invert_mass_matrix.tpp:40-47).
+  const int nglob = a->nglob;
+  std::vector<T> mass(2 * static_cast<size_t>(nglob), 0);
+  const T velocity_factor = static_cast<T>(-1.0) * static_cast<T>(dt) * static_cast<T>(0.5);
+  for (int c = 0; c < nspec; c++)
+    for (int iz = 0; iz < NGLL; iz++)
+      for (int ix = 0; ix < NGLL; ix++) {
+        size_t p = pidx<T>(c, iz, ix);
+        T m0 = a->rho[p], m1 = a->rho[p];
+        const T f = a->q.w[ix] * a->q.w[iz] * a->jac[p];
+        m0 *= f;
+        m1 *= f;
+        if (a->bnd_tag[p])
+          stacey_traction<T>(a->bnd_nx[p], a->bnd_nz[p], a->bnd_w[p], a->kappa[p], a->mu[p],
+                             a->rho[p], velocity_factor, velocity_factor, m0, m1);
+        const int g = a->medium_index[a->ibool[p]];
+        mass[g] += m0;
+        mass[static_cast<size_t>(nglob) + g] += m1;
+      }
+  a->mass_inverse.resize(2 * static_cast<size_t>(nglob));
+  for (size_t i = 0; i < mass.size(); i++) a->mass_inverse[i] = static_cast<T>(1.0) / mass[i];
+  return a;
+}
+
+// ---------------------------------------------------------------------------
+// locate_point: src/algorithms/locate_point.cpp:8-176
+// ---------------------------------------------------------------------------
+template <class T> struct Local {
+  int ispec;
+  T xi, gamma;
+};
+
+template <class T> Local<T> locate_point(const Assembly<T> &a, T gx, T gz) {
+  const int nspec = a.nspec, ngnod = a.ngnod;
+  // rough_location (locate_point.cpp:8-40)
+  T dist_min = std::numeric_limits<T>::max();
+  int ispec_g = 0, ix_g = 0, iz_g = 0;
+  for (int ispec = 0; ispec < nspec; ispec++)
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        size_t p = pidx<T>(ispec, j, i);
+        const T dx = gx - a.coord[2 * p], dz = gz - a.coord[2 * p + 1];
+        const T d = std::sqrt(dx * dx + dz * dz);
+        if (d < dist_min) {
+          ispec_g = ispec;
+          ix_g = i;
+          iz_g = j;
+          dist_min = d;
+        }
+      }
+  // get_best_candidates (locate_point.cpp:42-77)
+  auto ib = [&](int e, int j, int i) { return a.ibool[pidx<T>(e, j, i)]; };
+  std::vector<int> iglob_guess = {ib(ispec_g, 0, 0), ib(ispec_g, NGLL - 1, 0), ib(ispec_g, 0, NGLL - 1),
+                                  ib(ispec_g, NGLL - 1, NGLL - 1)};
+  std::vector<int> cand = {ispec_g};
+  for (int e = 0; e < nspec; e++) {
+    if (e == ispec_g) continue;
+    for (int j : {0, NGLL - 1})
+      for (int i : {0, NGLL - 1})
+        if (std::find(iglob_guess.begin(), iglob_guess.end(), ib(e, j, i)) != iglob_guess.end())
+          if (!cand.empty() && cand.back() != e) cand.push_back(e);
+  }
+  // get_best_location + selection (locate_point.cpp:79-176)
+  std::vector<T> sh(ngnod), d0(ngnod), d1(ngnod);
+  auto locations = [&](int e, T xi, T gamma, T &x, T &z) {
+    shape2d<T>(xi, gamma, ngnod, sh.data());
+    x = 0.0;
+    z = 0.0;
+    for (int in = 0; in < ngnod; in++) {
+      x += sh[in] * a.ctrl[(static_cast<size_t>(e) * ngnod + in) * 2];
+      z += sh[in] * a.ctrl[(static_cast<size_t>(e) * ngnod + in) * 2 + 1];
+    }
+  };
+  T final_dist = std::numeric_limits<T>::max();
+  Local<T> out{0, 0, 0};
+  for (int e : cand) {
+    T xi = a.q.xi[ix_g], gamma = a.q.xi[iz_g];
+    for (int it = 0; it < 100; it++) {
+      T x, z;
+      locations(e, xi, gamma, x, z);
+      dshape2d<T>(xi, gamma, ngnod, d0.data(), d1.data());
+      T xxi = 0.0, zxi = 0.0, xgamma = 0.0, zgamma = 0.0;
+      for (int in = 0; in < ngnod; in++) {
+        const T cx = a.ctrl[(static_cast<size_t>(e) * ngnod + in) * 2];
+        const T cz = a.ctrl[(static_cast<size_t>(e) * ngnod + in) * 2 + 1];
+        xxi += d0[in] * cx;
+        zxi += d0[in] * cz;
+        xgamma += d1[in] * cx;
+        zgamma += d1[in] * cz;
+      }
+      const T jacobian = xxi * zgamma - xgamma * zxi;
+      const T xix = zgamma / jacobian, gammax = -zxi / jacobian;
+      const T xiz = -xgamma / jacobian, gammaz = xxi / jacobian;
+      const T dx = -(x - gx), dz = -(z - gz);
+      xi += xix * dx + xiz * dz;
+      gamma += gammax * dx + gammaz * dz;
+      if (xi > 1.01) xi = 1.01;
+      if (xi < -1.01) xi = -1.01;
+      if (gamma > 1.01) gamma = 1.01;
+      if (gamma < -1.01) gamma = -1.01;
+    }
+    T x, z;
+    locations(e, xi, gamma, x, z);
+    const T dx = gx - x, dz = gz - z;
+    const T dist = std::sqrt(dx * dx + dz * dz);
+    if (dist < final_dist) {
+      out = {e, xi, gamma};
+      final_dist = dist;
+    }
+    ix_g = static_cast<int>(NGLL / 2.0);
+    iz_g = static_cast<int>(NGLL / 2.0);
+  }
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// Time loop: include/solver/time_marching.tpp:9-124 (forward, elastic only),
+// Newmark include/timescheme/newmark.tpp:11-116, kernels
+// include/kokkos_kernels/domain_kernels.hpp:69-126.
+// ---------------------------------------------------------------------------
+struct SourceIn {
+  int type;     // 0 force, 1 moment tensor
+  double x, z;
+  double angle;                 // force
+  double mxx, mzz, mxz;         // moment tensor
+  double f0, tshift, factor;    // Ricker
+};
+
+struct ReceiverIn {
+  double x, z, angle;
+};
+
+template <class T> struct Run {
+  const Assembly<T> *a;
+  std::vector<T> u, v, acc;  // (comp, nglob) on the per-medium index
+};
+
+// Ricker, src/source_time_function/ricker.cpp:39-50 and utilities.cpp:27-32.
+template <class T> T ricker(T t, T f0, T factor) {
+  const T pi = static_cast<T>(M_PI);
+  const T a = pi * pi * f0 * f0;
+  const T d2 = (1.0 - 2.0 * a * t * t) * std::exp(-a * t * t);
+  return -1.0 * factor * d2;
+}
+
+// One element's weak-form internal force (stiffness kernel body,
+// include/kokkos_kernels/impl/compute_stiffness_interaction.tpp:143-259):
+// gradient (include/algorithms/gradient.hpp:81-103), stress
+// (include/medium/dim2/elastic/isotropic/stress.hpp:25-48), F = J sigma d
+// (core/specfem/point/stress.hpp:118-139), divergence
+// (include/algorithms/divergence.hpp:86-98), negation, Stacey.
+template <class T>
+void element_force(const Assembly<T> &a, int e, const T *ux, const T *uz, const T *vx_g, const T *vz_g,
+                   T out[NGLL][NGLL][2]) {
+  const auto &q = a.q;
+  T lx[NGLL][NGLL], lz[NGLL][NGLL];
+  for (int iz = 0; iz < NGLL; iz++)
+    for (int ix = 0; ix < NGLL; ix++) {
+      const int g = a.medium_index[a.ibool[pidx<T>(e, iz, ix)]];
+      lx[iz][ix] = ux[g];
+      lz[iz][ix] = uz[g];
+    }
+  T F[NGLL][NGLL][2][2];
+  for (int iz = 0; iz < NGLL; iz++)
+    for (int ix = 0; ix < NGLL; ix++) {
+      T dxi[2] = {0.0, 0.0}, dgam[2] = {0.0, 0.0};
+      for (int l = 0; l < NGLL; l++) {
+        dxi[0] += q.hprime[ix][l] * lx[iz][l];
+        dgam[0] += q.hprime[iz][l] * lx[l][ix];
+        dxi[1] += q.hprime[ix][l] * lz[iz][l];
+        dgam[1] += q.hprime[iz][l] * lz[l][ix];
+      }
+      const size_t p = pidx<T>(e, iz, ix);
+      const T xix = a.xix[p], xiz = a.xiz[p], gammax = a.gammax[p], gammaz = a.gammaz[p], J = a.jac[p];
+      T du[2][2];
+      for (int c = 0; c < 2; c++) {
+        du[c][0] = xix * dxi[c] + gammax * dgam[c];
+        du[c][1] = xiz * dxi[c] + gammaz * dgam[c];
+      }
+      const T mu = a.mu[p];
+      const T lp2m = a.kappa[p] + static_cast<T>(4.0 / 3.0) * mu;
+      const T lambda = lp2m - static_cast<T>(2.0) * mu;
+      const T sxx = lp2m * du[0][0] + lambda * du[1][1];
+      const T szz = lp2m * du[1][1] + lambda * du[0][0];
+      const T sxz = mu * (du[0][1] + du[1][0]);
+      const T Tm[2][2] = {{sxx, sxz}, {sxz, szz}};
+      for (int c = 0; c < 2; c++) {
+        F[iz][ix][c][0] = J * (Tm[c][0] * xix + Tm[c][1] * xiz);
+        F[iz][ix][c][1] = J * (Tm[c][0] * gammax + Tm[c][1] * gammaz);
+      }
+    }
+  for (int iz = 0; iz < NGLL; iz++)
+    for (int ix = 0; ix < NGLL; ix++) {
+      T t1[2] = {0.0, 0.0}, t2[2] = {0.0, 0.0};
+      for (int l = 0; l < NGLL; l++) {
+        for (int c = 0; c < 2; c++) t1[c] += F[iz][l][c][0] * q.hprime_wgll[ix][l];
+        for (int c = 0; c < 2; c++) t2[c] += F[l][ix][c][1] * q.hprime_wgll[iz][l];
+      }
+      T r[2];
+      for (int c = 0; c < 2; c++) r[c] = q.w[iz] * t1[c] + q.w[ix] * t2[c];
+      for (int c = 0; c < 2; c++) r[c] *= static_cast<T>(-1.0);
+      const size_t p = pidx<T>(e, iz, ix);
+      if (a.bnd_tag[p]) {
+        const int g = a.medium_index[a.ibool[p]];
+        stacey_traction<T>(a.bnd_nx[p], a.bnd_nz[p], a.bnd_w[p], a.kappa[p], a.mu[p], a.rho[p], vx_g[g],
+                           vz_g[g], r[0], r[1]);
+      }
+      out[iz][ix][0] = r[0];
+      out[iz][ix][1] = r[1];
+    }
+}
+
+// Forward simulation.  Seismogram output layout: (isig, itype, irec, 2) as
+// the reference's (max_sig_step, ntypes, nrec, 2) array before rotation
+// (include/kokkos_kernels/impl/compute_seismogram.tpp:103-135); types are
+// 0 displacement, 1 velocity, 2 acceleration.
+template <class T>
+void run_forward(const Assembly<T> &a, int nsrc, const SourceIn *src, int nrec, const ReceiverIn *rec, int nstep,
+                 double dt_in, int nsb, int ntypes, const int *types, double *seis_out, double *t0_out,
+                 int nthreads, T *final_fields /* optional (3, 2, nglob) */) {
+  const int nglob = a.nglob;
+  const T dt = static_cast<T>(dt_in);
+  const T deltat = dt, deltatover2 = dt / 2.0, deltasquareover2 = dt * dt / 2.0;
+
+  // Sources: t0 = min over sources of (-1.2/f0 + tshift), then each source's
+  // tshift becomes its t0 - min_t0 (src/io/sources.cpp:128-158,
+  // ricker.cpp:15-20, ricker.hpp:49).
+  T min_t0 = std::numeric_limits<T>::max();
+  std::vector<T> src_t0(nsrc);
+  for (int s = 0; s < nsrc; s++) {
+    T hdur = 1.0 / static_cast<T>(src[s].f0);
+    src_t0[s] = -1.2 * hdur + static_cast<T>(src[s].tshift);
+    if (src_t0[s] < min_t0) min_t0 = src_t0[s];
+  }
+  const T t0 = min_t0;
+  if (t0_out) *t0_out = t0;
+  struct Src {
+    int ispec;
+    T arr[2][NGLL][NGLL];
+    std::vector<T> stf;
+  };
+  std::vector<Src> S(nsrc);
+  for (int s = 0; s < nsrc; s++) {
+    auto loc = locate_point<T>(a, static_cast<T>(src[s].x), static_cast<T>(src[s].z));
+    S[s].ispec = loc.ispec;
+    T hx[NGLL], hpx[NGLL], hg[NGLL], hpg[NGLL];
+    lagrange_interpolants<T>(loc.xi, a.q.xi, hx, hpx);
+    lagrange_interpolants<T>(loc.gamma, a.q.xi, hg, hpg);
+    if (src[s].type == 0) {
+      // force source, src/source/force_source.cpp:70-78
+      const T ang = static_cast<T>(src[s].angle);
+      const T pi = static_cast<T>(M_PI);
+      for (int iz = 0; iz < NGLL; iz++)
+        for (int ix = 0; ix < NGLL; ix++) {
+          const T hl = hx[ix] * hg[iz];
+          S[s].arr[0][iz][ix] = std::sin(pi / 180 * ang) * hl;
+          S[s].arr[1][iz][ix] = -1.0 * std::cos(pi / 180 * ang) * hl;
+        }
+    } else {
+      // moment tensor source, src/source/moment_tensor_source.cpp:88-130:
+      // the GLL-point derivatives are interpolated to the source location
+      // (include/algorithms/interpolate.hpp), then M contracts the gradient of
+      // the Lagrange product.
+      const int e = loc.ispec;
+      T xixs = 0, xizs = 0, gammaxs = 0, gammazs = 0;
+      for (int iz = 0; iz < NGLL; iz++)
+        for (int ix = 0; ix < NGLL; ix++) {
+          const T hl = hx[ix] * hg[iz];
+          const size_t p = pidx<T>(e, iz, ix);
+          xixs += hl * a.xix[p];
+          gammaxs += hl * a.gammax[p];
+          xizs += hl * a.xiz[p];
+          gammazs += hl * a.gammaz[p];
+        }
+      const T Mxx = static_cast<T>(src[s].mxx), Mzz = static_cast<T>(src[s].mzz), Mxz = static_cast<T>(src[s].mxz);
+      for (int iz = 0; iz < NGLL; iz++)
+        for (int ix = 0; ix < NGLL; ix++) {
+          const T dsx = (hpx[ix] * xixs) * hg[iz] + hx[ix] * (hpg[iz] * gammaxs);
+          const T dsz = (hpx[ix] * xizs) * hg[iz] + hx[ix] * (hpg[iz] * gammazs);
+          S[s].arr[0][iz][ix] = Mxx * dsx + Mxz * dsz;
+          S[s].arr[1][iz][ix] = Mxz * dsx + Mzz * dsz;
+        }
+    }
+    S[s].stf.resize(nstep);
+    const T tshift = src_t0[s] - min_t0;
+    for (int i = 0; i < nstep; i++)
+      S[s].stf[i] = ricker<T>(t0 + i * dt - tshift, static_cast<T>(src[s].f0), static_cast<T>(src[s].factor));
+  }
+  // Source application order: element type (none, then Stacey), then source
+  // order (src/compute/compute_sources.cpp:232-270).
+  std::vector<int> src_order;
+  for (int pass = 0; pass < 2; pass++)
+    for (int s = 0; s < nsrc; s++)
+      if (a.is_stacey[S[s].ispec] == pass) src_order.push_back(s);
+
+  struct Rec {
+    int ispec;
+    T h[NGLL][NGLL];
+  };
+  std::vector<Rec> R(nrec);
+  for (int r = 0; r < nrec; r++) {
+    auto loc = locate_point<T>(a, static_cast<T>(rec[r].x), static_cast<T>(rec[r].z));
+    R[r].ispec = loc.ispec;
+    T hx[NGLL], hg[NGLL];
+    lagrange_interpolants<T>(loc.xi, a.q.xi, hx, nullptr);
+    lagrange_interpolants<T>(loc.gamma, a.q.xi, hg, nullptr);
+    for (int iz = 0; iz < NGLL; iz++)
+      for (int ix = 0; ix < NGLL; ix++) R[r].h[iz][ix] = hx[ix] * hg[iz];
+  }
+
+  std::vector<T> u(2 * static_cast<size_t>(nglob), 0), v(2 * static_cast<size_t>(nglob), 0),
+      acc(2 * static_cast<size_t>(nglob), 0);
+  T *ux = u.data(), *uz = u.data() + nglob, *vx = v.data(), *vz = v.data() + nglob;
+  T *ax = acc.data(), *az = acc.data() + nglob;
+  const T *minv = a.mass_inverse.data();
+
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+  const bool serial = nthreads <= 1;
+  int isig = 0;
+  for (int istep = 0; istep < nstep; istep++) {
+    // predictor (newmark.tpp:60-116)
+#pragma omp parallel for if (!serial) schedule(static)
+    for (int g = 0; g < nglob; g++) {
+      for (int c = 0; c < 2; c++) {
+        const size_t i = static_cast<size_t>(c) * nglob + g;
+        T add_u = 0;
+        add_u += deltat * v[i] + deltasquareover2 * acc[i];
+        T add_v = 0;
+        add_v += deltatover2 * acc[i];
+        u[i] += add_u;
+        v[i] += add_v;
+        acc[i] = 0;
+      }
+    }
+    // sources (compute_source_interaction.tpp:80-95)
+    for (int s : src_order) {
+      const T sv = S[s].stf[istep];
+      for (int iz = 0; iz < NGLL; iz++)
+        for (int ix = 0; ix < NGLL; ix++) {
+          const int g = a.medium_index[a.ibool[pidx<T>(S[s].ispec, iz, ix)]];
+          ax[g] += sv * S[s].arr[0][iz][ix];
+          az[g] += sv * S[s].arr[1][iz][ix];
+        }
+    }
+    // stiffness: elements in compute order (none, then Stacey)
+    if (serial) {
+      T out[NGLL][NGLL][2];
+      for (int e = 0; e < a.nspec; e++) {
+        element_force<T>(a, e, ux, uz, vx, vz, out);
+        for (int iz = 0; iz < NGLL; iz++)
+          for (int ix = 0; ix < NGLL; ix++) {
+            const int g = a.medium_index[a.ibool[pidx<T>(e, iz, ix)]];
+            ax[g] += out[iz][ix][0];
+            az[g] += out[iz][ix][1];
+          }
+      }
+    } else {
+#pragma omp parallel for schedule(static)
+      for (int e = 0; e < a.nspec; e++) {
+        T out[NGLL][NGLL][2];
+        element_force<T>(a, e, ux, uz, vx, vz, out);
+        for (int iz = 0; iz < NGLL; iz++)
+          for (int ix = 0; ix < NGLL; ix++) {
+            const int g = a.medium_index[a.ibool[pidx<T>(e, iz, ix)]];
+#pragma omp atomic
+            ax[g] += out[iz][ix][0];
+#pragma omp atomic
+            az[g] += out[iz][ix][1];
+          }
+      }
+    }
+    // divide by mass (divide_mass_matrix.tpp:42-49), corrector (newmark.tpp:11-56)
+#pragma omp parallel for if (!serial) schedule(static)
+    for (int g = 0; g < nglob; g++) {
+      for (int c = 0; c < 2; c++) {
+        const size_t i = static_cast<size_t>(c) * nglob + g;
+        acc[i] = acc[i] * minv[i];
+        T add_v = 0;
+        add_v += deltatover2 * acc[i];
+        v[i] += add_v;
+      }
+    }
+    // seismograms (time_marching.tpp:75-78; compute_seismogram.tpp:103-135;
+    // include/algorithms/interpolate.hpp:176-237)
+    if (istep % nsb == 0) {
+      for (int it = 0; it < ntypes; it++) {
+        const T *fx = types[it] == 0 ? ux : (types[it] == 1 ? vx : ax);
+        const T *fz = types[it] == 0 ? uz : (types[it] == 1 ? vz : az);
+        for (int r = 0; r < nrec; r++) {
+          T s0 = 0, s1 = 0;
+          for (int iz = 0; iz < NGLL; iz++)
+            for (int ix = 0; ix < NGLL; ix++) {
+              const int g = a.medium_index[a.ibool[pidx<T>(R[r].ispec, iz, ix)]];
+              s0 += R[r].h[iz][ix] * fx[g];
+              s1 += R[r].h[iz][ix] * fz[g];
+            }
+          const size_t o = ((static_cast<size_t>(isig) * ntypes + it) * nrec + r) * 2;
+          seis_out[o] = s0;
+          seis_out[o + 1] = s1;
+        }
+      }
+      isig++;
+    }
+  }
+  if (final_fields) {
+    std::copy(u.begin(), u.end(), final_fields);
+    std::copy(v.begin(), v.end(), final_fields + 2 * static_cast<size_t>(nglob));
+    std::copy(acc.begin(), acc.end(), final_fields + 4 * static_cast<size_t>(nglob));
+  }
+}
+
+// Apply the assembled stiffness operator once: out = sum_e (-K_e u) with the
+// Stacey term driven by v (used by the brute-force weak-form checks).
+template <class T> void apply_stiffness(const Assembly<T> &a, const T *u, const T *v, T *out) {
+  const int nglob = a.nglob;
+  std::fill(out, out + 2 * static_cast<size_t>(nglob), static_cast<T>(0));
+  T o[NGLL][NGLL][2];
+  for (int e = 0; e < a.nspec; e++) {
+    element_force<T>(a, e, u, u + nglob, v, v + nglob, o);
+    for (int iz = 0; iz < NGLL; iz++)
+      for (int ix = 0; ix < NGLL; ix++) {
+        const int g = a.medium_index[a.ibool[pidx<T>(e, iz, ix)]];
+        out[g] += o[iz][ix][0];
+        out[static_cast<size_t>(nglob) + g] += o[iz][ix][1];
+      }
+  }
+}
+
+}  // namespace orc
+
+// ---------------------------------------------------------------------------
+// C interface for ctypes (tests, smoke, bench cpu_baseline)
+// ---------------------------------------------------------------------------
+using namespace orc;
+
+namespace {
+thread_local std::string g_err;
+template <class F> int guard(F &&f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+struct Handle {
+  int precision;  // 4 or 8
+  void *a;
+};
+template <class T> Assembly<T> &asm_of(void *h) { return *static_cast<Assembly<T> *>(static_cast<Handle *>(h)->a); }
+}  // namespace
+
+#define DISPATCH(h, ...)                                       \
+  do {                                                         \
+    if (static_cast<Handle *>(h)->precision == 4) {            \
+      using T = float;                                         \
+      __VA_ARGS__;                                             \
+    } else {                                                   \
+      using T = double;                                        \
+      __VA_ARGS__;                                             \
+    }                                                          \
+  } while (0)
+
+extern "C" {
+
+const char *orc_last_error() { return g_err.c_str(); }
+
+int orc2d_create(int precision, int nspec, int ngnod, int npgeo, const double *coorg, const int *knods,
+                 const int *kmato, int nmat, const double *rho, const double *cp, const double *cs, int nabs,
+                 const int *abs_ispec, const int *abs_type, int chunk, double dt, void **out) {
+  return guard([&] {
+    MeshIn m{nspec, ngnod, npgeo, coorg, knods, kmato, nmat, rho, cp, cs, nabs, abs_ispec, abs_type};
+    auto *h = new Handle{precision, nullptr};
+    if (precision == 4)
+      h->a = build_assembly<float>(m, chunk, dt);
+    else if (precision == 8)
+      h->a = build_assembly<double>(m, chunk, dt);
+    else
+      throw std::invalid_argument("precision must be 4 or 8");
+    *out = h;
+  });
+}
+
+void orc2d_destroy(void *h) {
+  if (!h) return;
+  DISPATCH(h, delete &asm_of<T>(h));
+  delete static_cast<Handle *>(h);
+}
+
+int orc2d_nglob(void *h) {
+  int n = 0;
+  DISPATCH(h, n = asm_of<T>(h).nglob);
+  return n;
+}
+
+// ibool (nspec, 5, 5) compute order; medium index (nglob); compute_to_mesh (nspec)
+void orc2d_numbering(void *h, int *ibool, int *medium_index, int *compute_to_mesh) {
+  DISPATCH(h, {
+    auto &a = asm_of<T>(h);
+    if (ibool) std::copy(a.ibool.begin(), a.ibool.end(), ibool);
+    if (medium_index) std::copy(a.medium_index.begin(), a.medium_index.end(), medium_index);
+    if (compute_to_mesh) std::copy(a.compute_to_mesh.begin(), a.compute_to_mesh.end(), compute_to_mesh);
+  });
+}
+
+// point coordinates (nspec,5,5,2), jacobian and derivatives (nspec,5,5)
+void orc2d_geometry(void *h, double *coord, double *xix, double *xiz, double *gammax, double *gammaz,
+                    double *jac) {
+  DISPATCH(h, {
+    auto &a = asm_of<T>(h);
+    const size_t np = a.xix.size();
+    for (size_t i = 0; i < np; i++) {
+      if (coord) { coord[2 * i] = a.coord[2 * i]; coord[2 * i + 1] = a.coord[2 * i + 1]; }
+      if (xix) xix[i] = a.xix[i];
+      if (xiz) xiz[i] = a.xiz[i];
+      if (gammax) gammax[i] = a.gammax[i];
+      if (gammaz) gammaz[i] = a.gammaz[i];
+      if (jac) jac[i] = a.jac[i];
+    }
+  });
+}
+
+// mass inverse (2, nglob) on the per-medium index
+void orc2d_mass_inverse(void *h, double *out) {
+  DISPATCH(h, {
+    auto &a = asm_of<T>(h);
+    for (size_t i = 0; i < a.mass_inverse.size(); i++) out[i] = a.mass_inverse[i];
+  });
+}
+
+void orc2d_quadrature(void *h, double *xi, double *w, double *hprime) {
+  DISPATCH(h, {
+    auto &a = asm_of<T>(h);
+    for (int i = 0; i < NGLL; i++) {
+      xi[i] = a.q.xi[i];
+      w[i] = a.q.w[i];
+      for (int j = 0; j < NGLL; j++) hprime[i * NGLL + j] = a.q.hprime[i][j];
+    }
+  });
+}
+
+int orc2d_locate(void *h, double x, double z, int *ispec, double *xi, double *gamma) {
+  return guard([&] {
+    DISPATCH(h, {
+      auto loc = locate_point<T>(asm_of<T>(h), static_cast<T>(x), static_cast<T>(z));
+      *ispec = loc.ispec;
+      *xi = loc.xi;
+      *gamma = loc.gamma;
+    });
+  });
+}
+
+// sources: (nsrc, 10) rows [type, x, z, angle, mxx, mzz, mxz, f0, tshift, factor]
+// receivers: (nrec, 3) rows [x, z, angle]
+int orc2d_forward(void *h, int nsrc, const double *src, int nrec, const double *rec, int nstep, double dt, int nsb,
+                  int ntypes, const int *types, double *seis, double *t0, int nthreads, double *final_fields) {
+  return guard([&] {
+    std::vector<SourceIn> S(nsrc);
+    for (int s = 0; s < nsrc; s++) {
+      const double *r = src + 10 * s;
+      S[s] = SourceIn{static_cast<int>(r[0]), r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9]};
+    }
+    std::vector<ReceiverIn> R(nrec);
+    for (int i = 0; i < nrec; i++) R[i] = ReceiverIn{rec[3 * i], rec[3 * i + 1], rec[3 * i + 2]};
+    DISPATCH(h, {
+      auto &a = asm_of<T>(h);
+      std::vector<T> ff;
+      if (final_fields) ff.resize(6 * static_cast<size_t>(a.nglob));
+      run_forward<T>(a, nsrc, S.data(), nrec, R.data(), nstep, dt, nsb, ntypes, types, seis, t0, nthreads,
+                     final_fields ? ff.data() : nullptr);
+      if (final_fields)
+        for (size_t i = 0; i < ff.size(); i++) final_fields[i] = ff[i];
+    });
+  });
+}
+
+// out (2, nglob) = assembled -K u (+ Stacey driven by v); u, v (2, nglob)
+void orc2d_apply_stiffness(void *h, const double *u, const double *v, double *out) {
+  DISPATCH(h, {
+    auto &a = asm_of<T>(h);
+    const size_t n = 2 * static_cast<size_t>(a.nglob);
+    std::vector<T> uu(u, u + n), vv(v, v + n), oo(n);
+    apply_stiffness<T>(a, uu.data(), vv.data(), oo.data());
+    for (size_t i = 0; i < n; i++) out[i] = oo[i];
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+"""Write the reference's own test data for the 2-D elastic path as compact
+fixtures under tests/golden/.
+
+Run once in a container that has the reference checkout:
+
+    python tests/golden/make_fixtures.py /root/reference
+
+Only data files are taken (mesh databases, SPECFEM2D traces, index mapping and
+mass-matrix arrays, station and source parameters).  They come from
+PrincetonUniversity/SPECFEMPP ``tests/unit-tests`` (GPL-3.0) and are stored
+re-encoded (gzip / npz) with their provenance recorded in ``MANIFEST.json``.
+"""
+from __future__ import annotations
+
+import gzip
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from specfempp_amd.io.fortran_db import read_fortran_array, read_stations, read_trace  # noqa: E402
+
+UT = "tests/unit-tests"
+CASES = {
+    # case name: (fixture dir under tests/unit-tests, seismogram types, source positions)
+    "elastic_free_80x60": (
+        "displacement_tests/Newmark/serial/HomogeneousElasticDomain",
+        ["d", "v", "a"],
+        [dict(x=2500.0, z=2500.0, angle=0.0, f0=10.0, factor=1e10, tshift=0.0)],
+    ),
+    "elastic_stacey_80x60": (
+        "displacement_tests/Newmark/serial/HomogeneousElasticDomainStaceyDirichletBC",
+        ["d"],
+        [dict(x=x, z=z, angle=0.0, f0=10.0, factor=1e10, tshift=0.0)
+         for (x, z) in [(500.0, 500.0), (3500.0, 500.0), (500.0, 2500.0), (3500.0, 2500.0)]],
+    ),
+}
+
+
+def gz_copy(src: str, dst: str) -> None:
+    with open(src, "rb") as fi, open(dst, "wb") as raw:
+        with gzip.GzipFile(fileobj=raw, mode="wb", compresslevel=9, mtime=0) as fo:
+            fo.write(fi.read())
+
+
+def main(ref: str) -> None:
+    manifest = {}
+    for name, (rel, kinds, sources) in CASES.items():
+        src_dir = os.path.join(ref, UT, rel)
+        out = os.path.join(HERE, name)
+        os.makedirs(out, exist_ok=True)
+        gz_copy(os.path.join(src_dir, "database.bin"), os.path.join(out, "database.bin.gz"))
+        stations = read_stations(os.path.join(src_dir, "STATIONS"))
+        traces = {}
+        time = None
+        for k in kinds:
+            arr = np.zeros((len(stations), 2, 0))
+            rows = []
+            for (sta, net, _x, _z) in stations:
+                comps = []
+                for comp in ("BXX", "BXZ"):
+                    tr = read_trace(os.path.join(src_dir, "traces", f"{net}.{sta}.S2.{comp}.sem{k}"))
+                    if time is None:
+                        time = tr[:, 0]
+                    comps.append(tr[:, 1])
+                rows.append(comps)
+            arr = np.asarray(rows)  # (nrec, 2, nsamples)
+            traces[k] = arr
+        np.savez_compressed(os.path.join(out, "traces.npz"), time=time, **traces)
+        with open(os.path.join(out, "case.json"), "w") as fh:
+            json.dump({
+                "source_dir": f"{UT}/{rel}",
+                "dt": 1.1e-3, "nstep": 300, "nstep_between_samples": 1,
+                "stations": [dict(name=s, network=n, x=x, z=z) for (s, n, x, z) in stations],
+                "sources": sources, "trace_kinds": kinds, "tolerance": 1e-3,
+            }, fh, indent=1)
+        manifest[name] = f"{UT}/{rel}"
+
+    # Mass-matrix fixture (tests/unit-tests/domain/serial/test1).
+    rel = "domain/serial/test1"
+    src_dir = os.path.join(ref, UT, rel)
+    out = os.path.join(HERE, "mass_80x60")
+    os.makedirs(out, exist_ok=True)
+    gz_copy(os.path.join(src_dir, "database.bin"), os.path.join(out, "database.bin.gz"))
+    ibool = read_fortran_array(os.path.join(src_dir, "index_mapping.bin"), "<i4")
+    rmass = read_fortran_array(os.path.join(src_dir, "rmass_inverse_elastic.bin"), "<f8")
+    np.savez_compressed(os.path.join(out, "mass.npz"), index_mapping=ibool, rmass_inverse=rmass)
+    with open(os.path.join(out, "case.json"), "w") as fh:
+        json.dump({"source_dir": f"{UT}/{rel}", "dt": 1.1e-3, "tolerance": 1e-5,
+                   "index_mapping_layout": "(nspec, ngllz, ngllx) row-major, 0-based, mesh order",
+                   "rmass_layout": "(nglob, 2) row-major"}, fh, indent=1)
+    manifest["mass_80x60"] = f"{UT}/{rel}"
+
+    with open(os.path.join(HERE, "MANIFEST.json"), "w") as fh:
+        json.dump({"origin": "PrincetonUniversity/SPECFEMPP tests/unit-tests (GPL-3.0)",
+                   "cases": manifest}, fh, indent=1)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

@@ -1,0 +1,65 @@
+"""The oracle against the reference's own fixtures and known answers."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN, load_case, rel_l1_per_station
+
+TYPES = {"d": "displacement", "v": "velocity", "a": "acceleration"}
+
+
+def test_gll_known_answers():
+    # tests/unit-tests/gll/gll_tests.cpp:165-178 (zwgljd, np = 5, alpha = beta = 0)
+    mesh, _ = load_case("elastic_free_80x60")
+    xi, w, hp = oracle.Oracle2D(mesh, 8).quadrature()
+    np.testing.assert_allclose(xi, [-1.0, -0.6546536707, 0.0, 0.6546536707, 1.0], atol=1e-6)
+    np.testing.assert_allclose(w, [0.1, 0.5444444444, 0.7111111111, 0.5444444444, 0.1], atol=1e-6)
+    # tests/unit-tests/lagrange/Lagrange_tests.cpp:9-47: hprime(i, j) = l_j'(xi_i),
+    # corner values -/+ N(N+1)/4 with N = 4, zero on the interior diagonal.
+    assert hp[0, 0] == pytest.approx(-5.0)
+    assert hp[4, 4] == pytest.approx(5.0)
+    assert np.allclose(np.diag(hp)[1:4], 0.0)
+    # the derivative of a constant vanishes; of a linear function is 1
+    np.testing.assert_allclose(hp.sum(axis=1), 0.0, atol=1e-12)
+    np.testing.assert_allclose(hp @ xi, 1.0, atol=1e-12)
+    # GLL quadrature is exact for degree <= 2N-3 = 5
+    for k in range(6):
+        exact = 0.0 if k % 2 else 2.0 / (k + 1)
+        assert np.dot(w, xi ** k) == pytest.approx(exact, abs=1e-12)
+
+
+def test_mass_inverse_matches_specfem2d():
+    # tests/unit-tests/domain/rmass_inverse_tests.cpp:176-234 (rel. L1 <= 1e-5)
+    mesh, cfg = load_case("mass_80x60")
+    d = np.load(os.path.join(GOLDEN, "mass_80x60", "mass.npz"))
+    ref_ibool = d["index_mapping"].reshape(mesh.nspec, 5, 5)
+    rmass = d["rmass_inverse"].reshape(-1, 2)
+    for prec in (4, 8):
+        o = oracle.Oracle2D(mesh, prec, 1, cfg["dt"])
+        ibool, midx, c2m = o.numbering()
+        mi = o.mass_inverse()
+        ours = mi[:, midx[ibool]]                 # (2, nspec, 5, 5) compute order
+        ref = rmass[ref_ibool[c2m]].transpose(3, 0, 1, 2)
+        err = np.abs(ours - ref).sum() / np.abs(ref).sum()
+        assert err < cfg["tolerance"], (prec, err)
+
+
+@pytest.mark.parametrize("case", ["elastic_free_80x60", "elastic_stacey_80x60"])
+def test_forward_traces_match_specfem2d(case):
+    # tests/unit-tests/displacement_tests/Newmark/newmark_tests.cpp:300-346
+    mesh, cfg = load_case(case)
+    traces = np.load(os.path.join(GOLDEN, case, "traces.npz"))
+    o = oracle.Oracle2D(mesh, 4, 1, cfg["dt"])
+    types = [TYPES[k] for k in cfg["trace_kinds"]]
+    seis, t0 = o.forward(cfg["sources"], cfg["stations"], cfg["nstep"], cfg["dt"], 1, types)
+    time = t0 + np.arange(cfg["nstep"]) * cfg["dt"]
+    assert np.abs(time - traces["time"]).max() < 1e-3
+    for it, k in enumerate(cfg["trace_kinds"]):
+        ours = seis[:, it].transpose(1, 2, 0)
+        err = rel_l1_per_station(ours, traces[k])
+        assert err.max() < cfg["tolerance"], (k, err.max())
+        # a dropped or sign-flipped term would be far outside this
+        if k == "d":
+            assert err.max() < 1e-4

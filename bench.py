@@ -1,0 +1,130 @@
+"""Benchmark: GLL point-steps per second of the 3-D elastic NGLL=5 FP32
+forward Newmark loop (64^3 layered cube) on one MI355X, with the roofline of
+the dominant kernel (stiffness) and the CPU oracle timed beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 64] [--no-cpu]
+
+Prints one JSON line.  A "step" is one Newmark step of the whole hot path:
+sources, stiffness (gather, gradient, stress, divergence, atomic scatter),
+Stacey, seismograms, fused mass division + corrector + predictor.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def algorithmic_bytes_3d(nspec: int, nglob: int) -> dict:
+    """Bytes each kernel must move per launch (FP32 / int32), every array the
+    data model defines touched once per pass:
+      stiffness: per local point gather index + kappa + mu (12 B); per global
+                 point displacement read (12 B) and acceleration read-modify-write (24 B)
+      update:    per global point read u, v, a, M^-1 (48 B), write u, v, a (36 B)."""
+    stiff = nspec * 125 * 12 + nglob * 36
+    upd = nglob * 84
+    return {"stiffness": stiff, "update": upd, "per_pt_step": (stiff + upd) / nglob}
+
+
+def cpu_baseline(w, nsteps: int) -> dict:
+    """The FP32 oracle (single thread) on the same 64^3 workload for a few steps."""
+    import oracle
+
+    t = time.perf_counter()
+    o = oracle.Oracle3D(w.mesh, 4, w.dt)
+    build_s = time.perf_counter() - t
+    t = time.perf_counter()
+    o.forward(w.sources, w.receivers, nsteps, w.dt)
+    el = time.perf_counter() - t
+    return {"value": o.nglob * nsteps / el, "unit": "GLL pt-steps/s", "cores": 1, "kind": "port",
+            "sample": f"FP32 oracle (C++ restatement, 1 thread), {nsteps} steps of the same "
+                      f"{w.mesh['nx']}^3 workload, {el:.1f} s timed (+{build_s:.1f} s setup untimed)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine is single-GPU).",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from specfempp_amd import Engine3D, meshes
+
+    w = meshes.config3(n=args.n, nstep=args.warmup + args.steps)
+    t = time.perf_counter()
+    eng = Engine3D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, 1, ("displacement",))
+    setup_s = time.perf_counter() - t
+    eng.initialize()
+    if args.warmup:
+        eng.run(0, args.warmup)
+    eng.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.run(args.warmup, args.steps)
+    eng.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    tm = eng.timings()
+    seis = eng.seismograms()
+    finite = bool(np.isfinite(seis).all())
+    nglob, nspec = eng.nglob, eng.nspec
+    eng.close()
+
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = nglob * args.steps / elapsed
+    ab = algorithmic_bytes_3d(nspec, nglob)
+    stiff_s = tm["stiffness_ms"] * 1e-3
+    achieved = ab["stiffness"] / stiff_s / 1e9 if stiff_s > 0 else 0.0
+    upd_gbs = ab["update"] / (tm["update_ms"] * 1e-3) / 1e9 if tm["update_ms"] > 0 else 0.0
+    step_gbs = ab["per_pt_step"] * value / 1e9
+    out = {
+        "metric": "GLL pt-steps/s, 3D elastic NGLL=5 FP32 forward (64^3 layered, Stacey + free surface)",
+        "value": value,
+        "unit": "GLL pt-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"3d_layered_cube_{args.n}^3_ngll5", "nspec": nspec, "nglob": nglob,
+                   "dt": w.dt, "dt_stated": w.dt_stated, "nrec": len(w.receivers), "source": "moment tensor",
+                   "setup_s": round(setup_s, 2)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "stiffness_3d_kernel",
+                     "algorithmic_bytes_per_launch": ab["stiffness"], "avg_ms": tm["stiffness_ms"]},
+        "kernels_ms": tm,
+        "update_kernel_gbs": upd_gbs,
+        "step_hbm_gbs": step_gbs,
+        "step_hbm_frac": step_gbs / HBM_PEAK_GBS,
+        "algorithmic_bytes_per_pt_step": ab["per_pt_step"],
+        "seismograms_finite": finite,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

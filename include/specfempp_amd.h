@@ -1,0 +1,183 @@
+/*
+ * specfempp_amd — MI355X-native spectral-element Newmark engine, C ABI.
+ *
+ * This header is the drop-in boundary for the SPECFEM++ Newmark time loop.
+ * Each entry point names the reference interface it replaces.  All functions
+ * return 0 on success and a negative status on error; sfx_last_error() gives
+ * the message of the last failing call on the calling thread.  No C++
+ * exception crosses the boundary.  Host arrays passed in are borrowed for the
+ * duration of the call only (the engine copies what it keeps).
+ *
+ * Threading: an engine is single-threaded (one host thread drives it); two
+ * engines are independent.  All device work of an engine runs in order on the
+ * engine's own HIP stream.
+ */
+#ifndef SPECFEMPP_AMD_H
+#define SPECFEMPP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFX_OK 0
+#define SFX_ERR_ARG -1
+#define SFX_ERR_HIP -2
+#define SFX_ERR_STATE -3
+#define SFX_ERR_MESH -4
+
+typedef struct sfx_engine sfx_engine;
+
+/* Edge codes of an absorbing edge (read_boundaries.cpp:166-179). */
+#define SFX_EDGE_BOTTOM 0
+#define SFX_EDGE_RIGHT 1
+#define SFX_EDGE_TOP 2
+#define SFX_EDGE_LEFT 3
+
+/* 2-D mesh: the content of specfem::mesh::mesh<dim2> for an elastic P-SV
+ * isotropic domain (include/mesh/dim2/mesh.hpp; read by
+ * src/io/mesh/impl/fortran/dim2/mesh.cpp:26-193). */
+typedef struct {
+  int nspec;            /* spectral elements */
+  int ngnod;            /* control nodes per element: 4 or 9 */
+  int npgeo;            /* control nodes */
+  const double *coorg;  /* (npgeo, 2): x, z */
+  const int *knods;     /* (nspec, ngnod): 0-based control node of each element */
+  const int *material;  /* (nspec): 0-based material of each element */
+  int nmaterials;
+  const double *rho, *vp, *vs; /* per material (kg/m^3, m/s, m/s) */
+  int nabs;             /* absorbing (Stacey) edges */
+  const int *abs_ispec; /* (nabs): mesh element of each edge */
+  const int *abs_edge;  /* (nabs): SFX_EDGE_* */
+} sfx_mesh2d;
+
+/* 3-D structured hexahedral mesh (NGLL = 5).  Elements are numbered x
+ * fastest, then y, then z; z points up.  The reference holds no 3-D solver:
+ * this is the build's own mesh model, following the SPECFEM3D database
+ * semantics read by src/io/mesh/impl/fortran/dim3/mesh.cpp:43-797 (ibool,
+ * regular-element geometry, per-point kappa/mu/rho, Stacey faces). */
+typedef struct {
+  int nx, ny, nz;
+  double x0, y0, z0;    /* origin (minimum corner) */
+  double hx, hy, hz;    /* element size */
+  const int *material;  /* (nz, ny, nx): 0-based material of each element */
+  int nmaterials;
+  const double *rho, *vp, *vs;
+  int absorbing[6];     /* Stacey on faces xmin, xmax, ymin, ymax, zmin, zmax (1 = yes) */
+} sfx_mesh3d;
+
+/* Source kinds (src/source/force_source.cpp, moment_tensor_source.cpp). */
+#define SFX_SOURCE_FORCE 0
+#define SFX_SOURCE_MOMENT_TENSOR 1
+
+/* One source with a Ricker time function (src/source_time_function/ricker.cpp).
+ * 2-D: position (x, z), force angle in degrees, moment tensor
+ * (m[0]=Mxx, m[1]=Mzz, m[2]=Mxz).  3-D: position (x, y, z), force vector
+ * (fx, fy, fz), moment tensor m = (Mxx, Myy, Mzz, Mxy, Mxz, Myz). */
+typedef struct {
+  int kind;
+  double x, y, z;
+  double angle;
+  double fx, fy, fz;
+  double m[6];
+  double f0, tshift, factor;
+} sfx_source;
+
+typedef struct {
+  double x, y, z;
+  double angle; /* 2-D station rotation in degrees (receivers.hpp:126-141) */
+} sfx_receiver;
+
+/* Seismogram kinds (specfem::wavefield::type). */
+#define SFX_DISPLACEMENT 0
+#define SFX_VELOCITY 1
+#define SFX_ACCELERATION 2
+
+typedef struct {
+  double dt;
+  int nstep;
+  int nstep_between_samples;
+  int nseismo_types;
+  int seismo_types[3];
+  int chunk_size;   /* storage chunk size of the numbering (chunk_config.hpp:12-28); 1 = CPU no-SIMD */
+} sfx_run_desc;
+
+/* ---- library ---------------------------------------------------------- */
+int sfx_version(void);
+const char *sfx_last_error(void);
+
+/* ---- engine lifetime ------------------------------------------------------
+ * Replaces the construction of specfem::compute::assembly
+ * (include/compute/assembly/assembly.hpp:34-119; src/compute/assembly/assembly.cpp:6-101)
+ * followed by time_scheme::link_assembly (include/timescheme/newmark.hpp:103-105).
+ * Builds numbering, geometry, properties, boundaries, sources and receivers on
+ * the host and copies them to the current HIP device. */
+int sfx_engine_create_2d(const sfx_mesh2d *mesh, int nsources, const sfx_source *sources, int nreceivers,
+                         const sfx_receiver *receivers, const sfx_run_desc *run, sfx_engine **out);
+int sfx_engine_create_3d(const sfx_mesh3d *mesh, int nsources, const sfx_source *sources, int nreceivers,
+                         const sfx_receiver *receivers, const sfx_run_desc *run, sfx_engine **out);
+void sfx_engine_destroy(sfx_engine *engine);
+
+/* Sizes: dim, nspec, nglob, ngll, nsig (seismogram samples), nrec, nsrc. */
+int sfx_get_sizes(const sfx_engine *engine, int64_t *sizes7);
+/* Simulation start time t0 (src/io/sources.cpp:128-158). */
+int sfx_get_t0(const sfx_engine *engine, double *t0);
+
+/* ---- data contract (bit-exact comparison against the reference) ------- */
+/* index_mapping (nspec, ngll, ngll[, ngll]) in compute order
+ * (specfem::compute::points::h_index_mapping, compute_mesh.cpp:45-170). */
+int sfx_get_index_mapping(const sfx_engine *engine, int32_t *out);
+/* per-medium numbering iglob -> field index (field_impl.tpp:39-62). */
+int sfx_get_field_index(const sfx_engine *engine, int32_t *out);
+/* compute order -> mesh order (mesh_to_compute_mapping, compute_mesh.cpp:240-279). */
+int sfx_get_compute_to_mesh(const sfx_engine *engine, int32_t *out);
+/* element and reference coordinates of every source / receiver
+ * (locate_point, src/algorithms/locate_point.cpp:120-176). */
+int sfx_get_locations(const sfx_engine *engine, int which /*0 sources, 1 receivers*/, int32_t *ispec,
+                      double *xi_eta_gamma /* (n, dim) */);
+
+/* ---- time-scheme seam (drop-in for time_scheme / domain_kernels) ---------
+ * sfx_initialize: kernels.initialize(dt) — mass matrix and its inverse
+ *   (domain_kernels.hpp:136-163); also zeroes the wavefield.
+ * sfx_predictor: apply_predictor_phase_forward (newmark.tpp:140-159); returns
+ *   the number of degrees of freedom updated (nglob * ncomp).
+ * sfx_update_wavefields: update_wavefields<elastic> (domain_kernels.hpp:69-126):
+ *   sources, stiffness (+ Stacey), mass division; returns elements updated.
+ * sfx_corrector: apply_corrector_phase_forward (newmark.tpp:119-137).
+ * sfx_compute_seismograms: compute_seismograms(isig) (domain_kernels.hpp:172-186). */
+int sfx_initialize(sfx_engine *engine);
+int sfx_predictor(sfx_engine *engine);
+int sfx_update_wavefields(sfx_engine *engine, int istep);
+int sfx_corrector(sfx_engine *engine);
+int sfx_compute_seismograms(sfx_engine *engine, int isig);
+
+/* ---- fused fast path --------------------------------------------------------
+ * One Newmark step of time_marching<forward>::run (time_marching.tpp:33-109)
+ * with the corrector of step istep, the seismogram sample and the predictor of
+ * step istep+1 fused into one pass over the global points.
+ * sfx_run: steps [first, first + count) — the whole loop when (0, nstep). */
+int sfx_step(sfx_engine *engine, int istep);
+int sfx_run(sfx_engine *engine, int first, int count);
+int sfx_synchronize(sfx_engine *engine);
+
+/* ---- outputs ------------------------------------------------------------- */
+/* seismograms (nsig, ntypes, nrec, ncomp) before station rotation
+ * (receivers seismogram_components, compute_seismogram.tpp:103-135). */
+int sfx_get_seismograms(const sfx_engine *engine, float *out);
+/* which: 0 displacement, 1 velocity, 2 acceleration, 3 mass inverse;
+ * out (ncomp, nglob) on the per-medium field index. */
+int sfx_get_field(const sfx_engine *engine, int which, float *out);
+int sfx_set_field(sfx_engine *engine, int which, const float *in);
+
+/* ---- measurement -------------------------------------------------------------
+ * Average device time (ms) of each kernel family over the last sfx_run, taken
+ * with HIP events on the engine stream: out[0] stiffness, out[1] fused update,
+ * out[2] boundary+source+seismogram, out[3] whole step. */
+int sfx_get_timings(const sfx_engine *engine, double *out4);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPECFEMPP_AMD_H */

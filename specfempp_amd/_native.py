@@ -1,0 +1,96 @@
+"""Build and load the HIP C-ABI library ``_lib/libspecfempp_amd.so``.
+
+``build()`` compiles every source under ``csrc/`` for gfx950 with hipcc (no
+GPU needed).  ``load()`` only loads what is built; it never falls back to
+anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB_DIR = os.path.join(HERE, "_lib")
+LIB_PATH = os.path.join(LIB_DIR, "libspecfempp_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "specfempp_amd.h")
+
+SOURCES = ["engine.hip", "kernels.hip", "assembly.cpp"]
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # packed FP32 instructions stay off ("compiler flags")
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+_lib = None
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
+        if cand and (os.path.sep not in cand or os.path.exists(cand)):
+            return cand
+    return "hipcc"
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(LIB_DIR, exist_ok=True)
+    tmp = LIB_PATH + ".tmp"
+    cmd = [hipcc()] + HIPCC_FLAGS + [os.path.join(CSRC, s) for s in SOURCES] + ["-o", tmp]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + res.stderr[-4000:])
+    if verbose:
+        print(res.stderr)
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+def load():
+    """Load the built library (building it first only if it is absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    vp, i32, i64, dp = C.c_void_p, C.c_int, C.c_int64, C.c_void_p
+    sig = {
+        "sfx_version": ([], i32),
+        "sfx_last_error": ([], C.c_char_p),
+        "sfx_engine_create_2d": ([vp, i32, vp, i32, vp, vp, vp], i32),
+        "sfx_engine_create_3d": ([vp, i32, vp, i32, vp, vp, vp], i32),
+        "sfx_engine_destroy": ([vp], None),
+        "sfx_get_sizes": ([vp, vp], i32),
+        "sfx_get_t0": ([vp, vp], i32),
+        "sfx_get_index_mapping": ([vp, vp], i32),
+        "sfx_get_field_index": ([vp, vp], i32),
+        "sfx_get_compute_to_mesh": ([vp, vp], i32),
+        "sfx_get_locations": ([vp, i32, vp, vp], i32),
+        "sfx_initialize": ([vp], i32),
+        "sfx_predictor": ([vp], i32),
+        "sfx_update_wavefields": ([vp, i32], i32),
+        "sfx_corrector": ([vp], i32),
+        "sfx_compute_seismograms": ([vp, i32], i32),
+        "sfx_step": ([vp, i32], i32),
+        "sfx_run": ([vp, i32, i32], i32),
+        "sfx_synchronize": ([vp], i32),
+        "sfx_get_seismograms": ([vp, vp], i32),
+        "sfx_get_field": ([vp, i32, vp], i32),
+        "sfx_set_field": ([vp, i32, vp], i32),
+        "sfx_get_timings": ([vp, vp], i32),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def declared_symbols():
+    """Function names declared in include/specfempp_amd.h."""
+    import re
+
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(sfx_[a-z0-9_]+)\s*\(", text)))

@@ -1,0 +1,788 @@
+// Host assembly of the 2-D and 3-D engines.  See assembly.hpp for the layout.
+#include "assembly.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <map>
+#include <stdexcept>
+
+#include "quadrature.hpp"
+
+namespace sfx {
+
+namespace {
+
+struct SortPoint {
+  float x, z;
+  int iloc, iglob;
+};
+
+// Mesh-order element types -> compute order
+// (src/compute/compute_mesh.cpp:240-279 with the ELEMENT_TYPES order of
+// include/enumerations/material_definitions.hpp:76-102: elastic P-SV
+// isotropic without boundary, then with Stacey).
+void compute_ordering(const sfx_mesh2d &m, HostAssembly &a, std::vector<int> &is_stacey_mesh) {
+  is_stacey_mesh.assign(m.nspec, 0);
+  for (int i = 0; i < m.nabs; i++) {
+    if (m.abs_ispec[i] < 0 || m.abs_ispec[i] >= m.nspec) throw std::invalid_argument("abs_ispec out of range");
+    if (m.abs_edge[i] < 0 || m.abs_edge[i] > 3) throw std::invalid_argument("abs_edge out of range");
+    is_stacey_mesh[m.abs_ispec[i]] = 1;
+  }
+  a.compute_to_mesh.clear();
+  a.compute_to_mesh.reserve(m.nspec);
+  for (int pass = 0; pass < 2; pass++)
+    for (int e = 0; e < m.nspec; e++)
+      if (is_stacey_mesh[e] == pass) a.compute_to_mesh.push_back(e);
+  a.nspec_none = 0;
+  for (int e = 0; e < m.nspec; e++) a.nspec_none += is_stacey_mesh[e] ? 0 : 1;
+}
+
+// Tolerance of the point merge, src/compute/compute_mesh.cpp:19-43: taken on
+// consecutive groups of ngllxz *sorted* points, starting the maxima at the
+// smallest positive float, exactly as the reference does.
+float merge_tolerance(const std::vector<SortPoint> &pts, int nspec, int ngllxz) {
+  float typ = std::numeric_limits<float>::max();
+  for (int e = 0; e < nspec; e++) {
+    float xmax = std::numeric_limits<float>::min(), xmin = std::numeric_limits<float>::max();
+    float zmax = std::numeric_limits<float>::min(), zmin = std::numeric_limits<float>::max();
+    const SortPoint *g = &pts[static_cast<size_t>(e) * ngllxz];
+    for (int k = 0; k < ngllxz; k++) {
+      xmax = std::max(xmax, g[k].x);
+      xmin = std::min(xmin, g[k].x);
+      zmax = std::max(zmax, g[k].z);
+      zmin = std::min(zmin, g[k].z);
+    }
+    typ = std::min(typ, xmax - xmin);
+    typ = std::min(typ, zmax - zmin);
+  }
+  return 1e-6 * typ;
+}
+
+// Numbering, src/compute/compute_mesh.cpp:45-170: sort the points by (x, z),
+// merge neighbours closer than the tolerance, then number by first visit in
+// (chunk, iz, ix, element) order.
+void number_points_2d(HostAssembly &a, const std::vector<float> &px, const std::vector<float> &pz, int chunk) {
+  const int nspec = a.nspec, npe = 25;
+  const size_t nloc = static_cast<size_t>(nspec) * npe;
+  std::vector<SortPoint> pts(nloc);
+  std::vector<size_t> visit(nloc);  // iloc -> (e*25 + p)
+  size_t iloc = 0;
+  for (int c0 = 0; c0 < nspec; c0 += chunk)
+    for (int iz = 0; iz < NGLL; iz++)
+      for (int ix = 0; ix < NGLL; ix++)
+        for (int ie = 0; ie < chunk && c0 + ie < nspec; ie++) {
+          const size_t p = static_cast<size_t>(c0 + ie) * npe + iz * NGLL + ix;
+          pts[iloc] = SortPoint{px[p], pz[p], static_cast<int>(iloc), 0};
+          visit[iloc] = p;
+          iloc++;
+        }
+  std::vector<SortPoint> sorted = pts;
+  std::sort(sorted.begin(), sorted.end(), [](const SortPoint &p, const SortPoint &q) {
+    if (p.x != q.x) return p.x < q.x;
+    return p.z < q.z;
+  });
+  const float tol = merge_tolerance(sorted, nspec, npe);
+  int ig = 0;
+  sorted[0].iglob = 0;
+  for (size_t i = 1; i < nloc; i++) {
+    if (std::abs(sorted[i].x - sorted[i - 1].x) > tol || std::abs(sorted[i].z - sorted[i - 1].z) > tol) ig++;
+    sorted[i].iglob = ig;
+  }
+  for (const auto &s : sorted) pts[s.iloc].iglob = s.iglob;
+  const int nglob = ig + 1;
+  std::vector<int> first(nglob, -1);
+  a.index_mapping.assign(nloc, -1);
+  int inum = 0;
+  for (size_t i = 0; i < nloc; i++) {
+    int &f = first[pts[i].iglob];
+    if (f < 0) f = inum++;
+    a.index_mapping[visit[i]] = f;
+  }
+  if (inum != nglob) throw std::runtime_error("numbering: merged point count mismatch");
+  a.nglob = nglob;
+  // per-medium field index, include/compute/fields/impl/field_impl.tpp:39-62
+  // (loop order chunk, ix, iz, element)
+  a.field_index.assign(nglob, -1);
+  int count = 0;
+  for (int c0 = 0; c0 < nspec; c0 += chunk)
+    for (int ix = 0; ix < NGLL; ix++)
+      for (int iz = 0; iz < NGLL; iz++)
+        for (int ie = 0; ie < chunk && c0 + ie < nspec; ie++) {
+          const int g = a.index_mapping[static_cast<size_t>(c0 + ie) * npe + iz * NGLL + ix];
+          if (a.field_index[g] < 0) a.field_index[g] = count++;
+        }
+}
+
+struct Locator2D {
+  const HostAssembly &a;
+  const std::vector<float> &ctrl;  // (nspec, ngnod, 2)
+  const std::vector<float> &px, &pz;
+  int ngnod;
+
+  void position(int e, float xi, float gamma, float &x, float &z) const {
+    float s[9];
+    shape2d(xi, gamma, ngnod, s);
+    x = 0.0f;
+    z = 0.0f;
+    for (int n = 0; n < ngnod; n++) {
+      x += s[n] * ctrl[(static_cast<size_t>(e) * ngnod + n) * 2];
+      z += s[n] * ctrl[(static_cast<size_t>(e) * ngnod + n) * 2 + 1];
+    }
+  }
+  void inverse_jacobian(int e, float xi, float gamma, float &xix, float &gammax, float &xiz, float &gammaz) const {
+    float d0[9], d1[9];
+    dshape2d(xi, gamma, ngnod, d0, d1);
+    float xxi = 0.0f, zxi = 0.0f, xgamma = 0.0f, zgamma = 0.0f;
+    for (int n = 0; n < ngnod; n++) {
+      const float cx = ctrl[(static_cast<size_t>(e) * ngnod + n) * 2];
+      const float cz = ctrl[(static_cast<size_t>(e) * ngnod + n) * 2 + 1];
+      xxi += d0[n] * cx;
+      zxi += d0[n] * cz;
+      xgamma += d1[n] * cx;
+      zgamma += d1[n] * cz;
+    }
+    const float J = xxi * zgamma - xgamma * zxi;
+    xix = zgamma / J;
+    gammax = -zxi / J;
+    xiz = -xgamma / J;
+    gammaz = xxi / J;
+  }
+
+  // src/algorithms/locate_point.cpp:120-176: nearest GLL point (strict <,
+  // first in (element, iz, ix) order), candidate elements sharing a corner
+  // point of the guess, 100 clamped Newton steps per candidate, smallest
+  // distance wins.
+  void locate(float gx, float gz, int &ispec, float &xi_out, float &gamma_out) const {
+    const int nspec = a.nspec;
+    float dmin = std::numeric_limits<float>::max();
+    int eg = 0, ixg = 0, izg = 0;
+    for (int e = 0; e < nspec; e++)
+      for (int j = 0; j < NGLL; j++)
+        for (int i = 0; i < NGLL; i++) {
+          const size_t p = static_cast<size_t>(e) * 25 + j * NGLL + i;
+          const float dx = gx - px[p], dz = gz - pz[p];
+          const float d = std::sqrt(dx * dx + dz * dz);
+          if (d < dmin) {
+            dmin = d;
+            eg = e;
+            ixg = i;
+            izg = j;
+          }
+        }
+    auto im = [&](int e, int j, int i) { return a.index_mapping[static_cast<size_t>(e) * 25 + j * NGLL + i]; };
+    const int corners[4] = {im(eg, 0, 0), im(eg, NGLL - 1, 0), im(eg, 0, NGLL - 1), im(eg, NGLL - 1, NGLL - 1)};
+    std::vector<int> cand{eg};
+    for (int e = 0; e < nspec; e++) {
+      if (e == eg) continue;
+      for (int j : {0, NGLL - 1})
+        for (int i : {0, NGLL - 1}) {
+          const int g = im(e, j, i);
+          if (std::find(corners, corners + 4, g) != corners + 4 && cand.back() != e) cand.push_back(e);
+        }
+    }
+    float best = std::numeric_limits<float>::max();
+    ispec = eg;
+    xi_out = a.xi[ixg];
+    gamma_out = a.xi[izg];
+    for (int e : cand) {
+      float xi = a.xi[ixg], gamma = a.xi[izg];
+      for (int it = 0; it < 100; it++) {
+        float x, z, xix, gammax, xiz, gammaz;
+        position(e, xi, gamma, x, z);
+        inverse_jacobian(e, xi, gamma, xix, gammax, xiz, gammaz);
+        const float dx = -(x - gx), dz = -(z - gz);
+        xi += xix * dx + xiz * dz;
+        gamma += gammax * dx + gammaz * dz;
+        xi = std::min(std::max(xi, -1.01f), 1.01f);
+        gamma = std::min(std::max(gamma, -1.01f), 1.01f);
+      }
+      float x, z;
+      position(e, xi, gamma, x, z);
+      const float dx = gx - x, dz = gz - z;
+      const float d = std::sqrt(dx * dx + dz * dz);
+      if (d < best) {
+        best = d;
+        ispec = e;
+        xi_out = xi;
+        gamma_out = gamma;
+      }
+      ixg = NGLL / 2;
+      izg = NGLL / 2;
+    }
+  }
+};
+
+// Stacey traction for a point (include/boundary_conditions/stacey/stacey.tpp:36-72)
+// with the velocity replaced by a constant, as the mass-matrix term uses it
+// (stacey.tpp:641-677).
+template <int D>
+void stacey_term(const float *n, float w, float rho_vp, float rho_vs, float vconst, float *out) {
+  float vn = 0.0f, nn = 0.0f;
+  for (int d = 0; d < D; d++) {
+    vn += vconst * n[d];
+    nn += n[d] * n[d];
+  }
+  const float j1 = std::sqrt(nn);
+  for (int d = 0; d < D; d++) {
+    const float f = ((vn * n[d] / (j1 * j1)) * (rho_vp - rho_vs)) + vconst * rho_vs;
+    out[d] += -1.0f * f * j1 * w;
+  }
+}
+
+void setup_time_function(HostAssembly &a, int nsrc, const sfx_source *src, const sfx_run_desc &run) {
+  // t0 = min over sources of (-1.2 / f0 + tshift); every source's shift
+  // becomes its own t0 - min t0 (src/io/sources.cpp:128-158, ricker.cpp:15-20).
+  float min_t0 = std::numeric_limits<float>::max();
+  std::vector<float> t0s(nsrc);
+  for (int s = 0; s < nsrc; s++) {
+    if (!(src[s].f0 > 0)) throw std::invalid_argument("source f0 must be positive");
+    const float hdur = 1.0f / static_cast<float>(src[s].f0);
+    t0s[s] = -1.2 * hdur + static_cast<float>(src[s].tshift);
+    min_t0 = std::min(min_t0, t0s[s]);
+  }
+  if (nsrc == 0) min_t0 = 0.0f;
+  a.t0 = min_t0;
+  const float dt = static_cast<float>(run.dt);
+  a.stf.assign(static_cast<size_t>(run.nstep) * nsrc, 0.0f);
+  for (int s = 0; s < nsrc; s++) {
+    const float shift = t0s[s] - min_t0;
+    for (int i = 0; i < run.nstep; i++)
+      a.stf[static_cast<size_t>(i) * nsrc + s] =
+          ricker(min_t0 + i * dt - shift, static_cast<float>(src[s].f0), static_cast<float>(src[s].factor));
+  }
+}
+
+void check_run(const sfx_run_desc &run) {
+  if (!(run.dt > 0)) throw std::invalid_argument("dt must be positive");
+  if (run.nstep < 0) throw std::invalid_argument("nstep must be >= 0");
+  if (run.nstep_between_samples < 1) throw std::invalid_argument("nstep_between_samples must be >= 1");
+  if (run.nseismo_types < 0 || run.nseismo_types > 3) throw std::invalid_argument("nseismo_types must be 0..3");
+  for (int i = 0; i < run.nseismo_types; i++)
+    if (run.seismo_types[i] < 0 || run.seismo_types[i] > 2) throw std::invalid_argument("bad seismogram type");
+  if (run.chunk_size < 1) throw std::invalid_argument("chunk_size must be >= 1");
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// 2-D
+// ---------------------------------------------------------------------------
+HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *src, int nrec,
+                               const sfx_receiver *rec, const sfx_run_desc &run) {
+  check_run(run);
+  if (m.nspec <= 0 || m.npgeo <= 0) throw std::invalid_argument("empty mesh");
+  if (m.ngnod != 4 && m.ngnod != 9) throw std::invalid_argument("ngnod must be 4 or 9");
+  if (!m.coorg || !m.knods || !m.material || !m.rho || !m.vp || !m.vs) throw std::invalid_argument("null mesh array");
+  for (int i = 0; i < m.nmaterials; i++)
+    if (!(m.vs[i] > 0) || !(m.rho[i] > 0) || !(m.vp[i] > 0))
+      throw std::invalid_argument("only elastic isotropic materials (vs > 0) are supported");
+  HostAssembly a;
+  a.dim = 2;
+  a.ncomp = 2;
+  a.npe = 25;
+  a.nspec = m.nspec;
+  const Gll g = make_gll();
+  std::copy(g.xi, g.xi + 5, a.xi);
+  std::copy(g.w, g.w + 5, a.w);
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j < 5; j++) a.hprime[i][j] = g.hprime[i][j];
+
+  std::vector<int> stacey_mesh;
+  compute_ordering(m, a, stacey_mesh);
+  const int nspec = m.nspec, ngnod = m.ngnod, npe = 25;
+
+  // control nodes in compute order, as type_real (compute_mesh.cpp:172-197)
+  std::vector<float> ctrl(static_cast<size_t>(nspec) * ngnod * 2);
+  for (int e = 0; e < nspec; e++) {
+    const int em = a.compute_to_mesh[e];
+    for (int n = 0; n < ngnod; n++) {
+      const int node = m.knods[static_cast<size_t>(em) * ngnod + n];
+      if (node < 0 || node >= m.npgeo) throw std::invalid_argument("knods out of range");
+      ctrl[(static_cast<size_t>(e) * ngnod + n) * 2] = static_cast<float>(m.coorg[2 * static_cast<size_t>(node)]);
+      ctrl[(static_cast<size_t>(e) * ngnod + n) * 2 + 1] =
+          static_cast<float>(m.coorg[2 * static_cast<size_t>(node) + 1]);
+    }
+  }
+  // GLL point coordinates: float products accumulated in double, stored as
+  // float (compute_mesh.cpp:299-333).
+  const size_t nloc = static_cast<size_t>(nspec) * npe;
+  std::vector<float> px(nloc), pz(nloc);
+  float shp[5][5][9];
+  for (int iz = 0; iz < NGLL; iz++)
+    for (int ix = 0; ix < NGLL; ix++) shape2d(a.xi[ix], a.xi[iz], ngnod, shp[iz][ix]);
+  for (int e = 0; e < nspec; e++)
+    for (int iz = 0; iz < NGLL; iz++)
+      for (int ix = 0; ix < NGLL; ix++) {
+        double x = 0.0, z = 0.0;
+        for (int n = 0; n < ngnod; n++) {
+          x += ctrl[(static_cast<size_t>(e) * ngnod + n) * 2] * shp[iz][ix][n];
+          z += ctrl[(static_cast<size_t>(e) * ngnod + n) * 2 + 1] * shp[iz][ix][n];
+        }
+        const size_t p = static_cast<size_t>(e) * npe + iz * NGLL + ix;
+        px[p] = static_cast<float>(x);
+        pz[p] = static_cast<float>(z);
+      }
+  number_points_2d(a, px, pz, run.chunk_size);
+  a.nglob_pad = (a.nglob + 63) / 64 * 64;
+  a.gather.resize(nloc);
+  for (size_t p = 0; p < nloc; p++) a.gather[p] = a.field_index[a.index_mapping[p]];
+
+  // Geometry (compute_partial_derivatives.cpp:46-91; jacobian.cpp:232-257)
+  a.xix.resize(nloc);
+  a.xiz.resize(nloc);
+  a.gammax.resize(nloc);
+  a.gammaz.resize(nloc);
+  a.jac.resize(nloc);
+  float dsh0[5][5][9], dsh1[5][5][9];
+  for (int iz = 0; iz < NGLL; iz++)
+    for (int ix = 0; ix < NGLL; ix++) dshape2d(a.xi[ix], a.xi[iz], ngnod, dsh0[iz][ix], dsh1[iz][ix]);
+  for (int e = 0; e < nspec; e++)
+    for (int iz = 0; iz < NGLL; iz++)
+      for (int ix = 0; ix < NGLL; ix++) {
+        float xxi = 0.0f, zxi = 0.0f, xgamma = 0.0f, zgamma = 0.0f;
+        for (int n = 0; n < ngnod; n++) {
+          const float cx = ctrl[(static_cast<size_t>(e) * ngnod + n) * 2];
+          const float cz = ctrl[(static_cast<size_t>(e) * ngnod + n) * 2 + 1];
+          xxi += dsh0[iz][ix][n] * cx;
+          zxi += dsh0[iz][ix][n] * cz;
+          xgamma += dsh1[iz][ix][n] * cx;
+          zgamma += dsh1[iz][ix][n] * cz;
+        }
+        const float J = xxi * zgamma - xgamma * zxi;
+        if (J < 1e-10f)  // check_small_jacobian, compute_partial_derivatives.cpp:114-150
+          throw std::runtime_error("element " + std::to_string(a.compute_to_mesh[e]) + " has a small jacobian");
+        const size_t p = static_cast<size_t>(e) * npe + iz * NGLL + ix;
+        a.xix[p] = zgamma / J;
+        a.gammax[p] = -zxi / J;
+        a.xiz[p] = -xgamma / J;
+        a.gammaz[p] = xxi / J;
+        a.jac[p] = J;
+      }
+
+  // Properties (include/medium/dim2/elastic/isotropic/material.hpp:62-77)
+  a.kappa.resize(nloc);
+  a.mu.resize(nloc);
+  a.rho.resize(nloc);
+  for (int e = 0; e < nspec; e++) {
+    const int mat = m.material[a.compute_to_mesh[e]];
+    if (mat < 0 || mat >= m.nmaterials) throw std::invalid_argument("material index out of range");
+    const float rho = static_cast<float>(m.rho[mat]);
+    const float cs = static_cast<float>(m.vs[mat]), cp = static_cast<float>(m.vp[mat]);
+    const float mu = rho * cs * cs;
+    const float kappa = rho * (cp * cp - (4.0 / 3.0) * cs * cs);
+    for (int p = 0; p < npe; p++) {
+      a.kappa[static_cast<size_t>(e) * npe + p] = kappa;
+      a.mu[static_cast<size_t>(e) * npe + p] = mu;
+      a.rho[static_cast<size_t>(e) * npe + p] = rho;
+    }
+  }
+
+  // Stacey entries (src/compute/boundaries/impl/stacey.cpp:64-181): for every
+  // element point on an absorbing edge, the outward normal scaled by the
+  // jacobian and the edge weight; the last listed edge of an element wins at
+  // a point shared by two of its edges.
+  std::vector<int> mesh_to_compute(nspec);
+  for (int e = 0; e < nspec; e++) mesh_to_compute[a.compute_to_mesh[e]] = e;
+  std::map<int, std::vector<int>> edges_of;
+  for (int i = 0; i < m.nabs; i++) edges_of[mesh_to_compute[m.abs_ispec[i]]].push_back(i);
+  std::vector<int> bnd_of(nloc, -1);  // element point -> Stacey entry
+  for (const auto &kv : edges_of) {
+    const int e = kv.first;
+    int slot[25];
+    std::fill(slot, slot + 25, -1);
+    for (int i : kv.second) {
+      const int edge = m.abs_edge[i];
+      for (int iz = 0; iz < NGLL; iz++)
+        for (int ix = 0; ix < NGLL; ix++) {
+          const bool on = (edge == SFX_EDGE_TOP && iz == NGLL - 1) || (edge == SFX_EDGE_BOTTOM && iz == 0) ||
+                          (edge == SFX_EDGE_LEFT && ix == 0) || (edge == SFX_EDGE_RIGHT && ix == NGLL - 1);
+          if (!on) continue;
+          const int lp = iz * NGLL + ix;
+          const size_t p = static_cast<size_t>(e) * npe + lp;
+          const float J = a.jac[p];
+          float nx, nz, w;
+          if (edge == SFX_EDGE_LEFT) {
+            nx = -1.0f * a.xix[p] * J;
+            nz = -1.0f * a.xiz[p] * J;
+            w = a.w[iz];
+          } else if (edge == SFX_EDGE_RIGHT) {
+            nx = a.xix[p] * J;
+            nz = a.xiz[p] * J;
+            w = a.w[iz];
+          } else if (edge == SFX_EDGE_TOP) {
+            nx = a.gammax[p] * J;
+            nz = a.gammaz[p] * J;
+            w = a.w[ix];
+          } else {
+            nx = -1.0f * a.gammax[p] * J;
+            nz = -1.0f * a.gammaz[p] * J;
+            w = a.w[ix];
+          }
+          int k = slot[lp];
+          if (k < 0) {
+            k = static_cast<int>(a.bnd_gather.size());
+            slot[lp] = k;
+            bnd_of[p] = k;
+            a.bnd_gather.push_back(a.gather[p]);
+            a.bnd_normal.push_back(0);
+            a.bnd_normal.push_back(0);
+            a.bnd_weight.push_back(0);
+            const float kap = a.kappa[p], mu = a.mu[p], rho = a.rho[p];
+            const float lp2m = kap + static_cast<float>(4.0 / 3.0) * mu;
+            a.bnd_rho_vp.push_back(std::sqrt(lp2m / rho) * rho);
+            a.bnd_rho_vs.push_back(std::sqrt(rho * mu));
+          }
+          a.bnd_normal[2 * k] = nx;
+          a.bnd_normal[2 * k + 1] = nz;
+          a.bnd_weight[k] = w;
+        }
+    }
+  }
+
+  // Mass matrix and inverse (compute_mass_matrix.tpp:80-114,
+  // invert_mass_matrix.tpp:40-47), accumulated in element order.
+  {
+    std::vector<float> mass(2 * static_cast<size_t>(a.nglob), 0.0f);
+    const float vfac = -1.0f * static_cast<float>(run.dt) * 0.5f;
+    for (int e = 0; e < nspec; e++)
+      for (int iz = 0; iz < NGLL; iz++)
+        for (int ix = 0; ix < NGLL; ix++) {
+          const size_t p = static_cast<size_t>(e) * npe + iz * NGLL + ix;
+          float mm[2] = {a.rho[p], a.rho[p]};
+          const float f = a.w[ix] * a.w[iz] * a.jac[p];
+          mm[0] *= f;
+          mm[1] *= f;
+          const int k = bnd_of[p];
+          if (k >= 0) stacey_term<2>(&a.bnd_normal[2 * k], a.bnd_weight[k], a.bnd_rho_vp[k], a.bnd_rho_vs[k], vfac, mm);
+          const int gi = a.gather[p];
+          mass[gi] += mm[0];
+          mass[a.nglob + gi] += mm[1];
+        }
+    a.mass_inverse.assign(2 * static_cast<size_t>(a.nglob), 0.0f);
+    for (size_t i = 0; i < mass.size(); i++) a.mass_inverse[i] = 1.0f / mass[i];
+  }
+
+  // Sources and receivers (locate_point; force_source.cpp:70-78;
+  // moment_tensor_source.cpp:88-130; compute_receivers.cpp:59-83)
+  Locator2D loc{a, ctrl, px, pz, ngnod};
+  setup_time_function(a, nsrc, src, run);
+  a.nsrc = nsrc;
+  a.src_ispec.resize(nsrc);
+  a.src_array.assign(static_cast<size_t>(nsrc) * 2 * npe, 0.0f);
+  a.src_local.resize(2 * static_cast<size_t>(nsrc));
+  const float pi = static_cast<float>(M_PI);
+  for (int s = 0; s < nsrc; s++) {
+    int e;
+    float xi, gamma;
+    loc.locate(static_cast<float>(src[s].x), static_cast<float>(src[s].z), e, xi, gamma);
+    a.src_ispec[s] = e;
+    a.src_local[2 * s] = xi;
+    a.src_local[2 * s + 1] = gamma;
+    float hx[5], hpx[5], hg[5], hpg[5];
+    lagrange<float>(xi, a.xi, hx, hpx);
+    lagrange<float>(gamma, a.xi, hg, hpg);
+    float *arr = &a.src_array[static_cast<size_t>(s) * 2 * npe];
+    if (src[s].kind == SFX_SOURCE_FORCE) {
+      const float ang = static_cast<float>(src[s].angle);
+      for (int iz = 0; iz < NGLL; iz++)
+        for (int ix = 0; ix < NGLL; ix++) {
+          const float hl = hx[ix] * hg[iz];
+          arr[iz * NGLL + ix] = std::sin(pi / 180 * ang) * hl;
+          arr[npe + iz * NGLL + ix] = -1.0 * std::cos(pi / 180 * ang) * hl;
+        }
+    } else if (src[s].kind == SFX_SOURCE_MOMENT_TENSOR) {
+      float xixs = 0, gammaxs = 0, xizs = 0, gammazs = 0;
+      for (int iz = 0; iz < NGLL; iz++)
+        for (int ix = 0; ix < NGLL; ix++) {
+          const float hl = hx[ix] * hg[iz];
+          const size_t p = static_cast<size_t>(e) * npe + iz * NGLL + ix;
+          xixs += hl * a.xix[p];
+          gammaxs += hl * a.gammax[p];
+          xizs += hl * a.xiz[p];
+          gammazs += hl * a.gammaz[p];
+        }
+      const float Mxx = static_cast<float>(src[s].m[0]), Mzz = static_cast<float>(src[s].m[1]),
+                  Mxz = static_cast<float>(src[s].m[2]);
+      for (int iz = 0; iz < NGLL; iz++)
+        for (int ix = 0; ix < NGLL; ix++) {
+          const float dsx = (hpx[ix] * xixs) * hg[iz] + hx[ix] * (hpg[iz] * gammaxs);
+          const float dsz = (hpx[ix] * xizs) * hg[iz] + hx[ix] * (hpg[iz] * gammazs);
+          arr[iz * NGLL + ix] = Mxx * dsx + Mxz * dsz;
+          arr[npe + iz * NGLL + ix] = Mxz * dsx + Mzz * dsz;
+        }
+    } else {
+      throw std::invalid_argument("unknown source kind");
+    }
+  }
+  // application order: element type (no boundary, then Stacey), then source
+  // index (src/compute/compute_sources.cpp:232-270)
+  for (int pass = 0; pass < 2; pass++)
+    for (int s = 0; s < nsrc; s++)
+      if ((a.src_ispec[s] >= a.nspec_none) == (pass == 1)) a.src_order.push_back(s);
+
+  a.nrec = nrec;
+  a.rec_ispec.resize(nrec);
+  a.rec_h.resize(static_cast<size_t>(nrec) * npe);
+  a.rec_local.resize(2 * static_cast<size_t>(nrec));
+  for (int r = 0; r < nrec; r++) {
+    int e;
+    float xi, gamma;
+    loc.locate(static_cast<float>(rec[r].x), static_cast<float>(rec[r].z), e, xi, gamma);
+    a.rec_ispec[r] = e;
+    a.rec_local[2 * r] = xi;
+    a.rec_local[2 * r + 1] = gamma;
+    float hx[5], hg[5];
+    lagrange<float>(xi, a.xi, hx, nullptr);
+    lagrange<float>(gamma, a.xi, hg, nullptr);
+    for (int iz = 0; iz < NGLL; iz++)
+      for (int ix = 0; ix < NGLL; ix++) a.rec_h[static_cast<size_t>(r) * npe + iz * NGLL + ix] = hx[ix] * hg[iz];
+  }
+  return a;
+}
+
+// ---------------------------------------------------------------------------
+// 3-D structured hexahedra
+// ---------------------------------------------------------------------------
+namespace {
+
+// Element containing a point, and its reference coordinates, on the
+// structured grid: the exact inverse of x = x0 + (e + (1 + xi) / 2) h.  A point
+// on an element face belongs to the element above it, except on the last face.
+void locate_structured(double x, double x0, double h, int n, int &e, double &xi) {
+  const double s = (x - x0) / h;
+  int k = static_cast<int>(std::floor(s));
+  k = std::min(std::max(k, 0), n - 1);
+  e = k;
+  xi = 2.0 * (s - k) - 1.0;
+  xi = std::min(std::max(xi, -1.0), 1.0);
+}
+
+}  // namespace
+
+HostAssembly build_assembly_3d(const sfx_mesh3d &m, int nsrc, const sfx_source *src, int nrec,
+                               const sfx_receiver *rec, const sfx_run_desc &run) {
+  check_run(run);
+  if (m.nx <= 0 || m.ny <= 0 || m.nz <= 0) throw std::invalid_argument("empty 3-D mesh");
+  if (!(m.hx > 0 && m.hy > 0 && m.hz > 0)) throw std::invalid_argument("element sizes must be positive");
+  if (!m.material || !m.rho || !m.vp || !m.vs) throw std::invalid_argument("null mesh array");
+  for (int i = 0; i < m.nmaterials; i++)
+    if (!(m.vs[i] > 0) || !(m.rho[i] > 0) || !(m.vp[i] > 0))
+      throw std::invalid_argument("only elastic isotropic materials (vs > 0) are supported");
+  HostAssembly a;
+  a.dim = 3;
+  a.ncomp = 3;
+  a.npe = 125;
+  a.nx = m.nx;
+  a.ny = m.ny;
+  a.nz = m.nz;
+  const int64_t nspec64 = static_cast<int64_t>(m.nx) * m.ny * m.nz;
+  if (nspec64 > (1 << 27)) throw std::invalid_argument("3-D mesh too large");
+  a.nspec = static_cast<int>(nspec64);
+  a.nspec_none = a.nspec;
+  const Gll g = make_gll();
+  std::copy(g.xi, g.xi + 5, a.xi);
+  std::copy(g.w, g.w + 5, a.w);
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j < 5; j++) a.hprime[i][j] = g.hprime[i][j];
+  const int nspec = a.nspec, npe = 125;
+  a.compute_to_mesh.resize(nspec);
+  for (int e = 0; e < nspec; e++) a.compute_to_mesh[e] = e;
+
+  // Numbering: first visit in (element, k, j, i) order over the lattice of GLL
+  // points, the 3-D analogue of compute_mesh.cpp:45-170 with chunk size 1.
+  // Points shared by neighbouring elements are identified by their lattice
+  // position (4 ex + i, 4 ey + j, 4 ez + k), which is exact on this mesh.
+  const int64_t LX = 4 * static_cast<int64_t>(m.nx) + 1, LY = 4 * static_cast<int64_t>(m.ny) + 1,
+                LZ = 4 * static_cast<int64_t>(m.nz) + 1;
+  const int64_t nlat = LX * LY * LZ;
+  if (nlat > (int64_t(1) << 31) - 1) throw std::invalid_argument("3-D mesh too large");
+  std::vector<int32_t> first(nlat, -1);
+  const size_t nloc = static_cast<size_t>(nspec) * npe;
+  a.index_mapping.resize(nloc);
+  int inum = 0;
+  for (int ez = 0; ez < m.nz; ez++)
+    for (int ey = 0; ey < m.ny; ey++)
+      for (int ex = 0; ex < m.nx; ex++) {
+        const size_t e = (static_cast<size_t>(ez) * m.ny + ey) * m.nx + ex;
+        for (int k = 0; k < 5; k++)
+          for (int j = 0; j < 5; j++)
+            for (int i = 0; i < 5; i++) {
+              const int64_t L = ((4 * static_cast<int64_t>(ez) + k) * LY + (4 * ey + j)) * LX + (4 * ex + i);
+              int32_t &f = first[L];
+              if (f < 0) f = inum++;
+              a.index_mapping[e * npe + (k * 5 + j) * 5 + i] = f;
+            }
+      }
+  a.nglob = inum;
+  a.nglob_pad = (a.nglob + 63) / 64 * 64;
+  a.field_index.resize(a.nglob);
+  for (int i = 0; i < a.nglob; i++) a.field_index[i] = i;
+  a.gather = a.index_mapping;
+
+  // Regular geometry: d(xi)/dx = 2 / hx etc., jacobian hx hy hz / 8
+  // (the regular-element compression of the 3-D database,
+  // src/io/mesh/impl/fortran/dim3/mesh.cpp:150-151).
+  a.xix3 = static_cast<float>(2.0 / m.hx);
+  a.ety3 = static_cast<float>(2.0 / m.hy);
+  a.gaz3 = static_cast<float>(2.0 / m.hz);
+  a.jac3 = static_cast<float>(m.hx * m.hy * m.hz / 8.0);
+
+  // Properties per point (per-point kappa/mu/rho of the 3-D database,
+  // mesh.cpp:208-209,265), kappa = rho (vp^2 - 4/3 vs^2), mu = rho vs^2.
+  a.kappa.resize(nloc);
+  a.mu.resize(nloc);
+  a.rho.resize(nloc);
+  for (int e = 0; e < nspec; e++) {
+    const int mat = m.material[e];
+    if (mat < 0 || mat >= m.nmaterials) throw std::invalid_argument("material index out of range");
+    const float rho = static_cast<float>(m.rho[mat]);
+    const float cs = static_cast<float>(m.vs[mat]), cp = static_cast<float>(m.vp[mat]);
+    const float mu = rho * cs * cs;
+    const float kappa = rho * (cp * cp - (4.0 / 3.0) * cs * cs);
+    std::fill(&a.kappa[static_cast<size_t>(e) * npe], &a.kappa[static_cast<size_t>(e) * npe] + npe, kappa);
+    std::fill(&a.mu[static_cast<size_t>(e) * npe], &a.mu[static_cast<size_t>(e) * npe] + npe, mu);
+    std::fill(&a.rho[static_cast<size_t>(e) * npe], &a.rho[static_cast<size_t>(e) * npe] + npe, rho);
+  }
+
+  // Stacey faces: one entry per element point of every absorbing face (a point
+  // on two absorbing faces of one element gets both terms, as the face lists
+  // of the 3-D database do).  Normal = outward unit normal times the face
+  // jacobian; weight = product of the two in-face GLL weights.
+  const double fx = m.hy * m.hz / 4.0, fy = m.hx * m.hz / 4.0, fz = m.hx * m.hy / 4.0;
+  auto add_face = [&](int e, int face) {
+    for (int b = 0; b < 5; b++)
+      for (int c = 0; c < 5; c++) {
+        int i, j, k;
+        float n[3] = {0, 0, 0};
+        float w;
+        switch (face) {
+          case 0: i = 0; j = c; k = b; n[0] = static_cast<float>(-fx); w = a.w[j] * a.w[k]; break;
+          case 1: i = 4; j = c; k = b; n[0] = static_cast<float>(fx); w = a.w[j] * a.w[k]; break;
+          case 2: j = 0; i = c; k = b; n[1] = static_cast<float>(-fy); w = a.w[i] * a.w[k]; break;
+          case 3: j = 4; i = c; k = b; n[1] = static_cast<float>(fy); w = a.w[i] * a.w[k]; break;
+          case 4: k = 0; i = c; j = b; n[2] = static_cast<float>(-fz); w = a.w[i] * a.w[j]; break;
+          default: k = 4; i = c; j = b; n[2] = static_cast<float>(fz); w = a.w[i] * a.w[j]; break;
+        }
+        const size_t p = static_cast<size_t>(e) * npe + (k * 5 + j) * 5 + i;
+        a.bnd_gather.push_back(a.gather[p]);
+        a.bnd_normal.push_back(n[0]);
+        a.bnd_normal.push_back(n[1]);
+        a.bnd_normal.push_back(n[2]);
+        a.bnd_weight.push_back(w);
+        const float kap = a.kappa[p], mu = a.mu[p], rho = a.rho[p];
+        const float lp2m = kap + static_cast<float>(4.0 / 3.0) * mu;
+        a.bnd_rho_vp.push_back(std::sqrt(lp2m / rho) * rho);
+        a.bnd_rho_vs.push_back(std::sqrt(rho * mu));
+      }
+  };
+  for (int ez = 0; ez < m.nz; ez++)
+    for (int ey = 0; ey < m.ny; ey++)
+      for (int ex = 0; ex < m.nx; ex++) {
+        const int e = (ez * m.ny + ey) * m.nx + ex;
+        if (m.absorbing[0] && ex == 0) add_face(e, 0);
+        if (m.absorbing[1] && ex == m.nx - 1) add_face(e, 1);
+        if (m.absorbing[2] && ey == 0) add_face(e, 2);
+        if (m.absorbing[3] && ey == m.ny - 1) add_face(e, 3);
+        if (m.absorbing[4] && ez == 0) add_face(e, 4);
+        if (m.absorbing[5] && ez == m.nz - 1) add_face(e, 5);
+      }
+
+  // Mass matrix with the implicit Stacey term (the 3-D form of
+  // compute_mass_matrix.tpp:80-114 and stacey.tpp:641-677)
+  {
+    std::vector<float> mass(3 * static_cast<size_t>(a.nglob), 0.0f);
+    for (int e = 0; e < nspec; e++)
+      for (int k = 0; k < 5; k++)
+        for (int j = 0; j < 5; j++)
+          for (int i = 0; i < 5; i++) {
+            const size_t p = static_cast<size_t>(e) * npe + (k * 5 + j) * 5 + i;
+            const float f = a.w[i] * a.w[j] * a.w[k] * a.jac3;
+            const float mm = a.rho[p] * f;
+            const int gi = a.gather[p];
+            for (int c = 0; c < 3; c++) mass[static_cast<size_t>(c) * a.nglob + gi] += mm;
+          }
+    const float vfac = -1.0f * static_cast<float>(run.dt) * 0.5f;
+    for (size_t q = 0; q < a.bnd_gather.size(); q++) {
+      float t[3] = {0, 0, 0};
+      stacey_term<3>(&a.bnd_normal[3 * q], a.bnd_weight[q], a.bnd_rho_vp[q], a.bnd_rho_vs[q], vfac, t);
+      for (int c = 0; c < 3; c++) mass[static_cast<size_t>(c) * a.nglob + a.bnd_gather[q]] += t[c];
+    }
+    a.mass_inverse.resize(mass.size());
+    for (size_t i = 0; i < mass.size(); i++) a.mass_inverse[i] = 1.0f / mass[i];
+  }
+
+  // Sources (3-D force and moment tensor on the structured inverse map)
+  setup_time_function(a, nsrc, src, run);
+  a.nsrc = nsrc;
+  a.src_ispec.resize(nsrc);
+  a.src_array.assign(static_cast<size_t>(nsrc) * 3 * npe, 0.0f);
+  a.src_local.resize(3 * static_cast<size_t>(nsrc));
+  for (int s = 0; s < nsrc; s++) {
+    int ex, ey, ez;
+    double xi, eta, ga;
+    locate_structured(src[s].x, m.x0, m.hx, m.nx, ex, xi);
+    locate_structured(src[s].y, m.y0, m.hy, m.ny, ey, eta);
+    locate_structured(src[s].z, m.z0, m.hz, m.nz, ez, ga);
+    const int e = (ez * m.ny + ey) * m.nx + ex;
+    a.src_ispec[s] = e;
+    a.src_local[3 * s] = xi;
+    a.src_local[3 * s + 1] = eta;
+    a.src_local[3 * s + 2] = ga;
+    float hx[5], hpx[5], hy[5], hpy[5], hz[5], hpz[5];
+    lagrange<float>(static_cast<float>(xi), a.xi, hx, hpx);
+    lagrange<float>(static_cast<float>(eta), a.xi, hy, hpy);
+    lagrange<float>(static_cast<float>(ga), a.xi, hz, hpz);
+    float *arr = &a.src_array[static_cast<size_t>(s) * 3 * npe];
+    for (int k = 0; k < 5; k++)
+      for (int j = 0; j < 5; j++)
+        for (int i = 0; i < 5; i++) {
+          const int p = (k * 5 + j) * 5 + i;
+          if (src[s].kind == SFX_SOURCE_FORCE) {
+            const float h = hx[i] * hy[j] * hz[k];
+            arr[p] = static_cast<float>(src[s].fx) * h;
+            arr[npe + p] = static_cast<float>(src[s].fy) * h;
+            arr[2 * npe + p] = static_cast<float>(src[s].fz) * h;
+          } else if (src[s].kind == SFX_SOURCE_MOMENT_TENSOR) {
+            const float dx = hpx[i] * hy[j] * hz[k] * a.xix3;
+            const float dy = hx[i] * hpy[j] * hz[k] * a.ety3;
+            const float dz = hx[i] * hy[j] * hpz[k] * a.gaz3;
+            const float Mxx = static_cast<float>(src[s].m[0]), Myy = static_cast<float>(src[s].m[1]),
+                        Mzz = static_cast<float>(src[s].m[2]), Mxy = static_cast<float>(src[s].m[3]),
+                        Mxz = static_cast<float>(src[s].m[4]), Myz = static_cast<float>(src[s].m[5]);
+            arr[p] = Mxx * dx + Mxy * dy + Mxz * dz;
+            arr[npe + p] = Mxy * dx + Myy * dy + Myz * dz;
+            arr[2 * npe + p] = Mxz * dx + Myz * dy + Mzz * dz;
+          } else {
+            throw std::invalid_argument("unknown source kind");
+          }
+        }
+  }
+  for (int s = 0; s < nsrc; s++) a.src_order.push_back(s);
+
+  a.nrec = nrec;
+  a.rec_ispec.resize(nrec);
+  a.rec_h.resize(static_cast<size_t>(nrec) * npe);
+  a.rec_local.resize(3 * static_cast<size_t>(nrec));
+  for (int r = 0; r < nrec; r++) {
+    int ex, ey, ez;
+    double xi, eta, ga;
+    locate_structured(rec[r].x, m.x0, m.hx, m.nx, ex, xi);
+    locate_structured(rec[r].y, m.y0, m.hy, m.ny, ey, eta);
+    locate_structured(rec[r].z, m.z0, m.hz, m.nz, ez, ga);
+    a.rec_ispec[r] = (ez * m.ny + ey) * m.nx + ex;
+    a.rec_local[3 * r] = xi;
+    a.rec_local[3 * r + 1] = eta;
+    a.rec_local[3 * r + 2] = ga;
+    float hx[5], hy[5], hz[5];
+    lagrange<float>(static_cast<float>(xi), a.xi, hx, nullptr);
+    lagrange<float>(static_cast<float>(eta), a.xi, hy, nullptr);
+    lagrange<float>(static_cast<float>(ga), a.xi, hz, nullptr);
+    for (int k = 0; k < 5; k++)
+      for (int j = 0; j < 5; j++)
+        for (int i = 0; i < 5; i++) a.rec_h[static_cast<size_t>(r) * npe + (k * 5 + j) * 5 + i] = hx[i] * hy[j] * hz[k];
+  }
+  return a;
+}
+
+}  // namespace sfx

@@ -1,0 +1,74 @@
+// Host-side assembly: everything the device kernels read, laid out for the
+// GPU.  Built once per engine from the mesh description; the engine copies it
+// to HBM and keeps only the small pieces it returns through the C ABI.
+//
+// This is the build's counterpart of specfem::compute::assembly
+// (include/compute/assembly/assembly.hpp:34-119).  Layout differences from
+// the reference (all internal):
+//   - element arrays are element-major, point-minor ([e][p], p = (k,)j,i with i
+//     fastest) so that a wavefront reads them with unit stride;
+//   - the two-level index (index_mapping -> assembly_index_mapping,
+//     include/compute/fields/simulation_field.hpp:134-145) is composed on the
+//     host into one `gather` index per local point;
+//   - Stacey boundaries are a flat list of (element point, normal, weight,
+//     impedances) entries instead of per-element tags;
+//   - fields are component-major (ncomp, nglob_pad) like the reference's
+//     LayoutLeft (nglob, ncomp) views, padded to a multiple of 64 points.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/specfempp_amd.h"
+
+namespace sfx {
+
+struct HostAssembly {
+  int dim = 2, ncomp = 2, npe = 25;      // npe: GLL points per element
+  int nspec = 0, nglob = 0, nglob_pad = 0;
+  int nspec_none = 0;                    // 2-D: elements [0, nspec_none) carry no Stacey edge
+  // numbering
+  std::vector<int32_t> index_mapping;    // (nspec, npe) reference iglob, compute order
+  std::vector<int32_t> field_index;      // iglob -> per-medium field index
+  std::vector<int32_t> compute_to_mesh;  // (nspec)
+  std::vector<int32_t> gather;           // (nspec, npe) field index of each local point
+  // quadrature (float, the reference's type_real)
+  float xi[5], w[5], hprime[5][5];
+  // 2-D geometry per local point (compute_partial_derivatives.cpp:46-91)
+  std::vector<float> xix, xiz, gammax, gammaz, jac;
+  // 3-D regular geometry (all elements are boxes of the same size)
+  float xix3 = 0, ety3 = 0, gaz3 = 0, jac3 = 0;
+  // properties per local point (kappa, mu; rho for mass and Stacey)
+  std::vector<float> kappa, mu, rho;
+  // Stacey entries: one per (element, local point) on an absorbing edge/face
+  std::vector<int32_t> bnd_gather;
+  std::vector<float> bnd_normal;         // (nbnd, dim) outward normal scaled as the reference stores it
+  std::vector<float> bnd_weight;         // (nbnd) edge/face quadrature weight
+  std::vector<float> bnd_rho_vp, bnd_rho_vs;
+  // mass inverse (ncomp, nglob) on the field index
+  std::vector<float> mass_inverse;
+  // sources
+  int nsrc = 0;
+  std::vector<int32_t> src_ispec;
+  std::vector<float> src_array;          // (nsrc, ncomp, npe)
+  std::vector<float> stf;                // (nstep, nsrc)
+  std::vector<double> src_local;         // (nsrc, dim) reference coordinates
+  std::vector<int32_t> src_order;        // application order (element type, then source)
+  double t0 = 0;
+  // receivers
+  int nrec = 0;
+  std::vector<int32_t> rec_ispec;
+  std::vector<float> rec_h;              // (nrec, npe) interpolation weights
+  std::vector<double> rec_local;         // (nrec, dim)
+  // 3-D structured sizes
+  int nx = 0, ny = 0, nz = 0;
+};
+
+// Throw std::runtime_error / std::invalid_argument on bad input.
+HostAssembly build_assembly_2d(const sfx_mesh2d &mesh, int nsrc, const sfx_source *src, int nrec,
+                               const sfx_receiver *rec, const sfx_run_desc &run);
+HostAssembly build_assembly_3d(const sfx_mesh3d &mesh, int nsrc, const sfx_source *src, int nrec,
+                               const sfx_receiver *rec, const sfx_run_desc &run);
+
+}  // namespace sfx

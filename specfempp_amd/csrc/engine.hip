@@ -1,0 +1,471 @@
+// Engine object and C ABI (include/specfempp_amd.h).
+//
+// The engine owns every device array (the reference's assembly owns its
+// views, include/compute/assembly/assembly.hpp:34-119), drives the kernels of
+// kernels.hip on its own stream, and converts C++ exceptions into status
+// codes at the boundary.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/specfempp_amd.h"
+#include "assembly.hpp"
+#include "kernels.hpp"
+
+namespace sfx {
+namespace {
+
+thread_local std::string g_last_error;
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define SFX_HIP(call)                                                                                  \
+  do {                                                                                                 \
+    hipError_t err_ = (call);                                                                          \
+    if (err_ != hipSuccess)                                                                            \
+      throw HipError(std::string(#call) + " failed: " + hipGetErrorString(err_) + " (" __FILE__ ":" + \
+                     std::to_string(__LINE__) + ")");                                                  \
+  } while (0)
+
+template <class T> struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void alloc(size_t count) {
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+    }
+    n = count;
+    if (count) SFX_HIP(hipMalloc(&p, count * sizeof(T)));
+  }
+  void upload(const std::vector<T> &v, hipStream_t s) {
+    alloc(v.size());
+    if (!v.empty()) SFX_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+};
+
+}  // namespace
+}  // namespace sfx
+
+using namespace sfx;
+
+struct sfx_engine {
+  HostAssembly h;  // host copy (numbering and locations are returned from here)
+  sfx_run_desc run{};
+  hipStream_t stream = nullptr;
+  GllConst q{};
+  NewmarkConst nc{};
+  int nsig = 0;
+  // device arrays
+  DevBuf<int> gather, bnd_gather, src_order, src_ispec, rec_ispec, types;
+  DevBuf<float> xix, xiz, gammax, gammaz, jac, kappa, mu;
+  DevBuf<float> bnd_normal, bnd_weight, bnd_rvp, bnd_rvs;
+  DevBuf<float> src_array, stf, rec_h, seismo;
+  DevBuf<float> fields;  // u, v, a, minv: 4 * ncomp * nglob_pad
+  bool initialized = false;
+  std::vector<hipEvent_t> ev;
+  double timings[4] = {0, 0, 0, 0};
+
+  Fields f() const {
+    const size_t stride = static_cast<size_t>(h.ncomp) * h.nglob_pad;
+    return Fields{fields.p, fields.p + stride, fields.p + 2 * stride, fields.p + 3 * stride, h.nglob_pad};
+  }
+
+  ~sfx_engine() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void upload() {
+    SFX_HIP(hipStreamCreate(&stream));
+    hipStream_t s = stream;
+    gather.upload(h.gather, s);
+    kappa.upload(h.kappa, s);
+    mu.upload(h.mu, s);
+    if (h.dim == 2) {
+      xix.upload(h.xix, s);
+      xiz.upload(h.xiz, s);
+      gammax.upload(h.gammax, s);
+      gammaz.upload(h.gammaz, s);
+      jac.upload(h.jac, s);
+    }
+    bnd_gather.upload(h.bnd_gather, s);
+    bnd_normal.upload(h.bnd_normal, s);
+    bnd_weight.upload(h.bnd_weight, s);
+    bnd_rvp.upload(h.bnd_rho_vp, s);
+    bnd_rvs.upload(h.bnd_rho_vs, s);
+    src_order.upload(h.src_order, s);
+    src_ispec.upload(h.src_ispec, s);
+    src_array.upload(h.src_array, s);
+    stf.upload(h.stf, s);
+    rec_ispec.upload(h.rec_ispec, s);
+    rec_h.upload(h.rec_h, s);
+    std::vector<int> t(run.seismo_types, run.seismo_types + run.nseismo_types);
+    types.upload(t, s);
+    const int nsb = run.nstep_between_samples;
+    nsig = (run.nstep + nsb - 1) / nsb;
+    seismo.alloc(static_cast<size_t>(nsig) * run.nseismo_types * h.nrec * h.ncomp);
+    if (seismo.n) launch_fill(s, seismo.p, seismo.n, 0.0f);
+    const size_t stride = static_cast<size_t>(h.ncomp) * h.nglob_pad;
+    fields.alloc(4 * stride);
+    launch_fill(s, fields.p, 4 * stride, 0.0f);
+    // mass inverse into the padded layout (padding stays 0)
+    std::vector<float> minv(stride, 0.0f);
+    for (int c = 0; c < h.ncomp; c++)
+      std::memcpy(&minv[static_cast<size_t>(c) * h.nglob_pad], &h.mass_inverse[static_cast<size_t>(c) * h.nglob],
+                  h.nglob * sizeof(float));
+    SFX_HIP(hipMemcpyAsync(fields.p + 3 * stride, minv.data(), stride * sizeof(float), hipMemcpyHostToDevice, s));
+    SFX_HIP(hipStreamSynchronize(s));
+    // quadrature constants
+    for (int i = 0; i < 5; i++) {
+      q.w[i] = h.w[i];
+      for (int l = 0; l < 5; l++) {
+        q.hp[i][l] = h.hprime[i][l];
+        q.hpw[i][l] = h.hprime[l][i] * h.w[l];
+      }
+    }
+    const float dt = static_cast<float>(run.dt);
+    nc.dt = dt;
+    nc.dto2 = dt / 2.0;
+    nc.dt2o2 = dt * dt / 2.0;
+  }
+
+  void stiffness(Fields ff) {
+    if (h.dim == 2)
+      launch_stiffness_2d(stream, h.nspec, gather.p, xix.p, xiz.p, gammax.p, gammaz.p, jac.p, kappa.p, mu.p, q, ff);
+    else
+      launch_stiffness_3d(stream, h.nspec, gather.p, kappa.p, mu.p, h.xix3, h.ety3, h.gaz3, h.jac3, q, ff);
+  }
+  void boundary(Fields ff) {
+    launch_stacey(stream, h.dim, static_cast<int>(h.bnd_gather.size()), bnd_gather.p, bnd_normal.p, bnd_weight.p,
+                  bnd_rvp.p, bnd_rvs.p, ff);
+  }
+  void sources(Fields ff, int istep) {
+    if (istep < 0 || istep >= run.nstep) throw std::invalid_argument("istep out of range");
+    launch_sources(stream, h.dim, h.npe, h.nsrc, istep, src_order.p, src_ispec.p, gather.p, src_array.p, stf.p, ff);
+  }
+  void seismograms(Fields ff, int isig, int corrected) {
+    if (isig < 0 || isig >= nsig) throw std::invalid_argument("seismogram step out of range");
+    launch_seismograms(stream, h.dim, h.npe, h.nrec, run.nseismo_types, types.p, isig, rec_ispec.p, gather.p, rec_h.p,
+                       ff, nc.dto2, corrected, seismo.p);
+  }
+  void require_init() const {
+    if (!initialized) throw std::logic_error("sfx_initialize must be called first");
+  }
+
+  // One fused Newmark step; `evs` (5 events) brackets the kernel families.
+  void step(int istep, hipEvent_t *evs) {
+    Fields ff = f();
+    if (evs) SFX_HIP(hipEventRecord(evs[0], stream));
+    sources(ff, istep);
+    if (evs) SFX_HIP(hipEventRecord(evs[1], stream));
+    stiffness(ff);
+    if (evs) SFX_HIP(hipEventRecord(evs[2], stream));
+    boundary(ff);
+    if (istep % run.nstep_between_samples == 0) seismograms(ff, istep / run.nstep_between_samples, 1);
+    if (evs) SFX_HIP(hipEventRecord(evs[3], stream));
+    launch_update_fused(stream, h.ncomp, ff, nc);
+    if (evs) SFX_HIP(hipEventRecord(evs[4], stream));
+  }
+};
+
+namespace {
+
+template <class F> int guarded(F &&fn) {
+  try {
+    fn();
+    return SFX_OK;
+  } catch (const HipError &e) {
+    g_last_error = e.what();
+    return SFX_ERR_HIP;
+  } catch (const std::invalid_argument &e) {
+    g_last_error = e.what();
+    return SFX_ERR_ARG;
+  } catch (const std::logic_error &e) {
+    g_last_error = e.what();
+    return SFX_ERR_STATE;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return SFX_ERR_MESH;
+  }
+}
+
+void check_engine(const sfx_engine *e) {
+  if (!e) throw std::invalid_argument("null engine");
+}
+
+template <class Build> int create(Build &&build, const sfx_run_desc *run, sfx_engine **out) {
+  return guarded([&] {
+    if (!run || !out) throw std::invalid_argument("null argument");
+    *out = nullptr;
+    auto e = std::make_unique<sfx_engine>();
+    e->run = *run;
+    e->h = build();
+    e->upload();
+    *out = e.release();
+  });
+}
+
+}  // namespace
+
+extern "C" {
+
+int sfx_version(void) { return 100; }
+
+const char *sfx_last_error(void) { return g_last_error.c_str(); }
+
+int sfx_engine_create_2d(const sfx_mesh2d *mesh, int nsources, const sfx_source *sources, int nreceivers,
+                         const sfx_receiver *receivers, const sfx_run_desc *run, sfx_engine **out) {
+  return create(
+      [&] {
+        if (!mesh) throw std::invalid_argument("null mesh");
+        if (nsources < 0 || nreceivers < 0) throw std::invalid_argument("negative count");
+        if ((nsources && !sources) || (nreceivers && !receivers)) throw std::invalid_argument("null array");
+        return build_assembly_2d(*mesh, nsources, sources, nreceivers, receivers, *run);
+      },
+      run, out);
+}
+
+int sfx_engine_create_3d(const sfx_mesh3d *mesh, int nsources, const sfx_source *sources, int nreceivers,
+                         const sfx_receiver *receivers, const sfx_run_desc *run, sfx_engine **out) {
+  return create(
+      [&] {
+        if (!mesh) throw std::invalid_argument("null mesh");
+        if (nsources < 0 || nreceivers < 0) throw std::invalid_argument("negative count");
+        if ((nsources && !sources) || (nreceivers && !receivers)) throw std::invalid_argument("null array");
+        return build_assembly_3d(*mesh, nsources, sources, nreceivers, receivers, *run);
+      },
+      run, out);
+}
+
+void sfx_engine_destroy(sfx_engine *engine) {
+  if (!engine) return;
+  if (engine->stream) (void)hipStreamSynchronize(engine->stream);
+  delete engine;
+}
+
+int sfx_get_sizes(const sfx_engine *e, int64_t *s) {
+  return guarded([&] {
+    check_engine(e);
+    if (!s) throw std::invalid_argument("null output");
+    s[0] = e->h.dim;
+    s[1] = e->h.nspec;
+    s[2] = e->h.nglob;
+    s[3] = 5;
+    s[4] = e->nsig;
+    s[5] = e->h.nrec;
+    s[6] = e->h.nsrc;
+  });
+}
+
+int sfx_get_t0(const sfx_engine *e, double *t0) {
+  return guarded([&] {
+    check_engine(e);
+    if (!t0) throw std::invalid_argument("null output");
+    *t0 = e->h.t0;
+  });
+}
+
+int sfx_get_index_mapping(const sfx_engine *e, int32_t *out) {
+  return guarded([&] {
+    check_engine(e);
+    if (!out) throw std::invalid_argument("null output");
+    std::memcpy(out, e->h.index_mapping.data(), e->h.index_mapping.size() * sizeof(int32_t));
+  });
+}
+
+int sfx_get_field_index(const sfx_engine *e, int32_t *out) {
+  return guarded([&] {
+    check_engine(e);
+    if (!out) throw std::invalid_argument("null output");
+    std::memcpy(out, e->h.field_index.data(), e->h.field_index.size() * sizeof(int32_t));
+  });
+}
+
+int sfx_get_compute_to_mesh(const sfx_engine *e, int32_t *out) {
+  return guarded([&] {
+    check_engine(e);
+    if (!out) throw std::invalid_argument("null output");
+    std::memcpy(out, e->h.compute_to_mesh.data(), e->h.compute_to_mesh.size() * sizeof(int32_t));
+  });
+}
+
+int sfx_get_locations(const sfx_engine *e, int which, int32_t *ispec, double *local) {
+  return guarded([&] {
+    check_engine(e);
+    if (which != 0 && which != 1) throw std::invalid_argument("which must be 0 or 1");
+    const auto &is = which == 0 ? e->h.src_ispec : e->h.rec_ispec;
+    const auto &lc = which == 0 ? e->h.src_local : e->h.rec_local;
+    if (ispec) std::memcpy(ispec, is.data(), is.size() * sizeof(int32_t));
+    if (local) std::memcpy(local, lc.data(), lc.size() * sizeof(double));
+  });
+}
+
+int sfx_initialize(sfx_engine *e) {
+  return guarded([&] {
+    check_engine(e);
+    const size_t stride = static_cast<size_t>(e->h.ncomp) * e->h.nglob_pad;
+    launch_fill(e->stream, e->fields.p, 3 * stride, 0.0f);
+    if (e->seismo.n) launch_fill(e->stream, e->seismo.p, e->seismo.n, 0.0f);
+    SFX_HIP(hipGetLastError());
+    e->initialized = true;
+  });
+}
+
+int sfx_predictor(sfx_engine *e) {
+  int dofs = 0;
+  const int rc = guarded([&] {
+    check_engine(e);
+    e->require_init();
+    launch_predictor(e->stream, e->h.ncomp, e->f(), e->nc);
+    SFX_HIP(hipGetLastError());
+    dofs = e->h.nglob * e->h.ncomp;
+  });
+  return rc == SFX_OK ? dofs : rc;
+}
+
+int sfx_update_wavefields(sfx_engine *e, int istep) {
+  int nelem = 0;
+  const int rc = guarded([&] {
+    check_engine(e);
+    e->require_init();
+    Fields ff = e->f();
+    e->sources(ff, istep);
+    e->stiffness(ff);
+    e->boundary(ff);
+    launch_divide_corrector(e->stream, e->h.ncomp, ff, e->nc, 1, 0);
+    SFX_HIP(hipGetLastError());
+    nelem = e->h.nspec;
+  });
+  return rc == SFX_OK ? nelem : rc;
+}
+
+int sfx_corrector(sfx_engine *e) {
+  int dofs = 0;
+  const int rc = guarded([&] {
+    check_engine(e);
+    e->require_init();
+    launch_divide_corrector(e->stream, e->h.ncomp, e->f(), e->nc, 0, 1);
+    SFX_HIP(hipGetLastError());
+    dofs = e->h.nglob * e->h.ncomp;
+  });
+  return rc == SFX_OK ? dofs : rc;
+}
+
+int sfx_compute_seismograms(sfx_engine *e, int isig) {
+  return guarded([&] {
+    check_engine(e);
+    e->require_init();
+    e->seismograms(e->f(), isig, 0);
+    SFX_HIP(hipGetLastError());
+  });
+}
+
+int sfx_step(sfx_engine *e, int istep) {
+  return guarded([&] {
+    check_engine(e);
+    e->require_init();
+    e->step(istep, nullptr);
+    SFX_HIP(hipGetLastError());
+  });
+}
+
+int sfx_run(sfx_engine *e, int first, int count) {
+  return guarded([&] {
+    check_engine(e);
+    e->require_init();
+    if (first < 0 || count < 0 || first + count > e->run.nstep) throw std::invalid_argument("step range out of bounds");
+    // Every step records 5 events on the engine stream; they are read after
+    // the last step, so the host never waits inside the loop.
+    const size_t need = 5 * static_cast<size_t>(count);
+    while (e->ev.size() < need) {
+      hipEvent_t ev;
+      SFX_HIP(hipEventCreate(&ev));
+      e->ev.push_back(ev);
+    }
+    for (int i = 0; i < count; i++) e->step(first + i, &e->ev[5 * static_cast<size_t>(i)]);
+    SFX_HIP(hipGetLastError());
+    SFX_HIP(hipStreamSynchronize(e->stream));
+    double acc[4] = {0, 0, 0, 0};
+    for (int i = 0; i < count; i++) {
+      hipEvent_t *v = &e->ev[5 * static_cast<size_t>(i)];
+      float stiff, upd, src, bnd, all;
+      SFX_HIP(hipEventElapsedTime(&src, v[0], v[1]));
+      SFX_HIP(hipEventElapsedTime(&stiff, v[1], v[2]));
+      SFX_HIP(hipEventElapsedTime(&bnd, v[2], v[3]));
+      SFX_HIP(hipEventElapsedTime(&upd, v[3], v[4]));
+      SFX_HIP(hipEventElapsedTime(&all, v[0], v[4]));
+      acc[0] += stiff;
+      acc[1] += upd;
+      acc[2] += src + bnd;
+      acc[3] += all;
+    }
+    for (int k = 0; k < 4; k++) e->timings[k] = count ? acc[k] / count : 0.0;
+  });
+}
+
+int sfx_synchronize(sfx_engine *e) {
+  return guarded([&] {
+    check_engine(e);
+    SFX_HIP(hipStreamSynchronize(e->stream));
+  });
+}
+
+int sfx_get_seismograms(const sfx_engine *e, float *out) {
+  return guarded([&] {
+    check_engine(e);
+    if (!out) throw std::invalid_argument("null output");
+    SFX_HIP(hipStreamSynchronize(e->stream));
+    if (e->seismo.n) SFX_HIP(hipMemcpy(out, e->seismo.p, e->seismo.n * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+int sfx_get_field(const sfx_engine *e, int which, float *out) {
+  return guarded([&] {
+    check_engine(e);
+    if (which < 0 || which > 3) throw std::invalid_argument("which must be 0..3");
+    if (!out) throw std::invalid_argument("null output");
+    SFX_HIP(hipStreamSynchronize(e->stream));
+    const size_t stride = static_cast<size_t>(e->h.ncomp) * e->h.nglob_pad;
+    for (int c = 0; c < e->h.ncomp; c++)
+      SFX_HIP(hipMemcpy(out + static_cast<size_t>(c) * e->h.nglob,
+                        e->fields.p + which * stride + static_cast<size_t>(c) * e->h.nglob_pad,
+                        e->h.nglob * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+int sfx_set_field(sfx_engine *e, int which, const float *in) {
+  return guarded([&] {
+    check_engine(e);
+    if (which < 0 || which > 2) throw std::invalid_argument("which must be 0..2");
+    if (!in) throw std::invalid_argument("null input");
+    SFX_HIP(hipStreamSynchronize(e->stream));
+    const size_t stride = static_cast<size_t>(e->h.ncomp) * e->h.nglob_pad;
+    for (int c = 0; c < e->h.ncomp; c++)
+      SFX_HIP(hipMemcpy(e->fields.p + which * stride + static_cast<size_t>(c) * e->h.nglob_pad,
+                        in + static_cast<size_t>(c) * e->h.nglob, e->h.nglob * sizeof(float),
+                        hipMemcpyHostToDevice));
+  });
+}
+
+int sfx_get_timings(const sfx_engine *e, double *out4) {
+  return guarded([&] {
+    check_engine(e);
+    if (!out4) throw std::invalid_argument("null output");
+    for (int k = 0; k < 4; k++) out4[k] = e->timings[k];
+  });
+}
+
+}  // extern "C"

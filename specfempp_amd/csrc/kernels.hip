@@ -1,0 +1,478 @@
+// Newmark-step kernels for MI355X (gfx950, wave64).
+//
+// Reference kernels these replace (include/kokkos_kernels/impl/):
+//   stiffness_2d / stiffness_3d  compute_stiffness_interaction.tpp:143-259
+//   stacey                       the Stacey branch of the same kernel
+//                                (boundary_conditions/stacey/stacey.tpp:36-72)
+//   sources                      compute_source_interaction.tpp:80-95
+//   seismograms                  compute_seismogram.tpp:103-135
+//   update_fused                 divide_mass_matrix.tpp:42-49 + newmark.tpp:11-116
+//                                (corrector of step n and predictor of n+1 in one pass)
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace sfx {
+
+namespace {
+
+__device__ __forceinline__ void atomic_add(float *p, float v) { unsafeAtomicAdd(p, v); }
+
+// ---------------------------------------------------------------------------
+// 2-D stiffness: 5 elements per 128-thread block, one thread per GLL point.
+// The element's displacement and its stress integrand F are staged in LDS
+// (the reference's chunk_element field and stress_integrand).
+// ---------------------------------------------------------------------------
+constexpr int E2 = 5;  // elements per block
+
+__global__ __launch_bounds__(128) void stiffness_2d_kernel(
+    int nspec, const int *__restrict__ gather, const float *__restrict__ xix_, const float *__restrict__ xiz_,
+    const float *__restrict__ gammax_, const float *__restrict__ gammaz_, const float *__restrict__ jac_,
+    const float *__restrict__ kappa_, const float *__restrict__ mu_, GllConst q, Fields f) {
+  __shared__ float su[E2][2][25];
+  __shared__ float sF[E2][4][25];  // F(c, d) as [2c + d]
+
+  const int t = threadIdx.x;
+  const int el = t / 25;
+  const int p = t - el * 25;
+  const int iz = p / 5, ix = p - (p / 5) * 5;
+  const int e = blockIdx.x * E2 + el;
+  const bool active = (el < E2) && (e < nspec);
+  const size_t idx = static_cast<size_t>(e) * 25 + p;
+
+  int g = 0;
+  if (active) {
+    g = gather[idx];
+    su[el][0][p] = f.u[g];
+    su[el][1][p] = f.u[f.nglob_pad + g];
+  }
+  __syncthreads();
+
+  // Per-thread rows of the derivative matrices, loaded once (the row index is
+  // lane-dependent, so they are held in VGPRs rather than re-read).
+  float hpx[5], hpz[5], hwx[5], hwz[5];
+#pragma unroll
+  for (int l = 0; l < 5; l++) {
+    hpx[l] = q.hp[ix][l];
+    hpz[l] = q.hp[iz][l];
+    hwx[l] = q.hpw[ix][l];
+    hwz[l] = q.hpw[iz][l];
+  }
+  const float wx = q.w[ix], wz = q.w[iz];
+  float xix = 0, xiz = 0, gammax = 0, gammaz = 0, J = 0, kappa = 0, mu = 0;
+  if (active) {
+    xix = xix_[idx];
+    xiz = xiz_[idx];
+    gammax = gammax_[idx];
+    gammaz = gammaz_[idx];
+    J = jac_[idx];
+    kappa = kappa_[idx];
+    mu = mu_[idx];
+    float dxi[2] = {0.0f, 0.0f}, dga[2] = {0.0f, 0.0f};
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        dxi[c] += hpx[l] * su[el][c][iz * 5 + l];
+        dga[c] += hpz[l] * su[el][c][l * 5 + ix];
+      }
+    }
+    float du[2][2];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      du[c][0] = xix * dxi[c] + gammax * dga[c];
+      du[c][1] = xiz * dxi[c] + gammaz * dga[c];
+    }
+    // include/medium/dim2/elastic/isotropic/stress.hpp:25-48 with
+    // lambda + 2 mu = kappa + 4/3 mu (core/specfem/point/properties.hpp:141-161)
+    const float lp2m = kappa + (4.0f / 3.0f) * mu;
+    const float lambda = lp2m - 2.0f * mu;
+    const float sxx = lp2m * du[0][0] + lambda * du[1][1];
+    const float szz = lp2m * du[1][1] + lambda * du[0][0];
+    const float sxz = mu * (du[0][1] + du[1][0]);
+    // core/specfem/point/stress.hpp:129-136
+    sF[el][0][p] = J * (sxx * xix + sxz * xiz);
+    sF[el][1][p] = J * (sxx * gammax + sxz * gammaz);
+    sF[el][2][p] = J * (sxz * xix + szz * xiz);
+    sF[el][3][p] = J * (sxz * gammax + szz * gammaz);
+  }
+  __syncthreads();
+  if (!active) return;
+  // include/algorithms/divergence.hpp:86-98
+  float t1[2] = {0.0f, 0.0f}, t2[2] = {0.0f, 0.0f};
+#pragma unroll
+  for (int l = 0; l < 5; l++) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      t1[c] += sF[el][2 * c][iz * 5 + l] * hwx[l];
+      t2[c] += sF[el][2 * c + 1][l * 5 + ix] * hwz[l];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const float r = -(wz * t1[c] + wx * t2[c]);
+    atomic_add(&f.a[c * f.nglob_pad + g], r);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3-D stiffness on regular hexahedra.  One thread owns a (i, j) column of an
+// element and keeps its 5 points along k in registers: the gamma-direction
+// contractions never touch LDS; xi and eta contractions read the element's
+// displacement and stress integrand from LDS.  5 elements per 128-thread block
+// (125 active threads).
+// ---------------------------------------------------------------------------
+constexpr int E3 = 5;
+
+__global__ __launch_bounds__(128) void stiffness_3d_kernel(int nspec, const int *__restrict__ gather,
+                                                           const float *__restrict__ kappa_,
+                                                           const float *__restrict__ mu_, float xix, float ety,
+                                                           float gaz, float jac, GllConst q, Fields f) {
+  __shared__ float su[E3][3][125];
+  __shared__ float sF0[E3][3][125];  // J xix sigma(c, x)
+  __shared__ float sF1[E3][3][125];  // J ety sigma(c, y)
+
+  const int t = threadIdx.x;
+  const int el = t / 25;
+  const int col = t - el * 25;
+  const int j = col / 5, i = col - (col / 5) * 5;
+  const int e = blockIdx.x * E3 + el;
+  const bool active = (el < E3) && (e < nspec);
+  const size_t base = static_cast<size_t>(e) * 125 + col;
+  const int np = f.nglob_pad;
+
+  float hpi[5], hpj[5], hwi[5], hwj[5];
+#pragma unroll
+  for (int l = 0; l < 5; l++) {
+    hpi[l] = q.hp[i][l];
+    hpj[l] = q.hp[j][l];
+    hwi[l] = q.hpw[i][l];
+    hwj[l] = q.hpw[j][l];
+  }
+  const float wi = q.w[i], wj = q.w[j];
+
+  int g[5];
+  float u[5][3];
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < 5; k++) g[k] = gather[base + k * 25];
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        u[k][c] = f.u[c * np + g[k]];
+        su[el][c][k * 25 + col] = u[k][c];
+      }
+    }
+  }
+  __syncthreads();
+
+  float F2[5][3];  // J gaz sigma(c, z), kept in registers
+  if (active) {
+    const float sx = jac * xix, sy = jac * ety, sz = jac * gaz;
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      float dxi[3] = {0, 0, 0}, det[3] = {0, 0, 0}, dga[3] = {0, 0, 0};
+#pragma unroll
+      for (int l = 0; l < 5; l++) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          dxi[c] += hpi[l] * su[el][c][k * 25 + j * 5 + l];
+          det[c] += hpj[l] * su[el][c][k * 25 + l * 5 + i];
+          dga[c] += q.hp[k][l] * u[l][c];
+        }
+      }
+      const float kappa = kappa_[base + k * 25];
+      const float mu = mu_[base + k * 25];
+      const float lp2m = kappa + (4.0f / 3.0f) * mu;
+      const float lambda = lp2m - 2.0f * mu;
+      const float exx = xix * dxi[0], eyy = ety * det[1], ezz = gaz * dga[2];
+      const float sxx = lp2m * exx + lambda * (eyy + ezz);
+      const float syy = lp2m * eyy + lambda * (exx + ezz);
+      const float szz = lp2m * ezz + lambda * (exx + eyy);
+      const float sxy = mu * (ety * det[0] + xix * dxi[1]);
+      const float sxz = mu * (gaz * dga[0] + xix * dxi[2]);
+      const float syz = mu * (gaz * dga[1] + ety * det[2]);
+      const int p = k * 25 + col;
+      sF0[el][0][p] = sx * sxx;
+      sF0[el][1][p] = sx * sxy;
+      sF0[el][2][p] = sx * sxz;
+      sF1[el][0][p] = sy * sxy;
+      sF1[el][1][p] = sy * syy;
+      sF1[el][2][p] = sy * syz;
+      F2[k][0] = sz * sxz;
+      F2[k][1] = sz * syz;
+      F2[k][2] = sz * szz;
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    float r0[3] = {0, 0, 0}, r1[3] = {0, 0, 0}, r2[3] = {0, 0, 0};
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        r0[c] += sF0[el][c][k * 25 + j * 5 + l] * hwi[l];
+        r1[c] += sF1[el][c][k * 25 + l * 5 + i] * hwj[l];
+        r2[c] += F2[l][c] * q.hpw[k][l];
+      }
+    }
+    const float wjk = wj * q.w[k], wik = wi * q.w[k], wij = wi * wj;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const float r = -(wjk * r0[c] + wik * r1[c] + wij * r2[c]);
+      atomic_add(&f.a[c * np + g[k]], r);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stacey absorbing traction on a flat list of boundary element points
+// (stacey.tpp:36-72): t = -[(v.n / |n|^2) n (rho vp - rho vs) + v rho vs] |n| w
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(256) void stacey_kernel(int nbnd, const int *__restrict__ gather,
+                                                     const float *__restrict__ normal,
+                                                     const float *__restrict__ weight,
+                                                     const float *__restrict__ rho_vp,
+                                                     const float *__restrict__ rho_vs, Fields f) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nbnd) return;
+  const int g = gather[q];
+  const int np = f.nglob_pad;
+  float n[D], v[D];
+  float vn = 0.0f, nn = 0.0f;
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    n[d] = normal[q * D + d];
+    v[d] = f.v[d * np + g];
+    vn += v[d] * n[d];
+    nn += n[d] * n[d];
+  }
+  const float j1 = sqrtf(nn);
+  const float w = weight[q], rvp = rho_vp[q], rvs = rho_vs[q];
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    const float fac = ((vn * n[d] / (j1 * j1)) * (rvp - rvs)) + v[d] * rvs;
+    atomic_add(&f.a[d * np + g], -1.0f * fac * j1 * w);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sources: a += stf(istep, s) * source_array(s, c, p)
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(128) void sources_kernel(int npe, int nsrc, int istep, const int *__restrict__ order,
+                                                      const int *__restrict__ ispec, const int *__restrict__ gather,
+                                                      const float *__restrict__ array,
+                                                      const float *__restrict__ stf, Fields f) {
+  const int p = threadIdx.x;
+  if (p >= npe) return;
+  for (int k = 0; k < nsrc; k++) {
+    const int s = order[k];
+    const float sv = stf[static_cast<size_t>(istep) * nsrc + s];
+    const int g = gather[static_cast<size_t>(ispec[s]) * npe + p];
+#pragma unroll
+    for (int c = 0; c < D; c++)
+      atomic_add(&f.a[c * f.nglob_pad + g], sv * array[(static_cast<size_t>(s) * D + c) * npe + p]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Seismograms: one block per receiver, Lagrange interpolation of the
+// element field, block reduction.
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(128) void seismogram_kernel(int npe, int nrec, int ntypes, const int *__restrict__ types,
+                                                         int isig, const int *__restrict__ ispec,
+                                                         const int *__restrict__ gather, const float *__restrict__ h,
+                                                         Fields f, float dto2, int corrected,
+                                                         float *__restrict__ seismo) {
+  __shared__ float red[2][D][64];
+  const int r = blockIdx.x;
+  const int p = threadIdx.x;
+  const int np = f.nglob_pad;
+  int g = 0;
+  float hw = 0.0f;
+  if (p < npe) {
+    g = gather[static_cast<size_t>(ispec[r]) * npe + p];
+    hw = h[static_cast<size_t>(r) * npe + p];
+  }
+  for (int it = 0; it < ntypes; it++) {
+    const int type = types[it];
+    float val[D];
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      float x = 0.0f;
+      if (p < npe) {
+        if (type == 0) {
+          x = f.u[c * np + g];
+        } else {
+          float acc = f.a[c * np + g];
+          if (corrected) acc = acc * f.minv[c * np + g];
+          if (type == 1) {
+            x = f.v[c * np + g];
+            if (corrected) x = x + dto2 * acc;
+          } else {
+            x = acc;
+          }
+        }
+      }
+      val[c] = hw * x;
+    }
+    // wave reduction then across the two waves
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      float s = val[c];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+      if ((p & 63) == 0) red[0][c][p >> 6] = s;
+    }
+    __syncthreads();
+    if (p == 0) {
+#pragma unroll
+      for (int c = 0; c < D; c++)
+        seismo[((static_cast<size_t>(isig) * ntypes + it) * nrec + r) * D + c] = red[0][c][0] + red[0][c][1];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused mass division + corrector(n) + predictor(n+1), 4 points per thread.
+//   a <- a M^-1; v <- v + dt/2 a            (divide_mass_matrix, corrector)
+//   u <- u + (dt v + dt^2/2 a); v <- v + dt/2 a; a <- 0   (predictor)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void update_fused_kernel(int n4, int ncomp, Fields f, NewmarkConst nc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n4) return;
+  for (int c = 0; c < ncomp; c++) {
+    const size_t o = static_cast<size_t>(c) * (f.nglob_pad / 4) + i;
+    float4 a = reinterpret_cast<const float4 *>(f.a)[o];
+    const float4 m = reinterpret_cast<const float4 *>(f.minv)[o];
+    float4 u = reinterpret_cast<const float4 *>(f.u)[o];
+    float4 v = reinterpret_cast<const float4 *>(f.v)[o];
+#define SFX_UPD(X)                                   \
+  {                                                  \
+    const float ac = a.X * m.X;                      \
+    const float v1 = v.X + nc.dto2 * ac;             \
+    u.X = u.X + (nc.dt * v1 + nc.dt2o2 * ac);        \
+    v.X = v1 + nc.dto2 * ac;                         \
+  }
+    SFX_UPD(x) SFX_UPD(y) SFX_UPD(z) SFX_UPD(w)
+#undef SFX_UPD
+    reinterpret_cast<float4 *>(f.u)[o] = u;
+    reinterpret_cast<float4 *>(f.v)[o] = v;
+    reinterpret_cast<float4 *>(f.a)[o] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+}
+
+// Predictor alone (newmark.tpp:60-116).
+__global__ __launch_bounds__(256) void predictor_kernel(int n, int ncomp, Fields f, NewmarkConst nc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < ncomp; c++) {
+    const size_t o = static_cast<size_t>(c) * f.nglob_pad + i;
+    const float a = f.a[o], v = f.v[o];
+    f.u[o] = f.u[o] + (nc.dt * v + nc.dt2o2 * a);
+    f.v[o] = v + nc.dto2 * a;
+    f.a[o] = 0.0f;
+  }
+}
+
+// Mass division (divide_mass_matrix.tpp:42-49) and/or corrector (newmark.tpp:11-56).
+__global__ __launch_bounds__(256) void divide_corrector_kernel(int n, int ncomp, Fields f, NewmarkConst nc,
+                                                               int divide, int correct) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < ncomp; c++) {
+    const size_t o = static_cast<size_t>(c) * f.nglob_pad + i;
+    float a = f.a[o];
+    if (divide) {
+      a = a * f.minv[o];
+      f.a[o] = a;
+    }
+    if (correct) f.v[o] = f.v[o] + nc.dto2 * a;
+  }
+}
+
+__global__ void fill_kernel(float *p, size_t n, float value) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = value;
+}
+
+inline unsigned blocks(size_t n, unsigned per) { return static_cast<unsigned>((n + per - 1) / per); }
+
+}  // namespace
+
+void launch_stiffness_2d(hipStream_t s, int nspec, const int *gather, const float *xix, const float *xiz,
+                         const float *gammax, const float *gammaz, const float *jac, const float *kappa,
+                         const float *mu, const GllConst &q, Fields f) {
+  if (nspec <= 0) return;
+  hipLaunchKernelGGL(stiffness_2d_kernel, dim3(blocks(nspec, E2)), dim3(128), 0, s, nspec, gather, xix, xiz, gammax,
+                     gammaz, jac, kappa, mu, q, f);
+}
+
+void launch_stiffness_3d(hipStream_t s, int nspec, const int *gather, const float *kappa, const float *mu, float xix,
+                         float ety, float gaz, float jac, const GllConst &q, Fields f) {
+  if (nspec <= 0) return;
+  hipLaunchKernelGGL(stiffness_3d_kernel, dim3(blocks(nspec, E3)), dim3(128), 0, s, nspec, gather, kappa, mu, xix,
+                     ety, gaz, jac, q, f);
+}
+
+void launch_stacey(hipStream_t s, int dim, int nbnd, const int *gather, const float *normal, const float *weight,
+                   const float *rho_vp, const float *rho_vs, Fields f) {
+  if (nbnd <= 0) return;
+  if (dim == 2)
+    hipLaunchKernelGGL(stacey_kernel<2>, dim3(blocks(nbnd, 256)), dim3(256), 0, s, nbnd, gather, normal, weight,
+                       rho_vp, rho_vs, f);
+  else
+    hipLaunchKernelGGL(stacey_kernel<3>, dim3(blocks(nbnd, 256)), dim3(256), 0, s, nbnd, gather, normal, weight,
+                       rho_vp, rho_vs, f);
+}
+
+void launch_sources(hipStream_t s, int dim, int npe, int nsrc, int istep, const int *order, const int *ispec,
+                    const int *gather, const float *array, const float *stf, Fields f) {
+  if (nsrc <= 0) return;
+  if (dim == 2)
+    hipLaunchKernelGGL(sources_kernel<2>, dim3(1), dim3(128), 0, s, npe, nsrc, istep, order, ispec, gather, array, stf,
+                       f);
+  else
+    hipLaunchKernelGGL(sources_kernel<3>, dim3(1), dim3(128), 0, s, npe, nsrc, istep, order, ispec, gather, array, stf,
+                       f);
+}
+
+void launch_seismograms(hipStream_t s, int dim, int npe, int nrec, int ntypes, const int *types, int isig,
+                        const int *ispec, const int *gather, const float *h, Fields f, float dto2, int corrected,
+                        float *seismo) {
+  if (nrec <= 0 || ntypes <= 0) return;
+  if (dim == 2)
+    hipLaunchKernelGGL(seismogram_kernel<2>, dim3(nrec), dim3(128), 0, s, npe, nrec, ntypes, types, isig, ispec,
+                       gather, h, f, dto2, corrected, seismo);
+  else
+    hipLaunchKernelGGL(seismogram_kernel<3>, dim3(nrec), dim3(128), 0, s, npe, nrec, ntypes, types, isig, ispec,
+                       gather, h, f, dto2, corrected, seismo);
+}
+
+void launch_update_fused(hipStream_t s, int ncomp, Fields f, NewmarkConst nc) {
+  const int n4 = f.nglob_pad / 4;
+  hipLaunchKernelGGL(update_fused_kernel, dim3(blocks(n4, 256)), dim3(256), 0, s, n4, ncomp, f, nc);
+}
+
+void launch_predictor(hipStream_t s, int ncomp, Fields f, NewmarkConst nc) {
+  hipLaunchKernelGGL(predictor_kernel, dim3(blocks(f.nglob_pad, 256)), dim3(256), 0, s, f.nglob_pad, ncomp, f, nc);
+}
+
+void launch_divide_corrector(hipStream_t s, int ncomp, Fields f, NewmarkConst nc, int divide, int correct) {
+  hipLaunchKernelGGL(divide_corrector_kernel, dim3(blocks(f.nglob_pad, 256)), dim3(256), 0, s, f.nglob_pad, ncomp, f,
+                     nc, divide, correct);
+}
+
+void launch_fill(hipStream_t s, float *p, size_t n, float value) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(fill_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, p, n, value);
+}
+
+}  // namespace sfx

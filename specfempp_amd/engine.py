@@ -1,0 +1,273 @@
+"""Host-side mirror of the SPECFEM++ Newmark solver interface over the C ABI.
+
+``Engine2D`` / ``Engine3D`` wrap one ``sfx_engine`` (include/specfempp_amd.h).
+Their methods carry the names of the reference seams they replace:
+
+=============================  ==================================================
+method                          reference
+=============================  ==================================================
+``initialize()``                ``domain_kernels::initialize(dt)``
+                                (include/kokkos_kernels/domain_kernels.hpp:136-163)
+``apply_predictor_phase()``     ``newmark::apply_predictor_phase_forward``
+                                (include/timescheme/newmark.tpp:140-159)
+``update_wavefields(istep)``    ``domain_kernels::update_wavefields<elastic>``
+                                (domain_kernels.hpp:69-126)
+``apply_corrector_phase()``     ``newmark::apply_corrector_phase_forward``
+                                (newmark.tpp:119-137)
+``compute_seismograms(isig)``   ``domain_kernels::compute_seismograms``
+                                (domain_kernels.hpp:172-186)
+``run()``                       ``time_marching<forward>::run``
+                                (include/solver/time_marching.tpp:9-124)
+=============================  ==================================================
+
+Errors raise :class:`SfxError` with the library's message.  The library is the
+HIP build in ``specfempp_amd/_lib``; there is no CPU fallback: without a GPU
+the engine constructor fails with the HIP error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _native
+
+DISPLACEMENT, VELOCITY, ACCELERATION = 0, 1, 2
+_TYPE_CODES = {"displacement": 0, "velocity": 1, "acceleration": 2}
+
+
+class SfxError(RuntimeError):
+    pass
+
+
+class _Mesh2D(C.Structure):
+    _fields_ = [("nspec", C.c_int), ("ngnod", C.c_int), ("npgeo", C.c_int), ("coorg", C.c_void_p),
+                ("knods", C.c_void_p), ("material", C.c_void_p), ("nmaterials", C.c_int), ("rho", C.c_void_p),
+                ("vp", C.c_void_p), ("vs", C.c_void_p), ("nabs", C.c_int), ("abs_ispec", C.c_void_p),
+                ("abs_edge", C.c_void_p)]
+
+
+class _Mesh3D(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("x0", C.c_double), ("y0", C.c_double),
+                ("z0", C.c_double), ("hx", C.c_double), ("hy", C.c_double), ("hz", C.c_double),
+                ("material", C.c_void_p), ("nmaterials", C.c_int), ("rho", C.c_void_p), ("vp", C.c_void_p),
+                ("vs", C.c_void_p), ("absorbing", C.c_int * 6)]
+
+
+class _Source(C.Structure):
+    _fields_ = [("kind", C.c_int), ("x", C.c_double), ("y", C.c_double), ("z", C.c_double), ("angle", C.c_double),
+                ("fx", C.c_double), ("fy", C.c_double), ("fz", C.c_double), ("m", C.c_double * 6),
+                ("f0", C.c_double), ("tshift", C.c_double), ("factor", C.c_double)]
+
+
+class _Receiver(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double), ("angle", C.c_double)]
+
+
+class _Run(C.Structure):
+    _fields_ = [("dt", C.c_double), ("nstep", C.c_int), ("nstep_between_samples", C.c_int),
+                ("nseismo_types", C.c_int), ("seismo_types", C.c_int * 3), ("chunk_size", C.c_int)]
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _sources(sources, dim):
+    arr = (_Source * max(len(sources), 1))()
+    for i, s in enumerate(sources):
+        kind = s.get("type", "force")
+        arr[i].kind = 1 if kind == "moment_tensor" else 0
+        arr[i].x = s["x"]
+        arr[i].y = s.get("y", 0.0)
+        arr[i].z = s["z"]
+        arr[i].angle = s.get("angle", 0.0)
+        arr[i].fx, arr[i].fy, arr[i].fz = s.get("fx", 0.0), s.get("fy", 0.0), s.get("fz", 0.0)
+        if dim == 2:
+            m = [s.get("mxx", 0.0), s.get("mzz", 0.0), s.get("mxz", 0.0), 0.0, 0.0, 0.0]
+        else:
+            m = [s.get("mxx", 0.0), s.get("myy", 0.0), s.get("mzz", 0.0), s.get("mxy", 0.0), s.get("mxz", 0.0),
+                 s.get("myz", 0.0)]
+        for k in range(6):
+            arr[i].m[k] = m[k]
+        arr[i].f0 = s["f0"]
+        arr[i].tshift = s.get("tshift", 0.0)
+        arr[i].factor = s.get("factor", 1.0)
+    return arr
+
+
+def _receivers(receivers):
+    arr = (_Receiver * max(len(receivers), 1))()
+    for i, r in enumerate(receivers):
+        arr[i].x, arr[i].y, arr[i].z, arr[i].angle = r["x"], r.get("y", 0.0), r["z"], r.get("angle", 0.0)
+    return arr
+
+
+def _run_desc(dt, nstep, nsb, types, chunk):
+    run = _Run()
+    run.dt, run.nstep, run.nstep_between_samples, run.chunk_size = dt, nstep, nsb, chunk
+    run.nseismo_types = len(types)
+    for i, t in enumerate(types):
+        run.seismo_types[i] = _TYPE_CODES[t] if isinstance(t, str) else int(t)
+    return run
+
+
+class _EngineBase:
+    dim = 0
+
+    def _check(self, rc):
+        if rc < 0:
+            raise SfxError(self.lib.sfx_last_error().decode())
+        return rc
+
+    def _finish_init(self, types):
+        self.types = list(types)
+        sizes = (C.c_int64 * 7)()
+        self._check(self.lib.sfx_get_sizes(self.h, sizes))
+        self.dim, self.nspec, self.nglob, self.ngll, self.nsig, self.nrec, self.nsrc = [int(x) for x in sizes]
+        self.ncomp = self.dim
+        t0 = C.c_double()
+        self._check(self.lib.sfx_get_t0(self.h, C.byref(t0)))
+        self.t0 = t0.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sfx_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- data contract ----
+    def index_mapping(self):
+        npe = 25 if self.dim == 2 else 125
+        out = np.zeros(self.nspec * npe, dtype=np.int32)
+        self._check(self.lib.sfx_get_index_mapping(self.h, _ptr(out)))
+        return out.reshape((self.nspec,) + (5,) * self.dim)
+
+    def field_index(self):
+        out = np.zeros(self.nglob, dtype=np.int32)
+        self._check(self.lib.sfx_get_field_index(self.h, _ptr(out)))
+        return out
+
+    def compute_to_mesh(self):
+        out = np.zeros(self.nspec, dtype=np.int32)
+        self._check(self.lib.sfx_get_compute_to_mesh(self.h, _ptr(out)))
+        return out
+
+    def locations(self, which="sources"):
+        n = self.nsrc if which == "sources" else self.nrec
+        ispec = np.zeros(n, dtype=np.int32)
+        loc = np.zeros((n, self.dim))
+        self._check(self.lib.sfx_get_locations(self.h, 0 if which == "sources" else 1, _ptr(ispec), _ptr(loc)))
+        return ispec, loc
+
+    # ---- time scheme seam ----
+    def initialize(self):
+        self._check(self.lib.sfx_initialize(self.h))
+
+    def apply_predictor_phase(self):
+        return self._check(self.lib.sfx_predictor(self.h))
+
+    def update_wavefields(self, istep):
+        return self._check(self.lib.sfx_update_wavefields(self.h, istep))
+
+    def apply_corrector_phase(self):
+        return self._check(self.lib.sfx_corrector(self.h))
+
+    def compute_seismograms(self, isig):
+        self._check(self.lib.sfx_compute_seismograms(self.h, isig))
+
+    # ---- fused path ----
+    def step(self, istep):
+        self._check(self.lib.sfx_step(self.h, istep))
+
+    def run(self, first=0, count=None):
+        if count is None:
+            count = self.nstep - first
+        self._check(self.lib.sfx_run(self.h, first, count))
+
+    def synchronize(self):
+        self._check(self.lib.sfx_synchronize(self.h))
+
+    def timings(self):
+        out = (C.c_double * 4)()
+        self._check(self.lib.sfx_get_timings(self.h, out))
+        return {"stiffness_ms": out[0], "update_ms": out[1], "boundary_source_seismo_ms": out[2], "step_ms": out[3]}
+
+    # ---- outputs ----
+    def seismograms(self):
+        out = np.zeros((self.nsig, len(self.types), self.nrec, self.ncomp), dtype=np.float32)
+        self._check(self.lib.sfx_get_seismograms(self.h, _ptr(out)))
+        return out
+
+    def field(self, which):
+        codes = {"displacement": 0, "velocity": 1, "acceleration": 2, "mass_inverse": 3}
+        out = np.zeros((self.ncomp, self.nglob), dtype=np.float32)
+        self._check(self.lib.sfx_get_field(self.h, codes[which], _ptr(out)))
+        return out
+
+    def set_field(self, which, values):
+        codes = {"displacement": 0, "velocity": 1, "acceleration": 2}
+        arr = np.ascontiguousarray(values, dtype=np.float32).reshape(self.ncomp, self.nglob)
+        self._check(self.lib.sfx_set_field(self.h, codes[which], _ptr(arr)))
+
+
+class Engine2D(_EngineBase):
+    """2-D elastic P-SV engine built from a :class:`Mesh2D`-like object."""
+
+    def __init__(self, mesh, sources, receivers, dt, nstep, nstep_between_samples=1,
+                 seismogram_types=("displacement",), chunk_size=1):
+        self.lib = _native.load()
+        self.nstep = nstep
+        coorg = np.ascontiguousarray(np.asarray(mesh.coorg, dtype=np.float64).T)
+        knods = np.ascontiguousarray(np.asarray(mesh.knods, dtype=np.int32).T)
+        kmato = np.ascontiguousarray(mesh.kmato, dtype=np.int32)
+        rho = np.array([m.density for m in mesh.materials], dtype=np.float64)
+        vp = np.array([m.cp for m in mesh.materials], dtype=np.float64)
+        vs = np.array([m.cs for m in mesh.materials], dtype=np.float64)
+        abs_ispec = np.ascontiguousarray(mesh.abs_ispec, dtype=np.int32)
+        abs_edge = np.ascontiguousarray(mesh.abs_type, dtype=np.int32)
+        self._keep = (coorg, knods, kmato, rho, vp, vs, abs_ispec, abs_edge)
+        m = _Mesh2D(mesh.nspec, mesh.ngnod, mesh.npgeo, _ptr(coorg), _ptr(knods), _ptr(kmato), len(rho), _ptr(rho),
+                    _ptr(vp), _ptr(vs), len(abs_ispec), _ptr(abs_ispec), _ptr(abs_edge))
+        run = _run_desc(dt, nstep, nstep_between_samples, seismogram_types, chunk_size)
+        h = C.c_void_p()
+        self.h = None
+        self._check(self.lib.sfx_engine_create_2d(C.byref(m), len(sources), _sources(sources, 2), len(receivers),
+                                                  _receivers(receivers), C.byref(run), C.byref(h)))
+        self.h = h
+        self._finish_init(seismogram_types)
+
+
+class Engine3D(_EngineBase):
+    """3-D elastic engine on a structured hexahedral mesh (see ``mesh3d``)."""
+
+    def __init__(self, mesh, sources, receivers, dt, nstep, nstep_between_samples=1,
+                 seismogram_types=("displacement",)):
+        self.lib = _native.load()
+        self.nstep = nstep
+        mat = np.ascontiguousarray(mesh["material"], dtype=np.int32).ravel()
+        rho = np.ascontiguousarray(mesh["rho"], dtype=np.float64)
+        vp = np.ascontiguousarray(mesh["vp"], dtype=np.float64)
+        vs = np.ascontiguousarray(mesh["vs"], dtype=np.float64)
+        self._keep = (mat, rho, vp, vs)
+        m = _Mesh3D()
+        m.nx, m.ny, m.nz = mesh["nx"], mesh["ny"], mesh["nz"]
+        m.x0, m.y0, m.z0 = mesh.get("x0", 0.0), mesh.get("y0", 0.0), mesh.get("z0", 0.0)
+        m.hx, m.hy, m.hz = mesh["hx"], mesh["hy"], mesh["hz"]
+        m.material, m.nmaterials = _ptr(mat), len(rho)
+        m.rho, m.vp, m.vs = _ptr(rho), _ptr(vp), _ptr(vs)
+        for k in range(6):
+            m.absorbing[k] = int(mesh["absorbing"][k])
+        run = _run_desc(dt, nstep, nstep_between_samples, seismogram_types, 1)
+        h = C.c_void_p()
+        self.h = None
+        self._check(self.lib.sfx_engine_create_3d(C.byref(m), len(sources), _sources(sources, 3), len(receivers),
+                                                  _receivers(receivers), C.byref(run), C.byref(h)))
+        self.h = h
+        self._finish_init(seismogram_types)

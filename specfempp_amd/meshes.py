@@ -1,0 +1,143 @@
+"""Structured meshes and the benchmark workloads.
+
+2-D meshes use the layout of the SPECFEM 2-D mesher's database (4-node
+elements numbered x fastest; control nodes ordered bottom-left, bottom-right,
+top-right, top-left as the shape functions of src/jacobian/shape_functions.cpp
+expect; absorbing edges as (element, edge) pairs).  3-D meshes are the
+structured hexahedral description of ``sfx_mesh3d``.
+
+Time steps: the stable step of the NGLL = 5 scheme scales with the smallest
+GLL spacing, 0.1727 h.  Workloads whose stated dt exceeds a Courant number of
+0.5 on that spacing use dt = 0.4 * 0.1727 h / vp_max instead, and record both
+(``dt_stated`` / ``dt``).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .io.fortran_db import EDGE_BOTTOM, EDGE_LEFT, EDGE_RIGHT, EDGE_TOP, Material, Mesh2D
+
+GLL_MIN_SPACING = 0.5 * (1.0 - 0.6546536707079771)  # smallest GLL gap on [-1, 1] is 0.3453 -> 0.1727 h
+
+
+def courant(dt: float, h: float, vp_max: float) -> float:
+    return dt * vp_max / (GLL_MIN_SPACING * h)
+
+
+def stable_dt(h: float, vp_max: float, c: float = 0.4) -> float:
+    return c * GLL_MIN_SPACING * h / vp_max
+
+
+def quad_mesh(nx: int, nz: int, xmin: float, xmax: float, zmin: float, zmax: float, material=None,
+              materials=None, absorbing=("bottom", "right", "left")) -> Mesh2D:
+    """Structured 4-node quad mesh; ``material`` is an (nz, nx) index array."""
+    xs = np.linspace(xmin, xmax, nx + 1)
+    zs = np.linspace(zmin, zmax, nz + 1)
+    X, Z = np.meshgrid(xs, zs)  # (nz+1, nx+1)
+    coorg = np.stack([X.ravel(), Z.ravel()])  # (2, npgeo), node = iz*(nx+1) + ix
+    iz, ix = np.meshgrid(np.arange(nz), np.arange(nx), indexing="ij")
+    n0 = (iz * (nx + 1) + ix).ravel()
+    knods = np.stack([n0, n0 + 1, n0 + 1 + (nx + 1), n0 + (nx + 1)]).astype(np.int32)  # (4, nspec)
+    if material is None:
+        material = np.zeros((nz, nx), dtype=np.int32)
+    kmato = np.ascontiguousarray(material, dtype=np.int32).ravel()
+    abs_ispec, abs_type = [], []
+    flags = set(absorbing)
+    for e in range(nx * nz):
+        ez, ex = divmod(e, nx)
+        for name, code, on in (("bottom", EDGE_BOTTOM, ez == 0), ("right", EDGE_RIGHT, ex == nx - 1),
+                               ("top", EDGE_TOP, ez == nz - 1), ("left", EDGE_LEFT, ex == 0)):
+            if on and name in flags:
+                abs_ispec.append(e)
+                abs_type.append(code)
+    return Mesh2D(nspec=nx * nz, npgeo=(nx + 1) * (nz + 1), ngnod=4, coorg=coorg, knods=knods, kmato=kmato,
+                  materials=list(materials), abs_ispec=np.array(abs_ispec, dtype=np.int32),
+                  abs_type=np.array(abs_type, dtype=np.int32))
+
+
+def elastic(rho: float, vp: float, vs: float) -> Material:
+    return Material(indic=1, density=rho, cp=vp, cs=vs, qkappa=9999.0, qmu=9999.0,
+                    raw=(rho, vp, vs, 0, 0, 9999.0, 9999.0, 0, 0, 0, 0, 0, 0))
+
+
+@dataclass
+class Workload2D:
+    name: str
+    mesh: Mesh2D
+    sources: list
+    receivers: list
+    dt: float
+    nstep: int
+    dt_stated: float = 0.0
+    notes: dict = field(default_factory=dict)
+
+
+def config1(nstep: int = 1600, n: int = 80, h: float = 50.0) -> Workload2D:
+    """2-D homogeneous P-SV: 80x80 quads of 50 m, free top, Stacey elsewhere,
+    vertical Ricker force (10 Hz) at (2000, 2000), 50 receivers 500 m deep."""
+    L = n * h
+    mesh = quad_mesh(n, n, 0.0, L, 0.0, L, materials=[elastic(2200.0, 2500.0, 1443.0)],
+                     absorbing=("bottom", "right", "left"))
+    src = [dict(x=L / 2, z=L / 2, angle=0.0, f0=10.0, factor=1e10, tshift=0.0)]
+    rec = [dict(x=40.0 + 80.0 * i * (L / 4000.0), z=L - 500.0 * (L / 4000.0)) for i in range(50)]
+    return Workload2D("2d_homogeneous_80x80", mesh, src, rec, 1e-3, nstep, 1e-3,
+                      {"courant": courant(1e-3, h, 2500.0)})
+
+
+@dataclass
+class Workload3D:
+    name: str
+    mesh: dict
+    sources: list
+    receivers: list
+    dt: float
+    nstep: int
+    dt_stated: float = 0.0
+    notes: dict = field(default_factory=dict)
+
+
+def hex_mesh(nx, ny, nz, h, material, rho, vp, vs, absorbing=(1, 1, 1, 1, 1, 0)):
+    return dict(nx=nx, ny=ny, nz=nz, x0=0.0, y0=0.0, z0=0.0, hx=h, hy=h, hz=h,
+                material=np.ascontiguousarray(material, dtype=np.int32), rho=np.asarray(rho, dtype=np.float64),
+                vp=np.asarray(vp, dtype=np.float64), vs=np.asarray(vs, dtype=np.float64), absorbing=list(absorbing))
+
+
+def moment_tensor(seed: int = 3):
+    """Seeded random symmetric moment tensor of unit Frobenius norm."""
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((3, 3))
+    M = 0.5 * (A + A.T)
+    return M / np.linalg.norm(M)
+
+
+def config3(n: int = 64, nstep: int = 1000, h: float = 100.0, nrec_side: int = 20) -> Workload3D:
+    """3-D layered elastic cube: n^3 hexahedra of h, 4 horizontal layers of
+    n h / 4 from the top (vp 2000/3000/4500/6000 m/s, vs = vp / 1.73, rho
+    2000/2300/2600/2900), free top, Stacey on the other five faces, seeded
+    unit-norm moment tensor (Ricker 5 Hz) at 3 km depth under the centre,
+    nrec_side^2 surface receivers.  The full configuration is n = 64."""
+    L = n * h
+    vp = np.array([2000.0, 3000.0, 4500.0, 6000.0])
+    vs = vp / 1.73
+    rho = np.array([2000.0, 2300.0, 2600.0, 2900.0])
+    zc = (np.arange(n) + 0.5) * h  # element centre heights
+    layer = np.minimum(3, ((L - zc) / (L / 4)).astype(np.int32))
+    material = np.broadcast_to(layer[:, None, None], (n, n, n)).copy()
+    mesh = hex_mesh(n, n, n, h, material, rho, vp, vs)
+    M = moment_tensor(3)
+    depth = min(3000.0, 0.47 * L)
+    src = [dict(type="moment_tensor", x=L / 2, y=L / 2, z=L - depth, mxx=M[0, 0], myy=M[1, 1], mzz=M[2, 2],
+                mxy=M[0, 1], mxz=M[0, 2], myz=M[1, 2], f0=5.0, factor=1e15, tshift=0.0)]
+    spacing = L / nrec_side
+    rec = [dict(x=spacing / 2 + spacing * i, y=spacing / 2 + spacing * j, z=L)
+           for j in range(nrec_side) for i in range(nrec_side)]
+    dt_stated = 2e-3
+    dt = dt_stated if courant(dt_stated, h, vp.max()) <= 0.5 else 1e-3
+    return Workload3D(f"3d_layered_{n}^3", mesh, src, rec, dt, nstep, dt_stated,
+                      {"courant": courant(dt, h, vp.max()), "courant_stated": courant(dt_stated, h, vp.max())})
+
+
+def nglob_3d(n: int) -> int:
+    return (4 * n + 1) ** 3

@@ -48,6 +48,24 @@ def cpu_baseline(w, nsteps: int) -> dict:
                       f"{w.mesh['nx']}^3 workload, {el:.1f} s timed (+{build_s:.1f} s setup untimed)"}
 
 
+def copy_calibration(nbytes: int = 1 << 30, reps: int = 5) -> dict:
+    """A plain device copy of known size: its FETCH_SIZE / WRITE_SIZE counters
+    give the scale factor of the byte counters, its time the copy bandwidth."""
+    import torch
+
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t) / reps
+    return {"bytes_read_per_copy": nbytes, "bytes_written_per_copy": nbytes, "copies": reps + 1,
+            "copy_GBs": 2 * nbytes / el / 1e9}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,6 +74,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 1 GiB device-to-device copy (scale reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine is single-GPU).",
@@ -120,6 +140,8 @@ def main() -> int:
         "algorithmic_bytes_per_pt_step": ab["per_pt_step"],
         "seismograms_finite": finite,
     }
+    if args.calibrate_copy:
+        out["copy_calibration"] = copy_calibration()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps)
     print(json.dumps(out))

@@ -101,7 +101,21 @@ typedef struct {
   int nseismo_types;
   int seismo_types[3];
   int chunk_size;   /* storage chunk size of the numbering (chunk_config.hpp:12-28); 1 = CPU no-SIMD */
+  int scatter;      /* weak-form scatter: SFX_SCATTER_COLORED, _ATOMIC or _GATHER (3-D) */
 } sfx_run_desc;
+
+/* Scatter of element forces into the global acceleration.  COLORED groups the
+ * elements into colors whose members share no global point and adds with plain
+ * read-modify-write, one launch per color; ATOMIC adds every element point with
+ * a float atomic in one launch (the reference's atomic_add_on_device,
+ * include/compute/fields/data_access.tpp:683-720). */
+#define SFX_SCATTER_COLORED 0
+#define SFX_SCATTER_ATOMIC 1
+/* GATHER (3-D structured meshes): the stiffness kernel stores each element's
+ * forces with coalesced writes; the fused update then sums, for every global
+ * point, the contributions of its elements in a fixed order (deterministic,
+ * no atomics, the acceleration is never stored between the two). */
+#define SFX_SCATTER_GATHER 2
 
 /* ---- library ---------------------------------------------------------- */
 int sfx_version(void);

@@ -261,6 +261,8 @@ void check_run(const sfx_run_desc &run) {
   for (int i = 0; i < run.nseismo_types; i++)
     if (run.seismo_types[i] < 0 || run.seismo_types[i] > 2) throw std::invalid_argument("bad seismogram type");
   if (run.chunk_size < 1) throw std::invalid_argument("chunk_size must be >= 1");
+  if (run.scatter != SFX_SCATTER_COLORED && run.scatter != SFX_SCATTER_ATOMIC && run.scatter != SFX_SCATTER_GATHER)
+    throw std::invalid_argument("scatter must be SFX_SCATTER_COLORED, SFX_SCATTER_ATOMIC or SFX_SCATTER_GATHER");
 }
 
 }  // namespace
@@ -271,6 +273,7 @@ void check_run(const sfx_run_desc &run) {
 HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *src, int nrec,
                                const sfx_receiver *rec, const sfx_run_desc &run) {
   check_run(run);
+  if (run.scatter == SFX_SCATTER_GATHER) throw std::invalid_argument("SFX_SCATTER_GATHER is for 3-D meshes");
   if (m.nspec <= 0 || m.npgeo <= 0) throw std::invalid_argument("empty mesh");
   if (m.ngnod != 4 && m.ngnod != 9) throw std::invalid_argument("ngnod must be 4 or 9");
   if (!m.coorg || !m.knods || !m.material || !m.rho || !m.vp || !m.vs) throw std::invalid_argument("null mesh array");
@@ -459,8 +462,10 @@ HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *
           mass[gi] += mm[0];
           mass[a.nglob + gi] += mm[1];
         }
-    a.mass_inverse.assign(2 * static_cast<size_t>(a.nglob), 0.0f);
-    for (size_t i = 0; i < mass.size(); i++) a.mass_inverse[i] = 1.0f / mass[i];
+    a.mass_inverse.assign(2 * static_cast<size_t>(a.nglob_pad), 0.0f);
+    for (int c = 0; c < 2; c++)
+      for (int g = 0; g < a.nglob; g++)
+        a.mass_inverse[static_cast<size_t>(c) * a.nglob_pad + g] = 1.0f / mass[static_cast<size_t>(c) * a.nglob + g];
   }
 
   // Sources and receivers (locate_point; force_source.cpp:70-78;
@@ -615,10 +620,36 @@ HostAssembly build_assembly_3d(const sfx_mesh3d &m, int nsrc, const sfx_source *
             }
       }
   a.nglob = inum;
-  a.nglob_pad = (a.nglob + 63) / 64 * 64;
+  if (static_cast<int64_t>(inum) != nlat) throw std::runtime_error("3-D numbering: lattice not fully covered");
   a.field_index.resize(a.nglob);
   for (int i = 0; i < a.nglob; i++) a.field_index[i] = i;
-  a.gather = a.index_mapping;
+  // Device fields are stored in 4x4x4 tiles of lattice points: tile (tx, ty, tz)
+  // holds the points 4 tx <= I < 4 tx + 4 etc., x fastest inside the tile and
+  // tiles x fastest.  One extra tile per axis holds the last lattice plane.  An
+  // element's 64 points with local indices < 4 are one contiguous tile.
+  // gather holds the device index of every local point; lattice_of maps the
+  // reference numbering to it.
+  const int64_t TX = m.nx + 1, TY = m.ny + 1, TZ = m.nz + 1;
+  if (TX * TY * TZ * 64 > (int64_t(1) << 31) - 1) throw std::invalid_argument("3-D mesh too large");
+  a.nglob_pad = static_cast<int>(TX * TY * TZ * 64);
+  auto tiled = [&](int64_t I, int64_t J, int64_t K) {
+    return static_cast<int32_t>((((K >> 2) * TY + (J >> 2)) * TX + (I >> 2)) * 64 + ((K & 3) * 4 + (J & 3)) * 4 + (I & 3));
+  };
+  a.lattice_of.assign(a.nglob, -1);
+  a.gather.resize(nloc);
+  for (int ez = 0; ez < m.nz; ez++)
+    for (int ey = 0; ey < m.ny; ey++)
+      for (int ex = 0; ex < m.nx; ex++) {
+        const size_t e = (static_cast<size_t>(ez) * m.ny + ey) * m.nx + ex;
+        for (int k = 0; k < 5; k++)
+          for (int j = 0; j < 5; j++)
+            for (int i = 0; i < 5; i++) {
+              const int32_t d = tiled(4 * ex + i, 4 * ey + j, 4 * ez + k);
+              const size_t p = e * npe + (k * 5 + j) * 5 + i;
+              a.gather[p] = d;
+              a.lattice_of[a.index_mapping[p]] = d;
+            }
+      }
 
   // Regular geometry: d(xi)/dx = 2 / hx etc., jacobian hx hy hz / 8
   // (the regular-element compression of the 3-D database,
@@ -666,6 +697,7 @@ HostAssembly build_assembly_3d(const sfx_mesh3d &m, int nsrc, const sfx_source *
         }
         const size_t p = static_cast<size_t>(e) * npe + (k * 5 + j) * 5 + i;
         a.bnd_gather.push_back(a.gather[p]);
+        a.bnd_slot.push_back(static_cast<int32_t>(p));
         a.bnd_normal.push_back(n[0]);
         a.bnd_normal.push_back(n[1]);
         a.bnd_normal.push_back(n[2]);
@@ -688,10 +720,41 @@ HostAssembly build_assembly_3d(const sfx_mesh3d &m, int nsrc, const sfx_source *
         if (m.absorbing[5] && ez == m.nz - 1) add_face(e, 5);
       }
 
+  // Group the entries of one element point (a point on two or three
+  // absorbing faces of its element) so one thread adds them in a fixed order.
+  {
+    const size_t nb = a.bnd_slot.size();
+    std::vector<size_t> order(nb);
+    for (size_t q = 0; q < nb; q++) order[q] = q;
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return a.bnd_slot[x] < a.bnd_slot[y]; });
+    auto reorder_i = [&](std::vector<int32_t> &v, int w) {
+      std::vector<int32_t> o(v.size());
+      for (size_t q = 0; q < nb; q++)
+        for (int d = 0; d < w; d++) o[q * w + d] = v[order[q] * w + d];
+      v.swap(o);
+    };
+    auto reorder_f = [&](std::vector<float> &v, int w) {
+      std::vector<float> o(v.size());
+      for (size_t q = 0; q < nb; q++)
+        for (int d = 0; d < w; d++) o[q * w + d] = v[order[q] * w + d];
+      v.swap(o);
+    };
+    reorder_i(a.bnd_gather, 1);
+    reorder_i(a.bnd_slot, 1);
+    reorder_f(a.bnd_normal, 3);
+    reorder_f(a.bnd_weight, 1);
+    reorder_f(a.bnd_rho_vp, 1);
+    reorder_f(a.bnd_rho_vs, 1);
+    a.bnd_seg.clear();
+    for (size_t q = 0; q < nb; q++)
+      if (q == 0 || a.bnd_slot[q] != a.bnd_slot[q - 1]) a.bnd_seg.push_back(static_cast<int32_t>(q));
+    a.bnd_seg.push_back(static_cast<int32_t>(nb));
+  }
+
   // Mass matrix with the implicit Stacey term (the 3-D form of
   // compute_mass_matrix.tpp:80-114 and stacey.tpp:641-677)
   {
-    std::vector<float> mass(3 * static_cast<size_t>(a.nglob), 0.0f);
+    std::vector<float> mass(3 * static_cast<size_t>(a.nglob_pad), 0.0f);
     for (int e = 0; e < nspec; e++)
       for (int k = 0; k < 5; k++)
         for (int j = 0; j < 5; j++)
@@ -700,16 +763,18 @@ HostAssembly build_assembly_3d(const sfx_mesh3d &m, int nsrc, const sfx_source *
             const float f = a.w[i] * a.w[j] * a.w[k] * a.jac3;
             const float mm = a.rho[p] * f;
             const int gi = a.gather[p];
-            for (int c = 0; c < 3; c++) mass[static_cast<size_t>(c) * a.nglob + gi] += mm;
+            for (int c = 0; c < 3; c++) mass[static_cast<size_t>(c) * a.nglob_pad + gi] += mm;
           }
     const float vfac = -1.0f * static_cast<float>(run.dt) * 0.5f;
     for (size_t q = 0; q < a.bnd_gather.size(); q++) {
       float t[3] = {0, 0, 0};
       stacey_term<3>(&a.bnd_normal[3 * q], a.bnd_weight[q], a.bnd_rho_vp[q], a.bnd_rho_vs[q], vfac, t);
-      for (int c = 0; c < 3; c++) mass[static_cast<size_t>(c) * a.nglob + a.bnd_gather[q]] += t[c];
+      for (int c = 0; c < 3; c++) mass[static_cast<size_t>(c) * a.nglob_pad + a.bnd_gather[q]] += t[c];
     }
-    a.mass_inverse.resize(mass.size());
-    for (size_t i = 0; i < mass.size(); i++) a.mass_inverse[i] = 1.0f / mass[i];
+    // padding slots (no lattice point) keep a zero inverse mass
+    a.mass_inverse.assign(mass.size(), 0.0f);
+    for (size_t i = 0; i < mass.size(); i++)
+      if (mass[i] != 0.0f) a.mass_inverse[i] = 1.0f / mass[i];
   }
 
   // Sources (3-D force and moment tensor on the structured inverse map)
@@ -783,6 +848,63 @@ HostAssembly build_assembly_3d(const sfx_mesh3d &m, int nsrc, const sfx_source *
         for (int i = 0; i < 5; i++) a.rec_h[static_cast<size_t>(r) * npe + (k * 5 + j) * 5 + i] = hx[i] * hy[j] * hz[k];
   }
   return a;
+}
+
+// ---------------------------------------------------------------------------
+// Element coloring
+// ---------------------------------------------------------------------------
+void color_elements(HostAssembly &a, int scatter) {
+  const int nspec = a.nspec, npe = a.npe;
+  std::vector<int> color(nspec, 0);
+  int ncolors = 1;
+  if (scatter == SFX_SCATTER_COLORED) {  // ATOMIC and GATHER keep compute order
+    if (a.dim == 3) {
+      // structured hexahedra: the parity of (ex, ey, ez) is a valid 8-coloring
+      // (it is also what first-fit greedy produces in element order)
+      for (int e = 0; e < nspec; e++) {
+        const int ex = e % a.nx, ey = (e / a.nx) % a.ny, ez = e / (a.nx * a.ny);
+        color[e] = (ex & 1) | ((ey & 1) << 1) | ((ez & 1) << 2);
+      }
+      ncolors = 8;
+    } else {
+      // first-fit greedy in compute order over the point-sharing graph
+      std::vector<int> start(a.nglob + 1, 0);
+      for (size_t p = 0; p < a.index_mapping.size(); p++) start[a.index_mapping[p] + 1]++;
+      for (int g = 0; g < a.nglob; g++) start[g + 1] += start[g];
+      std::vector<int> fill(start.begin(), start.end() - 1), elems(a.index_mapping.size());
+      for (int e = 0; e < nspec; e++)
+        for (int p = 0; p < npe; p++) elems[fill[a.index_mapping[static_cast<size_t>(e) * npe + p]]++] = e;
+      std::fill(color.begin(), color.end(), -1);
+      ncolors = 0;
+      std::vector<int> stamp(64, -1);
+      for (int e = 0; e < nspec; e++) {
+        for (int p = 0; p < npe; p++) {
+          const int g = a.index_mapping[static_cast<size_t>(e) * npe + p];
+          for (int k = start[g]; k < start[g + 1]; k++) {
+            const int c = color[elems[k]];
+            if (c >= 0) stamp[c] = e;
+          }
+        }
+        int c = 0;
+        while (c < 64 && stamp[c] == e) c++;
+        if (c == 64) throw std::runtime_error("coloring needs more than 64 colors");
+        color[e] = c;
+        ncolors = std::max(ncolors, c + 1);
+      }
+    }
+  }
+  // stable counting sort by color
+  a.color_offsets.assign(ncolors + 1, 0);
+  for (int e = 0; e < nspec; e++) a.color_offsets[color[e] + 1]++;
+  for (int c = 0; c < ncolors; c++) a.color_offsets[c + 1] += a.color_offsets[c];
+  std::vector<int> pos(a.color_offsets.begin(), a.color_offsets.end() - 1);
+  a.perm.resize(nspec);
+  a.inv_perm.resize(nspec);
+  for (int e = 0; e < nspec; e++) {
+    const int s = pos[color[e]]++;
+    a.perm[s] = e;
+    a.inv_perm[e] = s;
+  }
 }
 
 }  // namespace sfx

@@ -13,7 +13,8 @@
 //   - Stacey boundaries are a flat list of (element point, normal, weight,
 //     impedances) entries instead of per-element tags;
 //   - fields are component-major (ncomp, nglob_pad) like the reference's
-//     LayoutLeft (nglob, ncomp) views, padded to a multiple of 64 points.
+//     LayoutLeft (nglob, ncomp) views, padded to a multiple of 64 points;
+//     3-D fields are stored in 4x4x4 tiles of lattice points.
 #pragma once
 
 #include <cstdint>
@@ -43,10 +44,13 @@ struct HostAssembly {
   std::vector<float> kappa, mu, rho;
   // Stacey entries: one per (element, local point) on an absorbing edge/face
   std::vector<int32_t> bnd_gather;
+  std::vector<int32_t> bnd_slot;          // 3-D: element * npe + local point of the entry
+  std::vector<int32_t> bnd_seg;           // 3-D: entries [bnd_seg[s], bnd_seg[s+1]) share one slot
   std::vector<float> bnd_normal;         // (nbnd, dim) outward normal scaled as the reference stores it
   std::vector<float> bnd_weight;         // (nbnd) edge/face quadrature weight
   std::vector<float> bnd_rho_vp, bnd_rho_vs;
-  // mass inverse (ncomp, nglob) on the field index
+  // mass inverse (ncomp, nglob_pad) in device layout (2-D: field index; 3-D:
+  // tiled lattice index); padding slots hold 0
   std::vector<float> mass_inverse;
   // sources
   int nsrc = 0;
@@ -61,9 +65,20 @@ struct HostAssembly {
   std::vector<int32_t> rec_ispec;
   std::vector<float> rec_h;              // (nrec, npe) interpolation weights
   std::vector<double> rec_local;         // (nrec, dim)
-  // 3-D structured sizes
+  // 3-D structured sizes; device fields in lattice order, lattice_of maps the
+  // reference numbering (iglob) to the lattice index
   int nx = 0, ny = 0, nz = 0;
+  std::vector<int32_t> lattice_of;
+  // Element colors for the scatter: device element slot s holds compute
+  // element perm[s]; slots [color_offsets[c], color_offsets[c+1]) form color c,
+  // whose elements share no global point.  One color when scatter is atomic.
+  std::vector<int32_t> perm, inv_perm, color_offsets;
 };
+
+// Group elements into colors (no two elements of a color share a global point)
+// and fill perm / inv_perm / color_offsets.  scatter = SFX_SCATTER_ATOMIC keeps
+// compute order in a single color.
+void color_elements(HostAssembly &a, int scatter);
 
 // Throw std::runtime_error / std::invalid_argument on bad input.
 HostAssembly build_assembly_2d(const sfx_mesh2d &mesh, int nsrc, const sfx_source *src, int nrec,

@@ -69,12 +69,15 @@ struct sfx_engine {
   NewmarkConst nc{};
   int nsig = 0;
   // device arrays
-  DevBuf<int> gather, bnd_gather, src_order, src_ispec, rec_ispec, types;
+  DevBuf<int> gather, bnd_gather, bnd_slot, bnd_seg, src_order, src_ispec, rec_ispec, types;
   DevBuf<float> xix, xiz, gammax, gammaz, jac, kappa, mu;
   DevBuf<float> bnd_normal, bnd_weight, bnd_rvp, bnd_rvs;
   DevBuf<float> src_array, stf, rec_h, seismo;
   DevBuf<float> fields;  // u, v, a, minv: 4 * ncomp * nglob_pad
+  DevBuf<float> re;      // 3-D gather path: element forces (nspec, 3, 125)
   bool initialized = false;
+  bool a_stale = false;  // fused gather steps leave the stored acceleration behind
+  Lattice lat{};
   std::vector<hipEvent_t> ev;
   double timings[4] = {0, 0, 0, 0};
 
@@ -88,29 +91,50 @@ struct sfx_engine {
     if (stream) (void)hipStreamDestroy(stream);
   }
 
+  // Per-element array in device slot order (colors contiguous).
+  template <class T> std::vector<T> permuted(const std::vector<T> &v) const {
+    const size_t npe = h.npe;
+    std::vector<T> out(v.size());
+    for (int slot = 0; slot < h.nspec; slot++)
+      std::memcpy(&out[slot * npe], &v[static_cast<size_t>(h.perm[slot]) * npe], npe * sizeof(T));
+    return out;
+  }
+  std::vector<int32_t> slots_of(const std::vector<int32_t> &ispec) const {
+    std::vector<int32_t> out(ispec.size());
+    for (size_t i = 0; i < ispec.size(); i++) out[i] = h.inv_perm[ispec[i]];
+    return out;
+  }
+
+  bool gather_path() const { return h.dim == 3 && run.scatter == SFX_SCATTER_GATHER; }
+
   void upload() {
+    color_elements(h, run.scatter);
+    lat = Lattice{h.nx, h.ny, h.nz, h.nx + 1, h.ny + 1, h.nz + 1};
     SFX_HIP(hipStreamCreate(&stream));
     hipStream_t s = stream;
-    gather.upload(h.gather, s);
-    kappa.upload(h.kappa, s);
-    mu.upload(h.mu, s);
+    gather.upload(permuted(h.gather), s);
+    kappa.upload(permuted(h.kappa), s);
+    mu.upload(permuted(h.mu), s);
     if (h.dim == 2) {
-      xix.upload(h.xix, s);
-      xiz.upload(h.xiz, s);
-      gammax.upload(h.gammax, s);
-      gammaz.upload(h.gammaz, s);
-      jac.upload(h.jac, s);
+      xix.upload(permuted(h.xix), s);
+      xiz.upload(permuted(h.xiz), s);
+      gammax.upload(permuted(h.gammax), s);
+      gammaz.upload(permuted(h.gammaz), s);
+      jac.upload(permuted(h.jac), s);
     }
     bnd_gather.upload(h.bnd_gather, s);
+    bnd_slot.upload(h.bnd_slot, s);
+    bnd_seg.upload(h.bnd_seg, s);
+    if (gather_path()) re.alloc(static_cast<size_t>(h.nspec) * 375);
     bnd_normal.upload(h.bnd_normal, s);
     bnd_weight.upload(h.bnd_weight, s);
     bnd_rvp.upload(h.bnd_rho_vp, s);
     bnd_rvs.upload(h.bnd_rho_vs, s);
     src_order.upload(h.src_order, s);
-    src_ispec.upload(h.src_ispec, s);
+    src_ispec.upload(slots_of(h.src_ispec), s);
     src_array.upload(h.src_array, s);
     stf.upload(h.stf, s);
-    rec_ispec.upload(h.rec_ispec, s);
+    rec_ispec.upload(slots_of(h.rec_ispec), s);
     rec_h.upload(h.rec_h, s);
     std::vector<int> t(run.seismo_types, run.seismo_types + run.nseismo_types);
     types.upload(t, s);
@@ -121,12 +145,10 @@ struct sfx_engine {
     const size_t stride = static_cast<size_t>(h.ncomp) * h.nglob_pad;
     fields.alloc(4 * stride);
     launch_fill(s, fields.p, 4 * stride, 0.0f);
-    // mass inverse into the padded layout (padding stays 0)
-    std::vector<float> minv(stride, 0.0f);
-    for (int c = 0; c < h.ncomp; c++)
-      std::memcpy(&minv[static_cast<size_t>(c) * h.nglob_pad], &h.mass_inverse[static_cast<size_t>(c) * h.nglob],
-                  h.nglob * sizeof(float));
-    SFX_HIP(hipMemcpyAsync(fields.p + 3 * stride, minv.data(), stride * sizeof(float), hipMemcpyHostToDevice, s));
+    // mass inverse, already in device layout (padding slots hold 0)
+    if (h.mass_inverse.size() != stride) throw std::logic_error("internal: mass inverse layout");
+    SFX_HIP(hipMemcpyAsync(fields.p + 3 * stride, h.mass_inverse.data(), stride * sizeof(float),
+                           hipMemcpyHostToDevice, s));
     SFX_HIP(hipStreamSynchronize(s));
     // quadrature constants
     for (int i = 0; i < 5; i++) {
@@ -142,11 +164,36 @@ struct sfx_engine {
     nc.dt2o2 = dt * dt / 2.0;
   }
 
+  // Gather path: element forces into re, then sources and Stacey into re.
+  void element_forces(Fields ff, int istep) {
+    if (istep < 0 || istep >= run.nstep) throw std::invalid_argument("istep out of range");
+    launch_elemforce_3d(stream, lat, kappa.p, mu.p, h.xix3, h.ety3, h.gaz3, h.jac3, q, ff.u, h.nglob_pad, re.p);
+  }
+  void element_boundary_sources(Fields ff, int istep) {
+    launch_sources_re(stream, h.nsrc, istep, src_ispec.p, src_array.p, stf.p, re.p);
+    launch_stacey_re(stream, static_cast<int>(h.bnd_seg.size()) - 1, bnd_seg.p, bnd_gather.p, bnd_slot.p,
+                     bnd_normal.p, bnd_weight.p, bnd_rvp.p, bnd_rvs.p, ff, re.p);
+  }
+  // Bring the stored acceleration up to date after fused gather steps.
+  void refresh_acceleration() {
+    if (gather_path() && a_stale) {
+      launch_assemble_3d(stream, lat, re.p, f(), 1);
+      a_stale = false;
+    }
+  }
+
+  // One launch per color (plain read-modify-write), or one atomic launch.
   void stiffness(Fields ff) {
-    if (h.dim == 2)
-      launch_stiffness_2d(stream, h.nspec, gather.p, xix.p, xiz.p, gammax.p, gammaz.p, jac.p, kappa.p, mu.p, q, ff);
-    else
-      launch_stiffness_3d(stream, h.nspec, gather.p, kappa.p, mu.p, h.xix3, h.ety3, h.gaz3, h.jac3, q, ff);
+    const bool atomic = run.scatter == SFX_SCATTER_ATOMIC;
+    for (size_t c = 0; c + 1 < h.color_offsets.size(); c++) {
+      const int first = h.color_offsets[c], count = h.color_offsets[c + 1] - first;
+      if (h.dim == 2)
+        launch_stiffness_2d(stream, first, count, atomic, gather.p, xix.p, xiz.p, gammax.p, gammaz.p, jac.p, kappa.p,
+                            mu.p, q, ff);
+      else
+        launch_stiffness_3d(stream, first, count, atomic, gather.p, kappa.p, mu.p, h.xix3, h.ety3, h.gaz3, h.jac3, q,
+                            ff);
+    }
   }
   void boundary(Fields ff) {
     launch_stacey(stream, h.dim, static_cast<int>(h.bnd_gather.size()), bnd_gather.p, bnd_normal.p, bnd_weight.p,
@@ -168,6 +215,24 @@ struct sfx_engine {
   // One fused Newmark step; `evs` (5 events) brackets the kernel families.
   void step(int istep, hipEvent_t *evs) {
     Fields ff = f();
+    if (gather_path()) {
+      if (evs) SFX_HIP(hipEventRecord(evs[0], stream));
+      if (evs) SFX_HIP(hipEventRecord(evs[1], stream));
+      element_forces(ff, istep);
+      if (evs) SFX_HIP(hipEventRecord(evs[2], stream));
+      element_boundary_sources(ff, istep);
+      if (istep % run.nstep_between_samples == 0) {
+        const int isig = istep / run.nstep_between_samples;
+        if (isig >= nsig) throw std::invalid_argument("seismogram step out of range");
+        launch_seismograms_re(stream, lat, h.nrec, run.nseismo_types, types.p, isig, rec_ispec.p, rec_h.p, re.p, ff,
+                              nc.dto2, seismo.p);
+      }
+      if (evs) SFX_HIP(hipEventRecord(evs[3], stream));
+      launch_assemble_update_3d(stream, lat, re.p, ff, nc);
+      if (evs) SFX_HIP(hipEventRecord(evs[4], stream));
+      a_stale = true;
+      return;
+    }
     if (evs) SFX_HIP(hipEventRecord(evs[0], stream));
     sources(ff, istep);
     if (evs) SFX_HIP(hipEventRecord(evs[1], stream));
@@ -319,6 +384,7 @@ int sfx_initialize(sfx_engine *e) {
     const size_t stride = static_cast<size_t>(e->h.ncomp) * e->h.nglob_pad;
     launch_fill(e->stream, e->fields.p, 3 * stride, 0.0f);
     if (e->seismo.n) launch_fill(e->stream, e->seismo.p, e->seismo.n, 0.0f);
+    e->a_stale = false;
     SFX_HIP(hipGetLastError());
     e->initialized = true;
   });
@@ -329,6 +395,7 @@ int sfx_predictor(sfx_engine *e) {
   const int rc = guarded([&] {
     check_engine(e);
     e->require_init();
+    e->refresh_acceleration();
     launch_predictor(e->stream, e->h.ncomp, e->f(), e->nc);
     SFX_HIP(hipGetLastError());
     dofs = e->h.nglob * e->h.ncomp;
@@ -342,10 +409,17 @@ int sfx_update_wavefields(sfx_engine *e, int istep) {
     check_engine(e);
     e->require_init();
     Fields ff = e->f();
-    e->sources(ff, istep);
-    e->stiffness(ff);
-    e->boundary(ff);
-    launch_divide_corrector(e->stream, e->h.ncomp, ff, e->nc, 1, 0);
+    if (e->gather_path()) {
+      e->element_forces(ff, istep);
+      e->element_boundary_sources(ff, istep);
+      launch_assemble_3d(e->stream, e->lat, e->re.p, ff, 1);
+      e->a_stale = false;
+    } else {
+      e->sources(ff, istep);
+      e->stiffness(ff);
+      e->boundary(ff);
+      launch_divide_corrector(e->stream, e->h.ncomp, ff, e->nc, 1, 0);
+    }
     SFX_HIP(hipGetLastError());
     nelem = e->h.nspec;
   });
@@ -357,6 +431,7 @@ int sfx_corrector(sfx_engine *e) {
   const int rc = guarded([&] {
     check_engine(e);
     e->require_init();
+    e->refresh_acceleration();
     launch_divide_corrector(e->stream, e->h.ncomp, e->f(), e->nc, 0, 1);
     SFX_HIP(hipGetLastError());
     dofs = e->h.nglob * e->h.ncomp;
@@ -368,6 +443,7 @@ int sfx_compute_seismograms(sfx_engine *e, int isig) {
   return guarded([&] {
     check_engine(e);
     e->require_init();
+    e->refresh_acceleration();
     e->seismograms(e->f(), isig, 0);
     SFX_HIP(hipGetLastError());
   });
@@ -432,17 +508,27 @@ int sfx_get_seismograms(const sfx_engine *e, float *out) {
   });
 }
 
-int sfx_get_field(const sfx_engine *e, int which, float *out) {
+int sfx_get_field(const sfx_engine *ce, int which, float *out) {
   return guarded([&] {
-    check_engine(e);
+    check_engine(ce);
     if (which < 0 || which > 3) throw std::invalid_argument("which must be 0..3");
     if (!out) throw std::invalid_argument("null output");
+    // the stored acceleration is refreshed lazily after fused gather steps
+    sfx_engine *e = const_cast<sfx_engine *>(ce);
+    if (which == 2) e->refresh_acceleration();
     SFX_HIP(hipStreamSynchronize(e->stream));
     const size_t stride = static_cast<size_t>(e->h.ncomp) * e->h.nglob_pad;
-    for (int c = 0; c < e->h.ncomp; c++)
-      SFX_HIP(hipMemcpy(out + static_cast<size_t>(c) * e->h.nglob,
-                        e->fields.p + which * stride + static_cast<size_t>(c) * e->h.nglob_pad,
-                        e->h.nglob * sizeof(float), hipMemcpyDeviceToHost));
+    const size_t ng = e->h.nglob;
+    std::vector<float> buf(e->h.nglob_pad);
+    for (int c = 0; c < e->h.ncomp; c++) {
+      SFX_HIP(hipMemcpy(buf.data(), e->fields.p + which * stride + static_cast<size_t>(c) * e->h.nglob_pad,
+                        e->h.nglob_pad * sizeof(float), hipMemcpyDeviceToHost));
+      float *o = out + static_cast<size_t>(c) * ng;
+      if (e->h.lattice_of.empty())
+        std::memcpy(o, buf.data(), ng * sizeof(float));
+      else
+        for (size_t g = 0; g < ng; g++) o[g] = buf[e->h.lattice_of[g]];
+    }
   });
 }
 
@@ -453,10 +539,18 @@ int sfx_set_field(sfx_engine *e, int which, const float *in) {
     if (!in) throw std::invalid_argument("null input");
     SFX_HIP(hipStreamSynchronize(e->stream));
     const size_t stride = static_cast<size_t>(e->h.ncomp) * e->h.nglob_pad;
-    for (int c = 0; c < e->h.ncomp; c++)
-      SFX_HIP(hipMemcpy(e->fields.p + which * stride + static_cast<size_t>(c) * e->h.nglob_pad,
-                        in + static_cast<size_t>(c) * e->h.nglob, e->h.nglob * sizeof(float),
-                        hipMemcpyHostToDevice));
+    const size_t ng = e->h.nglob;
+    std::vector<float> buf(e->h.nglob_pad, 0.0f);
+    for (int c = 0; c < e->h.ncomp; c++) {
+      const float *src = in + static_cast<size_t>(c) * ng;
+      if (e->h.lattice_of.empty())
+        std::memcpy(buf.data(), src, ng * sizeof(float));
+      else
+        for (size_t g = 0; g < ng; g++) buf[e->h.lattice_of[g]] = src[g];
+      SFX_HIP(hipMemcpy(e->fields.p + which * stride + static_cast<size_t>(c) * e->h.nglob_pad, buf.data(),
+                        e->h.nglob_pad * sizeof(float), hipMemcpyHostToDevice));
+    }
+    if (which == 2) e->a_stale = false;
   });
 }
 

@@ -18,6 +18,15 @@ namespace {
 
 __device__ __forceinline__ void atomic_add(float *p, float v) { unsafeAtomicAdd(p, v); }
 
+// Element-force scatter: a float atomic, or a plain read-modify-write when the
+// launch covers one color (no other thread of the launch touches the point).
+template <bool ATOMIC> __device__ __forceinline__ void scatter_add(float *p, float v) {
+  if constexpr (ATOMIC)
+    unsafeAtomicAdd(p, v);
+  else
+    *p += v;
+}
+
 // ---------------------------------------------------------------------------
 // 2-D stiffness: 5 elements per 128-thread block, one thread per GLL point.
 // The element's displacement and its stress integrand F are staged in LDS
@@ -25,8 +34,9 @@ __device__ __forceinline__ void atomic_add(float *p, float v) { unsafeAtomicAdd(
 // ---------------------------------------------------------------------------
 constexpr int E2 = 5;  // elements per block
 
+template <bool ATOMIC>
 __global__ __launch_bounds__(128) void stiffness_2d_kernel(
-    int nspec, const int *__restrict__ gather, const float *__restrict__ xix_, const float *__restrict__ xiz_,
+    int first, int count, const int *__restrict__ gather, const float *__restrict__ xix_, const float *__restrict__ xiz_,
     const float *__restrict__ gammax_, const float *__restrict__ gammaz_, const float *__restrict__ jac_,
     const float *__restrict__ kappa_, const float *__restrict__ mu_, GllConst q, Fields f) {
   __shared__ float su[E2][2][25];
@@ -36,8 +46,9 @@ __global__ __launch_bounds__(128) void stiffness_2d_kernel(
   const int el = t / 25;
   const int p = t - el * 25;
   const int iz = p / 5, ix = p - (p / 5) * 5;
-  const int e = blockIdx.x * E2 + el;
-  const bool active = (el < E2) && (e < nspec);
+  const int el_idx = blockIdx.x * E2 + el;
+  const bool active = (el < E2) && (el_idx < count);
+  const int e = first + el_idx;
   const size_t idx = static_cast<size_t>(e) * 25 + p;
 
   int g = 0;
@@ -111,7 +122,7 @@ __global__ __launch_bounds__(128) void stiffness_2d_kernel(
 #pragma unroll
   for (int c = 0; c < 2; c++) {
     const float r = -(wz * t1[c] + wx * t2[c]);
-    atomic_add(&f.a[c * f.nglob_pad + g], r);
+    scatter_add<ATOMIC>(&f.a[c * f.nglob_pad + g], r);
   }
 }
 
@@ -124,7 +135,8 @@ __global__ __launch_bounds__(128) void stiffness_2d_kernel(
 // ---------------------------------------------------------------------------
 constexpr int E3 = 5;
 
-__global__ __launch_bounds__(128) void stiffness_3d_kernel(int nspec, const int *__restrict__ gather,
+template <bool ATOMIC>
+__global__ __launch_bounds__(128) void stiffness_3d_kernel(int first, int count, const int *__restrict__ gather,
                                                            const float *__restrict__ kappa_,
                                                            const float *__restrict__ mu_, float xix, float ety,
                                                            float gaz, float jac, GllConst q, Fields f) {
@@ -136,8 +148,9 @@ __global__ __launch_bounds__(128) void stiffness_3d_kernel(int nspec, const int 
   const int el = t / 25;
   const int col = t - el * 25;
   const int j = col / 5, i = col - (col / 5) * 5;
-  const int e = blockIdx.x * E3 + el;
-  const bool active = (el < E3) && (e < nspec);
+  const int el_idx = blockIdx.x * E3 + el;
+  const bool active = (el < E3) && (el_idx < count);
+  const int e = first + el_idx;
   const size_t base = static_cast<size_t>(e) * 125 + col;
   const int np = f.nglob_pad;
 
@@ -223,7 +236,7 @@ __global__ __launch_bounds__(128) void stiffness_3d_kernel(int nspec, const int 
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const float r = -(wjk * r0[c] + wik * r1[c] + wij * r2[c]);
-      atomic_add(&f.a[c * np + g[k]], r);
+      scatter_add<ATOMIC>(&f.a[c * np + g[k]], r);
     }
   }
 }
@@ -398,6 +411,327 @@ __global__ __launch_bounds__(256) void divide_corrector_kernel(int n, int ncomp,
   }
 }
 
+// ---------------------------------------------------------------------------
+// 3-D structured path with buffered element forces.
+//
+// elemforce_3d: the stiffness of stiffness_3d_kernel, but the element's
+// -K_e u_e is stored to re (element-major, coalesced) instead of scattered.
+// Displacement addresses come from the tiled lattice position; the
+// workgroup -> element-block mapping is XCD-aware: the 8 XCDs receive
+// workgroups round-robin, so each XCD is given one contiguous slab of
+// elements and its L2 sees the shared faces of neighbouring elements.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int xcd_block(int b, int nblocks) {
+  constexpr int NXCD = 8;
+  const int per = nblocks / NXCD;  // blocks with index < per * NXCD are remapped
+  if (b >= per * NXCD) return b;
+  return (b % NXCD) * per + b / NXCD;
+}
+
+// Device index of local point (i, j, k) (0..4) of element (ex, ey, ez).
+__device__ __forceinline__ int tiled_index(const Lattice &lat, int ex, int ey, int ez, int i, int j, int k) {
+  const int tx = ex + (i >> 2), ty = ey + (j >> 2), tz = ez + (k >> 2);
+  return ((tz * lat.TY + ty) * lat.TX + tx) * 64 + (((k & 3) * 4 + (j & 3)) * 4 + (i & 3));
+}
+
+// Slot of local point (i, j, k) in an element's block of re.
+__device__ __forceinline__ int re_slot(int i, int j, int k) {
+  const int fi = i >> 2, fj = j >> 2, fk = k >> 2;  // 1 on the local index-4 faces
+  if (!(fi | fj | fk)) return (k * 4 + j) * 4 + i;
+  if (fi & !fj & !fk) return 64 + k * 4 + j;
+  if (!fi & fj & !fk) return 80 + k * 4 + i;
+  if (!fi & !fj & fk) return 96 + j * 4 + i;
+  if (fi & fj & !fk) return 112 + k;
+  if (fi & !fj & fk) return 116 + j;
+  if (!fi & fj & fk) return 120 + i;
+  return 124;
+}
+
+__global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const float *__restrict__ kappa_,
+                                                           const float *__restrict__ mu_, float xix, float ety,
+                                                           float gaz, float jac, GllConst q,
+                                                           const float *__restrict__ uf, int np,
+                                                           float *__restrict__ re) {
+  __shared__ float su[E3][3][125];
+  __shared__ float sF0[E3][3][125];
+  __shared__ float sF1[E3][3][125];
+
+  const int nspec = lat.nx * lat.ny * lat.nz;
+  const int t = threadIdx.x;
+  const int el = t / 25;
+  const int col = t - el * 25;
+  const int j = col / 5, i = col - (col / 5) * 5;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int e = blk * E3 + el;
+  const bool active = (el < E3) && (e < nspec);
+  const size_t base = static_cast<size_t>(e) * 125 + col;
+
+  float hpi[5], hpj[5], hwi[5], hwj[5];
+#pragma unroll
+  for (int l = 0; l < 5; l++) {
+    hpi[l] = q.hp[i][l];
+    hpj[l] = q.hp[j][l];
+    hwi[l] = q.hpw[i][l];
+    hwj[l] = q.hpw[j][l];
+  }
+  const float wi = q.w[i], wj = q.w[j];
+
+  float u[5][3];
+  if (active) {
+    const int ex = e % lat.nx, ey = (e / lat.nx) % lat.ny, ez = e / (lat.nx * lat.ny);
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      const int d = tiled_index(lat, ex, ey, ez, i, j, k);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        u[k][c] = uf[c * np + d];
+        su[el][c][k * 25 + col] = u[k][c];
+      }
+    }
+  }
+  __syncthreads();
+
+  float F2[5][3];
+  if (active) {
+    const float sx = jac * xix, sy = jac * ety, sz = jac * gaz;
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      float dxi[3] = {0, 0, 0}, det[3] = {0, 0, 0}, dga[3] = {0, 0, 0};
+#pragma unroll
+      for (int l = 0; l < 5; l++) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          dxi[c] += hpi[l] * su[el][c][k * 25 + j * 5 + l];
+          det[c] += hpj[l] * su[el][c][k * 25 + l * 5 + i];
+          dga[c] += q.hp[k][l] * u[l][c];
+        }
+      }
+      const float kappa = kappa_[base + k * 25];
+      const float mu = mu_[base + k * 25];
+      const float lp2m = kappa + (4.0f / 3.0f) * mu;
+      const float lambda = lp2m - 2.0f * mu;
+      const float exx = xix * dxi[0], eyy = ety * det[1], ezz = gaz * dga[2];
+      const float sxx = lp2m * exx + lambda * (eyy + ezz);
+      const float syy = lp2m * eyy + lambda * (exx + ezz);
+      const float szz = lp2m * ezz + lambda * (exx + eyy);
+      const float sxy = mu * (ety * det[0] + xix * dxi[1]);
+      const float sxz = mu * (gaz * dga[0] + xix * dxi[2]);
+      const float syz = mu * (gaz * dga[1] + ety * det[2]);
+      const int p = k * 25 + col;
+      sF0[el][0][p] = sx * sxx;
+      sF0[el][1][p] = sx * sxy;
+      sF0[el][2][p] = sx * sxz;
+      sF1[el][0][p] = sy * sxy;
+      sF1[el][1][p] = sy * syy;
+      sF1[el][2][p] = sy * syz;
+      F2[k][0] = sz * sxz;
+      F2[k][1] = sz * syz;
+      F2[k][2] = sz * szz;
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  float *out = re + static_cast<size_t>(e) * 375;
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    float r0[3] = {0, 0, 0}, r1[3] = {0, 0, 0}, r2[3] = {0, 0, 0};
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        r0[c] += sF0[el][c][k * 25 + j * 5 + l] * hwi[l];
+        r1[c] += sF1[el][c][k * 25 + l * 5 + i] * hwj[l];
+        r2[c] += F2[l][c] * q.hpw[k][l];
+      }
+    }
+    const float wjk = wj * q.w[k], wik = wi * q.w[k], wij = wi * wj;
+    const int slot = re_slot(i, j, k);
+#pragma unroll
+    for (int c = 0; c < 3; c++) out[c * 125 + slot] = -(wjk * r0[c] + wik * r1[c] + wij * r2[c]);
+  }
+}
+
+// Sum, in a fixed order, of the element forces at lattice point
+// (4 tx + i, 4 ty + j, 4 tz + k), i, j, k < 4: the element owning the tile,
+// then the elements below in x, y, z, xy, xz, yz and xyz that share the point.
+__device__ __forceinline__ void assemble_tiled(const Lattice &lat, const float *__restrict__ re, int tx, int ty,
+                                               int tz, int i, int j, int k, float acc[3]) {
+  acc[0] = acc[1] = acc[2] = 0.0f;
+  auto add = [&](int ex, int ey, int ez, int slot) {
+    if (ex < 0 || ey < 0 || ez < 0 || ex >= lat.nx || ey >= lat.ny || ez >= lat.nz) return;
+    const float *r = re + ((static_cast<size_t>(ez) * lat.ny + ey) * lat.nx + ex) * 375 + slot;
+    acc[0] += r[0];
+    acc[1] += r[125];
+    acc[2] += r[250];
+  };
+  add(tx, ty, tz, (k * 4 + j) * 4 + i);
+  if (i == 0) add(tx - 1, ty, tz, 64 + k * 4 + j);
+  if (j == 0) add(tx, ty - 1, tz, 80 + k * 4 + i);
+  if (k == 0) add(tx, ty, tz - 1, 96 + j * 4 + i);
+  if (i == 0 && j == 0) add(tx - 1, ty - 1, tz, 112 + k);
+  if (i == 0 && k == 0) add(tx - 1, ty, tz - 1, 116 + j);
+  if (j == 0 && k == 0) add(tx, ty - 1, tz - 1, 120 + i);
+  if (i == 0 && j == 0 && k == 0) add(tx - 1, ty - 1, tz - 1, 124);
+}
+
+// Device slot d -> (tile, local) and whether it is a lattice point.
+__device__ __forceinline__ bool tiled_point(const Lattice &lat, int d, int &tx, int &ty, int &tz, int &i, int &j,
+                                            int &k) {
+  const int tile = d >> 6, l = d & 63;
+  i = l & 3;
+  j = (l >> 2) & 3;
+  k = l >> 4;
+  tx = tile % lat.TX;
+  ty = (tile / lat.TX) % lat.TY;
+  tz = tile / (lat.TX * lat.TY);
+  return (tx < lat.nx || i == 0) && (ty < lat.ny || j == 0) && (tz < lat.nz || k == 0);
+}
+
+__global__ __launch_bounds__(256) void assemble_update_3d_kernel(Lattice lat, const float *__restrict__ re, Fields f,
+                                                                 NewmarkConst nc) {
+  const int n = lat.TX * lat.TY * lat.TZ * 64;
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= n) return;
+  int tx, ty, tz, i, j, k;
+  if (!tiled_point(lat, d, tx, ty, tz, i, j, k)) return;
+  float acc[3];
+  assemble_tiled(lat, re, tx, ty, tz, i, j, k, acc);
+  const int np = f.nglob_pad;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const size_t o = static_cast<size_t>(c) * np + d;
+    const float a = acc[c] * f.minv[o];
+    const float v1 = f.v[o] + nc.dto2 * a;
+    f.u[o] = f.u[o] + (nc.dt * v1 + nc.dt2o2 * a);
+    f.v[o] = v1 + nc.dto2 * a;
+  }
+}
+
+__global__ __launch_bounds__(256) void assemble_3d_kernel(Lattice lat, const float *__restrict__ re, Fields f,
+                                                          int divide) {
+  const int n = lat.TX * lat.TY * lat.TZ * 64;
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= n) return;
+  int tx, ty, tz, i, j, k;
+  if (!tiled_point(lat, d, tx, ty, tz, i, j, k)) return;
+  float acc[3];
+  assemble_tiled(lat, re, tx, ty, tz, i, j, k, acc);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const size_t o = static_cast<size_t>(c) * f.nglob_pad + d;
+    f.a[o] = divide ? acc[c] * f.minv[o] : acc[c];
+  }
+}
+
+// One thread per element point adds every source in source order: two
+// sources in one element meet in the same thread, so plain adds suffice.
+__global__ __launch_bounds__(128) void sources_re_kernel(int nsrc, int istep, const int *__restrict__ ispec,
+                                                         const float *__restrict__ array,
+                                                         const float *__restrict__ stf, float *__restrict__ re) {
+  const int p = threadIdx.x;
+  if (p >= 125) return;
+  const int slot = re_slot(p % 5, (p / 5) % 5, p / 25);
+  for (int s = 0; s < nsrc; s++) {
+    const float sv = stf[static_cast<size_t>(istep) * nsrc + s];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      re[(static_cast<size_t>(ispec[s]) * 3 + c) * 125 + slot] += sv * array[(static_cast<size_t>(s) * 3 + c) * 125 + p];
+  }
+}
+
+// One thread per element point on the absorbing boundary; its entries (one per
+// absorbing face through the point) are summed in entry order.
+__global__ __launch_bounds__(256) void stacey_re_kernel(int nseg, const int *__restrict__ seg,
+                                                        const int *__restrict__ gather,
+                                                        const int *__restrict__ slot,
+                                                        const float *__restrict__ normal,
+                                                        const float *__restrict__ weight,
+                                                        const float *__restrict__ rho_vp,
+                                                        const float *__restrict__ rho_vs, Fields f,
+                                                        float *__restrict__ re) {
+  const int sidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sidx >= nseg) return;
+  const int q0 = seg[sidx], q1 = seg[sidx + 1];
+  const int g = gather[q0];
+  const int sl = slot[q0];
+  const int e = sl / 125, p = sl - (sl / 125) * 125;
+  const int rs = re_slot(p % 5, (p / 5) % 5, p / 25);
+  const int np = f.nglob_pad;
+  float v[3], t[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int d = 0; d < 3; d++) v[d] = f.v[d * np + g];
+  for (int q = q0; q < q1; q++) {
+    float n[3];
+    float vn = 0.0f, nn = 0.0f;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      n[d] = normal[q * 3 + d];
+      vn += v[d] * n[d];
+      nn += n[d] * n[d];
+    }
+    const float j1 = sqrtf(nn);
+    const float w = weight[q], rvp = rho_vp[q], rvs = rho_vs[q];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const float fac = ((vn * n[d] / (j1 * j1)) * (rvp - rvs)) + v[d] * rvs;
+      t[d] += -1.0f * fac * j1 * w;
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) re[(static_cast<size_t>(e) * 3 + d) * 125 + rs] += t[d];
+}
+
+__global__ __launch_bounds__(128) void seismogram_re_kernel(Lattice lat, int nrec, int ntypes,
+                                                            const int *__restrict__ types, int isig,
+                                                            const int *__restrict__ ispec,
+                                                            const float *__restrict__ h,
+                                                            const float *__restrict__ re, Fields f, float dto2,
+                                                            float *__restrict__ seismo) {
+  __shared__ float red[3][2];
+  const int r = blockIdx.x;
+  const int p = threadIdx.x;
+  const int np = f.nglob_pad;
+  float hw = 0.0f;
+  int L = 0;
+  float acc[3] = {0, 0, 0};
+  if (p < 125) {
+    const int e = ispec[r];
+    const int ex = e % lat.nx, ey = (e / lat.nx) % lat.ny, ez = e / (lat.nx * lat.ny);
+    const int i = p % 5, j = (p / 5) % 5, k = p / 25;
+    L = tiled_index(lat, ex, ey, ez, i, j, k);
+    hw = h[static_cast<size_t>(r) * 125 + p];
+    assemble_tiled(lat, re, ex + (i >> 2), ey + (j >> 2), ez + (k >> 2), i & 3, j & 3, k & 3, acc);
+  }
+  for (int it = 0; it < ntypes; it++) {
+    const int type = types[it];
+    float val[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      float x = 0.0f;
+      if (p < 125) {
+        const size_t o = static_cast<size_t>(c) * np + L;
+        const float a = acc[c] * f.minv[o];
+        x = type == 0 ? f.u[o] : (type == 1 ? f.v[o] + dto2 * a : a);
+      }
+      val[c] = hw * x;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      float s = val[c];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+      if ((p & 63) == 0) red[c][p >> 6] = s;
+    }
+    __syncthreads();
+    if (p == 0) {
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        seismo[((static_cast<size_t>(isig) * ntypes + it) * nrec + r) * 3 + c] = red[c][0] + red[c][1];
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void fill_kernel(float *p, size_t n, float value) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = value;
@@ -407,19 +741,27 @@ inline unsigned blocks(size_t n, unsigned per) { return static_cast<unsigned>((n
 
 }  // namespace
 
-void launch_stiffness_2d(hipStream_t s, int nspec, const int *gather, const float *xix, const float *xiz,
-                         const float *gammax, const float *gammaz, const float *jac, const float *kappa,
-                         const float *mu, const GllConst &q, Fields f) {
-  if (nspec <= 0) return;
-  hipLaunchKernelGGL(stiffness_2d_kernel, dim3(blocks(nspec, E2)), dim3(128), 0, s, nspec, gather, xix, xiz, gammax,
-                     gammaz, jac, kappa, mu, q, f);
+void launch_stiffness_2d(hipStream_t s, int first, int count, bool atomic, const int *gather, const float *xix,
+                         const float *xiz, const float *gammax, const float *gammaz, const float *jac,
+                         const float *kappa, const float *mu, const GllConst &q, Fields f) {
+  if (count <= 0) return;
+  if (atomic)
+    hipLaunchKernelGGL(stiffness_2d_kernel<true>, dim3(blocks(count, E2)), dim3(128), 0, s, first, count, gather, xix,
+                       xiz, gammax, gammaz, jac, kappa, mu, q, f);
+  else
+    hipLaunchKernelGGL(stiffness_2d_kernel<false>, dim3(blocks(count, E2)), dim3(128), 0, s, first, count, gather, xix,
+                       xiz, gammax, gammaz, jac, kappa, mu, q, f);
 }
 
-void launch_stiffness_3d(hipStream_t s, int nspec, const int *gather, const float *kappa, const float *mu, float xix,
-                         float ety, float gaz, float jac, const GllConst &q, Fields f) {
-  if (nspec <= 0) return;
-  hipLaunchKernelGGL(stiffness_3d_kernel, dim3(blocks(nspec, E3)), dim3(128), 0, s, nspec, gather, kappa, mu, xix,
-                     ety, gaz, jac, q, f);
+void launch_stiffness_3d(hipStream_t s, int first, int count, bool atomic, const int *gather, const float *kappa,
+                         const float *mu, float xix, float ety, float gaz, float jac, const GllConst &q, Fields f) {
+  if (count <= 0) return;
+  if (atomic)
+    hipLaunchKernelGGL(stiffness_3d_kernel<true>, dim3(blocks(count, E3)), dim3(128), 0, s, first, count, gather, kappa,
+                       mu, xix, ety, gaz, jac, q, f);
+  else
+    hipLaunchKernelGGL(stiffness_3d_kernel<false>, dim3(blocks(count, E3)), dim3(128), 0, s, first, count, gather,
+                       kappa, mu, xix, ety, gaz, jac, q, f);
 }
 
 void launch_stacey(hipStream_t s, int dim, int nbnd, const int *gather, const float *normal, const float *weight,
@@ -473,6 +815,45 @@ void launch_divide_corrector(hipStream_t s, int ncomp, Fields f, NewmarkConst nc
 void launch_fill(hipStream_t s, float *p, size_t n, float value) {
   if (n == 0) return;
   hipLaunchKernelGGL(fill_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, p, n, value);
+}
+
+void launch_elemforce_3d(hipStream_t s, Lattice lat, const float *kappa, const float *mu, float xix, float ety,
+                         float gaz, float jac, const GllConst &q, const float *u, int nglob_pad, float *re) {
+  const int nspec = lat.nx * lat.ny * lat.nz;
+  if (nspec <= 0) return;
+  hipLaunchKernelGGL(elemforce_3d_kernel, dim3(blocks(nspec, E3)), dim3(128), 0, s, lat, kappa, mu, xix, ety, gaz, jac,
+                     q, u, nglob_pad, re);
+}
+
+void launch_sources_re(hipStream_t s, int nsrc, int istep, const int *ispec, const float *array, const float *stf,
+                       float *re) {
+  if (nsrc <= 0) return;
+  hipLaunchKernelGGL(sources_re_kernel, dim3(1), dim3(128), 0, s, nsrc, istep, ispec, array, stf, re);
+}
+
+void launch_stacey_re(hipStream_t s, int nseg, const int *seg, const int *gather, const int *slot,
+                      const float *normal, const float *weight, const float *rho_vp, const float *rho_vs, Fields f,
+                      float *re) {
+  if (nseg <= 0) return;
+  hipLaunchKernelGGL(stacey_re_kernel, dim3(blocks(nseg, 256)), dim3(256), 0, s, nseg, seg, gather, slot, normal,
+                     weight, rho_vp, rho_vs, f, re);
+}
+
+void launch_assemble_update_3d(hipStream_t s, Lattice lat, const float *re, Fields f, NewmarkConst nc) {
+  const size_t n = static_cast<size_t>(lat.TX) * lat.TY * lat.TZ * 64;
+  hipLaunchKernelGGL(assemble_update_3d_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, lat, re, f, nc);
+}
+
+void launch_assemble_3d(hipStream_t s, Lattice lat, const float *re, Fields f, int divide) {
+  const size_t n = static_cast<size_t>(lat.TX) * lat.TY * lat.TZ * 64;
+  hipLaunchKernelGGL(assemble_3d_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, lat, re, f, divide);
+}
+
+void launch_seismograms_re(hipStream_t s, Lattice lat, int nrec, int ntypes, const int *types, int isig,
+                           const int *ispec, const float *h, const float *re, Fields f, float dto2, float *seismo) {
+  if (nrec <= 0 || ntypes <= 0) return;
+  hipLaunchKernelGGL(seismogram_re_kernel, dim3(nrec), dim3(128), 0, s, lat, nrec, ntypes, types, isig, ispec, h, re, f,
+                     dto2, seismo);
 }
 
 }  // namespace sfx

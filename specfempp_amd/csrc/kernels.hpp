@@ -27,11 +27,42 @@ struct NewmarkConst {
   float dt, dto2, dt2o2;
 };
 
-void launch_stiffness_2d(hipStream_t s, int nspec, const int *gather, const float *xix, const float *xiz,
-                         const float *gammax, const float *gammaz, const float *jac, const float *kappa,
-                         const float *mu, const GllConst &q, Fields f);
-void launch_stiffness_3d(hipStream_t s, int nspec, const int *gather, const float *kappa, const float *mu,
-                         float xix, float ety, float gaz, float jac, const GllConst &q, Fields f);
+// Stiffness over device element slots [first, first + count).  atomic = false
+// requires that no two of these elements share a global point (one color).
+void launch_stiffness_2d(hipStream_t s, int first, int count, bool atomic, const int *gather, const float *xix,
+                         const float *xiz, const float *gammax, const float *gammaz, const float *jac,
+                         const float *kappa, const float *mu, const GllConst &q, Fields f);
+void launch_stiffness_3d(hipStream_t s, int first, int count, bool atomic, const int *gather, const float *kappa,
+                         const float *mu, float xix, float ety, float gaz, float jac, const GllConst &q, Fields f);
+// ---- 3-D structured path with buffered element forces (SFX_SCATTER_GATHER) ----
+// Tiled lattice of GLL points: point (I, J, K) lives in tile
+// (I / 4, J / 4, K / 4) of TX = nx + 1 by TY by TZ tiles, at device index
+// ((tz TY + ty) TX + tx) 64 + ((K % 4) 4 + J % 4) 4 + I % 4.
+// Element forces re[(e * 3 + c) * 125 + slot] with the slot order of
+// re_slot(): the element's own tile (local i, j, k < 4) first, then its faces,
+// edges and corner at local index 4.
+struct Lattice {
+  int nx, ny, nz, TX, TY, TZ;
+};
+// Element forces -K_e u_e of every element into re (plain coalesced stores).
+void launch_elemforce_3d(hipStream_t s, Lattice lat, const float *kappa, const float *mu, float xix, float ety,
+                         float gaz, float jac, const GllConst &q, const float *u, int nglob_pad, float *re);
+// re += source contributions (stf(istep) * source_array) of each source element.
+void launch_sources_re(hipStream_t s, int nsrc, int istep, const int *ispec, const float *array, const float *stf,
+                       float *re);
+// re += Stacey traction of every boundary entry (slot = element * 125 + point);
+// entries [seg[i], seg[i+1]) share one slot and are summed by one thread.
+void launch_stacey_re(hipStream_t s, int nseg, const int *seg, const int *gather, const int *slot,
+                      const float *normal, const float *weight, const float *rho_vp, const float *rho_vs, Fields f,
+                      float *re);
+// Per lattice point: a = M^-1 sum_e re(e, p); corrector(n) + predictor(n+1) on u, v.
+void launch_assemble_update_3d(hipStream_t s, Lattice lat, const float *re, Fields f, NewmarkConst nc);
+// a = sum_e re(e, p) (times M^-1 when divide) for the seam path and read-out.
+void launch_assemble_3d(hipStream_t s, Lattice lat, const float *re, Fields f, int divide);
+// Seismograms sampled before the fused update: a from re, v corrected.
+void launch_seismograms_re(hipStream_t s, Lattice lat, int nrec, int ntypes, const int *types, int isig,
+                           const int *ispec, const float *h, const float *re, Fields f, float dto2, float *seismo);
+
 void launch_stacey(hipStream_t s, int dim, int nbnd, const int *gather, const float *normal, const float *weight,
                    const float *rho_vp, const float *rho_vs, Fields f);
 void launch_sources(hipStream_t s, int dim, int npe, int nsrc, int istep, const int *order, const int *ispec,

@@ -67,7 +67,8 @@ class _Receiver(C.Structure):
 
 class _Run(C.Structure):
     _fields_ = [("dt", C.c_double), ("nstep", C.c_int), ("nstep_between_samples", C.c_int),
-                ("nseismo_types", C.c_int), ("seismo_types", C.c_int * 3), ("chunk_size", C.c_int)]
+                ("nseismo_types", C.c_int), ("seismo_types", C.c_int * 3), ("chunk_size", C.c_int),
+                ("scatter", C.c_int)]
 
 
 def _ptr(a):
@@ -104,9 +105,13 @@ def _receivers(receivers):
     return arr
 
 
-def _run_desc(dt, nstep, nsb, types, chunk):
+SCATTER_COLORED, SCATTER_ATOMIC, SCATTER_GATHER = 0, 1, 2
+
+
+def _run_desc(dt, nstep, nsb, types, chunk, scatter=SCATTER_COLORED):
     run = _Run()
     run.dt, run.nstep, run.nstep_between_samples, run.chunk_size = dt, nstep, nsb, chunk
+    run.scatter = {"colored": 0, "atomic": 1, "gather": 2}.get(scatter, scatter)
     run.nseismo_types = len(types)
     for i, t in enumerate(types):
         run.seismo_types[i] = _TYPE_CODES[t] if isinstance(t, str) else int(t)
@@ -221,7 +226,7 @@ class Engine2D(_EngineBase):
     """2-D elastic P-SV engine built from a :class:`Mesh2D`-like object."""
 
     def __init__(self, mesh, sources, receivers, dt, nstep, nstep_between_samples=1,
-                 seismogram_types=("displacement",), chunk_size=1):
+                 seismogram_types=("displacement",), chunk_size=1, scatter="colored"):
         self.lib = _native.load()
         self.nstep = nstep
         coorg = np.ascontiguousarray(np.asarray(mesh.coorg, dtype=np.float64).T)
@@ -235,7 +240,7 @@ class Engine2D(_EngineBase):
         self._keep = (coorg, knods, kmato, rho, vp, vs, abs_ispec, abs_edge)
         m = _Mesh2D(mesh.nspec, mesh.ngnod, mesh.npgeo, _ptr(coorg), _ptr(knods), _ptr(kmato), len(rho), _ptr(rho),
                     _ptr(vp), _ptr(vs), len(abs_ispec), _ptr(abs_ispec), _ptr(abs_edge))
-        run = _run_desc(dt, nstep, nstep_between_samples, seismogram_types, chunk_size)
+        run = _run_desc(dt, nstep, nstep_between_samples, seismogram_types, chunk_size, scatter)
         h = C.c_void_p()
         self.h = None
         self._check(self.lib.sfx_engine_create_2d(C.byref(m), len(sources), _sources(sources, 2), len(receivers),
@@ -248,7 +253,7 @@ class Engine3D(_EngineBase):
     """3-D elastic engine on a structured hexahedral mesh (see ``mesh3d``)."""
 
     def __init__(self, mesh, sources, receivers, dt, nstep, nstep_between_samples=1,
-                 seismogram_types=("displacement",)):
+                 seismogram_types=("displacement",), scatter="gather"):
         self.lib = _native.load()
         self.nstep = nstep
         mat = np.ascontiguousarray(mesh["material"], dtype=np.int32).ravel()
@@ -264,7 +269,7 @@ class Engine3D(_EngineBase):
         m.rho, m.vp, m.vs = _ptr(rho), _ptr(vp), _ptr(vs)
         for k in range(6):
             m.absorbing[k] = int(mesh["absorbing"][k])
-        run = _run_desc(dt, nstep, nstep_between_samples, seismogram_types, 1)
+        run = _run_desc(dt, nstep, nstep_between_samples, seismogram_types, 1, scatter)
         h = C.c_void_p()
         self.h = None
         self._check(self.lib.sfx_engine_create_3d(C.byref(m), len(sources), _sources(sources, 3), len(receivers),

@@ -23,14 +23,20 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
 def algorithmic_bytes_3d(nspec: int, nglob: int) -> dict:
-    """Bytes each kernel must move per launch (FP32 / int32), every array the
-    data model defines touched once per pass:
-      stiffness: per local point gather index + kappa + mu (12 B); per global
-                 point displacement read (12 B) and acceleration read-modify-write (24 B)
-      update:    per global point read u, v, a, M^-1 (48 B), write u, v, a (36 B)."""
-    stiff = nspec * 125 * 12 + nglob * 36
-    upd = nglob * 84
-    return {"stiffness": stiff, "update": upd, "per_pt_step": (stiff + upd) / nglob}
+    """Bytes each kernel of the 3-D gather path must move per launch (FP32):
+      elemforce_3d:       kappa + mu per local point (8 B x 125 / element), the
+                          displacement once per global point (12 B), the element
+                          forces stored once (12 B x 125 / element)
+      assemble_update_3d: the element forces loaded once (12 B x 125 / element),
+                          u, v, M^-1 read (36 B) and u, v written (24 B) per point
+    and the survey's data-model figure for a whole step (143 B per point-step:
+    gather index + kappa + mu per local point, displacement gather, acceleration
+    read-modify-write, fused update read u v a M^-1 / write u v a)."""
+    elem = nspec * 125 * 8 + nglob * 12 + nspec * 125 * 12
+    upd = nspec * 125 * 12 + nglob * 60
+    survey = nspec * 125 * 12 + nglob * 36 + nglob * 84
+    return {"elemforce": elem, "update": upd, "per_pt_step": (elem + upd) / nglob,
+            "survey_per_pt_step": survey / nglob}
 
 
 def cpu_baseline(w, nsteps: int) -> dict:
@@ -72,7 +78,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=64)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (scale reference for HBM byte counters)")
@@ -110,10 +116,15 @@ def main() -> int:
     ms_per_step = 1e3 * elapsed / args.steps
     value = nglob * args.steps / elapsed
     ab = algorithmic_bytes_3d(nspec, nglob)
-    stiff_s = tm["stiffness_ms"] * 1e-3
-    achieved = ab["stiffness"] / stiff_s / 1e9 if stiff_s > 0 else 0.0
-    upd_gbs = ab["update"] / (tm["update_ms"] * 1e-3) / 1e9 if tm["update_ms"] > 0 else 0.0
+    kern = {"elemforce_3d_kernel": (ab["elemforce"], tm["stiffness_ms"]),
+            "assemble_update_3d_kernel": (ab["update"], tm["update_ms"])}
+    dominant = max(kern, key=lambda k: kern[k][1])
+    nbytes, ms = kern[dominant]
+    achieved = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    per_kernel = {k: {"avg_ms": v[1], "algorithmic_bytes": v[0],
+                      "GBs": (v[0] / (v[1] * 1e-3) / 1e9 if v[1] > 0 else 0.0)} for k, v in kern.items()}
     step_gbs = ab["per_pt_step"] * value / 1e9
+    survey_gbs = ab["survey_per_pt_step"] * value / 1e9
     out = {
         "metric": "GLL pt-steps/s, 3D elastic NGLL=5 FP32 forward (64^3 layered, Stacey + free surface)",
         "value": value,
@@ -129,15 +140,17 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"3d_layered_cube_{args.n}^3_ngll5", "nspec": nspec, "nglob": nglob,
                    "dt": w.dt, "dt_stated": w.dt_stated, "nrec": len(w.receivers), "source": "moment tensor",
-                   "setup_s": round(setup_s, 2)},
+                   "scatter": "gather", "setup_s": round(setup_s, 2)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "stiffness_3d_kernel",
-                     "algorithmic_bytes_per_launch": ab["stiffness"], "avg_ms": tm["stiffness_ms"]},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dominant,
+                     "algorithmic_bytes_per_launch": nbytes, "avg_ms": ms},
+        "kernels": per_kernel,
         "kernels_ms": tm,
-        "update_kernel_gbs": upd_gbs,
         "step_hbm_gbs": step_gbs,
         "step_hbm_frac": step_gbs / HBM_PEAK_GBS,
         "algorithmic_bytes_per_pt_step": ab["per_pt_step"],
+        "survey_bytes_per_pt_step": ab["survey_per_pt_step"],
+        "survey_step_hbm_frac": survey_gbs / HBM_PEAK_GBS,
         "seismograms_finite": finite,
     }
     if args.calibrate_copy:

@@ -74,7 +74,8 @@ struct sfx_engine {
   DevBuf<float> bnd_normal, bnd_weight, bnd_rvp, bnd_rvs;
   DevBuf<float> src_array, stf, rec_h, seismo;
   DevBuf<float> fields;  // u, v, a, minv: 4 * ncomp * nglob_pad
-  DevBuf<float> re;      // 3-D gather path: element forces (nspec, 3, 125)
+  DevBuf<float> re;      // 3-D gather path: element forces (nspec, 3, 128)
+  DevBuf<float> zero;    // 384 zeros read in place of absent element contributions
   bool initialized = false;
   bool a_stale = false;  // fused gather steps leave the stored acceleration behind
   Lattice lat{};
@@ -125,7 +126,12 @@ struct sfx_engine {
     bnd_gather.upload(h.bnd_gather, s);
     bnd_slot.upload(h.bnd_slot, s);
     bnd_seg.upload(h.bnd_seg, s);
-    if (gather_path()) re.alloc(static_cast<size_t>(h.nspec) * 375);
+    if (gather_path()) {
+      re.alloc(static_cast<size_t>(h.nspec) * 384);  // 3 components x 128 (125 used)
+      launch_fill(s, re.p, re.n, 0.0f);
+      zero.alloc(384);
+      launch_fill(s, zero.p, zero.n, 0.0f);
+    }
     bnd_normal.upload(h.bnd_normal, s);
     bnd_weight.upload(h.bnd_weight, s);
     bnd_rvp.upload(h.bnd_rho_vp, s);
@@ -228,7 +234,7 @@ struct sfx_engine {
                               nc.dto2, seismo.p);
       }
       if (evs) SFX_HIP(hipEventRecord(evs[3], stream));
-      launch_assemble_update_3d(stream, lat, re.p, ff, nc);
+      launch_assemble_update_3d(stream, lat, re.p, zero.p, ff, nc);
       if (evs) SFX_HIP(hipEventRecord(evs[4], stream));
       a_stale = true;
       return;

@@ -434,6 +434,11 @@ __device__ __forceinline__ int tiled_index(const Lattice &lat, int ex, int ey, i
   return ((tz * lat.TY + ty) * lat.TX + tx) * 64 + (((k & 3) * 4 + (j & 3)) * 4 + (i & 3));
 }
 
+// Element-force block: 3 components of 128 floats (125 used), so every
+// component starts 16-byte aligned.
+constexpr int RE_STRIDE = 128;
+constexpr int RE_ELEM = 3 * RE_STRIDE;
+
 // Slot of local point (i, j, k) in an element's block of re.
 __device__ __forceinline__ int re_slot(int i, int j, int k) {
   const int fi = i >> 2, fj = j >> 2, fk = k >> 2;  // 1 on the local index-4 faces
@@ -531,7 +536,7 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
   }
   __syncthreads();
   if (!active) return;
-  float *out = re + static_cast<size_t>(e) * 375;
+  float *out = re + static_cast<size_t>(e) * RE_ELEM;
 #pragma unroll
   for (int k = 0; k < 5; k++) {
     float r0[3] = {0, 0, 0}, r1[3] = {0, 0, 0}, r2[3] = {0, 0, 0};
@@ -547,30 +552,32 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
     const float wjk = wj * q.w[k], wik = wi * q.w[k], wij = wi * wj;
     const int slot = re_slot(i, j, k);
 #pragma unroll
-    for (int c = 0; c < 3; c++) out[c * 125 + slot] = -(wjk * r0[c] + wik * r1[c] + wij * r2[c]);
+    for (int c = 0; c < 3; c++) out[c * RE_STRIDE + slot] = -(wjk * r0[c] + wik * r1[c] + wij * r2[c]);
   }
 }
 
 // Sum, in a fixed order, of the element forces at lattice point
 // (4 tx + i, 4 ty + j, 4 tz + k), i, j, k < 4: the element owning the tile,
-// then the elements below in x, y, z, xy, xz, yz and xyz that share the point.
+// then the elements below in y, z, yz, x, xy, xz and xyz that share the point
+// (the order of the 4-point row version below, so both give equal bits).
 __device__ __forceinline__ void assemble_tiled(const Lattice &lat, const float *__restrict__ re, int tx, int ty,
                                                int tz, int i, int j, int k, float acc[3]) {
-  acc[0] = acc[1] = acc[2] = 0.0f;
+  bool first = true;
   auto add = [&](int ex, int ey, int ez, int slot) {
     if (ex < 0 || ey < 0 || ez < 0 || ex >= lat.nx || ey >= lat.ny || ez >= lat.nz) return;
-    const float *r = re + ((static_cast<size_t>(ez) * lat.ny + ey) * lat.nx + ex) * 375 + slot;
-    acc[0] += r[0];
-    acc[1] += r[125];
-    acc[2] += r[250];
+    const float *r = re + ((static_cast<size_t>(ez) * lat.ny + ey) * lat.nx + ex) * RE_ELEM + slot;
+#pragma unroll
+    for (int c = 0; c < 3; c++) acc[c] = first ? r[c * RE_STRIDE] : acc[c] + r[c * RE_STRIDE];
+    first = false;
   };
+  acc[0] = acc[1] = acc[2] = 0.0f;
   add(tx, ty, tz, (k * 4 + j) * 4 + i);
-  if (i == 0) add(tx - 1, ty, tz, 64 + k * 4 + j);
   if (j == 0) add(tx, ty - 1, tz, 80 + k * 4 + i);
   if (k == 0) add(tx, ty, tz - 1, 96 + j * 4 + i);
+  if (j == 0 && k == 0) add(tx, ty - 1, tz - 1, 120 + i);
+  if (i == 0) add(tx - 1, ty, tz, 64 + k * 4 + j);
   if (i == 0 && j == 0) add(tx - 1, ty - 1, tz, 112 + k);
   if (i == 0 && k == 0) add(tx - 1, ty, tz - 1, 116 + j);
-  if (j == 0 && k == 0) add(tx, ty - 1, tz - 1, 120 + i);
   if (i == 0 && j == 0 && k == 0) add(tx - 1, ty - 1, tz - 1, 124);
 }
 
@@ -587,23 +594,72 @@ __device__ __forceinline__ bool tiled_point(const Lattice &lat, int d, int &tx, 
   return (tx < lat.nx || i == 0) && (ty < lat.ny || j == 0) && (tz < lat.nz || k == 0);
 }
 
-__global__ __launch_bounds__(256) void assemble_update_3d_kernel(Lattice lat, const float *__restrict__ re, Fields f,
+// One thread per row of 4 points (i = 0..3) of a tile: 128-bit loads and
+// stores of u, v, M^-1 and of the element-force rows.  Contributions a point
+// does not receive are read from `zero` (an all-zero element block), so every
+// load is unconditional and the thread issues them all before the first wait.
+__global__ __launch_bounds__(256) void assemble_update_3d_kernel(Lattice lat, const float *__restrict__ re,
+                                                                 const float *__restrict__ zero, Fields f,
                                                                  NewmarkConst nc) {
-  const int n = lat.TX * lat.TY * lat.TZ * 64;
-  const int d = blockIdx.x * blockDim.x + threadIdx.x;
-  if (d >= n) return;
-  int tx, ty, tz, i, j, k;
-  if (!tiled_point(lat, d, tx, ty, tz, i, j, k)) return;
-  float acc[3];
-  assemble_tiled(lat, re, tx, ty, tz, i, j, k, acc);
-  const int np = f.nglob_pad;
+  const int nrows = lat.TX * lat.TY * lat.TZ * 16;
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= nrows) return;
+  const int tile = row >> 4, j = row & 3, k = (row >> 2) & 3;
+  const int tx = tile % lat.TX, ty = (tile / lat.TX) % lat.TY, tz = tile / (lat.TX * lat.TY);
+  if ((ty == lat.ny && j > 0) || (tz == lat.nz && k > 0)) return;  // padding rows
+  auto elem = [&](bool use, int ex, int ey, int ez) -> const float * {
+    if (!use || ex < 0 || ey < 0 || ez < 0 || ex >= lat.nx || ey >= lat.ny || ez >= lat.nz) return zero;
+    return re + ((static_cast<size_t>(ez) * lat.ny + ey) * lat.nx + ex) * RE_ELEM;
+  };
+  const bool j0 = j == 0, k0 = k == 0;
+  const float *r_own = elem(true, tx, ty, tz);
+  const float *r_y = elem(j0, tx, ty - 1, tz);
+  const float *r_z = elem(k0, tx, ty, tz - 1);
+  const float *r_yz = elem(j0 && k0, tx, ty - 1, tz - 1);
+  const float *r_x = elem(true, tx - 1, ty, tz);
+  const float *r_xy = elem(j0, tx - 1, ty - 1, tz);
+  const float *r_xz = elem(k0, tx - 1, ty, tz - 1);
+  const float *r_xyz = elem(j0 && k0, tx - 1, ty - 1, tz - 1);
+  const int n4 = f.nglob_pad / 4;
+  float4 m[3], u[3], v[3], acc[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    const size_t o = static_cast<size_t>(c) * np + d;
-    const float a = acc[c] * f.minv[o];
-    const float v1 = f.v[o] + nc.dto2 * a;
-    f.u[o] = f.u[o] + (nc.dt * v1 + nc.dt2o2 * a);
-    f.v[o] = v1 + nc.dto2 * a;
+    const size_t o = static_cast<size_t>(c) * n4 + row;
+    m[c] = reinterpret_cast<const float4 *>(f.minv)[o];
+    u[c] = reinterpret_cast<const float4 *>(f.u)[o];
+    v[c] = reinterpret_cast<const float4 *>(f.v)[o];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const int cs = c * RE_STRIDE;
+    const float4 a0 = *reinterpret_cast<const float4 *>(r_own + cs + (k * 4 + j) * 4);
+    const float4 a1 = *reinterpret_cast<const float4 *>(r_y + cs + 80 + k * 4);
+    const float4 a2 = *reinterpret_cast<const float4 *>(r_z + cs + 96 + j * 4);
+    const float4 a3 = *reinterpret_cast<const float4 *>(r_yz + cs + 120);
+    const float b0 = r_x[cs + 64 + k * 4 + j];
+    const float b1 = r_xy[cs + 112 + k];
+    const float b2 = r_xz[cs + 116 + j];
+    const float b3 = r_xyz[cs + 124];
+    // order: own, y, z, yz, then (point i = 0) x, xy, xz, xyz
+    acc[c].x = ((((((a0.x + a1.x) + a2.x) + a3.x) + b0) + b1) + b2) + b3;
+    acc[c].y = ((a0.y + a1.y) + a2.y) + a3.y;
+    acc[c].z = ((a0.z + a1.z) + a2.z) + a3.z;
+    acc[c].w = ((a0.w + a1.w) + a2.w) + a3.w;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const size_t o = static_cast<size_t>(c) * n4 + row;
+#define SFX_UPD(X)                                   \
+  {                                                  \
+    const float a = acc[c].X * m[c].X;               \
+    const float v1 = v[c].X + nc.dto2 * a;           \
+    u[c].X = u[c].X + (nc.dt * v1 + nc.dt2o2 * a);   \
+    v[c].X = v1 + nc.dto2 * a;                       \
+  }
+    SFX_UPD(x) SFX_UPD(y) SFX_UPD(z) SFX_UPD(w)
+#undef SFX_UPD
+    reinterpret_cast<float4 *>(f.u)[o] = u[c];
+    reinterpret_cast<float4 *>(f.v)[o] = v[c];
   }
 }
 
@@ -635,7 +691,8 @@ __global__ __launch_bounds__(128) void sources_re_kernel(int nsrc, int istep, co
     const float sv = stf[static_cast<size_t>(istep) * nsrc + s];
 #pragma unroll
     for (int c = 0; c < 3; c++)
-      re[(static_cast<size_t>(ispec[s]) * 3 + c) * 125 + slot] += sv * array[(static_cast<size_t>(s) * 3 + c) * 125 + p];
+      re[static_cast<size_t>(ispec[s]) * RE_ELEM + c * RE_STRIDE + slot] +=
+          sv * array[(static_cast<size_t>(s) * 3 + c) * 125 + p];
   }
 }
 
@@ -678,7 +735,7 @@ __global__ __launch_bounds__(256) void stacey_re_kernel(int nseg, const int *__r
     }
   }
 #pragma unroll
-  for (int d = 0; d < 3; d++) re[(static_cast<size_t>(e) * 3 + d) * 125 + rs] += t[d];
+  for (int d = 0; d < 3; d++) re[static_cast<size_t>(e) * RE_ELEM + d * RE_STRIDE + rs] += t[d];
 }
 
 __global__ __launch_bounds__(128) void seismogram_re_kernel(Lattice lat, int nrec, int ntypes,
@@ -839,9 +896,10 @@ void launch_stacey_re(hipStream_t s, int nseg, const int *seg, const int *gather
                      weight, rho_vp, rho_vs, f, re);
 }
 
-void launch_assemble_update_3d(hipStream_t s, Lattice lat, const float *re, Fields f, NewmarkConst nc) {
-  const size_t n = static_cast<size_t>(lat.TX) * lat.TY * lat.TZ * 64;
-  hipLaunchKernelGGL(assemble_update_3d_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, lat, re, f, nc);
+void launch_assemble_update_3d(hipStream_t s, Lattice lat, const float *re, const float *zero, Fields f,
+                               NewmarkConst nc) {
+  const size_t n = static_cast<size_t>(lat.TX) * lat.TY * lat.TZ * 16;
+  hipLaunchKernelGGL(assemble_update_3d_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, lat, re, zero, f, nc);
 }
 
 void launch_assemble_3d(hipStream_t s, Lattice lat, const float *re, Fields f, int divide) {

@@ -38,7 +38,7 @@ void launch_stiffness_3d(hipStream_t s, int first, int count, bool atomic, const
 // Tiled lattice of GLL points: point (I, J, K) lives in tile
 // (I / 4, J / 4, K / 4) of TX = nx + 1 by TY by TZ tiles, at device index
 // ((tz TY + ty) TX + tx) 64 + ((K % 4) 4 + J % 4) 4 + I % 4.
-// Element forces re[(e * 3 + c) * 125 + slot] with the slot order of
+// Element forces re[e * 384 + c * 128 + slot] with the slot order of
 // re_slot(): the element's own tile (local i, j, k < 4) first, then its faces,
 // edges and corner at local index 4.
 struct Lattice {
@@ -56,7 +56,9 @@ void launch_stacey_re(hipStream_t s, int nseg, const int *seg, const int *gather
                       const float *normal, const float *weight, const float *rho_vp, const float *rho_vs, Fields f,
                       float *re);
 // Per lattice point: a = M^-1 sum_e re(e, p); corrector(n) + predictor(n+1) on u, v.
-void launch_assemble_update_3d(hipStream_t s, Lattice lat, const float *re, Fields f, NewmarkConst nc);
+// zero: an all-zero block of 384 floats read in place of absent contributions.
+void launch_assemble_update_3d(hipStream_t s, Lattice lat, const float *re, const float *zero, Fields f,
+                               NewmarkConst nc);
 // a = sum_e re(e, p) (times M^-1 when divide) for the seam path and read-out.
 void launch_assemble_3d(hipStream_t s, Lattice lat, const float *re, Fields f, int divide);
 // Seismograms sampled before the fused update: a from re, v corrected.

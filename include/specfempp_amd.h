@@ -101,7 +101,7 @@ typedef struct {
   int nseismo_types;
   int seismo_types[3];
   int chunk_size;   /* storage chunk size of the numbering (chunk_config.hpp:12-28); 1 = CPU no-SIMD */
-  int scatter;      /* weak-form scatter: SFX_SCATTER_COLORED, _ATOMIC or _GATHER (3-D) */
+  int scatter;      /* weak-form scatter: SFX_SCATTER_COLORED, _ATOMIC, _GATHER or _BLOCK (3-D) */
 } sfx_run_desc;
 
 /* Scatter of element forces into the global acceleration.  COLORED groups the
@@ -116,6 +116,12 @@ typedef struct {
  * point, the contributions of its elements in a fixed order (deterministic,
  * no atomics, the acceleration is never stored between the two). */
 #define SFX_SCATTER_GATHER 2
+/* BLOCK (3-D structured meshes with even nx, ny, nz): one workgroup computes
+ * 2 x 2 x 2 elements, sums their forces in on-chip memory in a fixed order and
+ * updates the points inside the block directly; only the forces on block
+ * faces pass through device memory to a second, small update kernel
+ * (deterministic, no atomics). */
+#define SFX_SCATTER_BLOCK 3
 
 /* ---- library ---------------------------------------------------------- */
 int sfx_version(void);
@@ -183,6 +189,33 @@ int sfx_get_seismograms(const sfx_engine *engine, float *out);
  * out (ncomp, nglob) on the per-medium field index. */
 int sfx_get_field(const sfx_engine *engine, int which, float *out);
 int sfx_set_field(sfx_engine *engine, int which, const float *in);
+
+/* ---- adjoint (2-D) ----------------------------------------------------------
+ * sfx_forward_record: the whole forward loop (as sfx_run(0, nstep)) that also
+ *   keeps what the combined run reads back: the local force of every point of
+ *   every Stacey element at every step (the forward branch that stores
+ *   boundary_values, compute_stiffness_interaction.tpp:251-255;
+ *   boundary_medium_container.hpp:26-82) and the final corrected wavefield
+ *   (the ForwardWavefield written by io/wavefield/writer.tpp).  Replaces a
+ *   forward run with the wavefield writer task.
+ * sfx_set_adjoint_sources: adjoint sources (src/source/adjoint_source.cpp:9-90)
+ *   at `positions` (x, z used) with external time functions
+ *   (src/source_time_function/external.cpp) stf (nadj, nstep, 2), in forward
+ *   time order.
+ * sfx_adjoint_run: time_marching<combined>::run (time_marching.tpp:126-247):
+ *   istep from nstep - 1 down to 0, the adjoint field forward in time, the
+ *   backward field with -dt (newmark.tpp:161-242) replaying the recorded
+ *   Stacey elements (compute_stiffness_interaction.tpp:119-136), and
+ *   frechet_kernels::compute_derivatives (compute_material_derivatives.tpp)
+ *   every step.  The seismogram buffer then holds the backward field's
+ *   seismograms, latest sample first.  Consumes the recorded forward state.
+ * sfx_get_kernels: kernels (6, nspec, ngll, ngll) in mesh element order, in the
+ *   order rho, mu, kappa, rhop, alpha, beta (point::kernels,
+ *   core/specfem/point/kernels.hpp:55-332; frechet_derivative.hpp:14-138). */
+int sfx_forward_record(sfx_engine *engine);
+int sfx_set_adjoint_sources(sfx_engine *engine, int nadj, const sfx_receiver *positions, const float *stf);
+int sfx_adjoint_run(sfx_engine *engine);
+int sfx_get_kernels(const sfx_engine *engine, float *out);
 
 /* ---- measurement -------------------------------------------------------------
  * Average device time (ms) of each kernel family over the last sfx_run, taken

@@ -50,6 +50,13 @@ def load():
     lib.orc2d_forward.argtypes = [vp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_double, C.c_int, C.c_int, _ip,
                                   _dp, C.POINTER(C.c_double), C.c_int, C.c_void_p]
     lib.orc2d_apply_stiffness.argtypes = [vp, _dp, _dp, _dp]
+    lib.orc2d_nstacey.argtypes = [vp]
+    lib.orc2d_set_properties.argtypes = [vp, _dp, _dp, _dp]
+    lib.orc2d_properties.argtypes = [vp, _dp, _dp, _dp, _dp]
+    lib.orc2d_forward_record.argtypes = [vp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_double, C.c_int, C.c_int,
+                                         _ip, _dp, C.POINTER(C.c_double), C.c_int, _dp, _dp]
+    lib.orc2d_combined.argtypes = [vp, C.c_int, _dp, C.c_int, _dp, _dp, C.c_int, _dp, C.c_int, C.c_double,
+                                   C.c_int, C.c_int, _ip, _dp, _dp, _dp, _dp, C.c_int]
     lib.orc3d_last_error.restype = C.c_char_p
     lib.orc3d_create.argtypes = [C.c_int] + [C.c_int] * 3 + [C.c_double] * 6 + [_ip, _dp, _dp, _dp, _ip, C.c_double,
                                                                                  C.POINTER(vp)]
@@ -137,17 +144,77 @@ class Oracle2D:
         _check(load().orc2d_locate(self.h, x, z, C.byref(i), C.byref(xi), C.byref(g)))
         return i.value, xi.value, g.value
 
-    def forward(self, sources, receivers, nstep: int, dt: float, nsb: int = 1,
-                types=("displacement",), nthreads: int = 1, return_fields: bool = False):
-        """Run the forward loop; returns (seismograms (nsig, ntypes, nrec, 2), t0[, fields])."""
+    @staticmethod
+    def _src_rows(sources):
         src = np.zeros((len(sources), 10))
         for i, s in enumerate(sources):
             kind = 1 if s.get("type", "force") == "moment_tensor" else 0
             src[i] = [kind, s["x"], s["z"], s.get("angle", 0.0), s.get("mxx", 0.0), s.get("mzz", 0.0),
                       s.get("mxz", 0.0), s["f0"], s.get("tshift", 0.0), s["factor"]]
+        return src
+
+    @staticmethod
+    def _rec_rows(receivers):
         rec = np.zeros((len(receivers), 3))
         for i, r in enumerate(receivers):
             rec[i] = [r["x"], r["z"], r.get("angle", 0.0)]
+        return rec
+
+    def properties(self):
+        """(rho, kappa, mu, w_i w_j J), each (nspec, 5, 5) in mesh element order."""
+        out = [np.zeros((self.nspec, 5, 5)) for _ in range(4)]
+        load().orc2d_properties(self.h, *out)
+        return tuple(out)
+
+    def set_properties(self, rho, kappa, mu):
+        """Replace the per-point properties ((nspec, 5, 5), mesh order) and rebuild the mass matrix."""
+        c = [np.ascontiguousarray(x, dtype=np.float64) for x in (rho, kappa, mu)]
+        load().orc2d_set_properties(self.h, *c)
+
+    @property
+    def nstacey(self) -> int:
+        return load().orc2d_nstacey(self.h)
+
+    def forward_record(self, sources, receivers, nstep: int, dt: float, nsb: int = 1,
+                       types=("displacement",), nthreads: int = 1):
+        """Forward run that keeps what a combined run needs: returns
+        (seismograms, t0, final fields (3, 2, nglob), Stacey-element record
+        (nstep, nstacey, 5, 5, 2))."""
+        src, rec = self._src_rows(sources), self._rec_rows(receivers)
+        tcodes = np.array([SEISMO_TYPES[t] for t in types], dtype=np.int32)
+        nsig = -(-nstep // nsb)
+        seis = np.zeros((nsig, len(types), len(receivers), 2))
+        t0 = C.c_double()
+        fields = np.zeros((3, 2, self.nglob))
+        record = np.zeros((nstep, self.nstacey, 5, 5, 2))
+        _check(load().orc2d_forward_record(self.h, len(sources), src, len(receivers), rec, nstep, dt, nsb,
+                                           len(types), tcodes, seis, C.byref(t0), nthreads, fields, record))
+        return seis, t0.value, fields, record
+
+    def combined(self, sources, adjoint_positions, adjoint_stf, receivers, nstep: int, dt: float, fields, record,
+                 nsb: int = 1, types=("displacement",), nthreads: int = 1):
+        """Combined adjoint + backward run; adjoint_stf (nadj, nstep, 2) in
+        forward time order.  Returns (kernels (6, nspec, 5, 5) mesh order: rho,
+        mu, kappa, rhop, alpha, beta; backward seismograms, latest first)."""
+        src, rec = self._src_rows(sources), self._rec_rows(receivers)
+        adj = self._rec_rows(adjoint_positions)
+        stf = np.ascontiguousarray(adjoint_stf, dtype=np.float64)
+        assert stf.shape == (len(adjoint_positions), nstep, 2)
+        tcodes = np.array([SEISMO_TYPES[t] for t in types], dtype=np.int32)
+        nsig = -(-nstep // nsb)
+        seis = np.zeros((nsig, len(types), len(receivers), 2))
+        nspec = self.nspec
+        kernels = np.zeros((6, nspec, 5, 5))
+        _check(load().orc2d_combined(self.h, len(sources), src, len(adjoint_positions), adj, stf, len(receivers),
+                                     rec, nstep, dt, nsb, len(types), tcodes,
+                                     np.ascontiguousarray(fields, dtype=np.float64),
+                                     np.ascontiguousarray(record, dtype=np.float64), kernels, seis, nthreads))
+        return kernels, seis
+
+    def forward(self, sources, receivers, nstep: int, dt: float, nsb: int = 1,
+                types=("displacement",), nthreads: int = 1, return_fields: bool = False):
+        """Run the forward loop; returns (seismograms (nsig, ntypes, nrec, 2), t0[, fields])."""
+        src, rec = self._src_rows(sources), self._rec_rows(receivers)
         tcodes = np.array([SEISMO_TYPES[t] for t in types], dtype=np.int32)
         nsig = nstep // nsb if nstep % nsb == 0 else nstep // nsb + 1
         seis = np.zeros((nsig, len(types), len(receivers), 2))

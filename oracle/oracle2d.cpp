@@ -251,7 +251,10 @@ template <class T> struct Assembly {
   std::vector<int> bnd_tag;          // (ispec, iz, ix), 0 for non-Stacey elements
   std::vector<T> bnd_nx, bnd_nz, bnd_w;
   std::vector<T> mass_inverse;       // (comp, nglob) LayoutLeft on medium index
+  double dt = 0;                     // time step of the Stacey mass term
 };
+
+template <class T> void compute_mass(Assembly<T> &a);
 
 // src/compute/compute_mesh.cpp:19-43 (note: evaluated on the *sorted* points,
 // and numeric_limits::min() is the smallest positive value, both as in the
@@ -523,9 +526,18 @@ template <class T> Assembly<T> *build_assembly(const MeshIn &m, int chunk, doubl
     }
   }
 
-  // Mass matrix (include/kokkos_kernels/impl/compute_mass_matrix.tpp:80-114,
-  // include/medium/dim2/elastic/isotropic/mass_matrix.tpp:5-16, Stacey term
-  // stacey.tpp:641-677), then inversion (invert_mass_matrix.tpp:40-47).
+  a->dt = dt;
+  compute_mass(*a);
+  return a;
+}
+
+// Mass matrix (include/kokkos_kernels/impl/compute_mass_matrix.tpp:80-114,
+// include/medium/dim2/elastic/isotropic/mass_matrix.tpp:5-16, Stacey term
+// stacey.tpp:641-677), then inversion (invert_mass_matrix.tpp:40-47).
+template <class T> void compute_mass(Assembly<T> &A) {
+  Assembly<T> *a = &A;
+  const int nspec = a->nspec;
+  const double dt = a->dt;
   const int nglob = a->nglob;
   std::vector<T> mass(2 * static_cast<size_t>(nglob), 0);
   const T velocity_factor = static_cast<T>(-1.0) * static_cast<T>(dt) * static_cast<T>(0.5);
@@ -546,7 +558,6 @@ template <class T> Assembly<T> *build_assembly(const MeshIn &m, int chunk, doubl
       }
   a->mass_inverse.resize(2 * static_cast<size_t>(nglob));
   for (size_t i = 0; i < mass.size(); i++) a->mass_inverse[i] = static_cast<T>(1.0) / mass[i];
-  return a;
 }
 
 // ---------------------------------------------------------------------------
@@ -657,11 +668,6 @@ struct ReceiverIn {
   double x, z, angle;
 };
 
-template <class T> struct Run {
-  const Assembly<T> *a;
-  std::vector<T> u, v, acc;  // (comp, nglob) on the per-medium index
-};
-
 // Ricker, src/source_time_function/ricker.cpp:39-50 and utilities.cpp:27-32.
 template <class T> T ricker(T t, T f0, T factor) {
   const T pi = static_cast<T>(M_PI);
@@ -737,21 +743,29 @@ void element_force(const Assembly<T> &a, int e, const T *ux, const T *uz, const 
     }
 }
 
-// Forward simulation.  Seismogram output layout: (isig, itype, irec, 2) as
-// the reference's (max_sig_step, ntypes, nrec, 2) array before rotation
-// (include/kokkos_kernels/impl/compute_seismogram.tpp:103-135); types are
-// 0 displacement, 1 velocity, 2 acceleration.
-template <class T>
-void run_forward(const Assembly<T> &a, int nsrc, const SourceIn *src, int nrec, const ReceiverIn *rec, int nstep,
-                 double dt_in, int nsb, int ntypes, const int *types, double *seis_out, double *t0_out,
-                 int nthreads, T *final_fields /* optional (3, 2, nglob) */) {
-  const int nglob = a.nglob;
-  const T dt = static_cast<T>(dt_in);
-  const T deltat = dt, deltatover2 = dt / 2.0, deltasquareover2 = dt * dt / 2.0;
+// Point forcing of one wavefield: source arrays (ncomp, 5, 5), time functions
+// stf(istep, comp) (include/compute/sources/source_medium.tpp:19-38) and the
+// application order (element type, none then Stacey, then source order;
+// src/compute/compute_sources.cpp:232-270).
+template <class T> struct Forcing {
+  int nstep = 0;
+  std::vector<int> ispec;
+  std::vector<T> arr;  // (nsrc, 2, 5, 5)
+  std::vector<T> stf;  // (nsrc, nstep, 2)
+  std::vector<int> order;
+  void finish_order(const Assembly<T> &a) {
+    order.clear();
+    for (int pass = 0; pass < 2; pass++)
+      for (size_t s = 0; s < ispec.size(); s++)
+        if (a.is_stacey[ispec[s]] == pass) order.push_back(static_cast<int>(s));
+  }
+};
 
-  // Sources: t0 = min over sources of (-1.2/f0 + tshift), then each source's
-  // tshift becomes its t0 - min_t0 (src/io/sources.cpp:128-158,
-  // ricker.cpp:15-20, ricker.hpp:49).
+// Forward sources.  t0 = min over sources of (-1.2/f0 + tshift), then each
+// source's tshift becomes its t0 - min_t0 (src/io/sources.cpp:128-158,
+// ricker.cpp:15-20, ricker.hpp:49).
+template <class T>
+Forcing<T> make_sources(const Assembly<T> &a, int nsrc, const SourceIn *src, int nstep, T dt, T *t0_out) {
   T min_t0 = std::numeric_limits<T>::max();
   std::vector<T> src_t0(nsrc);
   for (int s = 0; s < nsrc; s++) {
@@ -761,15 +775,15 @@ void run_forward(const Assembly<T> &a, int nsrc, const SourceIn *src, int nrec, 
   }
   const T t0 = min_t0;
   if (t0_out) *t0_out = t0;
-  struct Src {
-    int ispec;
-    T arr[2][NGLL][NGLL];
-    std::vector<T> stf;
-  };
-  std::vector<Src> S(nsrc);
+  Forcing<T> f;
+  f.nstep = nstep;
+  f.ispec.resize(nsrc);
+  f.arr.assign(static_cast<size_t>(nsrc) * 2 * NGLL * NGLL, 0);
+  f.stf.assign(static_cast<size_t>(nsrc) * nstep * 2, 0);
   for (int s = 0; s < nsrc; s++) {
     auto loc = locate_point<T>(a, static_cast<T>(src[s].x), static_cast<T>(src[s].z));
-    S[s].ispec = loc.ispec;
+    f.ispec[s] = loc.ispec;
+    T *arr = &f.arr[static_cast<size_t>(s) * 2 * NGLL * NGLL];
     T hx[NGLL], hpx[NGLL], hg[NGLL], hpg[NGLL];
     lagrange_interpolants<T>(loc.xi, a.q.xi, hx, hpx);
     lagrange_interpolants<T>(loc.gamma, a.q.xi, hg, hpg);
@@ -780,8 +794,8 @@ void run_forward(const Assembly<T> &a, int nsrc, const SourceIn *src, int nrec, 
       for (int iz = 0; iz < NGLL; iz++)
         for (int ix = 0; ix < NGLL; ix++) {
           const T hl = hx[ix] * hg[iz];
-          S[s].arr[0][iz][ix] = std::sin(pi / 180 * ang) * hl;
-          S[s].arr[1][iz][ix] = -1.0 * std::cos(pi / 180 * ang) * hl;
+          arr[iz * NGLL + ix] = std::sin(pi / 180 * ang) * hl;
+          arr[NGLL * NGLL + iz * NGLL + ix] = -1.0 * std::cos(pi / 180 * ang) * hl;
         }
     } else {
       // moment tensor source, src/source/moment_tensor_source.cpp:88-130:
@@ -804,137 +818,353 @@ void run_forward(const Assembly<T> &a, int nsrc, const SourceIn *src, int nrec, 
         for (int ix = 0; ix < NGLL; ix++) {
           const T dsx = (hpx[ix] * xixs) * hg[iz] + hx[ix] * (hpg[iz] * gammaxs);
           const T dsz = (hpx[ix] * xizs) * hg[iz] + hx[ix] * (hpg[iz] * gammazs);
-          S[s].arr[0][iz][ix] = Mxx * dsx + Mxz * dsz;
-          S[s].arr[1][iz][ix] = Mxz * dsx + Mzz * dsz;
+          arr[iz * NGLL + ix] = Mxx * dsx + Mxz * dsz;
+          arr[NGLL * NGLL + iz * NGLL + ix] = Mxz * dsx + Mzz * dsz;
         }
     }
-    S[s].stf.resize(nstep);
     const T tshift = src_t0[s] - min_t0;
-    for (int i = 0; i < nstep; i++)
-      S[s].stf[i] = ricker<T>(t0 + i * dt - tshift, static_cast<T>(src[s].f0), static_cast<T>(src[s].factor));
+    for (int i = 0; i < nstep; i++) {
+      const T v = ricker<T>(t0 + i * dt - tshift, static_cast<T>(src[s].f0), static_cast<T>(src[s].factor));
+      f.stf[(static_cast<size_t>(s) * nstep + i) * 2] = v;
+      f.stf[(static_cast<size_t>(s) * nstep + i) * 2 + 1] = v;
+    }
   }
-  // Source application order: element type (none, then Stacey), then source
-  // order (src/compute/compute_sources.cpp:232-270).
-  std::vector<int> src_order;
-  for (int pass = 0; pass < 2; pass++)
-    for (int s = 0; s < nsrc; s++)
-      if (a.is_stacey[S[s].ispec] == pass) src_order.push_back(s);
+  f.finish_order(a);
+  return f;
+}
 
-  struct Rec {
-    int ispec;
-    T h[NGLL][NGLL];
-  };
-  std::vector<Rec> R(nrec);
+// Adjoint sources (src/source/adjoint_source.cpp:9-90): the Lagrange product
+// at the location for both components, with an external time function per
+// component (src/source_time_function/external.cpp) given as stf
+// (nadj, nstep, 2) in forward time order.
+template <class T>
+Forcing<T> make_adjoint_sources(const Assembly<T> &a, int nadj, const ReceiverIn *pos, int nstep, const double *stf) {
+  Forcing<T> f;
+  f.nstep = nstep;
+  f.ispec.resize(nadj);
+  f.arr.assign(static_cast<size_t>(nadj) * 2 * NGLL * NGLL, 0);
+  f.stf.resize(static_cast<size_t>(nadj) * nstep * 2);
+  for (int s = 0; s < nadj; s++) {
+    auto loc = locate_point<T>(a, static_cast<T>(pos[s].x), static_cast<T>(pos[s].z));
+    f.ispec[s] = loc.ispec;
+    T hx[NGLL], hg[NGLL];
+    lagrange_interpolants<T>(loc.xi, a.q.xi, hx, nullptr);
+    lagrange_interpolants<T>(loc.gamma, a.q.xi, hg, nullptr);
+    T *arr = &f.arr[static_cast<size_t>(s) * 2 * NGLL * NGLL];
+    for (int iz = 0; iz < NGLL; iz++)
+      for (int ix = 0; ix < NGLL; ix++) {
+        const T hl = hx[ix] * hg[iz];
+        arr[iz * NGLL + ix] = hl;
+        arr[NGLL * NGLL + iz * NGLL + ix] = hl;
+      }
+  }
+  for (size_t i = 0; i < f.stf.size(); i++) f.stf[i] = static_cast<T>(stf[i]);
+  f.finish_order(a);
+  return f;
+}
+
+// Receivers: element and Lagrange products (src/compute/compute_receivers.cpp:59-83).
+template <class T> struct Receivers {
+  std::vector<int> ispec;
+  std::vector<T> h;  // (nrec, 5, 5)
+};
+template <class T> Receivers<T> make_receivers(const Assembly<T> &a, int nrec, const ReceiverIn *rec) {
+  Receivers<T> R;
+  R.ispec.resize(nrec);
+  R.h.resize(static_cast<size_t>(nrec) * NGLL * NGLL);
   for (int r = 0; r < nrec; r++) {
     auto loc = locate_point<T>(a, static_cast<T>(rec[r].x), static_cast<T>(rec[r].z));
-    R[r].ispec = loc.ispec;
+    R.ispec[r] = loc.ispec;
     T hx[NGLL], hg[NGLL];
     lagrange_interpolants<T>(loc.xi, a.q.xi, hx, nullptr);
     lagrange_interpolants<T>(loc.gamma, a.q.xi, hg, nullptr);
     for (int iz = 0; iz < NGLL; iz++)
-      for (int ix = 0; ix < NGLL; ix++) R[r].h[iz][ix] = hx[ix] * hg[iz];
+      for (int ix = 0; ix < NGLL; ix++) R.h[(static_cast<size_t>(r) * NGLL + iz) * NGLL + ix] = hx[ix] * hg[iz];
   }
+  return R;
+}
 
-  std::vector<T> u(2 * static_cast<size_t>(nglob), 0), v(2 * static_cast<size_t>(nglob), 0),
-      acc(2 * static_cast<size_t>(nglob), 0);
-  T *ux = u.data(), *uz = u.data() + nglob, *vx = v.data(), *vz = v.data() + nglob;
-  T *ax = acc.data(), *az = acc.data() + nglob;
+// One simulation field (include/compute/fields/impl/field_impl.hpp:30-38):
+// (comp, nglob) on the per-medium index.
+template <class T> struct Field {
+  int nglob = 0;
+  std::vector<T> u, v, acc;
+  explicit Field(int n) : nglob(n), u(2 * static_cast<size_t>(n), 0), v(u), acc(u) {}
+};
+
+// Newmark predictor (newmark.tpp:60-116).  The backward field uses
+// (-dt, -dt/2, dt^2/2) (newmark.tpp:225-242).
+template <class T> void predictor(Field<T> &f, T deltat, T deltatover2, T deltasquareover2, bool serial) {
+  const size_t n = 2 * static_cast<size_t>(f.nglob);
+#pragma omp parallel for if (!serial) schedule(static)
+  for (size_t i = 0; i < n; i++) {
+    T add_u = 0;
+    add_u += deltat * f.v[i] + deltasquareover2 * f.acc[i];
+    T add_v = 0;
+    add_v += deltatover2 * f.acc[i];
+    f.u[i] += add_u;
+    f.v[i] += add_v;
+    f.acc[i] = 0;
+  }
+}
+
+// Mass division (divide_mass_matrix.tpp:42-49) and corrector (newmark.tpp:11-56;
+// -dt/2 for the backward field, :204-222).
+template <class T> void divide_correct(const Assembly<T> &a, Field<T> &f, T deltatover2, bool serial) {
+  const size_t n = 2 * static_cast<size_t>(f.nglob);
   const T *minv = a.mass_inverse.data();
+#pragma omp parallel for if (!serial) schedule(static)
+  for (size_t i = 0; i < n; i++) {
+    f.acc[i] = f.acc[i] * minv[i];
+    T add_v = 0;
+    add_v += deltatover2 * f.acc[i];
+    f.v[i] += add_v;
+  }
+}
 
+// Stacey-element record: the whole local contribution (stiffness + traction)
+// of every point of every Stacey element, per step
+// (compute_stiffness_interaction.tpp:251-255, boundary_medium_container.hpp:26-82),
+// stored here as (nstep, Stacey element in compute order, 5, 5, 2).
+enum class StaceyMode { normal, record, replay };
+
+// Sources then stiffness (domain_kernels.hpp:69-126), elements in compute
+// order (none, then Stacey).  record: also store the Stacey elements' local
+// contributions at rec_step; replay: add rec_step's values for the Stacey
+// elements instead of computing them (the backward field,
+// compute_stiffness_interaction.tpp:119-136).
+template <class T>
+void update_forces(const Assembly<T> &a, Field<T> &f, const Forcing<T> &F, int istep, StaceyMode mode, T *rec_step,
+                   bool serial) {
+  const int nglob = f.nglob;
+  T *ax = f.acc.data(), *az = f.acc.data() + nglob;
+  const T *ux = f.u.data(), *uz = f.u.data() + nglob, *vx = f.v.data(), *vz = f.v.data() + nglob;
+  for (int s : F.order) {
+    const T *arr = &F.arr[static_cast<size_t>(s) * 2 * NGLL * NGLL];
+    const T sx = F.stf[(static_cast<size_t>(s) * F.nstep + istep) * 2];
+    const T sz = F.stf[(static_cast<size_t>(s) * F.nstep + istep) * 2 + 1];
+    for (int iz = 0; iz < NGLL; iz++)
+      for (int ix = 0; ix < NGLL; ix++) {
+        const int g = a.medium_index[a.ibool[pidx<T>(F.ispec[s], iz, ix)]];
+        ax[g] += sx * arr[iz * NGLL + ix];
+        az[g] += sz * arr[NGLL * NGLL + iz * NGLL + ix];
+      }
+  }
+  auto element = [&](int e, T out[NGLL][NGLL][2]) {
+    const bool st = a.is_stacey[e];
+    T *rec = st && mode != StaceyMode::normal ? rec_step + static_cast<size_t>(e - a.nspec_none) * NGLL * NGLL * 2
+                                               : nullptr;
+    if (rec && mode == StaceyMode::replay) {
+      for (int p = 0; p < NGLL * NGLL; p++) {
+        out[p / NGLL][p % NGLL][0] = rec[2 * p];
+        out[p / NGLL][p % NGLL][1] = rec[2 * p + 1];
+      }
+      return;
+    }
+    element_force<T>(a, e, ux, uz, vx, vz, out);
+    if (rec)
+      for (int p = 0; p < NGLL * NGLL; p++) {
+        rec[2 * p] = out[p / NGLL][p % NGLL][0];
+        rec[2 * p + 1] = out[p / NGLL][p % NGLL][1];
+      }
+  };
+  if (serial) {
+    T out[NGLL][NGLL][2];
+    for (int e = 0; e < a.nspec; e++) {
+      element(e, out);
+      for (int iz = 0; iz < NGLL; iz++)
+        for (int ix = 0; ix < NGLL; ix++) {
+          const int g = a.medium_index[a.ibool[pidx<T>(e, iz, ix)]];
+          ax[g] += out[iz][ix][0];
+          az[g] += out[iz][ix][1];
+        }
+    }
+  } else {
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < a.nspec; e++) {
+      T out[NGLL][NGLL][2];
+      element(e, out);
+      for (int iz = 0; iz < NGLL; iz++)
+        for (int ix = 0; ix < NGLL; ix++) {
+          const int g = a.medium_index[a.ibool[pidx<T>(e, iz, ix)]];
+#pragma omp atomic
+          ax[g] += out[iz][ix][0];
+#pragma omp atomic
+          az[g] += out[iz][ix][1];
+        }
+    }
+  }
+}
+
+// Seismograms (compute_seismogram.tpp:103-135; include/algorithms/interpolate.hpp:176-237).
+template <class T>
+void seismograms(const Assembly<T> &a, const Field<T> &f, const Receivers<T> &R, int ntypes, const int *types,
+                 int isig, double *seis_out) {
+  const int nglob = f.nglob, nrec = static_cast<int>(R.ispec.size());
+  for (int it = 0; it < ntypes; it++) {
+    const std::vector<T> &fld = types[it] == 0 ? f.u : (types[it] == 1 ? f.v : f.acc);
+    const T *fx = fld.data(), *fz = fld.data() + nglob;
+    for (int r = 0; r < nrec; r++) {
+      T s0 = 0, s1 = 0;
+      for (int iz = 0; iz < NGLL; iz++)
+        for (int ix = 0; ix < NGLL; ix++) {
+          const int g = a.medium_index[a.ibool[pidx<T>(R.ispec[r], iz, ix)]];
+          const T h = R.h[(static_cast<size_t>(r) * NGLL + iz) * NGLL + ix];
+          s0 += h * fx[g];
+          s1 += h * fz[g];
+        }
+      const size_t o = ((static_cast<size_t>(isig) * ntypes + it) * nrec + r) * 2;
+      seis_out[o] = s0;
+      seis_out[o + 1] = s1;
+    }
+  }
+}
+
+// Forward simulation (time_marching.tpp:9-124).  Seismogram output layout:
+// (isig, itype, irec, 2) as the reference's (max_sig_step, ntypes, nrec, 2)
+// array before rotation; types 0 displacement, 1 velocity, 2 acceleration.
+// record: optional Stacey-element record (nstep, nstacey, 5, 5, 2).
+template <class T>
+void run_forward(const Assembly<T> &a, int nsrc, const SourceIn *src, int nrec, const ReceiverIn *rec, int nstep,
+                 double dt_in, int nsb, int ntypes, const int *types, double *seis_out, double *t0_out,
+                 int nthreads, T *final_fields /* optional (3, 2, nglob) */, T *record = nullptr) {
+  const T dt = static_cast<T>(dt_in);
+  const T deltat = dt, deltatover2 = dt / 2.0, deltasquareover2 = dt * dt / 2.0;
+  T t0 = 0;
+  const Forcing<T> S = make_sources<T>(a, nsrc, src, nstep, dt, &t0);
+  if (t0_out) *t0_out = t0;
+  const Receivers<T> R = make_receivers<T>(a, nrec, rec);
+  Field<T> f(a.nglob);
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #endif
   const bool serial = nthreads <= 1;
+  const size_t rec_stride = static_cast<size_t>(a.nspec - a.nspec_none) * NGLL * NGLL * 2;
   int isig = 0;
   for (int istep = 0; istep < nstep; istep++) {
-    // predictor (newmark.tpp:60-116)
-#pragma omp parallel for if (!serial) schedule(static)
-    for (int g = 0; g < nglob; g++) {
-      for (int c = 0; c < 2; c++) {
-        const size_t i = static_cast<size_t>(c) * nglob + g;
-        T add_u = 0;
-        add_u += deltat * v[i] + deltasquareover2 * acc[i];
-        T add_v = 0;
-        add_v += deltatover2 * acc[i];
-        u[i] += add_u;
-        v[i] += add_v;
-        acc[i] = 0;
-      }
-    }
-    // sources (compute_source_interaction.tpp:80-95)
-    for (int s : src_order) {
-      const T sv = S[s].stf[istep];
-      for (int iz = 0; iz < NGLL; iz++)
-        for (int ix = 0; ix < NGLL; ix++) {
-          const int g = a.medium_index[a.ibool[pidx<T>(S[s].ispec, iz, ix)]];
-          ax[g] += sv * S[s].arr[0][iz][ix];
-          az[g] += sv * S[s].arr[1][iz][ix];
-        }
-    }
-    // stiffness: elements in compute order (none, then Stacey)
-    if (serial) {
-      T out[NGLL][NGLL][2];
-      for (int e = 0; e < a.nspec; e++) {
-        element_force<T>(a, e, ux, uz, vx, vz, out);
-        for (int iz = 0; iz < NGLL; iz++)
-          for (int ix = 0; ix < NGLL; ix++) {
-            const int g = a.medium_index[a.ibool[pidx<T>(e, iz, ix)]];
-            ax[g] += out[iz][ix][0];
-            az[g] += out[iz][ix][1];
-          }
-      }
-    } else {
-#pragma omp parallel for schedule(static)
-      for (int e = 0; e < a.nspec; e++) {
-        T out[NGLL][NGLL][2];
-        element_force<T>(a, e, ux, uz, vx, vz, out);
-        for (int iz = 0; iz < NGLL; iz++)
-          for (int ix = 0; ix < NGLL; ix++) {
-            const int g = a.medium_index[a.ibool[pidx<T>(e, iz, ix)]];
-#pragma omp atomic
-            ax[g] += out[iz][ix][0];
-#pragma omp atomic
-            az[g] += out[iz][ix][1];
-          }
-      }
-    }
-    // divide by mass (divide_mass_matrix.tpp:42-49), corrector (newmark.tpp:11-56)
-#pragma omp parallel for if (!serial) schedule(static)
-    for (int g = 0; g < nglob; g++) {
-      for (int c = 0; c < 2; c++) {
-        const size_t i = static_cast<size_t>(c) * nglob + g;
-        acc[i] = acc[i] * minv[i];
-        T add_v = 0;
-        add_v += deltatover2 * acc[i];
-        v[i] += add_v;
-      }
-    }
-    // seismograms (time_marching.tpp:75-78; compute_seismogram.tpp:103-135;
-    // include/algorithms/interpolate.hpp:176-237)
-    if (istep % nsb == 0) {
-      for (int it = 0; it < ntypes; it++) {
-        const T *fx = types[it] == 0 ? ux : (types[it] == 1 ? vx : ax);
-        const T *fz = types[it] == 0 ? uz : (types[it] == 1 ? vz : az);
-        for (int r = 0; r < nrec; r++) {
-          T s0 = 0, s1 = 0;
-          for (int iz = 0; iz < NGLL; iz++)
-            for (int ix = 0; ix < NGLL; ix++) {
-              const int g = a.medium_index[a.ibool[pidx<T>(R[r].ispec, iz, ix)]];
-              s0 += R[r].h[iz][ix] * fx[g];
-              s1 += R[r].h[iz][ix] * fz[g];
-            }
-          const size_t o = ((static_cast<size_t>(isig) * ntypes + it) * nrec + r) * 2;
-          seis_out[o] = s0;
-          seis_out[o + 1] = s1;
-        }
-      }
-      isig++;
-    }
+    predictor<T>(f, deltat, deltatover2, deltasquareover2, serial);
+    update_forces<T>(a, f, S, istep, record ? StaceyMode::record : StaceyMode::normal,
+                     record ? record + istep * rec_stride : nullptr, serial);
+    divide_correct<T>(a, f, deltatover2, serial);
+    if (istep % nsb == 0) seismograms<T>(a, f, R, ntypes, types, isig++, seis_out);
   }
   if (final_fields) {
-    std::copy(u.begin(), u.end(), final_fields);
-    std::copy(v.begin(), v.end(), final_fields + 2 * static_cast<size_t>(nglob));
-    std::copy(acc.begin(), acc.end(), final_fields + 4 * static_cast<size_t>(nglob));
+    std::copy(f.u.begin(), f.u.end(), final_fields);
+    std::copy(f.v.begin(), f.v.end(), final_fields + 2 * static_cast<size_t>(a.nglob));
+    std::copy(f.acc.begin(), f.acc.end(), final_fields + 4 * static_cast<size_t>(a.nglob));
+  }
+}
+
+// Fréchet derivatives of one step (compute_material_derivatives.tpp:17-154,
+// include/medium/dim2/elastic/isotropic/frechet_derivative.hpp:14-138):
+// kernels (6, nspec, 5, 5) in compute order, rho, mu, kappa, rhop, alpha,
+// beta, accumulated.
+template <class T>
+void frechet_step(const Assembly<T> &a, const Field<T> &adj, const Field<T> &bwd, T dt, T *K, bool serial) {
+  const int nglob = a.nglob;
+  const size_t npts = static_cast<size_t>(a.nspec) * NGLL * NGLL;
+  const auto &q = a.q;
+#pragma omp parallel for if (!serial) schedule(static)
+  for (int e = 0; e < a.nspec; e++) {
+    T ad[2][NGLL][NGLL], bw[2][NGLL][NGLL];
+    for (int iz = 0; iz < NGLL; iz++)
+      for (int ix = 0; ix < NGLL; ix++) {
+        const int g = a.medium_index[a.ibool[pidx<T>(e, iz, ix)]];
+        for (int c = 0; c < 2; c++) {
+          ad[c][iz][ix] = adj.u[static_cast<size_t>(c) * nglob + g];
+          bw[c][iz][ix] = bwd.u[static_cast<size_t>(c) * nglob + g];
+        }
+      }
+    for (int iz = 0; iz < NGLL; iz++)
+      for (int ix = 0; ix < NGLL; ix++) {
+        const size_t p = pidx<T>(e, iz, ix);
+        const T xix = a.xix[p], xiz = a.xiz[p], gammax = a.gammax[p], gammaz = a.gammaz[p];
+        T df[2][2], dg[2][2];  // du(c, j) = d u_c / d x_j (gradient.hpp:81-103)
+        for (int c = 0; c < 2; c++) {
+          T fxi = 0, fga = 0, gxi = 0, gga = 0;
+          for (int l = 0; l < NGLL; l++) {
+            fxi += q.hprime[ix][l] * ad[c][iz][l];
+            fga += q.hprime[iz][l] * ad[c][l][ix];
+            gxi += q.hprime[ix][l] * bw[c][iz][l];
+            gga += q.hprime[iz][l] * bw[c][l][ix];
+          }
+          df[c][0] = xix * fxi + gammax * fga;
+          df[c][1] = xiz * fxi + gammaz * fga;
+          dg[c][0] = xix * gxi + gammax * gga;
+          dg[c][1] = xiz * gxi + gammaz * gga;
+        }
+        const T mu = a.mu[p], rho = a.rho[p];
+        const T lp2m = a.kappa[p] + static_cast<T>(4.0 / 3.0) * mu;
+        const T kappa = lp2m - mu;
+        const T ad_dsxx = df[0][0];
+        const T ad_dsxz = static_cast<T>(0.5) * (df[1][0] + df[0][1]);
+        const T ad_dszz = df[1][1];
+        const T b_dsxx = dg[0][0];
+        const T b_dsxz = static_cast<T>(0.5) * (dg[1][0] + dg[0][1]);
+        const T b_dszz = dg[1][1];
+        T kappa_kl = (ad_dsxx + ad_dszz) * (b_dsxx + b_dszz);
+        T mu_kl = (ad_dsxx * b_dsxx + ad_dszz * b_dszz + static_cast<T>(2.0) * ad_dsxz * b_dsxz -
+                   static_cast<T>(1.0 / 3.0) * kappa_kl);
+        const int g = a.medium_index[a.ibool[p]];
+        T rho_kl = adj.acc[g] * bwd.u[g] + adj.acc[static_cast<size_t>(nglob) + g] * bwd.u[static_cast<size_t>(nglob) + g];
+        kappa_kl = static_cast<T>(-1.0) * kappa * dt * kappa_kl;
+        mu_kl = static_cast<T>(-2.0) * mu * dt * mu_kl;
+        rho_kl = static_cast<T>(-1.0) * rho * dt * rho_kl;
+        const T rhop_kl = rho_kl + kappa_kl + mu_kl;
+        const T beta_kl = static_cast<T>(2.0) * (mu_kl - static_cast<T>(4.0 / 3.0) * mu / kappa * kappa_kl);
+        const T alpha_kl = static_cast<T>(2.0) * (static_cast<T>(1.0) + static_cast<T>(4.0 / 3.0) * mu / kappa) * kappa_kl;
+        K[0 * npts + p] += rho_kl;
+        K[1 * npts + p] += mu_kl;
+        K[2 * npts + p] += kappa_kl;
+        K[3 * npts + p] += rhop_kl;
+        K[4 * npts + p] += alpha_kl;
+        K[5 * npts + p] += beta_kl;
+      }
+  }
+}
+
+// Combined adjoint + backward loop (time_marching.tpp:126-247): istep from
+// nstep - 1 down to 0; the adjoint field steps forward in time with the
+// adjoint sources, the backward field steps with -dt from the forward run's
+// final state (copied in after the first backward step), replaying the
+// recorded Stacey-element contributions.  Kernels accumulate every step;
+// backward seismograms are sampled at istep % nsb == 0 (sample 0 is the
+// latest time).
+template <class T>
+void run_combined(const Assembly<T> &a, int nsrc, const SourceIn *src, int nadj, const ReceiverIn *adj_pos,
+                  const double *adj_stf, int nrec, const ReceiverIn *rec, int nstep, double dt_in, int nsb,
+                  int ntypes, const int *types, const T *final_fields, const T *record, T *K, double *seis_out,
+                  int nthreads) {
+  const T dt = static_cast<T>(dt_in);
+  const T deltat = dt, deltatover2 = dt / 2.0, deltasquareover2 = dt * dt / 2.0;
+  const Forcing<T> S = make_sources<T>(a, nsrc, src, nstep, dt, nullptr);
+  const Forcing<T> A = make_adjoint_sources<T>(a, nadj, adj_pos, nstep, adj_stf);
+  const Receivers<T> R = make_receivers<T>(a, nrec, rec);
+  Field<T> adj(a.nglob), bwd(a.nglob);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+  const bool serial = nthreads <= 1;
+  const size_t rec_stride = static_cast<size_t>(a.nspec - a.nspec_none) * NGLL * NGLL * 2;
+  std::vector<T> scratch;
+  int isig = 0;
+  for (int istep = nstep - 1; istep >= 0; istep--) {
+    predictor<T>(adj, deltat, deltatover2, deltasquareover2, serial);
+    update_forces<T>(a, adj, A, istep, StaceyMode::normal, nullptr, serial);
+    divide_correct<T>(a, adj, deltatover2, serial);
+
+    predictor<T>(bwd, -deltat, -deltatover2, deltasquareover2, serial);
+    scratch.assign(record + istep * rec_stride, record + (istep + 1) * rec_stride);
+    update_forces<T>(a, bwd, S, istep, StaceyMode::replay, scratch.data(), serial);
+    divide_correct<T>(a, bwd, -deltatover2, serial);
+    if (istep == nstep - 1) {
+      const size_t n = 2 * static_cast<size_t>(a.nglob);
+      std::copy(final_fields, final_fields + n, bwd.u.begin());
+      std::copy(final_fields + n, final_fields + 2 * n, bwd.v.begin());
+      std::copy(final_fields + 2 * n, final_fields + 3 * n, bwd.acc.begin());
+    }
+    frechet_step<T>(a, adj, bwd, dt, K, serial);
+    if (istep % nsb == 0) seismograms<T>(a, bwd, R, ntypes, types, isig++, seis_out);
   }
 }
 
@@ -1080,18 +1310,33 @@ int orc2d_locate(void *h, double x, double z, int *ispec, double *xi, double *ga
   });
 }
 
+}  // extern "C"
+
+namespace {
+std::vector<SourceIn> sources_in(int nsrc, const double *src) {
+  std::vector<SourceIn> S(nsrc);
+  for (int s = 0; s < nsrc; s++) {
+    const double *r = src + 10 * s;
+    S[s] = SourceIn{static_cast<int>(r[0]), r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9]};
+  }
+  return S;
+}
+std::vector<ReceiverIn> receivers_in(int nrec, const double *rec) {
+  std::vector<ReceiverIn> R(nrec);
+  for (int i = 0; i < nrec; i++) R[i] = ReceiverIn{rec[3 * i], rec[3 * i + 1], rec[3 * i + 2]};
+  return R;
+}
+}  // namespace
+
+extern "C" {
+
 // sources: (nsrc, 10) rows [type, x, z, angle, mxx, mzz, mxz, f0, tshift, factor]
 // receivers: (nrec, 3) rows [x, z, angle]
 int orc2d_forward(void *h, int nsrc, const double *src, int nrec, const double *rec, int nstep, double dt, int nsb,
                   int ntypes, const int *types, double *seis, double *t0, int nthreads, double *final_fields) {
   return guard([&] {
-    std::vector<SourceIn> S(nsrc);
-    for (int s = 0; s < nsrc; s++) {
-      const double *r = src + 10 * s;
-      S[s] = SourceIn{static_cast<int>(r[0]), r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9]};
-    }
-    std::vector<ReceiverIn> R(nrec);
-    for (int i = 0; i < nrec; i++) R[i] = ReceiverIn{rec[3 * i], rec[3 * i + 1], rec[3 * i + 2]};
+    auto S = sources_in(nsrc, src);
+    auto R = receivers_in(nrec, rec);
     DISPATCH(h, {
       auto &a = asm_of<T>(h);
       std::vector<T> ff;
@@ -1100,6 +1345,100 @@ int orc2d_forward(void *h, int nsrc, const double *src, int nrec, const double *
                      final_fields ? ff.data() : nullptr);
       if (final_fields)
         for (size_t i = 0; i < ff.size(); i++) final_fields[i] = ff[i];
+    });
+  });
+}
+
+// Replace the per-point properties (nspec, 5, 5) in mesh element order and
+// rebuild the mass matrix (finite-difference checks of the kernels).
+void orc2d_set_properties(void *h, const double *rho, const double *kappa, const double *mu) {
+  DISPATCH(h, {
+    auto &a = asm_of<T>(h);
+    for (int c = 0; c < a.nspec; c++)
+      for (int p = 0; p < NGLL * NGLL; p++) {
+        const size_t src = static_cast<size_t>(a.compute_to_mesh[c]) * NGLL * NGLL + p;
+        const size_t dst = static_cast<size_t>(c) * NGLL * NGLL + p;
+        a.rho[dst] = static_cast<T>(rho[src]);
+        a.kappa[dst] = static_cast<T>(kappa[src]);
+        a.mu[dst] = static_cast<T>(mu[src]);
+      }
+    compute_mass(a);
+  });
+}
+
+// Per-point properties and Jacobian weights w_i w_j J (nspec, 5, 5) in mesh
+// element order.
+void orc2d_properties(void *h, double *rho, double *kappa, double *mu, double *wj) {
+  DISPATCH(h, {
+    auto &a = asm_of<T>(h);
+    for (int c = 0; c < a.nspec; c++)
+      for (int iz = 0; iz < NGLL; iz++)
+        for (int ix = 0; ix < NGLL; ix++) {
+          const size_t dst = (static_cast<size_t>(a.compute_to_mesh[c]) * NGLL + iz) * NGLL + ix;
+          const size_t src = pidx<T>(c, iz, ix);
+          rho[dst] = a.rho[src];
+          kappa[dst] = a.kappa[src];
+          mu[dst] = a.mu[src];
+          wj[dst] = a.q.w[ix] * a.q.w[iz] * a.jac[src];
+        }
+  });
+}
+
+int orc2d_nstacey(void *h) {
+  int n = 0;
+  DISPATCH(h, n = asm_of<T>(h).nspec - asm_of<T>(h).nspec_none);
+  return n;
+}
+
+
+// Forward run that also returns the final fields (3, 2, nglob) and the
+// Stacey-element record (nstep, nstacey, 5, 5, 2) for a combined run.
+int orc2d_forward_record(void *h, int nsrc, const double *src, int nrec, const double *rec, int nstep, double dt,
+                         int nsb, int ntypes, const int *types, double *seis, double *t0, int nthreads,
+                         double *final_fields, double *record) {
+  return guard([&] {
+    auto S = sources_in(nsrc, src);
+    auto R = receivers_in(nrec, rec);
+    DISPATCH(h, {
+      auto &a = asm_of<T>(h);
+      if (!a.is_stacey.empty())
+        for (int e = 0; e < a.nspec; e++)
+          if ((e >= a.nspec_none) != (a.is_stacey[e] != 0))
+            throw std::logic_error("Stacey elements are not the last in compute order");
+      std::vector<T> ff(6 * static_cast<size_t>(a.nglob));
+      std::vector<T> rr(static_cast<size_t>(nstep) * (a.nspec - a.nspec_none) * NGLL * NGLL * 2);
+      run_forward<T>(a, nsrc, S.data(), nrec, R.data(), nstep, dt, nsb, ntypes, types, seis, t0, nthreads,
+                     ff.data(), rr.data());
+      for (size_t i = 0; i < ff.size(); i++) final_fields[i] = ff[i];
+      for (size_t i = 0; i < rr.size(); i++) record[i] = rr[i];
+    });
+  });
+}
+
+// Combined adjoint + backward run.  adj_pos (nadj, 3) [x, z, angle];
+// adj_stf (nadj, nstep, 2); kernels out (6, nspec, 5, 5) in mesh element
+// order: rho, mu, kappa, rhop, alpha, beta; seis (nsig, ntypes, nrec, 2) of
+// the backward field, latest sample first.
+int orc2d_combined(void *h, int nsrc, const double *src, int nadj, const double *adj_pos, const double *adj_stf,
+                   int nrec, const double *rec, int nstep, double dt, int nsb, int ntypes, const int *types,
+                   const double *final_fields, const double *record, double *kernels, double *seis, int nthreads) {
+  return guard([&] {
+    auto S = sources_in(nsrc, src);
+    auto A = receivers_in(nadj, adj_pos);
+    auto R = receivers_in(nrec, rec);
+    DISPATCH(h, {
+      auto &a = asm_of<T>(h);
+      std::vector<T> ff(final_fields, final_fields + 6 * static_cast<size_t>(a.nglob));
+      std::vector<T> rr(record, record + static_cast<size_t>(nstep) * (a.nspec - a.nspec_none) * NGLL * NGLL * 2);
+      const size_t npts = static_cast<size_t>(a.nspec) * NGLL * NGLL;
+      std::vector<T> K(6 * npts, 0);
+      run_combined<T>(a, nsrc, S.data(), nadj, A.data(), adj_stf, nrec, R.data(), nstep, dt, nsb, ntypes, types,
+                      ff.data(), rr.data(), K.data(), seis, nthreads);
+      for (int k = 0; k < 6; k++)
+        for (int c = 0; c < a.nspec; c++)
+          for (int p = 0; p < NGLL * NGLL; p++)
+            kernels[(k * static_cast<size_t>(a.nspec) + a.compute_to_mesh[c]) * NGLL * NGLL + p] =
+                K[k * npts + static_cast<size_t>(c) * NGLL * NGLL + p];
     });
   });
 }

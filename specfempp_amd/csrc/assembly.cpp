@@ -261,8 +261,9 @@ void check_run(const sfx_run_desc &run) {
   for (int i = 0; i < run.nseismo_types; i++)
     if (run.seismo_types[i] < 0 || run.seismo_types[i] > 2) throw std::invalid_argument("bad seismogram type");
   if (run.chunk_size < 1) throw std::invalid_argument("chunk_size must be >= 1");
-  if (run.scatter != SFX_SCATTER_COLORED && run.scatter != SFX_SCATTER_ATOMIC && run.scatter != SFX_SCATTER_GATHER)
-    throw std::invalid_argument("scatter must be SFX_SCATTER_COLORED, SFX_SCATTER_ATOMIC or SFX_SCATTER_GATHER");
+  if (run.scatter != SFX_SCATTER_COLORED && run.scatter != SFX_SCATTER_ATOMIC && run.scatter != SFX_SCATTER_GATHER &&
+      run.scatter != SFX_SCATTER_BLOCK)
+    throw std::invalid_argument("scatter must be SFX_SCATTER_COLORED, _ATOMIC, _GATHER or _BLOCK");
 }
 
 }  // namespace
@@ -273,7 +274,8 @@ void check_run(const sfx_run_desc &run) {
 HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *src, int nrec,
                                const sfx_receiver *rec, const sfx_run_desc &run) {
   check_run(run);
-  if (run.scatter == SFX_SCATTER_GATHER) throw std::invalid_argument("SFX_SCATTER_GATHER is for 3-D meshes");
+  if (run.scatter == SFX_SCATTER_GATHER || run.scatter == SFX_SCATTER_BLOCK)
+    throw std::invalid_argument("SFX_SCATTER_GATHER and SFX_SCATTER_BLOCK are for 3-D meshes");
   if (m.nspec <= 0 || m.npgeo <= 0) throw std::invalid_argument("empty mesh");
   if (m.ngnod != 4 && m.ngnod != 9) throw std::invalid_argument("ngnod must be 4 or 9");
   if (!m.coorg || !m.knods || !m.material || !m.rho || !m.vp || !m.vs) throw std::invalid_argument("null mesh array");
@@ -428,6 +430,7 @@ HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *
             slot[lp] = k;
             bnd_of[p] = k;
             a.bnd_gather.push_back(a.gather[p]);
+            a.bnd_slot.push_back(static_cast<int32_t>(p));
             a.bnd_normal.push_back(0);
             a.bnd_normal.push_back(0);
             a.bnd_weight.push_back(0);
@@ -543,7 +546,49 @@ HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *
     for (int iz = 0; iz < NGLL; iz++)
       for (int ix = 0; ix < NGLL; ix++) a.rec_h[static_cast<size_t>(r) * npe + iz * NGLL + ix] = hx[ix] * hg[iz];
   }
+  // kept for locating adjoint sources after the build
+  a.ngnod2d = ngnod;
+  a.ctrl2d = std::move(ctrl);
+  a.px2d = std::move(px);
+  a.pz2d = std::move(pz);
   return a;
+}
+
+// Adjoint sources (src/source/adjoint_source.cpp:9-90): the Lagrange product
+// at the located point for both components, each component driven by its own
+// external time function (src/source_time_function/external.cpp).  Stored as
+// two single-component sources per adjoint source (x, then z) so the source
+// kernel's one time function per source applies.
+void place_adjoint_sources_2d(HostAssembly &a, int nadj, const sfx_receiver *pos, const float *stf, int nstep) {
+  if (a.dim != 2) throw std::invalid_argument("adjoint sources are implemented for 2-D meshes");
+  if (nadj < 0 || (nadj && (!pos || !stf))) throw std::invalid_argument("null adjoint source array");
+  const int npe = 25;
+  Locator2D loc{a, a.ctrl2d, a.px2d, a.pz2d, a.ngnod2d};
+  a.nadj = nadj;
+  a.adj_ispec.assign(2 * static_cast<size_t>(nadj), 0);
+  a.adj_array.assign(2 * static_cast<size_t>(nadj) * 2 * npe, 0.0f);
+  a.adj_stf.assign(static_cast<size_t>(nstep) * 2 * nadj, 0.0f);
+  for (int s = 0; s < nadj; s++) {
+    int e;
+    float xi, gamma;
+    loc.locate(static_cast<float>(pos[s].x), static_cast<float>(pos[s].z), e, xi, gamma);
+    float hx[5], hg[5];
+    lagrange<float>(xi, a.xi, hx, nullptr);
+    lagrange<float>(gamma, a.xi, hg, nullptr);
+    for (int c = 0; c < 2; c++) {
+      const int v = 2 * s + c;
+      a.adj_ispec[v] = e;
+      float *arr = &a.adj_array[static_cast<size_t>(v) * 2 * npe + static_cast<size_t>(c) * npe];
+      for (int iz = 0; iz < NGLL; iz++)
+        for (int ix = 0; ix < NGLL; ix++) arr[iz * NGLL + ix] = hx[ix] * hg[iz];
+      for (int i = 0; i < nstep; i++)
+        a.adj_stf[static_cast<size_t>(i) * 2 * nadj + v] = stf[(static_cast<size_t>(s) * nstep + i) * 2 + c];
+    }
+  }
+  a.adj_order.clear();
+  for (int pass = 0; pass < 2; pass++)
+    for (int v = 0; v < 2 * nadj; v++)
+      if ((a.adj_ispec[v] >= a.nspec_none) == (pass == 1)) a.adj_order.push_back(v);
 }
 
 // ---------------------------------------------------------------------------
@@ -569,6 +614,8 @@ HostAssembly build_assembly_3d(const sfx_mesh3d &m, int nsrc, const sfx_source *
                                const sfx_receiver *rec, const sfx_run_desc &run) {
   check_run(run);
   if (m.nx <= 0 || m.ny <= 0 || m.nz <= 0) throw std::invalid_argument("empty 3-D mesh");
+  if (run.scatter == SFX_SCATTER_BLOCK && ((m.nx | m.ny | m.nz) & 1))
+    throw std::invalid_argument("SFX_SCATTER_BLOCK needs even element counts nx, ny, nz");
   if (!(m.hx > 0 && m.hy > 0 && m.hz > 0)) throw std::invalid_argument("element sizes must be positive");
   if (!m.material || !m.rho || !m.vp || !m.vs) throw std::invalid_argument("null mesh array");
   for (int i = 0; i < m.nmaterials; i++)
@@ -857,7 +904,7 @@ void color_elements(HostAssembly &a, int scatter) {
   const int nspec = a.nspec, npe = a.npe;
   std::vector<int> color(nspec, 0);
   int ncolors = 1;
-  if (scatter == SFX_SCATTER_COLORED) {  // ATOMIC and GATHER keep compute order
+  if (scatter == SFX_SCATTER_COLORED) {  // ATOMIC, GATHER and BLOCK keep compute order
     if (a.dim == 3) {
       // structured hexahedra: the parity of (ex, ey, ez) is a valid 8-coloring
       // (it is also what first-fit greedy produces in element order)

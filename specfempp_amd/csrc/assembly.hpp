@@ -44,7 +44,7 @@ struct HostAssembly {
   std::vector<float> kappa, mu, rho;
   // Stacey entries: one per (element, local point) on an absorbing edge/face
   std::vector<int32_t> bnd_gather;
-  std::vector<int32_t> bnd_slot;          // 3-D: element * npe + local point of the entry
+  std::vector<int32_t> bnd_slot;          // compute element * npe + local point of the entry
   std::vector<int32_t> bnd_seg;           // 3-D: entries [bnd_seg[s], bnd_seg[s+1]) share one slot
   std::vector<float> bnd_normal;         // (nbnd, dim) outward normal scaled as the reference stores it
   std::vector<float> bnd_weight;         // (nbnd) edge/face quadrature weight
@@ -69,6 +69,14 @@ struct HostAssembly {
   // reference numbering (iglob) to the lattice index
   int nx = 0, ny = 0, nz = 0;
   std::vector<int32_t> lattice_of;
+  // 2-D locator data (control nodes (nspec, ngnod, 2), GLL point coordinates)
+  int ngnod2d = 0;
+  std::vector<float> ctrl2d, px2d, pz2d;
+  // adjoint sources as single-component sources (see place_adjoint_sources_2d)
+  int nadj = 0;
+  std::vector<int32_t> adj_ispec, adj_order;
+  std::vector<float> adj_array;          // (2 nadj, ncomp, npe)
+  std::vector<float> adj_stf;            // (nstep, 2 nadj)
   // Element colors for the scatter: device element slot s holds compute
   // element perm[s]; slots [color_offsets[c], color_offsets[c+1]) form color c,
   // whose elements share no global point.  One color when scatter is atomic.
@@ -79,6 +87,9 @@ struct HostAssembly {
 // and fill perm / inv_perm / color_offsets.  scatter = SFX_SCATTER_ATOMIC keeps
 // compute order in a single color.
 void color_elements(HostAssembly &a, int scatter);
+
+// Locate adjoint sources (pos, nadj) with time functions stf (nadj, nstep, 2).
+void place_adjoint_sources_2d(HostAssembly &a, int nadj, const sfx_receiver *pos, const float *stf, int nstep);
 
 // Throw std::runtime_error / std::invalid_argument on bad input.
 HostAssembly build_assembly_2d(const sfx_mesh2d &mesh, int nsrc, const sfx_source *src, int nrec,

@@ -6,8 +6,10 @@
 // codes at the boundary.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
+#include <numeric>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -15,6 +17,7 @@
 #include "../../include/specfempp_amd.h"
 #include "assembly.hpp"
 #include "kernels.hpp"
+#include "lattice.hpp"
 
 namespace sfx {
 namespace {
@@ -76,8 +79,27 @@ struct sfx_engine {
   DevBuf<float> fields;  // u, v, a, minv: 4 * ncomp * nglob_pad
   DevBuf<float> re;      // 3-D gather path: element forces (nspec, 3, 128)
   DevBuf<float> zero;    // 384 zeros read in place of absent element contributions
+  DevBuf<float> bs;      // 3-D block path: block-surface force sums (nblocks, 3, 388)
+  DevBuf<float> snap;    // 3-D block path: u, v at receiver element points (nrec, 125, 6)
+  DevBuf<int> rec_block; // 3-D block path: 1 for blocks holding a receiver element
+  // 2-D adjoint: Stacey-element record, adjoint field, kernels, adjoint sources
+  DevBuf<float> record;      // (nstep, nstacey, 25, 2)
+  DevBuf<int> rec_index;     // per slot: Stacey element index or -1
+  DevBuf<int> bnd_rec;       // per Stacey entry: Stacey element index * 25 + point
+  DevBuf<float> rho;         // per point, slot order
+  DevBuf<float> adj_fields;  // u, v, a of the adjoint field
+  DevBuf<float> kern;        // (6, nspec, 25) slot order
+  DevBuf<int> adj_order, adj_slot;
+  DevBuf<float> adj_array, adj_stf;
+  int nstacey = 0;
+  bool recording = false;    // forward steps store the Stacey-element record
+  bool recorded = false;     // a complete recorded forward run is in the fields
+  bool adj_ready = false;
+  int rec_mode = 0;          // stiffness: 0 plain, 1 record, 2 replay
+  float *rec_step = nullptr;
   bool initialized = false;
   bool a_stale = false;  // fused gather steps leave the stored acceleration behind
+  int nbseg = 0;         // Stacey segments (3-D)
   Lattice lat{};
   std::vector<hipEvent_t> ev;
   double timings[4] = {0, 0, 0, 0};
@@ -107,6 +129,58 @@ struct sfx_engine {
   }
 
   bool gather_path() const { return h.dim == 3 && run.scatter == SFX_SCATTER_GATHER; }
+  bool block_path() const { return h.dim == 3 && run.scatter == SFX_SCATTER_BLOCK; }
+
+  // Stacey entries regrouped for the block path: entry slot = owner * 386 +
+  // slot of its point for the block of its element (lattice.hpp), entries
+  // sorted (stably) by that slot, one segment per slot.
+  void upload_block_boundary(hipStream_t s) {
+    const size_t nb = h.bnd_gather.size();
+    const int nbx = h.nx / 2, nby = h.ny / 2;
+    std::vector<int32_t> bslot(nb);
+    for (size_t q = 0; q < nb; q++) {
+      const int e = h.bnd_slot[q] / 125, p = h.bnd_slot[q] % 125;
+      const int ex = e % h.nx, ey = (e / h.nx) % h.ny, ez = e / (h.nx * h.ny);
+      const int bi = 4 * (ex & 1) + p % 5, bj = 4 * (ey & 1) + (p / 5) % 5, bk = 4 * (ez & 1) + p / 25;
+      int owner, slot;
+      if (!block_point_owner(nbx, nby, ex / 2, ey / 2, ez / 2, bi, bj, bk, owner, slot))
+        throw std::logic_error("internal: Stacey point inside a block");
+      bslot[q] = owner * OWNER_SLOTS + slot;
+    }
+    std::vector<int32_t> order(nb);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return bslot[a] < bslot[b]; });
+    std::vector<int32_t> g(nb), sl(nb), seg;
+    std::vector<float> nrm(3 * nb), wt(nb), rvp(nb), rvs(nb);
+    for (size_t q = 0; q < nb; q++) {
+      const int o = order[q];
+      g[q] = h.bnd_gather[o];
+      sl[q] = bslot[o];
+      for (int d = 0; d < 3; d++) nrm[3 * q + d] = h.bnd_normal[3 * o + d];
+      wt[q] = h.bnd_weight[o];
+      rvp[q] = h.bnd_rho_vp[o];
+      rvs[q] = h.bnd_rho_vs[o];
+      if (q == 0 || sl[q] != sl[q - 1]) seg.push_back(static_cast<int32_t>(q));
+    }
+    seg.push_back(static_cast<int32_t>(nb));
+    bnd_gather.upload(g, s);
+    bnd_slot.upload(sl, s);
+    bnd_seg.upload(seg, s);
+    bnd_normal.upload(nrm, s);
+    bnd_weight.upload(wt, s);
+    bnd_rvp.upload(rvp, s);
+    bnd_rvs.upload(rvs, s);
+    nbseg = static_cast<int>(seg.size()) - 1;
+    std::vector<int32_t> rb(static_cast<size_t>(nbx) * nby * (h.nz / 2), 0);
+    for (int e : h.rec_ispec) {
+      const int ex = e % h.nx, ey = (e / h.nx) % h.ny, ez = e / (h.nx * h.ny);
+      rb[(ez / 2 * nby + ey / 2) * nbx + ex / 2] = 1;
+    }
+    rec_block.upload(rb, s);
+    bs.alloc(block3d_surface_floats(lat));
+    launch_fill(s, bs.p, bs.n, 0.0f);
+    snap.alloc(static_cast<size_t>(h.nrec) * 125 * 6);
+  }
 
   void upload() {
     color_elements(h, run.scatter);
@@ -122,10 +196,23 @@ struct sfx_engine {
       gammax.upload(permuted(h.gammax), s);
       gammaz.upload(permuted(h.gammaz), s);
       jac.upload(permuted(h.jac), s);
+      rho.upload(permuted(h.rho), s);
+      // Stacey elements are the last ones in compute order
+      nstacey = h.nspec - h.nspec_none;
+      std::vector<int32_t> ri(h.nspec), br(h.bnd_slot.size());
+      for (int slot = 0; slot < h.nspec; slot++) ri[slot] = h.perm[slot] >= h.nspec_none ? h.perm[slot] - h.nspec_none : -1;
+      for (size_t q = 0; q < br.size(); q++) {
+        const int c = h.bnd_slot[q] / h.npe, lp = h.bnd_slot[q] % h.npe;
+        if (c < h.nspec_none) throw std::logic_error("internal: Stacey entry outside the Stacey elements");
+        br[q] = (c - h.nspec_none) * h.npe + lp;
+      }
+      rec_index.upload(ri, s);
+      bnd_rec.upload(br, s);
     }
     bnd_gather.upload(h.bnd_gather, s);
     bnd_slot.upload(h.bnd_slot, s);
     bnd_seg.upload(h.bnd_seg, s);
+    nbseg = static_cast<int>(h.bnd_seg.size()) - 1;
     if (gather_path()) {
       re.alloc(static_cast<size_t>(h.nspec) * 384);  // 3 components x 128 (125 used)
       launch_fill(s, re.p, re.n, 0.0f);
@@ -136,6 +223,7 @@ struct sfx_engine {
     bnd_weight.upload(h.bnd_weight, s);
     bnd_rvp.upload(h.bnd_rho_vp, s);
     bnd_rvs.upload(h.bnd_rho_vs, s);
+    if (block_path()) upload_block_boundary(s);
     src_order.upload(h.src_order, s);
     src_ispec.upload(slots_of(h.src_ispec), s);
     src_array.upload(h.src_array, s);
@@ -177,8 +265,19 @@ struct sfx_engine {
   }
   void element_boundary_sources(Fields ff, int istep) {
     launch_sources_re(stream, h.nsrc, istep, src_ispec.p, src_array.p, stf.p, re.p);
-    launch_stacey_re(stream, static_cast<int>(h.bnd_seg.size()) - 1, bnd_seg.p, bnd_gather.p, bnd_slot.p,
-                     bnd_normal.p, bnd_weight.p, bnd_rvp.p, bnd_rvs.p, ff, re.p);
+    launch_stacey_re(stream, nbseg, bnd_seg.p, bnd_gather.p, bnd_slot.p, bnd_normal.p, bnd_weight.p, bnd_rvp.p,
+                     bnd_rvs.p, ff, re.p);
+  }
+  // Block path: element forces, sources and block-interior assembly, then
+  // Stacey into the block-surface sums.
+  void block_forces(Fields ff, int istep, int store_a, int divide_only) {
+    if (istep < 0 || istep >= run.nstep) throw std::invalid_argument("istep out of range");
+    launch_block_step_3d(stream, lat, kappa.p, mu.p, h.xix3, h.ety3, h.gaz3, h.jac3, q, ff, nc, h.nsrc, istep,
+                         src_ispec.p, src_array.p, stf.p, rec_block.p, store_a, divide_only, bs.p);
+  }
+  void block_boundary(Fields ff) {
+    launch_stacey_bs(stream, nbseg, bnd_seg.p, bnd_gather.p, bnd_slot.p, bnd_normal.p, bnd_weight.p, bnd_rvp.p,
+                     bnd_rvs.p, ff, bs.p);
   }
   // Bring the stored acceleration up to date after fused gather steps.
   void refresh_acceleration() {
@@ -195,7 +294,7 @@ struct sfx_engine {
       const int first = h.color_offsets[c], count = h.color_offsets[c + 1] - first;
       if (h.dim == 2)
         launch_stiffness_2d(stream, first, count, atomic, gather.p, xix.p, xiz.p, gammax.p, gammaz.p, jac.p, kappa.p,
-                            mu.p, q, ff);
+                            mu.p, q, ff, rec_mode, rec_index.p, rec_step);
       else
         launch_stiffness_3d(stream, first, count, atomic, gather.p, kappa.p, mu.p, h.xix3, h.ety3, h.gaz3, h.jac3, q,
                             ff);
@@ -203,7 +302,7 @@ struct sfx_engine {
   }
   void boundary(Fields ff) {
     launch_stacey(stream, h.dim, static_cast<int>(h.bnd_gather.size()), bnd_gather.p, bnd_normal.p, bnd_weight.p,
-                  bnd_rvp.p, bnd_rvs.p, ff);
+                  bnd_rvp.p, bnd_rvs.p, ff, bnd_rec.p, rec_mode == 1 ? rec_step : nullptr);
   }
   void sources(Fields ff, int istep) {
     if (istep < 0 || istep >= run.nstep) throw std::invalid_argument("istep out of range");
@@ -219,8 +318,28 @@ struct sfx_engine {
   }
 
   // One fused Newmark step; `evs` (5 events) brackets the kernel families.
-  void step(int istep, hipEvent_t *evs) {
+  // `last`: the final step of a call; the block path then stores the
+  // acceleration of every point so that the fields read back are complete.
+  void step(int istep, hipEvent_t *evs, bool last) {
     Fields ff = f();
+    if (block_path()) {
+      const bool sample = istep % run.nstep_between_samples == 0;
+      const int isig = istep / run.nstep_between_samples;
+      if (sample && isig >= nsig) throw std::invalid_argument("seismogram step out of range");
+      if (evs) SFX_HIP(hipEventRecord(evs[0], stream));
+      if (sample) launch_receiver_snapshot(stream, lat, h.nrec, rec_ispec.p, ff, snap.p);
+      if (evs) SFX_HIP(hipEventRecord(evs[1], stream));
+      block_forces(ff, istep, last ? 2 : (sample ? 1 : 0), 0);
+      if (evs) SFX_HIP(hipEventRecord(evs[2], stream));
+      block_boundary(ff);
+      if (sample)
+        launch_seismograms_block(stream, lat, h.nrec, run.nseismo_types, types.p, isig, rec_ispec.p, rec_h.p, bs.p,
+                                 snap.p, ff, nc.dto2, seismo.p);
+      if (evs) SFX_HIP(hipEventRecord(evs[3], stream));
+      launch_surface_update_3d(stream, lat, bs.p, ff, nc, last ? 1 : 0, 0);
+      if (evs) SFX_HIP(hipEventRecord(evs[4], stream));
+      return;
+    }
     if (gather_path()) {
       if (evs) SFX_HIP(hipEventRecord(evs[0], stream));
       if (evs) SFX_HIP(hipEventRecord(evs[1], stream));
@@ -239,6 +358,10 @@ struct sfx_engine {
       a_stale = true;
       return;
     }
+    if (recording) {
+      rec_mode = 1;
+      rec_step = record.p + static_cast<size_t>(istep) * nstacey * h.npe * h.ncomp;
+    }
     if (evs) SFX_HIP(hipEventRecord(evs[0], stream));
     sources(ff, istep);
     if (evs) SFX_HIP(hipEventRecord(evs[1], stream));
@@ -247,8 +370,58 @@ struct sfx_engine {
     boundary(ff);
     if (istep % run.nstep_between_samples == 0) seismograms(ff, istep / run.nstep_between_samples, 1);
     if (evs) SFX_HIP(hipEventRecord(evs[3], stream));
-    launch_update_fused(stream, h.ncomp, ff, nc);
+    // a recorded run ends on the corrected state of its last step, the state
+    // the backward field starts from
+    if (recording && last)
+      launch_divide_corrector(stream, h.ncomp, ff, nc, 1, 1);
+    else
+      launch_update_fused(stream, h.ncomp, ff, nc);
     if (evs) SFX_HIP(hipEventRecord(evs[4], stream));
+    rec_mode = 0;
+    rec_step = nullptr;
+  }
+
+  // Combined adjoint + backward loop (time_marching.tpp:126-247), 2-D.  The
+  // backward field is the forward field itself: it holds the forward run's
+  // final state, which is where the reference's backward field stands after
+  // its first step (the state read back is copied in at istep = nstep - 1), so
+  // that step is not run.
+  void adjoint_run() {
+    const size_t stride = static_cast<size_t>(h.ncomp) * h.nglob_pad;
+    adj_fields.alloc(3 * stride);
+    launch_fill(stream, adj_fields.p, adj_fields.n, 0.0f);
+    kern.alloc(6 * static_cast<size_t>(h.nspec) * h.npe);
+    launch_fill(stream, kern.p, kern.n, 0.0f);
+    if (seismo.n) launch_fill(stream, seismo.p, seismo.n, 0.0f);
+    Fields bwd = f();
+    Fields adj{adj_fields.p, adj_fields.p + stride, adj_fields.p + 2 * stride, bwd.minv, h.nglob_pad};
+    const float dt = nc.dt;
+    const NewmarkConst back{-dt, -nc.dto2, nc.dt2o2};  // newmark.tpp:204-242
+    const int nstep = run.nstep, nsb = run.nstep_between_samples;
+    int isig = 0;
+    for (int istep = nstep - 1; istep >= 0; istep--) {
+      // adjoint field: forward-in-time Newmark with the adjoint sources
+      launch_predictor(stream, h.ncomp, adj, nc);
+      launch_sources(stream, 2, h.npe, 2 * h.nadj, istep, adj_order.p, adj_slot.p, gather.p, adj_array.p, adj_stf.p,
+                     adj);
+      stiffness(adj);
+      boundary(adj);
+      launch_divide_corrector(stream, h.ncomp, adj, nc, 1, 1);
+      // backward field: -dt, forward sources, recorded Stacey elements
+      if (istep < nstep - 1) {
+        launch_predictor(stream, h.ncomp, bwd, back);
+        sources(bwd, istep);
+        rec_mode = 2;
+        rec_step = record.p + static_cast<size_t>(istep) * nstacey * h.npe * h.ncomp;
+        stiffness(bwd);
+        rec_mode = 0;
+        rec_step = nullptr;
+        launch_divide_corrector(stream, h.ncomp, bwd, back, 1, 1);
+      }
+      launch_frechet_2d(stream, h.nspec, gather.p, xix.p, xiz.p, gammax.p, gammaz.p, kappa.p, mu.p, rho.p, q, adj, bwd,
+                        dt, kern.p);
+      if (istep % nsb == 0) seismograms(bwd, isig++, 0);
+    }
   }
 };
 
@@ -415,7 +588,11 @@ int sfx_update_wavefields(sfx_engine *e, int istep) {
     check_engine(e);
     e->require_init();
     Fields ff = e->f();
-    if (e->gather_path()) {
+    if (e->block_path()) {
+      e->block_forces(ff, istep, 0, 1);
+      e->block_boundary(ff);
+      launch_surface_update_3d(e->stream, e->lat, e->bs.p, ff, e->nc, 0, 1);
+    } else if (e->gather_path()) {
       e->element_forces(ff, istep);
       e->element_boundary_sources(ff, istep);
       launch_assemble_3d(e->stream, e->lat, e->re.p, ff, 1);
@@ -459,7 +636,7 @@ int sfx_step(sfx_engine *e, int istep) {
   return guarded([&] {
     check_engine(e);
     e->require_init();
-    e->step(istep, nullptr);
+    e->step(istep, nullptr, true);
     SFX_HIP(hipGetLastError());
   });
 }
@@ -477,7 +654,7 @@ int sfx_run(sfx_engine *e, int first, int count) {
       SFX_HIP(hipEventCreate(&ev));
       e->ev.push_back(ev);
     }
-    for (int i = 0; i < count; i++) e->step(first + i, &e->ev[5 * static_cast<size_t>(i)]);
+    for (int i = 0; i < count; i++) e->step(first + i, &e->ev[5 * static_cast<size_t>(i)], i == count - 1);
     SFX_HIP(hipGetLastError());
     SFX_HIP(hipStreamSynchronize(e->stream));
     double acc[4] = {0, 0, 0, 0};
@@ -557,6 +734,72 @@ int sfx_set_field(sfx_engine *e, int which, const float *in) {
                         e->h.nglob_pad * sizeof(float), hipMemcpyHostToDevice));
     }
     if (which == 2) e->a_stale = false;
+  });
+}
+
+int sfx_forward_record(sfx_engine *e) {
+  return guarded([&] {
+    check_engine(e);
+    e->require_init();
+    if (e->h.dim != 2) throw std::invalid_argument("adjoint runs are implemented for 2-D meshes");
+    const size_t need = static_cast<size_t>(e->run.nstep) * e->nstacey * e->h.npe * e->h.ncomp;
+    if (e->record.n != need) e->record.alloc(need);
+    e->recorded = false;
+    e->recording = true;
+    try {
+      for (int istep = 0; istep < e->run.nstep; istep++) e->step(istep, nullptr, istep == e->run.nstep - 1);
+    } catch (...) {
+      e->recording = false;
+      throw;
+    }
+    e->recording = false;
+    SFX_HIP(hipGetLastError());
+    SFX_HIP(hipStreamSynchronize(e->stream));
+    e->recorded = true;
+  });
+}
+
+int sfx_set_adjoint_sources(sfx_engine *e, int nadj, const sfx_receiver *positions, const float *stf) {
+  return guarded([&] {
+    check_engine(e);
+    place_adjoint_sources_2d(e->h, nadj, positions, stf, e->run.nstep);
+    hipStream_t s = e->stream;
+    e->adj_order.upload(e->h.adj_order, s);
+    e->adj_slot.upload(e->slots_of(e->h.adj_ispec), s);
+    e->adj_array.upload(e->h.adj_array, s);
+    e->adj_stf.upload(e->h.adj_stf, s);
+    SFX_HIP(hipStreamSynchronize(s));
+    e->adj_ready = true;
+  });
+}
+
+int sfx_adjoint_run(sfx_engine *e) {
+  return guarded([&] {
+    check_engine(e);
+    if (e->h.dim != 2) throw std::invalid_argument("adjoint runs are implemented for 2-D meshes");
+    if (!e->recorded) throw std::logic_error("sfx_forward_record must complete before sfx_adjoint_run");
+    if (!e->adj_ready) throw std::logic_error("sfx_set_adjoint_sources must be called before sfx_adjoint_run");
+    e->recorded = false;  // the backward field consumes the forward state
+    e->adjoint_run();
+    SFX_HIP(hipGetLastError());
+    SFX_HIP(hipStreamSynchronize(e->stream));
+  });
+}
+
+int sfx_get_kernels(const sfx_engine *e, float *out) {
+  return guarded([&] {
+    check_engine(e);
+    if (!out) throw std::invalid_argument("null output");
+    if (!e->kern.n) throw std::logic_error("no kernels: run sfx_adjoint_run first");
+    SFX_HIP(hipStreamSynchronize(e->stream));
+    std::vector<float> buf(e->kern.n);
+    SFX_HIP(hipMemcpy(buf.data(), e->kern.p, buf.size() * sizeof(float), hipMemcpyDeviceToHost));
+    const size_t npe = e->h.npe, nspec = e->h.nspec;
+    for (int k = 0; k < 6; k++)
+      for (size_t slot = 0; slot < nspec; slot++) {
+        const size_t m = e->h.compute_to_mesh[e->h.perm[slot]];
+        std::memcpy(out + (k * nspec + m) * npe, buf.data() + (k * nspec + slot) * npe, npe * sizeof(float));
+      }
   });
 }
 

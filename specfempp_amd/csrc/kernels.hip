@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
+#include "lattice.hpp"
 
 namespace sfx {
 
@@ -34,11 +35,17 @@ template <bool ATOMIC> __device__ __forceinline__ void scatter_add(float *p, flo
 // ---------------------------------------------------------------------------
 constexpr int E2 = 5;  // elements per block
 
+// Stacey-element record (compute_stiffness_interaction.tpp:119-136,251-255):
+// rec_mode 1 stores each Stacey element's local force (the Stacey traction is
+// added by the boundary kernel), rec_mode 2 replaces it by the stored value
+// (the backward field).  rec_index: Stacey element index of each slot, or -1;
+// rec: (nstacey, 25, 2) of the current step.
 template <bool ATOMIC>
 __global__ __launch_bounds__(128) void stiffness_2d_kernel(
     int first, int count, const int *__restrict__ gather, const float *__restrict__ xix_, const float *__restrict__ xiz_,
     const float *__restrict__ gammax_, const float *__restrict__ gammaz_, const float *__restrict__ jac_,
-    const float *__restrict__ kappa_, const float *__restrict__ mu_, GllConst q, Fields f) {
+    const float *__restrict__ kappa_, const float *__restrict__ mu_, GllConst q, Fields f, int rec_mode,
+    const int *__restrict__ rec_index, float *__restrict__ rec) {
   __shared__ float su[E2][2][25];
   __shared__ float sF[E2][4][25];  // F(c, d) as [2c + d]
 
@@ -119,11 +126,98 @@ __global__ __launch_bounds__(128) void stiffness_2d_kernel(
       t2[c] += sF[el][2 * c + 1][l * 5 + ix] * hwz[l];
     }
   }
+  float r[2];
+#pragma unroll
+  for (int c = 0; c < 2; c++) r[c] = -(wz * t1[c] + wx * t2[c]);
+  if (rec_mode) {
+    const int ri = rec_index[e];
+    if (ri >= 0) {
+      float *rp = rec + (static_cast<size_t>(ri) * 25 + p) * 2;
+      if (rec_mode == 1) {
+        rp[0] = r[0];
+        rp[1] = r[1];
+      } else {
+        r[0] = rp[0];
+        r[1] = rp[1];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 2; c++) scatter_add<ATOMIC>(&f.a[c * f.nglob_pad + g], r[c]);
+}
+
+// Fréchet derivatives of one step (compute_material_derivatives.tpp:17-154,
+// include/medium/dim2/elastic/isotropic/frechet_derivative.hpp:14-138):
+// gradients of the adjoint and backward displacements per element point,
+// kernels K[k][slot * 25 + p] += (rho, mu, kappa, rhop, alpha, beta).
+__global__ __launch_bounds__(128) void frechet_2d_kernel(
+    int nspec, const int *__restrict__ gather, const float *__restrict__ xix_, const float *__restrict__ xiz_,
+    const float *__restrict__ gammax_, const float *__restrict__ gammaz_, const float *__restrict__ kappa_,
+    const float *__restrict__ mu_, const float *__restrict__ rho_, GllConst q, Fields adj, Fields bwd, float dt,
+    float *__restrict__ K) {
+  __shared__ float sa[E2][2][25];
+  __shared__ float sb[E2][2][25];
+  const int t = threadIdx.x;
+  const int el = t / 25;
+  const int p = t - el * 25;
+  const int iz = p / 5, ix = p - (p / 5) * 5;
+  const int e = blockIdx.x * E2 + el;
+  const bool active = (el < E2) && (e < nspec);
+  const size_t idx = static_cast<size_t>(e) * 25 + p;
+  const int np = adj.nglob_pad;
+  int g = 0;
+  if (active) {
+    g = gather[idx];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      sa[el][c][p] = adj.u[c * np + g];
+      sb[el][c][p] = bwd.u[c * np + g];
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  const float xix = xix_[idx], xiz = xiz_[idx], gammax = gammax_[idx], gammaz = gammaz_[idx];
+  float df[2][2], dg[2][2];  // d u_c / d x_j (gradient.hpp:81-103)
 #pragma unroll
   for (int c = 0; c < 2; c++) {
-    const float r = -(wz * t1[c] + wx * t2[c]);
-    scatter_add<ATOMIC>(&f.a[c * f.nglob_pad + g], r);
+    float fxi = 0.0f, fga = 0.0f, gxi = 0.0f, gga = 0.0f;
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+      fxi += q.hp[ix][l] * sa[el][c][iz * 5 + l];
+      fga += q.hp[iz][l] * sa[el][c][l * 5 + ix];
+      gxi += q.hp[ix][l] * sb[el][c][iz * 5 + l];
+      gga += q.hp[iz][l] * sb[el][c][l * 5 + ix];
+    }
+    df[c][0] = xix * fxi + gammax * fga;
+    df[c][1] = xiz * fxi + gammaz * fga;
+    dg[c][0] = xix * gxi + gammax * gga;
+    dg[c][1] = xiz * gxi + gammaz * gga;
   }
+  const float mu = mu_[idx], rho = rho_[idx];
+  const float lp2m = kappa_[idx] + (4.0f / 3.0f) * mu;
+  const float kappa = lp2m - mu;
+  const float ad_dsxx = df[0][0];
+  const float ad_dsxz = 0.5f * (df[1][0] + df[0][1]);
+  const float ad_dszz = df[1][1];
+  const float b_dsxx = dg[0][0];
+  const float b_dsxz = 0.5f * (dg[1][0] + dg[0][1]);
+  const float b_dszz = dg[1][1];
+  float kappa_kl = (ad_dsxx + ad_dszz) * (b_dsxx + b_dszz);
+  float mu_kl = (ad_dsxx * b_dsxx + ad_dszz * b_dszz + 2.0f * ad_dsxz * b_dsxz - (1.0f / 3.0f) * kappa_kl);
+  float rho_kl = adj.a[g] * bwd.u[g] + adj.a[np + g] * bwd.u[np + g];
+  kappa_kl = -1.0f * kappa * dt * kappa_kl;
+  mu_kl = -2.0f * mu * dt * mu_kl;
+  rho_kl = -1.0f * rho * dt * rho_kl;
+  const float rhop_kl = rho_kl + kappa_kl + mu_kl;
+  const float beta_kl = 2.0f * (mu_kl - (4.0f / 3.0f) * mu / kappa * kappa_kl);
+  const float alpha_kl = 2.0f * (1.0f + (4.0f / 3.0f) * mu / kappa) * kappa_kl;
+  const size_t n = static_cast<size_t>(nspec) * 25;
+  K[idx] += rho_kl;
+  K[n + idx] += mu_kl;
+  K[2 * n + idx] += kappa_kl;
+  K[3 * n + idx] += rhop_kl;
+  K[4 * n + idx] += alpha_kl;
+  K[5 * n + idx] += beta_kl;
 }
 
 // ---------------------------------------------------------------------------
@@ -250,7 +344,8 @@ __global__ __launch_bounds__(256) void stacey_kernel(int nbnd, const int *__rest
                                                      const float *__restrict__ normal,
                                                      const float *__restrict__ weight,
                                                      const float *__restrict__ rho_vp,
-                                                     const float *__restrict__ rho_vs, Fields f) {
+                                                     const float *__restrict__ rho_vs, Fields f,
+                                                     const int *__restrict__ bnd_rec, float *__restrict__ rec) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nbnd) return;
   const int g = gather[q];
@@ -269,7 +364,9 @@ __global__ __launch_bounds__(256) void stacey_kernel(int nbnd, const int *__rest
 #pragma unroll
   for (int d = 0; d < D; d++) {
     const float fac = ((vn * n[d] / (j1 * j1)) * (rvp - rvs)) + v[d] * rvs;
-    atomic_add(&f.a[d * np + g], -1.0f * fac * j1 * w);
+    const float tr = -1.0f * fac * j1 * w;
+    atomic_add(&f.a[d * np + g], tr);
+    if (rec) rec[static_cast<size_t>(bnd_rec[q]) * D + d] += tr;
   }
 }
 
@@ -421,37 +518,6 @@ __global__ __launch_bounds__(256) void divide_corrector_kernel(int n, int ncomp,
 // workgroups round-robin, so each XCD is given one contiguous slab of
 // elements and its L2 sees the shared faces of neighbouring elements.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int xcd_block(int b, int nblocks) {
-  constexpr int NXCD = 8;
-  const int per = nblocks / NXCD;  // blocks with index < per * NXCD are remapped
-  if (b >= per * NXCD) return b;
-  return (b % NXCD) * per + b / NXCD;
-}
-
-// Device index of local point (i, j, k) (0..4) of element (ex, ey, ez).
-__device__ __forceinline__ int tiled_index(const Lattice &lat, int ex, int ey, int ez, int i, int j, int k) {
-  const int tx = ex + (i >> 2), ty = ey + (j >> 2), tz = ez + (k >> 2);
-  return ((tz * lat.TY + ty) * lat.TX + tx) * 64 + (((k & 3) * 4 + (j & 3)) * 4 + (i & 3));
-}
-
-// Element-force block: 3 components of 128 floats (125 used), so every
-// component starts 16-byte aligned.
-constexpr int RE_STRIDE = 128;
-constexpr int RE_ELEM = 3 * RE_STRIDE;
-
-// Slot of local point (i, j, k) in an element's block of re.
-__device__ __forceinline__ int re_slot(int i, int j, int k) {
-  const int fi = i >> 2, fj = j >> 2, fk = k >> 2;  // 1 on the local index-4 faces
-  if (!(fi | fj | fk)) return (k * 4 + j) * 4 + i;
-  if (fi & !fj & !fk) return 64 + k * 4 + j;
-  if (!fi & fj & !fk) return 80 + k * 4 + i;
-  if (!fi & !fj & fk) return 96 + j * 4 + i;
-  if (fi & fj & !fk) return 112 + k;
-  if (fi & !fj & fk) return 116 + j;
-  if (!fi & fj & fk) return 120 + i;
-  return 124;
-}
-
 __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const float *__restrict__ kappa_,
                                                            const float *__restrict__ mu_, float xix, float ety,
                                                            float gaz, float jac, GllConst q,
@@ -800,14 +866,23 @@ inline unsigned blocks(size_t n, unsigned per) { return static_cast<unsigned>((n
 
 void launch_stiffness_2d(hipStream_t s, int first, int count, bool atomic, const int *gather, const float *xix,
                          const float *xiz, const float *gammax, const float *gammaz, const float *jac,
-                         const float *kappa, const float *mu, const GllConst &q, Fields f) {
+                         const float *kappa, const float *mu, const GllConst &q, Fields f, int rec_mode,
+                         const int *rec_index, float *rec) {
   if (count <= 0) return;
   if (atomic)
     hipLaunchKernelGGL(stiffness_2d_kernel<true>, dim3(blocks(count, E2)), dim3(128), 0, s, first, count, gather, xix,
-                       xiz, gammax, gammaz, jac, kappa, mu, q, f);
+                       xiz, gammax, gammaz, jac, kappa, mu, q, f, rec_mode, rec_index, rec);
   else
     hipLaunchKernelGGL(stiffness_2d_kernel<false>, dim3(blocks(count, E2)), dim3(128), 0, s, first, count, gather, xix,
-                       xiz, gammax, gammaz, jac, kappa, mu, q, f);
+                       xiz, gammax, gammaz, jac, kappa, mu, q, f, rec_mode, rec_index, rec);
+}
+
+void launch_frechet_2d(hipStream_t s, int nspec, const int *gather, const float *xix, const float *xiz,
+                       const float *gammax, const float *gammaz, const float *kappa, const float *mu, const float *rho,
+                       const GllConst &q, Fields adj, Fields bwd, float dt, float *K) {
+  if (nspec <= 0) return;
+  hipLaunchKernelGGL(frechet_2d_kernel, dim3(blocks(nspec, E2)), dim3(128), 0, s, nspec, gather, xix, xiz, gammax,
+                     gammaz, kappa, mu, rho, q, adj, bwd, dt, K);
 }
 
 void launch_stiffness_3d(hipStream_t s, int first, int count, bool atomic, const int *gather, const float *kappa,
@@ -822,14 +897,14 @@ void launch_stiffness_3d(hipStream_t s, int first, int count, bool atomic, const
 }
 
 void launch_stacey(hipStream_t s, int dim, int nbnd, const int *gather, const float *normal, const float *weight,
-                   const float *rho_vp, const float *rho_vs, Fields f) {
+                   const float *rho_vp, const float *rho_vs, Fields f, const int *bnd_rec, float *rec) {
   if (nbnd <= 0) return;
   if (dim == 2)
     hipLaunchKernelGGL(stacey_kernel<2>, dim3(blocks(nbnd, 256)), dim3(256), 0, s, nbnd, gather, normal, weight,
-                       rho_vp, rho_vs, f);
+                       rho_vp, rho_vs, f, bnd_rec, rec);
   else
     hipLaunchKernelGGL(stacey_kernel<3>, dim3(blocks(nbnd, 256)), dim3(256), 0, s, nbnd, gather, normal, weight,
-                       rho_vp, rho_vs, f);
+                       rho_vp, rho_vs, f, bnd_rec, rec);
 }
 
 void launch_sources(hipStream_t s, int dim, int npe, int nsrc, int istep, const int *order, const int *ispec,

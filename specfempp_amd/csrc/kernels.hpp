@@ -29,9 +29,18 @@ struct NewmarkConst {
 
 // Stiffness over device element slots [first, first + count).  atomic = false
 // requires that no two of these elements share a global point (one color).
+// rec_mode 0: plain; 1: store the local forces of Stacey elements
+// (rec_index[slot] >= 0) into rec (nstacey, 25, 2); 2: add the stored values
+// for those elements instead of their computed forces.
 void launch_stiffness_2d(hipStream_t s, int first, int count, bool atomic, const int *gather, const float *xix,
                          const float *xiz, const float *gammax, const float *gammaz, const float *jac,
-                         const float *kappa, const float *mu, const GllConst &q, Fields f);
+                         const float *kappa, const float *mu, const GllConst &q, Fields f, int rec_mode = 0,
+                         const int *rec_index = nullptr, float *rec = nullptr);
+// K (6, nspec, 25) += Fréchet derivatives of one step from the adjoint and
+// backward fields.
+void launch_frechet_2d(hipStream_t s, int nspec, const int *gather, const float *xix, const float *xiz,
+                       const float *gammax, const float *gammaz, const float *kappa, const float *mu, const float *rho,
+                       const GllConst &q, Fields adj, Fields bwd, float dt, float *K);
 void launch_stiffness_3d(hipStream_t s, int first, int count, bool atomic, const int *gather, const float *kappa,
                          const float *mu, float xix, float ety, float gaz, float jac, const GllConst &q, Fields f);
 // ---- 3-D structured path with buffered element forces (SFX_SCATTER_GATHER) ----
@@ -65,8 +74,35 @@ void launch_assemble_3d(hipStream_t s, Lattice lat, const float *re, Fields f, i
 void launch_seismograms_re(hipStream_t s, Lattice lat, int nrec, int ntypes, const int *types, int isig,
                            const int *ispec, const float *h, const float *re, Fields f, float dto2, float *seismo);
 
+// ---- 3-D structured path with block assembly (SFX_SCATTER_BLOCK) ----
+// Workgroup = 2 x 2 x 2 elements (nx, ny, nz even).  Block-interior points are
+// assembled and updated inside launch_block_step_3d; block-surface sums go to
+// bs (block_surface_floats floats) and are finished by launch_surface_update_3d.
+// store_a: 0 keeps f.a untouched, 1 stores a = M^-1 f in the interior of blocks
+// with rec_block set, 2 stores it everywhere.  divide_only: only a = M^-1 f is
+// stored (the update_wavefields seam), u and v are left as they are.
+size_t block3d_surface_floats(const Lattice &lat);
+void launch_block_step_3d(hipStream_t s, Lattice lat, const float *kappa, const float *mu, float xix, float ety,
+                          float gaz, float jac, const GllConst &q, Fields f, NewmarkConst nc, int nsrc, int istep,
+                          const int *src_ispec, const float *src_array, const float *stf, const int *rec_block,
+                          int store_a, int divide_only, float *bs);
+void launch_surface_update_3d(hipStream_t s, Lattice lat, const float *bs, Fields f, NewmarkConst nc, int store_a,
+                              int divide_only);
+// bs += Stacey traction; bslot = owner * 386 + slot (lattice.hpp), entries
+// [seg[i], seg[i+1]) share one slot.
+void launch_stacey_bs(hipStream_t s, int nseg, const int *seg, const int *gather, const int *bslot,
+                      const float *normal, const float *weight, const float *rho_vp, const float *rho_vs, Fields f,
+                      float *bs);
+// u, v at the receiver elements' points before the block kernel overwrites them.
+void launch_receiver_snapshot(hipStream_t s, Lattice lat, int nrec, const int *ispec, Fields f, float *snap);
+void launch_seismograms_block(hipStream_t s, Lattice lat, int nrec, int ntypes, const int *types, int isig,
+                              const int *ispec, const float *h, const float *bs, const float *snap, Fields f,
+                              float dto2, float *seismo);
+
+// rec != nullptr: also add each entry's traction to rec[bnd_rec[entry] * dim + d].
 void launch_stacey(hipStream_t s, int dim, int nbnd, const int *gather, const float *normal, const float *weight,
-                   const float *rho_vp, const float *rho_vs, Fields f);
+                   const float *rho_vp, const float *rho_vs, Fields f, const int *bnd_rec = nullptr,
+                   float *rec = nullptr);
 void launch_sources(hipStream_t s, int dim, int npe, int nsrc, int istep, const int *order, const int *ispec,
                     const int *gather, const float *array, const float *stf, Fields f);
 // corrected = 1: sample the state after the (not yet applied) mass division and

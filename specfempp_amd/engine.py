@@ -105,13 +105,14 @@ def _receivers(receivers):
     return arr
 
 
-SCATTER_COLORED, SCATTER_ATOMIC, SCATTER_GATHER = 0, 1, 2
+SCATTER_COLORED, SCATTER_ATOMIC, SCATTER_GATHER, SCATTER_BLOCK = 0, 1, 2, 3
+_SCATTER_CODES = {"colored": 0, "atomic": 1, "gather": 2, "block": 3}
 
 
 def _run_desc(dt, nstep, nsb, types, chunk, scatter=SCATTER_COLORED):
     run = _Run()
     run.dt, run.nstep, run.nstep_between_samples, run.chunk_size = dt, nstep, nsb, chunk
-    run.scatter = {"colored": 0, "atomic": 1, "gather": 2}.get(scatter, scatter)
+    run.scatter = _SCATTER_CODES.get(scatter, scatter)
     run.nseismo_types = len(types)
     for i, t in enumerate(types):
         run.seismo_types[i] = _TYPE_CODES[t] if isinstance(t, str) else int(t)
@@ -225,6 +226,33 @@ class _EngineBase:
 class Engine2D(_EngineBase):
     """2-D elastic P-SV engine built from a :class:`Mesh2D`-like object."""
 
+    KERNEL_NAMES = ("rho", "mu", "kappa", "rhop", "alpha", "beta")
+
+    # ---- adjoint (time_marching<combined>) ----
+    def forward_record(self):
+        """Whole forward loop that keeps the Stacey-element record and the
+        final state for :meth:`adjoint_run`."""
+        self._check(self.lib.sfx_forward_record(self.h))
+
+    def set_adjoint_sources(self, positions, stf):
+        """positions: [{"x", "z"}]; stf (nadj, nstep, 2) in forward time order."""
+        arr = np.ascontiguousarray(stf, dtype=np.float32)
+        if arr.shape != (len(positions), self.nstep, 2):
+            raise ValueError(f"stf must be (nadj, nstep, 2) = ({len(positions)}, {self.nstep}, 2)")
+        self._adj_keep = (arr, _receivers(positions))
+        self._check(self.lib.sfx_set_adjoint_sources(self.h, len(positions), self._adj_keep[1], _ptr(arr)))
+
+    def adjoint_run(self):
+        """Combined adjoint + backward loop; seismograms() then returns the
+        backward field's, latest sample first."""
+        self._check(self.lib.sfx_adjoint_run(self.h))
+
+    def kernels(self):
+        """dict name -> (nspec, 5, 5) in mesh element order."""
+        out = np.zeros((6, self.nspec, 5, 5), dtype=np.float32)
+        self._check(self.lib.sfx_get_kernels(self.h, _ptr(out)))
+        return dict(zip(self.KERNEL_NAMES, out))
+
     def __init__(self, mesh, sources, receivers, dt, nstep, nstep_between_samples=1,
                  seismogram_types=("displacement",), chunk_size=1, scatter="colored"):
         self.lib = _native.load()
@@ -250,10 +278,18 @@ class Engine2D(_EngineBase):
 
 
 class Engine3D(_EngineBase):
-    """3-D elastic engine on a structured hexahedral mesh (see ``mesh3d``)."""
+    """3-D elastic engine on a structured hexahedral mesh (see ``mesh3d``).
+
+    ``scatter="auto"`` picks block assembly when nx, ny and nz are even and the
+    gather path otherwise; "block", "gather", "colored" and "atomic" force one.
+    """
 
     def __init__(self, mesh, sources, receivers, dt, nstep, nstep_between_samples=1,
-                 seismogram_types=("displacement",), scatter="gather"):
+                 seismogram_types=("displacement",), scatter="auto"):
+        if scatter == "auto":
+            even = all(int(mesh[k]) % 2 == 0 for k in ("nx", "ny", "nz"))
+            scatter = "block" if even else "gather"
+        self.scatter = scatter
         self.lib = _native.load()
         self.nstep = nstep
         mat = np.ascontiguousarray(mesh["material"], dtype=np.int32).ravel()

@@ -1,0 +1,102 @@
+"""GPU parity of the 2-D combined adjoint + backward run against the oracle.
+
+The oracle's combined loop is pinned on the CPU (test_oracle_adjoint.py:
+time reversibility and finite differences of the misfit).  Here the HIP
+engine's forward record, backward seismograms and the six kernels are
+compared with the FP32 and FP64 oracle on the same inputs.  Bars: relative L2
+<= 1e-5 against the FP32 oracle, or, where FP32 rounding alone moves the
+result further (the kernels are sums over every step of products of two
+wavefields), a distance to the FP64 oracle within 1.5x the FP32 oracle's own.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_l2
+from specfempp_amd import meshes
+
+pytestmark = pytest.mark.gpu
+
+N, H, DT, NSTEP = 16, 50.0, 1e-3, 500
+TYPES = ("displacement",)
+SRC = [dict(x=310.0, z=420.0, angle=30.0, f0=15.0, factor=1e10)]
+REC = [dict(x=600.0, z=650.0), dict(x=180.0, z=200.0), dict(x=520.0, z=130.0)]
+NAMES = ("rho", "mu", "kappa", "rhop", "alpha", "beta")
+
+
+def _mesh():
+    L = N * H
+    return meshes.quad_mesh(N, N, 0, L, 0, L, materials=[meshes.elastic(2400.0, 3000.0, 1732.0)],
+                            absorbing=("bottom", "right", "left", "top"))
+
+
+def _report(name, **vals):
+    print(f"[adjoint] {name}: " + ", ".join(f"{k}={v:.3e}" for k, v in vals.items()), flush=True)
+
+
+def _check(e32, e64, floor):
+    assert e32 <= 1e-5 or e64 <= 1.5 * floor, (e32, e64, floor)
+
+
+@pytest.fixture(scope="module")
+def oracle_runs():
+    mesh = _mesh()
+    out = {}
+    rng = np.random.default_rng(7)
+    # a seeded adjoint source (smooth random traces at the receivers)
+    t = np.arange(NSTEP) * DT
+    stf = np.zeros((len(REC), NSTEP, 2))
+    for r in range(len(REC)):
+        for c in range(2):
+            a, f, ph = rng.uniform(0.5, 1.5), rng.uniform(8, 20), rng.uniform(0, 2 * np.pi)
+            stf[r, :, c] = 1e-6 * a * np.sin(2 * np.pi * f * t + ph) * np.exp(-((t - 0.25) / 0.1) ** 2)
+    for prec in (4, 8):
+        o = oracle.Oracle2D(mesh, prec, 1, DT)
+        seis, _, fields, record = o.forward_record(SRC, REC, NSTEP, DT, 1, TYPES)
+        kern, bseis = o.combined(SRC, REC, stf, REC, NSTEP, DT, fields, record, 1, TYPES)
+        out[prec] = (seis, kern, bseis)
+    return mesh, stf, out
+
+
+@pytest.mark.parametrize("scatter", ["colored", "atomic"])
+def test_adjoint_kernels_match_oracle(oracle_runs, scatter):
+    from specfempp_amd import Engine2D
+
+    mesh, stf, out = oracle_runs
+    eng = Engine2D(mesh, SRC, REC, DT, NSTEP, 1, TYPES, scatter=scatter)
+    eng.initialize()
+    eng.forward_record()
+    fwd = eng.seismograms().astype(np.float64)
+    eng.set_adjoint_sources(REC, stf)
+    eng.adjoint_run()
+    bwd = eng.seismograms().astype(np.float64)
+    kern = eng.kernels()
+    eng.close()
+    (s32, k32, b32), (s64, k64, b64) = out[4], out[8]
+    e32 = rel_l2(fwd, s32)
+    _report(f"{scatter}/forward", vs_oracle_fp32=e32, vs_oracle_fp64=rel_l2(fwd, s64))
+    assert e32 <= 1e-5
+    e32, e64, floor = rel_l2(bwd, b32), rel_l2(bwd, b64), rel_l2(b32, b64)
+    _report(f"{scatter}/backward_seismograms", vs_oracle_fp32=e32, vs_oracle_fp64=e64, fp32_floor=floor)
+    _check(e32, e64, floor)
+    for k, name in enumerate(NAMES):
+        ours = kern[name].astype(np.float64)
+        e32, e64, floor = rel_l2(ours, k32[k]), rel_l2(ours, k64[k]), rel_l2(k32[k], k64[k])
+        _report(f"{scatter}/K_{name}", vs_oracle_fp32=e32, vs_oracle_fp64=e64, fp32_floor=floor)
+        _check(e32, e64, floor)
+
+
+def test_adjoint_requires_recorded_forward():
+    from specfempp_amd import Engine2D
+    from specfempp_amd.engine import SfxError
+
+    eng = Engine2D(_mesh(), SRC, REC, DT, 50, 1, TYPES)
+    eng.initialize()
+    eng.set_adjoint_sources(REC, np.zeros((len(REC), 50, 2)))
+    with pytest.raises(SfxError, match="sfx_forward_record"):
+        eng.adjoint_run()
+    eng.forward_record()
+    eng.adjoint_run()
+    with pytest.raises(SfxError, match="sfx_forward_record"):
+        eng.adjoint_run()  # the recorded state is consumed
+    eng.close()

@@ -136,8 +136,8 @@ def test_config1_homogeneous_80x80():
 
 # The Ricker wavelet (f0 = 5 Hz) peaks at step 240 (t0 = -1.2 / f0); windows
 # reach well past it so the traces are signal, not rounding noise.
-@pytest.mark.parametrize("n,nstep,scatter", [(6, 600, "gather"), (10, 500, "gather"), (6, 600, "colored"),
-                                            (6, 600, "atomic")])
+@pytest.mark.parametrize("n,nstep,scatter", [(6, 600, "block"), (10, 500, "block"), (6, 600, "gather"),
+                                            (10, 500, "gather"), (6, 600, "colored"), (6, 600, "atomic")])
 def test_3d_layered_matches_oracle(n, nstep, scatter):
     from specfempp_amd import Engine3D, meshes
 
@@ -162,17 +162,17 @@ def test_3d_layered_matches_oracle(n, nstep, scatter):
             assert e32 <= TOL_SEISMO
 
 
-def test_3d_seam_path_and_fields_match_fused_path():
+@pytest.mark.parametrize("n,scatter", [(5, "gather"), (6, "block")])
+def test_3d_seam_path_and_fields_match_fused_path(n, scatter):
     """Seam calls (predictor / update_wavefields / corrector / seismograms) on
-    the 3-D gather path equal the fused step, and the wavefield read back after
-    fused steps (including the lazily assembled acceleration) equals the seam
-    path's; the gather path sums in a fixed order, so two fused runs agree bit
-    for bit."""
+    the 3-D gather and block paths equal the fused step, and the wavefield read
+    back after fused steps (including the acceleration) equals the seam path's;
+    both paths sum in a fixed order, so two fused runs agree bit for bit."""
     from specfempp_amd import Engine3D, meshes
 
-    w = meshes.config3(n=5, nstep=300, h=100.0, nrec_side=3)
+    w = meshes.config3(n=n, nstep=300, h=100.0, nrec_side=3)
     types = ("displacement", "velocity", "acceleration")
-    eng = Engine3D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, 1, types)
+    eng = Engine3D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, 1, types, scatter=scatter)
     eng.initialize()
     for istep in range(w.nstep):
         eng.apply_predictor_phase()
@@ -199,3 +199,76 @@ def test_3d_seam_path_and_fields_match_fused_path():
         assert err <= bar
     assert rel_l2(a, seam_fields[2]) <= 2e-4
     assert rel_l2(v, seam_fields[1]) <= 2e-5
+
+
+def _box_workload():
+    """8 x 6 x 4 elements (2-element blocks along every axis), two seeded
+    materials, Stacey on all six faces, a force and a moment-tensor source, and
+    receivers spread through the volume: on block faces, edges and corners and
+    strictly inside blocks."""
+    from specfempp_amd import meshes
+
+    rng = np.random.default_rng(11)
+    nx, ny, nz, h = 8, 6, 4, 100.0
+    material = rng.integers(0, 2, size=(nz, ny, nx)).astype(np.int32)
+    mesh = meshes.hex_mesh(nx, ny, nz, h, material, [2000.0, 2500.0], [3000.0, 4200.0], [1700.0, 2400.0],
+                           absorbing=(1, 1, 1, 1, 1, 1))
+    M = meshes.moment_tensor(5)
+    src = [dict(type="force", x=330.0, y=270.0, z=190.0, fx=0.3, fy=-0.5, fz=0.8, f0=12.0, factor=1e10),
+           dict(type="moment_tensor", x=610.0, y=420.0, z=230.0, mxx=M[0, 0], myy=M[1, 1], mzz=M[2, 2],
+                mxy=M[0, 1], mxz=M[0, 2], myz=M[1, 2], f0=12.0, factor=1e12, tshift=0.01)]
+    pts = [(400.0, 300.0, 200.0), (200.0, 200.0, 200.0), (150.0, 250.0, 120.0), (575.0, 125.0, 330.0),
+           (400.0, 440.0, 60.0), (720.0, 530.0, 390.0), (50.0, 50.0, 10.0), (800.0, 600.0, 400.0)]
+    rec = [dict(x=x, y=y, z=z) for x, y, z in pts]
+    return mesh, src, rec, 1e-3, 400
+
+
+def test_3d_block_box_all_absorbing_matches_oracle():
+    from specfempp_amd import Engine3D
+
+    mesh, src, rec, dt, nstep = _box_workload()
+    types = ("displacement", "velocity", "acceleration")
+    o32 = oracle.Oracle3D(mesh, 4, dt)
+    s32, _ = o32.forward(src, rec, nstep, dt, 1, types)
+    s64, _ = oracle.Oracle3D(mesh, 8, dt).forward(src, rec, nstep, dt, 1, types)
+    runs = {}
+    for scatter in ("block", "gather"):
+        eng = Engine3D(mesh, src, rec, dt, nstep, 1, types, scatter=scatter)
+        assert np.array_equal(eng.index_mapping(), o32.ibool())
+        eng.initialize()
+        eng.run()
+        runs[scatter] = eng.seismograms().astype(np.float64)
+        eng.close()
+    for it, t in enumerate(types):
+        ours = runs["block"][:, it]
+        e32, e64 = rel_l2(ours, s32[:, it]), rel_l2(ours, s64[:, it])
+        floor = rel_l2(s32[:, it], s64[:, it])
+        _report(f"3d_box_block/{t}", vs_oracle_fp32=e32, vs_oracle_fp64=e64, fp32_floor=floor,
+                vs_gather=rel_l2(ours, runs["gather"][:, it]))
+        _check(e32, e64, floor)
+        if t == "displacement":
+            assert e32 <= TOL_SEISMO
+
+
+def test_3d_block_step_calls_equal_run_bitwise():
+    """Stepping one call per step stores the full acceleration after every
+    step; the displacement and velocity updates are the same instructions, so
+    the seismograms equal those of one run() call bit for bit."""
+    from specfempp_amd import Engine3D
+
+    mesh, src, rec, dt, _ = _box_workload()
+    nstep = 150
+    types = ("displacement", "velocity", "acceleration")
+    eng = Engine3D(mesh, src, rec, dt, nstep, 1, types, scatter="block")
+    eng.initialize()
+    eng.run()
+    run = eng.seismograms()
+    a_run = eng.field("acceleration")
+    eng.initialize()
+    for istep in range(nstep):
+        eng.step(istep)
+    stepped = eng.seismograms()
+    a_step = eng.field("acceleration")
+    eng.close()
+    assert np.array_equal(run, stepped)
+    assert np.array_equal(a_run, a_step)

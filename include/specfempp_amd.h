@@ -93,6 +93,9 @@ typedef struct {
 #define SFX_DISPLACEMENT 0
 #define SFX_VELOCITY 1
 #define SFX_ACCELERATION 2
+/* -(lambda + 2/3 mu) div u at the receiver (wavefield.hpp:49-85), 2-D; its
+ * second component is 0. */
+#define SFX_PRESSURE 3
 
 typedef struct {
   double dt;
@@ -197,7 +200,11 @@ int sfx_set_field(sfx_engine *engine, int which, const float *in);
  *   boundary_values, compute_stiffness_interaction.tpp:251-255;
  *   boundary_medium_container.hpp:26-82) and the final corrected wavefield
  *   (the ForwardWavefield written by io/wavefield/writer.tpp).  Replaces a
- *   forward run with the wavefield writer task.
+ *   forward run with the wavefield writer task.  snapshot_every > 0 also keeps
+ *   the corrected wavefield after steps 0, k, 2k, ... (< nstep - 1) in device
+ *   memory; the combined run resets its backward field to them, bounding the
+ *   drift of the time-reversed reconstruction (0: pure time reversal, as the
+ *   reference).
  * sfx_set_adjoint_sources: adjoint sources (src/source/adjoint_source.cpp:9-90)
  *   at `positions` (x, z used) with external time functions
  *   (src/source_time_function/external.cpp) stf (nadj, nstep, 2), in forward
@@ -212,10 +219,45 @@ int sfx_set_field(sfx_engine *engine, int which, const float *in);
  * sfx_get_kernels: kernels (6, nspec, ngll, ngll) in mesh element order, in the
  *   order rho, mu, kappa, rhop, alpha, beta (point::kernels,
  *   core/specfem/point/kernels.hpp:55-332; frechet_derivative.hpp:14-138). */
-int sfx_forward_record(sfx_engine *engine);
+int sfx_forward_record(sfx_engine *engine, int snapshot_every);
+/* Replace the per-point properties rho, kappa (bulk modulus), mu, each
+ * (nspec, ngll, ngll) in mesh element order, as a model update of an inversion
+ * does (the reference rebuilds compute::properties from a new model,
+ * src/compute/compute_properties.cpp, with the mass matrix and the Stacey
+ * impedances that depend on them).  2-D. */
+int sfx_set_properties(sfx_engine *engine, const float *rho, const float *kappa, const float *mu);
 int sfx_set_adjoint_sources(sfx_engine *engine, int nadj, const sfx_receiver *positions, const float *stf);
 int sfx_adjoint_run(sfx_engine *engine);
 int sfx_get_kernels(const sfx_engine *engine, float *out);
+/* Integration weight w_i w_j J of every element point, (nspec, ngll, ngll) in
+ * mesh element order (the factor of compute_mass_matrix.tpp:99-101): a kernel
+ * density K at a point times this weight is the derivative of the misfit with
+ * respect to the logarithm of that point's parameter.  2-D. */
+int sfx_get_point_weights(const sfx_engine *engine, float *out);
+
+/* ---- batched runs (2-D) -------------------------------------------------------
+ * The reference runs one simulation per process (time_marching<forward>::run,
+ * time_marching.tpp:9-124).  A batch advances many independent 2-D
+ * simulations (meshes of any size, own sources, receivers, dt and step count)
+ * in the same launches: one launch per kernel family per step covers every
+ * simulation still running, and runs of steps between two simulation ends
+ * replay one captured HIP graph.  Engines are only read at creation (the batch
+ * copies their data); they must record the same seismogram types.
+ * sfx_batch_tune: times elements per workgroup (2, 5, 10) with and without
+ *   graphs on the batch itself and keeps the fastest; choice3 = (elements per
+ *   workgroup, graphs on, colors).  sfx_batch_run starts from a zero state.
+ * sfx_batch_get_seismograms: simulation i (creation order), laid out as
+ *   sfx_get_seismograms of its engine.
+ * sfx_batch_get_info: last run's device time (ms), elements per workgroup,
+ *   graphs on, colors, padded points, total point-steps. */
+typedef struct sfx_batch sfx_batch;
+int sfx_batch_create(int n, sfx_engine *const *engines, sfx_batch **out);
+void sfx_batch_destroy(sfx_batch *batch);
+int sfx_batch_tune(sfx_batch *batch, int steps_per_candidate, int *choice3);
+int sfx_batch_set_config(sfx_batch *batch, int elems_per_block, int use_graph);
+int sfx_batch_run(sfx_batch *batch);
+int sfx_batch_get_seismograms(const sfx_batch *batch, int i, float *out);
+int sfx_batch_get_info(const sfx_batch *batch, double *out6);
 
 /* ---- measurement -------------------------------------------------------------
  * Average device time (ms) of each kernel family over the last sfx_run, taken

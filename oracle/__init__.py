@@ -51,12 +51,13 @@ def load():
                                   _dp, C.POINTER(C.c_double), C.c_int, C.c_void_p]
     lib.orc2d_apply_stiffness.argtypes = [vp, _dp, _dp, _dp]
     lib.orc2d_nstacey.argtypes = [vp]
+    lib.orc2d_sample.argtypes = [vp, C.c_int, _dp, C.c_int, _ip, _dp, _dp]
     lib.orc2d_set_properties.argtypes = [vp, _dp, _dp, _dp]
     lib.orc2d_properties.argtypes = [vp, _dp, _dp, _dp, _dp]
     lib.orc2d_forward_record.argtypes = [vp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_double, C.c_int, C.c_int,
-                                         _ip, _dp, C.POINTER(C.c_double), C.c_int, _dp, _dp]
+                                         _ip, _dp, C.POINTER(C.c_double), C.c_int, _dp, _dp, C.c_int, _dp]
     lib.orc2d_combined.argtypes = [vp, C.c_int, _dp, C.c_int, _dp, _dp, C.c_int, _dp, C.c_int, C.c_double,
-                                   C.c_int, C.c_int, _ip, _dp, _dp, _dp, _dp, C.c_int]
+                                   C.c_int, C.c_int, _ip, _dp, _dp, _dp, _dp, C.c_int, C.c_int, _dp]
     lib.orc3d_last_error.restype = C.c_char_p
     lib.orc3d_create.argtypes = [C.c_int] + [C.c_int] * 3 + [C.c_double] * 6 + [_ip, _dp, _dp, _dp, _ip, C.c_double,
                                                                                  C.POINTER(vp)]
@@ -80,7 +81,7 @@ def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
-SEISMO_TYPES = {"displacement": 0, "velocity": 1, "acceleration": 2}
+SEISMO_TYPES = {"displacement": 0, "velocity": 1, "acceleration": 2, "pressure": 3}
 
 
 class Oracle2D:
@@ -160,6 +161,15 @@ class Oracle2D:
             rec[i] = [r["x"], r["z"], r.get("angle", 0.0)]
         return rec
 
+    def sample(self, receivers, fields, types=("displacement",)):
+        """Seismogram values (ntypes, nrec, 2) of given fields (3, 2, nglob) [u, v, a]."""
+        rec = self._rec_rows(receivers)
+        tcodes = np.array([SEISMO_TYPES[t] for t in types], dtype=np.int32)
+        out = np.zeros((len(types), len(receivers), 2))
+        _check(load().orc2d_sample(self.h, len(receivers), rec, len(types), tcodes,
+                                   np.ascontiguousarray(fields, dtype=np.float64), out))
+        return out
+
     def properties(self):
         """(rho, kappa, mu, w_i w_j J), each (nspec, 5, 5) in mesh element order."""
         out = [np.zeros((self.nspec, 5, 5)) for _ in range(4)]
@@ -176,10 +186,12 @@ class Oracle2D:
         return load().orc2d_nstacey(self.h)
 
     def forward_record(self, sources, receivers, nstep: int, dt: float, nsb: int = 1,
-                       types=("displacement",), nthreads: int = 1):
+                       types=("displacement",), nthreads: int = 1, snapshot_every: int = 0):
         """Forward run that keeps what a combined run needs: returns
         (seismograms, t0, final fields (3, 2, nglob), Stacey-element record
-        (nstep, nstacey, 5, 5, 2))."""
+        (nstep, nstacey, 5, 5, 2)) and, with snapshot_every > 0, the
+        checkpoints (nsnap, 3, 2, nglob) of steps 0, k, 2k, ... < nstep - 1 as a
+        fifth item."""
         src, rec = self._src_rows(sources), self._rec_rows(receivers)
         tcodes = np.array([SEISMO_TYPES[t] for t in types], dtype=np.int32)
         nsig = -(-nstep // nsb)
@@ -187,12 +199,18 @@ class Oracle2D:
         t0 = C.c_double()
         fields = np.zeros((3, 2, self.nglob))
         record = np.zeros((nstep, self.nstacey, 5, 5, 2))
+        nsnap = (nstep - 2) // snapshot_every + 1 if snapshot_every > 0 else 0
+        snaps = np.zeros((max(nsnap, 1), 3, 2, self.nglob))
         _check(load().orc2d_forward_record(self.h, len(sources), src, len(receivers), rec, nstep, dt, nsb,
-                                           len(types), tcodes, seis, C.byref(t0), nthreads, fields, record))
+                                           len(types), tcodes, seis, C.byref(t0), nthreads, fields, record,
+                                           snapshot_every, snaps))
+        if snapshot_every > 0:
+            return seis, t0.value, fields, record, snaps[:nsnap]
         return seis, t0.value, fields, record
 
     def combined(self, sources, adjoint_positions, adjoint_stf, receivers, nstep: int, dt: float, fields, record,
-                 nsb: int = 1, types=("displacement",), nthreads: int = 1):
+                 nsb: int = 1, types=("displacement",), nthreads: int = 1, snapshot_every: int = 0,
+                 snapshots=None):
         """Combined adjoint + backward run; adjoint_stf (nadj, nstep, 2) in
         forward time order.  Returns (kernels (6, nspec, 5, 5) mesh order: rho,
         mu, kappa, rhop, alpha, beta; backward seismograms, latest first)."""
@@ -208,7 +226,10 @@ class Oracle2D:
         _check(load().orc2d_combined(self.h, len(sources), src, len(adjoint_positions), adj, stf, len(receivers),
                                      rec, nstep, dt, nsb, len(types), tcodes,
                                      np.ascontiguousarray(fields, dtype=np.float64),
-                                     np.ascontiguousarray(record, dtype=np.float64), kernels, seis, nthreads))
+                                     np.ascontiguousarray(record, dtype=np.float64), kernels, seis, nthreads,
+                                     snapshot_every,
+                                     np.ascontiguousarray(snapshots if snapshots is not None else np.zeros(1),
+                                                          dtype=np.float64)))
         return kernels, seis
 
     def forward(self, sources, receivers, nstep: int, dt: float, nsb: int = 1,

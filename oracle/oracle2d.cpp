@@ -996,12 +996,57 @@ void update_forces(const Assembly<T> &a, Field<T> &f, const Forcing<T> &F, int i
   }
 }
 
+// Pressure at the points of element e (include/medium/dim2/elastic/isotropic/
+// wavefield.hpp:49-85): -(lambda + 2/3 mu) (du_x/dx + du_z/dz), the factor in
+// double as the reference's double constant promotes it.
+template <class T> void element_pressure(const Assembly<T> &a, const Field<T> &f, int e, T out[NGLL][NGLL]) {
+  const int nglob = f.nglob;
+  const auto &q = a.q;
+  T lx[NGLL][NGLL], lz[NGLL][NGLL];
+  for (int iz = 0; iz < NGLL; iz++)
+    for (int ix = 0; ix < NGLL; ix++) {
+      const int g = a.medium_index[a.ibool[pidx<T>(e, iz, ix)]];
+      lx[iz][ix] = f.u[g];
+      lz[iz][ix] = f.u[static_cast<size_t>(nglob) + g];
+    }
+  for (int iz = 0; iz < NGLL; iz++)
+    for (int ix = 0; ix < NGLL; ix++) {
+      T dxi[2] = {0.0, 0.0}, dgam[2] = {0.0, 0.0};
+      for (int l = 0; l < NGLL; l++) {
+        dxi[0] += q.hprime[ix][l] * lx[iz][l];
+        dgam[0] += q.hprime[iz][l] * lx[l][ix];
+        dxi[1] += q.hprime[ix][l] * lz[iz][l];
+        dgam[1] += q.hprime[iz][l] * lz[l][ix];
+      }
+      const size_t p = pidx<T>(e, iz, ix);
+      const T du00 = a.xix[p] * dxi[0] + a.gammax[p] * dgam[0];
+      const T du11 = a.xiz[p] * dxi[1] + a.gammaz[p] * dgam[1];
+      const T mu = a.mu[p];
+      const T lp2m = a.kappa[p] + static_cast<T>(4.0 / 3.0) * mu;
+      const T lambda = lp2m - static_cast<T>(2.0) * mu;
+      out[iz][ix] = static_cast<T>(-1.0 * ((lambda + (2.0 / 3.0) * mu) * (du00 + du11)));
+    }
+}
+
 // Seismograms (compute_seismogram.tpp:103-135; include/algorithms/interpolate.hpp:176-237).
 template <class T>
 void seismograms(const Assembly<T> &a, const Field<T> &f, const Receivers<T> &R, int ntypes, const int *types,
                  int isig, double *seis_out) {
   const int nglob = f.nglob, nrec = static_cast<int>(R.ispec.size());
   for (int it = 0; it < ntypes; it++) {
+    if (types[it] == 3) {
+      for (int r = 0; r < nrec; r++) {
+        T pr[NGLL][NGLL];
+        element_pressure<T>(a, f, R.ispec[r], pr);
+        T s0 = 0;
+        for (int iz = 0; iz < NGLL; iz++)
+          for (int ix = 0; ix < NGLL; ix++) s0 += R.h[(static_cast<size_t>(r) * NGLL + iz) * NGLL + ix] * pr[iz][ix];
+        const size_t o = ((static_cast<size_t>(isig) * ntypes + it) * nrec + r) * 2;
+        seis_out[o] = s0;
+        seis_out[o + 1] = 0.0;
+      }
+      continue;
+    }
     const std::vector<T> &fld = types[it] == 0 ? f.u : (types[it] == 1 ? f.v : f.acc);
     const T *fx = fld.data(), *fz = fld.data() + nglob;
     for (int r = 0; r < nrec; r++) {
@@ -1020,6 +1065,20 @@ void seismograms(const Assembly<T> &a, const Field<T> &f, const Receivers<T> &R,
   }
 }
 
+// Snapshot of one field, (3, 2, nglob): u, v, a.
+template <class T> void store_state(const Field<T> &f, T *out) {
+  const size_t n = 2 * static_cast<size_t>(f.nglob);
+  std::copy(f.u.begin(), f.u.end(), out);
+  std::copy(f.v.begin(), f.v.end(), out + n);
+  std::copy(f.acc.begin(), f.acc.end(), out + 2 * n);
+}
+template <class T> void load_state(Field<T> &f, const T *in) {
+  const size_t n = 2 * static_cast<size_t>(f.nglob);
+  std::copy(in, in + n, f.u.begin());
+  std::copy(in + n, in + 2 * n, f.v.begin());
+  std::copy(in + 2 * n, in + 3 * n, f.acc.begin());
+}
+
 // Forward simulation (time_marching.tpp:9-124).  Seismogram output layout:
 // (isig, itype, irec, 2) as the reference's (max_sig_step, ntypes, nrec, 2)
 // array before rotation; types 0 displacement, 1 velocity, 2 acceleration.
@@ -1027,7 +1086,8 @@ void seismograms(const Assembly<T> &a, const Field<T> &f, const Receivers<T> &R,
 template <class T>
 void run_forward(const Assembly<T> &a, int nsrc, const SourceIn *src, int nrec, const ReceiverIn *rec, int nstep,
                  double dt_in, int nsb, int ntypes, const int *types, double *seis_out, double *t0_out,
-                 int nthreads, T *final_fields /* optional (3, 2, nglob) */, T *record = nullptr) {
+                 int nthreads, T *final_fields /* optional (3, 2, nglob) */, T *record = nullptr,
+                 int snapshot_every = 0, T *snapshots = nullptr /* (nsnap, 3, 2, nglob) */) {
   const T dt = static_cast<T>(dt_in);
   const T deltat = dt, deltatover2 = dt / 2.0, deltasquareover2 = dt * dt / 2.0;
   T t0 = 0;
@@ -1047,6 +1107,8 @@ void run_forward(const Assembly<T> &a, int nsrc, const SourceIn *src, int nrec, 
                      record ? record + istep * rec_stride : nullptr, serial);
     divide_correct<T>(a, f, deltatover2, serial);
     if (istep % nsb == 0) seismograms<T>(a, f, R, ntypes, types, isig++, seis_out);
+    if (snapshots && snapshot_every > 0 && istep % snapshot_every == 0 && istep < nstep - 1)
+      store_state<T>(f, snapshots + static_cast<size_t>(istep / snapshot_every) * 6 * a.nglob);
   }
   if (final_fields) {
     std::copy(f.u.begin(), f.u.end(), final_fields);
@@ -1134,7 +1196,7 @@ template <class T>
 void run_combined(const Assembly<T> &a, int nsrc, const SourceIn *src, int nadj, const ReceiverIn *adj_pos,
                   const double *adj_stf, int nrec, const ReceiverIn *rec, int nstep, double dt_in, int nsb,
                   int ntypes, const int *types, const T *final_fields, const T *record, T *K, double *seis_out,
-                  int nthreads) {
+                  int nthreads, int snapshot_every = 0, const T *snapshots = nullptr) {
   const T dt = static_cast<T>(dt_in);
   const T deltat = dt, deltatover2 = dt / 2.0, deltasquareover2 = dt * dt / 2.0;
   const Forcing<T> S = make_sources<T>(a, nsrc, src, nstep, dt, nullptr);
@@ -1157,12 +1219,11 @@ void run_combined(const Assembly<T> &a, int nsrc, const SourceIn *src, int nadj,
     scratch.assign(record + istep * rec_stride, record + (istep + 1) * rec_stride);
     update_forces<T>(a, bwd, S, istep, StaceyMode::replay, scratch.data(), serial);
     divide_correct<T>(a, bwd, -deltatover2, serial);
-    if (istep == nstep - 1) {
-      const size_t n = 2 * static_cast<size_t>(a.nglob);
-      std::copy(final_fields, final_fields + n, bwd.u.begin());
-      std::copy(final_fields + n, final_fields + 2 * n, bwd.v.begin());
-      std::copy(final_fields + 2 * n, final_fields + 3 * n, bwd.acc.begin());
-    }
+    if (istep == nstep - 1) load_state<T>(bwd, final_fields);
+    // checkpoint: the backward state after istep is the forward state after
+    // step istep; reset it from the snapshot taken in the forward run
+    if (snapshots && snapshot_every > 0 && istep % snapshot_every == 0 && istep < nstep - 1)
+      load_state<T>(bwd, snapshots + static_cast<size_t>(istep / snapshot_every) * 6 * a.nglob);
     frechet_step<T>(a, adj, bwd, dt, K, serial);
     if (istep % nsb == 0) seismograms<T>(a, bwd, R, ntypes, types, isig++, seis_out);
   }
@@ -1349,6 +1410,27 @@ int orc2d_forward(void *h, int nsrc, const double *src, int nrec, const double *
   });
 }
 
+// Sample given fields (3, 2, nglob) [u, v, a] at receivers rec (nrec, 3):
+// out (ntypes, nrec, 2).
+int orc2d_sample(void *h, int nrec, const double *rec, int ntypes, const int *types, const double *fields,
+                 double *out) {
+  return guard([&] {
+    auto R = receivers_in(nrec, rec);
+    DISPATCH(h, {
+      auto &a = asm_of<T>(h);
+      Field<T> f(a.nglob);
+      const size_t n = 2 * static_cast<size_t>(a.nglob);
+      for (size_t i = 0; i < n; i++) {
+        f.u[i] = static_cast<T>(fields[i]);
+        f.v[i] = static_cast<T>(fields[n + i]);
+        f.acc[i] = static_cast<T>(fields[2 * n + i]);
+      }
+      const Receivers<T> RR = make_receivers<T>(a, nrec, R.data());
+      seismograms<T>(a, f, RR, ntypes, types, 0, out);
+    });
+  });
+}
+
 // Replace the per-point properties (nspec, 5, 5) in mesh element order and
 // rebuild the mass matrix (finite-difference checks of the kernels).
 void orc2d_set_properties(void *h, const double *rho, const double *kappa, const double *mu) {
@@ -1395,7 +1477,7 @@ int orc2d_nstacey(void *h) {
 // Stacey-element record (nstep, nstacey, 5, 5, 2) for a combined run.
 int orc2d_forward_record(void *h, int nsrc, const double *src, int nrec, const double *rec, int nstep, double dt,
                          int nsb, int ntypes, const int *types, double *seis, double *t0, int nthreads,
-                         double *final_fields, double *record) {
+                         double *final_fields, double *record, int snapshot_every, double *snapshots) {
   return guard([&] {
     auto S = sources_in(nsrc, src);
     auto R = receivers_in(nrec, rec);
@@ -1407,10 +1489,13 @@ int orc2d_forward_record(void *h, int nsrc, const double *src, int nrec, const d
             throw std::logic_error("Stacey elements are not the last in compute order");
       std::vector<T> ff(6 * static_cast<size_t>(a.nglob));
       std::vector<T> rr(static_cast<size_t>(nstep) * (a.nspec - a.nspec_none) * NGLL * NGLL * 2);
+      const int nsnap = snapshot_every > 0 ? (nstep - 2) / snapshot_every + 1 : 0;
+      std::vector<T> ss(static_cast<size_t>(nsnap) * 6 * a.nglob);
       run_forward<T>(a, nsrc, S.data(), nrec, R.data(), nstep, dt, nsb, ntypes, types, seis, t0, nthreads,
-                     ff.data(), rr.data());
+                     ff.data(), rr.data(), snapshot_every, ss.data());
       for (size_t i = 0; i < ff.size(); i++) final_fields[i] = ff[i];
       for (size_t i = 0; i < rr.size(); i++) record[i] = rr[i];
+      for (size_t i = 0; i < ss.size(); i++) snapshots[i] = ss[i];
     });
   });
 }
@@ -1421,7 +1506,8 @@ int orc2d_forward_record(void *h, int nsrc, const double *src, int nrec, const d
 // the backward field, latest sample first.
 int orc2d_combined(void *h, int nsrc, const double *src, int nadj, const double *adj_pos, const double *adj_stf,
                    int nrec, const double *rec, int nstep, double dt, int nsb, int ntypes, const int *types,
-                   const double *final_fields, const double *record, double *kernels, double *seis, int nthreads) {
+                   const double *final_fields, const double *record, double *kernels, double *seis, int nthreads,
+                   int snapshot_every, const double *snapshots) {
   return guard([&] {
     auto S = sources_in(nsrc, src);
     auto A = receivers_in(nadj, adj_pos);
@@ -1432,8 +1518,10 @@ int orc2d_combined(void *h, int nsrc, const double *src, int nadj, const double 
       std::vector<T> rr(record, record + static_cast<size_t>(nstep) * (a.nspec - a.nspec_none) * NGLL * NGLL * 2);
       const size_t npts = static_cast<size_t>(a.nspec) * NGLL * NGLL;
       std::vector<T> K(6 * npts, 0);
+      const int nsnap = snapshot_every > 0 ? (nstep - 2) / snapshot_every + 1 : 0;
+      std::vector<T> ss(snapshots, snapshots + static_cast<size_t>(nsnap) * 6 * a.nglob);
       run_combined<T>(a, nsrc, S.data(), nadj, A.data(), adj_stf, nrec, R.data(), nstep, dt, nsb, ntypes, types,
-                      ff.data(), rr.data(), K.data(), seis, nthreads);
+                      ff.data(), rr.data(), K.data(), seis, nthreads, snapshot_every, ss.data());
       for (int k = 0; k < 6; k++)
         for (int c = 0; c < a.nspec; c++)
           for (int p = 0; p < NGLL * NGLL; p++)

@@ -16,7 +16,7 @@ LIB_DIR = os.path.join(HERE, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libspecfempp_amd.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "specfempp_amd.h")
 
-SOURCES = ["engine.hip", "kernels.hip", "block3d.hip", "assembly.cpp"]
+SOURCES = ["engine.hip", "kernels.hip", "block3d.hip", "batch.hip", "assembly.cpp"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions stay off: some of their forms have given wrong
@@ -80,10 +80,19 @@ def load():
         "sfx_get_field": ([vp, i32, vp], i32),
         "sfx_set_field": ([vp, i32, vp], i32),
         "sfx_get_timings": ([vp, vp], i32),
-        "sfx_forward_record": ([vp], i32),
+        "sfx_forward_record": ([vp, i32], i32),
+        "sfx_set_properties": ([vp, vp, vp, vp], i32),
         "sfx_set_adjoint_sources": ([vp, i32, vp, vp], i32),
         "sfx_adjoint_run": ([vp], i32),
         "sfx_get_kernels": ([vp, vp], i32),
+        "sfx_get_point_weights": ([vp, vp], i32),
+        "sfx_batch_create": ([i32, vp, vp], i32),
+        "sfx_batch_destroy": ([vp], None),
+        "sfx_batch_tune": ([vp, i32, vp], i32),
+        "sfx_batch_set_config": ([vp, i32, i32], i32),
+        "sfx_batch_run": ([vp], i32),
+        "sfx_batch_get_seismograms": ([vp, i32, vp], i32),
+        "sfx_batch_get_info": ([vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -259,7 +259,7 @@ void check_run(const sfx_run_desc &run) {
   if (run.nstep_between_samples < 1) throw std::invalid_argument("nstep_between_samples must be >= 1");
   if (run.nseismo_types < 0 || run.nseismo_types > 3) throw std::invalid_argument("nseismo_types must be 0..3");
   for (int i = 0; i < run.nseismo_types; i++)
-    if (run.seismo_types[i] < 0 || run.seismo_types[i] > 2) throw std::invalid_argument("bad seismogram type");
+    if (run.seismo_types[i] < 0 || run.seismo_types[i] > 3) throw std::invalid_argument("bad seismogram type");
   if (run.chunk_size < 1) throw std::invalid_argument("chunk_size must be >= 1");
   if (run.scatter != SFX_SCATTER_COLORED && run.scatter != SFX_SCATTER_ATOMIC && run.scatter != SFX_SCATTER_GATHER &&
       run.scatter != SFX_SCATTER_BLOCK)
@@ -271,6 +271,59 @@ void check_run(const sfx_run_desc &run) {
 // ---------------------------------------------------------------------------
 // 2-D
 // ---------------------------------------------------------------------------
+// Mass matrix and inverse (compute_mass_matrix.tpp:80-114,
+// invert_mass_matrix.tpp:40-47), accumulated in element order, with the
+// Stacey term at the points of the Stacey entries.
+void mass_2d(HostAssembly &a, double dt) {
+  const int npe = 25;
+  std::vector<int> bnd_of(static_cast<size_t>(a.nspec) * npe, -1);
+  for (size_t q = 0; q < a.bnd_slot.size(); q++) bnd_of[a.bnd_slot[q]] = static_cast<int>(q);
+  std::vector<float> mass(2 * static_cast<size_t>(a.nglob), 0.0f);
+  const float vfac = -1.0f * static_cast<float>(dt) * 0.5f;
+  for (int e = 0; e < a.nspec; e++)
+    for (int iz = 0; iz < NGLL; iz++)
+      for (int ix = 0; ix < NGLL; ix++) {
+        const size_t p = static_cast<size_t>(e) * npe + iz * NGLL + ix;
+        float mm[2] = {a.rho[p], a.rho[p]};
+        const float f = a.w[ix] * a.w[iz] * a.jac[p];
+        mm[0] *= f;
+        mm[1] *= f;
+        const int k = bnd_of[p];
+        if (k >= 0) stacey_term<2>(&a.bnd_normal[2 * k], a.bnd_weight[k], a.bnd_rho_vp[k], a.bnd_rho_vs[k], vfac, mm);
+        const int gi = a.gather[p];
+        mass[gi] += mm[0];
+        mass[a.nglob + gi] += mm[1];
+      }
+  a.mass_inverse.assign(2 * static_cast<size_t>(a.nglob_pad), 0.0f);
+  for (int c = 0; c < 2; c++)
+    for (int g = 0; g < a.nglob; g++)
+      a.mass_inverse[static_cast<size_t>(c) * a.nglob_pad + g] = 1.0f / mass[static_cast<size_t>(c) * a.nglob + g];
+}
+
+// New per-point properties (nspec, 25) in mesh element order: properties,
+// Stacey impedances (stacey.cpp:64-181) and mass matrix are rebuilt.
+void set_properties_2d(HostAssembly &a, const float *rho, const float *kappa, const float *mu, double dt) {
+  if (a.dim != 2) throw std::invalid_argument("set_properties is implemented for 2-D meshes");
+  if (!rho || !kappa || !mu) throw std::invalid_argument("null property array");
+  const int npe = 25;
+  for (int e = 0; e < a.nspec; e++)
+    for (int p = 0; p < npe; p++) {
+      const size_t src = static_cast<size_t>(a.compute_to_mesh[e]) * npe + p, dst = static_cast<size_t>(e) * npe + p;
+      if (!(rho[src] > 0.0f) || !(mu[src] > 0.0f) || !(kappa[src] + (4.0f / 3.0f) * mu[src] > 0.0f))
+        throw std::invalid_argument("properties must give rho > 0, mu > 0 and lambda + 2 mu > 0");
+      a.rho[dst] = rho[src];
+      a.kappa[dst] = kappa[src];
+      a.mu[dst] = mu[src];
+    }
+  for (size_t q = 0; q < a.bnd_slot.size(); q++) {
+    const size_t p = a.bnd_slot[q];
+    const float lp2m = a.kappa[p] + static_cast<float>(4.0 / 3.0) * a.mu[p];
+    a.bnd_rho_vp[q] = std::sqrt(lp2m / a.rho[p]) * a.rho[p];
+    a.bnd_rho_vs[q] = std::sqrt(a.rho[p] * a.mu[p]);
+  }
+  mass_2d(a, dt);
+}
+
 HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *src, int nrec,
                                const sfx_receiver *rec, const sfx_run_desc &run) {
   check_run(run);
@@ -446,30 +499,7 @@ HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *
     }
   }
 
-  // Mass matrix and inverse (compute_mass_matrix.tpp:80-114,
-  // invert_mass_matrix.tpp:40-47), accumulated in element order.
-  {
-    std::vector<float> mass(2 * static_cast<size_t>(a.nglob), 0.0f);
-    const float vfac = -1.0f * static_cast<float>(run.dt) * 0.5f;
-    for (int e = 0; e < nspec; e++)
-      for (int iz = 0; iz < NGLL; iz++)
-        for (int ix = 0; ix < NGLL; ix++) {
-          const size_t p = static_cast<size_t>(e) * npe + iz * NGLL + ix;
-          float mm[2] = {a.rho[p], a.rho[p]};
-          const float f = a.w[ix] * a.w[iz] * a.jac[p];
-          mm[0] *= f;
-          mm[1] *= f;
-          const int k = bnd_of[p];
-          if (k >= 0) stacey_term<2>(&a.bnd_normal[2 * k], a.bnd_weight[k], a.bnd_rho_vp[k], a.bnd_rho_vs[k], vfac, mm);
-          const int gi = a.gather[p];
-          mass[gi] += mm[0];
-          mass[a.nglob + gi] += mm[1];
-        }
-    a.mass_inverse.assign(2 * static_cast<size_t>(a.nglob_pad), 0.0f);
-    for (int c = 0; c < 2; c++)
-      for (int g = 0; g < a.nglob; g++)
-        a.mass_inverse[static_cast<size_t>(c) * a.nglob_pad + g] = 1.0f / mass[static_cast<size_t>(c) * a.nglob + g];
-  }
+  mass_2d(a, run.dt);
 
   // Sources and receivers (locate_point; force_source.cpp:70-78;
   // moment_tensor_source.cpp:88-130; compute_receivers.cpp:59-83)
@@ -614,6 +644,8 @@ HostAssembly build_assembly_3d(const sfx_mesh3d &m, int nsrc, const sfx_source *
                                const sfx_receiver *rec, const sfx_run_desc &run) {
   check_run(run);
   if (m.nx <= 0 || m.ny <= 0 || m.nz <= 0) throw std::invalid_argument("empty 3-D mesh");
+  for (int i = 0; i < run.nseismo_types; i++)
+    if (run.seismo_types[i] == SFX_PRESSURE) throw std::invalid_argument("pressure seismograms are 2-D only");
   if (run.scatter == SFX_SCATTER_BLOCK && ((m.nx | m.ny | m.nz) & 1))
     throw std::invalid_argument("SFX_SCATTER_BLOCK needs even element counts nx, ny, nz");
   if (!(m.hx > 0 && m.hy > 0 && m.hz > 0)) throw std::invalid_argument("element sizes must be positive");

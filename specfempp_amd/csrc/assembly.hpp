@@ -88,6 +88,11 @@ struct HostAssembly {
 // compute order in a single color.
 void color_elements(HostAssembly &a, int scatter);
 
+// Replace the per-point properties ((nspec, 25), mesh element order) and
+// rebuild the Stacey impedances and the mass matrix (2-D).
+void set_properties_2d(HostAssembly &a, const float *rho, const float *kappa, const float *mu, double dt);
+void mass_2d(HostAssembly &a, double dt);
+
 // Locate adjoint sources (pos, nadj) with time functions stf (nadj, nstep, 2).
 void place_adjoint_sources_2d(HostAssembly &a, int nadj, const sfx_receiver *pos, const float *stf, int nstep);
 

@@ -18,48 +18,13 @@
 #include "assembly.hpp"
 #include "kernels.hpp"
 #include "lattice.hpp"
+#include "runtime.hpp"
 
 namespace sfx {
-namespace {
-
-thread_local std::string g_last_error;
-
-struct HipError : std::runtime_error {
-  using std::runtime_error::runtime_error;
-};
-
-#define SFX_HIP(call)                                                                                  \
-  do {                                                                                                 \
-    hipError_t err_ = (call);                                                                          \
-    if (err_ != hipSuccess)                                                                            \
-      throw HipError(std::string(#call) + " failed: " + hipGetErrorString(err_) + " (" __FILE__ ":" + \
-                     std::to_string(__LINE__) + ")");                                                  \
-  } while (0)
-
-template <class T> struct DevBuf {
-  T *p = nullptr;
-  size_t n = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf &) = delete;
-  DevBuf &operator=(const DevBuf &) = delete;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
-  void alloc(size_t count) {
-    if (p) {
-      (void)hipFree(p);
-      p = nullptr;
-    }
-    n = count;
-    if (count) SFX_HIP(hipMalloc(&p, count * sizeof(T)));
-  }
-  void upload(const std::vector<T> &v, hipStream_t s) {
-    alloc(v.size());
-    if (!v.empty()) SFX_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
-  }
-};
-
-}  // namespace
+std::string &last_error() {
+  static thread_local std::string msg;
+  return msg;
+}
 }  // namespace sfx
 
 using namespace sfx;
@@ -84,6 +49,8 @@ struct sfx_engine {
   DevBuf<int> rec_block; // 3-D block path: 1 for blocks holding a receiver element
   // 2-D adjoint: Stacey-element record, adjoint field, kernels, adjoint sources
   DevBuf<float> record;      // (nstep, nstacey, 25, 2)
+  DevBuf<float> snapshots;   // checkpoints (nsnap, 3, ncomp, nglob_pad): u, v, a after steps 0, k, 2k, ...
+  int snapshot_every = 0;
   DevBuf<int> rec_index;     // per slot: Stacey element index or -1
   DevBuf<int> bnd_rec;       // per Stacey entry: Stacey element index * 25 + point
   DevBuf<float> rho;         // per point, slot order
@@ -311,7 +278,8 @@ struct sfx_engine {
   void seismograms(Fields ff, int isig, int corrected) {
     if (isig < 0 || isig >= nsig) throw std::invalid_argument("seismogram step out of range");
     launch_seismograms(stream, h.dim, h.npe, h.nrec, run.nseismo_types, types.p, isig, rec_ispec.p, gather.p, rec_h.p,
-                       ff, nc.dto2, corrected, seismo.p);
+                       ff, nc.dto2, corrected, seismo.p, PointGeom2D{xix.p, xiz.p, gammax.p, gammaz.p, kappa.p, mu.p},
+                       q);
   }
   void require_init() const {
     if (!initialized) throw std::logic_error("sfx_initialize must be called first");
@@ -371,14 +339,27 @@ struct sfx_engine {
     if (istep % run.nstep_between_samples == 0) seismograms(ff, istep / run.nstep_between_samples, 1);
     if (evs) SFX_HIP(hipEventRecord(evs[3], stream));
     // a recorded run ends on the corrected state of its last step, the state
-    // the backward field starts from
-    if (recording && last)
+    // the backward field starts from; checkpoint steps keep their corrected
+    // state too
+    const bool snap = recording && is_checkpoint(istep);
+    if (recording && (last || snap)) {
       launch_divide_corrector(stream, h.ncomp, ff, nc, 1, 1);
-    else
+      if (snap) {
+        const size_t n3 = 3 * static_cast<size_t>(h.ncomp) * h.nglob_pad;
+        SFX_HIP(hipMemcpyAsync(snapshots.p + static_cast<size_t>(istep / snapshot_every) * n3, fields.p,
+                               n3 * sizeof(float), hipMemcpyDeviceToDevice, stream));
+      }
+      if (!last) launch_predictor(stream, h.ncomp, ff, nc);
+    } else {
       launch_update_fused(stream, h.ncomp, ff, nc);
+    }
     if (evs) SFX_HIP(hipEventRecord(evs[4], stream));
     rec_mode = 0;
     rec_step = nullptr;
+  }
+
+  bool is_checkpoint(int istep) const {
+    return snapshot_every > 0 && istep % snapshot_every == 0 && istep < run.nstep - 1;
   }
 
   // Combined adjoint + backward loop (time_marching.tpp:126-247), 2-D.  The
@@ -417,6 +398,13 @@ struct sfx_engine {
         rec_mode = 0;
         rec_step = nullptr;
         launch_divide_corrector(stream, h.ncomp, bwd, back, 1, 1);
+        // checkpoint: after istep the backward state is the forward state
+        // after step istep; reset it to bound the reconstruction's drift
+        if (is_checkpoint(istep)) {
+          const size_t n3 = 3 * stride;
+          SFX_HIP(hipMemcpyAsync(fields.p, snapshots.p + static_cast<size_t>(istep / snapshot_every) * n3,
+                                 n3 * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        }
       }
       launch_frechet_2d(stream, h.nspec, gather.p, xix.p, xiz.p, gammax.p, gammaz.p, kappa.p, mu.p, rho.p, q, adj, bwd,
                         dt, kern.p);
@@ -425,26 +413,12 @@ struct sfx_engine {
   }
 };
 
-namespace {
+namespace sfx {
+const HostAssembly &engine_host(const sfx_engine *e) { return e->h; }
+const sfx_run_desc &engine_run(const sfx_engine *e) { return e->run; }
+}  // namespace sfx
 
-template <class F> int guarded(F &&fn) {
-  try {
-    fn();
-    return SFX_OK;
-  } catch (const HipError &e) {
-    g_last_error = e.what();
-    return SFX_ERR_HIP;
-  } catch (const std::invalid_argument &e) {
-    g_last_error = e.what();
-    return SFX_ERR_ARG;
-  } catch (const std::logic_error &e) {
-    g_last_error = e.what();
-    return SFX_ERR_STATE;
-  } catch (const std::exception &e) {
-    g_last_error = e.what();
-    return SFX_ERR_MESH;
-  }
-}
+namespace {
 
 void check_engine(const sfx_engine *e) {
   if (!e) throw std::invalid_argument("null engine");
@@ -468,7 +442,7 @@ extern "C" {
 
 int sfx_version(void) { return 100; }
 
-const char *sfx_last_error(void) { return g_last_error.c_str(); }
+const char *sfx_last_error(void) { return last_error().c_str(); }
 
 int sfx_engine_create_2d(const sfx_mesh2d *mesh, int nsources, const sfx_source *sources, int nreceivers,
                          const sfx_receiver *receivers, const sfx_run_desc *run, sfx_engine **out) {
@@ -737,13 +711,37 @@ int sfx_set_field(sfx_engine *e, int which, const float *in) {
   });
 }
 
-int sfx_forward_record(sfx_engine *e) {
+int sfx_set_properties(sfx_engine *e, const float *rho, const float *kappa, const float *mu) {
+  return guarded([&] {
+    check_engine(e);
+    SFX_HIP(hipStreamSynchronize(e->stream));
+    set_properties_2d(e->h, rho, kappa, mu, e->run.dt);
+    hipStream_t s = e->stream;
+    e->kappa.upload(e->permuted(e->h.kappa), s);
+    e->mu.upload(e->permuted(e->h.mu), s);
+    e->rho.upload(e->permuted(e->h.rho), s);
+    e->bnd_rvp.upload(e->h.bnd_rho_vp, s);
+    e->bnd_rvs.upload(e->h.bnd_rho_vs, s);
+    const size_t stride = static_cast<size_t>(e->h.ncomp) * e->h.nglob_pad;
+    SFX_HIP(hipMemcpyAsync(e->fields.p + 3 * stride, e->h.mass_inverse.data(), stride * sizeof(float),
+                           hipMemcpyHostToDevice, s));
+    SFX_HIP(hipStreamSynchronize(s));
+    e->recorded = false;
+  });
+}
+
+int sfx_forward_record(sfx_engine *e, int snapshot_every) {
   return guarded([&] {
     check_engine(e);
     e->require_init();
     if (e->h.dim != 2) throw std::invalid_argument("adjoint runs are implemented for 2-D meshes");
+    if (snapshot_every < 0) throw std::invalid_argument("snapshot_every must be >= 0");
     const size_t need = static_cast<size_t>(e->run.nstep) * e->nstacey * e->h.npe * e->h.ncomp;
     if (e->record.n != need) e->record.alloc(need);
+    e->snapshot_every = snapshot_every;
+    const int nsnap = snapshot_every > 0 && e->run.nstep >= 2 ? (e->run.nstep - 2) / snapshot_every + 1 : 0;
+    const size_t nsnap_f = static_cast<size_t>(nsnap) * 3 * e->h.ncomp * e->h.nglob_pad;
+    if (e->snapshots.n != nsnap_f) e->snapshots.alloc(nsnap_f);
     e->recorded = false;
     e->recording = true;
     try {
@@ -783,6 +781,21 @@ int sfx_adjoint_run(sfx_engine *e) {
     e->adjoint_run();
     SFX_HIP(hipGetLastError());
     SFX_HIP(hipStreamSynchronize(e->stream));
+  });
+}
+
+int sfx_get_point_weights(const sfx_engine *e, float *out) {
+  return guarded([&] {
+    check_engine(e);
+    if (!out) throw std::invalid_argument("null output");
+    if (e->h.dim != 2) throw std::invalid_argument("point weights are implemented for 2-D meshes");
+    const size_t npe = e->h.npe;
+    for (int c = 0; c < e->h.nspec; c++)
+      for (int iz = 0; iz < 5; iz++)
+        for (int ix = 0; ix < 5; ix++) {
+          const size_t p = static_cast<size_t>(c) * npe + iz * 5 + ix;
+          out[static_cast<size_t>(e->h.compute_to_mesh[c]) * npe + iz * 5 + ix] = e->h.w[ix] * e->h.w[iz] * e->h.jac[p];
+        }
   });
 }
 

@@ -40,21 +40,24 @@ constexpr int E2 = 5;  // elements per block
 // added by the boundary kernel), rec_mode 2 replaces it by the stored value
 // (the backward field).  rec_index: Stacey element index of each slot, or -1;
 // rec: (nstacey, 25, 2) of the current step.
-template <bool ATOMIC>
-__global__ __launch_bounds__(128) void stiffness_2d_kernel(
+// Threads of a block holding E elements: E * 25 rounded up to whole wavefronts.
+constexpr int threads_for(int E) { return (E * 25 + 63) / 64 * 64; }
+
+template <bool ATOMIC, int E = E2>
+__global__ __launch_bounds__(threads_for(E)) void stiffness_2d_kernel(
     int first, int count, const int *__restrict__ gather, const float *__restrict__ xix_, const float *__restrict__ xiz_,
     const float *__restrict__ gammax_, const float *__restrict__ gammaz_, const float *__restrict__ jac_,
     const float *__restrict__ kappa_, const float *__restrict__ mu_, GllConst q, Fields f, int rec_mode,
     const int *__restrict__ rec_index, float *__restrict__ rec) {
-  __shared__ float su[E2][2][25];
-  __shared__ float sF[E2][4][25];  // F(c, d) as [2c + d]
+  __shared__ float su[E][2][25];
+  __shared__ float sF[E][4][25];  // F(c, d) as [2c + d]
 
   const int t = threadIdx.x;
   const int el = t / 25;
   const int p = t - el * 25;
   const int iz = p / 5, ix = p - (p / 5) * 5;
-  const int el_idx = blockIdx.x * E2 + el;
-  const bool active = (el < E2) && (el_idx < count);
+  const int el_idx = blockIdx.x * E + el;
+  const bool active = (el < E) && (el_idx < count);
   const int e = first + el_idx;
   const size_t idx = static_cast<size_t>(e) * 25 + p;
 
@@ -394,13 +397,40 @@ __global__ __launch_bounds__(128) void sources_kernel(int npe, int nsrc, int ist
 // Seismograms: one block per receiver, Lagrange interpolation of the
 // element field, block reduction.
 // ---------------------------------------------------------------------------
+// Pressure at element point p of 2-D element e (include/medium/dim2/elastic/
+// isotropic/wavefield.hpp:49-85): -(lambda + 2/3 mu) div u from the element's
+// displacement (su, staged in LDS), the per-point derivatives and properties;
+// the factor is formed in double as the reference's double constant makes it.
+__device__ __forceinline__ float pressure_2d(const PointGeom2D &pg, const GllConst &q, const float (*su)[25], int e,
+                                             int p) {
+  const int iz = p / 5, ix = p - (p / 5) * 5;
+  float dxi[2] = {0.0f, 0.0f}, dga[2] = {0.0f, 0.0f};
+#pragma unroll
+  for (int l = 0; l < 5; l++) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      dxi[c] += q.hp[ix][l] * su[c][iz * 5 + l];
+      dga[c] += q.hp[iz][l] * su[c][l * 5 + ix];
+    }
+  }
+  const size_t idx = static_cast<size_t>(e) * 25 + p;
+  const float du00 = pg.xix[idx] * dxi[0] + pg.gammax[idx] * dga[0];
+  const float du11 = pg.xiz[idx] * dxi[1] + pg.gammaz[idx] * dga[1];
+  const float mu = pg.mu[idx];
+  const float lp2m = pg.kappa[idx] + (4.0f / 3.0f) * mu;
+  const float lambda = lp2m - 2.0f * mu;
+  const double fac = static_cast<double>(lambda) + (2.0 / 3.0) * static_cast<double>(mu);
+  return static_cast<float>(-1.0 * (fac * static_cast<double>(du00 + du11)));
+}
+
 template <int D>
 __global__ __launch_bounds__(128) void seismogram_kernel(int npe, int nrec, int ntypes, const int *__restrict__ types,
                                                          int isig, const int *__restrict__ ispec,
                                                          const int *__restrict__ gather, const float *__restrict__ h,
                                                          Fields f, float dto2, int corrected,
-                                                         float *__restrict__ seismo) {
+                                                         float *__restrict__ seismo, PointGeom2D pg, GllConst q) {
   __shared__ float red[2][D][64];
+  __shared__ float su[2][25];
   const int r = blockIdx.x;
   const int p = threadIdx.x;
   const int np = f.nglob_pad;
@@ -413,6 +443,16 @@ __global__ __launch_bounds__(128) void seismogram_kernel(int npe, int nrec, int 
   for (int it = 0; it < ntypes; it++) {
     const int type = types[it];
     float val[D];
+    if (D == 2 && type == 3) {  // pressure: one value, second component 0
+      if (p < 25) {
+        su[0][p] = f.u[g];
+        su[1][p] = f.u[np + g];
+      }
+      __syncthreads();
+      val[0] = p < 25 ? hw * pressure_2d(pg, q, su, ispec[r], p) : 0.0f;
+#pragma unroll
+      for (int c = 1; c < D; c++) val[c] = 0.0f;
+    } else
 #pragma unroll
     for (int c = 0; c < D; c++) {
       float x = 0.0f;
@@ -877,6 +917,35 @@ void launch_stiffness_2d(hipStream_t s, int first, int count, bool atomic, const
                        xiz, gammax, gammaz, jac, kappa, mu, q, f, rec_mode, rec_index, rec);
 }
 
+template <int E>
+void launch_stiffness_2d_e(hipStream_t s, int first, int count, bool atomic, const int *gather, const float *xix,
+                           const float *xiz, const float *gammax, const float *gammaz, const float *jac,
+                           const float *kappa, const float *mu, const GllConst &q, Fields f) {
+  if (count <= 0) return;
+  if (atomic)
+    hipLaunchKernelGGL((stiffness_2d_kernel<true, E>), dim3(blocks(count, E)), dim3(threads_for(E)), 0, s, first,
+                       count, gather, xix, xiz, gammax, gammaz, jac, kappa, mu, q, f, 0, nullptr, nullptr);
+  else
+    hipLaunchKernelGGL((stiffness_2d_kernel<false, E>), dim3(blocks(count, E)), dim3(threads_for(E)), 0, s, first,
+                       count, gather, xix, xiz, gammax, gammaz, jac, kappa, mu, q, f, 0, nullptr, nullptr);
+}
+
+void launch_stiffness_2d_tuned(hipStream_t s, int elems_per_block, int first, int count, bool atomic,
+                               const int *gather, const float *xix, const float *xiz, const float *gammax,
+                               const float *gammaz, const float *jac, const float *kappa, const float *mu,
+                               const GllConst &q, Fields f) {
+  switch (elems_per_block) {
+    case 2:
+      launch_stiffness_2d_e<2>(s, first, count, atomic, gather, xix, xiz, gammax, gammaz, jac, kappa, mu, q, f);
+      break;
+    case 10:
+      launch_stiffness_2d_e<10>(s, first, count, atomic, gather, xix, xiz, gammax, gammaz, jac, kappa, mu, q, f);
+      break;
+    default:
+      launch_stiffness_2d_e<5>(s, first, count, atomic, gather, xix, xiz, gammax, gammaz, jac, kappa, mu, q, f);
+  }
+}
+
 void launch_frechet_2d(hipStream_t s, int nspec, const int *gather, const float *xix, const float *xiz,
                        const float *gammax, const float *gammaz, const float *kappa, const float *mu, const float *rho,
                        const GllConst &q, Fields adj, Fields bwd, float dt, float *K) {
@@ -920,14 +989,14 @@ void launch_sources(hipStream_t s, int dim, int npe, int nsrc, int istep, const 
 
 void launch_seismograms(hipStream_t s, int dim, int npe, int nrec, int ntypes, const int *types, int isig,
                         const int *ispec, const int *gather, const float *h, Fields f, float dto2, int corrected,
-                        float *seismo) {
+                        float *seismo, const PointGeom2D &pg, const GllConst &q) {
   if (nrec <= 0 || ntypes <= 0) return;
   if (dim == 2)
     hipLaunchKernelGGL(seismogram_kernel<2>, dim3(nrec), dim3(128), 0, s, npe, nrec, ntypes, types, isig, ispec,
-                       gather, h, f, dto2, corrected, seismo);
+                       gather, h, f, dto2, corrected, seismo, pg, q);
   else
     hipLaunchKernelGGL(seismogram_kernel<3>, dim3(nrec), dim3(128), 0, s, npe, nrec, ntypes, types, isig, ispec,
-                       gather, h, f, dto2, corrected, seismo);
+                       gather, h, f, dto2, corrected, seismo, pg, q);
 }
 
 void launch_update_fused(hipStream_t s, int ncomp, Fields f, NewmarkConst nc) {

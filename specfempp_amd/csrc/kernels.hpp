@@ -36,6 +36,12 @@ void launch_stiffness_2d(hipStream_t s, int first, int count, bool atomic, const
                          const float *xiz, const float *gammax, const float *gammaz, const float *jac,
                          const float *kappa, const float *mu, const GllConst &q, Fields f, int rec_mode = 0,
                          const int *rec_index = nullptr, float *rec = nullptr);
+// Plain 2-D stiffness with a chosen number of elements per workgroup (2, 5 or
+// 10: 64, 128 or 256 threads); the batch autotuner picks it.
+void launch_stiffness_2d_tuned(hipStream_t s, int elems_per_block, int first, int count, bool atomic,
+                               const int *gather, const float *xix, const float *xiz, const float *gammax,
+                               const float *gammaz, const float *jac, const float *kappa, const float *mu,
+                               const GllConst &q, Fields f);
 // K (6, nspec, 25) += Fréchet derivatives of one step from the adjoint and
 // backward fields.
 void launch_frechet_2d(hipStream_t s, int nspec, const int *gather, const float *xix, const float *xiz,
@@ -105,11 +111,16 @@ void launch_stacey(hipStream_t s, int dim, int nbnd, const int *gather, const fl
                    float *rec = nullptr);
 void launch_sources(hipStream_t s, int dim, int npe, int nsrc, int istep, const int *order, const int *ispec,
                     const int *gather, const float *array, const float *stf, Fields f);
+// Per-point 2-D derivatives and properties read by the pressure seismogram.
+struct PointGeom2D {
+  const float *xix, *xiz, *gammax, *gammaz, *kappa, *mu;
+};
 // corrected = 1: sample the state after the (not yet applied) mass division and
-// corrector of this step, i.e. u, v + dt/2 a M^-1, a M^-1.
+// corrector of this step, i.e. u, v + dt/2 a M^-1, a M^-1.  Types 0..2
+// displacement, velocity, acceleration; 3 pressure (2-D).
 void launch_seismograms(hipStream_t s, int dim, int npe, int nrec, int ntypes, const int *types, int isig,
                         const int *ispec, const int *gather, const float *h, Fields f, float dto2, int corrected,
-                        float *seismo);
+                        float *seismo, const PointGeom2D &pg, const GllConst &q);
 void launch_update_fused(hipStream_t s, int ncomp, Fields f, NewmarkConst nc);
 void launch_predictor(hipStream_t s, int ncomp, Fields f, NewmarkConst nc);
 void launch_divide_corrector(hipStream_t s, int ncomp, Fields f, NewmarkConst nc, int divide, int correct);

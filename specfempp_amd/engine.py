@@ -34,7 +34,7 @@ import numpy as np
 from . import _native
 
 DISPLACEMENT, VELOCITY, ACCELERATION = 0, 1, 2
-_TYPE_CODES = {"displacement": 0, "velocity": 1, "acceleration": 2}
+_TYPE_CODES = {"displacement": 0, "velocity": 1, "acceleration": 2, "pressure": 3}
 
 
 class SfxError(RuntimeError):
@@ -229,10 +229,11 @@ class Engine2D(_EngineBase):
     KERNEL_NAMES = ("rho", "mu", "kappa", "rhop", "alpha", "beta")
 
     # ---- adjoint (time_marching<combined>) ----
-    def forward_record(self):
+    def forward_record(self, snapshot_every=0):
         """Whole forward loop that keeps the Stacey-element record and the
-        final state for :meth:`adjoint_run`."""
-        self._check(self.lib.sfx_forward_record(self.h))
+        final state for :meth:`adjoint_run`; snapshot_every > 0 also keeps
+        checkpoints the backward reconstruction is reset to."""
+        self._check(self.lib.sfx_forward_record(self.h, int(snapshot_every)))
 
     def set_adjoint_sources(self, positions, stf):
         """positions: [{"x", "z"}]; stf (nadj, nstep, 2) in forward time order."""
@@ -242,10 +243,21 @@ class Engine2D(_EngineBase):
         self._adj_keep = (arr, _receivers(positions))
         self._check(self.lib.sfx_set_adjoint_sources(self.h, len(positions), self._adj_keep[1], _ptr(arr)))
 
+    def set_properties(self, rho, kappa, mu):
+        """Per-point rho, kappa (bulk modulus), mu, each (nspec, 5, 5) in mesh element order."""
+        arrs = [np.ascontiguousarray(x, dtype=np.float32).reshape(self.nspec, 5, 5) for x in (rho, kappa, mu)]
+        self._check(self.lib.sfx_set_properties(self.h, *[_ptr(x) for x in arrs]))
+
     def adjoint_run(self):
         """Combined adjoint + backward loop; seismograms() then returns the
         backward field's, latest sample first."""
         self._check(self.lib.sfx_adjoint_run(self.h))
+
+    def point_weights(self):
+        """w_i w_j J of every element point, (nspec, 5, 5) in mesh element order."""
+        out = np.zeros((self.nspec, 5, 5), dtype=np.float32)
+        self._check(self.lib.sfx_get_point_weights(self.h, _ptr(out)))
+        return out
 
     def kernels(self):
         """dict name -> (nspec, 5, 5) in mesh element order."""
@@ -257,6 +269,7 @@ class Engine2D(_EngineBase):
                  seismogram_types=("displacement",), chunk_size=1, scatter="colored"):
         self.lib = _native.load()
         self.nstep = nstep
+        self.dt = dt
         coorg = np.ascontiguousarray(np.asarray(mesh.coorg, dtype=np.float64).T)
         knods = np.ascontiguousarray(np.asarray(mesh.knods, dtype=np.int32).T)
         kmato = np.ascontiguousarray(mesh.kmato, dtype=np.int32)
@@ -292,6 +305,7 @@ class Engine3D(_EngineBase):
         self.scatter = scatter
         self.lib = _native.load()
         self.nstep = nstep
+        self.dt = dt
         mat = np.ascontiguousarray(mesh["material"], dtype=np.int32).ravel()
         rho = np.ascontiguousarray(mesh["rho"], dtype=np.float64)
         vp = np.ascontiguousarray(mesh["vp"], dtype=np.float64)
@@ -312,3 +326,60 @@ class Engine3D(_EngineBase):
                                                   _receivers(receivers), C.byref(run), C.byref(h)))
         self.h = h
         self._finish_init(seismogram_types)
+
+
+class Batch:
+    """Many 2-D engines advanced together (``sfx_batch_*``).
+
+    The engines are only read at creation; the batch holds its own copy of
+    their device data, runs every simulation to its own step count from a zero
+    state, and returns each simulation's seismograms laid out as its engine's.
+    """
+
+    def __init__(self, engines):
+        self.lib = _native.load()
+        self.engines = list(engines)
+        arr = (C.c_void_p * len(self.engines))(*[e.h for e in self.engines])
+        h = C.c_void_p()
+        self.h = None
+        self._check(self.lib.sfx_batch_create(len(self.engines), arr, C.byref(h)))
+        self.h = h
+
+    def _check(self, rc):
+        if rc < 0:
+            raise SfxError(self.lib.sfx_last_error().decode())
+        return rc
+
+    def tune(self, steps_per_candidate=10):
+        out = (C.c_int * 3)()
+        self._check(self.lib.sfx_batch_tune(self.h, steps_per_candidate, out))
+        return {"elems_per_block": out[0], "graphs": bool(out[1]), "colors": out[2]}
+
+    def set_config(self, elems_per_block=5, graphs=True):
+        self._check(self.lib.sfx_batch_set_config(self.h, elems_per_block, int(graphs)))
+
+    def run(self):
+        self._check(self.lib.sfx_batch_run(self.h))
+
+    def seismograms(self, i):
+        e = self.engines[i]
+        out = np.zeros((e.nsig, len(e.types), e.nrec, e.ncomp), dtype=np.float32)
+        self._check(self.lib.sfx_batch_get_seismograms(self.h, i, _ptr(out)))
+        return out
+
+    def info(self):
+        out = (C.c_double * 6)()
+        self._check(self.lib.sfx_batch_get_info(self.h, out))
+        return {"run_ms": out[0], "elems_per_block": int(out[1]), "graphs": bool(out[2]), "colors": int(out[3]),
+                "padded_points": int(out[4]), "point_steps": out[5]}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sfx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -86,6 +86,35 @@ def config1(nstep: int = 1600, n: int = 80, h: float = 50.0) -> Workload2D:
                       {"courant": courant(1e-3, h, 2500.0)})
 
 
+def config4(nsim: int = 64, seed: int = 4, nmin: int = 40, nmax: int = 240, h: float = 50.0,
+            nstep_range=(800, 2400), nrec: int = 32):
+    """Batched 2-D multi-event workload: nsim homogeneous simulations, side
+    n ~ U{nmin..nmax} quads of h, vp ~ U[2000, 4000], vs = vp / sqrt(3),
+    rho 2200, free top and Stacey elsewhere, one vertical Ricker force
+    (f0 ~ U[5, 20] Hz) uniform in the inner 80 %, nstep ~ U{nstep_range},
+    dt at Courant 0.4, nrec receivers uniform in the inner 80 %.  One
+    PCG64 stream per simulation, spawned from the seed."""
+    out = []
+    for i, ss in enumerate(np.random.SeedSequence(seed).spawn(nsim)):
+        rng = np.random.default_rng(ss)
+        n = int(rng.integers(nmin, nmax + 1))
+        vp = float(rng.uniform(2000.0, 4000.0))
+        vs = vp / np.sqrt(3.0)
+        f0 = float(rng.uniform(5.0, 20.0))
+        nstep = int(rng.integers(nstep_range[0], nstep_range[1] + 1))
+        L = n * h
+        lo, hi = 0.1 * L, 0.9 * L
+        sx, sz = rng.uniform(lo, hi, size=2)
+        rx, rz = rng.uniform(lo, hi, size=(2, nrec))
+        mesh = quad_mesh(n, n, 0.0, L, 0.0, L, materials=[elastic(2200.0, vp, vs)],
+                         absorbing=("bottom", "right", "left"))
+        src = [dict(x=float(sx), z=float(sz), angle=0.0, f0=f0, factor=1e10, tshift=0.0)]
+        rec = [dict(x=float(x), z=float(z)) for x, z in zip(rx, rz)]
+        out.append(Workload2D(f"batch{i}_{n}x{n}", mesh, src, rec, stable_dt(h, vp), nstep, stable_dt(h, vp),
+                              {"n": n, "vp": vp, "f0": f0}))
+    return out
+
+
 @dataclass
 class Workload3D:
     name: str

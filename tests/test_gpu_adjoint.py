@@ -51,28 +51,31 @@ def oracle_runs():
             a, f, ph = rng.uniform(0.5, 1.5), rng.uniform(8, 20), rng.uniform(0, 2 * np.pi)
             stf[r, :, c] = 1e-6 * a * np.sin(2 * np.pi * f * t + ph) * np.exp(-((t - 0.25) / 0.1) ** 2)
     for prec in (4, 8):
-        o = oracle.Oracle2D(mesh, prec, 1, DT)
-        seis, _, fields, record = o.forward_record(SRC, REC, NSTEP, DT, 1, TYPES)
-        kern, bseis = o.combined(SRC, REC, stf, REC, NSTEP, DT, fields, record, 1, TYPES)
-        out[prec] = (seis, kern, bseis)
+        for k in (0, 23):
+            o = oracle.Oracle2D(mesh, prec, 1, DT)
+            seis, _, fields, record, *snaps = o.forward_record(SRC, REC, NSTEP, DT, 1, TYPES, snapshot_every=k)
+            kern, bseis = o.combined(SRC, REC, stf, REC, NSTEP, DT, fields, record, 1, TYPES, snapshot_every=k,
+                                     snapshots=snaps[0] if snaps else None)
+            out[prec, k] = (seis, kern, bseis)
     return mesh, stf, out
 
 
-@pytest.mark.parametrize("scatter", ["colored", "atomic"])
-def test_adjoint_kernels_match_oracle(oracle_runs, scatter):
+@pytest.mark.parametrize("scatter,snapshot_every", [("colored", 0), ("atomic", 0), ("colored", 23)])
+def test_adjoint_kernels_match_oracle(oracle_runs, scatter, snapshot_every):
     from specfempp_amd import Engine2D
 
     mesh, stf, out = oracle_runs
     eng = Engine2D(mesh, SRC, REC, DT, NSTEP, 1, TYPES, scatter=scatter)
     eng.initialize()
-    eng.forward_record()
+    eng.forward_record(snapshot_every)
     fwd = eng.seismograms().astype(np.float64)
     eng.set_adjoint_sources(REC, stf)
     eng.adjoint_run()
     bwd = eng.seismograms().astype(np.float64)
     kern = eng.kernels()
     eng.close()
-    (s32, k32, b32), (s64, k64, b64) = out[4], out[8]
+    (s32, k32, b32), (s64, k64, b64) = out[4, snapshot_every], out[8, snapshot_every]
+    scatter = f"{scatter}_k{snapshot_every}"
     e32 = rel_l2(fwd, s32)
     _report(f"{scatter}/forward", vs_oracle_fp32=e32, vs_oracle_fp64=rel_l2(fwd, s64))
     assert e32 <= 1e-5
@@ -100,3 +103,68 @@ def test_adjoint_requires_recorded_forward():
     with pytest.raises(SfxError, match="sfx_forward_record"):
         eng.adjoint_run()  # the recorded state is consumed
     eng.close()
+
+
+def test_autograd_gradient_matches_oracle_and_finite_differences():
+    """torch.autograd through the engine: the gradient of the L2 waveform
+    misfit with respect to per-point (rho, kappa, mu) equals the gradient from
+    the FP64 oracle's kernels, and predicts central finite differences of the
+    FP64 oracle's misfit along a smooth direction."""
+    import torch
+
+    from specfempp_amd import Engine2D
+    from specfempp_amd.autograd import SeismogramModel
+
+    mesh = _mesh()
+    o = oracle.Oracle2D(mesh, 8, 1, DT)
+    rho0, kap0, mu0, wj = o.properties()
+    ex, ez = np.arange(mesh.nspec) % N, np.arange(mesh.nspec) // N
+    gl = np.array([-1.0, -np.sqrt(3 / 7), 0.0, np.sqrt(3 / 7), 1.0])
+    X = (ex[:, None, None] + (gl[None, None, :] + 1) / 2 + 0 * gl[None, :, None]) * H
+    Z = (ez[:, None, None] + (gl[None, :, None] + 1) / 2 + 0 * gl[None, None, :]) * H
+    interior = ((ex >= 2) & (ex < N - 2) & (ez >= 2) & (ez < N - 2))[:, None, None]
+    bump = np.exp(-((X - 420) ** 2 + (Z - 380) ** 2) / 120.0 ** 2) * interior
+    o_true = oracle.Oracle2D(mesh, 8, 1, DT)
+    o_true.set_properties(rho0 * (1 + 0.02 * bump), kap0 * (1 + 0.02 * bump), mu0 * (1 - 0.02 * bump))
+    data = torch.from_numpy(o_true.forward(SRC, REC, NSTEP, DT, 1, TYPES)[0][:, 0])
+
+    eng = Engine2D(mesh, SRC, REC, DT, NSTEP, 1, TYPES)
+    model = SeismogramModel(eng, REC)
+    params = [torch.tensor(x, dtype=torch.float64, requires_grad=True) for x in (rho0, kap0, mu0)]
+    u = model(*params)
+    loss = 0.5 * torch.sum((u - data) ** 2) * DT
+    loss.backward()
+    grads = [p.grad.numpy() for p in params]
+    eng.close()
+
+    # the same gradient from the FP64 oracle's kernels
+    seis, _, fields, record = o.forward_record(SRC, REC, NSTEP, DT, 1, TYPES)
+    res = seis[:, 0] - data.numpy()
+    K = o.combined(SRC, REC, np.transpose(res, (1, 0, 2)), REC, NSTEP, DT, fields, record, 1, TYPES)[0]
+    lam = kap0 - 2.0 / 3.0 * mu0
+    D = -K[2] / (lam + mu0)
+    E = -K[1] / (2 * mu0) + D / 3.0
+    ref = (wj * K[0] / rho0, -wj * D, wj * (2.0 / 3.0 * D - 2.0 * E))
+    for name, g, r in zip(("rho", "kappa", "mu"), grads, ref):
+        err = rel_l2(g, r)
+        _report(f"autograd/d_{name}", vs_oracle_fp64=err)
+        assert err <= 1e-3
+
+    # directional derivative vs central differences of the FP64 misfit
+    g = np.exp(-((X - 380) ** 2 + (Z - 450) ** 2) / 100.0 ** 2) * interior
+    eps = 1e-3
+
+    def misfit(rho, kap, mu):
+        oo = oracle.Oracle2D(mesh, 8, 1, DT)
+        oo.set_properties(rho, kap, mu)
+        s = oo.forward(SRC, REC, NSTEP, DT, 1, TYPES)[0][:, 0]
+        return 0.5 * np.sum((s - data.numpy()) ** 2) * DT
+
+    dirs = {"rho": (rho0 * g, 0 * g, 0 * g), "kappa": (0 * g, kap0 * g, 0 * g), "mu": (0 * g, 0 * g, mu0 * g)}
+    for name, (dr, dk, dm) in dirs.items():
+        fd = (misfit(rho0 + eps * dr, kap0 + eps * dk, mu0 + eps * dm) -
+              misfit(rho0 - eps * dr, kap0 - eps * dk, mu0 - eps * dm)) / 2
+        pred = eps * sum(np.sum(gr * d) for gr, d in zip(grads, (dr, dk, dm)))
+        rel = abs(pred - fd) / abs(fd)
+        _report(f"autograd/fd_{name}", fd=fd, autograd=pred, rel=rel)
+        assert rel <= 1e-3

@@ -1,0 +1,82 @@
+"""GPU parity of batched 2-D runs.
+
+Each simulation of a batch must give the seismograms of its own engine run
+alone (same kernels and coloring; only the order of float atomics on Stacey
+and source points may differ, so the bar is relative L2 <= 1e-6) and those of
+the FP32 oracle (<= 1e-5, the bar of every forward parity test), whatever
+launch geometry the autotuner picks, with and without graph capture.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_l2
+from specfempp_amd import meshes
+
+pytestmark = pytest.mark.gpu
+
+TYPES = ("displacement", "velocity")
+
+
+def _workloads():
+    return meshes.config4(nsim=6, seed=11, nmin=8, nmax=24, nstep_range=(150, 400), nrec=4)
+
+
+def _engines(ws):
+    from specfempp_amd import Engine2D
+
+    return [Engine2D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, 1, TYPES) for w in ws]
+
+
+@pytest.fixture(scope="module")
+def alone():
+    ws = _workloads()
+    out = []
+    for e in _engines(ws):
+        e.initialize()
+        e.run()
+        out.append(e.seismograms().astype(np.float64))
+        e.close()
+    return ws, out
+
+
+@pytest.mark.parametrize("elems_per_block,graphs", [(5, True), (2, False), (10, True)])
+def test_batch_matches_single_runs(alone, elems_per_block, graphs):
+    from specfempp_amd import Batch
+
+    ws, ref = alone
+    engines = _engines(ws)
+    b = Batch(engines)
+    for e in engines:
+        e.close()  # the batch holds its own copy
+    b.set_config(elems_per_block, graphs)
+    b.run()
+    for i, w in enumerate(ws):
+        got = b.seismograms(i).astype(np.float64)
+        assert got.shape == ref[i].shape
+        err = rel_l2(got, ref[i])
+        print(f"[batch] E={elems_per_block} graphs={graphs} sim {i} ({w.name}, {w.nstep} steps): {err:.2e}",
+              flush=True)
+        assert err <= 1e-6
+    info = b.info()
+    assert info["point_steps"] == sum(((4 * w.notes["n"] + 1) ** 2) * w.nstep for w in ws)
+    b.close()
+
+
+def test_batch_matches_oracle_and_tuning_keeps_results(alone):
+    from specfempp_amd import Batch
+
+    ws, ref = alone
+    b = Batch(_engines(ws))
+    choice = b.tune(3)
+    assert choice["elems_per_block"] in (2, 5, 10)
+    b.run()
+    for i in (0, len(ws) - 1):
+        w = ws[i]
+        s32, _ = oracle.Oracle2D(w.mesh, 4, 1, w.dt).forward(w.sources, w.receivers, w.nstep, w.dt, 1, TYPES)
+        got = b.seismograms(i).astype(np.float64)
+        err = rel_l2(got[:, 0], s32[:, 0])
+        print(f"[batch] tuned {choice} sim {i} vs FP32 oracle: {err:.2e}", flush=True)
+        assert err <= 1e-5
+        assert rel_l2(got, ref[i]) <= 1e-6
+    b.close()

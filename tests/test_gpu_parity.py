@@ -272,3 +272,42 @@ def test_3d_block_step_calls_equal_run_bitwise():
     eng.close()
     assert np.array_equal(run, stepped)
     assert np.array_equal(a_run, a_step)
+
+
+def test_pressure_seismograms_match_oracle():
+    """Pressure (wavefield.hpp:49-85) beside displacement on the golden
+    Stacey case, and on a prescribed linear field, where the exact value is
+    -kappa (a + d) at every receiver."""
+    mesh, cfg = load_case("elastic_stacey_80x60")
+    types = ("displacement", "pressure")
+    eng = _engine2d(mesh, cfg, types)
+    eng.initialize()
+    eng.run()
+    ours = eng.seismograms().astype(np.float64)
+    s32, _ = oracle.Oracle2D(mesh, 4, 1, cfg["dt"]).forward(cfg["sources"], cfg["stations"], cfg["nstep"], cfg["dt"],
+                                                            1, types)
+    s64, _ = oracle.Oracle2D(mesh, 8, 1, cfg["dt"]).forward(cfg["sources"], cfg["stations"], cfg["nstep"], cfg["dt"],
+                                                            1, types)
+    for it, t in enumerate(types):
+        e32, e64 = rel_l2(ours[:, it], s32[:, it]), rel_l2(ours[:, it], s64[:, it])
+        floor = rel_l2(s32[:, it], s64[:, it])
+        _report(f"pressure_case/{t}", vs_oracle_fp32=e32, vs_oracle_fp64=e64, fp32_floor=floor)
+        _check(e32, e64, floor)
+    assert np.all(ours[:, 1, :, 1] == 0.0)
+    # linear field through the seam sampling
+    o = oracle.Oracle2D(mesh, 8, 1, cfg["dt"])
+    coord, _ = o.geometry()
+    ibool, midx, _ = o.numbering()
+    xg, zg = np.zeros(o.nglob), np.zeros(o.nglob)
+    xg[midx[ibool.ravel()]] = coord[..., 0].ravel()
+    zg[midx[ibool.ravel()]] = coord[..., 1].ravel()
+    a, b, c, d = 2e-4, -3e-4, 5e-4, 7e-4
+    eng.set_field("displacement", np.stack([a * xg + b * zg, c * xg + d * zg]))
+    eng.compute_seismograms(0)
+    got = eng.seismograms()[0, 1, :, 0].astype(np.float64)
+    m = mesh.materials[0]
+    kappa = m.density * (m.cp ** 2 - 4.0 / 3.0 * m.cs ** 2)
+    exact = -kappa * (a + d)
+    _report("pressure_linear_field", max_rel=float(np.max(np.abs(got - exact)) / abs(exact)))
+    assert np.max(np.abs(got - exact)) <= 1e-4 * abs(exact)
+    eng.close()

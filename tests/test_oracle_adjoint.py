@@ -62,6 +62,24 @@ def test_backward_field_retraces_forward(precision, bar):
     assert np.abs(kern).max() == 0.0
 
 
+def test_checkpoints_bound_fp32_drift():
+    """Resetting the backward field to forward checkpoints every k steps keeps
+    the FP32 reconstruction closer to the forward run than pure time reversal."""
+    mesh = _mesh()
+    o = oracle.Oracle2D(mesh, 4, 1, DT)
+    seis, _, fields, record, snaps = o.forward_record(SRC, REC, NSTEP, DT, 1, TYPES, snapshot_every=23)
+    assert snaps.shape == ((NSTEP - 2) // 23 + 1, 3, 2, o.nglob)
+    zero = np.zeros((len(REC), NSTEP, 2))
+    _, b_pure = o.combined(SRC, REC, zero, REC, NSTEP, DT, fields, record, 1, TYPES)
+    _, b_snap = o.combined(SRC, REC, zero, REC, NSTEP, DT, fields, record, 1, TYPES, snapshot_every=23,
+                           snapshots=snaps)
+    fw = seis[:, 0]
+    e_pure = np.linalg.norm(fw - b_pure[::-1, 0]) / np.linalg.norm(fw)
+    e_snap = np.linalg.norm(fw - b_snap[::-1, 0]) / np.linalg.norm(fw)
+    print(f"[adjoint] FP32 reconstruction: pure {e_pure:.3e}, checkpoints every 23 steps {e_snap:.3e}")
+    assert e_snap < 0.5 * e_pure
+
+
 def _misfit(mesh, rho, kappa, mu, data):
     o = oracle.Oracle2D(mesh, 8, 1, DT)
     o.set_properties(rho, kappa, mu)

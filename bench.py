@@ -1,12 +1,15 @@
 """Benchmark: GLL point-steps per second of the 3-D elastic NGLL=5 FP32
 forward Newmark loop (64^3 layered cube) on one MI355X, with the roofline of
-the dominant kernel (stiffness) and the CPU oracle timed beside it.
+the dominant kernel and the CPU oracle timed beside it; then the batched
+multi-event leg (64 2-D simulations in one batch vs the same 64 run one after
+another) and the adjoint leg (200^2 2-D gradient vs a plain forward run).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 64] [--no-cpu]
+                    [--no-batch] [--no-adjoint] [--scatter auto|block|gather]
 
 Prints one JSON line.  A "step" is one Newmark step of the whole hot path:
-sources, stiffness (gather, gradient, stress, divergence, atomic scatter),
-Stacey, seismograms, fused mass division + corrector + predictor.
+sources, stiffness (gradient, stress, divergence, assembly), Stacey,
+seismograms, fused mass division + corrector + predictor.
 """
 from __future__ import annotations
 
@@ -20,6 +23,27 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def algorithmic_bytes_block(n: int) -> dict:
+    """Bytes each kernel of the 3-D block path must move per launch (FP32),
+    n^3 elements in blocks of 2^3:
+      block_step_3d:     the displacement once per point (12 B), kappa + mu per
+                         local point (8 B x 125 / element), for the points
+                         inside blocks v and M^-1 read and u, v written (48 B),
+                         and the block-surface sums written (12 B per block and
+                         surface point, 386 per block)
+      surface_update_3d: those sums read, and u, v, M^-1 read and u, v written
+                         (60 B) at the block-surface points."""
+    nspec, nglob = n ** 3, (4 * n + 1) ** 3
+    nb = nspec // 8
+    nint = (7 * (n // 2)) ** 3
+    nsurf = nglob - nint
+    blk = nglob * 12 + nspec * 125 * 8 + nint * 48 + nb * 386 * 12
+    srf = nb * 386 * 12 + nsurf * 60
+    survey = nspec * 125 * 12 + nglob * 36 + nglob * 84
+    return {"block_step_3d_kernel": blk, "surface_update_3d_kernel": srf, "per_pt_step": (blk + srf) / nglob,
+            "survey_per_pt_step": survey / nglob}
 
 
 def algorithmic_bytes_3d(nspec: int, nglob: int) -> dict:
@@ -72,14 +96,108 @@ def copy_calibration(nbytes: int = 1 << 30, reps: int = 5) -> dict:
             "copy_GBs": 2 * nbytes / el / 1e9}
 
 
+def batched_leg(nsim: int, tune_steps: int) -> dict:
+    """Config 4: nsim 2-D simulations (40^2..240^2 quads, own vp, f0, source,
+    32 receivers, 800..2400 steps) advanced as one batch vs run one after
+    another with the plain engine.  Both from a zero state; the batch's
+    seismograms are checked against the sequential ones."""
+    import numpy as np
+
+    from specfempp_amd import Batch, Engine2D, meshes
+
+    ws = meshes.config4(nsim=nsim)
+    t = time.perf_counter()
+    engines = [Engine2D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, 1, ("displacement",)) for w in ws]
+    setup_s = time.perf_counter() - t
+    ptsteps = float(sum(e.nglob * w.nstep for e, w in zip(engines, ws)))
+    t = time.perf_counter()
+    seq = []
+    for e in engines:
+        e.initialize()
+        e.run()
+        seq.append(e.seismograms())
+    seq_s = time.perf_counter() - t
+    b = Batch(engines)
+    choice = b.tune(tune_steps)
+    t = time.perf_counter()
+    b.run()
+    batch_wall_s = time.perf_counter() - t
+    info = b.info()
+    worst = 0.0
+    for i in range(nsim):
+        got = b.seismograms(i).astype(np.float64)
+        ref = seq[i].astype(np.float64)
+        worst = max(worst, float(np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-30)))
+    b.close()
+    sum_nglob = int(sum(e.nglob for e in engines))
+    for e in engines:
+        e.close()
+    return {"workload": f"{nsim} 2-D simulations, 40^2..240^2 quads of 50 m, seed 4",
+            "point_steps": ptsteps, "sum_nglob": sum_nglob,
+            "batched_pt_steps_per_s": ptsteps / batch_wall_s, "batched_s": batch_wall_s,
+            "batched_device_ms": info["run_ms"],
+            "sequential_pt_steps_per_s": ptsteps / seq_s, "sequential_s": seq_s,
+            "speedup": seq_s / batch_wall_s, "tuned": choice, "max_rel_l2_vs_sequential": worst,
+            "setup_s": round(setup_s, 1)}
+
+
+def adjoint_leg(snapshot_every: int = 55) -> dict:
+    """Config 5: 200^2 2-D gradient of the L2 waveform misfit (8 sources, 100
+    receivers, 3000 steps, checkpoints every 55 steps) against a plain forward
+    run of the same model: forward record, combined adjoint + backward loop."""
+    import numpy as np
+
+    from specfempp_amd import Engine2D, meshes
+
+    true, start = meshes.config5()
+    ty = ("displacement",)
+    et = Engine2D(true.mesh, true.sources, true.receivers, true.dt, true.nstep, 1, ty)
+    et.initialize()
+    et.run()
+    data = et.seismograms()[:, 0].astype(np.float64)
+    et.close()
+    eng = Engine2D(start.mesh, start.sources, start.receivers, start.dt, start.nstep, 1, ty)
+    eng.initialize()
+    eng.run()  # warm-up
+    eng.initialize()
+    t = time.perf_counter()
+    eng.run()
+    plain_s = time.perf_counter() - t
+    eng.initialize()
+    t = time.perf_counter()
+    eng.forward_record(snapshot_every)
+    record_s = time.perf_counter() - t
+    u = eng.seismograms()[:, 0].astype(np.float64)
+    res = u - data
+    misfit = 0.5 * float(np.sum(res ** 2)) * start.dt
+    eng.set_adjoint_sources(start.receivers, np.transpose(res, (1, 0, 2)))
+    t = time.perf_counter()
+    eng.adjoint_run()
+    adjoint_s = time.perf_counter() - t
+    K = eng.kernels()
+    finite = bool(all(np.isfinite(v).all() for v in K.values()))
+    pts = eng.nglob * start.nstep
+    eng.close()
+    return {"workload": "2-D 200^2 quads of 20 m, checkerboard +-5 %, 8 sources, 100 receivers, 3000 steps",
+            "plain_forward_s": plain_s, "forward_record_s": record_s, "adjoint_run_s": adjoint_s,
+            "adjoint_over_forward": adjoint_s / plain_s,
+            "gradient_over_forward": (record_s + adjoint_s) / plain_s,
+            "combined_pt_steps_per_s": pts / adjoint_s, "forward_pt_steps_per_s": pts / plain_s,
+            "misfit": misfit, "snapshot_every": snapshot_every, "kernels_finite": finite}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--scatter", default="auto", choices=["auto", "block", "gather", "colored", "atomic"])
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--batch-sims", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (scale reference for HBM byte counters)")
     args = ap.parse_args()
@@ -95,8 +213,9 @@ def main() -> int:
 
     w = meshes.config3(n=args.n, nstep=args.warmup + args.steps)
     t = time.perf_counter()
-    eng = Engine3D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, 1, ("displacement",))
+    eng = Engine3D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, 1, ("displacement",), scatter=args.scatter)
     setup_s = time.perf_counter() - t
+    scatter = eng.scatter
     eng.initialize()
     if args.warmup:
         eng.run(0, args.warmup)
@@ -115,9 +234,14 @@ def main() -> int:
 
     ms_per_step = 1e3 * elapsed / args.steps
     value = nglob * args.steps / elapsed
-    ab = algorithmic_bytes_3d(nspec, nglob)
-    kern = {"elemforce_3d_kernel": (ab["elemforce"], tm["stiffness_ms"]),
-            "assemble_update_3d_kernel": (ab["update"], tm["update_ms"])}
+    if scatter == "block":
+        ab = algorithmic_bytes_block(args.n)
+        kern = {"block_step_3d_kernel": (ab["block_step_3d_kernel"], tm["stiffness_ms"]),
+                "surface_update_3d_kernel": (ab["surface_update_3d_kernel"], tm["update_ms"])}
+    else:
+        ab = algorithmic_bytes_3d(nspec, nglob)
+        kern = {"elemforce_3d_kernel": (ab["elemforce"], tm["stiffness_ms"]),
+                "assemble_update_3d_kernel": (ab["update"], tm["update_ms"])}
     dominant = max(kern, key=lambda k: kern[k][1])
     nbytes, ms = kern[dominant]
     achieved = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
@@ -140,7 +264,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"3d_layered_cube_{args.n}^3_ngll5", "nspec": nspec, "nglob": nglob,
                    "dt": w.dt, "dt_stated": w.dt_stated, "nrec": len(w.receivers), "source": "moment tensor",
-                   "scatter": "gather", "setup_s": round(setup_s, 2)},
+                   "scatter": scatter, "setup_s": round(setup_s, 2)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dominant,
                      "algorithmic_bytes_per_launch": nbytes, "avg_ms": ms},
@@ -157,6 +281,10 @@ def main() -> int:
         out["copy_calibration"] = copy_calibration()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps)
+    if not args.no_batch:
+        out["batched"] = batched_leg(args.batch_sims, 5)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg()
     print(json.dumps(out))
     return 0
 

@@ -1,5 +1,6 @@
 // Host assembly of the 2-D and 3-D engines.  See assembly.hpp for the layout.
 #include "assembly.hpp"
+#include "lattice.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -708,10 +709,24 @@ HostAssembly build_assembly_3d(const sfx_mesh3d &m, int nsrc, const sfx_source *
   // element's 64 points with local indices < 4 are one contiguous tile.
   // gather holds the device index of every local point; lattice_of maps the
   // reference numbering to it.
+  // The block path (SFX_SCATTER_BLOCK) stores the fields in the block layout of
+  // lattice.hpp instead: block interiors contiguous, then the surface points
+  // grouped by owner block.
   const int64_t TX = m.nx + 1, TY = m.ny + 1, TZ = m.nz + 1;
   if (TX * TY * TZ * 64 > (int64_t(1) << 31) - 1) throw std::invalid_argument("3-D mesh too large");
-  a.nglob_pad = static_cast<int>(TX * TY * TZ * 64);
+  const bool block_layout = run.scatter == SFX_SCATTER_BLOCK;
+  const int nbx = m.nx / 2, nby = m.ny / 2, nbz = m.nz / 2;
+  if (block_layout) {
+    a.nint_pad = (static_cast<int>(static_cast<int64_t>(nbx) * nby * nbz * 343) + 63) / 64 * 64;
+    const int64_t pad = a.nint_pad + static_cast<int64_t>(nbx + 1) * (nby + 1) * (nbz + 1) * OWNER_STRIDE;
+    a.nglob_pad = static_cast<int>((pad + 63) / 64 * 64);
+  } else {
+    a.nglob_pad = static_cast<int>(TX * TY * TZ * 64);
+  }
   auto tiled = [&](int64_t I, int64_t J, int64_t K) {
+    if (block_layout)
+      return static_cast<int32_t>(block_layout_index(nbx, nby, a.nint_pad, static_cast<int>(I), static_cast<int>(J),
+                                                     static_cast<int>(K)));
     return static_cast<int32_t>((((K >> 2) * TY + (J >> 2)) * TX + (I >> 2)) * 64 + ((K & 3) * 4 + (J & 3)) * 4 + (I & 3));
   };
   a.lattice_of.assign(a.nglob, -1);

@@ -68,6 +68,7 @@ struct HostAssembly {
   // 3-D structured sizes; device fields in lattice order, lattice_of maps the
   // reference numbering (iglob) to the lattice index
   int nx = 0, ny = 0, nz = 0;
+  int nint_pad = 0;                      // block layout: slots of the block-interior points
   std::vector<int32_t> lattice_of;
   // 2-D locator data (control nodes (nspec, ngnod, 2), GLL point coordinates)
   int ngnod2d = 0;

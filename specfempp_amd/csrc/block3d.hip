@@ -12,9 +12,10 @@
 // sums of every surface point, again in a fixed order, and updates it.  No
 // atomics; every point's force is summed in the same order each step.
 //
-// Global accesses go by 4 x 4 x 4 tiles: a block holds 8 whole tiles (its
-// points with local indices 0..7), read and written one tile per wavefront
-// (256 contiguous bytes per component), plus 217 points with a local index 8.
+// The fields are in the block layout (lattice.hpp): a block's 343 interior
+// points are one contiguous range per component, read and written with
+// coalesced accesses; its 386 surface points live in the owner slots of up to
+// 8 blocks.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -25,24 +26,11 @@ namespace sfx {
 namespace {
 
 constexpr int BP = 729;     // lattice points of a block
-constexpr int BFACE = 217;  // block points with a local index 8
+constexpr int BINT = 343;   // points strictly inside a block
+constexpr int BSURF = 386;  // points on a block's surface
 
-__device__ __forceinline__ int tiled_lattice(const Lattice &lat, int I, int J, int K) {
-  return (((K >> 2) * lat.TY + (J >> 2)) * lat.TX + (I >> 2)) * 64 + (((K & 3) * 4 + (J & 3)) * 4 + (I & 3));
-}
-
-// Block-local coordinates of the q-th point with a local index 8 (q < 217):
-// the bk = 8 plane, then bj = 8 (bk < 8), then bi = 8 (bj, bk < 8).
-__device__ __forceinline__ void face_point(int q, int &bi, int &bj, int &bk) {
-  if (q < 81) {
-    bk = 8, bj = q / 9, bi = q - (q / 9) * 9;
-  } else if (q < 153) {
-    const int r = q - 81;
-    bj = 8, bk = r / 9, bi = r - (r / 9) * 9;
-  } else {
-    const int r = q - 153;
-    bi = 8, bk = r >> 3, bj = r & 7;
-  }
+__device__ __forceinline__ int layout_index(const Lattice &lat, int I, int J, int K) {
+  return block_layout_index(lat.nx >> 1, lat.ny >> 1, lat.nint_pad, I, J, K);
 }
 
 // Owner-local coordinates of the owner's own surface point o (< 169).
@@ -115,22 +103,18 @@ __global__ __launch_bounds__(256) void block_step_3d_kernel(
     }
   }
 
-  // block displacement: 8 whole tiles, then the 217 points with an index 8
-  const int pt = t & 63;
-  const int ti = pt & 3, tj = (pt >> 2) & 3, tk = pt >> 4;
+  // block displacement: the contiguous interior, then the surface points
+  const size_t ibase = static_cast<size_t>(blk) * BINT;
+  for (int qi = t; qi < BINT; qi += 256) {
+    const int li = qi % 7 + 1, lj = (qi / 7) % 7 + 1, lk = qi / 49 + 1;
+    const int p = (lk * 9 + lj) * 9 + li;
 #pragma unroll
-  for (int pass = 0; pass < 2; pass++) {
-    const int tl = pass * 4 + (t >> 6);
-    const int txl = tl & 1, tyl = (tl >> 1) & 1, tzl = tl >> 2;
-    const int d = (((2 * bz + tzl) * lat.TY + 2 * by + tyl) * lat.TX + 2 * bx + txl) * 64 + pt;
-    const int p = ((4 * tzl + tk) * 9 + 4 * tyl + tj) * 9 + 4 * txl + ti;
-#pragma unroll
-    for (int c = 0; c < 3; c++) ub[c][p] = f.u[c * np + d];
+    for (int c = 0; c < 3; c++) ub[c][p] = f.u[c * np + ibase + qi];
   }
-  if (t < BFACE) {
+  for (int qs = t; qs < BSURF; qs += 256) {
     int bi, bj, bk;
-    face_point(t, bi, bj, bk);
-    const int d = tiled_lattice(lat, 8 * bx + bi, 8 * by + bj, 8 * bz + bk);
+    block_surface_point(qs, bi, bj, bk);
+    const int d = layout_index(lat, 8 * bx + bi, 8 * by + bj, 8 * bz + bk);
 #pragma unroll
     for (int c = 0; c < 3; c++) ub[c][(bk * 9 + bj) * 9 + bi] = f.u[c * np + d];
   }
@@ -256,22 +240,14 @@ __global__ __launch_bounds__(256) void block_step_3d_kernel(
     for (int c = 0; c < 3; c++) o[c * BS_STRIDE] = sum[c];
   };
   const bool store = divide_only || store_a == 2 || (store_a == 1 && rec_block[blk]);
-#pragma unroll
-  for (int pass = 0; pass < 2; pass++) {
-    const int tl = pass * 4 + (t >> 6);
-    const int txl = tl & 1, tyl = (tl >> 1) & 1, tzl = tl >> 2;
-    const int bi = 4 * txl + ti, bj = 4 * tyl + tj, bk = 4 * tzl + tk;
+  for (int qi = t; qi < BINT; qi += 256) {
+    const int li = qi % 7 + 1, lj = (qi / 7) % 7 + 1, lk = qi / 49 + 1;
+    const int p = (lk * 9 + lj) * 9 + li;
     float sum[3];
-    point_sum(bi, bj, bk, sum);
-    if (bi == 0 || bj == 0 || bk == 0) {
-      to_owner(bi, bj, bk, sum);
-      continue;
-    }
-    const int d = (((2 * bz + tzl) * lat.TY + 2 * by + tyl) * lat.TX + 2 * bx + txl) * 64 + pt;
-    const int p = (bk * 9 + bj) * 9 + bi;
+    point_sum(li, lj, lk, sum);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const size_t o = static_cast<size_t>(c) * np + d;
+      const size_t o = static_cast<size_t>(c) * np + ibase + qi;
       const float a = sum[c] * f.minv[o];
       if (store) f.a[o] = a;
       if (divide_only) continue;
@@ -280,9 +256,9 @@ __global__ __launch_bounds__(256) void block_step_3d_kernel(
       f.v[o] = v1 + nc.dto2 * a;
     }
   }
-  if (t < BFACE) {
+  for (int qs = t; qs < BSURF; qs += 256) {
     int bi, bj, bk;
-    face_point(t, bi, bj, bk);
+    block_surface_point(qs, bi, bj, bk);
     float sum[3];
     point_sum(bi, bj, bk, sum);
     to_owner(bi, bj, bk, sum);
@@ -304,7 +280,7 @@ __global__ __launch_bounds__(192) void surface_update_3d_kernel(Lattice lat, con
   if (I > 4 * lat.nx || J > 4 * lat.ny || K > 4 * lat.nz) return;
   float sum[3];
   owner_force(lat, bs, ox, oy, oz, li, lj, lk, sum);
-  const int d = tiled_lattice(lat, I, J, K);
+  const int d = lat.nint_pad + O * OWNER_STRIDE + t;
   const int np = f.nglob_pad;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
@@ -369,7 +345,7 @@ __global__ __launch_bounds__(128) void receiver_snapshot_kernel(Lattice lat, int
   if (r >= nrec || p >= 125) return;
   const int e = ispec[r];
   const int ex = e % lat.nx, ey = (e / lat.nx) % lat.ny, ez = e / (lat.nx * lat.ny);
-  const int d = tiled_index(lat, ex, ey, ez, p % 5, (p / 5) % 5, p / 25);
+  const int d = layout_index(lat, 4 * ex + p % 5, 4 * ey + (p / 5) % 5, 4 * ez + p / 25);
   float *o = snap + (static_cast<size_t>(r) * 125 + p) * 6;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
@@ -398,7 +374,7 @@ __global__ __launch_bounds__(128) void seismogram_block_kernel(Lattice lat, int 
     const int ex = e % lat.nx, ey = (e / lat.nx) % lat.ny, ez = e / (lat.nx * lat.ny);
     const int i = p % 5, j = (p / 5) % 5, k = p / 25;
     const int I = 4 * ex + i, J = 4 * ey + j, K = 4 * ez + k;
-    const int d = tiled_lattice(lat, I, J, K);
+    const int d = layout_index(lat, I, J, K);
     hw = h[static_cast<size_t>(r) * 125 + p];
     const float *s = snap + (static_cast<size_t>(r) * 125 + p) * 6;
 #pragma unroll

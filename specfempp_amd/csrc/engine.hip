@@ -151,7 +151,7 @@ struct sfx_engine {
 
   void upload() {
     color_elements(h, run.scatter);
-    lat = Lattice{h.nx, h.ny, h.nz, h.nx + 1, h.ny + 1, h.nz + 1};
+    lat = Lattice{h.nx, h.ny, h.nz, h.nx + 1, h.ny + 1, h.nz + 1, h.nint_pad};
     SFX_HIP(hipStreamCreate(&stream));
     hipStream_t s = stream;
     gather.upload(permuted(h.gather), s);

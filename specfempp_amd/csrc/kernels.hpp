@@ -58,6 +58,7 @@ void launch_stiffness_3d(hipStream_t s, int first, int count, bool atomic, const
 // edges and corner at local index 4.
 struct Lattice {
   int nx, ny, nz, TX, TY, TZ;
+  int nint_pad;  // block layout only: slots of the block-interior points (lattice.hpp)
 };
 // Element forces -K_e u_e of every element into re (plain coalesced stores).
 void launch_elemforce_3d(hipStream_t s, Lattice lat, const float *kappa, const float *mu, float xix, float ety,

@@ -87,4 +87,46 @@ __host__ __device__ __forceinline__ bool block_point_owner(int nbx, int nby, int
   return true;
 }
 
+// Block layout of the fields (SFX_SCATTER_BLOCK): the 343 points strictly
+// inside block b are slots b * 343 + ((lk - 1) * 7 + (lj - 1)) * 7 + li - 1;
+// then, from slot nint_pad (343 * blocks, rounded up to 64), every owner has
+// 176 slots of which its 169 own surface points use owner_slot(0, ...).  The
+// block kernel then reads and writes its interior as one contiguous range and
+// the surface kernel touches only surface slots.
+constexpr int OWNER_STRIDE = 176;
+__host__ __device__ __forceinline__ int block_layout_index(int nbx, int nby, int nint_pad, int I, int J, int K) {
+  const int li = I & 7, lj = J & 7, lk = K & 7;
+  if (li && lj && lk) {
+    const int b = ((K >> 3) * nby + (J >> 3)) * nbx + (I >> 3);
+    return b * 343 + ((lk - 1) * 7 + (lj - 1)) * 7 + (li - 1);
+  }
+  const int o = ((K >> 3) * (nby + 1) + (J >> 3)) * (nbx + 1) + (I >> 3);
+  return nint_pad + o * OWNER_STRIDE + owner_slot(0, li, lj, lk);
+}
+
+// Block-local coordinates (0..8) of the q-th of a block's 386 surface points:
+// the bk = 0 and bk = 8 planes, then the ring of 32 points of each layer
+// bk = 1..7.
+__host__ __device__ __forceinline__ void block_surface_point(int q, int &bi, int &bj, int &bk) {
+  if (q < 162) {
+    bk = q < 81 ? 0 : 8;
+    const int r = q < 81 ? q : q - 81;
+    bj = r / 9;
+    bi = r - bj * 9;
+    return;
+  }
+  const int r0 = q - 162;
+  bk = 1 + r0 / 32;
+  const int r = r0 - (r0 / 32) * 32;
+  if (r < 9) {
+    bj = 0, bi = r;
+  } else if (r < 18) {
+    bj = 8, bi = r - 9;
+  } else if (r < 25) {
+    bi = 0, bj = r - 17;
+  } else {
+    bi = 8, bj = r - 24;
+  }
+}
+
 }  // namespace sfx

@@ -293,15 +293,19 @@ class Engine2D(_EngineBase):
 class Engine3D(_EngineBase):
     """3-D elastic engine on a structured hexahedral mesh (see ``mesh3d``).
 
-    ``scatter="auto"`` picks block assembly when nx, ny and nz are even and the
-    gather path otherwise; "block", "gather", "colored" and "atomic" force one.
+    ``scatter="auto"`` picks the fastest measured path for the mesh (see
+    ``AUTO_3D``); "block", "gather", "colored" and "atomic" force one.
     """
+
+    # path "auto" selects: "gather" (block assembly needs even nx, ny, nz and
+    # is only chosen where it is faster)
+    AUTO_3D = "gather"
 
     def __init__(self, mesh, sources, receivers, dt, nstep, nstep_between_samples=1,
                  seismogram_types=("displacement",), scatter="auto"):
         if scatter == "auto":
             even = all(int(mesh[k]) % 2 == 0 for k in ("nx", "ny", "nz"))
-            scatter = "block" if even else "gather"
+            scatter = self.AUTO_3D if (self.AUTO_3D != "block" or even) else "gather"
         self.scatter = scatter
         self.lib = _native.load()
         self.nstep = nstep

@@ -115,6 +115,32 @@ def config4(nsim: int = 64, seed: int = 4, nmin: int = 40, nmax: int = 240, h: f
     return out
 
 
+def config5(n: int = 200, h: float = 20.0, nstep: int = 3000, dt: float = 5e-4, nsrc: int = 8, nrec: int = 100,
+            seed: int = 5, square: float = 400.0, contrast: float = 0.05):
+    """2-D adjoint workload: n x n quads of h, Stacey on all four sides;
+    background (vp, vs, rho) = (3000, 1732, 2400), true model the background
+    times (1 +- contrast) on a checkerboard of square-metre cells; nsrc Ricker
+    forces (f0 12 Hz, vertical) at seeded points of the inner 80 %, nrec
+    receivers on a line 100 m below the top.  Returns (true, start) workloads
+    sharing sources and receivers."""
+    L = n * h
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(0.1 * L, 0.9 * L, size=(nsrc, 2))
+    src = [dict(x=float(x), z=float(z), angle=0.0, f0=12.0, factor=1e10, tshift=0.0) for x, z in pos]
+    rec = [dict(x=float(x), z=L - 100.0) for x in np.linspace(0.05 * L, 0.95 * L, nrec)]
+    ab = ("bottom", "right", "left", "top")
+    cells = int(round(square / h))
+    ez, ex = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    checker = (((ex // cells) + (ez // cells)) % 2).astype(np.int32)
+    bg = (2400.0, 3000.0, 1732.0)
+    true_mats = [elastic(*(x * (1 + contrast) for x in bg)), elastic(*(x * (1 - contrast) for x in bg))]
+    true = Workload2D(f"2d_adjoint_{n}x{n}_true", quad_mesh(n, n, 0.0, L, 0.0, L, checker, true_mats, ab), src, rec,
+                      dt, nstep, dt, {"courant": courant(dt, h, 3000.0 * (1 + contrast))})
+    start = Workload2D(f"2d_adjoint_{n}x{n}", quad_mesh(n, n, 0.0, L, 0.0, L, None, [elastic(*bg)], ab), src, rec,
+                       dt, nstep, dt, {"courant": courant(dt, h, 3000.0)})
+    return true, start
+
+
 @dataclass
 class Workload3D:
     name: str

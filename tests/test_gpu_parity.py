@@ -308,6 +308,12 @@ def test_pressure_seismograms_match_oracle():
     m = mesh.materials[0]
     kappa = m.density * (m.cp ** 2 - 4.0 / 3.0 * m.cs ** 2)
     exact = -kappa * (a + d)
-    _report("pressure_linear_field", max_rel=float(np.max(np.abs(got - exact)) / abs(exact)))
-    assert np.max(np.abs(got - exact)) <= 1e-4 * abs(exact)
+    # FP32 rounding of the derivatives of a field of magnitude ~3 over 50 m
+    # elements: the FP32 oracle's own error on the same field sets the bar
+    lin = np.zeros((3, 2, o.nglob))
+    lin[0, 0], lin[0, 1] = a * xg + b * zg, c * xg + d * zg
+    o32 = oracle.Oracle2D(mesh, 4, 1, cfg["dt"]).sample(cfg["stations"], lin, ("pressure",))[0, :, 0]
+    err, floor = np.max(np.abs(got - exact)) / abs(exact), np.max(np.abs(o32 - exact)) / abs(exact)
+    _report("pressure_linear_field", max_rel=err, fp32_oracle_max_rel=floor)
+    assert err <= max(1e-4, 2.0 * floor)
     eng.close()

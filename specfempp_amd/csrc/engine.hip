@@ -56,6 +56,8 @@ struct sfx_engine {
   DevBuf<float> rho;         // per point, slot order
   DevBuf<float> adj_fields;  // u, v, a of the adjoint field
   DevBuf<float> kern;        // (6, nspec, 25) slot order
+  DevBuf<float> kacc;        // accumulators of the combined run (6, nspec, 25)
+  DevBuf<float> racc;        // per point: sum over steps of dt a+ . u_b
   DevBuf<int> adj_order, adj_slot;
   DevBuf<float> adj_array, adj_stf;
   int nstacey = 0;
@@ -63,6 +65,11 @@ struct sfx_engine {
   bool recorded = false;     // a complete recorded forward run is in the fields
   bool adj_ready = false;
   int rec_mode = 0;          // stiffness: 0 plain, 1 record, 2 replay
+  // 2-D graph capture: a device step counter read by the step-dependent
+  // kernels, and one captured graph per kind of regular step
+  DevBuf<int> d_step;
+  hipGraphExec_t g_fwd = nullptr, g_rec = nullptr, g_adj = nullptr;
+  bool use_graphs = true;
   float *rec_step = nullptr;
   bool initialized = false;
   bool a_stale = false;  // fused gather steps leave the stored acceleration behind
@@ -77,8 +84,60 @@ struct sfx_engine {
   }
 
   ~sfx_engine() {
+    invalidate_graphs();
     for (auto e : ev) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void invalidate_graphs() {
+    for (hipGraphExec_t *g : {&g_fwd, &g_rec, &g_adj})
+      if (*g) {
+        (void)hipGraphExecDestroy(*g);
+        *g = nullptr;
+      }
+  }
+  template <class Enqueue> hipGraphExec_t capture(Enqueue &&enqueue) {
+    hipGraph_t g;
+    SFX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue();
+    } catch (...) {
+      (void)hipStreamEndCapture(stream, &g);
+      throw;
+    }
+    SFX_HIP(hipStreamEndCapture(stream, &g));
+    hipGraphExec_t ex;
+    const hipError_t err = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    SFX_HIP(err);
+    return ex;
+  }
+  void set_step(int v) {
+    d_step.ensure(1);
+    SFX_HIP(hipMemcpyAsync(d_step.p, &v, sizeof(int), hipMemcpyHostToDevice, stream));
+  }
+  bool graph_2d() const { return use_graphs && h.dim == 2; }
+
+  // A regular 2-D forward step with every step-dependent value read from the
+  // device counter; recording adds the Stacey-element record.
+  void enqueue_forward_2d(bool record_on) {
+    Fields ff = f();
+    const long long rstride = static_cast<long long>(nstacey) * h.npe * h.ncomp;
+    launch_sources(stream, 2, h.npe, h.nsrc, 0, src_order.p, src_ispec.p, gather.p, src_array.p, stf.p, ff, d_step.p);
+    const bool atomic = run.scatter == SFX_SCATTER_ATOMIC;
+    for (size_t c = 0; c + 1 < h.color_offsets.size(); c++) {
+      const int first = h.color_offsets[c], count = h.color_offsets[c + 1] - first;
+      launch_stiffness_2d(stream, first, count, atomic, gather.p, xix.p, xiz.p, gammax.p, gammaz.p, jac.p, kappa.p,
+                          mu.p, q, ff, record_on ? 1 : 0, rec_index.p, record_on ? record.p : nullptr, d_step.p,
+                          rstride);
+    }
+    launch_stacey(stream, 2, static_cast<int>(h.bnd_gather.size()), bnd_gather.p, bnd_normal.p, bnd_weight.p,
+                  bnd_rvp.p, bnd_rvs.p, ff, bnd_rec.p, record_on ? record.p : nullptr, d_step.p, rstride);
+    launch_seismograms(stream, 2, h.npe, h.nrec, run.nseismo_types, types.p, 0, rec_ispec.p, gather.p, rec_h.p, ff,
+                       nc.dto2, 1, seismo.p, PointGeom2D{xix.p, xiz.p, gammax.p, gammaz.p, kappa.p, mu.p}, q,
+                       d_step.p, run.nstep_between_samples);
+    launch_update_fused(stream, h.ncomp, ff, nc);
+    launch_step_tick(stream, d_step.p, 1);
   }
 
   // Per-element array in device slot order (colors contiguous).
@@ -275,10 +334,11 @@ struct sfx_engine {
     if (istep < 0 || istep >= run.nstep) throw std::invalid_argument("istep out of range");
     launch_sources(stream, h.dim, h.npe, h.nsrc, istep, src_order.p, src_ispec.p, gather.p, src_array.p, stf.p, ff);
   }
-  void seismograms(Fields ff, int isig, int corrected) {
+  void seismograms(Fields ff, int isig, int corrected) { seismograms(ff, isig, corrected, nc.dto2); }
+  void seismograms(Fields ff, int isig, int corrected, float dto2) {
     if (isig < 0 || isig >= nsig) throw std::invalid_argument("seismogram step out of range");
     launch_seismograms(stream, h.dim, h.npe, h.nrec, run.nseismo_types, types.p, isig, rec_ispec.p, gather.p, rec_h.p,
-                       ff, nc.dto2, corrected, seismo.p, PointGeom2D{xix.p, xiz.p, gammax.p, gammaz.p, kappa.p, mu.p},
+                       ff, dto2, corrected, seismo.p, PointGeom2D{xix.p, xiz.p, gammax.p, gammaz.p, kappa.p, mu.p},
                        q);
   }
   void require_init() const {
@@ -366,50 +426,100 @@ struct sfx_engine {
   // backward field is the forward field itself: it holds the forward run's
   // final state, which is where the reference's backward field stands after
   // its first step (the state read back is copied in at istep = nstep - 1), so
-  // that step is not run.
+  // that step is not run.  Between steps both fields hold their predicted
+  // state.  A step: sources, both fields' forces in one launch per color (with
+  // the strain parts of the Fréchet kernels), the adjoint's Stacey traction,
+  // backward seismograms, and one fused pass of division, correctors, the rho
+  // kernel's product and the next predictors.  At a checkpoint step the
+  // backward state is reset to the forward snapshot after its corrector, so
+  // that step runs the phases separately.
   void adjoint_run() {
     const size_t stride = static_cast<size_t>(h.ncomp) * h.nglob_pad;
-    adj_fields.alloc(3 * stride);
+    adj_fields.ensure(3 * stride);
     launch_fill(stream, adj_fields.p, adj_fields.n, 0.0f);
-    kern.alloc(6 * static_cast<size_t>(h.nspec) * h.npe);
-    launch_fill(stream, kern.p, kern.n, 0.0f);
+    const size_t npts = static_cast<size_t>(h.nspec) * h.npe;
+    kern.ensure(6 * npts);
+    kacc.ensure(6 * npts);
+    launch_fill(stream, kacc.p, kacc.n, 0.0f);
+    racc.ensure(h.nglob_pad);
+    launch_fill(stream, racc.p, racc.n, 0.0f);
     if (seismo.n) launch_fill(stream, seismo.p, seismo.n, 0.0f);
     Fields bwd = f();
     Fields adj{adj_fields.p, adj_fields.p + stride, adj_fields.p + 2 * stride, bwd.minv, h.nglob_pad};
+    float *Kmu = kacc.p + npts, *Kkappa = kacc.p + 2 * npts;
     const float dt = nc.dt;
     const NewmarkConst back{-dt, -nc.dto2, nc.dt2o2};  // newmark.tpp:204-242
     const int nstep = run.nstep, nsb = run.nstep_between_samples;
+    const bool atomic = run.scatter == SFX_SCATTER_ATOMIC;
+    const size_t rec_stride = static_cast<size_t>(nstacey) * h.npe * h.ncomp;
+    const int sig_top = (nstep - 1) / nsb;  // sample index 0 is the latest sampled step
+    // a regular step (neither the first nor a checkpoint), all step-dependent
+    // values read from the device counter, which counts down
+    auto enqueue_regular = [&] {
+      launch_sources(stream, 2, h.npe, 2 * h.nadj, 0, adj_order.p, adj_slot.p, gather.p, adj_array.p, adj_stf.p, adj,
+                     d_step.p);
+      launch_sources(stream, 2, h.npe, h.nsrc, 0, src_order.p, src_ispec.p, gather.p, src_array.p, stf.p, bwd,
+                     d_step.p);
+      for (size_t c = 0; c + 1 < h.color_offsets.size(); c++) {
+        const int c0 = h.color_offsets[c], cnt = h.color_offsets[c + 1] - c0;
+        launch_stiffness_pair_2d(stream, c0, cnt, atomic, gather.p, xix.p, xiz.p, gammax.p, gammaz.p, jac.p, kappa.p,
+                                 mu.p, q, adj, bwd, 1, rec_index.p, record.p, 1, dt, h.nspec, Kmu, Kkappa, d_step.p,
+                                 static_cast<long long>(rec_stride));
+      }
+      boundary(adj);
+      launch_seismograms(stream, 2, h.npe, h.nrec, run.nseismo_types, types.p, 0, rec_ispec.p, gather.p, rec_h.p, bwd,
+                         back.dto2, 1, seismo.p, PointGeom2D{xix.p, xiz.p, gammax.p, gammaz.p, kappa.p, mu.p}, q,
+                         d_step.p, nsb, sig_top);
+      launch_update_pair(stream, h.ncomp, adj, bwd, nc, 1, racc.p);
+      launch_step_tick(stream, d_step.p, -1);
+    };
+    const bool graphs = graph_2d();
+    if (graphs) set_step(nstep - 1);
     int isig = 0;
     for (int istep = nstep - 1; istep >= 0; istep--) {
-      // adjoint field: forward-in-time Newmark with the adjoint sources
-      launch_predictor(stream, h.ncomp, adj, nc);
+      const bool first = istep == nstep - 1;
+      const bool ckpt = is_checkpoint(istep);
+      const bool sample = istep % nsb == 0;
+      if (graphs && !first && !ckpt) {
+        if (!g_adj) g_adj = capture(enqueue_regular);
+        SFX_HIP(hipGraphLaunch(g_adj, stream));
+        if (sample) isig++;
+        continue;
+      }
       launch_sources(stream, 2, h.npe, 2 * h.nadj, istep, adj_order.p, adj_slot.p, gather.p, adj_array.p, adj_stf.p,
                      adj);
-      stiffness(adj);
-      boundary(adj);
-      launch_divide_corrector(stream, h.ncomp, adj, nc, 1, 1);
-      // backward field: -dt, forward sources, recorded Stacey elements
-      if (istep < nstep - 1) {
-        launch_predictor(stream, h.ncomp, bwd, back);
-        sources(bwd, istep);
-        rec_mode = 2;
-        rec_step = record.p + static_cast<size_t>(istep) * nstacey * h.npe * h.ncomp;
-        stiffness(bwd);
-        rec_mode = 0;
-        rec_step = nullptr;
-        launch_divide_corrector(stream, h.ncomp, bwd, back, 1, 1);
-        // checkpoint: after istep the backward state is the forward state
-        // after step istep; reset it to bound the reconstruction's drift
-        if (is_checkpoint(istep)) {
-          const size_t n3 = 3 * stride;
-          SFX_HIP(hipMemcpyAsync(fields.p, snapshots.p + static_cast<size_t>(istep / snapshot_every) * n3,
-                                 n3 * sizeof(float), hipMemcpyDeviceToDevice, stream));
-        }
+      if (!first) sources(bwd, istep);
+      const float *rec_i = record.p + static_cast<size_t>(istep) * rec_stride;
+      for (size_t c = 0; c + 1 < h.color_offsets.size(); c++) {
+        const int c0 = h.color_offsets[c], cnt = h.color_offsets[c + 1] - c0;
+        launch_stiffness_pair_2d(stream, c0, cnt, atomic, gather.p, xix.p, xiz.p, gammax.p, gammaz.p, jac.p, kappa.p,
+                                 mu.p, q, adj, bwd, first ? 0 : 1, rec_index.p, rec_i, ckpt ? 0 : 1, dt, h.nspec, Kmu,
+                                 Kkappa);
       }
-      launch_frechet_2d(stream, h.nspec, gather.p, xix.p, xiz.p, gammax.p, gammaz.p, kappa.p, mu.p, rho.p, q, adj, bwd,
-                        dt, kern.p);
-      if (istep % nsb == 0) seismograms(bwd, isig++, 0);
+      boundary(adj);
+      if (ckpt) {
+        launch_divide_corrector(stream, h.ncomp, adj, nc, 1, 1);
+        launch_divide_corrector(stream, h.ncomp, bwd, back, 1, 1);
+        // after istep the backward state is the forward state after step
+        // istep: reset it to bound the reconstruction's drift
+        const size_t n3 = 3 * stride;
+        SFX_HIP(hipMemcpyAsync(fields.p, snapshots.p + static_cast<size_t>(istep / snapshot_every) * n3,
+                               n3 * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        launch_frechet_2d(stream, h.nspec, gather.p, xix.p, xiz.p, gammax.p, gammaz.p, kappa.p, mu.p, rho.p, q, adj,
+                          bwd, dt, kacc.p);
+        launch_rho_product(stream, h.ncomp, adj, bwd, dt, racc.p);
+        if (sample) seismograms(bwd, isig++, 0);
+        launch_predictor(stream, h.ncomp, adj, nc);
+        launch_predictor(stream, h.ncomp, bwd, back);
+      } else {
+        // the first backward state is already corrected; later ones are
+        // sampled as they will be after the backward corrector (-dt / 2)
+        if (sample) seismograms(bwd, isig++, first ? 0 : 1, back.dto2);
+        launch_update_pair(stream, h.ncomp, adj, bwd, nc, first ? 0 : 1, racc.p);
+      }
+      if (graphs) launch_step_tick(stream, d_step.p, -1);
     }
+    launch_frechet_finish(stream, static_cast<int>(npts), gather.p, kappa.p, mu.p, rho.p, racc.p, Kmu, Kkappa, kern.p);
   }
 };
 
@@ -620,6 +730,25 @@ int sfx_run(sfx_engine *e, int first, int count) {
     check_engine(e);
     e->require_init();
     if (first < 0 || count < 0 || first + count > e->run.nstep) throw std::invalid_argument("step range out of bounds");
+    if (e->graph_2d() && count > 1) {
+      // 2-D: one captured graph per step; only the whole-step time is taken
+      e->set_step(first);
+      if (!e->g_fwd) e->g_fwd = e->capture([&] { e->enqueue_forward_2d(false); });
+      while (e->ev.size() < 2) {
+        hipEvent_t ev;
+        SFX_HIP(hipEventCreate(&ev));
+        e->ev.push_back(ev);
+      }
+      SFX_HIP(hipEventRecord(e->ev[0], e->stream));
+      for (int i = 0; i < count; i++) SFX_HIP(hipGraphLaunch(e->g_fwd, e->stream));
+      SFX_HIP(hipEventRecord(e->ev[1], e->stream));
+      SFX_HIP(hipStreamSynchronize(e->stream));
+      float all = 0.0f;
+      SFX_HIP(hipEventElapsedTime(&all, e->ev[0], e->ev[1]));
+      e->timings[0] = e->timings[1] = e->timings[2] = 0.0;
+      e->timings[3] = all / count;
+      return;
+    }
     // Every step records 5 events on the engine stream; they are read after
     // the last step, so the host never waits inside the loop.
     const size_t need = 5 * static_cast<size_t>(count);
@@ -727,6 +856,7 @@ int sfx_set_properties(sfx_engine *e, const float *rho, const float *kappa, cons
                            hipMemcpyHostToDevice, s));
     SFX_HIP(hipStreamSynchronize(s));
     e->recorded = false;
+    e->invalidate_graphs();
   });
 }
 
@@ -745,7 +875,18 @@ int sfx_forward_record(sfx_engine *e, int snapshot_every) {
     e->recorded = false;
     e->recording = true;
     try {
-      for (int istep = 0; istep < e->run.nstep; istep++) e->step(istep, nullptr, istep == e->run.nstep - 1);
+      const bool graphs = e->graph_2d();
+      if (graphs) e->set_step(0);
+      for (int istep = 0; istep < e->run.nstep; istep++) {
+        const bool last = istep == e->run.nstep - 1;
+        if (graphs && !last && !e->is_checkpoint(istep)) {
+          if (!e->g_rec) e->g_rec = e->capture([&] { e->enqueue_forward_2d(true); });
+          SFX_HIP(hipGraphLaunch(e->g_rec, e->stream));
+        } else {
+          e->step(istep, nullptr, last);
+          if (graphs) launch_step_tick(e->stream, e->d_step.p, 1);
+        }
+      }
     } catch (...) {
       e->recording = false;
       throw;
@@ -768,6 +909,7 @@ int sfx_set_adjoint_sources(sfx_engine *e, int nadj, const sfx_receiver *positio
     e->adj_stf.upload(e->h.adj_stf, s);
     SFX_HIP(hipStreamSynchronize(s));
     e->adj_ready = true;
+    e->invalidate_graphs();
   });
 }
 

@@ -48,7 +48,7 @@ __global__ __launch_bounds__(threads_for(E)) void stiffness_2d_kernel(
     int first, int count, const int *__restrict__ gather, const float *__restrict__ xix_, const float *__restrict__ xiz_,
     const float *__restrict__ gammax_, const float *__restrict__ gammaz_, const float *__restrict__ jac_,
     const float *__restrict__ kappa_, const float *__restrict__ mu_, GllConst q, Fields f, int rec_mode,
-    const int *__restrict__ rec_index, float *__restrict__ rec) {
+    const int *__restrict__ rec_index, float *__restrict__ rec, const int *__restrict__ step, long long rec_stride) {
   __shared__ float su[E][2][25];
   __shared__ float sF[E][4][25];  // F(c, d) as [2c + d]
 
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(threads_for(E)) void stiffness_2d_kernel(
   if (rec_mode) {
     const int ri = rec_index[e];
     if (ri >= 0) {
-      float *rp = rec + (static_cast<size_t>(ri) * 25 + p) * 2;
+      float *rp = rec + (step ? *step * rec_stride : 0) + (static_cast<size_t>(ri) * 25 + p) * 2;
       if (rec_mode == 1) {
         rp[0] = r[0];
         rp[1] = r[1];
@@ -147,6 +147,222 @@ __global__ __launch_bounds__(threads_for(E)) void stiffness_2d_kernel(
   }
 #pragma unroll
   for (int c = 0; c < 2; c++) scatter_add<ATOMIC>(&f.a[c * f.nglob_pad + g], r[c]);
+}
+
+// Adjoint and backward fields together (the combined loop's two
+// update_wavefields calls, time_marching.tpp:160-186): one pass over the
+// elements loads the geometry and properties once and computes both fields'
+// forces.  The backward field replays the recorded force of Stacey elements
+// (compute_stiffness_interaction.tpp:119-136) and adds nothing when
+// bwd_active is 0.  With frechet set, the strain parts of the kernels
+// (frechet_derivative.hpp:58-118: mu and kappa, each times its coefficient and
+// dt) are accumulated at every element point from the same gradients: both
+// displacements are the fields' values after this step.
+template <bool ATOMIC>
+__global__ __launch_bounds__(128) void stiffness_pair_2d_kernel(
+    int first, int count, const int *__restrict__ gather, const float *__restrict__ xix_, const float *__restrict__ xiz_,
+    const float *__restrict__ gammax_, const float *__restrict__ gammaz_, const float *__restrict__ jac_,
+    const float *__restrict__ kappa_, const float *__restrict__ mu_, GllConst q, Fields adj, Fields bwd, int bwd_active,
+    const int *__restrict__ rec_index, const float *__restrict__ rec, int frechet, float dt, int nspec,
+    float *__restrict__ Kmu, float *__restrict__ Kkappa, const int *__restrict__ step, long long rec_stride) {
+  __shared__ float su[2][E2][2][25];  // [field][element][comp][point]
+  __shared__ float sF[2][E2][4][25];
+
+  const int t = threadIdx.x;
+  const int el = t / 25;
+  const int p = t - el * 25;
+  const int iz = p / 5, ix = p - (p / 5) * 5;
+  const int el_idx = blockIdx.x * E2 + el;
+  const bool active = (el < E2) && (el_idx < count);
+  const int e = first + el_idx;
+  const size_t idx = static_cast<size_t>(e) * 25 + p;
+  const int np = adj.nglob_pad;
+
+  int g = 0;
+  if (active) {
+    g = gather[idx];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      su[0][el][c][p] = adj.u[c * np + g];
+      su[1][el][c][p] = bwd.u[c * np + g];
+    }
+  }
+  __syncthreads();
+
+  float hpx[5], hpz[5], hwx[5], hwz[5];
+#pragma unroll
+  for (int l = 0; l < 5; l++) {
+    hpx[l] = q.hp[ix][l];
+    hpz[l] = q.hp[iz][l];
+    hwx[l] = q.hpw[ix][l];
+    hwz[l] = q.hpw[iz][l];
+  }
+  const float wx = q.w[ix], wz = q.w[iz];
+  if (active) {
+    const float xix = xix_[idx], xiz = xiz_[idx], gammax = gammax_[idx], gammaz = gammaz_[idx], J = jac_[idx];
+    const float kappa = kappa_[idx], mu = mu_[idx];
+    const float lp2m = kappa + (4.0f / 3.0f) * mu;
+    const float lambda = lp2m - 2.0f * mu;
+    float du[2][2][2];  // [field][c][j]
+#pragma unroll
+    for (int fi = 0; fi < 2; fi++) {
+      float dxi[2] = {0.0f, 0.0f}, dga[2] = {0.0f, 0.0f};
+#pragma unroll
+      for (int l = 0; l < 5; l++) {
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+          dxi[c] += hpx[l] * su[fi][el][c][iz * 5 + l];
+          dga[c] += hpz[l] * su[fi][el][c][l * 5 + ix];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        du[fi][c][0] = xix * dxi[c] + gammax * dga[c];
+        du[fi][c][1] = xiz * dxi[c] + gammaz * dga[c];
+      }
+      const float sxx = lp2m * du[fi][0][0] + lambda * du[fi][1][1];
+      const float szz = lp2m * du[fi][1][1] + lambda * du[fi][0][0];
+      const float sxz = mu * (du[fi][0][1] + du[fi][1][0]);
+      sF[fi][el][0][p] = J * (sxx * xix + sxz * xiz);
+      sF[fi][el][1][p] = J * (sxx * gammax + sxz * gammaz);
+      sF[fi][el][2][p] = J * (sxz * xix + szz * xiz);
+      sF[fi][el][3][p] = J * (sxz * gammax + szz * gammaz);
+    }
+    if (frechet) {
+      const float kap2 = lp2m - mu;
+      const float ad_dsxx = du[0][0][0];
+      const float ad_dsxz = 0.5f * (du[0][1][0] + du[0][0][1]);
+      const float ad_dszz = du[0][1][1];
+      const float b_dsxx = du[1][0][0];
+      const float b_dsxz = 0.5f * (du[1][1][0] + du[1][0][1]);
+      const float b_dszz = du[1][1][1];
+      const float kappa_kl = (ad_dsxx + ad_dszz) * (b_dsxx + b_dszz);
+      const float mu_kl =
+          (ad_dsxx * b_dsxx + ad_dszz * b_dszz + 2.0f * ad_dsxz * b_dsxz - (1.0f / 3.0f) * kappa_kl);
+      const size_t kidx = static_cast<size_t>(e) * 25 + p;
+      Kkappa[kidx] += -1.0f * kap2 * dt * kappa_kl;
+      Kmu[kidx] += -2.0f * mu * dt * mu_kl;
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+#pragma unroll
+  for (int fi = 0; fi < 2; fi++) {
+    if (fi == 1 && !bwd_active) break;
+    float t1[2] = {0.0f, 0.0f}, t2[2] = {0.0f, 0.0f};
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        t1[c] += sF[fi][el][2 * c][iz * 5 + l] * hwx[l];
+        t2[c] += sF[fi][el][2 * c + 1][l * 5 + ix] * hwz[l];
+      }
+    }
+    float r[2];
+#pragma unroll
+    for (int c = 0; c < 2; c++) r[c] = -(wz * t1[c] + wx * t2[c]);
+    if (fi == 1) {
+      const int ri = rec_index[e];
+      if (ri >= 0) {
+        const float *rp = rec + (step ? *step * rec_stride : 0) + (static_cast<size_t>(ri) * 25 + p) * 2;
+        r[0] = rp[0];
+        r[1] = rp[1];
+      }
+    }
+    Fields &f = fi == 0 ? adj : bwd;
+#pragma unroll
+    for (int c = 0; c < 2; c++) scatter_add<ATOMIC>(&f.a[c * np + g], r[c]);
+  }
+}
+
+// End of a combined step for both fields (float4 over points): mass division
+// and corrector (newmark.tpp:11-56; the backward with -dt/2, :204-222), the
+// rho kernel's pointwise product R += dt a+ . u_b (frechet_derivative.hpp:96-110;
+// a+ after the division, u_b this step's backward displacement), then the
+// predictors of the next step (+dt for the adjoint, -dt for the backward,
+// newmark.tpp:225-242).  bwd_correct = 0 when the backward state is already
+// corrected (the first step, where it is the forward run's final state).
+__global__ __launch_bounds__(256) void update_pair_kernel(int n4, int ncomp, Fields adj, Fields bwd, NewmarkConst nc,
+                                                          int bwd_correct, float *__restrict__ R) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n4) return;
+  const size_t n4s = adj.nglob_pad / 4;
+  float4 rho_acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  for (int c = 0; c < ncomp; c++) {
+    const size_t o = static_cast<size_t>(c) * n4s + i;
+    const float4 m = reinterpret_cast<const float4 *>(adj.minv)[o];
+    float4 a = reinterpret_cast<const float4 *>(adj.a)[o];
+    float4 u = reinterpret_cast<const float4 *>(adj.u)[o];
+    float4 v = reinterpret_cast<const float4 *>(adj.v)[o];
+    float4 ab = reinterpret_cast<const float4 *>(bwd.a)[o];
+    float4 ub = reinterpret_cast<const float4 *>(bwd.u)[o];
+    float4 vb = reinterpret_cast<const float4 *>(bwd.v)[o];
+#define SFX_PAIR(X)                                          \
+  {                                                          \
+    a.X = a.X * m.X;                                         \
+    v.X = v.X + nc.dto2 * a.X;                               \
+    if (bwd_correct) {                                       \
+      ab.X = ab.X * m.X;                                     \
+      vb.X = vb.X + (-nc.dto2) * ab.X;                       \
+    }                                                        \
+    rho_acc.X += a.X * ub.X;                                 \
+    u.X = u.X + (nc.dt * v.X + nc.dt2o2 * a.X);              \
+    v.X = v.X + nc.dto2 * a.X;                               \
+    ub.X = ub.X + (-nc.dt * vb.X + nc.dt2o2 * ab.X);         \
+    vb.X = vb.X + (-nc.dto2) * ab.X;                         \
+  }
+    SFX_PAIR(x) SFX_PAIR(y) SFX_PAIR(z) SFX_PAIR(w)
+#undef SFX_PAIR
+    reinterpret_cast<float4 *>(adj.u)[o] = u;
+    reinterpret_cast<float4 *>(adj.v)[o] = v;
+    reinterpret_cast<float4 *>(adj.a)[o] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    reinterpret_cast<float4 *>(bwd.u)[o] = ub;
+    reinterpret_cast<float4 *>(bwd.v)[o] = vb;
+    reinterpret_cast<float4 *>(bwd.a)[o] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  float4 rr = reinterpret_cast<float4 *>(R)[i];
+  rr.x += nc.dt * rho_acc.x;
+  rr.y += nc.dt * rho_acc.y;
+  rr.z += nc.dt * rho_acc.z;
+  rr.w += nc.dt * rho_acc.w;
+  reinterpret_cast<float4 *>(R)[i] = rr;
+}
+
+// R += dt a+ . u_b on corrected states (checkpoint steps).
+__global__ __launch_bounds__(256) void rho_product_kernel(int n, int ncomp, Fields adj, Fields bwd, float dt,
+                                                          float *__restrict__ R) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.0f;
+  for (int c = 0; c < ncomp; c++) s += adj.a[static_cast<size_t>(c) * adj.nglob_pad + i] *
+                                      bwd.u[static_cast<size_t>(c) * bwd.nglob_pad + i];
+  R[i] += dt * s;
+}
+
+// The six kernels from the accumulated mu and kappa kernels and R
+// (frechet_derivative.hpp:110-135): rho = -rho_p R(g), rhop = rho + kappa + mu,
+// beta = 2 (mu - 4/3 mu_p / kappa_p kappa), alpha = 2 (1 + 4/3 mu_p / kappa_p) kappa,
+// with kappa_p = lambda + mu.  out (6, nspec, 25).
+__global__ __launch_bounds__(256) void frechet_finish_kernel(int npts, const int *__restrict__ gather,
+                                                             const float *__restrict__ kappa_,
+                                                             const float *__restrict__ mu_,
+                                                             const float *__restrict__ rho_,
+                                                             const float *__restrict__ R,
+                                                             const float *__restrict__ Kmu,
+                                                             const float *__restrict__ Kkappa,
+                                                             float *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npts) return;
+  const float mu = mu_[i], rho = rho_[i];
+  const float kap2 = kappa_[i] + (4.0f / 3.0f) * mu - mu;
+  const float rho_kl = -1.0f * rho * R[gather[i]];
+  const float mu_kl = Kmu[i], kappa_kl = Kkappa[i];
+  out[i] = rho_kl;
+  out[npts + i] = mu_kl;
+  out[2 * static_cast<size_t>(npts) + i] = kappa_kl;
+  out[3 * static_cast<size_t>(npts) + i] = rho_kl + kappa_kl + mu_kl;
+  out[4 * static_cast<size_t>(npts) + i] = 2.0f * (1.0f + (4.0f / 3.0f) * mu / kap2) * kappa_kl;
+  out[5 * static_cast<size_t>(npts) + i] = 2.0f * (mu_kl - (4.0f / 3.0f) * mu / kap2 * kappa_kl);
 }
 
 // Fréchet derivatives of one step (compute_material_derivatives.tpp:17-154,
@@ -348,7 +564,8 @@ __global__ __launch_bounds__(256) void stacey_kernel(int nbnd, const int *__rest
                                                      const float *__restrict__ weight,
                                                      const float *__restrict__ rho_vp,
                                                      const float *__restrict__ rho_vs, Fields f,
-                                                     const int *__restrict__ bnd_rec, float *__restrict__ rec) {
+                                                     const int *__restrict__ bnd_rec, float *__restrict__ rec,
+                                                     const int *__restrict__ step, long long rec_stride) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nbnd) return;
   const int g = gather[q];
@@ -369,7 +586,7 @@ __global__ __launch_bounds__(256) void stacey_kernel(int nbnd, const int *__rest
     const float fac = ((vn * n[d] / (j1 * j1)) * (rvp - rvs)) + v[d] * rvs;
     const float tr = -1.0f * fac * j1 * w;
     atomic_add(&f.a[d * np + g], tr);
-    if (rec) rec[static_cast<size_t>(bnd_rec[q]) * D + d] += tr;
+    if (rec) rec[(step ? *step * rec_stride : 0) + static_cast<size_t>(bnd_rec[q]) * D + d] += tr;
   }
 }
 
@@ -377,12 +594,14 @@ __global__ __launch_bounds__(256) void stacey_kernel(int nbnd, const int *__rest
 // Sources: a += stf(istep, s) * source_array(s, c, p)
 // ---------------------------------------------------------------------------
 template <int D>
-__global__ __launch_bounds__(128) void sources_kernel(int npe, int nsrc, int istep, const int *__restrict__ order,
+__global__ __launch_bounds__(128) void sources_kernel(int npe, int nsrc, int istep_arg, const int *__restrict__ order,
                                                       const int *__restrict__ ispec, const int *__restrict__ gather,
                                                       const float *__restrict__ array,
-                                                      const float *__restrict__ stf, Fields f) {
+                                                      const float *__restrict__ stf, Fields f,
+                                                      const int *__restrict__ step) {
   const int p = threadIdx.x;
   if (p >= npe) return;
+  const int istep = step ? *step : istep_arg;
   for (int k = 0; k < nsrc; k++) {
     const int s = order[k];
     const float sv = stf[static_cast<size_t>(istep) * nsrc + s];
@@ -428,9 +647,15 @@ __global__ __launch_bounds__(128) void seismogram_kernel(int npe, int nrec, int 
                                                          int isig, const int *__restrict__ ispec,
                                                          const int *__restrict__ gather, const float *__restrict__ h,
                                                          Fields f, float dto2, int corrected,
-                                                         float *__restrict__ seismo, PointGeom2D pg, GllConst q) {
+                                                         float *__restrict__ seismo, PointGeom2D pg, GllConst q,
+                                                         const int *__restrict__ step, int nsb, int sig_top) {
   __shared__ float red[2][D][64];
   __shared__ float su[2][25];
+  if (step) {  // sample only on steps istep % nsb == 0 (the same for the whole block)
+    const int is = *step;
+    if (is % nsb != 0) return;
+    isig = sig_top >= 0 ? sig_top - is / nsb : is / nsb;  // sig_top: samples counted down in time
+  }
   const int r = blockIdx.x;
   const int p = threadIdx.x;
   const int np = f.nglob_pad;
@@ -895,6 +1120,10 @@ __global__ __launch_bounds__(128) void seismogram_re_kernel(Lattice lat, int nre
   }
 }
 
+__global__ void step_tick_kernel(int *step, int delta) {
+  if (threadIdx.x == 0) *step = *step + delta;
+}
+
 __global__ void fill_kernel(float *p, size_t n, float value) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = value;
@@ -907,14 +1136,14 @@ inline unsigned blocks(size_t n, unsigned per) { return static_cast<unsigned>((n
 void launch_stiffness_2d(hipStream_t s, int first, int count, bool atomic, const int *gather, const float *xix,
                          const float *xiz, const float *gammax, const float *gammaz, const float *jac,
                          const float *kappa, const float *mu, const GllConst &q, Fields f, int rec_mode,
-                         const int *rec_index, float *rec) {
+                         const int *rec_index, float *rec, const int *step, long long rec_stride) {
   if (count <= 0) return;
   if (atomic)
     hipLaunchKernelGGL(stiffness_2d_kernel<true>, dim3(blocks(count, E2)), dim3(128), 0, s, first, count, gather, xix,
-                       xiz, gammax, gammaz, jac, kappa, mu, q, f, rec_mode, rec_index, rec);
+                       xiz, gammax, gammaz, jac, kappa, mu, q, f, rec_mode, rec_index, rec, step, rec_stride);
   else
     hipLaunchKernelGGL(stiffness_2d_kernel<false>, dim3(blocks(count, E2)), dim3(128), 0, s, first, count, gather, xix,
-                       xiz, gammax, gammaz, jac, kappa, mu, q, f, rec_mode, rec_index, rec);
+                       xiz, gammax, gammaz, jac, kappa, mu, q, f, rec_mode, rec_index, rec, step, rec_stride);
 }
 
 template <int E>
@@ -924,10 +1153,10 @@ void launch_stiffness_2d_e(hipStream_t s, int first, int count, bool atomic, con
   if (count <= 0) return;
   if (atomic)
     hipLaunchKernelGGL((stiffness_2d_kernel<true, E>), dim3(blocks(count, E)), dim3(threads_for(E)), 0, s, first,
-                       count, gather, xix, xiz, gammax, gammaz, jac, kappa, mu, q, f, 0, nullptr, nullptr);
+                       count, gather, xix, xiz, gammax, gammaz, jac, kappa, mu, q, f, 0, nullptr, nullptr, nullptr, 0LL);
   else
     hipLaunchKernelGGL((stiffness_2d_kernel<false, E>), dim3(blocks(count, E)), dim3(threads_for(E)), 0, s, first,
-                       count, gather, xix, xiz, gammax, gammaz, jac, kappa, mu, q, f, 0, nullptr, nullptr);
+                       count, gather, xix, xiz, gammax, gammaz, jac, kappa, mu, q, f, 0, nullptr, nullptr, nullptr, 0LL);
 }
 
 void launch_stiffness_2d_tuned(hipStream_t s, int elems_per_block, int first, int count, bool atomic,
@@ -944,6 +1173,45 @@ void launch_stiffness_2d_tuned(hipStream_t s, int elems_per_block, int first, in
     default:
       launch_stiffness_2d_e<5>(s, first, count, atomic, gather, xix, xiz, gammax, gammaz, jac, kappa, mu, q, f);
   }
+}
+
+void launch_stiffness_pair_2d(hipStream_t s, int first, int count, bool atomic, const int *gather,
+                              const float *xix, const float *xiz, const float *gammax, const float *gammaz,
+                              const float *jac, const float *kappa, const float *mu, const GllConst &q, Fields adj,
+                              Fields bwd, int bwd_active, const int *rec_index, const float *rec, int frechet,
+                              float dt, int nspec, float *Kmu, float *Kkappa, const int *step,
+                              long long rec_stride) {
+  if (count <= 0) return;
+  if (atomic)
+    hipLaunchKernelGGL(stiffness_pair_2d_kernel<true>, dim3(blocks(count, E2)), dim3(128), 0, s, first, count, gather,
+                       xix, xiz, gammax, gammaz, jac, kappa, mu, q, adj, bwd, bwd_active, rec_index, rec, frechet, dt,
+                       nspec, Kmu, Kkappa, step, rec_stride);
+  else
+    hipLaunchKernelGGL(stiffness_pair_2d_kernel<false>, dim3(blocks(count, E2)), dim3(128), 0, s, first, count, gather,
+                       xix, xiz, gammax, gammaz, jac, kappa, mu, q, adj, bwd, bwd_active, rec_index, rec, frechet, dt,
+                       nspec, Kmu, Kkappa, step, rec_stride);
+}
+
+void launch_step_tick(hipStream_t s, int *step, int delta) {
+  hipLaunchKernelGGL(step_tick_kernel, dim3(1), dim3(64), 0, s, step, delta);
+}
+
+void launch_update_pair(hipStream_t s, int ncomp, Fields adj, Fields bwd, NewmarkConst nc, int bwd_correct, float *R) {
+  const int n4 = adj.nglob_pad / 4;
+  hipLaunchKernelGGL(update_pair_kernel, dim3(blocks(n4, 256)), dim3(256), 0, s, n4, ncomp, adj, bwd, nc, bwd_correct,
+                     R);
+}
+
+void launch_rho_product(hipStream_t s, int ncomp, Fields adj, Fields bwd, float dt, float *R) {
+  hipLaunchKernelGGL(rho_product_kernel, dim3(blocks(adj.nglob_pad, 256)), dim3(256), 0, s, adj.nglob_pad, ncomp, adj,
+                     bwd, dt, R);
+}
+
+void launch_frechet_finish(hipStream_t s, int npts, const int *gather, const float *kappa, const float *mu,
+                           const float *rho, const float *R, const float *Kmu, const float *Kkappa, float *out) {
+  if (npts <= 0) return;
+  hipLaunchKernelGGL(frechet_finish_kernel, dim3(blocks(npts, 256)), dim3(256), 0, s, npts, gather, kappa, mu, rho, R,
+                     Kmu, Kkappa, out);
 }
 
 void launch_frechet_2d(hipStream_t s, int nspec, const int *gather, const float *xix, const float *xiz,
@@ -966,37 +1234,39 @@ void launch_stiffness_3d(hipStream_t s, int first, int count, bool atomic, const
 }
 
 void launch_stacey(hipStream_t s, int dim, int nbnd, const int *gather, const float *normal, const float *weight,
-                   const float *rho_vp, const float *rho_vs, Fields f, const int *bnd_rec, float *rec) {
+                   const float *rho_vp, const float *rho_vs, Fields f, const int *bnd_rec, float *rec,
+                   const int *step, long long rec_stride) {
   if (nbnd <= 0) return;
   if (dim == 2)
     hipLaunchKernelGGL(stacey_kernel<2>, dim3(blocks(nbnd, 256)), dim3(256), 0, s, nbnd, gather, normal, weight,
-                       rho_vp, rho_vs, f, bnd_rec, rec);
+                       rho_vp, rho_vs, f, bnd_rec, rec, step, rec_stride);
   else
     hipLaunchKernelGGL(stacey_kernel<3>, dim3(blocks(nbnd, 256)), dim3(256), 0, s, nbnd, gather, normal, weight,
-                       rho_vp, rho_vs, f, bnd_rec, rec);
+                       rho_vp, rho_vs, f, bnd_rec, rec, step, rec_stride);
 }
 
 void launch_sources(hipStream_t s, int dim, int npe, int nsrc, int istep, const int *order, const int *ispec,
-                    const int *gather, const float *array, const float *stf, Fields f) {
+                    const int *gather, const float *array, const float *stf, Fields f, const int *step) {
   if (nsrc <= 0) return;
   if (dim == 2)
     hipLaunchKernelGGL(sources_kernel<2>, dim3(1), dim3(128), 0, s, npe, nsrc, istep, order, ispec, gather, array, stf,
-                       f);
+                       f, step);
   else
     hipLaunchKernelGGL(sources_kernel<3>, dim3(1), dim3(128), 0, s, npe, nsrc, istep, order, ispec, gather, array, stf,
-                       f);
+                       f, step);
 }
 
 void launch_seismograms(hipStream_t s, int dim, int npe, int nrec, int ntypes, const int *types, int isig,
                         const int *ispec, const int *gather, const float *h, Fields f, float dto2, int corrected,
-                        float *seismo, const PointGeom2D &pg, const GllConst &q) {
+                        float *seismo, const PointGeom2D &pg, const GllConst &q, const int *step, int nsb,
+                        int sig_top) {
   if (nrec <= 0 || ntypes <= 0) return;
   if (dim == 2)
     hipLaunchKernelGGL(seismogram_kernel<2>, dim3(nrec), dim3(128), 0, s, npe, nrec, ntypes, types, isig, ispec,
-                       gather, h, f, dto2, corrected, seismo, pg, q);
+                       gather, h, f, dto2, corrected, seismo, pg, q, step, nsb, sig_top);
   else
     hipLaunchKernelGGL(seismogram_kernel<3>, dim3(nrec), dim3(128), 0, s, npe, nrec, ntypes, types, isig, ispec,
-                       gather, h, f, dto2, corrected, seismo, pg, q);
+                       gather, h, f, dto2, corrected, seismo, pg, q, step, nsb, sig_top);
 }
 
 void launch_update_fused(hipStream_t s, int ncomp, Fields f, NewmarkConst nc) {

@@ -35,13 +35,32 @@ struct NewmarkConst {
 void launch_stiffness_2d(hipStream_t s, int first, int count, bool atomic, const int *gather, const float *xix,
                          const float *xiz, const float *gammax, const float *gammaz, const float *jac,
                          const float *kappa, const float *mu, const GllConst &q, Fields f, int rec_mode = 0,
-                         const int *rec_index = nullptr, float *rec = nullptr);
+                         const int *rec_index = nullptr, float *rec = nullptr, const int *step = nullptr,
+                         long long rec_stride = 0);
 // Plain 2-D stiffness with a chosen number of elements per workgroup (2, 5 or
 // 10: 64, 128 or 256 threads); the batch autotuner picks it.
 void launch_stiffness_2d_tuned(hipStream_t s, int elems_per_block, int first, int count, bool atomic,
                                const int *gather, const float *xix, const float *xiz, const float *gammax,
                                const float *gammaz, const float *jac, const float *kappa, const float *mu,
                                const GllConst &q, Fields f);
+// Combined adjoint step: both fields' forces in one pass per color (backward
+// replays the recorded Stacey elements; nothing added to it when bwd_active is
+// 0); with frechet set, Kmu / Kkappa (nspec, 25) accumulate the strain kernels.
+void launch_stiffness_pair_2d(hipStream_t s, int first, int count, bool atomic, const int *gather,
+                              const float *xix, const float *xiz, const float *gammax, const float *gammaz,
+                              const float *jac, const float *kappa, const float *mu, const GllConst &q, Fields adj,
+                              Fields bwd, int bwd_active, const int *rec_index, const float *rec, int frechet,
+                              float dt, int nspec, float *Kmu, float *Kkappa, const int *step = nullptr,
+                              long long rec_stride = 0);
+// *step += delta (the device step counter of graph-captured steps).
+void launch_step_tick(hipStream_t s, int *step, int delta);
+// Division + corrector of both fields, R += dt a+ . u_b, predictors of the
+// next step (+dt adjoint, -dt backward).
+void launch_update_pair(hipStream_t s, int ncomp, Fields adj, Fields bwd, NewmarkConst nc, int bwd_correct, float *R);
+void launch_rho_product(hipStream_t s, int ncomp, Fields adj, Fields bwd, float dt, float *R);
+// out (6, nspec, 25) from R (per point), Kmu, Kkappa.
+void launch_frechet_finish(hipStream_t s, int npts, const int *gather, const float *kappa, const float *mu,
+                           const float *rho, const float *R, const float *Kmu, const float *Kkappa, float *out);
 // K (6, nspec, 25) += Fréchet derivatives of one step from the adjoint and
 // backward fields.
 void launch_frechet_2d(hipStream_t s, int nspec, const int *gather, const float *xix, const float *xiz,
@@ -107,11 +126,14 @@ void launch_seismograms_block(hipStream_t s, Lattice lat, int nrec, int ntypes, 
                               float dto2, float *seismo);
 
 // rec != nullptr: also add each entry's traction to rec[bnd_rec[entry] * dim + d].
+// Kernels given a device step counter `step` read the step from it (and, for
+// records, offset rec by step * rec_stride) instead of the explicit argument,
+// so that one captured graph serves every step.
 void launch_stacey(hipStream_t s, int dim, int nbnd, const int *gather, const float *normal, const float *weight,
                    const float *rho_vp, const float *rho_vs, Fields f, const int *bnd_rec = nullptr,
-                   float *rec = nullptr);
+                   float *rec = nullptr, const int *step = nullptr, long long rec_stride = 0);
 void launch_sources(hipStream_t s, int dim, int npe, int nsrc, int istep, const int *order, const int *ispec,
-                    const int *gather, const float *array, const float *stf, Fields f);
+                    const int *gather, const float *array, const float *stf, Fields f, const int *step = nullptr);
 // Per-point 2-D derivatives and properties read by the pressure seismogram.
 struct PointGeom2D {
   const float *xix, *xiz, *gammax, *gammaz, *kappa, *mu;
@@ -121,7 +143,8 @@ struct PointGeom2D {
 // displacement, velocity, acceleration; 3 pressure (2-D).
 void launch_seismograms(hipStream_t s, int dim, int npe, int nrec, int ntypes, const int *types, int isig,
                         const int *ispec, const int *gather, const float *h, Fields f, float dto2, int corrected,
-                        float *seismo, const PointGeom2D &pg, const GllConst &q);
+                        float *seismo, const PointGeom2D &pg, const GllConst &q, const int *step = nullptr,
+                        int nsb = 1, int sig_top = -1);
 void launch_update_fused(hipStream_t s, int ncomp, Fields f, NewmarkConst nc);
 void launch_predictor(hipStream_t s, int ncomp, Fields f, NewmarkConst nc);
 void launch_divide_corrector(hipStream_t s, int ncomp, Fields f, NewmarkConst nc, int divide, int correct);

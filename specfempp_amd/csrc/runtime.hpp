@@ -49,6 +49,10 @@ template <class T> struct DevBuf {
     n = count;
     if (count) SFX_HIP(hipMalloc(&p, count * sizeof(T)));
   }
+  // allocate only if the size changes (keeps the address, e.g. for captured graphs)
+  void ensure(size_t count) {
+    if (count != n || (count && !p)) alloc(count);
+  }
   void upload(const std::vector<T> &v, hipStream_t s) {
     alloc(v.size());
     if (!v.empty()) SFX_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));

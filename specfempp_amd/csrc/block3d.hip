@@ -70,8 +70,8 @@ __device__ __forceinline__ void owner_force(const Lattice &lat, const float *__r
 }
 
 __global__ __launch_bounds__(256) void block_step_3d_kernel(
-    Lattice lat, const float *__restrict__ kappa_, const float *__restrict__ mu_, float xix, float ety, float gaz,
-    float jac, GllConst q, Fields f, NewmarkConst nc, int nsrc, int istep, const int *__restrict__ src_ispec,
+    Lattice lat, const float *__restrict__ kappa_, const float *__restrict__ mu_, Gll3 q, Fields f, NewmarkConst nc,
+    int nsrc, int istep, const int *__restrict__ src_ispec,
     const float *__restrict__ src_array, const float *__restrict__ stf, const int *__restrict__ rec_block,
     int store_a, int divide_only, float *__restrict__ bs) {
   __shared__ float ub[3][BP];
@@ -103,8 +103,16 @@ __global__ __launch_bounds__(256) void block_step_3d_kernel(
     }
   }
 
-  // block displacement: the contiguous interior, then the surface points
+  // block displacement: the contiguous interior, then the surface points;
+  // v and M^-1 of this thread's first interior point are fetched now so their
+  // latency overlaps the force phases
   const size_t ibase = static_cast<size_t>(blk) * BINT;
+  float pv[3], pm[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    pv[c] = f.v[c * np + ibase + t];
+    pm[c] = f.minv[c * np + ibase + t];
+  }
   for (int qi = t; qi < BINT; qi += 256) {
     const int li = qi % 7 + 1, lj = (qi / 7) % 7 + 1, lk = qi / 49 + 1;
     const int p = (lk * 9 + lj) * 9 + li;
@@ -123,10 +131,10 @@ __global__ __launch_bounds__(256) void block_step_3d_kernel(
   float hpi[5], hpj[5], hwi[5], hwj[5];
 #pragma unroll
   for (int l = 0; l < 5; l++) {
-    hpi[l] = q.hp[i][l];
-    hpj[l] = q.hp[j][l];
-    hwi[l] = q.hpw[i][l];
-    hwj[l] = q.hpw[j][l];
+    hpi[l] = q.hx[i][l];
+    hpj[l] = q.hy[j][l];
+    hwi[l] = q.wx[i][l];
+    hwj[l] = q.wy[j][l];
   }
   const float wi = q.w[i], wj = q.w[j];
   float u[5][3], F2[5][3], r[5][3];
@@ -135,40 +143,40 @@ __global__ __launch_bounds__(256) void block_step_3d_kernel(
     for (int k = 0; k < 5; k++)
 #pragma unroll
       for (int c = 0; c < 3; c++) u[k][c] = ub[c][o0 + (k * 9 + j) * 9 + i];
-    const float sx = jac * xix, sy = jac * ety, sz = jac * gaz;
 #pragma unroll
     for (int k = 0; k < 5; k++) {
-      float dxi[3] = {0, 0, 0}, det[3] = {0, 0, 0}, dga[3] = {0, 0, 0};
+      // strain components from the scaled contractions: d[c] = du_c / dx etc.
+      float dx[3] = {0, 0, 0}, dy[3] = {0, 0, 0}, dz[3] = {0, 0, 0};
 #pragma unroll
       for (int l = 0; l < 5; l++) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          dxi[c] += hpi[l] * ub[c][o0 + (k * 9 + j) * 9 + l];
-          det[c] += hpj[l] * ub[c][o0 + (k * 9 + l) * 9 + i];
-          dga[c] += q.hp[k][l] * u[l][c];
+          dx[c] += hpi[l] * ub[c][o0 + (k * 9 + j) * 9 + l];
+          dy[c] += hpj[l] * ub[c][o0 + (k * 9 + l) * 9 + i];
+          dz[c] += q.hz[k][l] * u[l][c];
         }
       }
       const float kappa = kap[k];
       const float mu = muv[k];
       const float lp2m = kappa + (4.0f / 3.0f) * mu;
       const float lambda = lp2m - 2.0f * mu;
-      const float exx = xix * dxi[0], eyy = ety * det[1], ezz = gaz * dga[2];
+      const float exx = dx[0], eyy = dy[1], ezz = dz[2];
       const float sxx = lp2m * exx + lambda * (eyy + ezz);
       const float syy = lp2m * eyy + lambda * (exx + ezz);
       const float szz = lp2m * ezz + lambda * (exx + eyy);
-      const float sxy = mu * (ety * det[0] + xix * dxi[1]);
-      const float sxz = mu * (gaz * dga[0] + xix * dxi[2]);
-      const float syz = mu * (gaz * dga[1] + ety * det[2]);
+      const float sxy = mu * (dy[0] + dx[1]);
+      const float sxz = mu * (dz[0] + dx[2]);
+      const float syz = mu * (dz[1] + dy[2]);
       const int p = k * 25 + col;
-      sF[el][0][p] = sx * sxx;
-      sF[el][1][p] = sx * sxy;
-      sF[el][2][p] = sx * sxz;
-      sF[el][3][p] = sy * sxy;
-      sF[el][4][p] = sy * syy;
-      sF[el][5][p] = sy * syz;
-      F2[k][0] = sz * sxz;
-      F2[k][1] = sz * syz;
-      F2[k][2] = sz * szz;
+      sF[el][0][p] = sxx;
+      sF[el][1][p] = sxy;
+      sF[el][2][p] = sxz;
+      sF[el][3][p] = sxy;
+      sF[el][4][p] = syy;
+      sF[el][5][p] = syz;
+      F2[k][0] = sxz;
+      F2[k][1] = syz;
+      F2[k][2] = szz;
     }
   }
   __syncthreads();
@@ -182,7 +190,7 @@ __global__ __launch_bounds__(256) void block_step_3d_kernel(
         for (int c = 0; c < 3; c++) {
           r0[c] += sF[el][c][k * 25 + j * 5 + l] * hwi[l];
           r1[c] += sF[el][3 + c][k * 25 + l * 5 + i] * hwj[l];
-          r2[c] += F2[l][c] * q.hpw[k][l];
+          r2[c] += F2[l][c] * q.wz[k][l];
         }
       }
       const float wjk = wj * q.w[k], wik = wi * q.w[k], wij = wi * wj;
@@ -245,13 +253,14 @@ __global__ __launch_bounds__(256) void block_step_3d_kernel(
     const int p = (lk * 9 + lj) * 9 + li;
     float sum[3];
     point_sum(li, lj, lk, sum);
+    const bool pre = qi == t;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const size_t o = static_cast<size_t>(c) * np + ibase + qi;
-      const float a = sum[c] * f.minv[o];
+      const float a = sum[c] * (pre ? pm[c] : f.minv[o]);
       if (store) f.a[o] = a;
       if (divide_only) continue;
-      const float v1 = f.v[o] + nc.dto2 * a;
+      const float v1 = (pre ? pv[c] : f.v[o]) + nc.dto2 * a;
       f.u[o] = ub[c][p] + (nc.dt * v1 + nc.dt2o2 * a);
       f.v[o] = v1 + nc.dto2 * a;
     }
@@ -431,8 +440,9 @@ void launch_block_step_3d(hipStream_t s, Lattice lat, const float *kappa, const 
                           int store_a, int divide_only, float *bs) {
   const int nb = (lat.nx / 2) * (lat.ny / 2) * (lat.nz / 2);
   if (nb <= 0) return;
-  hipLaunchKernelGGL(block_step_3d_kernel, dim3(nb), dim3(256), 0, s, lat, kappa, mu, xix, ety, gaz, jac, q, f, nc,
-                     nsrc, istep, src_ispec, src_array, stf, rec_block, store_a, divide_only, bs);
+  const Gll3 g = scaled_gll3(q, xix, ety, gaz, jac);
+  hipLaunchKernelGGL(block_step_3d_kernel, dim3(nb), dim3(256), 0, s, lat, kappa, mu, g, f, nc, nsrc, istep,
+                     src_ispec, src_array, stf, rec_block, store_a, divide_only, bs);
 }
 
 void launch_surface_update_3d(hipStream_t s, Lattice lat, const float *bs, Fields f, NewmarkConst nc, int store_a,

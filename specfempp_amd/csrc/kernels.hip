@@ -163,7 +163,7 @@ __global__ __launch_bounds__(128) void stiffness_pair_2d_kernel(
     int first, int count, const int *__restrict__ gather, const float *__restrict__ xix_, const float *__restrict__ xiz_,
     const float *__restrict__ gammax_, const float *__restrict__ gammaz_, const float *__restrict__ jac_,
     const float *__restrict__ kappa_, const float *__restrict__ mu_, GllConst q, Fields adj, Fields bwd, int bwd_active,
-    const int *__restrict__ rec_index, const float *__restrict__ rec, int frechet, float dt, int nspec,
+    const int *__restrict__ rec_index, const float *__restrict__ rec, int frechet, float dt,
     float *__restrict__ Kmu, float *__restrict__ Kkappa, const int *__restrict__ step, long long rec_stride) {
   __shared__ float su[2][E2][2][25];  // [field][element][comp][point]
   __shared__ float sF[2][E2][4][25];
@@ -591,7 +591,10 @@ __global__ __launch_bounds__(256) void stacey_kernel(int nbnd, const int *__rest
 }
 
 // ---------------------------------------------------------------------------
-// Sources: a += stf(istep, s) * source_array(s, c, p)
+// Sources: a += stf(istep, s) * source_array(s, c, p); one workgroup per
+// source (in application order), one thread per element point.  Sources
+// sharing a global point add with float atomics, as the reference's
+// atomic_add_on_device does.
 // ---------------------------------------------------------------------------
 template <int D>
 __global__ __launch_bounds__(128) void sources_kernel(int npe, int nsrc, int istep_arg, const int *__restrict__ order,
@@ -600,16 +603,15 @@ __global__ __launch_bounds__(128) void sources_kernel(int npe, int nsrc, int ist
                                                       const float *__restrict__ stf, Fields f,
                                                       const int *__restrict__ step) {
   const int p = threadIdx.x;
-  if (p >= npe) return;
+  const int k = blockIdx.x;
+  if (p >= npe || k >= nsrc) return;
   const int istep = step ? *step : istep_arg;
-  for (int k = 0; k < nsrc; k++) {
-    const int s = order[k];
-    const float sv = stf[static_cast<size_t>(istep) * nsrc + s];
-    const int g = gather[static_cast<size_t>(ispec[s]) * npe + p];
+  const int s = order[k];
+  const float sv = stf[static_cast<size_t>(istep) * nsrc + s];
+  const int g = gather[static_cast<size_t>(ispec[s]) * npe + p];
 #pragma unroll
-    for (int c = 0; c < D; c++)
-      atomic_add(&f.a[c * f.nglob_pad + g], sv * array[(static_cast<size_t>(s) * D + c) * npe + p]);
-  }
+  for (int c = 0; c < D; c++)
+    atomic_add(&f.a[c * f.nglob_pad + g], sv * array[(static_cast<size_t>(s) * D + c) * npe + p]);
 }
 
 // ---------------------------------------------------------------------------
@@ -784,8 +786,7 @@ __global__ __launch_bounds__(256) void divide_corrector_kernel(int n, int ncomp,
 // elements and its L2 sees the shared faces of neighbouring elements.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const float *__restrict__ kappa_,
-                                                           const float *__restrict__ mu_, float xix, float ety,
-                                                           float gaz, float jac, GllConst q,
+                                                           const float *__restrict__ mu_, Gll3 q,
                                                            const float *__restrict__ uf, int np,
                                                            float *__restrict__ re) {
   __shared__ float su[E3][3][125];
@@ -805,10 +806,10 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
   float hpi[5], hpj[5], hwi[5], hwj[5];
 #pragma unroll
   for (int l = 0; l < 5; l++) {
-    hpi[l] = q.hp[i][l];
-    hpj[l] = q.hp[j][l];
-    hwi[l] = q.hpw[i][l];
-    hwj[l] = q.hpw[j][l];
+    hpi[l] = q.hx[i][l];
+    hpj[l] = q.hy[j][l];
+    hwi[l] = q.wx[i][l];
+    hwj[l] = q.wy[j][l];
   }
   const float wi = q.w[i], wj = q.w[j];
 
@@ -829,40 +830,40 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
 
   float F2[5][3];
   if (active) {
-    const float sx = jac * xix, sy = jac * ety, sz = jac * gaz;
 #pragma unroll
     for (int k = 0; k < 5; k++) {
-      float dxi[3] = {0, 0, 0}, det[3] = {0, 0, 0}, dga[3] = {0, 0, 0};
+      // strain components from the scaled contractions: d[c] = du_c / dx etc.
+      float dx[3] = {0, 0, 0}, dy[3] = {0, 0, 0}, dz[3] = {0, 0, 0};
 #pragma unroll
       for (int l = 0; l < 5; l++) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          dxi[c] += hpi[l] * su[el][c][k * 25 + j * 5 + l];
-          det[c] += hpj[l] * su[el][c][k * 25 + l * 5 + i];
-          dga[c] += q.hp[k][l] * u[l][c];
+          dx[c] += hpi[l] * su[el][c][k * 25 + j * 5 + l];
+          dy[c] += hpj[l] * su[el][c][k * 25 + l * 5 + i];
+          dz[c] += q.hz[k][l] * u[l][c];
         }
       }
       const float kappa = kappa_[base + k * 25];
       const float mu = mu_[base + k * 25];
       const float lp2m = kappa + (4.0f / 3.0f) * mu;
       const float lambda = lp2m - 2.0f * mu;
-      const float exx = xix * dxi[0], eyy = ety * det[1], ezz = gaz * dga[2];
+      const float exx = dx[0], eyy = dy[1], ezz = dz[2];
       const float sxx = lp2m * exx + lambda * (eyy + ezz);
       const float syy = lp2m * eyy + lambda * (exx + ezz);
       const float szz = lp2m * ezz + lambda * (exx + eyy);
-      const float sxy = mu * (ety * det[0] + xix * dxi[1]);
-      const float sxz = mu * (gaz * dga[0] + xix * dxi[2]);
-      const float syz = mu * (gaz * dga[1] + ety * det[2]);
+      const float sxy = mu * (dy[0] + dx[1]);
+      const float sxz = mu * (dz[0] + dx[2]);
+      const float syz = mu * (dz[1] + dy[2]);
       const int p = k * 25 + col;
-      sF0[el][0][p] = sx * sxx;
-      sF0[el][1][p] = sx * sxy;
-      sF0[el][2][p] = sx * sxz;
-      sF1[el][0][p] = sy * sxy;
-      sF1[el][1][p] = sy * syy;
-      sF1[el][2][p] = sy * syz;
-      F2[k][0] = sz * sxz;
-      F2[k][1] = sz * syz;
-      F2[k][2] = sz * szz;
+      sF0[el][0][p] = sxx;
+      sF0[el][1][p] = sxy;
+      sF0[el][2][p] = sxz;
+      sF1[el][0][p] = sxy;
+      sF1[el][1][p] = syy;
+      sF1[el][2][p] = syz;
+      F2[k][0] = sxz;
+      F2[k][1] = syz;
+      F2[k][2] = szz;
     }
   }
   __syncthreads();
@@ -877,7 +878,7 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
       for (int c = 0; c < 3; c++) {
         r0[c] += sF0[el][c][k * 25 + j * 5 + l] * hwi[l];
         r1[c] += sF1[el][c][k * 25 + l * 5 + i] * hwj[l];
-        r2[c] += F2[l][c] * q.hpw[k][l];
+        r2[c] += F2[l][c] * q.wz[k][l];
       }
     }
     const float wjk = wj * q.w[k], wik = wi * q.w[k], wij = wi * wj;
@@ -1185,11 +1186,11 @@ void launch_stiffness_pair_2d(hipStream_t s, int first, int count, bool atomic, 
   if (atomic)
     hipLaunchKernelGGL(stiffness_pair_2d_kernel<true>, dim3(blocks(count, E2)), dim3(128), 0, s, first, count, gather,
                        xix, xiz, gammax, gammaz, jac, kappa, mu, q, adj, bwd, bwd_active, rec_index, rec, frechet, dt,
-                       nspec, Kmu, Kkappa, step, rec_stride);
+                       Kmu, Kkappa, step, rec_stride);
   else
     hipLaunchKernelGGL(stiffness_pair_2d_kernel<false>, dim3(blocks(count, E2)), dim3(128), 0, s, first, count, gather,
                        xix, xiz, gammax, gammaz, jac, kappa, mu, q, adj, bwd, bwd_active, rec_index, rec, frechet, dt,
-                       nspec, Kmu, Kkappa, step, rec_stride);
+                       Kmu, Kkappa, step, rec_stride);
 }
 
 void launch_step_tick(hipStream_t s, int *step, int delta) {
@@ -1249,11 +1250,11 @@ void launch_sources(hipStream_t s, int dim, int npe, int nsrc, int istep, const 
                     const int *gather, const float *array, const float *stf, Fields f, const int *step) {
   if (nsrc <= 0) return;
   if (dim == 2)
-    hipLaunchKernelGGL(sources_kernel<2>, dim3(1), dim3(128), 0, s, npe, nsrc, istep, order, ispec, gather, array, stf,
-                       f, step);
+    hipLaunchKernelGGL(sources_kernel<2>, dim3(nsrc), dim3(64), 0, s, npe, nsrc, istep, order, ispec, gather, array,
+                       stf, f, step);
   else
-    hipLaunchKernelGGL(sources_kernel<3>, dim3(1), dim3(128), 0, s, npe, nsrc, istep, order, ispec, gather, array, stf,
-                       f, step);
+    hipLaunchKernelGGL(sources_kernel<3>, dim3(nsrc), dim3(128), 0, s, npe, nsrc, istep, order, ispec, gather, array,
+                       stf, f, step);
 }
 
 void launch_seismograms(hipStream_t s, int dim, int npe, int nrec, int ntypes, const int *types, int isig,
@@ -1288,12 +1289,29 @@ void launch_fill(hipStream_t s, float *p, size_t n, float value) {
   hipLaunchKernelGGL(fill_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, p, n, value);
 }
 
+Gll3 scaled_gll3(const GllConst &q, float xix, float ety, float gaz, float jac) {
+  Gll3 g{};
+  for (int a = 0; a < 5; a++) {
+    g.w[a] = q.w[a];
+    for (int l = 0; l < 5; l++) {
+      g.hx[a][l] = xix * q.hp[a][l];
+      g.hy[a][l] = ety * q.hp[a][l];
+      g.hz[a][l] = gaz * q.hp[a][l];
+      g.wx[a][l] = (jac * xix) * q.hpw[a][l];
+      g.wy[a][l] = (jac * ety) * q.hpw[a][l];
+      g.wz[a][l] = (jac * gaz) * q.hpw[a][l];
+    }
+  }
+  return g;
+}
+
 void launch_elemforce_3d(hipStream_t s, Lattice lat, const float *kappa, const float *mu, float xix, float ety,
                          float gaz, float jac, const GllConst &q, const float *u, int nglob_pad, float *re) {
   const int nspec = lat.nx * lat.ny * lat.nz;
   if (nspec <= 0) return;
-  hipLaunchKernelGGL(elemforce_3d_kernel, dim3(blocks(nspec, E3)), dim3(128), 0, s, lat, kappa, mu, xix, ety, gaz, jac,
-                     q, u, nglob_pad, re);
+  const Gll3 g = scaled_gll3(q, xix, ety, gaz, jac);
+  hipLaunchKernelGGL(elemforce_3d_kernel, dim3(blocks(nspec, E3)), dim3(128), 0, s, lat, kappa, mu, g, u, nglob_pad,
+                     re);
 }
 
 void launch_sources_re(hipStream_t s, int nsrc, int istep, const int *ispec, const float *array, const float *stf,

@@ -16,6 +16,18 @@ struct GllConst {
   float w[5];
 };
 
+// Derivative matrices of a regular hexahedron with its constant geometry
+// folded in (built on the host by scaled_gll3): hx = xix hprime, hy = ety
+// hprime, hz = gaz hprime for the gradient, wx = J xix hprime_wgll, wy, wz for
+// the divergence.  The force kernels then form the strain from the
+// contractions directly and store the stress unscaled.
+struct Gll3 {
+  float hx[5][5], hy[5][5], hz[5][5];
+  float wx[5][5], wy[5][5], wz[5][5];
+  float w[5];
+};
+Gll3 scaled_gll3(const GllConst &q, float xix, float ety, float gaz, float jac);
+
 struct Fields {
   float *u, *v, *a;
   const float *minv;

@@ -16,6 +16,8 @@ import subprocess
 
 import numpy as np
 
+from specfempp_amd.meshes import material_arrays
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
 _lib = None
@@ -100,9 +102,7 @@ class Oracle2D:
         coorg = np.ascontiguousarray(np.asarray(mesh.coorg, dtype=np.float64).T)
         knods = np.ascontiguousarray(np.asarray(mesh.knods, dtype=np.int32).T)
         kmato = np.ascontiguousarray(mesh.kmato, dtype=np.int32)
-        rho = np.array([m.density for m in mesh.materials], dtype=np.float64)
-        cp = np.array([m.cp for m in mesh.materials], dtype=np.float64)
-        cs = np.array([m.cs for m in mesh.materials], dtype=np.float64)
+        rho, cp, cs = material_arrays(mesh.materials)
         abs_ispec = np.ascontiguousarray(mesh.abs_ispec, dtype=np.int32)
         abs_type = np.ascontiguousarray(mesh.abs_type, dtype=np.int32)
         h = C.c_void_p()

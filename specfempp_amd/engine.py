@@ -32,6 +32,7 @@ import os
 import numpy as np
 
 from . import _native
+from .meshes import material_arrays
 
 DISPLACEMENT, VELOCITY, ACCELERATION = 0, 1, 2
 _TYPE_CODES = {"displacement": 0, "velocity": 1, "acceleration": 2, "pressure": 3}
@@ -273,9 +274,7 @@ class Engine2D(_EngineBase):
         coorg = np.ascontiguousarray(np.asarray(mesh.coorg, dtype=np.float64).T)
         knods = np.ascontiguousarray(np.asarray(mesh.knods, dtype=np.int32).T)
         kmato = np.ascontiguousarray(mesh.kmato, dtype=np.int32)
-        rho = np.array([m.density for m in mesh.materials], dtype=np.float64)
-        vp = np.array([m.cp for m in mesh.materials], dtype=np.float64)
-        vs = np.array([m.cs for m in mesh.materials], dtype=np.float64)
+        rho, vp, vs = material_arrays(mesh.materials)
         abs_ispec = np.ascontiguousarray(mesh.abs_ispec, dtype=np.int32)
         abs_edge = np.ascontiguousarray(mesh.abs_type, dtype=np.int32)
         self._keep = (coorg, knods, kmato, rho, vp, vs, abs_ispec, abs_edge)

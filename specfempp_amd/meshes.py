@@ -43,18 +43,43 @@ def quad_mesh(nx: int, nz: int, xmin: float, xmax: float, zmin: float, zmax: flo
     if material is None:
         material = np.zeros((nz, nx), dtype=np.int32)
     kmato = np.ascontiguousarray(material, dtype=np.int32).ravel()
-    abs_ispec, abs_type = [], []
+    # absorbing edges element-major, within an element in the order bottom,
+    # right, top, left
     flags = set(absorbing)
-    for e in range(nx * nz):
-        ez, ex = divmod(e, nx)
-        for name, code, on in (("bottom", EDGE_BOTTOM, ez == 0), ("right", EDGE_RIGHT, ex == nx - 1),
-                               ("top", EDGE_TOP, ez == nz - 1), ("left", EDGE_LEFT, ex == 0)):
-            if on and name in flags:
-                abs_ispec.append(e)
-                abs_type.append(code)
+    ez, ex = np.divmod(np.arange(nx * nz, dtype=np.int64), nx)
+    on = np.stack([(ez == 0) & ("bottom" in flags), (ex == nx - 1) & ("right" in flags),
+                   (ez == nz - 1) & ("top" in flags), (ex == 0) & ("left" in flags)], axis=1)
+    codes = np.array([EDGE_BOTTOM, EDGE_RIGHT, EDGE_TOP, EDGE_LEFT], dtype=np.int32)
+    elem, which = np.nonzero(on)
+    if not isinstance(materials, MaterialArrays):
+        materials = list(materials)
     return Mesh2D(nspec=nx * nz, npgeo=(nx + 1) * (nz + 1), ngnod=4, coorg=coorg, knods=knods, kmato=kmato,
-                  materials=list(materials), abs_ispec=np.array(abs_ispec, dtype=np.int32),
-                  abs_type=np.array(abs_type, dtype=np.int32))
+                  materials=materials, abs_ispec=elem.astype(np.int32), abs_type=codes[which])
+
+
+class MaterialArrays:
+    """Isotropic elastic materials held as arrays (density, cp, cs), for
+    meshes with a material per element; indexing yields a ``Material``."""
+
+    def __init__(self, density, cp, cs):
+        self.density = np.ascontiguousarray(density, dtype=np.float64).ravel()
+        self.cp = np.ascontiguousarray(cp, dtype=np.float64).ravel()
+        self.cs = np.ascontiguousarray(cs, dtype=np.float64).ravel()
+
+    def __len__(self):
+        return len(self.density)
+
+    def __getitem__(self, i):
+        return elastic(float(self.density[i]), float(self.cp[i]), float(self.cs[i]))
+
+
+def material_arrays(materials):
+    """(density, cp, cs) float64 arrays of a material list or ``MaterialArrays``."""
+    if isinstance(materials, MaterialArrays):
+        return materials.density, materials.cp, materials.cs
+    return (np.array([m.density for m in materials], dtype=np.float64),
+            np.array([m.cp for m in materials], dtype=np.float64),
+            np.array([m.cs for m in materials], dtype=np.float64))
 
 
 def elastic(rho: float, vp: float, vs: float) -> Material:
@@ -84,6 +109,60 @@ def config1(nstep: int = 1600, n: int = 80, h: float = 50.0) -> Workload2D:
     rec = [dict(x=40.0 + 80.0 * i * (L / 4000.0), z=L - 500.0 * (L / 4000.0)) for i in range(50)]
     return Workload2D("2d_homogeneous_80x80", mesh, src, rec, 1e-3, nstep, 1e-3,
                       {"courant": courant(1e-3, h, 2500.0)})
+
+
+def gaussian_random_field(L: float, corr: float, seed, grid: float = None):
+    """Zero-mean, unit-variance Gaussian random field with Gaussian covariance
+    of correlation length corr on [0, L]^2: separable Gaussian filtering of
+    N(0, 1) white noise on a lattice of spacing grid (default corr / 8),
+    normalised, clipped at +-3 sigma.  Returns (axis, values[iz, ix])."""
+    rng = np.random.default_rng(seed)
+    grid = corr / 8.0 if grid is None else grid
+    n = int(round(L / grid)) + 1
+    x = np.arange(n) * grid
+    white = rng.standard_normal((n, n))
+    # a Gaussian covariance exp(-r^2 / corr^2) is white noise filtered by a
+    # Gaussian kernel of standard deviation corr / 2
+    sig = corr / 2.0 / grid
+    half = int(np.ceil(4 * sig))
+    t = np.arange(-half, half + 1)
+    ker = np.exp(-0.5 * (t / sig) ** 2)
+    ker /= ker.sum()
+    f = np.apply_along_axis(lambda r: np.convolve(r, ker, mode="same"), 1, white)
+    f = np.apply_along_axis(lambda c: np.convolve(c, ker, mode="same"), 0, f)
+    f = (f - f.mean()) / f.std()
+    return x, np.clip(f, -3.0, 3.0)
+
+
+def config2(n: int = 4000, L: float = 10000.0, nstep: int = 4000, nrec: int = 200, seed: int = 2,
+            corr: float = 200.0) -> Workload2D:
+    """2-D heterogeneous P-SV: n x n quads over L x L (the full configuration
+    is 4000^2 elements of 2.5 m), vp = 3000 (1 + 0.1 g) with g the Gaussian
+    random field above (sampled at element centres: one material per element),
+    vs = vp / sqrt(3), rho = 310 vp^0.25, Stacey on all sides, a vertical
+    Ricker force (15 Hz) at a seeded point of the inner 80 %, nrec receivers on
+    a seeded line; dt at Courant 0.4 (the stated 3e-4 s is unstable)."""
+    h = L / n
+    axis, g = gaussian_random_field(L, corr, seed)
+    c = (np.arange(n) + 0.5) * h
+    ix = np.clip(np.searchsorted(axis, c) - 1, 0, len(axis) - 2)
+    tx = (c - axis[ix]) / (axis[ix + 1] - axis[ix])
+    gx = g[:, ix] * (1 - tx) + g[:, ix + 1] * tx            # interpolate along x at every grid row
+    gz = gx[ix, :] * (1 - tx)[:, None] + gx[ix + 1, :] * tx[:, None]   # then along z
+    vp = 3000.0 * (1.0 + 0.1 * gz)                            # (nz, nx) per element
+    vs = vp / np.sqrt(3.0)
+    rho = 310.0 * vp ** 0.25
+    mats = MaterialArrays(rho, vp, vs)
+    material = np.arange(n * n, dtype=np.int32).reshape(n, n)
+    mesh = quad_mesh(n, n, 0.0, L, 0.0, L, material, mats, ("bottom", "right", "left", "top"))
+    rng = np.random.default_rng(seed)
+    sx, sz = rng.uniform(0.1 * L, 0.9 * L, size=2)
+    z_line = rng.uniform(0.2 * L, 0.8 * L)
+    rec = [dict(x=float(x), z=float(z_line)) for x in np.sort(rng.uniform(0.05 * L, 0.95 * L, size=nrec))]
+    src = [dict(x=float(sx), z=float(sz), angle=0.0, f0=15.0, factor=1e10, tshift=0.0)]
+    dt = stable_dt(h, float(vp.max()))
+    return Workload2D(f"2d_grf_{n}x{n}", mesh, src, rec, dt, nstep, 3e-4,
+                      {"h": h, "vp_range": (float(vp.min()), float(vp.max())), "courant_stated": courant(3e-4, h, float(vp.max()))})
 
 
 def config4(nsim: int = 64, seed: int = 4, nmin: int = 40, nmax: int = 240, h: float = 50.0,

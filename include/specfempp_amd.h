@@ -140,6 +140,16 @@ int sfx_engine_create_2d(const sfx_mesh2d *mesh, int nsources, const sfx_source 
                          const sfx_receiver *receivers, const sfx_run_desc *run, sfx_engine **out);
 int sfx_engine_create_3d(const sfx_mesh3d *mesh, int nsources, const sfx_source *sources, int nreceivers,
                          const sfx_receiver *receivers, const sfx_run_desc *run, sfx_engine **out);
+/* Host-only engines: the host half of the construction above (the
+ * specfem::compute::assembly host views: numbering compute_mesh.cpp:45-170,
+ * ordering :240-279, locate_point.cpp, sources and receivers) and no device
+ * work at all.  The data-contract getters below (sizes, t0, index mapping,
+ * field index, compute order, locations) and sfx_batch_create accept them;
+ * every other call returns an error. */
+int sfx_engine_create_host_2d(const sfx_mesh2d *mesh, int nsources, const sfx_source *sources, int nreceivers,
+                              const sfx_receiver *receivers, const sfx_run_desc *run, sfx_engine **out);
+int sfx_engine_create_host_3d(const sfx_mesh3d *mesh, int nsources, const sfx_source *sources, int nreceivers,
+                              const sfx_receiver *receivers, const sfx_run_desc *run, sfx_engine **out);
 void sfx_engine_destroy(sfx_engine *engine);
 
 /* Sizes: dim, nspec, nglob, ngll, nsig (seismogram samples), nrec, nsrc. */

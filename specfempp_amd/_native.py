@@ -61,6 +61,8 @@ def load():
         "sfx_last_error": ([], C.c_char_p),
         "sfx_engine_create_2d": ([vp, i32, vp, i32, vp, vp, vp], i32),
         "sfx_engine_create_3d": ([vp, i32, vp, i32, vp, vp, vp], i32),
+        "sfx_engine_create_host_2d": ([vp, i32, vp, i32, vp, vp, vp], i32),
+        "sfx_engine_create_host_3d": ([vp, i32, vp, i32, vp, vp, vp], i32),
         "sfx_engine_destroy": ([vp], None),
         "sfx_get_sizes": ([vp, vp], i32),
         "sfx_get_t0": ([vp, vp], i32),

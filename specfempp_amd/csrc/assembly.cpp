@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <limits>
 #include <map>
 #include <stdexcept>
@@ -14,10 +15,49 @@ namespace sfx {
 
 namespace {
 
-struct SortPoint {
-  float x, z;
-  int iloc, iglob;
-};
+// Unsigned key that orders like the float (+0 and -0, which compare equal,
+// share one key).
+inline uint32_t float_key(float f) {
+  if (f == 0.0f) f = 0.0f;
+  uint32_t u;
+  std::memcpy(&u, &f, sizeof u);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// Indices [0, n) ordered by (x, z) as float comparisons order them: LSD radix
+// sort on the 64-bit key (key(x), key(z)), 11-bit digits, passes over a
+// constant digit skipped.  Points with equal keys have equal coordinates, so
+// their relative order cannot change the merge or the numbering below.
+std::vector<uint32_t> sort_by_xz(const std::vector<float> &x, const std::vector<float> &z) {
+  const size_t n = x.size();
+  std::vector<uint64_t> key(n), key2(n);
+  std::vector<uint32_t> idx(n), idx2(n);
+  for (size_t i = 0; i < n; i++) {
+    key[i] = (static_cast<uint64_t>(float_key(x[i])) << 32) | float_key(z[i]);
+    idx[i] = static_cast<uint32_t>(i);
+  }
+  constexpr int BITS = 11, RADIX = 1 << BITS;
+  std::vector<size_t> count(RADIX);
+  for (int shift = 0; shift < 64; shift += BITS) {
+    std::fill(count.begin(), count.end(), 0);
+    for (size_t i = 0; i < n; i++) count[(key[i] >> shift) & (RADIX - 1)]++;
+    if (count[(key[0] >> shift) & (RADIX - 1)] == n) continue;
+    size_t sum = 0;
+    for (auto &c : count) {
+      const size_t t = c;
+      c = sum;
+      sum += t;
+    }
+    for (size_t i = 0; i < n; i++) {
+      const size_t d = count[(key[i] >> shift) & (RADIX - 1)]++;
+      key2[d] = key[i];
+      idx2[d] = idx[i];
+    }
+    key.swap(key2);
+    idx.swap(idx2);
+  }
+  return idx;
+}
 
 // Mesh-order element types -> compute order
 // (src/compute/compute_mesh.cpp:240-279 with the ELEMENT_TYPES order of
@@ -42,17 +82,18 @@ void compute_ordering(const sfx_mesh2d &m, HostAssembly &a, std::vector<int> &is
 // Tolerance of the point merge, src/compute/compute_mesh.cpp:19-43: taken on
 // consecutive groups of ngllxz *sorted* points, starting the maxima at the
 // smallest positive float, exactly as the reference does.
-float merge_tolerance(const std::vector<SortPoint> &pts, int nspec, int ngllxz) {
+float merge_tolerance(const std::vector<uint32_t> &order, const std::vector<float> &x,
+                      const std::vector<float> &z, int nspec, int ngllxz) {
   float typ = std::numeric_limits<float>::max();
   for (int e = 0; e < nspec; e++) {
     float xmax = std::numeric_limits<float>::min(), xmin = std::numeric_limits<float>::max();
     float zmax = std::numeric_limits<float>::min(), zmin = std::numeric_limits<float>::max();
-    const SortPoint *g = &pts[static_cast<size_t>(e) * ngllxz];
+    const uint32_t *g = &order[static_cast<size_t>(e) * ngllxz];
     for (int k = 0; k < ngllxz; k++) {
-      xmax = std::max(xmax, g[k].x);
-      xmin = std::min(xmin, g[k].x);
-      zmax = std::max(zmax, g[k].z);
-      zmin = std::min(zmin, g[k].z);
+      xmax = std::max(xmax, x[g[k]]);
+      xmin = std::min(xmin, x[g[k]]);
+      zmax = std::max(zmax, z[g[k]]);
+      zmin = std::min(zmin, z[g[k]]);
     }
     typ = std::min(typ, xmax - xmin);
     typ = std::min(typ, zmax - zmin);
@@ -66,37 +107,36 @@ float merge_tolerance(const std::vector<SortPoint> &pts, int nspec, int ngllxz) 
 void number_points_2d(HostAssembly &a, const std::vector<float> &px, const std::vector<float> &pz, int chunk) {
   const int nspec = a.nspec, npe = 25;
   const size_t nloc = static_cast<size_t>(nspec) * npe;
-  std::vector<SortPoint> pts(nloc);
-  std::vector<size_t> visit(nloc);  // iloc -> (e*25 + p)
+  // visit order iloc -> local point p = e * 25 + iz * 5 + ix
+  std::vector<uint32_t> visit(nloc);
+  std::vector<float> vx(nloc), vz(nloc);
   size_t iloc = 0;
   for (int c0 = 0; c0 < nspec; c0 += chunk)
     for (int iz = 0; iz < NGLL; iz++)
       for (int ix = 0; ix < NGLL; ix++)
         for (int ie = 0; ie < chunk && c0 + ie < nspec; ie++) {
           const size_t p = static_cast<size_t>(c0 + ie) * npe + iz * NGLL + ix;
-          pts[iloc] = SortPoint{px[p], pz[p], static_cast<int>(iloc), 0};
-          visit[iloc] = p;
+          visit[iloc] = static_cast<uint32_t>(p);
+          vx[iloc] = px[p];
+          vz[iloc] = pz[p];
           iloc++;
         }
-  std::vector<SortPoint> sorted = pts;
-  std::sort(sorted.begin(), sorted.end(), [](const SortPoint &p, const SortPoint &q) {
-    if (p.x != q.x) return p.x < q.x;
-    return p.z < q.z;
-  });
-  const float tol = merge_tolerance(sorted, nspec, npe);
+  const std::vector<uint32_t> order = sort_by_xz(vx, vz);
+  const float tol = merge_tolerance(order, vx, vz, nspec, npe);
+  std::vector<int32_t> iglob(nloc);
   int ig = 0;
-  sorted[0].iglob = 0;
+  iglob[order[0]] = 0;
   for (size_t i = 1; i < nloc; i++) {
-    if (std::abs(sorted[i].x - sorted[i - 1].x) > tol || std::abs(sorted[i].z - sorted[i - 1].z) > tol) ig++;
-    sorted[i].iglob = ig;
+    const uint32_t c = order[i], p = order[i - 1];
+    if (std::abs(vx[c] - vx[p]) > tol || std::abs(vz[c] - vz[p]) > tol) ig++;
+    iglob[c] = ig;
   }
-  for (const auto &s : sorted) pts[s.iloc].iglob = s.iglob;
   const int nglob = ig + 1;
   std::vector<int> first(nglob, -1);
   a.index_mapping.assign(nloc, -1);
   int inum = 0;
   for (size_t i = 0; i < nloc; i++) {
-    int &f = first[pts[i].iglob];
+    int &f = first[iglob[i]];
     if (f < 0) f = inum++;
     a.index_mapping[visit[i]] = f;
   }
@@ -120,6 +160,83 @@ struct Locator2D {
   const std::vector<float> &ctrl;  // (nspec, ngnod, 2)
   const std::vector<float> &px, &pz;
   int ngnod;
+  std::vector<float> box;              // (nspec, 4): x min, x max, z min, z max of the GLL points
+  std::vector<uint32_t> corner_start;  // (nglob + 1) CSR over the elements having a global point as corner
+  std::vector<int32_t> corner_elem;    // ascending element order within each point
+  // uniform G x G grid over the points' bounding box; cell c lists (CSR,
+  // ascending) every element whose bounding box overlaps it
+  int G = 1;
+  double X0 = 0, Z0 = 0, sx = 1, sz = 1;
+  std::vector<uint32_t> cell_start;
+  std::vector<int32_t> cell_elem;
+
+  int cell_x(double x) const { return static_cast<int>(std::min(std::max(std::floor((x - X0) * sx), 0.0), G - 1.0)); }
+  int cell_z(double z) const { return static_cast<int>(std::min(std::max(std::floor((z - Z0) * sz), 0.0), G - 1.0)); }
+
+  Locator2D(const HostAssembly &a_, const std::vector<float> &ctrl_, const std::vector<float> &px_,
+            const std::vector<float> &pz_, int ngnod_)
+      : a(a_), ctrl(ctrl_), px(px_), pz(pz_), ngnod(ngnod_) {
+    const int nspec = a.nspec;
+    box.resize(4 * static_cast<size_t>(nspec));
+    for (int e = 0; e < nspec; e++) {
+      float x0 = px[static_cast<size_t>(e) * 25], x1 = x0, z0 = pz[static_cast<size_t>(e) * 25], z1 = z0;
+      for (int p = 1; p < 25; p++) {
+        const float x = px[static_cast<size_t>(e) * 25 + p], z = pz[static_cast<size_t>(e) * 25 + p];
+        x0 = std::min(x0, x);
+        x1 = std::max(x1, x);
+        z0 = std::min(z0, z);
+        z1 = std::max(z1, z);
+      }
+      float *b = &box[4 * static_cast<size_t>(e)];
+      b[0] = x0;
+      b[1] = x1;
+      b[2] = z0;
+      b[3] = z1;
+    }
+    double X1 = box[0], Z1 = box[2];
+    X0 = box[0];
+    Z0 = box[2];
+    for (int e = 0; e < nspec; e++) {
+      const float *b = &box[4 * static_cast<size_t>(e)];
+      X0 = std::min<double>(X0, b[0]);
+      X1 = std::max<double>(X1, b[1]);
+      Z0 = std::min<double>(Z0, b[2]);
+      Z1 = std::max<double>(Z1, b[3]);
+    }
+    G = std::max(1, static_cast<int>(std::sqrt(static_cast<double>(nspec))));
+    sx = X1 > X0 ? G / (X1 - X0) : 0.0;
+    sz = Z1 > Z0 ? G / (Z1 - Z0) : 0.0;
+    cell_start.assign(static_cast<size_t>(G) * G + 1, 0);
+    for (int pass = 0; pass < 2; pass++) {
+      std::vector<uint32_t> fill;
+      if (pass == 1) {
+        for (size_t c = 0; c + 1 < cell_start.size(); c++) cell_start[c + 1] += cell_start[c];
+        cell_elem.resize(cell_start.back());
+        fill.assign(cell_start.begin(), cell_start.end() - 1);
+      }
+      for (int e = 0; e < nspec; e++) {
+        const float *b = &box[4 * static_cast<size_t>(e)];
+        for (int cz = cell_z(b[2]); cz <= cell_z(b[3]); cz++)
+          for (int cx = cell_x(b[0]); cx <= cell_x(b[1]); cx++) {
+            const size_t c = static_cast<size_t>(cz) * G + cx;
+            if (pass == 0)
+              cell_start[c + 1]++;
+            else
+              cell_elem[fill[c]++] = e;
+          }
+      }
+    }
+    corner_start.assign(static_cast<size_t>(a.nglob) + 1, 0);
+    for (int e = 0; e < nspec; e++)
+      for (int c : CORNERS) corner_start[a.index_mapping[static_cast<size_t>(e) * 25 + c] + 1]++;
+    for (int g = 0; g < a.nglob; g++) corner_start[g + 1] += corner_start[g];
+    corner_elem.resize(4 * static_cast<size_t>(nspec));
+    std::vector<uint32_t> fill(corner_start.begin(), corner_start.end() - 1);
+    for (int e = 0; e < nspec; e++)
+      for (int c : CORNERS) corner_elem[fill[a.index_mapping[static_cast<size_t>(e) * 25 + c]]++] = e;
+  }
+
+  static constexpr int CORNERS[4] = {0, 4, 20, 24};
 
   void position(int e, float xi, float gamma, float &x, float &z) const {
     float s[9];
@@ -154,11 +271,52 @@ struct Locator2D {
   // first in (element, iz, ix) order), candidate elements sharing a corner
   // point of the guess, 100 clamped Newton steps per candidate, smallest
   // distance wins.
+  //
+  // The nearest-point scan visits only elements that can hold a point within
+  // reach of the target.  U, the squared distance (in double) to the centre
+  // point of some element of the grid cells around the target, bounds the
+  // minimum; reach = sqrt(U) (1 + 1e-5).  Every element whose bounding box
+  // comes within reach overlaps a grid cell of the square of half-width
+  // reach (1 + 1e-5) / (1 - 1e-5) about the target (cell indices are
+  // monotonic in the coordinate), so it is in the candidate list, visited in
+  // ascending element order.  A skipped element lies farther than reach, so
+  // every float distance in it exceeds the float minimum (float distances
+  // are within a few ulp of the exact ones), and the first minimum in scan
+  // order is the one a scan over all elements finds.  The corner candidates
+  // come from the corner index in ascending element order, as a scan over all
+  // elements collects them.
   void locate(float gx, float gz, int &ispec, float &xi_out, float &gamma_out) const {
-    const int nspec = a.nspec;
+    if (!std::isfinite(gx) || !std::isfinite(gz)) throw std::invalid_argument("non-finite source or receiver position");
+    const double X = gx, Z = gz;
+    const int qx = cell_x(X), qz = cell_z(Z);
+    double U = std::numeric_limits<double>::max();
+    for (int r = 0; r <= G && U == std::numeric_limits<double>::max(); r++)
+      for (int cz = std::max(qz - r, 0); cz <= std::min(qz + r, G - 1); cz++)
+        for (int cx = std::max(qx - r, 0); cx <= std::min(qx + r, G - 1); cx++) {
+          if (std::max(std::abs(cz - qz), std::abs(cx - qx)) != r) continue;
+          const size_t c = static_cast<size_t>(cz) * G + cx;
+          for (uint32_t k = cell_start[c]; k < cell_start[c + 1]; k++) {
+            const size_t p = static_cast<size_t>(cell_elem[k]) * 25 + 12;
+            const double dx = X - px[p], dz = Z - pz[p];
+            U = std::min(U, dx * dx + dz * dz);
+          }
+        }
+    const double reach = std::sqrt(U) * (1.0 + 1e-5), R = reach * (1.0 + 1e-5) / (1.0 - 1e-5);
+    std::vector<int> near;
+    for (int cz = cell_z(Z - R); cz <= cell_z(Z + R); cz++)
+      for (int cx = cell_x(X - R); cx <= cell_x(X + R); cx++) {
+        const size_t c = static_cast<size_t>(cz) * G + cx;
+        near.insert(near.end(), cell_elem.begin() + cell_start[c], cell_elem.begin() + cell_start[c + 1]);
+      }
+    std::sort(near.begin(), near.end());
+    near.erase(std::unique(near.begin(), near.end()), near.end());
     float dmin = std::numeric_limits<float>::max();
     int eg = 0, ixg = 0, izg = 0;
-    for (int e = 0; e < nspec; e++)
+    for (int e : near) {
+      const float *b = &box[4 * static_cast<size_t>(e)];
+      const double bx = std::max({static_cast<double>(b[0]) - X, X - static_cast<double>(b[1]), 0.0});
+      const double bz = std::max({static_cast<double>(b[2]) - Z, Z - static_cast<double>(b[3]), 0.0});
+      if (std::sqrt(bx * bx + bz * bz) * (1.0 - 1e-5) > reach) continue;
       for (int j = 0; j < NGLL; j++)
         for (int i = 0; i < NGLL; i++) {
           const size_t p = static_cast<size_t>(e) * 25 + j * NGLL + i;
@@ -171,17 +329,16 @@ struct Locator2D {
             izg = j;
           }
         }
-    auto im = [&](int e, int j, int i) { return a.index_mapping[static_cast<size_t>(e) * 25 + j * NGLL + i]; };
-    const int corners[4] = {im(eg, 0, 0), im(eg, NGLL - 1, 0), im(eg, 0, NGLL - 1), im(eg, NGLL - 1, NGLL - 1)};
-    std::vector<int> cand{eg};
-    for (int e = 0; e < nspec; e++) {
-      if (e == eg) continue;
-      for (int j : {0, NGLL - 1})
-        for (int i : {0, NGLL - 1}) {
-          const int g = im(e, j, i);
-          if (std::find(corners, corners + 4, g) != corners + 4 && cand.back() != e) cand.push_back(e);
-        }
     }
+    std::vector<int> cand;
+    for (int c : CORNERS) {
+      const int g = a.index_mapping[static_cast<size_t>(eg) * 25 + c];
+      for (uint32_t k = corner_start[g]; k < corner_start[g + 1]; k++)
+        if (corner_elem[k] != eg) cand.push_back(corner_elem[k]);
+    }
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    cand.insert(cand.begin(), eg);
     float best = std::numeric_limits<float>::max();
     ispec = eg;
     xi_out = a.xi[ixg];

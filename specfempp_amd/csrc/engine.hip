@@ -32,6 +32,7 @@ using namespace sfx;
 struct sfx_engine {
   HostAssembly h;  // host copy (numbering and locations are returned from here)
   sfx_run_desc run{};
+  bool host_only = false;  // created by sfx_engine_create_host_*: no device state
   hipStream_t stream = nullptr;
   GllConst q{};
   NewmarkConst nc{};
@@ -534,16 +535,36 @@ void check_engine(const sfx_engine *e) {
   if (!e) throw std::invalid_argument("null engine");
 }
 
-template <class Build> int create(Build &&build, const sfx_run_desc *run, sfx_engine **out) {
+void check_device(const sfx_engine *e) {
+  check_engine(e);
+  if (e->host_only) throw std::invalid_argument("host-only engine (sfx_engine_create_host_*): no device state");
+}
+
+template <class Build> int create(Build &&build, const sfx_run_desc *run, sfx_engine **out, bool host_only) {
   return guarded([&] {
     if (!run || !out) throw std::invalid_argument("null argument");
     *out = nullptr;
     auto e = std::make_unique<sfx_engine>();
     e->run = *run;
     e->h = build();
-    e->upload();
+    e->host_only = host_only;
+    if (!host_only) e->upload();
     *out = e.release();
   });
+}
+
+template <class Mesh, class Assemble>
+int create_engine(const Mesh *mesh, int nsources, const sfx_source *sources, int nreceivers,
+                  const sfx_receiver *receivers, const sfx_run_desc *run, sfx_engine **out, bool host_only,
+                  Assemble &&assemble) {
+  return create(
+      [&] {
+        if (!mesh) throw std::invalid_argument("null mesh");
+        if (nsources < 0 || nreceivers < 0) throw std::invalid_argument("negative count");
+        if ((nsources && !sources) || (nreceivers && !receivers)) throw std::invalid_argument("null array");
+        return assemble(*mesh, nsources, sources, nreceivers, receivers, *run);
+      },
+      run, out, host_only);
 }
 
 }  // namespace
@@ -556,26 +577,22 @@ const char *sfx_last_error(void) { return last_error().c_str(); }
 
 int sfx_engine_create_2d(const sfx_mesh2d *mesh, int nsources, const sfx_source *sources, int nreceivers,
                          const sfx_receiver *receivers, const sfx_run_desc *run, sfx_engine **out) {
-  return create(
-      [&] {
-        if (!mesh) throw std::invalid_argument("null mesh");
-        if (nsources < 0 || nreceivers < 0) throw std::invalid_argument("negative count");
-        if ((nsources && !sources) || (nreceivers && !receivers)) throw std::invalid_argument("null array");
-        return build_assembly_2d(*mesh, nsources, sources, nreceivers, receivers, *run);
-      },
-      run, out);
+  return create_engine(mesh, nsources, sources, nreceivers, receivers, run, out, false, build_assembly_2d);
 }
 
 int sfx_engine_create_3d(const sfx_mesh3d *mesh, int nsources, const sfx_source *sources, int nreceivers,
                          const sfx_receiver *receivers, const sfx_run_desc *run, sfx_engine **out) {
-  return create(
-      [&] {
-        if (!mesh) throw std::invalid_argument("null mesh");
-        if (nsources < 0 || nreceivers < 0) throw std::invalid_argument("negative count");
-        if ((nsources && !sources) || (nreceivers && !receivers)) throw std::invalid_argument("null array");
-        return build_assembly_3d(*mesh, nsources, sources, nreceivers, receivers, *run);
-      },
-      run, out);
+  return create_engine(mesh, nsources, sources, nreceivers, receivers, run, out, false, build_assembly_3d);
+}
+
+int sfx_engine_create_host_2d(const sfx_mesh2d *mesh, int nsources, const sfx_source *sources, int nreceivers,
+                              const sfx_receiver *receivers, const sfx_run_desc *run, sfx_engine **out) {
+  return create_engine(mesh, nsources, sources, nreceivers, receivers, run, out, true, build_assembly_2d);
+}
+
+int sfx_engine_create_host_3d(const sfx_mesh3d *mesh, int nsources, const sfx_source *sources, int nreceivers,
+                              const sfx_receiver *receivers, const sfx_run_desc *run, sfx_engine **out) {
+  return create_engine(mesh, nsources, sources, nreceivers, receivers, run, out, true, build_assembly_3d);
 }
 
 void sfx_engine_destroy(sfx_engine *engine) {
@@ -643,7 +660,7 @@ int sfx_get_locations(const sfx_engine *e, int which, int32_t *ispec, double *lo
 
 int sfx_initialize(sfx_engine *e) {
   return guarded([&] {
-    check_engine(e);
+    check_device(e);
     const size_t stride = static_cast<size_t>(e->h.ncomp) * e->h.nglob_pad;
     launch_fill(e->stream, e->fields.p, 3 * stride, 0.0f);
     if (e->seismo.n) launch_fill(e->stream, e->seismo.p, e->seismo.n, 0.0f);
@@ -656,7 +673,7 @@ int sfx_initialize(sfx_engine *e) {
 int sfx_predictor(sfx_engine *e) {
   int dofs = 0;
   const int rc = guarded([&] {
-    check_engine(e);
+    check_device(e);
     e->require_init();
     e->refresh_acceleration();
     launch_predictor(e->stream, e->h.ncomp, e->f(), e->nc);
@@ -669,7 +686,7 @@ int sfx_predictor(sfx_engine *e) {
 int sfx_update_wavefields(sfx_engine *e, int istep) {
   int nelem = 0;
   const int rc = guarded([&] {
-    check_engine(e);
+    check_device(e);
     e->require_init();
     Fields ff = e->f();
     if (e->block_path()) {
@@ -696,7 +713,7 @@ int sfx_update_wavefields(sfx_engine *e, int istep) {
 int sfx_corrector(sfx_engine *e) {
   int dofs = 0;
   const int rc = guarded([&] {
-    check_engine(e);
+    check_device(e);
     e->require_init();
     e->refresh_acceleration();
     launch_divide_corrector(e->stream, e->h.ncomp, e->f(), e->nc, 0, 1);
@@ -708,7 +725,7 @@ int sfx_corrector(sfx_engine *e) {
 
 int sfx_compute_seismograms(sfx_engine *e, int isig) {
   return guarded([&] {
-    check_engine(e);
+    check_device(e);
     e->require_init();
     e->refresh_acceleration();
     e->seismograms(e->f(), isig, 0);
@@ -718,7 +735,7 @@ int sfx_compute_seismograms(sfx_engine *e, int isig) {
 
 int sfx_step(sfx_engine *e, int istep) {
   return guarded([&] {
-    check_engine(e);
+    check_device(e);
     e->require_init();
     e->step(istep, nullptr, true);
     SFX_HIP(hipGetLastError());
@@ -727,7 +744,7 @@ int sfx_step(sfx_engine *e, int istep) {
 
 int sfx_run(sfx_engine *e, int first, int count) {
   return guarded([&] {
-    check_engine(e);
+    check_device(e);
     e->require_init();
     if (first < 0 || count < 0 || first + count > e->run.nstep) throw std::invalid_argument("step range out of bounds");
     if (e->graph_2d() && count > 1) {
@@ -780,14 +797,14 @@ int sfx_run(sfx_engine *e, int first, int count) {
 
 int sfx_synchronize(sfx_engine *e) {
   return guarded([&] {
-    check_engine(e);
+    check_device(e);
     SFX_HIP(hipStreamSynchronize(e->stream));
   });
 }
 
 int sfx_get_seismograms(const sfx_engine *e, float *out) {
   return guarded([&] {
-    check_engine(e);
+    check_device(e);
     if (!out) throw std::invalid_argument("null output");
     SFX_HIP(hipStreamSynchronize(e->stream));
     if (e->seismo.n) SFX_HIP(hipMemcpy(out, e->seismo.p, e->seismo.n * sizeof(float), hipMemcpyDeviceToHost));
@@ -796,7 +813,7 @@ int sfx_get_seismograms(const sfx_engine *e, float *out) {
 
 int sfx_get_field(const sfx_engine *ce, int which, float *out) {
   return guarded([&] {
-    check_engine(ce);
+    check_device(ce);
     if (which < 0 || which > 3) throw std::invalid_argument("which must be 0..3");
     if (!out) throw std::invalid_argument("null output");
     // the stored acceleration is refreshed lazily after fused gather steps
@@ -820,7 +837,7 @@ int sfx_get_field(const sfx_engine *ce, int which, float *out) {
 
 int sfx_set_field(sfx_engine *e, int which, const float *in) {
   return guarded([&] {
-    check_engine(e);
+    check_device(e);
     if (which < 0 || which > 2) throw std::invalid_argument("which must be 0..2");
     if (!in) throw std::invalid_argument("null input");
     SFX_HIP(hipStreamSynchronize(e->stream));
@@ -842,7 +859,7 @@ int sfx_set_field(sfx_engine *e, int which, const float *in) {
 
 int sfx_set_properties(sfx_engine *e, const float *rho, const float *kappa, const float *mu) {
   return guarded([&] {
-    check_engine(e);
+    check_device(e);
     SFX_HIP(hipStreamSynchronize(e->stream));
     set_properties_2d(e->h, rho, kappa, mu, e->run.dt);
     hipStream_t s = e->stream;
@@ -862,7 +879,7 @@ int sfx_set_properties(sfx_engine *e, const float *rho, const float *kappa, cons
 
 int sfx_forward_record(sfx_engine *e, int snapshot_every) {
   return guarded([&] {
-    check_engine(e);
+    check_device(e);
     e->require_init();
     if (e->h.dim != 2) throw std::invalid_argument("adjoint runs are implemented for 2-D meshes");
     if (snapshot_every < 0) throw std::invalid_argument("snapshot_every must be >= 0");
@@ -900,7 +917,7 @@ int sfx_forward_record(sfx_engine *e, int snapshot_every) {
 
 int sfx_set_adjoint_sources(sfx_engine *e, int nadj, const sfx_receiver *positions, const float *stf) {
   return guarded([&] {
-    check_engine(e);
+    check_device(e);
     place_adjoint_sources_2d(e->h, nadj, positions, stf, e->run.nstep);
     hipStream_t s = e->stream;
     e->adj_order.upload(e->h.adj_order, s);
@@ -915,7 +932,7 @@ int sfx_set_adjoint_sources(sfx_engine *e, int nadj, const sfx_receiver *positio
 
 int sfx_adjoint_run(sfx_engine *e) {
   return guarded([&] {
-    check_engine(e);
+    check_device(e);
     if (e->h.dim != 2) throw std::invalid_argument("adjoint runs are implemented for 2-D meshes");
     if (!e->recorded) throw std::logic_error("sfx_forward_record must complete before sfx_adjoint_run");
     if (!e->adj_ready) throw std::logic_error("sfx_set_adjoint_sources must be called before sfx_adjoint_run");
@@ -928,7 +945,7 @@ int sfx_adjoint_run(sfx_engine *e) {
 
 int sfx_get_point_weights(const sfx_engine *e, float *out) {
   return guarded([&] {
-    check_engine(e);
+    check_device(e);
     if (!out) throw std::invalid_argument("null output");
     if (e->h.dim != 2) throw std::invalid_argument("point weights are implemented for 2-D meshes");
     const size_t npe = e->h.npe;
@@ -943,7 +960,7 @@ int sfx_get_point_weights(const sfx_engine *e, float *out) {
 
 int sfx_get_kernels(const sfx_engine *e, float *out) {
   return guarded([&] {
-    check_engine(e);
+    check_device(e);
     if (!out) throw std::invalid_argument("null output");
     if (!e->kern.n) throw std::logic_error("no kernels: run sfx_adjoint_run first");
     SFX_HIP(hipStreamSynchronize(e->stream));
@@ -960,7 +977,7 @@ int sfx_get_kernels(const sfx_engine *e, float *out) {
 
 int sfx_get_timings(const sfx_engine *e, double *out4) {
   return guarded([&] {
-    check_engine(e);
+    check_device(e);
     if (!out4) throw std::invalid_argument("null output");
     for (int k = 0; k < 4; k++) out4[k] = e->timings[k];
   });

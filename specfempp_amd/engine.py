@@ -267,8 +267,11 @@ class Engine2D(_EngineBase):
         return dict(zip(self.KERNEL_NAMES, out))
 
     def __init__(self, mesh, sources, receivers, dt, nstep, nstep_between_samples=1,
-                 seismogram_types=("displacement",), chunk_size=1, scatter="colored"):
+                 seismogram_types=("displacement",), chunk_size=1, scatter="colored", host_only=False):
+        """host_only: build the host assembly only (numbering, locations; usable
+        by the data-contract getters and ``Batch``), no device memory."""
         self.lib = _native.load()
+        self.host_only = host_only
         self.nstep = nstep
         self.dt = dt
         coorg = np.ascontiguousarray(np.asarray(mesh.coorg, dtype=np.float64).T)
@@ -283,8 +286,9 @@ class Engine2D(_EngineBase):
         run = _run_desc(dt, nstep, nstep_between_samples, seismogram_types, chunk_size, scatter)
         h = C.c_void_p()
         self.h = None
-        self._check(self.lib.sfx_engine_create_2d(C.byref(m), len(sources), _sources(sources, 2), len(receivers),
-                                                  _receivers(receivers), C.byref(run), C.byref(h)))
+        create = self.lib.sfx_engine_create_host_2d if host_only else self.lib.sfx_engine_create_2d
+        self._check(create(C.byref(m), len(sources), _sources(sources, 2), len(receivers), _receivers(receivers),
+                           C.byref(run), C.byref(h)))
         self.h = h
         self._finish_init(seismogram_types)
 
@@ -301,11 +305,12 @@ class Engine3D(_EngineBase):
     AUTO_3D = "gather"
 
     def __init__(self, mesh, sources, receivers, dt, nstep, nstep_between_samples=1,
-                 seismogram_types=("displacement",), scatter="auto"):
+                 seismogram_types=("displacement",), scatter="auto", host_only=False):
         if scatter == "auto":
             even = all(int(mesh[k]) % 2 == 0 for k in ("nx", "ny", "nz"))
             scatter = self.AUTO_3D if (self.AUTO_3D != "block" or even) else "gather"
         self.scatter = scatter
+        self.host_only = host_only
         self.lib = _native.load()
         self.nstep = nstep
         self.dt = dt
@@ -325,8 +330,9 @@ class Engine3D(_EngineBase):
         run = _run_desc(dt, nstep, nstep_between_samples, seismogram_types, 1, scatter)
         h = C.c_void_p()
         self.h = None
-        self._check(self.lib.sfx_engine_create_3d(C.byref(m), len(sources), _sources(sources, 3), len(receivers),
-                                                  _receivers(receivers), C.byref(run), C.byref(h)))
+        create = self.lib.sfx_engine_create_host_3d if host_only else self.lib.sfx_engine_create_3d
+        self._check(create(C.byref(m), len(sources), _sources(sources, 3), len(receivers), _receivers(receivers),
+                           C.byref(run), C.byref(h)))
         self.h = h
         self._finish_init(seismogram_types)
 

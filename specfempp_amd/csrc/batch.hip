@@ -407,6 +407,7 @@ struct sfx_batch {
 
   // Steps [first, last) (first = 0 after reset()).
   void advance(int first, int last) {
+    Throttle throttle(stream);
     int istep = first;
     while (istep < last) {
       const int k = running(istep);
@@ -417,7 +418,12 @@ struct sfx_batch {
         if (!graph_cache[k]) {
           hipGraph_t g;
           SFX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-          enqueue_step(k);
+          try {
+            enqueue_step(k);
+          } catch (...) {
+            (void)hipStreamEndCapture(stream, &g);
+            throw;
+          }
           SFX_HIP(hipStreamEndCapture(stream, &g));
           hipGraphExec_t ex;
           const hipError_t err = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
@@ -425,9 +431,15 @@ struct sfx_batch {
           SFX_HIP(err);
           graph_cache[k] = ex;
         }
-        for (int i = istep; i < end; i++) SFX_HIP(hipGraphLaunch(graph_cache[k], stream));
+        for (int i = istep; i < end; i++) {
+          SFX_HIP(hipGraphLaunch(graph_cache[k], stream));
+          throttle.tick();
+        }
       } else {
-        for (int i = istep; i < end; i++) enqueue_step(k);
+        for (int i = istep; i < end; i++) {
+          enqueue_step(k);
+          throttle.tick();
+        }
       }
       istep = end;
     }
@@ -499,7 +511,7 @@ int sfx_batch_tune(sfx_batch *b, int steps_per_candidate, int *choice3) {
       choice3[1] = best_g ? 1 : 0;
       choice3[2] = b->ncolors;
     }
-  });
+  }, b ? b->stream : nullptr);
 }
 
 int sfx_batch_set_config(sfx_batch *b, int elems_per_block, int use_graph) {
@@ -510,7 +522,7 @@ int sfx_batch_set_config(sfx_batch *b, int elems_per_block, int use_graph) {
     b->elems_per_block = elems_per_block;
     b->use_graph = use_graph != 0;
     b->clear_graphs();
-  });
+  }, b ? b->stream : nullptr);
 }
 
 int sfx_batch_run(sfx_batch *b) {
@@ -529,7 +541,7 @@ int sfx_batch_run(sfx_batch *b) {
     b->last_run_ms = ms;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(e);
-  });
+  }, b ? b->stream : nullptr);
 }
 
 int sfx_batch_get_seismograms(const sfx_batch *b, int i, float *out) {
@@ -540,7 +552,7 @@ int sfx_batch_get_seismograms(const sfx_batch *b, int i, float *out) {
     const int k = b->pos_of[i];
     const size_t n = b->seis_off[k + 1] - b->seis_off[k];
     if (n) SFX_HIP(hipMemcpy(out, b->seismo.p + b->seis_off[k], n * sizeof(float), hipMemcpyDeviceToHost));
-  });
+  }, b ? b->stream : nullptr);
 }
 
 int sfx_batch_get_info(const sfx_batch *b, double *out6) {

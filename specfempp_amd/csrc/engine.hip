@@ -476,6 +476,7 @@ struct sfx_engine {
     };
     const bool graphs = graph_2d();
     if (graphs) set_step(nstep - 1);
+    Throttle throttle(stream);
     int isig = 0;
     for (int istep = nstep - 1; istep >= 0; istep--) {
       const bool first = istep == nstep - 1;
@@ -484,6 +485,7 @@ struct sfx_engine {
       if (graphs && !first && !ckpt) {
         if (!g_adj) g_adj = capture(enqueue_regular);
         SFX_HIP(hipGraphLaunch(g_adj, stream));
+        throttle.tick();
         if (sample) isig++;
         continue;
       }
@@ -519,6 +521,7 @@ struct sfx_engine {
         launch_update_pair(stream, h.ncomp, adj, bwd, nc, first ? 0 : 1, racc.p);
       }
       if (graphs) launch_step_tick(stream, d_step.p, -1);
+      throttle.tick();
     }
     launch_frechet_finish(stream, static_cast<int>(npts), gather.p, kappa.p, mu.p, rho.p, racc.p, Kmu, Kkappa, kern.p);
   }
@@ -534,6 +537,8 @@ namespace {
 void check_engine(const sfx_engine *e) {
   if (!e) throw std::invalid_argument("null engine");
 }
+
+hipStream_t stream_of(const sfx_engine *e) { return e ? e->stream : nullptr; }
 
 void check_device(const sfx_engine *e) {
   check_engine(e);
@@ -667,7 +672,7 @@ int sfx_initialize(sfx_engine *e) {
     e->a_stale = false;
     SFX_HIP(hipGetLastError());
     e->initialized = true;
-  });
+  }, stream_of(e));
 }
 
 int sfx_predictor(sfx_engine *e) {
@@ -679,7 +684,7 @@ int sfx_predictor(sfx_engine *e) {
     launch_predictor(e->stream, e->h.ncomp, e->f(), e->nc);
     SFX_HIP(hipGetLastError());
     dofs = e->h.nglob * e->h.ncomp;
-  });
+  }, stream_of(e));
   return rc == SFX_OK ? dofs : rc;
 }
 
@@ -706,7 +711,7 @@ int sfx_update_wavefields(sfx_engine *e, int istep) {
     }
     SFX_HIP(hipGetLastError());
     nelem = e->h.nspec;
-  });
+  }, stream_of(e));
   return rc == SFX_OK ? nelem : rc;
 }
 
@@ -719,7 +724,7 @@ int sfx_corrector(sfx_engine *e) {
     launch_divide_corrector(e->stream, e->h.ncomp, e->f(), e->nc, 0, 1);
     SFX_HIP(hipGetLastError());
     dofs = e->h.nglob * e->h.ncomp;
-  });
+  }, stream_of(e));
   return rc == SFX_OK ? dofs : rc;
 }
 
@@ -730,7 +735,7 @@ int sfx_compute_seismograms(sfx_engine *e, int isig) {
     e->refresh_acceleration();
     e->seismograms(e->f(), isig, 0);
     SFX_HIP(hipGetLastError());
-  });
+  }, stream_of(e));
 }
 
 int sfx_step(sfx_engine *e, int istep) {
@@ -739,7 +744,7 @@ int sfx_step(sfx_engine *e, int istep) {
     e->require_init();
     e->step(istep, nullptr, true);
     SFX_HIP(hipGetLastError());
-  });
+  }, stream_of(e));
 }
 
 int sfx_run(sfx_engine *e, int first, int count) {
@@ -757,7 +762,11 @@ int sfx_run(sfx_engine *e, int first, int count) {
         e->ev.push_back(ev);
       }
       SFX_HIP(hipEventRecord(e->ev[0], e->stream));
-      for (int i = 0; i < count; i++) SFX_HIP(hipGraphLaunch(e->g_fwd, e->stream));
+      Throttle throttle(e->stream);
+      for (int i = 0; i < count; i++) {
+        SFX_HIP(hipGraphLaunch(e->g_fwd, e->stream));
+        throttle.tick();
+      }
       SFX_HIP(hipEventRecord(e->ev[1], e->stream));
       SFX_HIP(hipStreamSynchronize(e->stream));
       float all = 0.0f;
@@ -767,14 +776,18 @@ int sfx_run(sfx_engine *e, int first, int count) {
       return;
     }
     // Every step records 5 events on the engine stream; they are read after
-    // the last step, so the host never waits inside the loop.
+    // the last step (the host only waits to keep the queue bounded).
     const size_t need = 5 * static_cast<size_t>(count);
     while (e->ev.size() < need) {
       hipEvent_t ev;
       SFX_HIP(hipEventCreate(&ev));
       e->ev.push_back(ev);
     }
-    for (int i = 0; i < count; i++) e->step(first + i, &e->ev[5 * static_cast<size_t>(i)], i == count - 1);
+    Throttle throttle(e->stream);
+    for (int i = 0; i < count; i++) {
+      e->step(first + i, &e->ev[5 * static_cast<size_t>(i)], i == count - 1);
+      throttle.tick();
+    }
     SFX_HIP(hipGetLastError());
     SFX_HIP(hipStreamSynchronize(e->stream));
     double acc[4] = {0, 0, 0, 0};
@@ -792,14 +805,14 @@ int sfx_run(sfx_engine *e, int first, int count) {
       acc[3] += all;
     }
     for (int k = 0; k < 4; k++) e->timings[k] = count ? acc[k] / count : 0.0;
-  });
+  }, stream_of(e));
 }
 
 int sfx_synchronize(sfx_engine *e) {
   return guarded([&] {
     check_device(e);
     SFX_HIP(hipStreamSynchronize(e->stream));
-  });
+  }, stream_of(e));
 }
 
 int sfx_get_seismograms(const sfx_engine *e, float *out) {
@@ -808,7 +821,7 @@ int sfx_get_seismograms(const sfx_engine *e, float *out) {
     if (!out) throw std::invalid_argument("null output");
     SFX_HIP(hipStreamSynchronize(e->stream));
     if (e->seismo.n) SFX_HIP(hipMemcpy(out, e->seismo.p, e->seismo.n * sizeof(float), hipMemcpyDeviceToHost));
-  });
+  }, stream_of(e));
 }
 
 int sfx_get_field(const sfx_engine *ce, int which, float *out) {
@@ -832,7 +845,7 @@ int sfx_get_field(const sfx_engine *ce, int which, float *out) {
       else
         for (size_t g = 0; g < ng; g++) o[g] = buf[e->h.lattice_of[g]];
     }
-  });
+  }, stream_of(ce));
 }
 
 int sfx_set_field(sfx_engine *e, int which, const float *in) {
@@ -854,7 +867,7 @@ int sfx_set_field(sfx_engine *e, int which, const float *in) {
                         e->h.nglob_pad * sizeof(float), hipMemcpyHostToDevice));
     }
     if (which == 2) e->a_stale = false;
-  });
+  }, stream_of(e));
 }
 
 int sfx_set_properties(sfx_engine *e, const float *rho, const float *kappa, const float *mu) {
@@ -874,7 +887,7 @@ int sfx_set_properties(sfx_engine *e, const float *rho, const float *kappa, cons
     SFX_HIP(hipStreamSynchronize(s));
     e->recorded = false;
     e->invalidate_graphs();
-  });
+  }, stream_of(e));
 }
 
 int sfx_forward_record(sfx_engine *e, int snapshot_every) {
@@ -894,6 +907,7 @@ int sfx_forward_record(sfx_engine *e, int snapshot_every) {
     try {
       const bool graphs = e->graph_2d();
       if (graphs) e->set_step(0);
+      Throttle throttle(e->stream);
       for (int istep = 0; istep < e->run.nstep; istep++) {
         const bool last = istep == e->run.nstep - 1;
         if (graphs && !last && !e->is_checkpoint(istep)) {
@@ -903,6 +917,7 @@ int sfx_forward_record(sfx_engine *e, int snapshot_every) {
           e->step(istep, nullptr, last);
           if (graphs) launch_step_tick(e->stream, e->d_step.p, 1);
         }
+        throttle.tick();
       }
     } catch (...) {
       e->recording = false;
@@ -912,7 +927,7 @@ int sfx_forward_record(sfx_engine *e, int snapshot_every) {
     SFX_HIP(hipGetLastError());
     SFX_HIP(hipStreamSynchronize(e->stream));
     e->recorded = true;
-  });
+  }, stream_of(e));
 }
 
 int sfx_set_adjoint_sources(sfx_engine *e, int nadj, const sfx_receiver *positions, const float *stf) {
@@ -927,7 +942,7 @@ int sfx_set_adjoint_sources(sfx_engine *e, int nadj, const sfx_receiver *positio
     SFX_HIP(hipStreamSynchronize(s));
     e->adj_ready = true;
     e->invalidate_graphs();
-  });
+  }, stream_of(e));
 }
 
 int sfx_adjoint_run(sfx_engine *e) {
@@ -940,7 +955,7 @@ int sfx_adjoint_run(sfx_engine *e) {
     e->adjoint_run();
     SFX_HIP(hipGetLastError());
     SFX_HIP(hipStreamSynchronize(e->stream));
-  });
+  }, stream_of(e));
 }
 
 int sfx_get_point_weights(const sfx_engine *e, float *out) {
@@ -955,7 +970,7 @@ int sfx_get_point_weights(const sfx_engine *e, float *out) {
           const size_t p = static_cast<size_t>(c) * npe + iz * 5 + ix;
           out[static_cast<size_t>(e->h.compute_to_mesh[c]) * npe + iz * 5 + ix] = e->h.w[ix] * e->h.w[iz] * e->h.jac[p];
         }
-  });
+  }, stream_of(e));
 }
 
 int sfx_get_kernels(const sfx_engine *e, float *out) {
@@ -972,7 +987,7 @@ int sfx_get_kernels(const sfx_engine *e, float *out) {
         const size_t m = e->h.compute_to_mesh[e->h.perm[slot]];
         std::memcpy(out + (k * nspec + m) * npe, buf.data() + (k * nspec + slot) * npe, npe * sizeof(float));
       }
-  });
+  }, stream_of(e));
 }
 
 int sfx_get_timings(const sfx_engine *e, double *out4) {
@@ -980,7 +995,7 @@ int sfx_get_timings(const sfx_engine *e, double *out4) {
     check_device(e);
     if (!out4) throw std::invalid_argument("null output");
     for (int k = 0; k < 4; k++) out4[k] = e->timings[k];
-  });
+  }, stream_of(e));
 }
 
 }  // extern "C"

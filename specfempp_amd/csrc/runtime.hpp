@@ -59,23 +59,59 @@ template <class T> struct DevBuf {
   }
 };
 
-// Run fn; map exceptions to SFX_ERR_* and keep the message.
-template <class F> int guarded(F &&fn) {
+// Keeps at most two windows of steps queued on a stream: a loop of
+// asynchronous launches calls tick() after each step, and every window steps
+// the host waits for the work queued one window earlier.  Not for use while
+// the stream is being captured.
+struct Throttle {
+  hipStream_t s;
+  int window, n = 0;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  explicit Throttle(hipStream_t stream, int steps = 256) : s(stream), window(steps) {}
+  Throttle(const Throttle &) = delete;
+  Throttle &operator=(const Throttle &) = delete;
+  ~Throttle() {
+    for (auto e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  void tick() {
+    if (++n % window) return;
+    const int w = n / window;
+    hipEvent_t &mine = ev[w & 1];
+    if (!mine) SFX_HIP(hipEventCreateWithFlags(&mine, hipEventDisableTiming));
+    SFX_HIP(hipEventRecord(mine, s));
+    if (w >= 2) SFX_HIP(hipEventSynchronize(ev[(w - 1) & 1]));
+  }
+};
+
+// Run fn; map exceptions to SFX_ERR_* and keep the message.  When fn fails
+// and drain is a stream, the work fn already queued on it is waited for
+// before returning, so no kernel of a failed call is still running when the
+// caller frees or exits.
+template <class F> int guarded(F &&fn, hipStream_t drain = nullptr) {
   try {
     fn();
     return SFX_OK;
-  } catch (const HipError &e) {
-    last_error() = e.what();
-    return SFX_ERR_HIP;
-  } catch (const std::invalid_argument &e) {
-    last_error() = e.what();
-    return SFX_ERR_ARG;
-  } catch (const std::logic_error &e) {
-    last_error() = e.what();
-    return SFX_ERR_STATE;
-  } catch (const std::exception &e) {
-    last_error() = e.what();
-    return SFX_ERR_MESH;
+  } catch (...) {
+    if (drain) (void)hipStreamSynchronize(drain);
+    try {
+      throw;
+    } catch (const HipError &e) {
+      last_error() = e.what();
+      return SFX_ERR_HIP;
+    } catch (const std::invalid_argument &e) {
+      last_error() = e.what();
+      return SFX_ERR_ARG;
+    } catch (const std::logic_error &e) {
+      last_error() = e.what();
+      return SFX_ERR_STATE;
+    } catch (const std::exception &e) {
+      last_error() = e.what();
+      return SFX_ERR_MESH;
+    } catch (...) {
+      last_error() = "unknown error";
+      return SFX_ERR_STATE;
+    }
   }
 }
 

@@ -23,6 +23,11 @@ from conftest import GOLDEN, load_case, rel_l1_per_station, rel_l2
 
 pytestmark = pytest.mark.gpu
 
+# The 3-D block path is switched off on the device (an unexplained GPU memory
+# fault in its seam-call test); its tests stay skipped until the cause is found.
+BLOCK_OFF = pytest.mark.skip(reason="3-D block path switched off: its seam-call test ended in a GPU memory fault "
+                                    "whose cause has not been found")
+
 TOL_SEISMO = 1e-5
 
 
@@ -136,7 +141,8 @@ def test_config1_homogeneous_80x80():
 
 # The Ricker wavelet (f0 = 5 Hz) peaks at step 240 (t0 = -1.2 / f0); windows
 # reach well past it so the traces are signal, not rounding noise.
-@pytest.mark.parametrize("n,nstep,scatter", [(6, 600, "block"), (10, 500, "block"), (6, 600, "gather"),
+@pytest.mark.parametrize("n,nstep,scatter", [pytest.param(6, 600, "block", marks=BLOCK_OFF),
+                                            pytest.param(10, 500, "block", marks=BLOCK_OFF), (6, 600, "gather"),
                                             (10, 500, "gather"), (6, 600, "colored"), (6, 600, "atomic")])
 def test_3d_layered_matches_oracle(n, nstep, scatter):
     from specfempp_amd import Engine3D, meshes
@@ -162,7 +168,7 @@ def test_3d_layered_matches_oracle(n, nstep, scatter):
             assert e32 <= TOL_SEISMO
 
 
-@pytest.mark.parametrize("n,scatter", [(5, "gather"), (6, "block")])
+@pytest.mark.parametrize("n,scatter", [(5, "gather"), pytest.param(6, "block", marks=BLOCK_OFF)])
 def test_3d_seam_path_and_fields_match_fused_path(n, scatter):
     """Seam calls (predictor / update_wavefields / corrector / seismograms) on
     the 3-D gather and block paths equal the fused step, and the wavefield read
@@ -223,6 +229,7 @@ def _box_workload():
     return mesh, src, rec, 1e-3, 400
 
 
+@BLOCK_OFF
 def test_3d_block_box_all_absorbing_matches_oracle():
     from specfempp_amd import Engine3D
 
@@ -250,6 +257,7 @@ def test_3d_block_box_all_absorbing_matches_oracle():
             assert e32 <= TOL_SEISMO
 
 
+@BLOCK_OFF
 def test_3d_block_step_calls_equal_run_bitwise():
     """Stepping one call per step stores the full acceleration after every
     step; the displacement and velocity updates are the same instructions, so

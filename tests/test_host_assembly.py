@@ -86,3 +86,14 @@ def test_targets_on_gll_points_of_a_regular_mesh():
     flat = coord.reshape(-1, 2)
     pts = [dict(x=float(x), z=float(z)) for x, z in flat[rng.choice(len(flat), 40, replace=False)]]
     _compare(w.mesh, w.sources, pts, w.dt)
+
+
+def test_3d_block_path_is_off_on_the_device_and_host_assembly_still_builds():
+    from specfempp_amd import Engine3D, SfxError
+
+    w = meshes.config3(n=4, nstep=4, h=100.0, nrec_side=2)
+    with pytest.raises(SfxError, match="switched off"):
+        Engine3D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, scatter="block")
+    eng = Engine3D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, scatter="block", host_only=True)
+    assert np.array_equal(eng.index_mapping(), oracle.Oracle3D(w.mesh, 8, w.dt).ibool())
+    eng.close()

@@ -5,9 +5,12 @@ multi-event leg (64 2-D simulations in one batch vs the same 64 run one after
 another) and the adjoint leg (200^2 2-D gradient vs a plain forward run).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 64] [--no-cpu]
-                    [--no-batch] [--no-adjoint] [--scatter auto|block|gather]
+                    [--no-batch] [--no-adjoint] [--scatter auto|gather|colored|atomic]
+    python bench.py --config2 [--c2-n 4000] [--steps K] [--warmup W]
 
-Prints one JSON line.  A "step" is one Newmark step of the whole hot path:
+Prints one JSON line.  --config2 measures instead the 2-D heterogeneous
+workload (Gaussian-random-field medium, 4000^2 quads of 2.5 m, 256 M points)
+with the roofline of its stiffness kernel.  A "step" is one Newmark step of the whole hot path:
 sources, stiffness (gradient, stress, divergence, assembly), Stacey,
 seismograms, fused mass division + corrector + predictor.
 """
@@ -61,6 +64,21 @@ def algorithmic_bytes_3d(nspec: int, nglob: int) -> dict:
     survey = nspec * 125 * 12 + nglob * 36 + nglob * 84
     return {"elemforce": elem, "update": upd, "per_pt_step": (elem + upd) / nglob,
             "survey_per_pt_step": survey / nglob}
+
+
+def measured_traffic(workload: str, scatter: str, kernel: str):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE runs, each scaled by
+    the copy-kernel factor recorded with it), or None."""
+    path = os.path.join(ROOT, "profiles", "gather3d_64cube_pmc.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        pmc = json.load(fh)
+    if pmc.get("workload") != workload or pmc.get("scatter") != scatter or kernel not in pmc["kib_per_dispatch"]:
+        return None
+    c = pmc["kib_per_dispatch"][kernel]
+    return 1024.0 * (c["FETCH_SIZE"] * pmc["scale"]["FETCH_SIZE"] + c["WRITE_SIZE"] * pmc["scale"]["WRITE_SIZE"])
 
 
 def cpu_baseline(w, nsteps: int) -> dict:
@@ -186,6 +204,80 @@ def adjoint_leg(snapshot_every: int = 55) -> dict:
             "misfit": misfit, "snapshot_every": snapshot_every, "kernels_finite": finite}
 
 
+def algorithmic_bytes_2d(nspec: int, nglob: int) -> dict:
+    """Bytes each kernel of the 2-D path must move per step (FP32):
+      stiffness_2d (all colors): gather index, xix, xiz, gammax, gammaz, J,
+                   kappa, mu per local point (32 B x 25 / element), the
+                   displacement gathered once per point (8 B), the acceleration
+                   read-modify-written once per point (16 B)
+      update_fused: a, M^-1, u, v read (32 B) and u, v, a written (24 B) per point
+    (the data-model figure of 130 B per point-step for 25/16 local points per
+    point)."""
+    stiff = nspec * 25 * 32 + nglob * 24
+    upd = nglob * 56
+    return {"stiffness_2d": stiff, "update_fused": upd, "per_pt_step": (stiff + upd) / nglob}
+
+
+def config2_leg(n: int, steps: int, warmup: int, sample_steps: int = 5) -> dict:
+    """Config 2: the heterogeneous 2-D workload at n^2 quads of 2.5 m (full
+    size n = 4000).  Graph-launched steps are timed for the throughput; a few
+    single steps give the per-kernel times for the roofline."""
+    import numpy as np
+    import torch
+
+    from specfempp_amd import Engine2D, meshes
+
+    t = time.perf_counter()
+    w = meshes.config2(n=n, L=2.5 * n, nstep=warmup + steps + sample_steps)
+    mesh_s = time.perf_counter() - t
+    print(f"[config2] mesh {n}^2 built in {mesh_s:.1f} s", file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    eng = Engine2D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, 1, ("displacement",))
+    setup_s = time.perf_counter() - t
+    print(f"[config2] engine ({eng.nglob} points) built in {setup_s:.1f} s", file=sys.stderr, flush=True)
+    nglob, nspec = eng.nglob, eng.nspec
+    eng.initialize()
+    if warmup:
+        eng.run(0, warmup)
+    eng.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.run(warmup, steps)
+    eng.synchronize()
+    elapsed = time.perf_counter() - t0
+    acc = {"stiffness_ms": 0.0, "update_ms": 0.0, "boundary_source_seismo_ms": 0.0, "step_ms": 0.0}
+    for i in range(sample_steps):
+        eng.run(warmup + steps + i, 1)
+        for k, v in eng.timings().items():
+            acc[k] += v / sample_steps
+    finite = bool(np.isfinite(eng.seismograms()).all())
+    eng.close()
+    value = nglob * steps / elapsed
+    ab = algorithmic_bytes_2d(nspec, nglob)
+    kern = {"stiffness_2d_kernel": (ab["stiffness_2d"], acc["stiffness_ms"]),
+            "update_fused_kernel": (ab["update_fused"], acc["update_ms"])}
+    dominant = max(kern, key=lambda k: kern[k][1])
+    nbytes, ms = kern[dominant]
+    achieved = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    step_gbs = ab["per_pt_step"] * value / 1e9
+    return {
+        "metric": "GLL pt-steps/s, 2D elastic heterogeneous NGLL=5 FP32 forward (Gaussian random field)",
+        "value": value, "unit": "GLL pt-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": 1e3 * elapsed / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"2d_grf_{n}x{n}_ngll5", "nspec": nspec, "nglob": nglob, "h": 2.5, "dt": w.dt,
+                   "dt_stated": w.dt_stated, "nrec": len(w.receivers), "mesh_s": round(mesh_s, 1),
+                   "setup_s": round(setup_s, 1)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dominant,
+                     "algorithmic_bytes_per_launch": nbytes, "avg_ms": ms},
+        "kernels": {k: {"avg_ms": v[1], "algorithmic_bytes": v[0],
+                        "GBs": (v[0] / (v[1] * 1e-3) / 1e9 if v[1] > 0 else 0.0)} for k, v in kern.items()},
+        "kernels_ms": acc, "step_hbm_gbs": step_gbs, "step_hbm_frac": step_gbs / HBM_PEAK_GBS,
+        "algorithmic_bytes_per_pt_step": ab["per_pt_step"], "seismograms_finite": finite,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,11 +292,17 @@ def main() -> int:
     ap.add_argument("--batch-sims", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (scale reference for HBM byte counters)")
+    ap.add_argument("--config2", action="store_true",
+                    help="measure the 2-D heterogeneous workload instead (minutes of host setup at full size)")
+    ap.add_argument("--c2-n", type=int, default=4000, help="--config2: quads per side (2.5 m each)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine is single-GPU).",
               file=sys.stderr)
         return 2
+    if args.config2:
+        print(json.dumps(config2_leg(args.c2_n, args.steps, args.warmup)))
+        return 0
 
     import numpy as np
     import torch
@@ -266,8 +364,10 @@ def main() -> int:
                    "dt": w.dt, "dt_stated": w.dt_stated, "nrec": len(w.receivers), "source": "moment tensor",
                    "scatter": scatter, "setup_s": round(setup_s, 2)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dominant,
-                     "algorithmic_bytes_per_launch": nbytes, "avg_ms": ms},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(f"3d_layered_cube_{args.n}^3_ngll5", scatter, dominant),
+                     "traffic_source": "profiles/gather3d_64cube_pmc.json (rocprofv3 FETCH_SIZE, WRITE_SIZE)",
+                     "kernel": dominant, "algorithmic_bytes_per_launch": nbytes, "avg_ms": ms},
         "kernels": per_kernel,
         "kernels_ms": tm,
         "step_hbm_gbs": step_gbs,

@@ -19,12 +19,7 @@ import oracle
 from conftest import rel_l2
 from specfempp_amd import meshes
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="held off the GPU: the one GPU run that included these tests ended in a GPU fault "
-                            "whose cause has not been found (the kernels they use are those of the passing "
-                            "tests, here at 160 k elements and 3000-6000 graph-launched steps)"),
-]
+pytestmark = pytest.mark.gpu
 
 
 def _engine(w, sources, receivers):

@@ -789,7 +789,7 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
                                                            const float *__restrict__ mu_, Gll3 q,
                                                            const float *__restrict__ uf, int np,
                                                            float *__restrict__ re) {
-  __shared__ float su[E3][3][125];
+  __shared__ __align__(16) float su[E3][3][RE_STRIDE];  // displacement, then the forces in slot order
   __shared__ float sF0[E3][3][125];
   __shared__ float sF1[E3][3][125];
 
@@ -867,25 +867,34 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
     }
   }
   __syncthreads();
-  if (!active) return;
-  float *out = re + static_cast<size_t>(e) * RE_ELEM;
+  // element forces into LDS in slot order (the displacement there is dead)
+  if (active) {
 #pragma unroll
-  for (int k = 0; k < 5; k++) {
-    float r0[3] = {0, 0, 0}, r1[3] = {0, 0, 0}, r2[3] = {0, 0, 0};
+    for (int k = 0; k < 5; k++) {
+      float r0[3] = {0, 0, 0}, r1[3] = {0, 0, 0}, r2[3] = {0, 0, 0};
 #pragma unroll
-    for (int l = 0; l < 5; l++) {
+      for (int l = 0; l < 5; l++) {
 #pragma unroll
-      for (int c = 0; c < 3; c++) {
-        r0[c] += sF0[el][c][k * 25 + j * 5 + l] * hwi[l];
-        r1[c] += sF1[el][c][k * 25 + l * 5 + i] * hwj[l];
-        r2[c] += F2[l][c] * q.wz[k][l];
+        for (int c = 0; c < 3; c++) {
+          r0[c] += sF0[el][c][k * 25 + j * 5 + l] * hwi[l];
+          r1[c] += sF1[el][c][k * 25 + l * 5 + i] * hwj[l];
+          r2[c] += F2[l][c] * q.wz[k][l];
+        }
       }
-    }
-    const float wjk = wj * q.w[k], wik = wi * q.w[k], wij = wi * wj;
-    const int slot = re_slot(i, j, k);
+      const float wjk = wj * q.w[k], wik = wi * q.w[k], wij = wi * wj;
+      const int slot = re_slot(i, j, k);
 #pragma unroll
-    for (int c = 0; c < 3; c++) out[c * RE_STRIDE + slot] = -(wjk * r0[c] + wik * r1[c] + wij * r2[c]);
+      for (int c = 0; c < 3; c++) su[el][c][slot] = -(wjk * r0[c] + wik * r1[c] + wij * r2[c]);
+    }
   }
+  __syncthreads();
+  // the workgroup's elements are consecutive in re: one coalesced copy in
+  // 16-byte stores (scattered 4-byte stores cost about half the kernel)
+  const int first = blk * E3;
+  const int nel = nspec - first < E3 ? nspec - first : E3;
+  float4 *dst = reinterpret_cast<float4 *>(re + static_cast<size_t>(first) * RE_ELEM);
+  const float4 *from = reinterpret_cast<const float4 *>(&su[0][0][0]);
+  for (int qv = t; qv < nel * (RE_ELEM / 4); qv += 128) dst[qv] = from[qv];
 }
 
 // Sum, in a fixed order, of the element forces at lattice point

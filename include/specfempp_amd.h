@@ -105,6 +105,9 @@ typedef struct {
   int seismo_types[3];
   int chunk_size;   /* storage chunk size of the numbering (chunk_config.hpp:12-28); 1 = CPU no-SIMD */
   int scatter;      /* weak-form scatter: SFX_SCATTER_COLORED, _ATOMIC, _GATHER or _BLOCK (3-D) */
+  double start_time; /* 0: the earliest source onset (automatic); otherwise the user start time, the
+                      * negated time-scheme "t0" of the configuration (time_scheme.cpp:46-50), which
+                      * keeps every source's own tshift (src/io/sources.cpp:141-155) */
 } sfx_run_desc;
 
 /* Scatter of element forces into the global acceleration.  COLORED groups the
@@ -123,7 +126,9 @@ typedef struct {
  * 2 x 2 x 2 elements, sums their forces in on-chip memory in a fixed order and
  * updates the points inside the block directly; only the forces on block
  * faces pass through device memory to a second, small update kernel
- * (deterministic, no atomics). */
+ * (deterministic, no atomics).  Device engines refuse it for now (a GPU memory
+ * fault in its step-by-step calls is not yet explained); host-only engines
+ * build its layout. */
 #define SFX_SCATTER_BLOCK 3
 
 /* ---- library ---------------------------------------------------------- */

@@ -42,6 +42,7 @@ def load():
     lib.orc2d_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _ip, _ip, C.c_int, _dp, _dp, _dp,
                                  C.c_int, _ip, _ip, C.c_int, C.c_double, C.POINTER(vp)]
     lib.orc2d_destroy.argtypes = [vp]
+    lib.orc2d_set_start_time.argtypes = [vp, C.c_double]
     lib.orc2d_nglob.argtypes = [vp]
     lib.orc2d_numbering.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.orc2d_geometry.argtypes = [vp] + [C.c_void_p] * 6
@@ -175,6 +176,10 @@ class Oracle2D:
         out = [np.zeros((self.nspec, 5, 5)) for _ in range(4)]
         load().orc2d_properties(self.h, *out)
         return tuple(out)
+
+    def set_start_time(self, t: float):
+        """User start time for later runs (0: automatic, the earliest source onset)."""
+        load().orc2d_set_start_time(self.h, float(t))
 
     def set_properties(self, rho, kappa, mu):
         """Replace the per-point properties ((nspec, 5, 5), mesh order) and rebuild the mass matrix."""

@@ -252,6 +252,7 @@ template <class T> struct Assembly {
   std::vector<T> bnd_nx, bnd_nz, bnd_w;
   std::vector<T> mass_inverse;       // (comp, nglob) LayoutLeft on medium index
   double dt = 0;                     // time step of the Stacey mass term
+  double start_time = 0;             // user start time (0: automatic), see make_sources
 };
 
 template <class T> void compute_mass(Assembly<T> &a);
@@ -764,16 +765,24 @@ template <class T> struct Forcing {
 // Forward sources.  t0 = min over sources of (-1.2/f0 + tshift), then each
 // source's tshift becomes its t0 - min_t0 (src/io/sources.cpp:128-158,
 // ricker.cpp:15-20, ricker.hpp:49).
+// A user start time (Assembly::start_time != 0) is used as t0 instead, and
+// the sources keep their own tshift; it must not be later than
+// min t0 - min tshift (src/io/sources.cpp:141-155).
 template <class T>
 Forcing<T> make_sources(const Assembly<T> &a, int nsrc, const SourceIn *src, int nstep, T dt, T *t0_out) {
-  T min_t0 = std::numeric_limits<T>::max();
+  T min_t0 = std::numeric_limits<T>::max(), min_tshift = std::numeric_limits<T>::max();
   std::vector<T> src_t0(nsrc);
   for (int s = 0; s < nsrc; s++) {
     T hdur = 1.0 / static_cast<T>(src[s].f0);
     src_t0[s] = -1.2 * hdur + static_cast<T>(src[s].tshift);
     if (src_t0[s] < min_t0) min_t0 = src_t0[s];
+    if (static_cast<T>(src[s].tshift) < min_tshift) min_tshift = static_cast<T>(src[s].tshift);
   }
-  const T t0 = min_t0;
+  const T user = static_cast<T>(a.start_time);
+  const bool user_defined = std::abs(user) > std::numeric_limits<T>::epsilon();
+  if (user_defined && user > min_t0 - min_tshift)
+    throw std::invalid_argument("start time later than the earliest source onset allows");
+  const T t0 = user_defined ? user : min_t0;
   if (t0_out) *t0_out = t0;
   Forcing<T> f;
   f.nstep = nstep;
@@ -822,7 +831,7 @@ Forcing<T> make_sources(const Assembly<T> &a, int nsrc, const SourceIn *src, int
           arr[NGLL * NGLL + iz * NGLL + ix] = Mxz * dsx + Mzz * dsz;
         }
     }
-    const T tshift = src_t0[s] - min_t0;
+    const T tshift = user_defined ? static_cast<T>(src[s].tshift) : src_t0[s] - min_t0;
     for (int i = 0; i < nstep; i++) {
       const T v = ricker<T>(t0 + i * dt - tshift, static_cast<T>(src[s].f0), static_cast<T>(src[s].factor));
       f.stf[(static_cast<size_t>(s) * nstep + i) * 2] = v;
@@ -1464,6 +1473,10 @@ void orc2d_properties(void *h, double *rho, double *kappa, double *mu, double *w
           wj[dst] = a.q.w[ix] * a.q.w[iz] * a.jac[src];
         }
   });
+}
+
+void orc2d_set_start_time(void *h, double t) {
+  DISPATCH(h, { asm_of<T>(h).start_time = t; });
 }
 
 int orc2d_nstacey(void *h) {

@@ -389,25 +389,33 @@ void stacey_term(const float *n, float w, float rho_vp, float rho_vs, float vcon
 }
 
 void setup_time_function(HostAssembly &a, int nsrc, const sfx_source *src, const sfx_run_desc &run) {
-  // t0 = min over sources of (-1.2 / f0 + tshift); every source's shift
-  // becomes its own t0 - min t0 (src/io/sources.cpp:128-158, ricker.cpp:15-20).
-  float min_t0 = std::numeric_limits<float>::max();
+  // Automatic start: t0 = min over sources of (-1.2 / f0 + tshift), and every
+  // source's shift becomes its own t0 - min t0.  A user start time keeps the
+  // sources' own tshift and must not be later than min t0 - min tshift
+  // (src/io/sources.cpp:128-158, ricker.cpp:15-20).
+  float min_t0 = std::numeric_limits<float>::max(), min_tshift = std::numeric_limits<float>::max();
   std::vector<float> t0s(nsrc);
   for (int s = 0; s < nsrc; s++) {
     if (!(src[s].f0 > 0)) throw std::invalid_argument("source f0 must be positive");
     const float hdur = 1.0f / static_cast<float>(src[s].f0);
     t0s[s] = -1.2 * hdur + static_cast<float>(src[s].tshift);
     min_t0 = std::min(min_t0, t0s[s]);
+    min_tshift = std::min(min_tshift, static_cast<float>(src[s].tshift));
   }
-  if (nsrc == 0) min_t0 = 0.0f;
-  a.t0 = min_t0;
+  if (nsrc == 0) min_t0 = min_tshift = 0.0f;
+  const float user = static_cast<float>(run.start_time);
+  const bool user_defined = std::abs(user) > std::numeric_limits<float>::epsilon();
+  if (user_defined && user > min_t0 - min_tshift)
+    throw std::invalid_argument("start_time is later than the earliest source onset allows");
+  const float t0 = user_defined ? user : min_t0;
+  a.t0 = t0;
   const float dt = static_cast<float>(run.dt);
   a.stf.assign(static_cast<size_t>(run.nstep) * nsrc, 0.0f);
   for (int s = 0; s < nsrc; s++) {
-    const float shift = t0s[s] - min_t0;
+    const float shift = user_defined ? static_cast<float>(src[s].tshift) : t0s[s] - min_t0;
     for (int i = 0; i < run.nstep; i++)
       a.stf[static_cast<size_t>(i) * nsrc + s] =
-          ricker(min_t0 + i * dt - shift, static_cast<float>(src[s].f0), static_cast<float>(src[s].factor));
+          ricker(t0 + i * dt - shift, static_cast<float>(src[s].f0), static_cast<float>(src[s].factor));
   }
 }
 
@@ -419,6 +427,7 @@ void check_run(const sfx_run_desc &run) {
   for (int i = 0; i < run.nseismo_types; i++)
     if (run.seismo_types[i] < 0 || run.seismo_types[i] > 3) throw std::invalid_argument("bad seismogram type");
   if (run.chunk_size < 1) throw std::invalid_argument("chunk_size must be >= 1");
+  if (!std::isfinite(run.start_time)) throw std::invalid_argument("start_time must be finite");
   if (run.scatter != SFX_SCATTER_COLORED && run.scatter != SFX_SCATTER_ATOMIC && run.scatter != SFX_SCATTER_GATHER &&
       run.scatter != SFX_SCATTER_BLOCK)
     throw std::invalid_argument("scatter must be SFX_SCATTER_COLORED, _ATOMIC, _GATHER or _BLOCK");

@@ -69,7 +69,7 @@ class _Receiver(C.Structure):
 class _Run(C.Structure):
     _fields_ = [("dt", C.c_double), ("nstep", C.c_int), ("nstep_between_samples", C.c_int),
                 ("nseismo_types", C.c_int), ("seismo_types", C.c_int * 3), ("chunk_size", C.c_int),
-                ("scatter", C.c_int)]
+                ("scatter", C.c_int), ("start_time", C.c_double)]
 
 
 def _ptr(a):
@@ -110,9 +110,10 @@ SCATTER_COLORED, SCATTER_ATOMIC, SCATTER_GATHER, SCATTER_BLOCK = 0, 1, 2, 3
 _SCATTER_CODES = {"colored": 0, "atomic": 1, "gather": 2, "block": 3}
 
 
-def _run_desc(dt, nstep, nsb, types, chunk, scatter=SCATTER_COLORED):
+def _run_desc(dt, nstep, nsb, types, chunk, scatter=SCATTER_COLORED, start_time=0.0):
     run = _Run()
     run.dt, run.nstep, run.nstep_between_samples, run.chunk_size = dt, nstep, nsb, chunk
+    run.start_time = float(start_time)
     run.scatter = _SCATTER_CODES.get(scatter, scatter)
     run.nseismo_types = len(types)
     for i, t in enumerate(types):
@@ -267,9 +268,12 @@ class Engine2D(_EngineBase):
         return dict(zip(self.KERNEL_NAMES, out))
 
     def __init__(self, mesh, sources, receivers, dt, nstep, nstep_between_samples=1,
-                 seismogram_types=("displacement",), chunk_size=1, scatter="colored", host_only=False):
+                 seismogram_types=("displacement",), chunk_size=1, scatter="colored", host_only=False,
+                 start_time=0.0):
         """host_only: build the host assembly only (numbering, locations; usable
-        by the data-contract getters and ``Batch``), no device memory."""
+        by the data-contract getters and ``Batch``), no device memory.
+        start_time: 0 for the automatic start (earliest source onset), else the
+        user start time (the configuration's time-scheme t0, negated)."""
         self.lib = _native.load()
         self.host_only = host_only
         self.nstep = nstep
@@ -283,7 +287,7 @@ class Engine2D(_EngineBase):
         self._keep = (coorg, knods, kmato, rho, vp, vs, abs_ispec, abs_edge)
         m = _Mesh2D(mesh.nspec, mesh.ngnod, mesh.npgeo, _ptr(coorg), _ptr(knods), _ptr(kmato), len(rho), _ptr(rho),
                     _ptr(vp), _ptr(vs), len(abs_ispec), _ptr(abs_ispec), _ptr(abs_edge))
-        run = _run_desc(dt, nstep, nstep_between_samples, seismogram_types, chunk_size, scatter)
+        run = _run_desc(dt, nstep, nstep_between_samples, seismogram_types, chunk_size, scatter, start_time)
         h = C.c_void_p()
         self.h = None
         create = self.lib.sfx_engine_create_host_2d if host_only else self.lib.sfx_engine_create_2d
@@ -305,7 +309,7 @@ class Engine3D(_EngineBase):
     AUTO_3D = "gather"
 
     def __init__(self, mesh, sources, receivers, dt, nstep, nstep_between_samples=1,
-                 seismogram_types=("displacement",), scatter="auto", host_only=False):
+                 seismogram_types=("displacement",), scatter="auto", host_only=False, start_time=0.0):
         if scatter == "auto":
             even = all(int(mesh[k]) % 2 == 0 for k in ("nx", "ny", "nz"))
             scatter = self.AUTO_3D if (self.AUTO_3D != "block" or even) else "gather"
@@ -327,7 +331,7 @@ class Engine3D(_EngineBase):
         m.rho, m.vp, m.vs = _ptr(rho), _ptr(vp), _ptr(vs)
         for k in range(6):
             m.absorbing[k] = int(mesh["absorbing"][k])
-        run = _run_desc(dt, nstep, nstep_between_samples, seismogram_types, 1, scatter)
+        run = _run_desc(dt, nstep, nstep_between_samples, seismogram_types, 1, scatter, start_time)
         h = C.c_void_p()
         self.h = None
         create = self.lib.sfx_engine_create_host_3d if host_only else self.lib.sfx_engine_create_3d

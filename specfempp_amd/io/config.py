@@ -1,0 +1,198 @@
+"""Run inputs in the reference's formats: ``specfem_config.yaml``,
+``sources.yaml`` and STATIONS, so the reference's cases run unmodified.
+
+* sources (``src/io/sources.cpp:14-158``): a ``sources`` sequence with
+  ``number-of-sources`` entries.  Every entry is a ``force`` (x, z, angle,
+  default 0) or a ``moment-tensor`` (x, z, Mxx, Mzz, Mxz) with a ``Ricker``
+  time function (f0, factor, tshift default 0; ``ricker.cpp:22-37``).  A
+  ``source_surf`` flag is accepted and ignored; non-zero ``vx``/``vz`` (moving
+  sources), ``cosserat-force``, ``user-defined`` and the other time functions
+  (Dirac, dGaussian, External) are not on this engine's path and raise.
+  Adjoint sources are refused in a forward simulation, as the reference
+  refuses them (``sources.cpp:113-117``).
+* receivers (``src/io/receivers.cpp:15-96``): ``stations`` is a STATIONS file
+  (station name, network, x, z, ...: six columns) or a sequence of mappings
+  with ``network``, ``station``, ``x``, ``z``; one ``angle`` for all.
+* configuration (``src/parameter_parser``): time scheme Newmark with ``dt``,
+  ``nstep`` and optional ``t0`` (the start time is ``-t0``,
+  ``time_scheme.cpp:46-50``), quadrature GLL4 (NGLL = 5), receivers
+  (``seismogram-type``, ``nstep_between_samples``, ``angle``), the mesh
+  database and the sources file, the ASCII seismogram writer directory.
+
+Relative paths in a configuration are resolved against ``root`` (default: the
+configuration's directory).  When a file is missing there, the same name in
+the configuration's directory is used, with ``database.bin`` also found as
+``database.bin.gz``; the reference's test configurations name paths relative
+to its test tree and run from copies kept next to them.
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass, field
+
+import yaml
+
+_TYPES = ("displacement", "velocity", "acceleration", "pressure")
+
+
+def _load(node):
+    if isinstance(node, (str, os.PathLike)):
+        with open(node) as fh:
+            return yaml.safe_load(fh)
+    return node
+
+
+def read_sources(node, simulation: str = "forward"):
+    """Sources of a sources file (path) or node as engine source dicts."""
+    doc = _load(node)
+    entries = doc.get("sources")
+    if not isinstance(entries, list) or not entries:
+        raise ValueError("sources: expected a non-empty 'sources' sequence")
+    out, nadjoint = [], 0
+    for entry in entries:
+        if "force" in entry:
+            n = entry["force"]
+            if float(n.get("vx", 0.0)) != 0.0 or float(n.get("vz", 0.0)) != 0.0:
+                raise NotImplementedError("moving force sources (vx, vz != 0) are not supported")
+            src = dict(type="force", x=float(n["x"]), z=float(n["z"]), angle=float(n.get("angle", 0.0)))
+        elif "moment-tensor" in entry:
+            n = entry["moment-tensor"]
+            src = dict(type="moment_tensor", x=float(n["x"]), z=float(n["z"]), mxx=float(n["Mxx"]),
+                       mzz=float(n["Mzz"]), mxz=float(n["Mxz"]))
+        elif "adjoint-source" in entry:
+            nadjoint += 1
+            continue
+        else:
+            kind = next(iter(entry), "?")
+            raise NotImplementedError(f"source type '{kind}' is not supported (force and moment-tensor are)")
+        if "Ricker" not in n:
+            stf = [k for k in ("Dirac", "dGaussian", "External") if k in n]
+            raise NotImplementedError(f"time function {stf or '?'} is not supported (Ricker is)")
+        r = n["Ricker"]
+        src.update(f0=float(r["f0"]), factor=float(r["factor"]), tshift=float(r.get("tshift", 0.0)))
+        out.append(src)
+    if simulation == "forward" and nadjoint:
+        raise ValueError("adjoint sources found in the sources file of a forward simulation")
+    if not out:
+        raise ValueError("no sources found in the sources file")
+    expected = int(doc.get("number-of-sources", len(out) + nadjoint))
+    if expected != len(out) + nadjoint:
+        raise ValueError(f"found {len(out) + nadjoint} sources, number-of-sources says {expected}")
+    return out
+
+
+def read_stations_file(path: str, angle: float = 0.0):
+    """STATIONS file rows: station name, network, x, z, (two more columns)."""
+    out = []
+    with open(path) as fh:
+        for line in fh:
+            parts = line.split()
+            if not parts:
+                continue
+            if len(parts) != 6:
+                raise ValueError(f"{path}: a STATIONS line has 6 columns, got {len(parts)}: {line!r}")
+            out.append(dict(name=parts[0], network=parts[1], x=float(parts[2]), z=float(parts[3]),
+                            angle=float(angle)))
+    return out
+
+
+def read_receivers(stations, angle: float = 0.0):
+    """Receivers of a STATIONS path or a sequence of {network, station, x, z}."""
+    if isinstance(stations, (str, os.PathLike)):
+        return read_stations_file(stations, angle)
+    if not isinstance(stations, list) or not stations:
+        raise ValueError("stations: expected a STATIONS file or a non-empty sequence")
+    return [dict(name=str(s["station"]), network=str(s["network"]), x=float(s["x"]), z=float(s["z"]),
+                 angle=float(angle)) for s in stations]
+
+
+@dataclass
+class RunConfig:
+    dt: float
+    nstep: int
+    start_time: float                 # 0: automatic
+    simulation: str
+    seismogram_types: list
+    nstep_between_samples: int
+    angle: float
+    database: str
+    sources: list
+    receivers: list
+    writer_directory: str | None = None
+    extra: dict = field(default_factory=dict)
+
+
+def _resolve(path: str, root: str, here: str) -> str:
+    cand = path if os.path.isabs(path) else os.path.join(root, path)
+    if os.path.exists(cand):
+        return cand
+    local = os.path.join(here, os.path.basename(path))
+    for p in (local, local + ".gz"):
+        if os.path.exists(p):
+            return p
+    raise FileNotFoundError(f"{path} (looked in {root} and {here})")
+
+
+def read_config(path: str, root: str | None = None) -> RunConfig:
+    """Parse a ``specfem_config.yaml`` into everything a run needs."""
+    here = os.path.dirname(os.path.abspath(path))
+    root = here if root is None else root
+    with open(path) as fh:
+        doc = yaml.safe_load(fh)["parameters"]
+    setup = doc["simulation-setup"]
+    quad = setup.get("quadrature", {}).get("quadrature-type", "GLL4")
+    if quad != "GLL4":
+        raise NotImplementedError(f"quadrature {quad}: the engine is NGLL = 5 (GLL4)")
+    march = setup["solver"]["time-marching"]
+    simulation = march.get("type-of-simulation", "forward")
+    scheme = march["time-scheme"]
+    if scheme.get("type", "Newmark") != "Newmark":
+        raise NotImplementedError(f"time scheme {scheme.get('type')}: only Newmark")
+    rec = doc["receivers"]
+    types = [str(t) for t in rec.get("seismogram-type", ["displacement"])]
+    for t in types:
+        if t not in _TYPES:
+            raise ValueError(f"seismogram type {t!r} (expected one of {_TYPES})")
+    angle = float(rec.get("angle", 0.0))
+    stations = rec["stations"]
+    if isinstance(stations, str):
+        stations = _resolve(stations, root, here)
+    srcs = doc["sources"]
+    if isinstance(srcs, str):
+        srcs = _resolve(srcs, root, here)
+    writer = (setup.get("simulation-mode", {}).get("forward", {}).get("writer", {}).get("seismogram"))
+    return RunConfig(
+        dt=float(scheme["dt"]), nstep=int(scheme["nstep"]),
+        start_time=-float(scheme["t0"]) if "t0" in scheme else 0.0,
+        simulation=simulation, seismogram_types=types,
+        nstep_between_samples=int(rec.get("nstep_between_samples", 1)), angle=angle,
+        database=_resolve(doc["databases"]["mesh-database"], root, here),
+        sources=read_sources(srcs, simulation), receivers=read_receivers(stations, angle),
+        writer_directory=(writer or {}).get("directory"),
+        extra={"title": doc.get("header", {}).get("title")})
+
+
+def run_config(path: str, root: str | None = None, out_dir: str | None = None, scatter: str = "colored"):
+    """Run a 2-D forward configuration on the engine; write the ASCII
+    seismograms to out_dir when given.  Returns (seismograms, t0, config)."""
+    from ..engine import Engine2D
+    from .fortran_db import read_database_2d
+    from .seismograms import write_ascii
+
+    cfg = read_config(path, root)
+    if cfg.simulation != "forward":
+        raise NotImplementedError("configurations of type-of-simulation forward only (adjoint runs use the API)")
+    mesh = read_database_2d(cfg.database)
+    eng = Engine2D(mesh, cfg.sources, cfg.receivers, cfg.dt, cfg.nstep, cfg.nstep_between_samples,
+                   cfg.seismogram_types, scatter=scatter, start_time=cfg.start_time)
+    try:
+        eng.initialize()
+        eng.run()
+        seis = eng.seismograms()
+        t0 = eng.t0
+    finally:
+        eng.close()
+    if out_dir:
+        write_ascii(out_dir, [(r["name"], r["network"], r["angle"]) for r in cfg.receivers], seis,
+                    cfg.seismogram_types, t0, cfg.dt, cfg.nstep_between_samples)
+    return seis, t0, cfg

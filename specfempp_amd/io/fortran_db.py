@@ -112,9 +112,16 @@ _HEADER_TAIL = [
 
 
 def read_database_2d(path: str) -> Mesh2D:
-    """Parse a 2-D ``database.bin`` (``src/io/mesh/impl/fortran/dim2/mesh.cpp:26-193``)."""
-    with open(path, "rb") as fh:
-        r = _Records(fh.read())
+    """Parse a 2-D ``database.bin`` (``src/io/mesh/impl/fortran/dim2/mesh.cpp:26-193``);
+    a path ending in ``.gz`` is read through gzip."""
+    if str(path).endswith(".gz"):
+        import gzip
+
+        with gzip.open(path, "rb") as fh:
+            r = _Records(fh.read())
+    else:
+        with open(path, "rb") as fh:
+            r = _Records(fh.read())
 
     vals = []
     for fmt in _HEADER:

@@ -6,7 +6,8 @@ Run once in a container that has the reference checkout:
     python tests/golden/make_fixtures.py /root/reference
 
 Only data files are taken (mesh databases, SPECFEM2D traces, index mapping and
-mass-matrix arrays, station and source parameters).  They come from
+mass-matrix arrays, station and source parameters, and the text run inputs:
+specfem_config.yaml, sources.yaml, STATIONS and the source-reader test data).  They come from
 PrincetonUniversity/SPECFEMPP ``tests/unit-tests`` (GPL-3.0) and are stored
 re-encoded (gzip / npz) with their provenance recorded in ``MANIFEST.json``.
 """
@@ -15,6 +16,7 @@ from __future__ import annotations
 import gzip
 import json
 import os
+import shutil
 import sys
 
 import numpy as np
@@ -71,6 +73,9 @@ def main(ref: str) -> None:
             arr = np.asarray(rows)  # (nrec, 2, nsamples)
             traces[k] = arr
         np.savez_compressed(os.path.join(out, "traces.npz"), time=time, **traces)
+        # the run inputs as the reference's test holds them (text, unchanged)
+        for fname in ("specfem_config.yaml", "sources.yaml", "STATIONS"):
+            shutil.copyfile(os.path.join(src_dir, fname), os.path.join(out, fname))
         with open(os.path.join(out, "case.json"), "w") as fh:
             json.dump({
                 "source_dir": f"{UT}/{rel}",
@@ -94,6 +99,14 @@ def main(ref: str) -> None:
                    "index_mapping_layout": "(nspec, ngllz, ngllx) row-major, 0-based, mesh order",
                    "rmass_layout": "(nglob, 2) row-major"}, fh, indent=1)
     manifest["mass_80x60"] = f"{UT}/{rel}"
+
+    # source-file reader data (tests/unit-tests/io/sources)
+    rel = "io/sources"
+    out = os.path.join(HERE, "io_sources")
+    os.makedirs(out, exist_ok=True)
+    for fname in ("single_force.yaml", "single_moment_tensor.yaml", "single_cosserat_force.yaml"):
+        shutil.copyfile(os.path.join(ref, UT, rel, "data", fname), os.path.join(out, fname))
+    manifest["io_sources"] = f"{UT}/{rel}/data"
 
     with open(os.path.join(HERE, "MANIFEST.json"), "w") as fh:
         json.dump({"origin": "PrincetonUniversity/SPECFEMPP tests/unit-tests (GPL-3.0)",

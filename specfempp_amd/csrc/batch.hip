@@ -212,6 +212,12 @@ struct sfx_batch {
     for (int k = 0; k < nsim; k++) {
       H[k] = &engine_host(engines[order[k]]);
       R[k] = &engine_run(engines[order[k]]);
+      // every element must sit in exactly one color slot (the packing below
+      // maps elements through the coloring)
+      const HostAssembly &h = *H[k];
+      if (h.color_offsets.size() < 2 || h.color_offsets.front() != 0 || h.color_offsets.back() != h.nspec ||
+          static_cast<int>(h.perm.size()) != h.nspec)
+        throw std::logic_error("batch: an engine's element coloring is incomplete");
     }
     // quadrature constants (the same GLL rule for every engine)
     for (int i = 0; i < 5; i++) {
@@ -275,6 +281,9 @@ struct sfx_batch {
     }
     color_off[ncolors] = static_cast<int>(elem_sim.size());
     const size_t nel = elem_sim.size();
+    for (int k = 0; k < nsim; k++)
+      for (int b : bidx[k])
+        if (b < 0) throw std::logic_error("batch: an element was not packed");
     std::vector<int> g_h(nel * 25);
     std::vector<float> xix_h(nel * 25), xiz_h(nel * 25), gx_h(nel * 25), gz_h(nel * 25), j_h(nel * 25), k_h(nel * 25),
         m_h(nel * 25);

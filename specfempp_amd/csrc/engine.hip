@@ -210,7 +210,6 @@ struct sfx_engine {
   }
 
   void upload() {
-    color_elements(h, run.scatter);
     lat = Lattice{h.nx, h.ny, h.nz, h.nx + 1, h.ny + 1, h.nz + 1, h.nint_pad};
     SFX_HIP(hipStreamCreate(&stream));
     hipStream_t s = stream;
@@ -259,8 +258,6 @@ struct sfx_engine {
     rec_h.upload(h.rec_h, s);
     std::vector<int> t(run.seismo_types, run.seismo_types + run.nseismo_types);
     types.upload(t, s);
-    const int nsb = run.nstep_between_samples;
-    nsig = (run.nstep + nsb - 1) / nsb;
     seismo.alloc(static_cast<size_t>(nsig) * run.nseismo_types * h.nrec * h.ncomp);
     if (seismo.n) launch_fill(s, seismo.p, seismo.n, 0.0f);
     const size_t stride = static_cast<size_t>(h.ncomp) * h.nglob_pad;
@@ -557,6 +554,10 @@ template <class Build> int create(Build &&build, const sfx_run_desc *run, sfx_en
     auto e = std::make_unique<sfx_engine>();
     e->run = *run;
     e->h = build();
+    // the element coloring and the sample count are host data: host-only
+    // engines carry them too
+    color_elements(e->h, run->scatter);
+    e->nsig = (run->nstep + run->nstep_between_samples - 1) / run->nstep_between_samples;
     e->host_only = host_only;
     if (!host_only) e->upload();
     *out = e.release();

@@ -80,3 +80,20 @@ def test_batch_matches_oracle_and_tuning_keeps_results(alone):
         assert err <= 1e-5
         assert rel_l2(got, ref[i]) <= 1e-6
     b.close()
+
+
+def test_batch_of_host_only_engines(alone):
+    """A batch built from host-only engines (no device memory of their own)
+    gives the same seismograms."""
+    from specfempp_amd import Batch, Engine2D
+
+    ws, ref = alone
+    engines = [Engine2D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, 1, TYPES, host_only=True) for w in ws]
+    b = Batch(engines)
+    for e in engines:
+        e.close()
+    b.set_config(5, True)
+    b.run()
+    for i in range(len(ws)):
+        assert rel_l2(b.seismograms(i).astype(np.float64), ref[i]) <= 1e-6
+    b.close()

@@ -97,3 +97,16 @@ def test_3d_block_path_is_off_on_the_device_and_host_assembly_still_builds():
     eng = Engine3D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, scatter="block", host_only=True)
     assert np.array_equal(eng.index_mapping(), oracle.Oracle3D(w.mesh, 8, w.dt).ibool())
     eng.close()
+
+
+def test_host_only_engine_reports_the_run_sizes():
+    """Host-only engines carry every host-side quantity a batch packs: the
+    sample count and the element coloring (a batch of host-only engines
+    depends on both)."""
+    from specfempp_amd import Engine2D
+
+    mesh, cfg = load_case("elastic_free_80x60")
+    e = Engine2D(mesh, cfg["sources"], cfg["stations"], cfg["dt"], 37, 4, ("displacement", "velocity"),
+                 host_only=True)
+    assert e.nsig == 10 and e.nrec == len(cfg["stations"]) and e.nsrc == len(cfg["sources"])
+    e.close()

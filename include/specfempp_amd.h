@@ -102,7 +102,7 @@ typedef struct {
   int nstep;
   int nstep_between_samples;
   int nseismo_types;
-  int seismo_types[3];
+  int seismo_types[4]; /* up to the four kinds below (the reference records any subset) */
   int chunk_size;   /* storage chunk size of the numbering (chunk_config.hpp:12-28); 1 = CPU no-SIMD */
   int scatter;      /* weak-form scatter: SFX_SCATTER_COLORED, _ATOMIC, _GATHER or _BLOCK (3-D) */
   double start_time; /* 0: the earliest source onset (automatic); otherwise the user start time, the

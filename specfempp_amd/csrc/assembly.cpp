@@ -423,7 +423,7 @@ void check_run(const sfx_run_desc &run) {
   if (!(run.dt > 0)) throw std::invalid_argument("dt must be positive");
   if (run.nstep < 0) throw std::invalid_argument("nstep must be >= 0");
   if (run.nstep_between_samples < 1) throw std::invalid_argument("nstep_between_samples must be >= 1");
-  if (run.nseismo_types < 0 || run.nseismo_types > 3) throw std::invalid_argument("nseismo_types must be 0..3");
+  if (run.nseismo_types < 0 || run.nseismo_types > 4) throw std::invalid_argument("nseismo_types must be 0..4");
   for (int i = 0; i < run.nseismo_types; i++)
     if (run.seismo_types[i] < 0 || run.seismo_types[i] > 3) throw std::invalid_argument("bad seismogram type");
   if (run.chunk_size < 1) throw std::invalid_argument("chunk_size must be >= 1");

@@ -68,7 +68,7 @@ class _Receiver(C.Structure):
 
 class _Run(C.Structure):
     _fields_ = [("dt", C.c_double), ("nstep", C.c_int), ("nstep_between_samples", C.c_int),
-                ("nseismo_types", C.c_int), ("seismo_types", C.c_int * 3), ("chunk_size", C.c_int),
+                ("nseismo_types", C.c_int), ("seismo_types", C.c_int * 4), ("chunk_size", C.c_int),
                 ("scatter", C.c_int), ("start_time", C.c_double)]
 
 
@@ -115,6 +115,8 @@ def _run_desc(dt, nstep, nsb, types, chunk, scatter=SCATTER_COLORED, start_time=
     run.dt, run.nstep, run.nstep_between_samples, run.chunk_size = dt, nstep, nsb, chunk
     run.start_time = float(start_time)
     run.scatter = _SCATTER_CODES.get(scatter, scatter)
+    if len(types) > 4:
+        raise ValueError(f"at most 4 seismogram types, got {len(types)}")
     run.nseismo_types = len(types)
     for i, t in enumerate(types):
         run.seismo_types[i] = _TYPE_CODES[t] if isinstance(t, str) else int(t)

@@ -813,6 +813,15 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
   }
   const float wi = q.w[i], wj = q.w[j];
 
+  // material first, so its latency overlaps the displacement loads
+  float kap[5], muv[5];
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      kap[k] = kappa_[base + k * 25];
+      muv[k] = mu_[base + k * 25];
+    }
+  }
   float u[5][3];
   if (active) {
     const int ex = e % lat.nx, ey = (e / lat.nx) % lat.ny, ez = e / (lat.nx * lat.ny);
@@ -843,8 +852,8 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
           dz[c] += q.hz[k][l] * u[l][c];
         }
       }
-      const float kappa = kappa_[base + k * 25];
-      const float mu = mu_[base + k * 25];
+      const float kappa = kap[k];
+      const float mu = muv[k];
       const float lp2m = kappa + (4.0f / 3.0f) * mu;
       const float lambda = lp2m - 2.0f * mu;
       const float exx = dx[0], eyy = dy[1], ezz = dz[2];

@@ -204,18 +204,23 @@ def adjoint_leg(snapshot_every: int = 55) -> dict:
             "misfit": misfit, "snapshot_every": snapshot_every, "kernels_finite": finite}
 
 
-def algorithmic_bytes_2d(nspec: int, nglob: int) -> dict:
+def algorithmic_bytes_2d(nspec: int, nglob: int, per_element_geometry: bool) -> dict:
     """Bytes each kernel of the 2-D path must move per step (FP32):
-      stiffness_2d (all colors): gather index, xix, xiz, gammax, gammaz, J,
-                   kappa, mu per local point (32 B x 25 / element), the
-                   displacement gathered once per point (8 B), the acceleration
-                   read-modify-written once per point (16 B)
+      stiffness_2d (all colors): gather index, kappa, mu per local point
+                   (12 B x 25 / element) and the geometry: xix, xiz, gammax,
+                   gammaz, J per local point (20 B x 25 / element), or one
+                   32-byte record per element on affine meshes; the
+                   displacement gathered once per point (8 B), the
+                   acceleration read-modify-written once per point (16 B)
       update_fused: a, M^-1, u, v read (32 B) and u, v, a written (24 B) per point
-    (the data-model figure of 130 B per point-step for 25/16 local points per
-    point)."""
-    stiff = nspec * 25 * 32 + nglob * 24
+    With per-point geometry this is the data-model figure of 130 B per
+    point-step (25/16 local points per point); model_per_pt_step keeps it."""
+    geo = nspec * 32 if per_element_geometry else nspec * 25 * 20
+    stiff = nspec * 25 * 12 + geo + nglob * 24
     upd = nglob * 56
-    return {"stiffness_2d": stiff, "update_fused": upd, "per_pt_step": (stiff + upd) / nglob}
+    model = nspec * 25 * 32 + nglob * 24 + upd
+    return {"stiffness_2d": stiff, "update_fused": upd, "per_pt_step": (stiff + upd) / nglob,
+            "model_per_pt_step": model / nglob}
 
 
 def config2_leg(n: int, steps: int, warmup: int, sample_steps: int = 5) -> dict:
@@ -253,7 +258,9 @@ def config2_leg(n: int, steps: int, warmup: int, sample_steps: int = 5) -> dict:
     finite = bool(np.isfinite(eng.seismograms()).all())
     eng.close()
     value = nglob * steps / elapsed
-    ab = algorithmic_bytes_2d(nspec, nglob)
+    # the workload's elements are rectangles: the engine reads one geometry
+    # record per element
+    ab = algorithmic_bytes_2d(nspec, nglob, per_element_geometry=True)
     kern = {"stiffness_2d_kernel": (ab["stiffness_2d"], acc["stiffness_ms"]),
             "update_fused_kernel": (ab["update_fused"], acc["update_ms"])}
     dominant = max(kern, key=lambda k: kern[k][1])
@@ -274,7 +281,10 @@ def config2_leg(n: int, steps: int, warmup: int, sample_steps: int = 5) -> dict:
         "kernels": {k: {"avg_ms": v[1], "algorithmic_bytes": v[0],
                         "GBs": (v[0] / (v[1] * 1e-3) / 1e9 if v[1] > 0 else 0.0)} for k, v in kern.items()},
         "kernels_ms": acc, "step_hbm_gbs": step_gbs, "step_hbm_frac": step_gbs / HBM_PEAK_GBS,
-        "algorithmic_bytes_per_pt_step": ab["per_pt_step"], "seismograms_finite": finite,
+        "algorithmic_bytes_per_pt_step": ab["per_pt_step"],
+        "model_bytes_per_pt_step": ab["model_per_pt_step"],
+        "model_step_hbm_frac": ab["model_per_pt_step"] * value / 1e9 / HBM_PEAK_GBS,
+        "geometry": "one record per element (affine elements)", "seismograms_finite": finite,
     }
 
 

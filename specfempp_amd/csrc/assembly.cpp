@@ -584,6 +584,29 @@ HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *
         a.gammaz[p] = xxi / J;
         a.jac[p] = J;
       }
+  // constant geometry per element (bitwise) -> one record per element
+  {
+    bool regular = true;
+    for (int e = 0; e < nspec && regular; e++) {
+      const size_t b = static_cast<size_t>(e) * npe;
+      for (int p = 1; p < npe && regular; p++)
+        regular = a.xix[b + p] == a.xix[b] && a.xiz[b + p] == a.xiz[b] && a.gammax[b + p] == a.gammax[b] &&
+                  a.gammaz[b + p] == a.gammaz[b] && a.jac[b + p] == a.jac[b];
+    }
+    a.geo2d.clear();
+    if (regular) {
+      a.geo2d.assign(static_cast<size_t>(nspec) * 8, 0.0f);
+      for (int e = 0; e < nspec; e++) {
+        const size_t b = static_cast<size_t>(e) * npe;
+        float *g = &a.geo2d[static_cast<size_t>(e) * 8];
+        g[0] = a.xix[b];
+        g[1] = a.xiz[b];
+        g[2] = a.gammax[b];
+        g[3] = a.gammaz[b];
+        g[4] = a.jac[b];
+      }
+    }
+  }
 
   // Properties (include/medium/dim2/elastic/isotropic/material.hpp:62-77)
   a.kappa.resize(nloc);

@@ -38,6 +38,10 @@ struct HostAssembly {
   float xi[5], w[5], hprime[5][5];
   // 2-D geometry per local point (compute_partial_derivatives.cpp:46-91)
   std::vector<float> xix, xiz, gammax, gammaz, jac;
+  // 2-D: when every element's geometry is the same at all of its points
+  // (affine elements), one record per element (xix, xiz, gammax, gammaz, J,
+  // padding to 8 floats) in compute order; empty otherwise
+  std::vector<float> geo2d;
   // 3-D regular geometry (all elements are boxes of the same size)
   float xix3 = 0, ety3 = 0, gaz3 = 0, jac3 = 0;
   // properties per local point (kappa, mu; rho for mass and Stacey)

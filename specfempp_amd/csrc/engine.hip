@@ -40,6 +40,7 @@ struct sfx_engine {
   // device arrays
   DevBuf<int> gather, bnd_gather, bnd_slot, bnd_seg, src_order, src_ispec, rec_ispec, types;
   DevBuf<float> xix, xiz, gammax, gammaz, jac, kappa, mu;
+  DevBuf<float> geo;     // 2-D affine meshes: one geometry record per element (the forward stiffness reads it)
   DevBuf<float> bnd_normal, bnd_weight, bnd_rvp, bnd_rvs;
   DevBuf<float> src_array, stf, rec_h, seismo;
   DevBuf<float> fields;  // u, v, a, minv: 4 * ncomp * nglob_pad
@@ -130,7 +131,7 @@ struct sfx_engine {
       const int first = h.color_offsets[c], count = h.color_offsets[c + 1] - first;
       launch_stiffness_2d(stream, first, count, atomic, gather.p, xix.p, xiz.p, gammax.p, gammaz.p, jac.p, kappa.p,
                           mu.p, q, ff, record_on ? 1 : 0, rec_index.p, record_on ? record.p : nullptr, d_step.p,
-                          rstride);
+                          rstride, geo.p);
     }
     launch_stacey(stream, 2, static_cast<int>(h.bnd_gather.size()), bnd_gather.p, bnd_normal.p, bnd_weight.p,
                   bnd_rvp.p, bnd_rvs.p, ff, bnd_rec.p, record_on ? record.p : nullptr, d_step.p, rstride);
@@ -142,8 +143,8 @@ struct sfx_engine {
   }
 
   // Per-element array in device slot order (colors contiguous).
-  template <class T> std::vector<T> permuted(const std::vector<T> &v) const {
-    const size_t npe = h.npe;
+  template <class T> std::vector<T> permuted(const std::vector<T> &v, size_t width = 0) const {
+    const size_t npe = width ? width : h.npe;
     std::vector<T> out(v.size());
     for (int slot = 0; slot < h.nspec; slot++)
       std::memcpy(&out[slot * npe], &v[static_cast<size_t>(h.perm[slot]) * npe], npe * sizeof(T));
@@ -222,6 +223,7 @@ struct sfx_engine {
       gammax.upload(permuted(h.gammax), s);
       gammaz.upload(permuted(h.gammaz), s);
       jac.upload(permuted(h.jac), s);
+      if (!h.geo2d.empty()) geo.upload(permuted(h.geo2d, 8), s);
       rho.upload(permuted(h.rho), s);
       // Stacey elements are the last ones in compute order
       nstacey = h.nspec - h.nspec_none;
@@ -318,7 +320,7 @@ struct sfx_engine {
       const int first = h.color_offsets[c], count = h.color_offsets[c + 1] - first;
       if (h.dim == 2)
         launch_stiffness_2d(stream, first, count, atomic, gather.p, xix.p, xiz.p, gammax.p, gammaz.p, jac.p, kappa.p,
-                            mu.p, q, ff, rec_mode, rec_index.p, rec_step);
+                            mu.p, q, ff, rec_mode, rec_index.p, rec_step, nullptr, 0, geo.p);
       else
         launch_stiffness_3d(stream, first, count, atomic, gather.p, kappa.p, mu.p, h.xix3, h.ety3, h.gaz3, h.jac3, q,
                             ff);

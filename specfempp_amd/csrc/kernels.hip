@@ -48,7 +48,8 @@ __global__ __launch_bounds__(threads_for(E)) void stiffness_2d_kernel(
     int first, int count, const int *__restrict__ gather, const float *__restrict__ xix_, const float *__restrict__ xiz_,
     const float *__restrict__ gammax_, const float *__restrict__ gammaz_, const float *__restrict__ jac_,
     const float *__restrict__ kappa_, const float *__restrict__ mu_, GllConst q, Fields f, int rec_mode,
-    const int *__restrict__ rec_index, float *__restrict__ rec, const int *__restrict__ step, long long rec_stride) {
+    const int *__restrict__ rec_index, float *__restrict__ rec, const int *__restrict__ step, long long rec_stride,
+    const float *__restrict__ geo) {
   __shared__ float su[E][2][25];
   __shared__ float sF[E][4][25];  // F(c, d) as [2c + d]
 
@@ -82,11 +83,20 @@ __global__ __launch_bounds__(threads_for(E)) void stiffness_2d_kernel(
   const float wx = q.w[ix], wz = q.w[iz];
   float xix = 0, xiz = 0, gammax = 0, gammaz = 0, J = 0, kappa = 0, mu = 0;
   if (active) {
-    xix = xix_[idx];
-    xiz = xiz_[idx];
-    gammax = gammax_[idx];
-    gammaz = gammaz_[idx];
-    J = jac_[idx];
+    if (geo) {  // every element's geometry is constant over its points: one record per element
+      const float *ge = geo + static_cast<size_t>(e) * 8;
+      xix = ge[0];
+      xiz = ge[1];
+      gammax = ge[2];
+      gammaz = ge[3];
+      J = ge[4];
+    } else {
+      xix = xix_[idx];
+      xiz = xiz_[idx];
+      gammax = gammax_[idx];
+      gammaz = gammaz_[idx];
+      J = jac_[idx];
+    }
     kappa = kappa_[idx];
     mu = mu_[idx];
     float dxi[2] = {0.0f, 0.0f}, dga[2] = {0.0f, 0.0f};
@@ -1155,14 +1165,14 @@ inline unsigned blocks(size_t n, unsigned per) { return static_cast<unsigned>((n
 void launch_stiffness_2d(hipStream_t s, int first, int count, bool atomic, const int *gather, const float *xix,
                          const float *xiz, const float *gammax, const float *gammaz, const float *jac,
                          const float *kappa, const float *mu, const GllConst &q, Fields f, int rec_mode,
-                         const int *rec_index, float *rec, const int *step, long long rec_stride) {
+                         const int *rec_index, float *rec, const int *step, long long rec_stride, const float *geo) {
   if (count <= 0) return;
   if (atomic)
     hipLaunchKernelGGL(stiffness_2d_kernel<true>, dim3(blocks(count, E2)), dim3(128), 0, s, first, count, gather, xix,
-                       xiz, gammax, gammaz, jac, kappa, mu, q, f, rec_mode, rec_index, rec, step, rec_stride);
+                       xiz, gammax, gammaz, jac, kappa, mu, q, f, rec_mode, rec_index, rec, step, rec_stride, geo);
   else
     hipLaunchKernelGGL(stiffness_2d_kernel<false>, dim3(blocks(count, E2)), dim3(128), 0, s, first, count, gather, xix,
-                       xiz, gammax, gammaz, jac, kappa, mu, q, f, rec_mode, rec_index, rec, step, rec_stride);
+                       xiz, gammax, gammaz, jac, kappa, mu, q, f, rec_mode, rec_index, rec, step, rec_stride, geo);
 }
 
 template <int E>
@@ -1172,10 +1182,12 @@ void launch_stiffness_2d_e(hipStream_t s, int first, int count, bool atomic, con
   if (count <= 0) return;
   if (atomic)
     hipLaunchKernelGGL((stiffness_2d_kernel<true, E>), dim3(blocks(count, E)), dim3(threads_for(E)), 0, s, first,
-                       count, gather, xix, xiz, gammax, gammaz, jac, kappa, mu, q, f, 0, nullptr, nullptr, nullptr, 0LL);
+                       count, gather, xix, xiz, gammax, gammaz, jac, kappa, mu, q, f, 0, nullptr, nullptr, nullptr, 0LL,
+                       nullptr);
   else
     hipLaunchKernelGGL((stiffness_2d_kernel<false, E>), dim3(blocks(count, E)), dim3(threads_for(E)), 0, s, first,
-                       count, gather, xix, xiz, gammax, gammaz, jac, kappa, mu, q, f, 0, nullptr, nullptr, nullptr, 0LL);
+                       count, gather, xix, xiz, gammax, gammaz, jac, kappa, mu, q, f, 0, nullptr, nullptr, nullptr, 0LL,
+                       nullptr);
 }
 
 void launch_stiffness_2d_tuned(hipStream_t s, int elems_per_block, int first, int count, bool atomic,

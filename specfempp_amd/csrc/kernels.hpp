@@ -48,7 +48,7 @@ void launch_stiffness_2d(hipStream_t s, int first, int count, bool atomic, const
                          const float *xiz, const float *gammax, const float *gammaz, const float *jac,
                          const float *kappa, const float *mu, const GllConst &q, Fields f, int rec_mode = 0,
                          const int *rec_index = nullptr, float *rec = nullptr, const int *step = nullptr,
-                         long long rec_stride = 0);
+                         long long rec_stride = 0, const float *geo = nullptr);
 // Plain 2-D stiffness with a chosen number of elements per workgroup (2, 5 or
 // 10: 64, 128 or 256 threads); the batch autotuner picks it.
 void launch_stiffness_2d_tuned(hipStream_t s, int elems_per_block, int first, int count, bool atomic,

@@ -204,7 +204,7 @@ def adjoint_leg(snapshot_every: int = 55) -> dict:
             "misfit": misfit, "snapshot_every": snapshot_every, "kernels_finite": finite}
 
 
-def algorithmic_bytes_2d(nspec: int, nglob: int, per_element_geometry: bool) -> dict:
+def algorithmic_bytes_2d(nspec: int, nglob: int, per_element_geometry: bool, scatter: str = "colored") -> dict:
     """Bytes each kernel of the 2-D path must move per step (FP32):
       stiffness_2d (all colors): gather index, kappa, mu per local point
                    (12 B x 25 / element) and the geometry: xix, xiz, gammax,
@@ -214,16 +214,26 @@ def algorithmic_bytes_2d(nspec: int, nglob: int, per_element_geometry: bool) -> 
                    acceleration read-modify-written once per point (16 B)
       update_fused: a, M^-1, u, v read (32 B) and u, v, a written (24 B) per point
     With per-point geometry this is the data-model figure of 130 B per
-    point-step (25/16 local points per point); model_per_pt_step keeps it."""
+    point-step (25/16 local points per point); model_per_pt_step keeps it.
+    The gather path instead stores the element forces (8 B per local point)
+    and its update reads them once with the inverse index (4 B per local
+    point and 4 B per point), and u, v, M^-1 (24 B), writes u, v, a (24 B):
+      elemforce_2d: gather index, kappa, mu per local point, geometry, the
+                    displacement (8 B per point), the forces written
+      assemble_update_2d: index and forces read, the update as above."""
     geo = nspec * 32 if per_element_geometry else nspec * 25 * 20
-    stiff = nspec * 25 * 12 + geo + nglob * 24
-    upd = nglob * 56
+    if scatter == "gather":
+        stiff = nspec * 25 * (12 + 8) + geo + nglob * 8
+        upd = nspec * 25 * (8 + 4) + nglob * (4 + 24 + 24)
+    else:
+        stiff = nspec * 25 * 12 + geo + nglob * 24
+        upd = nglob * 56
     model = nspec * 25 * 32 + nglob * 24 + upd
     return {"stiffness_2d": stiff, "update_fused": upd, "per_pt_step": (stiff + upd) / nglob,
             "model_per_pt_step": model / nglob}
 
 
-def config2_leg(n: int, steps: int, warmup: int, sample_steps: int = 5) -> dict:
+def config2_leg(n: int, steps: int, warmup: int, sample_steps: int = 5, scatter: str = "gather") -> dict:
     """Config 2: the heterogeneous 2-D workload at n^2 quads of 2.5 m (full
     size n = 4000).  Graph-launched steps are timed for the throughput; a few
     single steps give the per-kernel times for the roofline."""
@@ -237,7 +247,7 @@ def config2_leg(n: int, steps: int, warmup: int, sample_steps: int = 5) -> dict:
     mesh_s = time.perf_counter() - t
     print(f"[config2] mesh {n}^2 built in {mesh_s:.1f} s", file=sys.stderr, flush=True)
     t = time.perf_counter()
-    eng = Engine2D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, 1, ("displacement",))
+    eng = Engine2D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, 1, ("displacement",), scatter=scatter)
     setup_s = time.perf_counter() - t
     print(f"[config2] engine ({eng.nglob} points) built in {setup_s:.1f} s", file=sys.stderr, flush=True)
     nglob, nspec = eng.nglob, eng.nspec
@@ -260,9 +270,13 @@ def config2_leg(n: int, steps: int, warmup: int, sample_steps: int = 5) -> dict:
     value = nglob * steps / elapsed
     # the workload's elements are rectangles: the engine reads one geometry
     # record per element
-    ab = algorithmic_bytes_2d(nspec, nglob, per_element_geometry=True)
-    kern = {"stiffness_2d_kernel": (ab["stiffness_2d"], acc["stiffness_ms"]),
-            "update_fused_kernel": (ab["update_fused"], acc["update_ms"])}
+    ab = algorithmic_bytes_2d(nspec, nglob, per_element_geometry=True, scatter=scatter)
+    if scatter == "gather":
+        kern = {"elemforce_2d_kernel": (ab["stiffness_2d"], acc["stiffness_ms"]),
+                "assemble_update_2d_kernel": (ab["update_fused"], acc["update_ms"])}
+    else:
+        kern = {"stiffness_2d_kernel": (ab["stiffness_2d"], acc["stiffness_ms"]),
+                "update_fused_kernel": (ab["update_fused"], acc["update_ms"])}
     dominant = max(kern, key=lambda k: kern[k][1])
     nbytes, ms = kern[dominant]
     achieved = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
@@ -284,7 +298,7 @@ def config2_leg(n: int, steps: int, warmup: int, sample_steps: int = 5) -> dict:
         "algorithmic_bytes_per_pt_step": ab["per_pt_step"],
         "model_bytes_per_pt_step": ab["model_per_pt_step"],
         "model_step_hbm_frac": ab["model_per_pt_step"] * value / 1e9 / HBM_PEAK_GBS,
-        "geometry": "one record per element (affine elements)", "seismograms_finite": finite,
+        "geometry": "one record per element (affine elements)", "scatter": scatter, "seismograms_finite": finite,
     }
 
 
@@ -305,13 +319,14 @@ def main() -> int:
     ap.add_argument("--config2", action="store_true",
                     help="measure the 2-D heterogeneous workload instead (minutes of host setup at full size)")
     ap.add_argument("--c2-n", type=int, default=4000, help="--config2: quads per side (2.5 m each)")
+    ap.add_argument("--c2-scatter", default="gather", choices=["gather", "colored", "atomic"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine is single-GPU).",
               file=sys.stderr)
         return 2
     if args.config2:
-        print(json.dumps(config2_leg(args.c2_n, args.steps, args.warmup)))
+        print(json.dumps(config2_leg(args.c2_n, args.steps, args.warmup, scatter=args.c2_scatter)))
         return 0
 
     import numpy as np

@@ -494,8 +494,7 @@ void set_properties_2d(HostAssembly &a, const float *rho, const float *kappa, co
 HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *src, int nrec,
                                const sfx_receiver *rec, const sfx_run_desc &run) {
   check_run(run);
-  if (run.scatter == SFX_SCATTER_GATHER || run.scatter == SFX_SCATTER_BLOCK)
-    throw std::invalid_argument("SFX_SCATTER_GATHER and SFX_SCATTER_BLOCK are for 3-D meshes");
+  if (run.scatter == SFX_SCATTER_BLOCK) throw std::invalid_argument("SFX_SCATTER_BLOCK is for 3-D meshes");
   if (m.nspec <= 0 || m.npgeo <= 0) throw std::invalid_argument("empty mesh");
   if (m.ngnod != 4 && m.ngnod != 9) throw std::invalid_argument("ngnod must be 4 or 9");
   if (!m.coorg || !m.knods || !m.material || !m.rho || !m.vp || !m.vs) throw std::invalid_argument("null mesh array");

@@ -41,6 +41,12 @@ struct sfx_engine {
   DevBuf<int> gather, bnd_gather, bnd_slot, bnd_seg, src_order, src_ispec, rec_ispec, types;
   DevBuf<float> xix, xiz, gammax, gammaz, jac, kappa, mu;
   DevBuf<float> geo;     // 2-D affine meshes: one geometry record per element (the forward stiffness reads it)
+  // 2-D gather path: element forces, the per-point inverse index, Stacey
+  // entries sorted by element point with their segments
+  DevBuf<float> re2;
+  DevBuf<int> inv_off, inv_ent, g2_seg, g2_gather, g2_slot;
+  DevBuf<float> g2_normal, g2_weight, g2_rvp, g2_rvs;
+  int g2_nseg = 0;
   DevBuf<float> bnd_normal, bnd_weight, bnd_rvp, bnd_rvs;
   DevBuf<float> src_array, stf, rec_h, seismo;
   DevBuf<float> fields;  // u, v, a, minv: 4 * ncomp * nglob_pad
@@ -124,6 +130,15 @@ struct sfx_engine {
   // device counter; recording adds the Stacey-element record.
   void enqueue_forward_2d(bool record_on) {
     Fields ff = f();
+    if (gather2d()) {
+      forces_2d(ff, 0, d_step.p);
+      launch_seismograms(stream, 2, h.npe, h.nrec, run.nseismo_types, types.p, 0, rec_ispec.p, gather.p, rec_h.p, ff,
+                         nc.dto2, 1, seismo.p, PointGeom2D{xix.p, xiz.p, gammax.p, gammaz.p, kappa.p, mu.p}, q,
+                         d_step.p, run.nstep_between_samples, -1, fs2());
+      launch_assemble_update_2d(stream, h.nglob, fs2(), ff, nc, 0);
+      launch_step_tick(stream, d_step.p, 1);
+      return;
+    }
     const long long rstride = static_cast<long long>(nstacey) * h.npe * h.ncomp;
     launch_sources(stream, 2, h.npe, h.nsrc, 0, src_order.p, src_ispec.p, gather.p, src_array.p, stf.p, ff, d_step.p);
     const bool atomic = run.scatter == SFX_SCATTER_ATOMIC;
@@ -157,6 +172,67 @@ struct sfx_engine {
   }
 
   bool gather_path() const { return h.dim == 3 && run.scatter == SFX_SCATTER_GATHER; }
+  bool gather2d() const { return h.dim == 2 && run.scatter == SFX_SCATTER_GATHER; }
+  ForceSum2D fs2() const { return ForceSum2D{inv_off.p, inv_ent.p, re2.p}; }
+
+  // 2-D gather path: per field point the re2 offsets of its element
+  // contributions (ascending element slot, then point), and the Stacey entries
+  // sorted by element point, one segment per point
+  void upload_gather_2d(hipStream_t s) {
+    const int npe = h.npe;
+    std::vector<int> off(static_cast<size_t>(h.nglob) + 1, 0);
+    for (int slot = 0; slot < h.nspec; slot++)
+      for (int p = 0; p < npe; p++) off[h.gather[static_cast<size_t>(h.perm[slot]) * npe + p] + 1]++;
+    for (int g = 0; g < h.nglob; g++) off[g + 1] += off[g];
+    std::vector<int> ent(off.back()), fill(off.begin(), off.end() - 1);
+    for (int slot = 0; slot < h.nspec; slot++)
+      for (int p = 0; p < npe; p++) {
+        const int g = h.gather[static_cast<size_t>(h.perm[slot]) * npe + p];
+        ent[fill[g]++] = slot * RE2_ELEM + 2 * p;
+      }
+    inv_off.upload(off, s);
+    inv_ent.upload(ent, s);
+    re2.alloc(static_cast<size_t>(h.nspec) * RE2_ELEM);
+    launch_fill(s, re2.p, re2.n, 0.0f);
+    const size_t nb = h.bnd_gather.size();
+    std::vector<size_t> order(nb);
+    std::iota(order.begin(), order.end(), 0);
+    auto dev_slot = [&](size_t q) {
+      const int c = h.bnd_slot[q] / npe, p = h.bnd_slot[q] - (h.bnd_slot[q] / npe) * npe;
+      return h.inv_perm[c] * npe + p;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return dev_slot(a) < dev_slot(b); });
+    std::vector<int> sg, gg(nb), sl(nb);
+    std::vector<float> nrm(2 * nb), wt(nb), rvp(nb), rvs(nb);
+    for (size_t k = 0; k < nb; k++) {
+      const size_t q = order[k];
+      gg[k] = h.bnd_gather[q];
+      sl[k] = dev_slot(q);
+      nrm[2 * k] = h.bnd_normal[2 * q];
+      nrm[2 * k + 1] = h.bnd_normal[2 * q + 1];
+      wt[k] = h.bnd_weight[q];
+      rvp[k] = h.bnd_rho_vp[q];
+      rvs[k] = h.bnd_rho_vs[q];
+      if (k == 0 || sl[k] != sl[k - 1]) sg.push_back(static_cast<int>(k));
+    }
+    sg.push_back(static_cast<int>(nb));
+    g2_nseg = static_cast<int>(sg.size()) - 1;
+    g2_seg.upload(sg, s);
+    g2_gather.upload(gg, s);
+    g2_slot.upload(sl, s);
+    g2_normal.upload(nrm, s);
+    g2_weight.upload(wt, s);
+    g2_rvp.upload(rvp, s);
+    g2_rvs.upload(rvs, s);
+  }
+  // element forces, Stacey and sources into re2 (the step's force, unassembled)
+  void forces_2d(Fields ff, int istep, const int *step) {
+    launch_elemforce_2d(stream, h.nspec, gather.p, xix.p, xiz.p, gammax.p, gammaz.p, jac.p, kappa.p, mu.p, q, ff.u,
+                        h.nglob_pad, geo.p, re2.p);
+    launch_stacey_re2(stream, g2_nseg, g2_seg.p, g2_gather.p, g2_slot.p, g2_normal.p, g2_weight.p, g2_rvp.p, g2_rvs.p,
+                      ff, re2.p);
+    launch_sources_re2(stream, h.nsrc, istep, src_order.p, src_ispec.p, src_array.p, stf.p, re2.p, step);
+  }
   bool block_path() const { return h.dim == 3 && run.scatter == SFX_SCATTER_BLOCK; }
 
   // Stacey entries regrouped for the block path: entry slot = owner * 386 +
@@ -236,6 +312,7 @@ struct sfx_engine {
       }
       rec_index.upload(ri, s);
       bnd_rec.upload(br, s);
+      if (gather2d()) upload_gather_2d(s);
     }
     bnd_gather.upload(h.bnd_gather, s);
     bnd_slot.upload(h.bnd_slot, s);
@@ -365,6 +442,25 @@ struct sfx_engine {
                                  snap.p, ff, nc.dto2, seismo.p);
       if (evs) SFX_HIP(hipEventRecord(evs[3], stream));
       launch_surface_update_3d(stream, lat, bs.p, ff, nc, last ? 1 : 0, 0);
+      if (evs) SFX_HIP(hipEventRecord(evs[4], stream));
+      return;
+    }
+    if (gather2d()) {
+      if (istep < 0 || istep >= run.nstep) throw std::invalid_argument("istep out of range");
+      if (recording) throw std::invalid_argument("adjoint runs need SFX_SCATTER_COLORED or _ATOMIC");
+      if (evs) SFX_HIP(hipEventRecord(evs[0], stream));
+      if (evs) SFX_HIP(hipEventRecord(evs[1], stream));
+      forces_2d(ff, istep, nullptr);
+      if (evs) SFX_HIP(hipEventRecord(evs[2], stream));
+      if (istep % run.nstep_between_samples == 0) {
+        const int isig = istep / run.nstep_between_samples;
+        if (isig >= nsig) throw std::invalid_argument("seismogram step out of range");
+        launch_seismograms(stream, 2, h.npe, h.nrec, run.nseismo_types, types.p, isig, rec_ispec.p, gather.p,
+                           rec_h.p, ff, nc.dto2, 1, seismo.p,
+                           PointGeom2D{xix.p, xiz.p, gammax.p, gammaz.p, kappa.p, mu.p}, q, nullptr, 1, -1, fs2());
+      }
+      if (evs) SFX_HIP(hipEventRecord(evs[3], stream));
+      launch_assemble_update_2d(stream, h.nglob, fs2(), ff, nc, 0);
       if (evs) SFX_HIP(hipEventRecord(evs[4], stream));
       return;
     }
@@ -711,6 +807,10 @@ int sfx_update_wavefields(sfx_engine *e, int istep) {
       e->element_boundary_sources(ff, istep);
       launch_assemble_3d(e->stream, e->lat, e->re.p, ff, 1);
       e->a_stale = false;
+    } else if (e->gather2d()) {
+      if (istep < 0 || istep >= e->run.nstep) throw std::invalid_argument("istep out of range");
+      e->forces_2d(ff, istep, nullptr);
+      launch_assemble_update_2d(e->stream, e->h.nglob, e->fs2(), ff, e->nc, 1);
     } else {
       e->sources(ff, istep);
       e->stiffness(ff);
@@ -892,6 +992,7 @@ int sfx_set_properties(sfx_engine *e, const float *rho, const float *kappa, cons
     const size_t stride = static_cast<size_t>(e->h.ncomp) * e->h.nglob_pad;
     SFX_HIP(hipMemcpyAsync(e->fields.p + 3 * stride, e->h.mass_inverse.data(), stride * sizeof(float),
                            hipMemcpyHostToDevice, s));
+    if (e->gather2d()) e->upload_gather_2d(s);  // the sorted Stacey entries carry the impedances
     SFX_HIP(hipStreamSynchronize(s));
     e->recorded = false;
     e->invalidate_graphs();
@@ -903,6 +1004,7 @@ int sfx_forward_record(sfx_engine *e, int snapshot_every) {
     check_device(e);
     e->require_init();
     if (e->h.dim != 2) throw std::invalid_argument("adjoint runs are implemented for 2-D meshes");
+    if (e->gather2d()) throw std::invalid_argument("adjoint runs need SFX_SCATTER_COLORED or _ATOMIC");
     if (snapshot_every < 0) throw std::invalid_argument("snapshot_every must be >= 0");
     const size_t need = static_cast<size_t>(e->run.nstep) * e->nstacey * e->h.npe * e->h.ncomp;
     if (e->record.n != need) e->record.alloc(need);

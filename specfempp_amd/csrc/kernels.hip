@@ -159,6 +159,193 @@ __global__ __launch_bounds__(threads_for(E)) void stiffness_2d_kernel(
   for (int c = 0; c < 2; c++) scatter_add<ATOMIC>(&f.a[c * f.nglob_pad + g], r[c]);
 }
 
+// ---------------------------------------------------------------------------
+// 2-D gather path (forward runs): the element forces of 5 elements per
+// workgroup are staged in LDS and stored with coalesced 16-byte writes
+// (RE2_ELEM floats per element); Stacey tractions and sources are added to
+// them; one update per field point then sums its contributions in a fixed
+// order (ascending element, then point) and applies the Newmark update, with
+// no scattered read-modify-write of the acceleration.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(128) void elemforce_2d_kernel(
+    int nspec, const int *__restrict__ gather, const float *__restrict__ xix_, const float *__restrict__ xiz_,
+    const float *__restrict__ gammax_, const float *__restrict__ gammaz_, const float *__restrict__ jac_,
+    const float *__restrict__ kappa_, const float *__restrict__ mu_, GllConst q, const float *__restrict__ uf,
+    int np, const float *__restrict__ geo, float *__restrict__ re2) {
+  __shared__ float su[E2][2][25];
+  __shared__ float sF[E2][4][25];  // F(c, d) as [2c + d]
+  __shared__ __align__(16) float st[E2 * RE2_ELEM];
+
+  const int t = threadIdx.x;
+  const int el = t / 25;
+  const int p = t - el * 25;
+  const int iz = p / 5, ix = p - (p / 5) * 5;
+  const int e = blockIdx.x * E2 + el;
+  const bool active = (el < E2) && (e < nspec);
+  const size_t idx = static_cast<size_t>(e) * 25 + p;
+
+  float kappa = 0, mu = 0, xix = 0, xiz = 0, gammax = 0, gammaz = 0, J = 0;
+  if (active) {
+    kappa = kappa_[idx];
+    mu = mu_[idx];
+    const int g = gather[idx];
+    su[el][0][p] = uf[g];
+    su[el][1][p] = uf[np + g];
+    if (geo) {
+      const float *ge = geo + static_cast<size_t>(e) * 8;
+      xix = ge[0];
+      xiz = ge[1];
+      gammax = ge[2];
+      gammaz = ge[3];
+      J = ge[4];
+    } else {
+      xix = xix_[idx];
+      xiz = xiz_[idx];
+      gammax = gammax_[idx];
+      gammaz = gammaz_[idx];
+      J = jac_[idx];
+    }
+  }
+  __syncthreads();
+  float hpx[5], hpz[5], hwx[5], hwz[5];
+#pragma unroll
+  for (int l = 0; l < 5; l++) {
+    hpx[l] = q.hp[ix][l];
+    hpz[l] = q.hp[iz][l];
+    hwx[l] = q.hpw[ix][l];
+    hwz[l] = q.hpw[iz][l];
+  }
+  const float wx = q.w[ix], wz = q.w[iz];
+  if (active) {
+    // the gradient, stress and stress integrand of stiffness_2d_kernel
+    float dxi[2] = {0.0f, 0.0f}, dga[2] = {0.0f, 0.0f};
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        dxi[c] += hpx[l] * su[el][c][iz * 5 + l];
+        dga[c] += hpz[l] * su[el][c][l * 5 + ix];
+      }
+    }
+    float du[2][2];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      du[c][0] = xix * dxi[c] + gammax * dga[c];
+      du[c][1] = xiz * dxi[c] + gammaz * dga[c];
+    }
+    const float lp2m = kappa + (4.0f / 3.0f) * mu;
+    const float lambda = lp2m - 2.0f * mu;
+    const float sxx = lp2m * du[0][0] + lambda * du[1][1];
+    const float szz = lp2m * du[1][1] + lambda * du[0][0];
+    const float sxz = mu * (du[0][1] + du[1][0]);
+    sF[el][0][p] = J * (sxx * xix + sxz * xiz);
+    sF[el][1][p] = J * (sxx * gammax + sxz * gammaz);
+    sF[el][2][p] = J * (sxz * xix + szz * xiz);
+    sF[el][3][p] = J * (sxz * gammax + szz * gammaz);
+  }
+  __syncthreads();
+  if (active) {
+    float t1[2] = {0.0f, 0.0f}, t2[2] = {0.0f, 0.0f};
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        t1[c] += sF[el][2 * c][iz * 5 + l] * hwx[l];
+        t2[c] += sF[el][2 * c + 1][l * 5 + ix] * hwz[l];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 2; c++) st[el * RE2_ELEM + 2 * p + c] = -(wz * t1[c] + wx * t2[c]);
+  }
+  __syncthreads();
+  const int first = blockIdx.x * E2;
+  const int nel = nspec - first < E2 ? nspec - first : E2;
+  float4 *dst = reinterpret_cast<float4 *>(re2 + static_cast<size_t>(first) * RE2_ELEM);
+  const float4 *from = reinterpret_cast<const float4 *>(st);
+  for (int qv = t; qv < nel * (RE2_ELEM / 4); qv += 128) dst[qv] = from[qv];
+}
+
+// One thread per element point on the absorbing boundary (entries sorted by
+// element point); its edge entries (two at a corner) are summed in order and
+// added to the point's element force once.
+__global__ __launch_bounds__(256) void stacey_re2_kernel(int nseg, const int *__restrict__ seg,
+                                                         const int *__restrict__ gather, const int *__restrict__ slot,
+                                                         const float *__restrict__ normal,
+                                                         const float *__restrict__ weight,
+                                                         const float *__restrict__ rho_vp,
+                                                         const float *__restrict__ rho_vs, Fields f,
+                                                         float *__restrict__ re2) {
+  const int sidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sidx >= nseg) return;
+  const int q0 = seg[sidx], q1 = seg[sidx + 1];
+  const int g = gather[q0];
+  const int sl = slot[q0];
+  const int e = sl / 25, p = sl - (sl / 25) * 25;
+  const int np = f.nglob_pad;
+  const float v[2] = {f.v[g], f.v[np + g]};
+  float sum[2] = {0.0f, 0.0f};
+  for (int qq = q0; qq < q1; qq++) {
+    const float n[2] = {normal[2 * qq], normal[2 * qq + 1]};
+    const float vn = v[0] * n[0] + v[1] * n[1];
+    const float nn = n[0] * n[0] + n[1] * n[1];
+    const float j1 = sqrtf(nn);
+    const float w = weight[qq], rvp = rho_vp[qq], rvs = rho_vs[qq];
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+      const float fac = ((vn * n[d] / (j1 * j1)) * (rvp - rvs)) + v[d] * rvs;
+      sum[d] += -1.0f * fac * j1 * w;
+    }
+  }
+  float *o = re2 + static_cast<size_t>(e) * RE2_ELEM + 2 * p;
+  o[0] += sum[0];
+  o[1] += sum[1];
+}
+
+// Sources in application order, one thread per element point.
+__global__ __launch_bounds__(64) void sources_re2_kernel(int nsrc, int istep_arg, const int *__restrict__ order,
+                                                         const int *__restrict__ ispec,
+                                                         const float *__restrict__ array,
+                                                         const float *__restrict__ stf, float *__restrict__ re2,
+                                                         const int *__restrict__ step) {
+  const int p = threadIdx.x;
+  if (p >= 25) return;
+  const int istep = step ? *step : istep_arg;
+  for (int k = 0; k < nsrc; k++) {
+    const int s = order[k];
+    const float sv = stf[static_cast<size_t>(istep) * nsrc + s];
+    float *o = re2 + static_cast<size_t>(ispec[s]) * RE2_ELEM + 2 * p;
+#pragma unroll
+    for (int c = 0; c < 2; c++) o[c] += sv * array[(static_cast<size_t>(s) * 2 + c) * 25 + p];
+  }
+}
+
+// One thread per field point: the assembled force (its element contributions
+// in a fixed order), a = M^-1 f, then (unless divide_only) corrector and
+// predictor: v += dt/2 a; u += dt v + dt^2/2 a; v += dt/2 a.  The stored a is
+// the step's acceleration.
+__global__ __launch_bounds__(256) void assemble_update_2d_kernel(int nglob, ForceSum2D fs, Fields f,
+                                                                 NewmarkConst nc, int divide_only) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nglob) return;
+  float sum[2] = {0.0f, 0.0f};
+  for (int k = fs.off[g]; k < fs.off[g + 1]; k++) {
+    const float2 r = *reinterpret_cast<const float2 *>(fs.re2 + fs.ent[k]);
+    sum[0] += r.x;
+    sum[1] += r.y;
+  }
+  const int np = f.nglob_pad;
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const size_t o = static_cast<size_t>(c) * np + g;
+    const float a = sum[c] * f.minv[o];
+    f.a[o] = a;
+    if (divide_only) continue;
+    const float v1 = f.v[o] + nc.dto2 * a;
+    f.u[o] = f.u[o] + (nc.dt * v1 + nc.dt2o2 * a);
+    f.v[o] = v1 + nc.dto2 * a;
+  }
+}
+
 // Adjoint and backward fields together (the combined loop's two
 // update_wavefields calls, time_marching.tpp:160-186): one pass over the
 // elements loads the geometry and properties once and computes both fields'
@@ -660,7 +847,8 @@ __global__ __launch_bounds__(128) void seismogram_kernel(int npe, int nrec, int 
                                                          const int *__restrict__ gather, const float *__restrict__ h,
                                                          Fields f, float dto2, int corrected,
                                                          float *__restrict__ seismo, PointGeom2D pg, GllConst q,
-                                                         const int *__restrict__ step, int nsb, int sig_top) {
+                                                         const int *__restrict__ step, int nsb, int sig_top,
+                                                         ForceSum2D fs) {
   __shared__ float red[2][D][64];
   __shared__ float su[2][25];
   if (step) {  // sample only on steps istep % nsb == 0 (the same for the whole block)
@@ -697,7 +885,13 @@ __global__ __launch_bounds__(128) void seismogram_kernel(int npe, int nrec, int 
         if (type == 0) {
           x = f.u[c * np + g];
         } else {
-          float acc = f.a[c * np + g];
+          float acc;
+          if (D == 2 && fs.re2) {  // gather path: the assembled force, summed as the update sums it
+            acc = 0.0f;
+            for (int k = fs.off[g]; k < fs.off[g + 1]; k++) acc += fs.re2[fs.ent[k] + c];
+          } else {
+            acc = f.a[c * np + g];
+          }
           if (corrected) acc = acc * f.minv[c * np + g];
           if (type == 1) {
             x = f.v[c * np + g];
@@ -1290,14 +1484,43 @@ void launch_sources(hipStream_t s, int dim, int npe, int nsrc, int istep, const 
 void launch_seismograms(hipStream_t s, int dim, int npe, int nrec, int ntypes, const int *types, int isig,
                         const int *ispec, const int *gather, const float *h, Fields f, float dto2, int corrected,
                         float *seismo, const PointGeom2D &pg, const GllConst &q, const int *step, int nsb,
-                        int sig_top) {
+                        int sig_top, ForceSum2D fs) {
   if (nrec <= 0 || ntypes <= 0) return;
   if (dim == 2)
     hipLaunchKernelGGL(seismogram_kernel<2>, dim3(nrec), dim3(128), 0, s, npe, nrec, ntypes, types, isig, ispec,
-                       gather, h, f, dto2, corrected, seismo, pg, q, step, nsb, sig_top);
+                       gather, h, f, dto2, corrected, seismo, pg, q, step, nsb, sig_top, fs);
   else
     hipLaunchKernelGGL(seismogram_kernel<3>, dim3(nrec), dim3(128), 0, s, npe, nrec, ntypes, types, isig, ispec,
-                       gather, h, f, dto2, corrected, seismo, pg, q, step, nsb, sig_top);
+                       gather, h, f, dto2, corrected, seismo, pg, q, step, nsb, sig_top, ForceSum2D{});
+}
+
+void launch_elemforce_2d(hipStream_t s, int nspec, const int *gather, const float *xix, const float *xiz,
+                         const float *gammax, const float *gammaz, const float *jac, const float *kappa,
+                         const float *mu, const GllConst &q, const float *u, int nglob_pad, const float *geo,
+                         float *re2) {
+  if (nspec <= 0) return;
+  hipLaunchKernelGGL(elemforce_2d_kernel, dim3(blocks(nspec, E2)), dim3(128), 0, s, nspec, gather, xix, xiz, gammax,
+                     gammaz, jac, kappa, mu, q, u, nglob_pad, geo, re2);
+}
+
+void launch_stacey_re2(hipStream_t s, int nseg, const int *seg, const int *gather, const int *slot,
+                       const float *normal, const float *weight, const float *rho_vp, const float *rho_vs, Fields f,
+                       float *re2) {
+  if (nseg <= 0) return;
+  hipLaunchKernelGGL(stacey_re2_kernel, dim3(blocks(nseg, 256)), dim3(256), 0, s, nseg, seg, gather, slot, normal,
+                     weight, rho_vp, rho_vs, f, re2);
+}
+
+void launch_sources_re2(hipStream_t s, int nsrc, int istep, const int *order, const int *ispec, const float *array,
+                        const float *stf, float *re2, const int *step) {
+  if (nsrc <= 0) return;
+  hipLaunchKernelGGL(sources_re2_kernel, dim3(1), dim3(64), 0, s, nsrc, istep, order, ispec, array, stf, re2, step);
+}
+
+void launch_assemble_update_2d(hipStream_t s, int nglob, ForceSum2D fs, Fields f, NewmarkConst nc, int divide_only) {
+  if (nglob <= 0) return;
+  hipLaunchKernelGGL(assemble_update_2d_kernel, dim3(blocks(nglob, 256)), dim3(256), 0, s, nglob, fs, f, nc,
+                     divide_only);
 }
 
 void launch_update_fused(hipStream_t s, int ncomp, Fields f, NewmarkConst nc) {

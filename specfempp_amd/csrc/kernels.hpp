@@ -153,10 +153,29 @@ struct PointGeom2D {
 // corrected = 1: sample the state after the (not yet applied) mass division and
 // corrector of this step, i.e. u, v + dt/2 a M^-1, a M^-1.  Types 0..2
 // displacement, velocity, acceleration; 3 pressure (2-D).
+// 2-D gather path: element forces re2 (RE2_ELEM floats per element, point p
+// component c at 2 p + c) and, per field point g, the re2 offsets of its
+// element contributions in [off[g], off[g + 1]) of ent (summed in that order).
+constexpr int RE2_ELEM = 52;
+struct ForceSum2D {
+  const int *off = nullptr, *ent = nullptr;
+  const float *re2 = nullptr;
+};
+
 void launch_seismograms(hipStream_t s, int dim, int npe, int nrec, int ntypes, const int *types, int isig,
                         const int *ispec, const int *gather, const float *h, Fields f, float dto2, int corrected,
                         float *seismo, const PointGeom2D &pg, const GllConst &q, const int *step = nullptr,
-                        int nsb = 1, int sig_top = -1);
+                        int nsb = 1, int sig_top = -1, ForceSum2D fs = ForceSum2D{});
+void launch_elemforce_2d(hipStream_t s, int nspec, const int *gather, const float *xix, const float *xiz,
+                         const float *gammax, const float *gammaz, const float *jac, const float *kappa,
+                         const float *mu, const GllConst &q, const float *u, int nglob_pad, const float *geo,
+                         float *re2);
+void launch_stacey_re2(hipStream_t s, int nseg, const int *seg, const int *gather, const int *slot,
+                       const float *normal, const float *weight, const float *rho_vp, const float *rho_vs, Fields f,
+                       float *re2);
+void launch_sources_re2(hipStream_t s, int nsrc, int istep, const int *order, const int *ispec, const float *array,
+                        const float *stf, float *re2, const int *step = nullptr);
+void launch_assemble_update_2d(hipStream_t s, int nglob, ForceSum2D fs, Fields f, NewmarkConst nc, int divide_only);
 void launch_update_fused(hipStream_t s, int ncomp, Fields f, NewmarkConst nc);
 void launch_predictor(hipStream_t s, int ncomp, Fields f, NewmarkConst nc);
 void launch_divide_corrector(hipStream_t s, int ncomp, Fields f, NewmarkConst nc, int divide, int correct);

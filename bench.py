@@ -228,7 +228,7 @@ def algorithmic_bytes_2d(nspec: int, nglob: int, per_element_geometry: bool, sca
     else:
         stiff = nspec * 25 * 12 + geo + nglob * 24
         upd = nglob * 56
-    model = nspec * 25 * 32 + nglob * 24 + upd
+    model = nspec * 25 * 32 + nglob * 24 + nglob * 56  # per-point geometry, colored scatter, fused update
     return {"stiffness_2d": stiff, "update_fused": upd, "per_pt_step": (stiff + upd) / nglob,
             "model_per_pt_step": model / nglob}
 

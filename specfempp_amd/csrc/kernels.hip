@@ -1095,7 +1095,13 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
         }
       }
       const float wjk = wj * q.w[k], wik = wi * q.w[k], wij = wi * wj;
-      const int slot = re_slot(i, j, k);
+      // re_slot(i, j, k) written as selects on the thread's (i, j) and the
+      // unrolled k: the switch of re_slot compiles to branches here
+      const bool fi = i == 4, fj = j == 4;
+      const int kb = !fi && !fj ? j * 4 + i : !fj ? 64 + j : !fi ? 80 + i : 112;
+      const int kst = !fi && !fj ? 16 : fi && fj ? 1 : 4;
+      const int b4 = !fi && !fj ? 96 + j * 4 + i : !fj ? 116 + j : !fi ? 120 + i : 124;
+      const int slot = k < 4 ? kb + k * kst : b4;
 #pragma unroll
       for (int c = 0; c < 3; c++) su[el][c][slot] = -(wjk * r0[c] + wik * r1[c] + wij * r2[c]);
     }

@@ -81,19 +81,45 @@ def measured_traffic(workload: str, scatter: str, kernel: str):
     return 1024.0 * (c["FETCH_SIZE"] * pmc["scale"]["FETCH_SIZE"] + c["WRITE_SIZE"] * pmc["scale"]["WRITE_SIZE"])
 
 
+def cpu_threads() -> int:
+    """Host threads for the CPU baseline: OMP_NUM_THREADS when set, else the
+    host's CPU count, at most 16."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    n = int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+    return max(1, min(16, n))
+
+
 def cpu_baseline(w, nsteps: int) -> dict:
-    """The FP32 oracle (single thread) on the same 64^3 workload for a few steps."""
+    """The FP32 oracle on the same 64^3 workload for a few steps: on all host
+    threads (the element loop in parallel, the serial loop's bits), and on one
+    thread.  The threaded rate is taken from the difference of a short and a
+    longer run, so the per-run setup (point-to-element index) is not counted."""
     import oracle
 
     t = time.perf_counter()
     o = oracle.Oracle3D(w.mesh, 4, w.dt)
     build_s = time.perf_counter() - t
     t = time.perf_counter()
-    o.forward(w.sources, w.receivers, nsteps, w.dt)
-    el = time.perf_counter() - t
-    return {"value": o.nglob * nsteps / el, "unit": "GLL pt-steps/s", "cores": 1, "kind": "port",
-            "sample": f"FP32 oracle (C++ restatement, 1 thread), {nsteps} steps of the same "
-                      f"{w.mesh['nx']}^3 workload, {el:.1f} s timed (+{build_s:.1f} s setup untimed)"}
+    o.forward(w.sources, w.receivers, nsteps, w.dt, nthreads=1)
+    el1 = time.perf_counter() - t
+    one = o.nglob * nsteps / el1
+    nt = cpu_threads()
+    out = {"value": one, "unit": "GLL pt-steps/s", "cores": 1, "kind": "port",
+           "sample": f"FP32 oracle (C++ restatement, 1 thread), {nsteps} steps of the same "
+                     f"{w.mesh['nx']}^3 workload, {el1:.1f} s timed (+{build_s:.1f} s setup untimed)"}
+    if nt > 1:
+        extra = 4 * nsteps
+        t = time.perf_counter()
+        o.forward(w.sources, w.receivers, 2, w.dt, nthreads=nt)
+        ta = time.perf_counter() - t
+        t = time.perf_counter()
+        o.forward(w.sources, w.receivers, 2 + extra, w.dt, nthreads=nt)
+        tb = time.perf_counter() - t
+        out.update(value=o.nglob * extra / (tb - ta), cores=nt, single_thread_value=one,
+                   sample=f"FP32 oracle (C++ restatement, OpenMP element loop, {nt} threads): "
+                          f"{2 + extra} minus 2 steps of the same {w.mesh['nx']}^3 workload, "
+                          f"{tb - ta:.1f} s; 1 thread: {nsteps} steps, {el1:.1f} s")
+    return out
 
 
 def copy_calibration(nbytes: int = 1 << 30, reps: int = 5) -> dict:

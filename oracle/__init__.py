@@ -71,6 +71,8 @@ def load():
     lib.orc3d_forward.argtypes = [vp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_double, C.c_int, C.c_int, _ip, _dp,
                                   C.POINTER(C.c_double), C.c_void_p]
     lib.orc3d_apply_stiffness.argtypes = [vp, _dp, _dp]
+    lib.orc3d_set_threads.argtypes = [C.c_int]
+    lib.orc3d_set_threads.restype = None
     _lib = lib
     return lib
 
@@ -297,7 +299,12 @@ class Oracle3D:
         load().orc3d_mass_inverse(self.h, out)
         return out
 
-    def forward(self, sources, receivers, nstep, dt, nsb=1, types=("displacement",), return_fields=False):
+    def forward(self, sources, receivers, nstep, dt, nsb=1, types=("displacement",), return_fields=False,
+                nthreads=1):
+        """nthreads > 1 runs the element loop on that many OpenMP threads (same
+        bits as the serial loop: every point still adds its element
+        contributions in increasing element order)."""
+        load().orc3d_set_threads(int(nthreads))
         src = np.zeros((len(sources), 16))
         for i, s in enumerate(sources):
             src[i] = [1 if s.get("type", "force") == "moment_tensor" else 0, s["x"], s.get("y", 0.0), s["z"],

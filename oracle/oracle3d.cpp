@@ -19,10 +19,14 @@
 // engine's regular-element shortcut.
 //
 // Pinning: dense brute-force weak-form assembly on tiny meshes (symmetry,
-// rigid-body null space, K u match), 2-D <-> 3-D consistency, energy
-// conservation without Stacey (tests/test_oracle_3d.py).  The numbering and
-// element location follow the engine's documented model, not a reference
-// artifact: "parity unpinned" against the reference for 3-D.
+// rigid-body null space, K u match; tests/test_oracle_bruteforce.py); in
+// tests/test_oracle_3d.py the discrete central-difference energy, conserved
+// to round-off without absorbing faces and balanced by the closed-form Stacey
+// damping with them, the closed-form mass and Stacey mass term, the
+// force-dipole limit of the moment-tensor source, and the threaded loop
+// giving the serial loop's bits.  The numbering and element location follow
+// the engine's documented model, not a reference artifact: "parity unpinned"
+// against the reference for those.
 #include <algorithm>
 #include <cmath>
 #include <limits>
@@ -31,6 +35,9 @@
 #include <vector>
 
 namespace orc3 {
+
+// Threads of the forward loop (1: the serial element loop).
+int g_threads3 = 1;
 
 constexpr int N = 5;
 
@@ -413,9 +420,27 @@ void forward(const Asm<T> &a, int nsrc, const double *src /*(nsrc,16)*/, int nre
   }
   std::vector<T> u(3 * static_cast<size_t>(ng), 0), v(u.size(), 0), acc(u.size(), 0);
   T out[125][3];
+  // Threaded variant (timing baseline only): element forces of all elements
+  // into ef in parallel, then each point adds its contributions in
+  // increasing element order -- the order of the serial element loop, so
+  // both give the same bits.
+  const int nthreads = std::max(1, g_threads3);
+  const long n3 = static_cast<long>(u.size());
+  std::vector<T> ef;
+  std::vector<int> inv_off, inv_ent;
+  if (nthreads > 1) {
+    ef.resize(static_cast<size_t>(a.nspec) * 375);
+    inv_off.assign(static_cast<size_t>(ng) + 1, 0);
+    for (size_t q = 0; q < a.ibool.size(); q++) inv_off[a.ibool[q] + 1]++;
+    for (int g = 0; g < ng; g++) inv_off[g + 1] += inv_off[g];
+    inv_ent.resize(a.ibool.size());
+    std::vector<int> fill(inv_off.begin(), inv_off.end() - 1);
+    for (size_t q = 0; q < a.ibool.size(); q++) inv_ent[fill[a.ibool[q]]++] = static_cast<int>(q);
+  }
   int isig = 0;
   for (int istep = 0; istep < nstep; istep++) {
-    for (size_t i = 0; i < u.size(); i++) {
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (long i = 0; i < n3; i++) {
       u[i] += dt * v[i] + dt2o2 * acc[i];
       v[i] += dto2 * acc[i];
       acc[i] = 0;
@@ -425,11 +450,21 @@ void forward(const Asm<T> &a, int nsrc, const double *src /*(nsrc,16)*/, int nre
         const int g = a.ibool[static_cast<size_t>(S[s].e) * 125 + p];
         for (int c = 0; c < 3; c++) acc[static_cast<size_t>(c) * ng + g] += S[s].stf[istep] * S[s].arr[c][p];
       }
-    for (int e = 0; e < a.nspec; e++) {
-      element_force<T>(a, e, u.data(), out);
-      for (int p = 0; p < 125; p++) {
-        const int g = a.ibool[static_cast<size_t>(e) * 125 + p];
-        for (int c = 0; c < 3; c++) acc[static_cast<size_t>(c) * ng + g] += out[p][c];
+    if (nthreads > 1) {
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+      for (int e = 0; e < a.nspec; e++)
+        element_force<T>(a, e, u.data(), reinterpret_cast<T(*)[3]>(&ef[static_cast<size_t>(e) * 375]));
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+      for (int g = 0; g < ng; g++)
+        for (int q = inv_off[g]; q < inv_off[g + 1]; q++)
+          for (int c = 0; c < 3; c++) acc[static_cast<size_t>(c) * ng + g] += ef[static_cast<size_t>(inv_ent[q]) * 3 + c];
+    } else {
+      for (int e = 0; e < a.nspec; e++) {
+        element_force<T>(a, e, u.data(), out);
+        for (int p = 0; p < 125; p++) {
+          const int g = a.ibool[static_cast<size_t>(e) * 125 + p];
+          for (int c = 0; c < 3; c++) acc[static_cast<size_t>(c) * ng + g] += out[p][c];
+        }
       }
     }
     for (const auto &B : a.bnd) {
@@ -439,7 +474,8 @@ void forward(const Asm<T> &a, int nsrc, const double *src /*(nsrc,16)*/, int nre
       stacey3(B.n, B.w, B.rvp, B.rvs, vv, t);
       for (int c = 0; c < 3; c++) acc[static_cast<size_t>(c) * ng + g] += t[c];
     }
-    for (size_t i = 0; i < u.size(); i++) {
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (long i = 0; i < n3; i++) {
       acc[i] = acc[i] * a.minv[i];
       v[i] += dto2 * acc[i];
     }
@@ -494,6 +530,10 @@ struct H3 {
 extern "C" {
 
 const char *orc3d_last_error() { return g_err3.c_str(); }
+
+// Threads used by orc3d_forward from now on (1: the serial loop; both give the
+// same bits).
+void orc3d_set_threads(int n) { orc3::g_threads3 = n < 1 ? 1 : n; }
 
 int orc3d_create(int precision, int nx, int ny, int nz, double x0, double y0, double z0, double hx, double hy,
                  double hz, const int *material, const double *rho, const double *vp, const double *vs,

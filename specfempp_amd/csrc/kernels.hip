@@ -994,8 +994,10 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
                                                            const float *__restrict__ uf, int np,
                                                            float *__restrict__ re) {
   __shared__ __align__(16) float su[E3][3][RE_STRIDE];  // displacement, then the forces in slot order
-  __shared__ float sF0[E3][3][125];
-  __shared__ float sF1[E3][3][125];
+  // the five distinct stresses sxx, sxy, sxz, syy, syz per point (sxy once: it
+  // is both sigma(x, y) and sigma(y, x)); 20.2 KB per workgroup lets 8
+  // workgroups share a CU's LDS
+  __shared__ float sS[E3][5][125];
 
   const int nspec = lat.nx * lat.ny * lat.nz;
   const int t = threadIdx.x;
@@ -1068,12 +1070,11 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
       const float sxz = mu * (dz[0] + dx[2]);
       const float syz = mu * (dz[1] + dy[2]);
       const int p = k * 25 + col;
-      sF0[el][0][p] = sxx;
-      sF0[el][1][p] = sxy;
-      sF0[el][2][p] = sxz;
-      sF1[el][0][p] = sxy;
-      sF1[el][1][p] = syy;
-      sF1[el][2][p] = syz;
+      sS[el][0][p] = sxx;
+      sS[el][1][p] = sxy;
+      sS[el][2][p] = sxz;
+      sS[el][3][p] = syy;
+      sS[el][4][p] = syz;
       F2[k][0] = sxz;
       F2[k][1] = syz;
       F2[k][2] = szz;
@@ -1089,8 +1090,9 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
       for (int l = 0; l < 5; l++) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          r0[c] += sF0[el][c][k * 25 + j * 5 + l] * hwi[l];
-          r1[c] += sF1[el][c][k * 25 + l * 5 + i] * hwj[l];
+          // row c of sigma along x (sxx, sxy, sxz) and along y (sxy, syy, syz)
+          r0[c] += sS[el][c][k * 25 + j * 5 + l] * hwi[l];
+          r1[c] += sS[el][c == 0 ? 1 : c + 2][k * 25 + l * 5 + i] * hwj[l];
           r2[c] += F2[l][c] * q.wz[k][l];
         }
       }

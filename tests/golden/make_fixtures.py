@@ -7,7 +7,7 @@ Run once in a container that has the reference checkout:
 
 Only data files are taken (mesh databases, SPECFEM2D traces, index mapping and
 mass-matrix arrays, station and source parameters, and the text run inputs:
-specfem_config.yaml, sources.yaml, STATIONS and the source-reader test data).  They come from
+specfem_config.yaml, sources.yaml, STATIONS, topography.dat and the source-reader test data).  They come from
 PrincetonUniversity/SPECFEMPP ``tests/unit-tests`` (GPL-3.0) and are stored
 re-encoded (gzip / npz) with their provenance recorded in ``MANIFEST.json``.
 """
@@ -40,6 +40,12 @@ CASES = {
         [dict(x=x, z=z, angle=0.0, f0=10.0, factor=1e10, tshift=0.0)
          for (x, z) in [(500.0, 500.0), (3500.0, 500.0), (500.0, 2500.0), (3500.0, 2500.0)]],
     ),
+}
+
+
+MESH_CASES = {
+    "flat_topography_80x60": "data/mesh/simple_mesh_flat_topography",
+    "curved_topography_80x60": "data/mesh/simple_mesh_curved_topography",
 }
 
 
@@ -99,6 +105,31 @@ def main(ref: str) -> None:
                    "index_mapping_layout": "(nspec, ngllz, ngllx) row-major, 0-based, mesh order",
                    "rmass_layout": "(nglob, 2) row-major"}, fh, indent=1)
     manifest["mass_80x60"] = f"{UT}/{rel}"
+
+    # Mesh-reader data (tests/unit-tests/data/mesh): two 80 x 60 meshes of
+    # 9-node elements under flat and curved topography, with the material
+    # properties the reader test expects (tests/unit-tests/mesh/materials/
+    # properties.cpp:15-35: kappa, mu, rho).  The run parameters (dt, nstep,
+    # the force source of sources.yaml) are for the engine-vs-oracle runs.
+    for name, rel in MESH_CASES.items():
+        src_dir = os.path.join(ref, UT, rel)
+        out = os.path.join(HERE, name)
+        os.makedirs(out, exist_ok=True)
+        gz_copy(os.path.join(src_dir, "database.bin"), os.path.join(out, "database.bin.gz"))
+        for fname in ("sources.yaml", "STATIONS", "topography.dat"):
+            shutil.copyfile(os.path.join(src_dir, fname), os.path.join(out, fname))
+        stations = read_stations(os.path.join(src_dir, "STATIONS"))
+        with open(os.path.join(out, "case.json"), "w") as fh:
+            json.dump({
+                "source_dir": f"{UT}/{rel}",
+                "properties_known_answer": {"kappa": 13499997600.24, "mu": 8100001799.82, "rho": 2700.0,
+                                            "from": f"{UT}/mesh/materials/properties.cpp:15-35"},
+                "nspec": 4800, "ngnod": 9,
+                "dt": 8e-4, "nstep": 1500, "nstep_between_samples": 1,
+                "stations": [dict(name=s, network=n, x=x, z=z) for (s, n, x, z) in stations],
+                "sources": [dict(x=2500.0, z=2500.0, angle=0.0, f0=10.0, factor=1e10, tshift=0.0)],
+            }, fh, indent=1)
+        manifest[name] = f"{UT}/{rel}"
 
     # source-file reader data (tests/unit-tests/io/sources)
     rel = "io/sources"

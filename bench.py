@@ -66,6 +66,32 @@ def algorithmic_bytes_3d(nspec: int, nglob: int) -> dict:
             "survey_per_pt_step": survey / nglob}
 
 
+FP32_VALU_PEAK_TFLOPS = 157.3  # MI355X vector FP32 (packed FP32 instructions)
+FP32_VALU_UNPACKED_TFLOPS = 78.6  # one FP32 operation per lane per cycle (packed FP32 is not built)
+
+
+def elemforce_3d_flops(nspec: int) -> float:
+    """FP32 operations elemforce_3d must perform per launch (FMA = 2): per
+    local point, the three 5-term derivative contractions of three components
+    (45 FMA), the isotropic stress (22), the three 5-term divergence
+    contractions (45 FMA) and their weighting (18): 220 per point, 125 points
+    per element."""
+    return nspec * 125 * (90 + 22 + 90 + 18)
+
+
+def measured_valu_flops(workload: str, scatter: str, kernel: str):
+    """FP32 VALU operations per launch of a kernel from the committed
+    rocprofv3 instruction-count summary, or None."""
+    path = os.path.join(ROOT, "profiles", "gather3d_64cube_valu.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        prof = json.load(fh)
+    if prof.get("workload") != workload or prof.get("scatter") != scatter or kernel not in prof["per_dispatch"]:
+        return None
+    return prof["per_dispatch"][kernel]["fp32_flops"]
+
+
 def measured_traffic(workload: str, scatter: str, kernel: str):
     """HBM bytes per launch of a kernel from the committed rocprofv3 counter
     summary of this workload (FETCH_SIZE and WRITE_SIZE runs, each scaled by
@@ -398,6 +424,16 @@ def main() -> int:
                       "GBs": (v[0] / (v[1] * 1e-3) / 1e9 if v[1] > 0 else 0.0)} for k, v in kern.items()}
     step_gbs = ab["per_pt_step"] * value / 1e9
     survey_gbs = ab["survey_per_pt_step"] * value / 1e9
+    valu = None
+    if scatter != "block" and tm["stiffness_ms"] > 0:
+        flops = elemforce_3d_flops(nspec)
+        tflops = flops / (tm["stiffness_ms"] * 1e-3) / 1e12
+        valu = {"kernel": "elemforce_3d_kernel", "algorithmic_flops_per_launch": flops, "achieved": tflops,
+                "unit": "TFLOP/s", "peak": FP32_VALU_PEAK_TFLOPS, "frac": tflops / FP32_VALU_PEAK_TFLOPS,
+                "peak_unpacked": FP32_VALU_UNPACKED_TFLOPS, "frac_unpacked": tflops / FP32_VALU_UNPACKED_TFLOPS,
+                "counted_flops_per_launch": measured_valu_flops(f"3d_layered_cube_{args.n}^3_ngll5", scatter,
+                                                                "elemforce_3d_kernel"),
+                "counted_source": "profiles/gather3d_64cube_valu.json (rocprofv3 SQ_INSTS_VALU_*_F32)"}
     out = {
         "metric": "GLL pt-steps/s, 3D elastic NGLL=5 FP32 forward (64^3 layered, Stacey + free surface)",
         "value": value,
@@ -421,6 +457,7 @@ def main() -> int:
                      "kernel": dominant, "algorithmic_bytes_per_launch": nbytes, "avg_ms": ms},
         "kernels": per_kernel,
         "kernels_ms": tm,
+        "valu": valu,
         "step_hbm_gbs": step_gbs,
         "step_hbm_frac": step_gbs / HBM_PEAK_GBS,
         "algorithmic_bytes_per_pt_step": ab["per_pt_step"],

@@ -992,6 +992,7 @@ __global__ __launch_bounds__(256) void divide_corrector_kernel(int n, int ncomp,
 __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const float *__restrict__ kappa_,
                                                            const float *__restrict__ mu_, Gll3 q,
                                                            const float *__restrict__ uf, int np,
+                                                           double inv_nx, double inv_nxy,
                                                            float *__restrict__ re) {
   __shared__ __align__(16) float su[E3][3][RE_STRIDE];  // displacement, then the forces in slot order
   // the five distinct stresses sxx, sxy, sxz, syy, syz per point (sxy once: it
@@ -1030,13 +1031,21 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
   }
   float u[5][3];
   if (active) {
-    const int ex = e % lat.nx, ey = (e / lat.nx) % lat.ny, ez = e / (lat.nx * lat.ny);
+    // e / nx and e / (nx ny) as floor((e + 1/2) / d) in FP64: the product's
+    // error (below 2^-51 relative) stays far from the 1/(2d) margin to the next
+    // integer for any int e, and it costs a few instructions instead of an
+    // integer division sequence
+    const int eyz = static_cast<int>((e + 0.5) * inv_nx), ez = static_cast<int>((e + 0.5) * inv_nxy);
+    const int ex = e - eyz * lat.nx, ey = eyz - ez * lat.ny;
+    // one 32-bit point offset per layer against the three (uniform) component
+    // bases, so the loads need no 64-bit address arithmetic per component
+    const float *__restrict__ ub[3] = {uf, uf + np, uf + 2 * static_cast<size_t>(np)};
 #pragma unroll
     for (int k = 0; k < 5; k++) {
-      const int d = tiled_index(lat, ex, ey, ez, i, j, k);
+      const unsigned d = static_cast<unsigned>(tiled_index(lat, ex, ey, ez, i, j, k));
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        u[k][c] = uf[c * np + d];
+        u[k][c] = ub[c][d];
         su[el][c][k * 25 + col] = u[k][c];
       }
     }
@@ -1096,7 +1105,9 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
           r2[c] += F2[l][c] * q.wz[k][l];
         }
       }
-      const float wjk = wj * q.w[k], wik = wi * q.w[k], wij = wi * wj;
+      // the weights negated: -(a r0 + b r1 + c r2) = (-a) r0 + (-b) r1 + (-c) r2
+      // exactly (rounding is symmetric), without a sign flip per component
+      const float wjk = -(wj * q.w[k]), wik = -(wi * q.w[k]), wij = -(wi * wj);
       // re_slot(i, j, k) written as selects on the thread's (i, j) and the
       // unrolled k: the switch of re_slot compiles to branches here
       const bool fi = i == 4, fj = j == 4;
@@ -1105,7 +1116,7 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
       const int b4 = !fi && !fj ? 96 + j * 4 + i : !fj ? 116 + j : !fi ? 120 + i : 124;
       const int slot = k < 4 ? kb + k * kst : b4;
 #pragma unroll
-      for (int c = 0; c < 3; c++) su[el][c][slot] = -(wjk * r0[c] + wik * r1[c] + wij * r2[c]);
+      for (int c = 0; c < 3; c++) su[el][c][slot] = wjk * r0[c] + wik * r1[c] + wij * r2[c];
     }
   }
   __syncthreads();
@@ -1571,8 +1582,9 @@ void launch_elemforce_3d(hipStream_t s, Lattice lat, const float *kappa, const f
   const int nspec = lat.nx * lat.ny * lat.nz;
   if (nspec <= 0) return;
   const Gll3 g = scaled_gll3(q, xix, ety, gaz, jac);
+  const double inv_nx = 1.0 / lat.nx, inv_nxy = 1.0 / (static_cast<double>(lat.nx) * lat.ny);
   hipLaunchKernelGGL(elemforce_3d_kernel, dim3(blocks(nspec, E3)), dim3(128), 0, s, lat, kappa, mu, g, u, nglob_pad,
-                     re);
+                     inv_nx, inv_nxy, re);
 }
 
 void launch_sources_re(hipStream_t s, int nsrc, int istep, const int *ispec, const float *array, const float *stf,

@@ -994,11 +994,16 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
                                                            const float *__restrict__ uf, int np,
                                                            double inv_nx, double inv_nxy,
                                                            float *__restrict__ re) {
-  __shared__ __align__(16) float su[E3][3][RE_STRIDE];  // displacement, then the forces in slot order
-  // the five distinct stresses sxx, sxy, sxz, syy, syz per point (sxy once: it
-  // is both sigma(x, y) and sigma(y, x)); 20.2 KB per workgroup lets 8
-  // workgroups share a CU's LDS
-  __shared__ float sS[E3][5][125];
+  // LDS: the displacement, then the forces in slot order, at su[el * SU_EL +
+  // c * RE_STRIDE + x]; the five distinct stresses sxx, sxy, sxz, syy, syz (sxy
+  // once: it is both sigma(x, y) and sigma(y, x)) at sS[el * SS_EL + m * 125 + p].
+  // The element strides are padded so that the three elements a wave touches
+  // fall on different LDS banks (392 = 8 and 632 = 24 mod 32, 8 and 56 mod 64):
+  // the row reads of the contractions are then free of bank conflicts.  20.4 KB
+  // per workgroup lets 8 workgroups share a CU's LDS.
+  constexpr int SU_EL = RE_ELEM + 8, SS_EL = 5 * 125 + 7;
+  __shared__ __align__(16) float su[(E3 - 1) * SU_EL + RE_ELEM];
+  __shared__ float sS[(E3 - 1) * SS_EL + 5 * 125];
 
   const int nspec = lat.nx * lat.ny * lat.nz;
   const int t = threadIdx.x;
@@ -1046,7 +1051,7 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         u[k][c] = ub[c][d];
-        su[el][c][k * 25 + col] = u[k][c];
+        su[el * SU_EL + c * RE_STRIDE + k * 25 + col] = u[k][c];
       }
     }
   }
@@ -1062,8 +1067,8 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
       for (int l = 0; l < 5; l++) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          dx[c] += hpi[l] * su[el][c][k * 25 + j * 5 + l];
-          dy[c] += hpj[l] * su[el][c][k * 25 + l * 5 + i];
+          dx[c] += hpi[l] * su[el * SU_EL + c * RE_STRIDE + k * 25 + j * 5 + l];
+          dy[c] += hpj[l] * su[el * SU_EL + c * RE_STRIDE + k * 25 + l * 5 + i];
           dz[c] += q.hz[k][l] * u[l][c];
         }
       }
@@ -1079,11 +1084,11 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
       const float sxz = mu * (dz[0] + dx[2]);
       const float syz = mu * (dz[1] + dy[2]);
       const int p = k * 25 + col;
-      sS[el][0][p] = sxx;
-      sS[el][1][p] = sxy;
-      sS[el][2][p] = sxz;
-      sS[el][3][p] = syy;
-      sS[el][4][p] = syz;
+      sS[el * SS_EL + 0 * 125 + p] = sxx;
+      sS[el * SS_EL + 1 * 125 + p] = sxy;
+      sS[el * SS_EL + 2 * 125 + p] = sxz;
+      sS[el * SS_EL + 3 * 125 + p] = syy;
+      sS[el * SS_EL + 4 * 125 + p] = syz;
       F2[k][0] = sxz;
       F2[k][1] = syz;
       F2[k][2] = szz;
@@ -1100,8 +1105,8 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
 #pragma unroll
         for (int c = 0; c < 3; c++) {
           // row c of sigma along x (sxx, sxy, sxz) and along y (sxy, syy, syz)
-          r0[c] += sS[el][c][k * 25 + j * 5 + l] * hwi[l];
-          r1[c] += sS[el][c == 0 ? 1 : c + 2][k * 25 + l * 5 + i] * hwj[l];
+          r0[c] += sS[el * SS_EL + c * 125 + k * 25 + j * 5 + l] * hwi[l];
+          r1[c] += sS[el * SS_EL + (c == 0 ? 1 : c + 2) * 125 + k * 25 + l * 5 + i] * hwj[l];
           r2[c] += F2[l][c] * q.wz[k][l];
         }
       }
@@ -1116,7 +1121,7 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
       const int b4 = !fi && !fj ? 96 + j * 4 + i : !fj ? 116 + j : !fi ? 120 + i : 124;
       const int slot = k < 4 ? kb + k * kst : b4;
 #pragma unroll
-      for (int c = 0; c < 3; c++) su[el][c][slot] = wjk * r0[c] + wik * r1[c] + wij * r2[c];
+      for (int c = 0; c < 3; c++) su[el * SU_EL + c * RE_STRIDE + slot] = wjk * r0[c] + wik * r1[c] + wij * r2[c];
     }
   }
   __syncthreads();
@@ -1125,8 +1130,11 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
   const int first = blk * E3;
   const int nel = nspec - first < E3 ? nspec - first : E3;
   float4 *dst = reinterpret_cast<float4 *>(re + static_cast<size_t>(first) * RE_ELEM);
-  const float4 *from = reinterpret_cast<const float4 *>(&su[0][0][0]);
-  for (int qv = t; qv < nel * (RE_ELEM / 4); qv += 128) dst[qv] = from[qv];
+  const float4 *from = reinterpret_cast<const float4 *>(su);
+  for (int qv = t; qv < nel * (RE_ELEM / 4); qv += 128) {
+    const int le = qv / (RE_ELEM / 4);
+    dst[qv] = from[le * (SU_EL / 4) + qv - le * (RE_ELEM / 4)];
+  }
 }
 
 // Sum, in a fixed order, of the element forces at lattice point

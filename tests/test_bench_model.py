@@ -1,0 +1,54 @@
+"""The counting models behind bench.py's roofline figures, checked against
+closed forms and against the committed rocprofv3 counter summaries (CPU only).
+
+- Algorithmic bytes of the 3-D gather kernels on the 64^3 cube: the per-element
+  and per-point terms written out, and the survey's 143 B per point-step.
+- FP32 operations of elemforce_3d: 220 per local point; the committed VALU
+  instruction counts (FMA counted twice), scaled to the 125 active lanes of
+  each 128-thread workgroup, agree within 1 %.
+- The committed HBM byte counters: the element-force writes are exactly the
+  1536 B per element of the force blocks, and the measured traffic bench.py
+  reports is FETCH x 2 + WRITE x 1 in KiB.
+"""
+import json
+import os
+
+import bench
+
+N = 64
+NSPEC = N ** 3
+NGLOB = (4 * N + 1) ** 3
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_algorithmic_bytes_3d_closed_form():
+    ab = bench.algorithmic_bytes_3d(NSPEC, NGLOB)
+    assert NGLOB == 16974593
+    assert ab["elemforce"] == NSPEC * 125 * (8 + 12) + NGLOB * 12
+    assert ab["update"] == NSPEC * 125 * 12 + NGLOB * 60
+    assert abs(ab["survey_per_pt_step"] - 143.16) < 0.01
+    assert abs(ab["per_pt_step"] - (ab["elemforce"] + ab["update"]) / NGLOB) < 1e-9
+
+
+def test_elemforce_flops_match_the_counters():
+    flops = bench.elemforce_3d_flops(NSPEC)
+    assert flops == NSPEC * 125 * 220
+    counted = bench.measured_valu_flops("3d_layered_cube_64^3_ngll5", "gather", "elemforce_3d_kernel")
+    assert counted is not None
+    active = counted * 125 / 128
+    assert abs(active / flops - 1) < 0.01
+
+
+def test_committed_byte_counters():
+    with open(os.path.join(ROOT, "profiles", "gather3d_64cube_pmc.json")) as fh:
+        pmc = json.load(fh)
+    assert pmc["scale"] == {"FETCH_SIZE": 2.0, "WRITE_SIZE": 1.0, "basis": pmc["scale"]["basis"]}
+    ef = pmc["kib_per_dispatch"]["elemforce_3d_kernel"]
+    assert ef["WRITE_SIZE"] * 1024 == NSPEC * 1536
+    t = bench.measured_traffic("3d_layered_cube_64^3_ngll5", "gather", "assemble_update_3d_kernel")
+    au = pmc["kib_per_dispatch"]["assemble_update_3d_kernel"]
+    assert t == 1024.0 * (2.0 * au["FETCH_SIZE"] + au["WRITE_SIZE"])
+    # traffic above the algorithmic bytes is over-fetch, below it cache hits; both within 25 %
+    alg = bench.algorithmic_bytes_3d(NSPEC, NGLOB)["update"]
+    assert 0.75 < t / alg < 1.25
+    assert bench.measured_traffic("other_workload", "gather", "assemble_update_3d_kernel") is None

@@ -82,6 +82,7 @@ struct sfx_engine {
   bool initialized = false;
   bool a_stale = false;  // fused gather steps leave the stored acceleration behind
   int nbseg = 0;         // Stacey segments (3-D)
+  bool sources_off_boundary = false;  // 3-D gather: no source element has an absorbing face
   Lattice lat{};
   std::vector<hipEvent_t> ev;
   double timings[4] = {0, 0, 0, 0};
@@ -323,6 +324,13 @@ struct sfx_engine {
       launch_fill(s, re.p, re.n, 0.0f);
       zero.alloc(384);
       launch_fill(s, zero.p, zero.n, 0.0f);
+      // sources and Stacey share one launch when no source element has an
+      // absorbing face (their entries are then disjoint)
+      std::vector<char> absorbing(h.nspec, 0);
+      for (int32_t sl : h.bnd_slot) absorbing[sl / 125] = 1;
+      sources_off_boundary = true;
+      for (int32_t sl : slots_of(h.src_ispec))
+        if (sl < 0 || sl >= h.nspec || absorbing[sl]) sources_off_boundary = false;
     }
     bnd_normal.upload(h.bnd_normal, s);
     bnd_weight.upload(h.bnd_weight, s);
@@ -367,6 +375,11 @@ struct sfx_engine {
     launch_elemforce_3d(stream, lat, kappa.p, mu.p, h.xix3, h.ety3, h.gaz3, h.jac3, q, ff.u, h.nglob_pad, re.p);
   }
   void element_boundary_sources(Fields ff, int istep) {
+    if (sources_off_boundary) {
+      launch_stacey_sources_re(stream, nbseg, bnd_seg.p, bnd_gather.p, bnd_slot.p, bnd_normal.p, bnd_weight.p,
+                               bnd_rvp.p, bnd_rvs.p, ff, h.nsrc, istep, src_ispec.p, src_array.p, stf.p, re.p);
+      return;
+    }
     launch_sources_re(stream, h.nsrc, istep, src_ispec.p, src_array.p, stf.p, re.p);
     launch_stacey_re(stream, nbseg, bnd_seg.p, bnd_gather.p, bnd_slot.p, bnd_normal.p, bnd_weight.p, bnd_rvp.p,
                      bnd_rvs.p, ff, re.p);

@@ -1262,11 +1262,9 @@ __global__ __launch_bounds__(256) void assemble_3d_kernel(Lattice lat, const flo
 
 // One thread per element point adds every source in source order: two
 // sources in one element meet in the same thread, so plain adds suffice.
-__global__ __launch_bounds__(128) void sources_re_kernel(int nsrc, int istep, const int *__restrict__ ispec,
-                                                         const float *__restrict__ array,
-                                                         const float *__restrict__ stf, float *__restrict__ re) {
-  const int p = threadIdx.x;
-  if (p >= 125) return;
+__device__ __forceinline__ void sources_re_point(int p, int nsrc, int istep, const int *__restrict__ ispec,
+                                                 const float *__restrict__ array, const float *__restrict__ stf,
+                                                 float *__restrict__ re) {
   const int slot = re_slot(p % 5, (p / 5) % 5, p / 25);
   for (int s = 0; s < nsrc; s++) {
     const float sv = stf[static_cast<size_t>(istep) * nsrc + s];
@@ -1275,6 +1273,12 @@ __global__ __launch_bounds__(128) void sources_re_kernel(int nsrc, int istep, co
       re[static_cast<size_t>(ispec[s]) * RE_ELEM + c * RE_STRIDE + slot] +=
           sv * array[(static_cast<size_t>(s) * 3 + c) * 125 + p];
   }
+}
+
+__global__ __launch_bounds__(128) void sources_re_kernel(int nsrc, int istep, const int *__restrict__ ispec,
+                                                         const float *__restrict__ array,
+                                                         const float *__restrict__ stf, float *__restrict__ re) {
+  if (threadIdx.x < 125) sources_re_point(threadIdx.x, nsrc, istep, ispec, array, stf, re);
 }
 
 // One thread per element point on the absorbing boundary; its entries (one per
@@ -1286,7 +1290,16 @@ __global__ __launch_bounds__(256) void stacey_re_kernel(int nseg, const int *__r
                                                         const float *__restrict__ weight,
                                                         const float *__restrict__ rho_vp,
                                                         const float *__restrict__ rho_vs, Fields f,
-                                                        float *__restrict__ re) {
+                                                        float *__restrict__ re, int src_block, int nsrc,
+                                                        int istep, const int *__restrict__ src_ispec,
+                                                        const float *__restrict__ src_array,
+                                                        const float *__restrict__ stf) {
+  // the block after the Stacey blocks adds the sources (only launched so when
+  // no source element has an absorbing face: the two touch disjoint entries)
+  if (static_cast<int>(blockIdx.x) == src_block) {
+    if (threadIdx.x < 125) sources_re_point(threadIdx.x, nsrc, istep, src_ispec, src_array, stf, re);
+    return;
+  }
   const int sidx = blockIdx.x * blockDim.x + threadIdx.x;
   if (sidx >= nseg) return;
   const int q0 = seg[sidx], q1 = seg[sidx + 1];
@@ -1606,7 +1619,20 @@ void launch_stacey_re(hipStream_t s, int nseg, const int *seg, const int *gather
                       float *re) {
   if (nseg <= 0) return;
   hipLaunchKernelGGL(stacey_re_kernel, dim3(blocks(nseg, 256)), dim3(256), 0, s, nseg, seg, gather, slot, normal,
-                     weight, rho_vp, rho_vs, f, re);
+                     weight, rho_vp, rho_vs, f, re, -1, 0, 0, nullptr, nullptr, nullptr);
+}
+void launch_stacey_sources_re(hipStream_t s, int nseg, const int *seg, const int *gather, const int *slot,
+                              const float *normal, const float *weight, const float *rho_vp, const float *rho_vs,
+                              Fields f, int nsrc, int istep, const int *src_ispec, const float *src_array,
+                              const float *stf, float *re) {
+  if (nseg <= 0) {
+    launch_sources_re(s, nsrc, istep, src_ispec, src_array, stf, re);
+    return;
+  }
+  const int nb = static_cast<int>(blocks(nseg, 256));
+  const int src_block = nsrc > 0 ? nb : -1;
+  hipLaunchKernelGGL(stacey_re_kernel, dim3(nb + (nsrc > 0 ? 1 : 0)), dim3(256), 0, s, nseg, seg, gather, slot,
+                     normal, weight, rho_vp, rho_vs, f, re, src_block, nsrc, istep, src_ispec, src_array, stf);
 }
 
 void launch_assemble_update_3d(hipStream_t s, Lattice lat, const float *re, const float *zero, Fields f,

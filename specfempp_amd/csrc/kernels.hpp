@@ -102,6 +102,12 @@ void launch_sources_re(hipStream_t s, int nsrc, int istep, const int *ispec, con
 void launch_stacey_re(hipStream_t s, int nseg, const int *seg, const int *gather, const int *slot,
                       const float *normal, const float *weight, const float *rho_vp, const float *rho_vs, Fields f,
                       float *re);
+// Both of the above in one launch (one extra block for the sources); only
+// valid when no source element has an absorbing face.
+void launch_stacey_sources_re(hipStream_t s, int nseg, const int *seg, const int *gather, const int *slot,
+                              const float *normal, const float *weight, const float *rho_vp, const float *rho_vs,
+                              Fields f, int nsrc, int istep, const int *src_ispec, const float *src_array,
+                              const float *stf, float *re);
 // Per lattice point: a = M^-1 sum_e re(e, p); corrector(n) + predictor(n+1) on u, v.
 // zero: an all-zero block of 384 floats read in place of absent contributions.
 void launch_assemble_update_3d(hipStream_t s, Lattice lat, const float *re, const float *zero, Fields f,

@@ -89,3 +89,28 @@ def test_no_sources_no_receivers(dim, scatter):
     assert s.shape == (20, len(TYPES), 0, dim)
     for f in fields:
         assert f.shape[1] > 0 and not np.any(f)
+
+
+def test_3d_source_in_absorbing_element_matches_oracle():
+    """A source in an element with an absorbing face: the gather path then adds
+    sources and Stacey terms in two launches (one launch otherwise)."""
+    from specfempp_amd import Engine3D
+
+    mesh, src, rec, dt, _ = _box_workload()
+    nstep = 300
+    src = [dict(src[0], x=40.0, y=270.0, z=190.0), src[1]]  # first source in an x- face element
+    print("[case] 3d box gather, source in an absorbing element", flush=True)
+    s32, _ = oracle.Oracle3D(mesh, 4, dt).forward(src, rec, nstep, dt, 1, TYPES)
+    s64, _ = oracle.Oracle3D(mesh, 8, dt).forward(src, rec, nstep, dt, 1, TYPES)
+    eng = Engine3D(mesh, src, rec, dt, nstep, 1, TYPES, scatter="gather")
+    eng.initialize()
+    eng.run()
+    ours = eng.seismograms().astype(np.float64)
+    eng.close()
+    for it, t in enumerate(TYPES):
+        e32, e64 = rel_l2(ours[:, it], s32[:, it]), rel_l2(ours[:, it], s64[:, it])
+        floor = rel_l2(s32[:, it], s64[:, it])
+        _report(f"3d_box_src_on_boundary/{t}", vs_oracle_fp32=e32, vs_oracle_fp64=e64, fp32_floor=floor)
+        _check(e32, e64, floor)
+        if t == "displacement":
+            assert e32 <= TOL_SEISMO

@@ -49,18 +49,44 @@ def algorithmic_bytes_block(n: int) -> dict:
             "survey_per_pt_step": survey / nglob}
 
 
-def algorithmic_bytes_3d(nspec: int, nglob: int) -> dict:
+def minv_bytes_3d(nx: int, ny: int, nz: int, absorbing) -> int:
+    """Bytes of M^-1 that assemble_update_3d reads per launch: one component
+    (16 B per row of 4 lattice points) in the tiles without absorbing-face
+    points, where the three components are equal, and all three (48 B per row)
+    in the tiles on an absorbing face (x-, x+, y-, y+, z-, z+ flags); the
+    padding rows of the last tile layer in y and z are not read."""
+    import numpy as np
+
+    TX, TY, TZ = nx + 1, ny + 1, nz + 1
+    ty, tz = np.meshgrid(np.arange(TY), np.arange(TZ), indexing="ij")
+    rows_j = np.where(ty == ny, 1, 4)
+    rows_k = np.where(tz == nz, 1, 4)
+    rows = (rows_j * rows_k).astype(np.int64)  # rows per tile, the same for every tx
+    on_face_yz = (((absorbing[2] and 1) and (ty == 0)) | ((absorbing[3] and 1) and (ty == ny)) |
+                  ((absorbing[4] and 1) and (tz == 0)) | ((absorbing[5] and 1) and (tz == nz)))
+    total = 0
+    for tx in range(TX):
+        on_face = on_face_yz | bool((absorbing[0] and tx == 0) or (absorbing[1] and tx == nx))
+        total += int(np.sum(np.where(on_face, 48, 16) * rows))
+    return total
+
+
+def algorithmic_bytes_3d(nspec: int, nglob: int, minv_bytes: int = None) -> dict:
     """Bytes each kernel of the 3-D gather path must move per launch (FP32):
       elemforce_3d:       kappa + mu per local point (8 B x 125 / element), the
                           displacement once per global point (12 B), the element
                           forces stored once (12 B x 125 / element)
       assemble_update_3d: the element forces loaded once (12 B x 125 / element),
-                          u, v, M^-1 read (36 B) and u, v written (24 B) per point
+                          u, v read (24 B) and u, v written (24 B) per point, and
+                          M^-1 (minv_bytes, see minv_bytes_3d; 12 B per point
+                          when not given)
     and the survey's data-model figure for a whole step (143 B per point-step:
     gather index + kappa + mu per local point, displacement gather, acceleration
     read-modify-write, fused update read u v a M^-1 / write u v a)."""
+    if minv_bytes is None:
+        minv_bytes = nglob * 12
     elem = nspec * 125 * 8 + nglob * 12 + nspec * 125 * 12
-    upd = nspec * 125 * 12 + nglob * 60
+    upd = nspec * 125 * 12 + nglob * 48 + minv_bytes
     survey = nspec * 125 * 12 + nglob * 36 + nglob * 84
     return {"elemforce": elem, "update": upd, "per_pt_step": (elem + upd) / nglob,
             "survey_per_pt_step": survey / nglob}
@@ -414,7 +440,7 @@ def main() -> int:
         kern = {"block_step_3d_kernel": (ab["block_step_3d_kernel"], tm["stiffness_ms"]),
                 "surface_update_3d_kernel": (ab["surface_update_3d_kernel"], tm["update_ms"])}
     else:
-        ab = algorithmic_bytes_3d(nspec, nglob)
+        ab = algorithmic_bytes_3d(nspec, nglob, minv_bytes_3d(args.n, args.n, args.n, w.mesh["absorbing"]))
         kern = {"elemforce_3d_kernel": (ab["elemforce"], tm["stiffness_ms"]),
                 "assemble_update_3d_kernel": (ab["update"], tm["update_ms"])}
     dominant = max(kern, key=lambda k: kern[k][1])

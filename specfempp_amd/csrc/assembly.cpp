@@ -849,6 +849,8 @@ HostAssembly build_assembly_3d(const sfx_mesh3d &m, int nsrc, const sfx_source *
   a.nx = m.nx;
   a.ny = m.ny;
   a.nz = m.nz;
+  for (int f = 0; f < 6; f++)
+    if (m.absorbing[f]) a.absorb_mask |= 1 << f;
   const int64_t nspec64 = static_cast<int64_t>(m.nx) * m.ny * m.nz;
   if (nspec64 > (1 << 27)) throw std::invalid_argument("3-D mesh too large");
   a.nspec = static_cast<int>(nspec64);

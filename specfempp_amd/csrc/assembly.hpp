@@ -72,6 +72,7 @@ struct HostAssembly {
   // 3-D structured sizes; device fields in lattice order, lattice_of maps the
   // reference numbering (iglob) to the lattice index
   int nx = 0, ny = 0, nz = 0;
+  int absorb_mask = 0;                   // bit f: face f (x-, x+, y-, y+, z-, z+) is absorbing
   int nint_pad = 0;                      // block layout: slots of the block-interior points
   std::vector<int32_t> lattice_of;
   // 2-D locator data (control nodes (nspec, ngnod, 2), GLL point coordinates)

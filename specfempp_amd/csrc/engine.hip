@@ -83,6 +83,7 @@ struct sfx_engine {
   bool a_stale = false;  // fused gather steps leave the stored acceleration behind
   int nbseg = 0;         // Stacey segments (3-D)
   bool sources_off_boundary = false;  // 3-D gather: no source element has an absorbing face
+  DevBuf<float> minv1;                // 3-D gather: M^-1 of one component (nglob_pad)
   Lattice lat{};
   std::vector<hipEvent_t> ev;
   double timings[4] = {0, 0, 0, 0};
@@ -324,6 +325,25 @@ struct sfx_engine {
       launch_fill(s, re.p, re.n, 0.0f);
       zero.alloc(384);
       launch_fill(s, zero.p, zero.n, 0.0f);
+      // M^-1 of one component for the tiles without absorbing-face points,
+      // where the three components are equal
+      {
+        const size_t np = h.nglob_pad;
+        std::vector<float> m1(h.mass_inverse.begin(), h.mass_inverse.begin() + np);
+        for (int tz = 0; tz < lat.TZ; tz++)
+          for (int ty = 0; ty < lat.TY; ty++)
+            for (int tx = 0; tx < lat.TX; tx++) {
+              const int am = h.absorb_mask;
+              if (((am & 1) && tx == 0) || ((am & 2) && tx == h.nx) || ((am & 4) && ty == 0) ||
+                  ((am & 8) && ty == h.ny) || ((am & 16) && tz == 0) || ((am & 32) && tz == h.nz))
+                continue;
+              const size_t t0 = ((static_cast<size_t>(tz) * lat.TY + ty) * lat.TX + tx) * 64;
+              for (size_t q = t0; q < t0 + 64; q++)
+                if (h.mass_inverse[np + q] != m1[q] || h.mass_inverse[2 * np + q] != m1[q])
+                  throw std::logic_error("internal: M^-1 components differ off the absorbing faces");
+            }
+        minv1.upload(m1, s);
+      }
       // sources and Stacey share one launch when no source element has an
       // absorbing face (their entries are then disjoint)
       std::vector<char> absorbing(h.nspec, 0);
@@ -490,7 +510,7 @@ struct sfx_engine {
                               nc.dto2, seismo.p);
       }
       if (evs) SFX_HIP(hipEventRecord(evs[3], stream));
-      launch_assemble_update_3d(stream, lat, re.p, zero.p, ff, nc);
+      launch_assemble_update_3d(stream, lat, re.p, zero.p, ff, nc, minv1.p, h.absorb_mask);
       if (evs) SFX_HIP(hipEventRecord(evs[4], stream));
       a_stale = true;
       return;

@@ -1181,7 +1181,8 @@ __device__ __forceinline__ bool tiled_point(const Lattice &lat, int d, int &tx, 
 // load is unconditional and the thread issues them all before the first wait.
 __global__ __launch_bounds__(256) void assemble_update_3d_kernel(Lattice lat, const float *__restrict__ re,
                                                                  const float *__restrict__ zero, Fields f,
-                                                                 NewmarkConst nc) {
+                                                                 NewmarkConst nc,
+                                                                 const float *__restrict__ minv1, int amask) {
   const int nrows = lat.TX * lat.TY * lat.TZ * 16;
   const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= nrows) return;
@@ -1203,10 +1204,23 @@ __global__ __launch_bounds__(256) void assemble_update_3d_kernel(Lattice lat, co
   const float *r_xyz = elem(j0 && k0, tx - 1, ty - 1, tz - 1);
   const int n4 = f.nglob_pad / 4;
   float4 m[3], u[3], v[3], acc[3];
+  // M^-1 is the same for the three components except at absorbing-face
+  // points (the Stacey mass term): tiles holding such points read all three,
+  // the others one shared value
+  const bool absorbing_tile = ((amask & 1) && tx == 0) || ((amask & 2) && tx == lat.nx) ||
+                              ((amask & 4) && ty == 0) || ((amask & 8) && ty == lat.ny) ||
+                              ((amask & 16) && tz == 0) || ((amask & 32) && tz == lat.nz);
+  if (absorbing_tile) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) m[c] = reinterpret_cast<const float4 *>(f.minv)[static_cast<size_t>(c) * n4 + row];
+  } else {
+    m[0] = reinterpret_cast<const float4 *>(minv1)[row];
+    m[1] = m[0];
+    m[2] = m[0];
+  }
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     const size_t o = static_cast<size_t>(c) * n4 + row;
-    m[c] = reinterpret_cast<const float4 *>(f.minv)[o];
     u[c] = reinterpret_cast<const float4 *>(f.u)[o];
     v[c] = reinterpret_cast<const float4 *>(f.v)[o];
   }
@@ -1636,9 +1650,10 @@ void launch_stacey_sources_re(hipStream_t s, int nseg, const int *seg, const int
 }
 
 void launch_assemble_update_3d(hipStream_t s, Lattice lat, const float *re, const float *zero, Fields f,
-                               NewmarkConst nc) {
+                               NewmarkConst nc, const float *minv1, int amask) {
   const size_t n = static_cast<size_t>(lat.TX) * lat.TY * lat.TZ * 16;
-  hipLaunchKernelGGL(assemble_update_3d_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, lat, re, zero, f, nc);
+  hipLaunchKernelGGL(assemble_update_3d_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, lat, re, zero, f, nc, minv1,
+                     amask);
 }
 
 void launch_assemble_3d(hipStream_t s, Lattice lat, const float *re, Fields f, int divide) {

@@ -110,8 +110,10 @@ void launch_stacey_sources_re(hipStream_t s, int nseg, const int *seg, const int
                               const float *stf, float *re);
 // Per lattice point: a = M^-1 sum_e re(e, p); corrector(n) + predictor(n+1) on u, v.
 // zero: an all-zero block of 384 floats read in place of absent contributions.
+// minv1: M^-1 of one component (equal for all three off the absorbing faces);
+// amask: bit f set when face f (x-, x+, y-, y+, z-, z+) is absorbing.
 void launch_assemble_update_3d(hipStream_t s, Lattice lat, const float *re, const float *zero, Fields f,
-                               NewmarkConst nc);
+                               NewmarkConst nc, const float *minv1, int amask);
 // a = sum_e re(e, p) (times M^-1 when divide) for the seam path and read-out.
 void launch_assemble_3d(hipStream_t s, Lattice lat, const float *re, Fields f, int divide);
 // Seismograms sampled before the fused update: a from re, v corrected.

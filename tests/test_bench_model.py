@@ -2,7 +2,8 @@
 closed forms and against the committed rocprofv3 counter summaries (CPU only).
 
 - Algorithmic bytes of the 3-D gather kernels on the 64^3 cube: the per-element
-  and per-point terms written out, and the survey's 143 B per point-step.
+  and per-point terms written out, and the survey's 143 B per point-step; the
+  M^-1 bytes of the update against a brute-force count of the rows it reads.
 - FP32 operations of elemforce_3d: 220 per local point; the committed VALU
   instruction counts (FMA counted twice), scaled to the 125 active lanes of
   each 128-thread workgroup, agree within 1 %.
@@ -12,6 +13,8 @@ closed forms and against the committed rocprofv3 counter summaries (CPU only).
 """
 import json
 import os
+
+import pytest
 
 import bench
 
@@ -28,6 +31,27 @@ def test_algorithmic_bytes_3d_closed_form():
     assert ab["update"] == NSPEC * 125 * 12 + NGLOB * 60
     assert abs(ab["survey_per_pt_step"] - 143.16) < 0.01
     assert abs(ab["per_pt_step"] - (ab["elemforce"] + ab["update"]) / NGLOB) < 1e-9
+
+
+def _minv_bytes_brute(nx, ny, nz, absorbing):
+    total = 0
+    for tz in range(nz + 1):
+        for ty in range(ny + 1):
+            for tx in range(nx + 1):
+                face = ((absorbing[0] and tx == 0) or (absorbing[1] and tx == nx) or (absorbing[2] and ty == 0)
+                        or (absorbing[3] and ty == ny) or (absorbing[4] and tz == 0) or (absorbing[5] and tz == nz))
+                for k in range(4):
+                    for j in range(4):
+                        if (ty == ny and j > 0) or (tz == nz and k > 0):
+                            continue
+                        total += 48 if face else 16
+    return total
+
+
+@pytest.mark.parametrize("dims", [(3, 2, 4), (5, 3, 7), (1, 1, 1)])
+@pytest.mark.parametrize("absorbing", [(1, 1, 1, 1, 1, 0), (0,) * 6, (1,) * 6, (0, 1, 0, 1, 1, 0)])
+def test_minv_bytes_match_the_rows_the_update_reads(dims, absorbing):
+    assert bench.minv_bytes_3d(*dims, list(absorbing)) == _minv_bytes_brute(*dims, absorbing)
 
 
 def test_elemforce_flops_match_the_counters():
@@ -49,6 +73,6 @@ def test_committed_byte_counters():
     au = pmc["kib_per_dispatch"]["assemble_update_3d_kernel"]
     assert t == 1024.0 * (2.0 * au["FETCH_SIZE"] + au["WRITE_SIZE"])
     # traffic above the algorithmic bytes is over-fetch, below it cache hits; both within 25 %
-    alg = bench.algorithmic_bytes_3d(NSPEC, NGLOB)["update"]
+    alg = bench.algorithmic_bytes_3d(NSPEC, NGLOB, bench.minv_bytes_3d(N, N, N, [1, 1, 1, 1, 1, 0]))["update"]
     assert 0.75 < t / alg < 1.25
     assert bench.measured_traffic("other_workload", "gather", "assemble_update_3d_kernel") is None

@@ -506,8 +506,10 @@ struct sfx_engine {
       if (istep % run.nstep_between_samples == 0) {
         const int isig = istep / run.nstep_between_samples;
         if (isig >= nsig) throw std::invalid_argument("seismogram step out of range");
+        bool need_a = false;
+        for (int it = 0; it < run.nseismo_types; it++) need_a = need_a || run.seismo_types[it] != SFX_DISPLACEMENT;
         launch_seismograms_re(stream, lat, h.nrec, run.nseismo_types, types.p, isig, rec_ispec.p, rec_h.p, re.p, ff,
-                              nc.dto2, seismo.p);
+                              nc.dto2, seismo.p, need_a);
       }
       if (evs) SFX_HIP(hipEventRecord(evs[3], stream));
       launch_assemble_update_3d(stream, lat, re.p, zero.p, ff, nc, minv1.p, h.absorb_mask);

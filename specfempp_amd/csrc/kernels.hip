@@ -1351,7 +1351,7 @@ __global__ __launch_bounds__(128) void seismogram_re_kernel(Lattice lat, int nre
                                                             const int *__restrict__ ispec,
                                                             const float *__restrict__ h,
                                                             const float *__restrict__ re, Fields f, float dto2,
-                                                            float *__restrict__ seismo) {
+                                                            float *__restrict__ seismo, int need_a) {
   __shared__ float red[3][2];
   const int r = blockIdx.x;
   const int p = threadIdx.x;
@@ -1365,7 +1365,8 @@ __global__ __launch_bounds__(128) void seismogram_re_kernel(Lattice lat, int nre
     const int i = p % 5, j = (p / 5) % 5, k = p / 25;
     L = tiled_index(lat, ex, ey, ez, i, j, k);
     hw = h[static_cast<size_t>(r) * 125 + p];
-    assemble_tiled(lat, re, ex + (i >> 2), ey + (j >> 2), ez + (k >> 2), i & 3, j & 3, k & 3, acc);
+    // the assembled acceleration only for velocity and acceleration traces
+    if (need_a) assemble_tiled(lat, re, ex + (i >> 2), ey + (j >> 2), ez + (k >> 2), i & 3, j & 3, k & 3, acc);
   }
   for (int it = 0; it < ntypes; it++) {
     const int type = types[it];
@@ -1375,8 +1376,12 @@ __global__ __launch_bounds__(128) void seismogram_re_kernel(Lattice lat, int nre
       float x = 0.0f;
       if (p < 125) {
         const size_t o = static_cast<size_t>(c) * np + L;
-        const float a = acc[c] * f.minv[o];
-        x = type == 0 ? f.u[o] : (type == 1 ? f.v[o] + dto2 * a : a);
+        if (type == 0) {
+          x = f.u[o];
+        } else {
+          const float a = acc[c] * f.minv[o];
+          x = type == 1 ? f.v[o] + dto2 * a : a;
+        }
       }
       val[c] = hw * x;
     }
@@ -1662,10 +1667,11 @@ void launch_assemble_3d(hipStream_t s, Lattice lat, const float *re, Fields f, i
 }
 
 void launch_seismograms_re(hipStream_t s, Lattice lat, int nrec, int ntypes, const int *types, int isig,
-                           const int *ispec, const float *h, const float *re, Fields f, float dto2, float *seismo) {
+                           const int *ispec, const float *h, const float *re, Fields f, float dto2, float *seismo,
+                           bool need_a) {
   if (nrec <= 0 || ntypes <= 0) return;
   hipLaunchKernelGGL(seismogram_re_kernel, dim3(nrec), dim3(128), 0, s, lat, nrec, ntypes, types, isig, ispec, h, re, f,
-                     dto2, seismo);
+                     dto2, seismo, need_a ? 1 : 0);
 }
 
 }  // namespace sfx

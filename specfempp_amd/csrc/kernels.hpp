@@ -116,9 +116,11 @@ void launch_assemble_update_3d(hipStream_t s, Lattice lat, const float *re, cons
                                NewmarkConst nc, const float *minv1, int amask);
 // a = sum_e re(e, p) (times M^-1 when divide) for the seam path and read-out.
 void launch_assemble_3d(hipStream_t s, Lattice lat, const float *re, Fields f, int divide);
-// Seismograms sampled before the fused update: a from re, v corrected.
+// Seismograms sampled before the fused update: a from re, v corrected
+// (need_a: some trace is a velocity or an acceleration).
 void launch_seismograms_re(hipStream_t s, Lattice lat, int nrec, int ntypes, const int *types, int isig,
-                           const int *ispec, const float *h, const float *re, Fields f, float dto2, float *seismo);
+                           const int *ispec, const float *h, const float *re, Fields f, float dto2, float *seismo,
+                           bool need_a);
 
 // ---- 3-D structured path with block assembly (SFX_SCATTER_BLOCK) ----
 // Workgroup = 2 x 2 x 2 elements (nx, ny, nz even).  Block-interior points are

@@ -502,14 +502,22 @@ struct sfx_engine {
       if (evs) SFX_HIP(hipEventRecord(evs[1], stream));
       element_forces(ff, istep);
       if (evs) SFX_HIP(hipEventRecord(evs[2], stream));
-      element_boundary_sources(ff, istep);
-      if (istep % run.nstep_between_samples == 0) {
-        const int isig = istep / run.nstep_between_samples;
-        if (isig >= nsig) throw std::invalid_argument("seismogram step out of range");
-        bool need_a = false;
-        for (int it = 0; it < run.nseismo_types; it++) need_a = need_a || run.seismo_types[it] != SFX_DISPLACEMENT;
-        launch_seismograms_re(stream, lat, h.nrec, run.nseismo_types, types.p, isig, rec_ispec.p, rec_h.p, re.p, ff,
-                              nc.dto2, seismo.p, need_a);
+      const bool sample = istep % run.nstep_between_samples == 0;
+      const int isig = istep / run.nstep_between_samples;
+      if (sample && isig >= nsig) throw std::invalid_argument("seismogram step out of range");
+      bool need_a = false;
+      for (int it = 0; it < run.nseismo_types; it++) need_a = need_a || run.seismo_types[it] != SFX_DISPLACEMENT;
+      if (sample && !need_a && sources_off_boundary) {
+        // Stacey, sources and displacement samples in one launch
+        launch_stacey_sources_seismo_re(stream, nbseg, bnd_seg.p, bnd_gather.p, bnd_slot.p, bnd_normal.p,
+                                        bnd_weight.p, bnd_rvp.p, bnd_rvs.p, ff, h.nsrc, istep, src_ispec.p,
+                                        src_array.p, stf.p, re.p, lat, h.nrec, run.nseismo_types, types.p, isig,
+                                        rec_ispec.p, rec_h.p, nc.dto2, seismo.p);
+      } else {
+        element_boundary_sources(ff, istep);
+        if (sample)
+          launch_seismograms_re(stream, lat, h.nrec, run.nseismo_types, types.p, isig, rec_ispec.p, rec_h.p, re.p, ff,
+                                nc.dto2, seismo.p, need_a);
       }
       if (evs) SFX_HIP(hipEventRecord(evs[3], stream));
       launch_assemble_update_3d(stream, lat, re.p, zero.p, ff, nc, minv1.p, h.absorb_mask);

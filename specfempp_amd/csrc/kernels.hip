@@ -1295,6 +1295,63 @@ __global__ __launch_bounds__(128) void sources_re_kernel(int nsrc, int istep, co
   if (threadIdx.x < 125) sources_re_point(threadIdx.x, nsrc, istep, ispec, array, stf, re);
 }
 
+// Seismogram samples of receiver r by one workgroup (its first 125 threads
+// hold the element's points; a workgroup of 128 or 256 threads): Lagrange
+// interpolation of u, v or a at the receiver, reduced over the workgroup.
+__device__ __forceinline__ void seismogram_re_block(int r, Lattice lat, int nrec, int ntypes,
+                                                    const int *__restrict__ types, int isig,
+                                                    const int *__restrict__ ispec, const float *__restrict__ h,
+                                                    const float *__restrict__ re, Fields f, float dto2,
+                                                    float *__restrict__ seismo, int need_a, float (*red)[2]) {
+  const int p = threadIdx.x;
+  const int np = f.nglob_pad;
+  float hw = 0.0f;
+  int L = 0;
+  float acc[3] = {0, 0, 0};
+  if (p < 125) {
+    const int e = ispec[r];
+    const int ex = e % lat.nx, ey = (e / lat.nx) % lat.ny, ez = e / (lat.nx * lat.ny);
+    const int i = p % 5, j = (p / 5) % 5, k = p / 25;
+    L = tiled_index(lat, ex, ey, ez, i, j, k);
+    hw = h[static_cast<size_t>(r) * 125 + p];
+    // the assembled acceleration only for velocity and acceleration traces
+    if (need_a) assemble_tiled(lat, re, ex + (i >> 2), ey + (j >> 2), ez + (k >> 2), i & 3, j & 3, k & 3, acc);
+  }
+  for (int it = 0; it < ntypes; it++) {
+    const int type = types[it];
+    float val[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      float x = 0.0f;
+      if (p < 125) {
+        const size_t o = static_cast<size_t>(c) * np + L;
+        if (type == 0) {
+          x = f.u[o];
+        } else {
+          const float a = acc[c] * f.minv[o];
+          x = type == 1 ? f.v[o] + dto2 * a : a;
+        }
+      }
+      val[c] = hw * x;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      float s = val[c];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+      // the 125 points are in the first two waves; later waves hold zeros
+      if ((p & 63) == 0 && p < 128) red[c][p >> 6] = s;
+    }
+    __syncthreads();
+    if (p == 0) {
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        seismo[((static_cast<size_t>(isig) * ntypes + it) * nrec + r) * 3 + c] = red[c][0] + red[c][1];
+    }
+    __syncthreads();
+  }
+}
+
 // One thread per element point on the absorbing boundary; its entries (one per
 // absorbing face through the point) are summed in entry order.
 __global__ __launch_bounds__(256) void stacey_re_kernel(int nseg, const int *__restrict__ seg,
@@ -1307,9 +1364,22 @@ __global__ __launch_bounds__(256) void stacey_re_kernel(int nseg, const int *__r
                                                         float *__restrict__ re, int src_block, int nsrc,
                                                         int istep, const int *__restrict__ src_ispec,
                                                         const float *__restrict__ src_array,
-                                                        const float *__restrict__ stf) {
-  // the block after the Stacey blocks adds the sources (only launched so when
-  // no source element has an absorbing face: the two touch disjoint entries)
+                                                        const float *__restrict__ stf, int seis_first,
+                                                        Lattice lat, int nrec, int ntypes,
+                                                        const int *__restrict__ types, int isig,
+                                                        const int *__restrict__ rec_ispec,
+                                                        const float *__restrict__ rec_h, float dto2,
+                                                        float *__restrict__ seismo) {
+  // after the Stacey blocks: one block adding the sources (only launched so
+  // when no source element has an absorbing face: the two touch disjoint
+  // entries), then one block per receiver for displacement traces, which read
+  // only u
+  if (seis_first >= 0 && static_cast<int>(blockIdx.x) >= seis_first) {
+    __shared__ float red[3][2];
+    seismogram_re_block(blockIdx.x - seis_first, lat, nrec, ntypes, types, isig, rec_ispec, rec_h, re, f, dto2,
+                        seismo, 0, red);
+    return;
+  }
   if (static_cast<int>(blockIdx.x) == src_block) {
     if (threadIdx.x < 125) sources_re_point(threadIdx.x, nsrc, istep, src_ispec, src_array, stf, re);
     return;
@@ -1353,53 +1423,7 @@ __global__ __launch_bounds__(128) void seismogram_re_kernel(Lattice lat, int nre
                                                             const float *__restrict__ re, Fields f, float dto2,
                                                             float *__restrict__ seismo, int need_a) {
   __shared__ float red[3][2];
-  const int r = blockIdx.x;
-  const int p = threadIdx.x;
-  const int np = f.nglob_pad;
-  float hw = 0.0f;
-  int L = 0;
-  float acc[3] = {0, 0, 0};
-  if (p < 125) {
-    const int e = ispec[r];
-    const int ex = e % lat.nx, ey = (e / lat.nx) % lat.ny, ez = e / (lat.nx * lat.ny);
-    const int i = p % 5, j = (p / 5) % 5, k = p / 25;
-    L = tiled_index(lat, ex, ey, ez, i, j, k);
-    hw = h[static_cast<size_t>(r) * 125 + p];
-    // the assembled acceleration only for velocity and acceleration traces
-    if (need_a) assemble_tiled(lat, re, ex + (i >> 2), ey + (j >> 2), ez + (k >> 2), i & 3, j & 3, k & 3, acc);
-  }
-  for (int it = 0; it < ntypes; it++) {
-    const int type = types[it];
-    float val[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      float x = 0.0f;
-      if (p < 125) {
-        const size_t o = static_cast<size_t>(c) * np + L;
-        if (type == 0) {
-          x = f.u[o];
-        } else {
-          const float a = acc[c] * f.minv[o];
-          x = type == 1 ? f.v[o] + dto2 * a : a;
-        }
-      }
-      val[c] = hw * x;
-    }
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      float s = val[c];
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-      if ((p & 63) == 0) red[c][p >> 6] = s;
-    }
-    __syncthreads();
-    if (p == 0) {
-#pragma unroll
-      for (int c = 0; c < 3; c++)
-        seismo[((static_cast<size_t>(isig) * ntypes + it) * nrec + r) * 3 + c] = red[c][0] + red[c][1];
-    }
-    __syncthreads();
-  }
+  seismogram_re_block(blockIdx.x, lat, nrec, ntypes, types, isig, ispec, h, re, f, dto2, seismo, need_a, red);
 }
 
 __global__ void step_tick_kernel(int *step, int delta) {
@@ -1638,7 +1662,8 @@ void launch_stacey_re(hipStream_t s, int nseg, const int *seg, const int *gather
                       float *re) {
   if (nseg <= 0) return;
   hipLaunchKernelGGL(stacey_re_kernel, dim3(blocks(nseg, 256)), dim3(256), 0, s, nseg, seg, gather, slot, normal,
-                     weight, rho_vp, rho_vs, f, re, -1, 0, 0, nullptr, nullptr, nullptr);
+                     weight, rho_vp, rho_vs, f, re, -1, 0, 0, nullptr, nullptr, nullptr, -1, Lattice{}, 0, 0, nullptr,
+                     0, nullptr, nullptr, 0.0f, nullptr);
 }
 void launch_stacey_sources_re(hipStream_t s, int nseg, const int *seg, const int *gather, const int *slot,
                               const float *normal, const float *weight, const float *rho_vp, const float *rho_vs,
@@ -1651,7 +1676,24 @@ void launch_stacey_sources_re(hipStream_t s, int nseg, const int *seg, const int
   const int nb = static_cast<int>(blocks(nseg, 256));
   const int src_block = nsrc > 0 ? nb : -1;
   hipLaunchKernelGGL(stacey_re_kernel, dim3(nb + (nsrc > 0 ? 1 : 0)), dim3(256), 0, s, nseg, seg, gather, slot,
-                     normal, weight, rho_vp, rho_vs, f, re, src_block, nsrc, istep, src_ispec, src_array, stf);
+                     normal, weight, rho_vp, rho_vs, f, re, src_block, nsrc, istep, src_ispec, src_array, stf, -1,
+                     Lattice{}, 0, 0, nullptr, 0, nullptr, nullptr, 0.0f, nullptr);
+}
+void launch_stacey_sources_seismo_re(hipStream_t s, int nseg, const int *seg, const int *gather, const int *slot,
+                                     const float *normal, const float *weight, const float *rho_vp,
+                                     const float *rho_vs, Fields f, int nsrc, int istep, const int *src_ispec,
+                                     const float *src_array, const float *stf, float *re, Lattice lat, int nrec,
+                                     int ntypes, const int *types, int isig, const int *rec_ispec, const float *rec_h,
+                                     float dto2, float *seismo) {
+  const int nb = nseg > 0 ? static_cast<int>(blocks(nseg, 256)) : 0;
+  const int src_block = nsrc > 0 ? nb : -1;
+  const int seis_first = nb + (nsrc > 0 ? 1 : 0);
+  const int nrec_blocks = nrec > 0 && ntypes > 0 ? nrec : 0;
+  if (seis_first + nrec_blocks == 0) return;
+  hipLaunchKernelGGL(stacey_re_kernel, dim3(seis_first + nrec_blocks), dim3(256), 0, s, nseg, seg, gather, slot,
+                     normal, weight, rho_vp, rho_vs, f, re, src_block, nsrc, istep, src_ispec, src_array, stf,
+                     nrec_blocks > 0 ? seis_first : -1, lat, nrec, ntypes, types, isig, rec_ispec, rec_h, dto2,
+                     seismo);
 }
 
 void launch_assemble_update_3d(hipStream_t s, Lattice lat, const float *re, const float *zero, Fields f,

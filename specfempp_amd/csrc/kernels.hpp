@@ -108,6 +108,14 @@ void launch_stacey_sources_re(hipStream_t s, int nseg, const int *seg, const int
                               const float *normal, const float *weight, const float *rho_vp, const float *rho_vs,
                               Fields f, int nsrc, int istep, const int *src_ispec, const float *src_array,
                               const float *stf, float *re);
+// The same plus one block per receiver for displacement traces (they read only
+// u, so they may run beside the Stacey and source blocks).
+void launch_stacey_sources_seismo_re(hipStream_t s, int nseg, const int *seg, const int *gather, const int *slot,
+                                     const float *normal, const float *weight, const float *rho_vp,
+                                     const float *rho_vs, Fields f, int nsrc, int istep, const int *src_ispec,
+                                     const float *src_array, const float *stf, float *re, Lattice lat, int nrec,
+                                     int ntypes, const int *types, int isig, const int *rec_ispec, const float *rec_h,
+                                     float dto2, float *seismo);
 // Per lattice point: a = M^-1 sum_e re(e, p); corrector(n) + predictor(n+1) on u, v.
 // zero: an all-zero block of 384 floats read in place of absent contributions.
 // minv1: M^-1 of one component (equal for all three off the absorbing faces);

@@ -133,3 +133,31 @@ def test_3d_displacement_only_traces_equal_those_of_a_full_run():
         eng.close()
     assert np.abs(runs[0]).max() > 0
     assert np.array_equal(runs[0][:, 0], runs[1][:, 0])
+
+
+@pytest.mark.parametrize("nsrc,nrec", [(2, 0), (0, 8), (0, 0)])
+def test_3d_displacement_sampling_launch_with_missing_sources_or_receivers(nsrc, nrec):
+    """Displacement traces are taken in the Stacey/sources launch: without
+    receivers the wavefield equals that of a run with velocity traces (which
+    takes the separate launches), and without sources everything stays zero."""
+    from specfempp_amd import Engine3D
+
+    mesh, src, rec, dt, _ = _box_workload()
+    src, rec = src[:nsrc], rec[:nrec]
+    nstep = 60
+    print(f"[case] 3d box gather, displacement sampling launch, nsrc={nsrc} nrec={nrec}", flush=True)
+    out = []
+    for types in (("displacement",), ("displacement", "velocity")):
+        eng = Engine3D(mesh, src, rec, dt, nstep, 1, types, scatter="gather")
+        eng.initialize()
+        eng.run()
+        out.append(([eng.field(k) for k in TYPES], eng.seismograms()))
+        eng.close()
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a, b)
+    assert out[0][1].shape == (nstep, 1, nrec, 3)
+    assert np.array_equal(out[0][1][:, 0], out[1][1][:, 0])
+    if nsrc == 0:
+        assert not np.any(out[0][0][0]) and not np.any(out[0][1])
+    else:
+        assert np.abs(out[0][0][0]).max() > 0

@@ -440,7 +440,10 @@ def main() -> int:
         kern = {"block_step_3d_kernel": (ab["block_step_3d_kernel"], tm["stiffness_ms"]),
                 "surface_update_3d_kernel": (ab["surface_update_3d_kernel"], tm["update_ms"])}
     else:
-        ab = algorithmic_bytes_3d(nspec, nglob, minv_bytes_3d(args.n, args.n, args.n, w.mesh["absorbing"]))
+        # the GATHER update reads one M^-1 component off the absorbing faces; the
+        # colored and atomic updates read all three
+        minv = minv_bytes_3d(args.n, args.n, args.n, w.mesh["absorbing"]) if scatter == "gather" else None
+        ab = algorithmic_bytes_3d(nspec, nglob, minv)
         kern = {"elemforce_3d_kernel": (ab["elemforce"], tm["stiffness_ms"]),
                 "assemble_update_3d_kernel": (ab["update"], tm["update_ms"])}
     dominant = max(kern, key=lambda k: kern[k][1])

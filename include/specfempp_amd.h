@@ -126,9 +126,7 @@ typedef struct {
  * 2 x 2 x 2 elements, sums their forces in on-chip memory in a fixed order and
  * updates the points inside the block directly; only the forces on block
  * faces pass through device memory to a second, small update kernel
- * (deterministic, no atomics).  Device engines refuse it for now (a GPU memory
- * fault in its step-by-step calls is not yet explained); host-only engines
- * build its layout. */
+ * (deterministic, no atomics). */
 #define SFX_SCATTER_BLOCK 3
 
 /* ---- library ---------------------------------------------------------- */

@@ -47,13 +47,30 @@ def build(verbose: bool = False) -> str:
     return LIB_PATH
 
 
+def stale_sources():
+    """Sources (and the header) newer than the built library."""
+    if not os.path.exists(LIB_PATH):
+        return []
+    built = os.path.getmtime(LIB_PATH)
+    paths = [os.path.join(CSRC, f) for f in os.listdir(CSRC)] + [HEADER]
+    return sorted(os.path.basename(p) for p in paths if os.path.getmtime(p) > built)
+
+
 def load():
-    """Load the built library (building it first only if it is absent)."""
+    """Load the built library (building it first only if it is absent).  A
+    library older than its sources is loaded as it is, with a warning: run
+    build() (``__graft_entry__.build()``) after changing a source."""
     global _lib
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
         build()
+    stale = stale_sources()
+    if stale:
+        import warnings
+
+        warnings.warn(f"{LIB_PATH} is older than {', '.join(stale)}: rebuild with specfempp_amd.build()",
+                      RuntimeWarning, stacklevel=2)
     lib = C.CDLL(LIB_PATH)
     vp, i32, i64, dp = C.c_void_p, C.c_int, C.c_int64, C.c_void_p
     sig = {

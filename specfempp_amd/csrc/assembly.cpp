@@ -496,6 +496,9 @@ HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *
   check_run(run);
   if (run.scatter == SFX_SCATTER_BLOCK) throw std::invalid_argument("SFX_SCATTER_BLOCK is for 3-D meshes");
   if (m.nspec <= 0 || m.npgeo <= 0) throw std::invalid_argument("empty mesh");
+  // element points are addressed as e * 25 + p in 32-bit device indices
+  if (static_cast<int64_t>(m.nspec) * 25 > (int64_t(1) << 31) - 1)
+    throw std::invalid_argument("2-D mesh too large: nspec x 25 points exceed 32-bit device indexing");
   if (m.ngnod != 4 && m.ngnod != 9) throw std::invalid_argument("ngnod must be 4 or 9");
   if (!m.coorg || !m.knods || !m.material || !m.rho || !m.vp || !m.vs) throw std::invalid_argument("null mesh array");
   for (int i = 0; i < m.nmaterials; i++)
@@ -548,6 +551,9 @@ HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *
         pz[p] = static_cast<float>(z);
       }
   number_points_2d(a, px, pz, run.chunk_size);
+  // a field component is addressed as c * nglob_pad + point (c < 2)
+  if (2 * ((static_cast<int64_t>(a.nglob) + 63) / 64 * 64) > (int64_t(1) << 31) - 1)
+    throw std::invalid_argument("2-D mesh too large: 2 x nglob exceeds 32-bit device indexing");
   a.nglob_pad = (a.nglob + 63) / 64 * 64;
   a.gather.resize(nloc);
   for (size_t p = 0; p < nloc; p++) a.gather[p] = a.field_index[a.index_mapping[p]];
@@ -829,6 +835,16 @@ void locate_structured(double x, double x0, double h, int n, int &e, double &xi)
 
 }  // namespace
 
+void check_index_range_3d(int64_t nglob_pad, int64_t nspec) {
+  const int64_t lim = (int64_t(1) << 31) - 1;
+  if (3 * nglob_pad > lim)
+    throw std::invalid_argument("3-D mesh too large: 3 x " + std::to_string(nglob_pad) +
+                                " padded field points exceed 32-bit device indexing");
+  if (nspec * 125 > lim)
+    throw std::invalid_argument("3-D mesh too large: " + std::to_string(nspec) +
+                                " elements x 125 points exceed 32-bit device indexing");
+}
+
 HostAssembly build_assembly_3d(const sfx_mesh3d &m, int nsrc, const sfx_source *src, int nrec,
                                const sfx_receiver *rec, const sfx_run_desc &run) {
   check_run(run);
@@ -852,7 +868,17 @@ HostAssembly build_assembly_3d(const sfx_mesh3d &m, int nsrc, const sfx_source *
   for (int f = 0; f < 6; f++)
     if (m.absorbing[f]) a.absorb_mask |= 1 << f;
   const int64_t nspec64 = static_cast<int64_t>(m.nx) * m.ny * m.nz;
-  if (nspec64 > (1 << 27)) throw std::invalid_argument("3-D mesh too large");
+  // Kernels address a field component as c * nglob_pad + point and an element
+  // point as e * 125 + p in 32-bit integers: both must stay below 2^31.  The
+  // check runs before any host array of the mesh's size is allocated.
+  const bool block_layout = run.scatter == SFX_SCATTER_BLOCK;
+  const int64_t TX = m.nx + 1, TY = m.ny + 1, TZ = m.nz + 1;
+  const int nbx = m.nx / 2, nby = m.ny / 2, nbz = m.nz / 2;
+  const int64_t nint64 = (static_cast<int64_t>(nbx) * nby * nbz * 343 + 63) / 64 * 64;
+  const int64_t pad64 =
+      block_layout ? (nint64 + static_cast<int64_t>(nbx + 1) * (nby + 1) * (nbz + 1) * OWNER_STRIDE + 63) / 64 * 64
+                   : TX * TY * TZ * 64;
+  check_index_range_3d(pad64, nspec64);
   a.nspec = static_cast<int>(nspec64);
   a.nspec_none = a.nspec;
   const Gll g = make_gll();
@@ -871,7 +897,6 @@ HostAssembly build_assembly_3d(const sfx_mesh3d &m, int nsrc, const sfx_source *
   const int64_t LX = 4 * static_cast<int64_t>(m.nx) + 1, LY = 4 * static_cast<int64_t>(m.ny) + 1,
                 LZ = 4 * static_cast<int64_t>(m.nz) + 1;
   const int64_t nlat = LX * LY * LZ;
-  if (nlat > (int64_t(1) << 31) - 1) throw std::invalid_argument("3-D mesh too large");
   std::vector<int32_t> first(nlat, -1);
   const size_t nloc = static_cast<size_t>(nspec) * npe;
   a.index_mapping.resize(nloc);
@@ -902,16 +927,11 @@ HostAssembly build_assembly_3d(const sfx_mesh3d &m, int nsrc, const sfx_source *
   // The block path (SFX_SCATTER_BLOCK) stores the fields in the block layout of
   // lattice.hpp instead: block interiors contiguous, then the surface points
   // grouped by owner block.
-  const int64_t TX = m.nx + 1, TY = m.ny + 1, TZ = m.nz + 1;
-  if (TX * TY * TZ * 64 > (int64_t(1) << 31) - 1) throw std::invalid_argument("3-D mesh too large");
-  const bool block_layout = run.scatter == SFX_SCATTER_BLOCK;
-  const int nbx = m.nx / 2, nby = m.ny / 2, nbz = m.nz / 2;
   if (block_layout) {
-    a.nint_pad = (static_cast<int>(static_cast<int64_t>(nbx) * nby * nbz * 343) + 63) / 64 * 64;
-    const int64_t pad = a.nint_pad + static_cast<int64_t>(nbx + 1) * (nby + 1) * (nbz + 1) * OWNER_STRIDE;
-    a.nglob_pad = static_cast<int>((pad + 63) / 64 * 64);
+    a.nint_pad = static_cast<int>(nint64);
+    a.nglob_pad = static_cast<int>(pad64);
   } else {
-    a.nglob_pad = static_cast<int>(TX * TY * TZ * 64);
+    a.nglob_pad = static_cast<int>(pad64);
   }
   auto tiled = [&](int64_t I, int64_t J, int64_t K) {
     if (block_layout)

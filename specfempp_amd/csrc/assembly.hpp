@@ -105,6 +105,9 @@ void place_adjoint_sources_2d(HostAssembly &a, int nadj, const sfx_receiver *pos
 // Throw std::runtime_error / std::invalid_argument on bad input.
 HostAssembly build_assembly_2d(const sfx_mesh2d &mesh, int nsrc, const sfx_source *src, int nrec,
                                const sfx_receiver *rec, const sfx_run_desc &run);
+// Refuses 3-D meshes whose padded field size (3 components) or element
+// point index (125 per element) would overflow 32-bit device indices.
+void check_index_range_3d(int64_t nglob_pad, int64_t nspec);
 HostAssembly build_assembly_3d(const sfx_mesh3d &mesh, int nsrc, const sfx_source *src, int nrec,
                                const sfx_receiver *rec, const sfx_run_desc &run);
 

@@ -83,10 +83,12 @@ __global__ __launch_bounds__(256) void block_step_3d_kernel(
   const int t = threadIdx.x;
   const int np = f.nglob_pad;
 
-  // thread = (element el, column (i, j)) of the force phases, points k = 0..4
+  // thread = (element el, column (i, j)) of the force phases, points k = 0..4.
+  // Threads 200..255 only load and update points; they still read the
+  // quadrature rows of a valid column (t % 25), so every index stays < 5.
   const bool active = t < 200;
   const int el = active ? t / 25 : 0;
-  const int col = t - el * 25;
+  const int col = active ? t - el * 25 : t % 25;
   const int i = col % 5, j = col / 5;
   const int dx = el & 1, dy = (el >> 1) & 1, dz = el >> 2;
   const int e = ((2 * bz + dz) * lat.ny + 2 * by + dy) * lat.nx + 2 * bx + dx;

@@ -687,11 +687,6 @@ template <class Build> int create(Build &&build, const sfx_run_desc *run, sfx_en
   return guarded([&] {
     if (!run || !out) throw std::invalid_argument("null argument");
     *out = nullptr;
-    // The 3-D block path is switched off on the device: its step-by-step seam
-    // calls ended in a GPU memory fault whose cause is not yet known.  Its host
-    // assembly (block layout) is still built for host-only engines.
-    if (run->scatter == SFX_SCATTER_BLOCK && !host_only)
-      throw std::invalid_argument("SFX_SCATTER_BLOCK is switched off on the device; use SFX_SCATTER_GATHER");
     auto e = std::make_unique<sfx_engine>();
     e->run = *run;
     e->h = build();

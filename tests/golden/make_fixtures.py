@@ -131,6 +131,38 @@ def main(ref: str) -> None:
             }, fh, indent=1)
         manifest[name] = f"{UT}/{rel}"
 
+    # Geometry and point-location data: the database shared by the
+    # partial-derivative test and the locate_point test (the two files are
+    # byte-identical), SPECFEM2D's xix, xiz, gammax, gammaz and jacobian
+    # ((nspec, ngllz, ngllx) row-major, mesh order; compute_partial_derivatives_tests.cpp)
+    # and the locate_point known answers (algorithms/locate.cpp:37-58: mesh
+    # element, xi, gamma of five points; ispec within 0, xi and gamma within
+    # 1e-4, and the inverse map within 1e-2 m).
+    rel = "compute/partial_derivatives/serial"
+    src_dir = os.path.join(ref, UT, rel)
+    out = os.path.join(HERE, "geometry_80x60")
+    os.makedirs(out, exist_ok=True)
+    db = os.path.join(src_dir, "Database00000.bin")
+    with open(db, "rb") as f1, open(os.path.join(ref, UT, "algorithms/serial/database.bin"), "rb") as f2:
+        assert f1.read() == f2.read(), "the two databases differ"
+    gz_copy(db, os.path.join(out, "database.bin.gz"))
+    arrays = {k: read_fortran_array(os.path.join(src_dir, f"{k}_00000.bin"), "<f8").reshape(-1, 5, 5)
+              for k in ("xix", "xiz", "gammax", "gammaz", "jacobian")}
+    np.savez_compressed(os.path.join(out, "partial_derivatives.npz"), **arrays)
+    with open(os.path.join(out, "case.json"), "w") as fh:
+        json.dump({"source_dir": f"{UT}/{rel}", "also": f"{UT}/algorithms/serial/database.bin",
+                   "partial_derivatives_layout": "(nspec, ngllz, ngllx) row-major, mesh order",
+                   "partial_derivatives_tolerance": 1e-2,
+                   "locate_known_answers": {
+                       "from": f"{UT}/algorithms/locate.cpp:37-58",
+                       "points": [[606.313, 957.341], [1835.050, 146.444], [1539.578, 242.743],
+                                  [3497.449, 2865.483], [3018.139, 530.283]],
+                       "mesh_ispec": [1452, 196, 350, 4549, 700],
+                       "xi": [-0.7475, 0.4020, 0.5831, 0.8980, -0.2744],
+                       "gamma": [0.2905, 0.05832, -0.2247, 0.4248, 0.3945],
+                       "tolerance_local": 1e-4, "tolerance_global": 1e-2}}, fh, indent=1)
+    manifest["geometry_80x60"] = f"{UT}/{rel} + {UT}/algorithms/serial"
+
     # source-file reader data (tests/unit-tests/io/sources)
     rel = "io/sources"
     out = os.path.join(HERE, "io_sources")

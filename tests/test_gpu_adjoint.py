@@ -168,3 +168,112 @@ def test_autograd_gradient_matches_oracle_and_finite_differences():
         rel = abs(pred - fd) / abs(fd)
         _report(f"autograd/fd_{name}", fd=fd, autograd=pred, rel=rel)
         assert rel <= 1e-3
+
+
+# ---- configuration 5, reduced: the adjoint workload at 100^2 -----------------
+# Same 20 m elements, 400 m checkerboard (+-5 %), 8 seeded sources, 100
+# receivers, dt 5e-4 and checkpoints every 55 steps as the full 200^2 x 3000
+# configuration; 100^2 elements and 1500 steps keep the FP32/FP64 oracle runs
+# (in parallel worker processes) within about a minute.
+C5_N, C5_NSTEP, C5_SNAP, C5_EPS = 100, 1500, 55, 1e-4
+
+
+@pytest.fixture(scope="module")
+def config5_reduced():
+    import _oracle_jobs as jobs
+
+    true, start = meshes.config5(n=C5_N, nstep=C5_NSTEP)
+    args = (start.sources, start.receivers, C5_NSTEP, start.dt)
+    with jobs.pool(2) as ex:
+        f_data = ex.submit(jobs.forward_2d, true.mesh, 8, *args)
+        f_start = ex.submit(jobs.forward_2d, start.mesh, 8, *args)
+        data = f_data.result()[:, 0]
+        s64 = f_start.result()[:, 0]
+    # one adjoint source for engine and oracle alike: the FP64 residual at the
+    # receivers, (nrec, nstep, 2) in forward time order
+    stf = np.ascontiguousarray(np.transpose(s64 - data, (1, 0, 2)))
+    return true, start, data, stf
+
+
+def _stacey_ring_mask(n):
+    """Points of elements off the absorbing ring (all four sides absorb).  The
+    reference's Fréchet kernels carry no derivative of the Stacey impedances
+    or of the mass matrix's absorbing term (frechet_derivative.hpp:14-138), so
+    the finite-difference directions leave the boundary elements alone."""
+    ez, ex = np.divmod(np.arange(n * n), n)
+    inner = (ex > 0) & (ex < n - 1) & (ez > 0) & (ez < n - 1)
+    return np.broadcast_to(inner[:, None, None], (n * n, 5, 5))
+
+
+def test_config5_reduced_kernels_match_oracle_and_finite_differences(config5_reduced):
+    """Forward record, backward seismograms and all six kernels against the
+    FP32/FP64 oracle (time_marching.tpp:126-247, frechet_derivative.hpp:14-138);
+    then the torch.autograd gradient of the FP64 L2 misfit against central
+    finite differences of the FP64 oracle's misfit along 4 of the 16 seeded
+    random perturbations (N(0, 1) per GLL point on rho, mu, kappa; seeds 5.1 to
+    5.4; eps = 1e-4 relative), bar 1e-3."""
+    import torch
+
+    import _oracle_jobs as jobs
+    from specfempp_amd import Engine2D
+    from specfempp_amd.autograd import SeismogramModel
+
+    true, start, data, stf = config5_reduced
+    dt, n = start.dt, C5_N
+    print(f"[case] config 5 reduced {n}^2 x {C5_NSTEP} steps, snapshots every {C5_SNAP}", flush=True)
+    o64 = oracle.Oracle2D(start.mesh, 8, 1, dt)
+    rho0, kap0, mu0, _ = o64.properties()
+    mask = _stacey_ring_mask(n)
+    dirs = []
+    for i in range(1, 5):
+        rng = np.random.default_rng([5, i])
+        dirs.append([rng.standard_normal(rho0.shape) * mask for _ in range(3)])  # rho, mu, kappa
+    args = (start.sources, start.receivers, C5_NSTEP, dt)
+    with jobs.pool(min(10, jobs.host_workers())) as ex:
+        f_adj = {prec: ex.submit(jobs.adjoint_2d, start.mesh, prec, *args, stf, C5_SNAP) for prec in (4, 8)}
+        f_fd = []
+        for d_rho, d_mu, d_kap in dirs:
+            for sgn in (1.0, -1.0):
+                props = (rho0 * (1 + sgn * C5_EPS * d_rho), kap0 * (1 + sgn * C5_EPS * d_kap),
+                         mu0 * (1 + sgn * C5_EPS * d_mu))
+                f_fd.append(ex.submit(jobs.misfit_2d, start.mesh, *args, data, props))
+
+        # engine: kernels for the same adjoint source
+        eng = Engine2D(start.mesh, start.sources, start.receivers, dt, C5_NSTEP, 1, TYPES, scatter="colored")
+        eng.initialize()
+        eng.forward_record(C5_SNAP)
+        fwd = eng.seismograms().astype(np.float64)
+        eng.set_adjoint_sources(start.receivers, stf)
+        eng.adjoint_run()
+        bwd = eng.seismograms().astype(np.float64)
+        kern = eng.kernels()
+        # engine: gradient of the misfit through torch.autograd
+        model = SeismogramModel(eng, start.receivers, snapshot_every=C5_SNAP)
+        params = [torch.tensor(x, dtype=torch.float64, requires_grad=True) for x in (rho0, kap0, mu0)]
+        u = model(*params)
+        loss = 0.5 * torch.sum((u - torch.from_numpy(data)) ** 2) * dt
+        loss.backward()
+        g_rho, g_kap, g_mu = [p.grad.numpy() for p in params]
+        eng.close()
+
+        (s32, k32, b32), (s64, k64, b64) = f_adj[4].result(), f_adj[8].result()
+        e32 = rel_l2(fwd, s32)
+        _report("config5_reduced/forward", vs_oracle_fp32=e32, vs_oracle_fp64=rel_l2(fwd, s64))
+        assert e32 <= 1e-5
+        e32, e64, floor = rel_l2(bwd, b32), rel_l2(bwd, b64), rel_l2(b32, b64)
+        _report("config5_reduced/backward_seismograms", vs_oracle_fp32=e32, vs_oracle_fp64=e64, fp32_floor=floor)
+        _check(e32, e64, floor)
+        for k, name in enumerate(NAMES):
+            ours = kern[name].astype(np.float64)
+            e32, e64, floor = rel_l2(ours, k32[k]), rel_l2(ours, k64[k]), rel_l2(k32[k], k64[k])
+            _report(f"config5_reduced/K_{name}", vs_oracle_fp32=e32, vs_oracle_fp64=e64, fp32_floor=floor)
+            _check(e32, e64, floor)
+            assert e64 <= 1e-4  # the north-star bar against the reference kernels (FP64 oracle)
+
+        J = [f.result() for f in f_fd]
+    for i, (d_rho, d_mu, d_kap) in enumerate(dirs):
+        fd = (J[2 * i] - J[2 * i + 1]) / 2.0
+        pred = C5_EPS * float(np.sum(g_rho * rho0 * d_rho + g_kap * kap0 * d_kap + g_mu * mu0 * d_mu))
+        rel = abs(pred - fd) / abs(fd)
+        _report(f"config5_reduced/fd_seed5.{i + 1}", fd=fd, autograd=pred, rel=rel)
+        assert rel <= 1e-3

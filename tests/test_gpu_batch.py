@@ -97,3 +97,40 @@ def test_batch_of_host_only_engines(alone):
     for i in range(len(ws)):
         assert rel_l2(b.seismograms(i).astype(np.float64), ref[i]) <= 1e-6
     b.close()
+
+
+def test_batch_at_the_configuration_extremes_matches_oracle():
+    """Members at the batched configuration's extremes (40^2 quads for 2400
+    steps, 240^2 quads for 800 steps; h 50 m, vp ~ U[2000, 4000], one Ricker
+    force of f0 ~ U[5, 20] Hz, 32 receivers, dt at Courant 0.4) in one batch
+    with a mid-size member: each against the FP32 oracle (<= 1e-5) and its own
+    engine run alone (<= 1e-6).  The oracle runs proceed in worker processes
+    while the GPU runs."""
+    import _oracle_jobs as jobs
+    from specfempp_amd import Batch
+
+    ws = (meshes.config4(nsim=1, seed=41, nmin=40, nmax=40, nstep_range=(2400, 2400))
+          + meshes.config4(nsim=1, seed=42, nmin=240, nmax=240, nstep_range=(800, 800))
+          + meshes.config4(nsim=1, seed=43, nmin=120, nmax=120, nstep_range=(1500, 1500)))
+    print("[case] batch extremes: " + ", ".join(f"{w.name} x {w.nstep}" for w in ws), flush=True)
+    with jobs.pool(len(ws)) as ex:
+        fut = [ex.submit(jobs.forward_2d, w.mesh, 4, w.sources, w.receivers, w.nstep, w.dt, TYPES) for w in ws]
+        alone = []
+        for e in _engines(ws):
+            e.initialize()
+            e.run()
+            alone.append(e.seismograms().astype(np.float64))
+            e.close()
+        b = Batch(_engines(ws))
+        choice = b.tune(3)
+        b.run()
+        got = [b.seismograms(i).astype(np.float64) for i in range(len(ws))]
+        b.close()
+        ref = [f.result() for f in fut]
+    for i, w in enumerate(ws):
+        e32 = rel_l2(got[i][:, 0], ref[i][:, 0])
+        e_alone = rel_l2(got[i], alone[i])
+        print(f"[batch] extremes {choice} {w.name} x {w.nstep}: vs FP32 oracle {e32:.2e}, vs alone {e_alone:.2e}",
+              flush=True)
+        assert e32 <= 1e-5
+        assert e_alone <= 1e-6

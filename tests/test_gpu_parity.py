@@ -23,11 +23,6 @@ from conftest import GOLDEN, load_case, rel_l1_per_station, rel_l2
 
 pytestmark = pytest.mark.gpu
 
-# The 3-D block path is switched off on the device (an unexplained GPU memory
-# fault in its seam-call test); its tests stay skipped until the cause is found.
-BLOCK_OFF = pytest.mark.skip(reason="3-D block path switched off: its seam-call test ended in a GPU memory fault "
-                                    "whose cause has not been found")
-
 TOL_SEISMO = 1e-5
 
 
@@ -177,8 +172,8 @@ def test_config1_homogeneous_80x80(scatter):
 
 # The Ricker wavelet (f0 = 5 Hz) peaks at step 240 (t0 = -1.2 / f0); windows
 # reach well past it so the traces are signal, not rounding noise.
-@pytest.mark.parametrize("n,nstep,scatter", [pytest.param(6, 600, "block", marks=BLOCK_OFF),
-                                            pytest.param(10, 500, "block", marks=BLOCK_OFF), (6, 600, "gather"),
+@pytest.mark.parametrize("n,nstep,scatter", [(6, 600, "block"),
+                                            (10, 500, "block"), (6, 600, "gather"),
                                             (10, 500, "gather"), (6, 600, "colored"), (6, 600, "atomic")])
 def test_3d_layered_matches_oracle(n, nstep, scatter):
     from specfempp_amd import Engine3D, meshes
@@ -204,7 +199,7 @@ def test_3d_layered_matches_oracle(n, nstep, scatter):
             assert e32 <= TOL_SEISMO
 
 
-@pytest.mark.parametrize("n,scatter", [(5, "gather"), pytest.param(6, "block", marks=BLOCK_OFF)])
+@pytest.mark.parametrize("n,scatter", [(5, "gather"), (6, "block")])
 def test_3d_seam_path_and_fields_match_fused_path(n, scatter):
     """Seam calls (predictor / update_wavefields / corrector / seismograms) on
     the 3-D gather and block paths equal the fused step, and the wavefield read
@@ -265,7 +260,6 @@ def _box_workload():
     return mesh, src, rec, 1e-3, 400
 
 
-@BLOCK_OFF
 def test_3d_block_box_all_absorbing_matches_oracle():
     from specfempp_amd import Engine3D
 
@@ -293,7 +287,6 @@ def test_3d_block_box_all_absorbing_matches_oracle():
             assert e32 <= TOL_SEISMO
 
 
-@BLOCK_OFF
 def test_3d_block_step_calls_equal_run_bitwise():
     """Stepping one call per step stores the full acceleration after every
     step; the displacement and velocity updates are the same instructions, so
@@ -361,3 +354,63 @@ def test_pressure_seismograms_match_oracle():
     _report("pressure_linear_field", max_rel=err, fp32_oracle_max_rel=floor)
     assert err <= max(1e-4, 2.0 * floor)
     eng.close()
+
+
+@pytest.mark.parametrize("scatter", ["colored", "gather"])
+def test_2d_moment_tensor_source_matches_oracle(scatter):
+    """A 2-D moment-tensor source (moment_tensor_source.cpp:18-140: M . grad
+    of the Lagrange interpolants, element-averaged derivatives of the
+    reference coordinates) next to a force source, with a time shift, on a
+    non-square mesh with Stacey on three sides."""
+    from specfempp_amd import Engine2D, meshes
+
+    nx, nz, h = 30, 24, 20.0
+    mesh = meshes.quad_mesh(nx, nz, 0.0, nx * h, 0.0, nz * h, materials=[meshes.elastic(2300.0, 2800.0, 1600.0)],
+                            absorbing=("bottom", "right", "left"))
+    src = [dict(type="moment_tensor", x=217.3, z=251.9, mxx=0.8, mzz=-0.45, mxz=0.35, f0=25.0, factor=1e12,
+                tshift=0.004),
+           dict(x=401.0, z=150.0, angle=35.0, f0=25.0, factor=1e10, tshift=0.0)]
+    rec = [dict(x=x, z=z) for x, z in ((80.5, 400.2), (300.0, 420.0), (517.3, 90.1), (250.0, 250.0))]
+    dt, nstep = meshes.stable_dt(h, 2800.0), 900
+    print(f"[case] 2-D moment tensor + force, {nx}x{nz}, {nstep} steps, scatter={scatter}", flush=True)
+    e = Engine2D(mesh, src, rec, dt, nstep, 1, ("displacement", "velocity"), scatter=scatter)
+    e.initialize()
+    e.run()
+    ours = e.seismograms().astype(np.float64)
+    e.close()
+    s32, _ = oracle.Oracle2D(mesh, 4, 1, dt).forward(src, rec, nstep, dt, 1, ("displacement", "velocity"))
+    s64, _ = oracle.Oracle2D(mesh, 8, 1, dt).forward(src, rec, nstep, dt, 1, ("displacement", "velocity"))
+    for it, t in enumerate(("displacement", "velocity")):
+        e32, e64, floor = rel_l2(ours[:, it], s32[:, it]), rel_l2(ours[:, it], s64[:, it]), rel_l2(s32[:, it],
+                                                                                                 s64[:, it])
+        _report(f"2d_moment_tensor_{scatter}/{t}", vs_oracle_fp32=e32, vs_oracle_fp64=e64, fp32_floor=floor)
+        assert e32 <= TOL_SEISMO
+
+
+@pytest.mark.parametrize("chunk", [2, 64])
+def test_storage_chunk_numbering_and_seismograms_match_oracle(chunk):
+    """Numbering at storage_chunk_size > 1 (chunk_config.hpp:12-28: 64 on HIP
+    builds, 32 on CUDA): index mapping, per-medium numbering and locations
+    bit-exact against the oracle run with the same chunk, and the forward run
+    (whose field layout follows that numbering) within the seismogram bar."""
+    from specfempp_amd import Engine2D
+
+    mesh, cfg = load_case("elastic_stacey_80x60")
+    print(f"[case] elastic_stacey_80x60 chunk={chunk}", flush=True)
+    eng = Engine2D(mesh, cfg["sources"], cfg["stations"], cfg["dt"], cfg["nstep"], 1, ("displacement",),
+                   chunk_size=chunk)
+    orc = oracle.Oracle2D(mesh, 4, chunk, cfg["dt"])
+    ibool, midx, c2m = orc.numbering()
+    assert np.array_equal(eng.index_mapping(), ibool)
+    assert np.array_equal(eng.field_index(), midx)
+    assert np.array_equal(eng.compute_to_mesh(), c2m)
+    ispec, _ = eng.locations("receivers")
+    assert [orc.locate(p["x"], p["z"])[0] for p in cfg["stations"]] == list(ispec)
+    eng.initialize()
+    eng.run()
+    ours = eng.seismograms().astype(np.float64)
+    eng.close()
+    s32, _ = orc.forward(cfg["sources"], cfg["stations"], cfg["nstep"], cfg["dt"], 1, ("displacement",))
+    e32 = rel_l2(ours, s32)
+    _report(f"chunk{chunk}", vs_oracle_fp32=e32)
+    assert e32 <= TOL_SEISMO

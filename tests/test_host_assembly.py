@@ -88,12 +88,10 @@ def test_targets_on_gll_points_of_a_regular_mesh():
     _compare(w.mesh, w.sources, pts, w.dt)
 
 
-def test_3d_block_path_is_off_on_the_device_and_host_assembly_still_builds():
-    from specfempp_amd import Engine3D, SfxError
+def test_3d_block_path_host_assembly_numbers_like_the_oracle():
+    from specfempp_amd import Engine3D
 
     w = meshes.config3(n=4, nstep=4, h=100.0, nrec_side=2)
-    with pytest.raises(SfxError, match="switched off"):
-        Engine3D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, scatter="block")
     eng = Engine3D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, scatter="block", host_only=True)
     assert np.array_equal(eng.index_mapping(), oracle.Oracle3D(w.mesh, 8, w.dt).ibool())
     eng.close()
@@ -110,3 +108,19 @@ def test_host_only_engine_reports_the_run_sizes():
                  host_only=True)
     assert e.nsig == 10 and e.nrec == len(cfg["stations"]) and e.nsrc == len(cfg["sources"])
     e.close()
+
+
+@pytest.mark.parametrize("chunk", [2, 32, 64])
+def test_storage_chunk_numbering_matches_oracle(chunk):
+    """Host assembly numbering at storage_chunk_size > 1 (chunk_config.hpp:12-28)
+    equals the oracle's on the reference's 80 x 60 Stacey mesh."""
+    from specfempp_amd import Engine2D
+
+    mesh, cfg = load_case("elastic_stacey_80x60")
+    eng = Engine2D(mesh, cfg["sources"], cfg["stations"], cfg["dt"], 4, 1, ("displacement",), chunk_size=chunk,
+                   host_only=True)
+    ibool, midx, c2m = oracle.Oracle2D(mesh, 4, chunk, cfg["dt"]).numbering()
+    assert np.array_equal(eng.index_mapping(), ibool)
+    assert np.array_equal(eng.field_index(), midx)
+    assert np.array_equal(eng.compute_to_mesh(), c2m)
+    eng.close()

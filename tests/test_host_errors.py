@@ -66,3 +66,22 @@ def test_3d_inputs_are_checked():
     eng = Engine3D(_mesh3d(), SRC3, REC3, 1e-3, 10, host_only=True)
     assert eng.nglob == 9 ** 3
     eng.close()
+
+
+def test_3d_mesh_beyond_32bit_device_indexing_is_refused():
+    """224^3 elements: 3 x nglob_pad = 3 x 225^3 x 64 > 2^31 - 1.  The guard
+    runs before any array of the mesh's size is allocated (assembly.cpp
+    check_index_range_3d), so the refusal is immediate."""
+    n = 224
+    mesh = meshes.hex_mesh(n, n, n, 100.0, np.zeros((n, n, n), dtype=np.int32), [2500.0], [3000.0], [1730.0])
+    with pytest.raises(SfxError, match="3-D mesh too large"):
+        Engine3D(mesh, SRC3, REC3, 1e-3, 10, 1, ("displacement",), host_only=True)
+
+
+def test_2d_mesh_beyond_32bit_device_indexing_is_refused():
+    """nspec x 25 element points above 2^31 - 1 is refused before the element
+    arrays are read."""
+    mesh = _mesh2d()
+    mesh.nspec = 90_000_000
+    with pytest.raises(SfxError, match="2-D mesh too large"):
+        Engine2D(mesh, SRC2, REC2, 1e-3, 10, 1, ("displacement",), host_only=True)

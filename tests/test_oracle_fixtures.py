@@ -63,3 +63,69 @@ def test_forward_traces_match_specfem2d(case):
         # a dropped or sign-flipped term would be far outside this
         if k == "d":
             assert err.max() < 1e-4
+
+
+def _geometry_case():
+    import json
+
+    from conftest import GOLDEN, load_case
+
+    mesh, cfg = load_case("geometry_80x60")
+    ref = np.load(os.path.join(GOLDEN, "geometry_80x60", "partial_derivatives.npz"))
+    return mesh, cfg, ref
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+def test_partial_derivatives_match_specfem2d(precision):
+    """xix, xiz, gammax, gammaz and the jacobian of every GLL point of the
+    reference's 80 x 60 test database against SPECFEM2D's arrays
+    (tests/unit-tests/compute/partial_derivatives/compute_partial_derivatives_tests.cpp:
+    absolute 1e-2 per value).  The restatement is also held to a relative bar:
+    the largest error relative to the largest magnitude of its pair (xix with
+    xiz, gammax with gammaz; xiz and gammax vanish on rectangles).  FP64 agrees
+    to rounding (~1e-13).  FP32 sums shape-function derivatives times float
+    control-node coordinates as the reference does in type_real
+    (src/jacobian/jacobian.cpp, the ThreadVectorRange reductions): 4 km
+    coordinates against 50 m elements cancel to ~3e-5 relative, 3e-2 absolute
+    on jacobians near 940, so the fixture's absolute 1e-2 holds for FP64 and
+    the relative bar 1e-4 for FP32."""
+    mesh, cfg, ref = _geometry_case()
+    o = oracle.Oracle2D(mesh, precision, 1, 1e-3)
+    _, arrs = o.geometry()
+    _, _, c2m = o.numbering()
+    rel_bar = 1e-12 if precision == 8 else 1e-4
+    scale = {"xix": ("xix", "xiz"), "xiz": ("xix", "xiz"), "gammax": ("gammax", "gammaz"),
+             "gammaz": ("gammax", "gammaz"), "jacobian": ("jacobian",)}
+    for name, ours in zip(("xix", "xiz", "gammax", "gammaz", "jacobian"), arrs):
+        expect = ref[name][c2m]  # fixture is in mesh order, the oracle's in compute order
+        err = np.abs(ours - expect).max()
+        rel = err / max(np.abs(ref[k]).max() for k in scale[name])
+        print(f"[geometry] fp{8 * precision} {name}: max abs {err:.3e}, rel {rel:.3e}")
+        if precision == 8:
+            assert err < cfg["partial_derivatives_tolerance"]
+        assert rel < rel_bar
+
+
+def test_locate_point_known_answers():
+    """The five points of tests/unit-tests/algorithms/locate.cpp:37-58 on the
+    same database: the element (mesh numbering) exactly, xi and gamma within
+    1e-4, from the oracle and from the library's host assembly."""
+    from specfempp_amd import Engine2D
+
+    mesh, cfg, _ = _geometry_case()
+    kat = cfg["locate_known_answers"]
+    o = oracle.Oracle2D(mesh, 4, 1, 1e-3)
+    _, _, c2m = o.numbering()
+    pts = [dict(x=x, z=z) for x, z in kat["points"]]
+    eng = Engine2D(mesh, [dict(x=2000.0, z=1500.0, f0=10.0, factor=1.0)], pts, 1e-3, 2, host_only=True)
+    ispec, loc = eng.locations("receivers")
+    eng.close()
+    for k, p in enumerate(pts):
+        e, xi, ga = o.locate(p["x"], p["z"])
+        print(f"[locate] {p}: oracle mesh element {c2m[e]} xi {xi:.5f} gamma {ga:.5f}; "
+              f"known {kat['mesh_ispec'][k]} {kat['xi'][k]} {kat['gamma'][k]}")
+        assert c2m[e] == kat["mesh_ispec"][k]
+        assert abs(xi - kat["xi"][k]) <= kat["tolerance_local"]
+        assert abs(ga - kat["gamma"][k]) <= kat["tolerance_local"]
+        assert ispec[k] == e
+        assert abs(loc[k, 0] - xi) < 1e-7 and abs(loc[k, 1] - ga) < 1e-7

@@ -188,6 +188,63 @@ int sfx_update_wavefields(sfx_engine *engine, int istep);
 int sfx_corrector(sfx_engine *engine);
 int sfx_compute_seismograms(sfx_engine *engine, int isig);
 
+/* ---- wavefield, medium and direction seams -------------------------------------
+ * The calls above act on the forward wavefield of the elastic medium.  These
+ * take the three selectors the reference's interfaces carry, so that an adapter
+ * can implement every pure virtual of specfem::time_scheme::time_scheme
+ * (include/timescheme/timescheme.hpp:173-207) and drive
+ * time_marching<combined>::run (include/solver/time_marching.tpp:126-247)
+ * seam by seam:
+ *   medium     specfem::element::medium_tag.  Phases of a medium with no
+ *              element in the mesh update nothing and return 0, as the
+ *              reference's do (newmark.tpp:11-116 loops over an empty field).
+ *   wavefield  specfem::wavefield::simulation_field: forward, adjoint,
+ *              backward, buffer (the forward state read back,
+ *              time_marching.tpp:201-209).  2-D engines hold all four (the
+ *              last three are allocated at first use and share the forward
+ *              field's mass matrix); 3-D engines hold the forward field.
+ *   direction  newmark<forward>/<combined> phases: forward steps with +dt,
+ *              backward with -dt (newmark.tpp:161-242).
+ * sfx_predictor_phase / sfx_corrector_phase: apply_{predictor,corrector}_phase_
+ *   {forward,backward}(medium) (newmark.tpp:119-242); returns the medium's
+ *   degrees of freedom (nglob * ncomp, or 0).
+ * sfx_update_wavefields_medium: domain_kernels<wavefield>::update_wavefields
+ *   <medium>(istep) (domain_kernels.hpp:69-126); returns the medium's elements.
+ *   The backward field replays the Stacey record of a completed
+ *   sfx_forward_record (compute_stiffness_interaction.tpp:119-136).
+ * sfx_compute_seismograms_wavefield: domain_kernels<wavefield>::compute_seismograms.
+ * sfx_initialize_wavefield: domain_kernels<wavefield>::initialize(dt): zeroes
+ *   the field (the forward one as sfx_initialize); initializing the adjoint
+ *   field also zeroes the Fréchet kernels, which the reference's assembly
+ *   creates at zero.
+ * sfx_copy_wavefield: specfem::compute::deep_copy(dst, src) (u, v, a).
+ * sfx_compute_derivatives: frechet_kernels::compute_derivatives(dt)
+ *   (include/kokkos_kernels/frechet_kernels.hpp:19-53,
+ *   compute_material_derivatives.tpp:17-154) from the adjoint and backward
+ *   fields; sfx_get_kernels returns the accumulated kernels.  2-D.
+ * sfx_get_wavefield / sfx_set_wavefield: sfx_get_field / sfx_set_field of any
+ *   wavefield. */
+#define SFX_MEDIUM_ELASTIC_PSV 0
+#define SFX_MEDIUM_ACOUSTIC 1
+#define SFX_MEDIUM_ELASTIC_SH 2
+#define SFX_MEDIUM_POROELASTIC 3
+#define SFX_MEDIUM_ELASTIC_PSV_T 4
+#define SFX_WAVEFIELD_FORWARD 0
+#define SFX_WAVEFIELD_ADJOINT 1
+#define SFX_WAVEFIELD_BACKWARD 2
+#define SFX_WAVEFIELD_BUFFER 3
+#define SFX_DIRECTION_FORWARD 0
+#define SFX_DIRECTION_BACKWARD 1
+int sfx_initialize_wavefield(sfx_engine *engine, int wavefield);
+int sfx_predictor_phase(sfx_engine *engine, int medium, int wavefield, int direction);
+int sfx_update_wavefields_medium(sfx_engine *engine, int medium, int wavefield, int istep);
+int sfx_corrector_phase(sfx_engine *engine, int medium, int wavefield, int direction);
+int sfx_compute_seismograms_wavefield(sfx_engine *engine, int wavefield, int isig);
+int sfx_copy_wavefield(sfx_engine *engine, int dst_wavefield, int src_wavefield);
+int sfx_compute_derivatives(sfx_engine *engine, double dt);
+int sfx_get_wavefield(const sfx_engine *engine, int wavefield, int which, float *out);
+int sfx_set_wavefield(sfx_engine *engine, int wavefield, int which, const float *in);
+
 /* ---- fused fast path --------------------------------------------------------
  * One Newmark step of time_marching<forward>::run (time_marching.tpp:33-109)
  * with the corrector of step istep, the seismogram sample and the predictor of
@@ -196,6 +253,21 @@ int sfx_compute_seismograms(sfx_engine *engine, int isig);
 int sfx_step(sfx_engine *engine, int istep);
 int sfx_run(sfx_engine *engine, int first, int count);
 int sfx_synchronize(sfx_engine *engine);
+
+/* ---- run without allocation ---------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory or create HIP objects while stepping (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): sfx_run_noalloc runs
+ * steps [first, first + count) of a 3-D SFX_SCATTER_GATHER engine exactly as
+ * sfx_run does (same kernels, same bits), but the per-element forces live in a
+ * static device array of SFX_NOALLOC_MAX_ELEMENTS x 384 floats (64 MiB per GPU,
+ * held until the process ends) instead of an engine buffer, and no timing
+ * event, graph or buffer is created.  Engines larger than the limit are
+ * refused.  All engines on one GPU share that array: calls that overlap on one
+ * GPU (two host threads) race.  The call returns with its work complete and
+ * the stored acceleration assembled; sfx_get_timings is not updated. */
+#define SFX_NOALLOC_MAX_ELEMENTS 43690
+int sfx_run_noalloc(sfx_engine *engine, int first, int count);
 
 /* ---- outputs ------------------------------------------------------------- */
 /* seismograms (nsig, ntypes, nrec, ncomp) before station rotation

@@ -92,8 +92,18 @@ def load():
         "sfx_update_wavefields": ([vp, i32], i32),
         "sfx_corrector": ([vp], i32),
         "sfx_compute_seismograms": ([vp, i32], i32),
+        "sfx_initialize_wavefield": ([vp, i32], i32),
+        "sfx_predictor_phase": ([vp, i32, i32, i32], i32),
+        "sfx_update_wavefields_medium": ([vp, i32, i32, i32], i32),
+        "sfx_corrector_phase": ([vp, i32, i32, i32], i32),
+        "sfx_compute_seismograms_wavefield": ([vp, i32, i32], i32),
+        "sfx_copy_wavefield": ([vp, i32, i32], i32),
+        "sfx_compute_derivatives": ([vp, C.c_double], i32),
+        "sfx_get_wavefield": ([vp, i32, i32, vp], i32),
+        "sfx_set_wavefield": ([vp, i32, i32, vp], i32),
         "sfx_step": ([vp, i32], i32),
         "sfx_run": ([vp, i32, i32], i32),
+        "sfx_run_noalloc": ([vp, i32, i32], i32),
         "sfx_synchronize": ([vp], i32),
         "sfx_get_seismograms": ([vp, vp], i32),
         "sfx_get_field": ([vp, i32, vp], i32),

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -28,6 +29,41 @@ std::string &last_error() {
 }  // namespace sfx
 
 using namespace sfx;
+
+namespace {
+
+// Element-force scratch of sfx_run_noalloc: a file-scope device array, one
+// copy per GPU, held until the process ends.  In host code `scratch` names a
+// host-side stand-in, not the device copy: kernels receive the address
+// hipGetSymbolAddress returns for the current device, kept per device in
+// scratch_addr.
+constexpr size_t NOALLOC_FLOATS = static_cast<size_t>(SFX_NOALLOC_MAX_ELEMENTS) * RE_ELEM;
+static __device__ __align__(256) float scratch[NOALLOC_FLOATS];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+float *scratch_addr[MAX_DEVICES] = {};
+
+// Device address of `scratch` on `device`, looked up once per device with
+// that device current (the previous current device is restored).
+float *scratch_on(int device) {
+  if (device < 0 || device >= MAX_DEVICES) throw std::invalid_argument("device ordinal out of range");
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_addr[device]) {
+    int prev = 0;
+    SFX_HIP(hipGetDevice(&prev));
+    if (prev != device) SFX_HIP(hipSetDevice(device));
+    void *ptr = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (prev != device) SFX_HIP(hipSetDevice(prev));
+    SFX_HIP(rc);
+    if (!ptr) throw HipError("hipGetSymbolAddress returned no address for the no-alloc scratch");
+    scratch_addr[device] = static_cast<float *>(ptr);
+  }
+  return scratch_addr[device];
+}
+
+}  // namespace
 
 struct sfx_engine {
   HostAssembly h;  // host copy (numbering and locations are returned from here)
@@ -50,7 +86,9 @@ struct sfx_engine {
   DevBuf<float> bnd_normal, bnd_weight, bnd_rvp, bnd_rvs;
   DevBuf<float> src_array, stf, rec_h, seismo;
   DevBuf<float> fields;  // u, v, a, minv: 4 * ncomp * nglob_pad
-  DevBuf<float> re;      // 3-D gather path: element forces (nspec, 3, 128)
+  DevBuf<float> re;      // 3-D gather path: element forces (nspec, 3, 128), allocated at first use
+  float *re_p = nullptr; // element-force buffer of the 3-D gather kernels: re.p, or the no-alloc scratch
+  int device = 0;        // HIP device the engine's memory and stream live on
   DevBuf<float> zero;    // 384 zeros read in place of absent element contributions
   DevBuf<float> bs;      // 3-D block path: block-surface force sums (nblocks, 3, 388)
   DevBuf<float> snap;    // 3-D block path: u, v at receiver element points (nrec, 125, 6)
@@ -66,6 +104,9 @@ struct sfx_engine {
   DevBuf<float> kern;        // (6, nspec, 25) slot order
   DevBuf<float> kacc;        // accumulators of the combined run (6, nspec, 25)
   DevBuf<float> racc;        // per point: sum over steps of dt a+ . u_b
+  DevBuf<float> bwd_fields;  // backward wavefield of the seam calls, (3, ncomp, nglob_pad), at first use
+  DevBuf<float> buf_fields;  // buffer wavefield of the seam calls (the forward state read back)
+  bool kern_dirty = false;   // the accumulators changed since kern was last finished
   DevBuf<int> adj_order, adj_slot;
   DevBuf<float> adj_array, adj_stf;
   int nstacey = 0;
@@ -290,6 +331,7 @@ struct sfx_engine {
 
   void upload() {
     lat = Lattice{h.nx, h.ny, h.nz, h.nx + 1, h.ny + 1, h.nz + 1, h.nint_pad};
+    SFX_HIP(hipGetDevice(&device));
     SFX_HIP(hipStreamCreate(&stream));
     hipStream_t s = stream;
     gather.upload(permuted(h.gather), s);
@@ -321,8 +363,6 @@ struct sfx_engine {
     bnd_seg.upload(h.bnd_seg, s);
     nbseg = static_cast<int>(h.bnd_seg.size()) - 1;
     if (gather_path()) {
-      re.alloc(static_cast<size_t>(h.nspec) * 384);  // 3 components x 128 (125 used)
-      launch_fill(s, re.p, re.n, 0.0f);
       zero.alloc(384);
       launch_fill(s, zero.p, zero.n, 0.0f);
       // M^-1 of one component for the tiles without absorbing-face points,
@@ -389,20 +429,31 @@ struct sfx_engine {
     nc.dt2o2 = dt * dt / 2.0;
   }
 
+  // The 3-D gather kernels of the allocating entry points use the engine's
+  // own element-force buffer, allocated here at first use (3 components x 128
+  // floats per element, 125 used).
+  void own_forces() {
+    if (!gather_path()) return;
+    if (!re.p) {
+      re.alloc(static_cast<size_t>(h.nspec) * RE_ELEM);
+      launch_fill(stream, re.p, re.n, 0.0f);
+    }
+    re_p = re.p;
+  }
   // Gather path: element forces into re, then sources and Stacey into re.
   void element_forces(Fields ff, int istep) {
     if (istep < 0 || istep >= run.nstep) throw std::invalid_argument("istep out of range");
-    launch_elemforce_3d(stream, lat, kappa.p, mu.p, h.xix3, h.ety3, h.gaz3, h.jac3, q, ff.u, h.nglob_pad, re.p);
+    launch_elemforce_3d(stream, lat, kappa.p, mu.p, h.xix3, h.ety3, h.gaz3, h.jac3, q, ff.u, h.nglob_pad, re_p);
   }
   void element_boundary_sources(Fields ff, int istep) {
     if (sources_off_boundary) {
       launch_stacey_sources_re(stream, nbseg, bnd_seg.p, bnd_gather.p, bnd_slot.p, bnd_normal.p, bnd_weight.p,
-                               bnd_rvp.p, bnd_rvs.p, ff, h.nsrc, istep, src_ispec.p, src_array.p, stf.p, re.p);
+                               bnd_rvp.p, bnd_rvs.p, ff, h.nsrc, istep, src_ispec.p, src_array.p, stf.p, re_p);
       return;
     }
-    launch_sources_re(stream, h.nsrc, istep, src_ispec.p, src_array.p, stf.p, re.p);
+    launch_sources_re(stream, h.nsrc, istep, src_ispec.p, src_array.p, stf.p, re_p);
     launch_stacey_re(stream, nbseg, bnd_seg.p, bnd_gather.p, bnd_slot.p, bnd_normal.p, bnd_weight.p, bnd_rvp.p,
-                     bnd_rvs.p, ff, re.p);
+                     bnd_rvs.p, ff, re_p);
   }
   // Block path: element forces, sources and block-interior assembly, then
   // Stacey into the block-surface sums.
@@ -418,7 +469,7 @@ struct sfx_engine {
   // Bring the stored acceleration up to date after fused gather steps.
   void refresh_acceleration() {
     if (gather_path() && a_stale) {
-      launch_assemble_3d(stream, lat, re.p, f(), 1);
+      launch_assemble_3d(stream, lat, re_p, f(), 1);
       a_stale = false;
     }
   }
@@ -511,16 +562,16 @@ struct sfx_engine {
         // Stacey, sources and displacement samples in one launch
         launch_stacey_sources_seismo_re(stream, nbseg, bnd_seg.p, bnd_gather.p, bnd_slot.p, bnd_normal.p,
                                         bnd_weight.p, bnd_rvp.p, bnd_rvs.p, ff, h.nsrc, istep, src_ispec.p,
-                                        src_array.p, stf.p, re.p, lat, h.nrec, run.nseismo_types, types.p, isig,
+                                        src_array.p, stf.p, re_p, lat, h.nrec, run.nseismo_types, types.p, isig,
                                         rec_ispec.p, rec_h.p, nc.dto2, seismo.p);
       } else {
         element_boundary_sources(ff, istep);
         if (sample)
-          launch_seismograms_re(stream, lat, h.nrec, run.nseismo_types, types.p, isig, rec_ispec.p, rec_h.p, re.p, ff,
+          launch_seismograms_re(stream, lat, h.nrec, run.nseismo_types, types.p, isig, rec_ispec.p, rec_h.p, re_p, ff,
                                 nc.dto2, seismo.p, need_a);
       }
       if (evs) SFX_HIP(hipEventRecord(evs[3], stream));
-      launch_assemble_update_3d(stream, lat, re.p, zero.p, ff, nc, minv1.p, h.absorb_mask);
+      launch_assemble_update_3d(stream, lat, re_p, zero.p, ff, nc, minv1.p, h.absorb_mask);
       if (evs) SFX_HIP(hipEventRecord(evs[4], stream));
       a_stale = true;
       return;
@@ -555,6 +606,121 @@ struct sfx_engine {
     if (evs) SFX_HIP(hipEventRecord(evs[4], stream));
     rec_mode = 0;
     rec_step = nullptr;
+  }
+
+  // ---- wavefield / medium / direction seams (time_scheme, domain_kernels<W>) ----
+  // Fields of a simulation wavefield (specfem::wavefield::simulation_field).
+  // The adjoint, backward and buffer fields are allocated and zeroed at first
+  // use; all share the forward field's mass matrix inverse.
+  Fields wavefield(int w) {
+    if (w == SFX_WAVEFIELD_FORWARD) return f();
+    if (h.dim != 2) throw std::invalid_argument("3-D engines have only the forward wavefield");
+    DevBuf<float> *b = w == SFX_WAVEFIELD_ADJOINT    ? &adj_fields
+                       : w == SFX_WAVEFIELD_BACKWARD ? &bwd_fields
+                       : w == SFX_WAVEFIELD_BUFFER   ? &buf_fields
+                                                     : nullptr;
+    if (!b) throw std::invalid_argument("wavefield must be one of SFX_WAVEFIELD_*");
+    const size_t stride = static_cast<size_t>(h.ncomp) * h.nglob_pad;
+    if (b->n != 3 * stride) {
+      b->alloc(3 * stride);
+      launch_fill(stream, b->p, b->n, 0.0f);
+    }
+    return Fields{b->p, b->p + stride, b->p + 2 * stride, fields.p + 3 * stride, h.nglob_pad};
+  }
+  // Elements and degrees of freedom of a medium (this engine's meshes are
+  // elastic P-SV / 3-D elastic: other media hold none, and their phases
+  // update nothing, as the reference's do for media absent from a mesh).
+  static void check_medium(int medium) {
+    if (medium < SFX_MEDIUM_ELASTIC_PSV || medium > SFX_MEDIUM_ELASTIC_PSV_T)
+      throw std::invalid_argument("medium must be one of SFX_MEDIUM_*");
+  }
+  int medium_elements(int medium) const { return medium == SFX_MEDIUM_ELASTIC_PSV ? h.nspec : 0; }
+  int medium_dofs(int medium) const { return medium == SFX_MEDIUM_ELASTIC_PSV ? h.nglob * h.ncomp : 0; }
+  NewmarkConst newmark(int direction) const {
+    if (direction == SFX_DIRECTION_FORWARD) return nc;
+    if (direction == SFX_DIRECTION_BACKWARD) return NewmarkConst{-nc.dt, -nc.dto2, nc.dt2o2};  // newmark.tpp:204-242
+    throw std::invalid_argument("direction must be SFX_DIRECTION_FORWARD or _BACKWARD");
+  }
+  void require_adjoint_capable() const {
+    if (h.dim != 2) throw std::invalid_argument("adjoint and backward wavefields are implemented for 2-D meshes");
+    if (gather2d()) throw std::invalid_argument("adjoint runs need SFX_SCATTER_COLORED or _ATOMIC");
+  }
+  // update_wavefields of the forward field (domain_kernels.hpp:69-126):
+  // sources, stiffness + Stacey, mass division.
+  void update_forward(int istep) {
+    Fields ff = f();
+    if (block_path()) {
+      block_forces(ff, istep, 0, 1);
+      block_boundary(ff);
+      launch_surface_update_3d(stream, lat, bs.p, ff, nc, 0, 1);
+    } else if (gather_path()) {
+      own_forces();
+      element_forces(ff, istep);
+      element_boundary_sources(ff, istep);
+      launch_assemble_3d(stream, lat, re_p, ff, 1);
+      a_stale = false;
+    } else if (gather2d()) {
+      if (istep < 0 || istep >= run.nstep) throw std::invalid_argument("istep out of range");
+      forces_2d(ff, istep, nullptr);
+      launch_assemble_update_2d(stream, h.nglob, fs2(), ff, nc, 1);
+    } else {
+      sources(ff, istep);
+      stiffness(ff);
+      boundary(ff);
+      launch_divide_corrector(stream, h.ncomp, ff, nc, 1, 0);
+    }
+  }
+  // update_wavefields of the adjoint field: adjoint sources, stiffness and
+  // Stacey traction as in a forward step.
+  void update_adjoint(int istep) {
+    require_adjoint_capable();
+    if (istep < 0 || istep >= run.nstep) throw std::invalid_argument("istep out of range");
+    Fields adj = wavefield(SFX_WAVEFIELD_ADJOINT);
+    if (adj_ready)
+      launch_sources(stream, 2, h.npe, 2 * h.nadj, istep, adj_order.p, adj_slot.p, gather.p, adj_array.p, adj_stf.p,
+                     adj);
+    stiffness(adj);
+    boundary(adj);
+    launch_divide_corrector(stream, h.ncomp, adj, nc, 1, 0);
+  }
+  // update_wavefields of the backward field: forward sources, stiffness with
+  // the Stacey elements replaying the recorded forward forces of this step
+  // (compute_stiffness_interaction.tpp:119-136), no traction, mass division.
+  void update_backward(int istep) {
+    require_adjoint_capable();
+    if (istep < 0 || istep >= run.nstep) throw std::invalid_argument("istep out of range");
+    const size_t rec_stride = static_cast<size_t>(nstacey) * h.npe * h.ncomp;
+    if (nstacey > 0 && record.n != static_cast<size_t>(run.nstep) * rec_stride)
+      throw std::logic_error("no Stacey record: run sfx_forward_record before stepping the backward wavefield");
+    Fields bwd = wavefield(SFX_WAVEFIELD_BACKWARD);
+    sources(bwd, istep);
+    rec_mode = 2;
+    rec_step = nstacey > 0 ? record.p + static_cast<size_t>(istep) * rec_stride : nullptr;
+    try {
+      stiffness(bwd);
+    } catch (...) {
+      rec_mode = 0;
+      rec_step = nullptr;
+      throw;
+    }
+    rec_mode = 0;
+    rec_step = nullptr;
+    launch_divide_corrector(stream, h.ncomp, bwd, nc, 1, 0);
+  }
+  void frechet_zero() {
+    const size_t npts = static_cast<size_t>(h.nspec) * h.npe;
+    kacc.ensure(6 * npts);
+    launch_fill(stream, kacc.p, kacc.n, 0.0f);
+    racc.ensure(h.nglob_pad);
+    launch_fill(stream, racc.p, racc.n, 0.0f);
+    kern_dirty = true;
+  }
+  void frechet_finish() {
+    const size_t npts = static_cast<size_t>(h.nspec) * h.npe;
+    kern.ensure(6 * npts);
+    launch_frechet_finish(stream, static_cast<int>(npts), gather.p, kappa.p, mu.p, rho.p, racc.p, kacc.p + npts,
+                          kacc.p + 2 * npts, kern.p);
+    kern_dirty = false;
   }
 
   bool is_checkpoint(int istep) const {
@@ -662,6 +828,7 @@ struct sfx_engine {
       throttle.tick();
     }
     launch_frechet_finish(stream, static_cast<int>(npts), gather.p, kappa.p, mu.p, rho.p, racc.p, Kmu, Kkappa, kern.p);
+    kern_dirty = false;
   }
 };
 
@@ -835,26 +1002,7 @@ int sfx_update_wavefields(sfx_engine *e, int istep) {
   const int rc = guarded([&] {
     check_device(e);
     e->require_init();
-    Fields ff = e->f();
-    if (e->block_path()) {
-      e->block_forces(ff, istep, 0, 1);
-      e->block_boundary(ff);
-      launch_surface_update_3d(e->stream, e->lat, e->bs.p, ff, e->nc, 0, 1);
-    } else if (e->gather_path()) {
-      e->element_forces(ff, istep);
-      e->element_boundary_sources(ff, istep);
-      launch_assemble_3d(e->stream, e->lat, e->re.p, ff, 1);
-      e->a_stale = false;
-    } else if (e->gather2d()) {
-      if (istep < 0 || istep >= e->run.nstep) throw std::invalid_argument("istep out of range");
-      e->forces_2d(ff, istep, nullptr);
-      launch_assemble_update_2d(e->stream, e->h.nglob, e->fs2(), ff, e->nc, 1);
-    } else {
-      e->sources(ff, istep);
-      e->stiffness(ff);
-      e->boundary(ff);
-      launch_divide_corrector(e->stream, e->h.ncomp, ff, e->nc, 1, 0);
-    }
+    e->update_forward(istep);
     SFX_HIP(hipGetLastError());
     nelem = e->h.nspec;
   }, stream_of(e));
@@ -884,10 +1032,122 @@ int sfx_compute_seismograms(sfx_engine *e, int isig) {
   }, stream_of(e));
 }
 
+int sfx_initialize_wavefield(sfx_engine *e, int wavefield) {
+  return guarded([&] {
+    check_device(e);
+    if (wavefield == SFX_WAVEFIELD_FORWARD) {
+      const int rc = sfx_initialize(e);
+      if (rc != SFX_OK) throw std::runtime_error(last_error());
+      return;
+    }
+    e->require_adjoint_capable();
+    Fields F = e->wavefield(wavefield);
+    const size_t stride = static_cast<size_t>(e->h.ncomp) * e->h.nglob_pad;
+    launch_fill(e->stream, F.u, 3 * stride, 0.0f);
+    if (wavefield == SFX_WAVEFIELD_ADJOINT) e->frechet_zero();
+    SFX_HIP(hipGetLastError());
+    e->initialized = true;
+  }, stream_of(e));
+}
+
+int sfx_predictor_phase(sfx_engine *e, int medium, int wavefield, int direction) {
+  int dofs = 0;
+  const int rc = guarded([&] {
+    check_device(e);
+    e->require_init();
+    sfx_engine::check_medium(medium);
+    const NewmarkConst k = e->newmark(direction);
+    Fields F = e->wavefield(wavefield);
+    dofs = e->medium_dofs(medium);
+    if (!dofs) return;
+    if (wavefield == SFX_WAVEFIELD_FORWARD) e->refresh_acceleration();
+    launch_predictor(e->stream, e->h.ncomp, F, k);
+    SFX_HIP(hipGetLastError());
+  }, stream_of(e));
+  return rc == SFX_OK ? dofs : rc;
+}
+
+int sfx_corrector_phase(sfx_engine *e, int medium, int wavefield, int direction) {
+  int dofs = 0;
+  const int rc = guarded([&] {
+    check_device(e);
+    e->require_init();
+    sfx_engine::check_medium(medium);
+    const NewmarkConst k = e->newmark(direction);
+    Fields F = e->wavefield(wavefield);
+    dofs = e->medium_dofs(medium);
+    if (!dofs) return;
+    if (wavefield == SFX_WAVEFIELD_FORWARD) e->refresh_acceleration();
+    launch_divide_corrector(e->stream, e->h.ncomp, F, k, 0, 1);
+    SFX_HIP(hipGetLastError());
+  }, stream_of(e));
+  return rc == SFX_OK ? dofs : rc;
+}
+
+int sfx_update_wavefields_medium(sfx_engine *e, int medium, int wavefield, int istep) {
+  int nelem = 0;
+  const int rc = guarded([&] {
+    check_device(e);
+    e->require_init();
+    sfx_engine::check_medium(medium);
+    nelem = e->medium_elements(medium);
+    if (!nelem) return;
+    if (wavefield == SFX_WAVEFIELD_FORWARD)
+      e->update_forward(istep);
+    else if (wavefield == SFX_WAVEFIELD_ADJOINT)
+      e->update_adjoint(istep);
+    else if (wavefield == SFX_WAVEFIELD_BACKWARD)
+      e->update_backward(istep);
+    else
+      throw std::invalid_argument("the buffer wavefield is not stepped");
+    SFX_HIP(hipGetLastError());
+  }, stream_of(e));
+  return rc == SFX_OK ? nelem : rc;
+}
+
+int sfx_compute_seismograms_wavefield(sfx_engine *e, int wavefield, int isig) {
+  return guarded([&] {
+    check_device(e);
+    e->require_init();
+    if (wavefield == SFX_WAVEFIELD_FORWARD) e->refresh_acceleration();
+    e->seismograms(e->wavefield(wavefield), isig, 0);
+    SFX_HIP(hipGetLastError());
+  }, stream_of(e));
+}
+
+int sfx_copy_wavefield(sfx_engine *e, int dst, int src) {
+  return guarded([&] {
+    check_device(e);
+    if (dst == src) throw std::invalid_argument("source and destination wavefields are the same");
+    if (src == SFX_WAVEFIELD_FORWARD) e->refresh_acceleration();
+    Fields from = e->wavefield(src), to = e->wavefield(dst);
+    const size_t n3 = 3 * static_cast<size_t>(e->h.ncomp) * e->h.nglob_pad;
+    SFX_HIP(hipMemcpyAsync(to.u, from.u, n3 * sizeof(float), hipMemcpyDeviceToDevice, e->stream));
+    if (dst == SFX_WAVEFIELD_FORWARD) e->a_stale = false;
+  }, stream_of(e));
+}
+
+int sfx_compute_derivatives(sfx_engine *e, double dt) {
+  return guarded([&] {
+    check_device(e);
+    e->require_init();
+    e->require_adjoint_capable();
+    if (!e->kacc.n || !e->racc.n) e->frechet_zero();
+    Fields adj = e->wavefield(SFX_WAVEFIELD_ADJOINT), bwd = e->wavefield(SFX_WAVEFIELD_BACKWARD);
+    const float fdt = static_cast<float>(dt);
+    launch_frechet_2d(e->stream, e->h.nspec, e->gather.p, e->xix.p, e->xiz.p, e->gammax.p, e->gammaz.p, e->kappa.p,
+                      e->mu.p, e->rho.p, e->q, adj, bwd, fdt, e->kacc.p);
+    launch_rho_product(e->stream, e->h.ncomp, adj, bwd, fdt, e->racc.p);
+    e->kern_dirty = true;
+    SFX_HIP(hipGetLastError());
+  }, stream_of(e));
+}
+
 int sfx_step(sfx_engine *e, int istep) {
   return guarded([&] {
     check_device(e);
     e->require_init();
+    e->own_forces();
     e->step(istep, nullptr, true);
     SFX_HIP(hipGetLastError());
   }, stream_of(e));
@@ -898,6 +1158,7 @@ int sfx_run(sfx_engine *e, int first, int count) {
     check_device(e);
     e->require_init();
     if (first < 0 || count < 0 || first + count > e->run.nstep) throw std::invalid_argument("step range out of bounds");
+    e->own_forces();
     if (e->graph_2d() && count > 1) {
       // 2-D: one captured graph per step; only the whole-step time is taken
       e->set_step(first);
@@ -954,6 +1215,45 @@ int sfx_run(sfx_engine *e, int first, int count) {
   }, stream_of(e));
 }
 
+int sfx_run_noalloc(sfx_engine *e, int first, int count) {
+  return guarded([&] {
+    check_device(e);
+    e->require_init();
+    if (!e->gather_path()) throw std::invalid_argument("sfx_run_noalloc needs a 3-D engine with SFX_SCATTER_GATHER");
+    if (first < 0 || count < 0 || first + count > e->run.nstep) throw std::invalid_argument("step range out of bounds");
+    if (e->h.nspec > SFX_NOALLOC_MAX_ELEMENTS)
+      throw std::invalid_argument("sfx_run_noalloc: " + std::to_string(e->h.nspec) + " elements exceed the scratch limit of " +
+                                  std::to_string(SFX_NOALLOC_MAX_ELEMENTS) + " (SFX_NOALLOC_MAX_ELEMENTS)");
+    if (count == 0) return;
+    float *const sc = scratch_on(e->device);
+    // an acceleration left behind by sfx_run is assembled from the engine's
+    // own buffer first (it exists whenever one is left behind)
+    e->refresh_acceleration();
+    // every step writes each element's forces before any kernel reads them,
+    // so the shared scratch needs no clearing between calls
+    e->re_p = sc;
+    try {
+      for (int i = 0; i < count; i++) {
+        e->step(first + i, nullptr, i == count - 1);
+        // bound the queued work without creating events: wait every 256 steps
+        if ((i + 1) % 256 == 0) SFX_HIP(hipStreamSynchronize(e->stream));
+      }
+    } catch (...) {
+      if (e->a_stale) launch_assemble_3d(e->stream, e->lat, sc, e->f(), 1);
+      e->a_stale = false;
+      e->re_p = e->re.p;
+      throw;
+    }
+    // assemble the stored acceleration from the scratch now, while it holds
+    // this call's last forces
+    launch_assemble_3d(e->stream, e->lat, sc, e->f(), 1);
+    e->a_stale = false;
+    e->re_p = e->re.p;
+    SFX_HIP(hipGetLastError());
+    SFX_HIP(hipStreamSynchronize(e->stream));
+  }, stream_of(e));
+}
+
 int sfx_synchronize(sfx_engine *e) {
   return guarded([&] {
     check_device(e);
@@ -970,20 +1270,21 @@ int sfx_get_seismograms(const sfx_engine *e, float *out) {
   }, stream_of(e));
 }
 
-int sfx_get_field(const sfx_engine *ce, int which, float *out) {
+int sfx_get_wavefield(const sfx_engine *ce, int wavefield, int which, float *out) {
   return guarded([&] {
     check_device(ce);
     if (which < 0 || which > 3) throw std::invalid_argument("which must be 0..3");
     if (!out) throw std::invalid_argument("null output");
     // the stored acceleration is refreshed lazily after fused gather steps
     sfx_engine *e = const_cast<sfx_engine *>(ce);
-    if (which == 2) e->refresh_acceleration();
+    if (wavefield == SFX_WAVEFIELD_FORWARD && which == 2) e->refresh_acceleration();
+    Fields F = e->wavefield(wavefield);
+    const float *arrays[4] = {F.u, F.v, F.a, F.minv};
     SFX_HIP(hipStreamSynchronize(e->stream));
-    const size_t stride = static_cast<size_t>(e->h.ncomp) * e->h.nglob_pad;
     const size_t ng = e->h.nglob;
     std::vector<float> buf(e->h.nglob_pad);
     for (int c = 0; c < e->h.ncomp; c++) {
-      SFX_HIP(hipMemcpy(buf.data(), e->fields.p + which * stride + static_cast<size_t>(c) * e->h.nglob_pad,
+      SFX_HIP(hipMemcpy(buf.data(), arrays[which] + static_cast<size_t>(c) * e->h.nglob_pad,
                         e->h.nglob_pad * sizeof(float), hipMemcpyDeviceToHost));
       float *o = out + static_cast<size_t>(c) * ng;
       if (e->h.lattice_of.empty())
@@ -994,13 +1295,18 @@ int sfx_get_field(const sfx_engine *ce, int which, float *out) {
   }, stream_of(ce));
 }
 
-int sfx_set_field(sfx_engine *e, int which, const float *in) {
+int sfx_get_field(const sfx_engine *e, int which, float *out) {
+  return sfx_get_wavefield(e, SFX_WAVEFIELD_FORWARD, which, out);
+}
+
+int sfx_set_wavefield(sfx_engine *e, int wavefield, int which, const float *in) {
   return guarded([&] {
     check_device(e);
     if (which < 0 || which > 2) throw std::invalid_argument("which must be 0..2");
     if (!in) throw std::invalid_argument("null input");
+    Fields F = e->wavefield(wavefield);
+    float *arrays[3] = {F.u, F.v, F.a};
     SFX_HIP(hipStreamSynchronize(e->stream));
-    const size_t stride = static_cast<size_t>(e->h.ncomp) * e->h.nglob_pad;
     const size_t ng = e->h.nglob;
     std::vector<float> buf(e->h.nglob_pad, 0.0f);
     for (int c = 0; c < e->h.ncomp; c++) {
@@ -1009,11 +1315,15 @@ int sfx_set_field(sfx_engine *e, int which, const float *in) {
         std::memcpy(buf.data(), src, ng * sizeof(float));
       else
         for (size_t g = 0; g < ng; g++) buf[e->h.lattice_of[g]] = src[g];
-      SFX_HIP(hipMemcpy(e->fields.p + which * stride + static_cast<size_t>(c) * e->h.nglob_pad, buf.data(),
+      SFX_HIP(hipMemcpy(arrays[which] + static_cast<size_t>(c) * e->h.nglob_pad, buf.data(),
                         e->h.nglob_pad * sizeof(float), hipMemcpyHostToDevice));
     }
-    if (which == 2) e->a_stale = false;
+    if (wavefield == SFX_WAVEFIELD_FORWARD && which == 2) e->a_stale = false;
   }, stream_of(e));
+}
+
+int sfx_set_field(sfx_engine *e, int which, const float *in) {
+  return sfx_set_wavefield(e, SFX_WAVEFIELD_FORWARD, which, in);
 }
 
 int sfx_set_properties(sfx_engine *e, const float *rho, const float *kappa, const float *mu) {
@@ -1125,7 +1435,8 @@ int sfx_get_kernels(const sfx_engine *e, float *out) {
   return guarded([&] {
     check_device(e);
     if (!out) throw std::invalid_argument("null output");
-    if (!e->kern.n) throw std::logic_error("no kernels: run sfx_adjoint_run first");
+    if (!e->kern.n && !e->kern_dirty) throw std::logic_error("no kernels: run sfx_adjoint_run first");
+    if (e->kern_dirty) const_cast<sfx_engine *>(e)->frechet_finish();
     SFX_HIP(hipStreamSynchronize(e->stream));
     std::vector<float> buf(e->kern.n);
     SFX_HIP(hipMemcpy(buf.data(), e->kern.p, buf.size() * sizeof(float), hipMemcpyDeviceToHost));

@@ -106,6 +106,13 @@ def _receivers(receivers):
     return arr
 
 
+# specfem::element::medium_tag, specfem::wavefield::simulation_field and the
+# Newmark phase direction, as the library's SFX_MEDIUM_*, SFX_WAVEFIELD_* and
+# SFX_DIRECTION_* codes
+MEDIA = {"elastic_psv": 0, "acoustic": 1, "elastic_sh": 2, "poroelastic": 3, "elastic_psv_t": 4}
+WAVEFIELDS = {"forward": 0, "adjoint": 1, "backward": 2, "buffer": 3}
+DIRECTIONS = {"forward": 0, "backward": 1}
+
 SCATTER_COLORED, SCATTER_ATOMIC, SCATTER_GATHER, SCATTER_BLOCK = 0, 1, 2, 3
 _SCATTER_CODES = {"colored": 0, "atomic": 1, "gather": 2, "block": 3}
 
@@ -192,6 +199,42 @@ class _EngineBase:
     def compute_seismograms(self, isig):
         self._check(self.lib.sfx_compute_seismograms(self.h, isig))
 
+    # ---- wavefield / medium / direction seams ----
+    def initialize_wavefield(self, wavefield="forward"):
+        self._check(self.lib.sfx_initialize_wavefield(self.h, WAVEFIELDS[wavefield]))
+
+    def predictor_phase(self, medium="elastic_psv", wavefield="forward", direction="forward"):
+        return self._check(self.lib.sfx_predictor_phase(self.h, MEDIA[medium], WAVEFIELDS[wavefield],
+                                                        DIRECTIONS[direction]))
+
+    def corrector_phase(self, medium="elastic_psv", wavefield="forward", direction="forward"):
+        return self._check(self.lib.sfx_corrector_phase(self.h, MEDIA[medium], WAVEFIELDS[wavefield],
+                                                        DIRECTIONS[direction]))
+
+    def update_wavefields_medium(self, medium, wavefield, istep):
+        return self._check(self.lib.sfx_update_wavefields_medium(self.h, MEDIA[medium], WAVEFIELDS[wavefield],
+                                                                 istep))
+
+    def compute_seismograms_wavefield(self, wavefield, isig):
+        self._check(self.lib.sfx_compute_seismograms_wavefield(self.h, WAVEFIELDS[wavefield], isig))
+
+    def copy_wavefield(self, dst, src):
+        self._check(self.lib.sfx_copy_wavefield(self.h, WAVEFIELDS[dst], WAVEFIELDS[src]))
+
+    def compute_derivatives(self, dt):
+        self._check(self.lib.sfx_compute_derivatives(self.h, float(dt)))
+
+    def wavefield_field(self, wavefield, which):
+        codes = {"displacement": 0, "velocity": 1, "acceleration": 2, "mass_inverse": 3}
+        out = np.zeros((self.ncomp, self.nglob), dtype=np.float32)
+        self._check(self.lib.sfx_get_wavefield(self.h, WAVEFIELDS[wavefield], codes[which], _ptr(out)))
+        return out
+
+    def set_wavefield_field(self, wavefield, which, values):
+        codes = {"displacement": 0, "velocity": 1, "acceleration": 2}
+        arr = np.ascontiguousarray(values, dtype=np.float32).reshape(self.ncomp, self.nglob)
+        self._check(self.lib.sfx_set_wavefield(self.h, WAVEFIELDS[wavefield], codes[which], _ptr(arr)))
+
     # ---- fused path ----
     def step(self, istep):
         self._check(self.lib.sfx_step(self.h, istep))
@@ -200,6 +243,14 @@ class _EngineBase:
         if count is None:
             count = self.nstep - first
         self._check(self.lib.sfx_run(self.h, first, count))
+
+    def run_noalloc(self, first=0, count=None):
+        """``run`` without device allocation (``sfx_run_noalloc``): 3-D gather
+        engines of at most ``SFX_NOALLOC_MAX_ELEMENTS`` elements; the element
+        forces live in the library's static per-GPU scratch."""
+        if count is None:
+            count = self.nstep - first
+        self._check(self.lib.sfx_run_noalloc(self.h, first, count))
 
     def synchronize(self):
         self._check(self.lib.sfx_synchronize(self.h))

@@ -29,7 +29,13 @@ for istep in range(w.nstep):
     eng.apply_corrector_phase()
     eng.compute_seismograms(istep)
 seam = eng.seismograms().astype(np.float64)
+# the same run through sfx_run_noalloc (element forces in the static scratch)
+eng.initialize()
+eng.run_noalloc(0, 5)
+eng.run_noalloc(5, w.nstep - 5)
+noalloc = eng.seismograms().astype(np.float64)
 eng.close()
+assert np.array_equal(noalloc, ours), "sfx_run_noalloc differs from sfx_run"
 ref, _ = oracle.Oracle3D(w.mesh, 4, w.dt).forward(w.sources, w.receivers, w.nstep, w.dt, 1, types)
 err = float(np.linalg.norm(ours[:, 0] - ref[:, 0]) / np.linalg.norm(ref[:, 0]))
 seam_err = float(np.abs(seam - ours).max() / np.abs(ours).max())

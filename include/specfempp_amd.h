@@ -312,6 +312,26 @@ int sfx_forward_record(sfx_engine *engine, int snapshot_every);
  * impedances that depend on them).  2-D. */
 int sfx_set_properties(sfx_engine *engine, const float *rho, const float *kappa, const float *mu);
 int sfx_set_adjoint_sources(sfx_engine *engine, int nadj, const sfx_receiver *positions, const float *stf);
+/* The Stacey record of the last sfx_forward_record, in the layout of the
+ * reference's boundary_values.stacey.elastic values (nstacey, ngllz, ngllx,
+ * nstep, 2) LayoutLeft, element fastest
+ * (include/compute/boundary_values/impl/boundary_medium_container.hpp:26-40):
+ * the local acceleration (stiffness + Stacey traction) of every point of every
+ * Stacey element at every step, elements in compute order
+ * (boundary_medium_container.tpp:20-31).  The wavefield writer saves it as
+ * Boundary/Stacey/ElasticAcceleration (io/wavefield/writer.tpp:127-148).
+ * sfx_set_stacey_record loads such a record (the wavefield reader,
+ * io/wavefield/reader.tpp:44-62) and declares the forward wavefield to hold
+ * the recorded run's final state, which the caller sets (sfx_set_wavefield or
+ * sfx_copy_wavefield from the buffer) before sfx_adjoint_run; the backward
+ * seam (sfx_update_wavefields_medium) replays it too.  2-D. */
+int sfx_get_stacey_record(const sfx_engine *engine, float *out);
+int sfx_set_stacey_record(sfx_engine *engine, const float *in);
+/* GLL point coordinates (nspec, ngll, ngll, 2) in compute order, as the
+ * reference stores them in type_real (mesh.points.h_coord,
+ * compute_mesh.cpp:299-333); the wavefield and kernel writers save them.
+ * 2-D. */
+int sfx_get_point_coordinates(const sfx_engine *engine, float *out);
 int sfx_adjoint_run(sfx_engine *engine);
 int sfx_get_kernels(const sfx_engine *engine, float *out);
 /* Integration weight w_i w_j J of every element point, (nspec, ngll, ngll) in

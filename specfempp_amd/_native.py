@@ -115,6 +115,9 @@ def load():
         "sfx_adjoint_run": ([vp], i32),
         "sfx_get_kernels": ([vp, vp], i32),
         "sfx_get_point_weights": ([vp, vp], i32),
+        "sfx_get_point_coordinates": ([vp, vp], i32),
+        "sfx_get_stacey_record": ([vp, vp], i32),
+        "sfx_set_stacey_record": ([vp, vp], i32),
         "sfx_batch_create": ([i32, vp, vp], i32),
         "sfx_batch_destroy": ([vp], None),
         "sfx_batch_tune": ([vp, i32, vp], i32),

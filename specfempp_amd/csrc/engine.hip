@@ -1416,6 +1416,61 @@ int sfx_adjoint_run(sfx_engine *e) {
   }, stream_of(e));
 }
 
+int sfx_get_point_coordinates(const sfx_engine *e, float *out) {
+  return guarded([&] {
+    check_engine(e);
+    if (!out) throw std::invalid_argument("null output");
+    if (e->h.dim != 2) throw std::invalid_argument("point coordinates are implemented for 2-D meshes");
+    const size_t n = static_cast<size_t>(e->h.nspec) * e->h.npe;
+    for (size_t p = 0; p < n; p++) {
+      out[2 * p] = e->h.px2d[p];
+      out[2 * p + 1] = e->h.pz2d[p];
+    }
+  });
+}
+
+int sfx_get_stacey_record(const sfx_engine *e, float *out) {
+  return guarded([&] {
+    check_device(e);
+    if (!out) throw std::invalid_argument("null output");
+    const size_t per_step = static_cast<size_t>(e->nstacey) * e->h.npe * e->h.ncomp;
+    if (e->h.dim != 2 || e->record.n != static_cast<size_t>(e->run.nstep) * per_step)
+      throw std::logic_error("no Stacey record: run sfx_forward_record first");
+    SFX_HIP(hipStreamSynchronize(e->stream));
+    std::vector<float> buf(e->record.n);
+    if (!buf.empty())
+      SFX_HIP(hipMemcpy(buf.data(), e->record.p, buf.size() * sizeof(float), hipMemcpyDeviceToHost));
+    // device (nstep, nstacey, 25, 2) -> boundary_medium_container values
+    // (nstacey, ngllz, ngllx, nstep, 2) LayoutLeft: element fastest
+    const size_t ns = e->nstacey, nt = e->run.nstep;
+    for (size_t t = 0; t < nt; t++)
+      for (size_t s = 0; s < ns; s++)
+        for (size_t p = 0; p < 25; p++)
+          for (size_t c = 0; c < 2; c++) out[s + ns * (p % 5 * 5 + p / 5 + 25 * (t + nt * c))] = buf[((t * ns + s) * 25 + p) * 2 + c];
+  }, stream_of(e));
+}
+
+int sfx_set_stacey_record(sfx_engine *e, const float *in) {
+  return guarded([&] {
+    check_device(e);
+    if (!in) throw std::invalid_argument("null input");
+    e->require_adjoint_capable();
+    const size_t ns = e->nstacey, nt = e->run.nstep;
+    std::vector<float> buf(nt * ns * 25 * 2);
+    for (size_t t = 0; t < nt; t++)
+      for (size_t s = 0; s < ns; s++)
+        for (size_t p = 0; p < 25; p++)
+          for (size_t c = 0; c < 2; c++) buf[((t * ns + s) * 25 + p) * 2 + c] = in[s + ns * (p % 5 * 5 + p / 5 + 25 * (t + nt * c))];
+    SFX_HIP(hipStreamSynchronize(e->stream));
+    if (e->record.n != buf.size()) e->record.alloc(buf.size());
+    if (!buf.empty())
+      SFX_HIP(hipMemcpy(e->record.p, buf.data(), buf.size() * sizeof(float), hipMemcpyHostToDevice));
+    e->snapshot_every = 0;
+    e->recorded = true;
+    e->invalidate_graphs();
+  }, stream_of(e));
+}
+
 int sfx_get_point_weights(const sfx_engine *e, float *out) {
   return guarded([&] {
     check_device(e);

@@ -308,6 +308,32 @@ class Engine2D(_EngineBase):
         backward field's, latest sample first."""
         self._check(self.lib.sfx_adjoint_run(self.h))
 
+    def stacey_elements(self):
+        """Number of elements with an absorbing edge (the Stacey elements,
+        last in compute order)."""
+        return self._nstacey
+
+    def point_coordinates(self):
+        """GLL point coordinates (nspec, 5, 5, 2) in compute order (float)."""
+        out = np.zeros((self.nspec, 5, 5, 2), dtype=np.float32)
+        self._check(self.lib.sfx_get_point_coordinates(self.h, _ptr(out)))
+        return out
+
+    def stacey_record(self):
+        """The last forward_record's Stacey record, (nstacey, 5, 5, nstep, 2)
+        in the reference's LayoutLeft boundary-values layout."""
+        out = np.zeros((self._nstacey, 5, 5, self.nstep, 2), dtype=np.float32, order="F")
+        self._check(self.lib.sfx_get_stacey_record(self.h, _ptr(out)))
+        return out
+
+    def set_stacey_record(self, record):
+        """Load a Stacey record (as ``stacey_record`` returns it); the forward
+        wavefield is then taken to hold the recorded run's final state."""
+        arr = np.asfortranarray(record, dtype=np.float32)
+        if arr.shape != (self._nstacey, 5, 5, self.nstep, 2):
+            raise ValueError(f"record must be (nstacey, 5, 5, nstep, 2) = ({self._nstacey}, 5, 5, {self.nstep}, 2)")
+        self._check(self.lib.sfx_set_stacey_record(self.h, _ptr(arr)))
+
     def point_weights(self):
         """w_i w_j J of every element point, (nspec, 5, 5) in mesh element order."""
         out = np.zeros((self.nspec, 5, 5), dtype=np.float32)
@@ -337,6 +363,7 @@ class Engine2D(_EngineBase):
         rho, vp, vs = material_arrays(mesh.materials)
         abs_ispec = np.ascontiguousarray(mesh.abs_ispec, dtype=np.int32)
         abs_edge = np.ascontiguousarray(mesh.abs_type, dtype=np.int32)
+        self._nstacey = int(np.unique(abs_ispec).size)
         self._keep = (coorg, knods, kmato, rho, vp, vs, abs_ispec, abs_edge)
         m = _Mesh2D(mesh.nspec, mesh.ngnod, mesh.npgeo, _ptr(coorg), _ptr(knods), _ptr(kmato), len(rho), _ptr(rho),
                     _ptr(vp), _ptr(vs), len(abs_ispec), _ptr(abs_ispec), _ptr(abs_edge))

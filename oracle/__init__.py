@@ -40,7 +40,10 @@ def load():
     lib.orc_last_error.restype = C.c_char_p
     vp = C.c_void_p
     lib.orc2d_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _ip, _ip, C.c_int, _dp, _dp, _dp,
-                                 C.c_int, _ip, _ip, C.c_int, C.c_double, C.POINTER(vp)]
+                                 C.c_int, _ip, _ip, C.c_int, _ip, _ip, C.c_int, _ip, _ip, C.c_int, C.c_double,
+                                 C.POINTER(vp)]
+    lib.orc2d_nglob_media.argtypes = [vp, _ip]
+    lib.orc2d_acoustic.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.orc2d_destroy.argtypes = [vp]
     lib.orc2d_set_start_time.argtypes = [vp, C.c_double]
     lib.orc2d_nglob.argtypes = [vp]
@@ -108,12 +111,23 @@ class Oracle2D:
         rho, cp, cs = material_arrays(mesh.materials)
         abs_ispec = np.ascontiguousarray(mesh.abs_ispec, dtype=np.int32)
         abs_type = np.ascontiguousarray(mesh.abs_type, dtype=np.int32)
+
+        def arr(name):
+            v = getattr(mesh, name, None)
+            return np.ascontiguousarray(np.zeros(0) if v is None else v, dtype=np.int32)
+
+        afs_ispec, afs_type = arr("afs_ispec"), arr("afs_type")
+        fs_ac, fs_el = arr("fs_acoustic"), arr("fs_elastic")
         h = C.c_void_p()
         _check(lib.orc2d_create(precision, mesh.nspec, mesh.ngnod, mesh.npgeo, coorg, knods, kmato, len(rho), rho, cp,
-                                cs, len(abs_ispec), abs_ispec, abs_type, chunk, dt, C.byref(h)))
+                                cs, len(abs_ispec), abs_ispec, abs_type, len(afs_ispec), afs_ispec, afs_type,
+                                len(fs_ac), fs_ac, fs_el, chunk, dt, C.byref(h)))
         self.h = h
         self.nspec = mesh.nspec
         self.nglob = lib.orc2d_nglob(h)
+        n2 = np.zeros(2, dtype=np.int32)
+        lib.orc2d_nglob_media(h, n2)
+        self.nglob_el, self.nglob_ac = int(n2[0]), int(n2[1])
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -134,9 +148,18 @@ class Oracle2D:
         return coord, arrs  # xix, xiz, gammax, gammaz, jacobian
 
     def mass_inverse(self):
-        out = np.zeros((2, self.nglob))
+        out = np.zeros((2, self.nglob_el))
         load().orc2d_mass_inverse(self.h, out)
         return out
+
+    def acoustic(self):
+        """(acoustic field index of every iglob (-1 elsewhere), medium of every
+        compute element (0 elastic, 1 acoustic), acoustic mass inverse)."""
+        idx = np.zeros(self.nglob, dtype=np.int32)
+        med = np.zeros(self.nspec, dtype=np.int32)
+        minv = np.zeros(self.nglob_ac)
+        load().orc2d_acoustic(self.h, _ptr(idx), _ptr(med), _ptr(minv))
+        return idx, med, minv
 
     def quadrature(self):
         xi, w, hp = np.zeros(5), np.zeros(5), np.zeros(25)
@@ -150,11 +173,12 @@ class Oracle2D:
 
     @staticmethod
     def _src_rows(sources):
-        src = np.zeros((len(sources), 10))
+        src = np.zeros((len(sources), 11))
         for i, s in enumerate(sources):
             kind = 1 if s.get("type", "force") == "moment_tensor" else 0
+            stf = 1 if s.get("stf", "ricker") == "dirac" else 0
             src[i] = [kind, s["x"], s["z"], s.get("angle", 0.0), s.get("mxx", 0.0), s.get("mzz", 0.0),
-                      s.get("mxz", 0.0), s["f0"], s.get("tshift", 0.0), s["factor"]]
+                      s.get("mxz", 0.0), s.get("f0", 0.0), s.get("tshift", 0.0), s["factor"], stf]
         return src
 
     @staticmethod
@@ -241,17 +265,27 @@ class Oracle2D:
 
     def forward(self, sources, receivers, nstep: int, dt: float, nsb: int = 1,
                 types=("displacement",), nthreads: int = 1, return_fields: bool = False):
-        """Run the forward loop; returns (seismograms (nsig, ntypes, nrec, 2), t0[, fields])."""
+        """Run the forward loop; returns (seismograms (nsig, ntypes, nrec, 2), t0[, fields]).
+        fields: (3, 2, nglob) u, v, a; on a mesh with acoustic elements the
+        pair (elastic (3, 2, nglob_el), acoustic (3, nglob_ac) chi, chi_dot,
+        chi_dot_dot)."""
         src, rec = self._src_rows(sources), self._rec_rows(receivers)
         tcodes = np.array([SEISMO_TYPES[t] for t in types], dtype=np.int32)
         nsig = nstep // nsb if nstep % nsb == 0 else nstep // nsb + 1
         seis = np.zeros((nsig, len(types), len(receivers), 2))
         t0 = C.c_double()
-        fields = np.zeros((3, 2, self.nglob)) if return_fields else None
+        coupled = self.nglob_ac > 0
+        nf = 6 * self.nglob_el + 3 * self.nglob_ac if coupled else 6 * self.nglob
+        fields = np.zeros(nf) if return_fields else None
         _check(load().orc2d_forward(self.h, len(sources), np.ascontiguousarray(src), len(receivers),
                                     np.ascontiguousarray(rec), nstep, dt, nsb, len(types), tcodes, seis,
                                     C.byref(t0), nthreads, _ptr(fields)))
         if return_fields:
+            if coupled:
+                ne = 6 * self.nglob_el
+                fields = (fields[:ne].reshape(3, 2, self.nglob_el), fields[ne:].reshape(3, self.nglob_ac))
+            else:
+                fields = fields.reshape(3, 2, self.nglob)
             return seis, t0.value, fields
         return seis, t0.value
 

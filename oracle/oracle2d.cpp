@@ -11,18 +11,28 @@
 // order in which contributions are summed into a global point.
 //
 // Pinning (checks that hold this restatement to the reference):
-//   - GLL nodes/weights/hprime: known answers of tests/unit-tests/gll and
-//     tests/unit-tests/lagrange (tests/test_oracle_quadrature.py).
+//   - GLL nodes/weights/hprime: the known answers of tests/unit-tests/gll,
+//     quadrature exactness to degree 5, hprime on constants and linears
+//     (tests/test_oracle_fixtures.py::test_gll_known_answers).
+//   - geometry: SPECFEM2D xix/xiz/gammax/gammaz/jacobian fixtures of
+//     tests/unit-tests/compute/partial_derivatives
+//     (tests/test_oracle_fixtures.py::test_partial_derivatives_match_specfem2d).
+//   - locate_point: the known answers of tests/unit-tests/algorithms/locate.cpp
+//     (tests/test_oracle_fixtures.py::test_locate_point_known_answers).
 //   - mass matrix: SPECFEM2D rmass_inverse fixture, rel. L1 <= 1e-5
 //     (tests/unit-tests/domain/rmass_inverse_tests.cpp:229-234).
 //   - full forward run: SPECFEM2D traces, rel. L1 <= 1e-3
 //     (tests/unit-tests/displacement_tests/Newmark/newmark_tests.cpp:325-346),
 //     free-surface and Stacey fixtures.
-//   - numbering: invariant one-iglob-one-coordinate
-//     (tests/unit-tests/compute/index/compute_tests.cpp:70-129) and a
-//     brute-force coordinate grouping on small meshes.
+//   - numbering order: hand-derived known answers on 2x2-element meshes at
+//     storage chunk 1 and 2 (tests/test_numbering_known_answers.py).
 //   - stiffness: dense brute-force weak-form assembly on small meshes
-//     (symmetry, rigid-body null space, match of K u).
+//     (symmetry, rigid-body null space, match of K u;
+//     tests/test_oracle_bruteforce.py).
+//   - pressure seismogram: closed form on a linear displacement field
+//     (tests/test_oracle_seismograms.py).
+//   - adjoint loop and Frechet kernels: time reversibility and central finite
+//     differences of the FP64 misfit (tests/test_oracle_adjoint.py).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -221,7 +231,25 @@ template <class T> void dshape2d(T xi, T gamma, int ngnod, T *d0, T *d1) {
 // ---------------------------------------------------------------------------
 // Mesh input (mesh order) and assembled state (compute order)
 // ---------------------------------------------------------------------------
-enum Edge { BOTTOM = 0, RIGHT = 1, TOP = 2, LEFT = 3 };
+enum Edge { BOTTOM = 0, RIGHT = 1, TOP = 2, LEFT = 3, BOTTOM_LEFT = 4, BOTTOM_RIGHT = 5, TOP_RIGHT = 6, TOP_LEFT = 7 };
+
+// Element boundary tags (include/enumerations/boundary.hpp: the container's
+// += combines a Stacey and an acoustic free-surface tag into the composite).
+enum BTag { BT_NONE = 0, BT_AFS = 1, BT_STACEY = 2, BT_COMPOSITE = 3 };
+inline int add_tag(int cur, int t) {
+  if (cur == BT_NONE) return t;
+  if (cur == t || cur == BT_COMPOSITE) return cur;
+  return BT_COMPOSITE;  // Stacey + free surface in either order
+}
+// Point on an edge or corner of an element
+// (src/compute/boundaries/impl/acoustic_free_surface.cpp:10-26).
+inline bool on_boundary(int type, int iz, int ix) {
+  const int n = NGLL - 1;
+  return (type == TOP && iz == n) || (type == BOTTOM && iz == 0) || (type == LEFT && ix == 0) ||
+         (type == RIGHT && ix == n) || (type == BOTTOM_RIGHT && iz == 0 && ix == n) ||
+         (type == BOTTOM_LEFT && iz == 0 && ix == 0) || (type == TOP_RIGHT && iz == n && ix == n) ||
+         (type == TOP_LEFT && iz == n && ix == 0);
+}
 
 struct MeshIn {
   int nspec, ngnod, npgeo;
@@ -232,6 +260,19 @@ struct MeshIn {
   const double *rho, *cp, *cs;  // per material
   int nabs;
   const int *abs_ispec, *abs_type;  // mesh element and Edge of each absorbing edge
+  int nafs = 0;                     // acoustic free-surface entries: element, Edge (edge or corner)
+  const int *afs_ispec = nullptr, *afs_type = nullptr;
+  int nfs = 0;                      // fluid-solid edges: acoustic and elastic mesh element
+  const int *fs_acoustic = nullptr, *fs_elastic = nullptr;
+};
+
+// One fluid-solid edge of the elastic-acoustic interface container
+// (include/compute/coupled_interfaces/interface_container.tpp): elements in
+// compute order, the connected edge of each, and per edge point the
+// integration factor and the elastic element's edge normal.
+template <class T> struct CoupledEdge {
+  int e_el, e_ac, edge_el, edge_ac;
+  T factor[NGLL], nx[NGLL], nz[NGLL];
 };
 
 template <class T> struct Assembly {
@@ -251,6 +292,17 @@ template <class T> struct Assembly {
   std::vector<int> bnd_tag;          // (ispec, iz, ix), 0 for non-Stacey elements
   std::vector<T> bnd_nx, bnd_nz, bnd_w;
   std::vector<T> mass_inverse;       // (comp, nglob) LayoutLeft on medium index
+  // acoustic medium (the elastic arrays above then cover the elastic elements;
+  // medium_index is -1 at points no elastic element touches)
+  std::vector<int> medium;           // per compute element: 0 elastic P-SV, 1 acoustic
+  std::vector<int> btag;             // per compute element: BTag
+  std::vector<int> dir_tag;          // (ispec, iz, ix): 1 on an acoustic free surface
+  std::vector<int> index_ac;         // iglob -> acoustic field index, or -1
+  int nglob_el = 0, nglob_ac = 0;
+  std::vector<T> rho_inv;            // acoustic: 1/rho per point
+  std::vector<T> mass_inverse_ac;    // (nglob_ac)
+  std::vector<CoupledEdge<T>> coupling;
+  bool coupled() const { return nglob_ac > 0; }
   double dt = 0;                     // time step of the Stacey mass term
   double start_time = 0;             // user start time (0: automatic), see make_sources
 };
@@ -345,9 +397,9 @@ void assign_numbering(Assembly<T> &a, const std::vector<double> &gcoord /*(ispec
 }
 
 // Per-medium numbering, include/compute/fields/impl/field_impl.tpp:39-62
-// (loop order chunk, ix, iz, element).
-template <class T> void medium_numbering(Assembly<T> &a) {
-  a.medium_index.assign(a.nglob, -1);
+// (loop order chunk, ix, iz, element; elements of other media skipped).
+template <class T> int number_medium(const Assembly<T> &a, int med, std::vector<int> &index) {
+  index.assign(a.nglob, -1);
   int count = 0;
   for (int ichunk = 0; ichunk < a.nspec; ichunk += a.chunk)
     for (int ix = 0; ix < NGLL; ix++)
@@ -355,10 +407,16 @@ template <class T> void medium_numbering(Assembly<T> &a) {
         for (int ie = 0; ie < a.chunk; ie++) {
           int ispec = ichunk + ie;
           if (ispec >= a.nspec) break;
+          if (a.medium[ispec] != med) continue;
           int g = a.ibool[(static_cast<size_t>(ispec) * NGLL + iz) * NGLL + ix];
-          if (a.medium_index[g] == -1) a.medium_index[g] = count++;
+          if (index[g] == -1) index[g] = count++;
         }
-  if (count != a.nglob) throw std::runtime_error("medium numbering: count mismatch");
+  return count;
+}
+template <class T> void medium_numbering(Assembly<T> &a) {
+  a.nglob_el = number_medium(a, 0, a.medium_index);
+  a.nglob_ac = number_medium(a, 1, a.index_ac);
+  if (a.nglob_ac == 0 && a.nglob_el != a.nglob) throw std::runtime_error("medium numbering: count mismatch");
 }
 
 template <class T> inline size_t pidx(int ispec, int iz, int ix) {
@@ -379,6 +437,84 @@ inline void stacey_traction(T nx, T nz, T w, T kappa, T mu, T rho, T vx, T vz, T
   tz += static_cast<T>(-1.0) * f1 * jacobian1d * w;
 }
 
+// Points of an edge in increasing order, (ix, iz) (interface_container.tpp:15-48).
+inline void edge_point(int edge, int i, int &ix, int &iz) {
+  switch (edge) {
+    case BOTTOM: ix = i; iz = 0; break;
+    case TOP: ix = i; iz = NGLL - 1; break;
+    case LEFT: ix = 0; iz = i; break;
+    default: ix = NGLL - 1; iz = i; break;  // RIGHT
+  }
+}
+
+// Edge normal scaled by the Jacobian (core/specfem/point/partial_derivatives.hpp:371-396).
+template <class T> void edge_normal(const Assembly<T> &a, size_t p, int edge, T &nx, T &nz) {
+  const T J = a.jac[p];
+  switch (edge) {
+    case BOTTOM: nx = static_cast<T>(-1.0) * a.gammax[p] * J; nz = static_cast<T>(-1.0) * a.gammaz[p] * J; break;
+    case TOP: nx = a.gammax[p] * J; nz = a.gammaz[p] * J; break;
+    case LEFT: nx = static_cast<T>(-1.0) * a.xix[p] * J; nz = static_cast<T>(-1.0) * a.xiz[p] * J; break;
+    default: nx = a.xix[p] * J; nz = a.xiz[p] * J; break;  // RIGHT
+  }
+}
+
+// Elastic-acoustic interface container
+// (include/compute/coupled_interfaces/interface_container.tpp:50-365):
+// the one pair of edges of the two elements whose points coincide (relative
+// distance <= 1e-10 of the first element's edge spacing), the GLL weight
+// along the edge as the factor and the elastic element's edge normals.
+template <class T> void build_coupling(Assembly<T> &a, const MeshIn &m) {
+  const int edges[4] = {BOTTOM, TOP, LEFT, RIGHT};
+  a.coupling.clear();
+  for (int i = 0; i < m.nfs; i++) {
+    CoupledEdge<T> ce{};
+    ce.e_el = a.mesh_to_compute[m.fs_elastic[i]];
+    ce.e_ac = a.mesh_to_compute[m.fs_acoustic[i]];
+    if (!((a.medium[ce.e_el] == 0 && a.medium[ce.e_ac] == 1) || (a.medium[ce.e_el] == 1 && a.medium[ce.e_ac] == 0)))
+      throw std::runtime_error("Coupled Interfaces: Interface is not between the correct mediums");
+    const int e1 = ce.e_el, e2 = ce.e_ac;
+    int found = 0;
+    for (int ed1 : edges)
+      for (int ed2 : edges) {
+        if (ed1 == ed2) continue;
+        const T edge_size = (ed1 == BOTTOM || ed1 == TOP)
+                                ? a.coord[2 * pidx<T>(e1, 0, 1)] - a.coord[2 * pidx<T>(e1, 0, 0)]
+                                : a.coord[2 * pidx<T>(e1, 1, 0) + 1] - a.coord[2 * pidx<T>(e1, 0, 0) + 1];
+        bool connected = true;
+        for (int ip = 0; ip < NGLL && connected; ip++) {
+          int ix1, iz1, ix2, iz2;
+          edge_point(ed1, ip, ix1, iz1);
+          edge_point(ed2, ip, ix2, iz2);
+          const size_t p1 = pidx<T>(e1, iz1, ix1), p2 = pidx<T>(e2, iz2, ix2);
+          const T dx = a.coord[2 * p1] - a.coord[2 * p2], dz = a.coord[2 * p1 + 1] - a.coord[2 * p2 + 1];
+          const T distance = std::sqrt(dx * dx + dz * dz);
+          if (distance / edge_size > 1.e-10) connected = false;
+        }
+        if (connected) {
+          ce.edge_el = ed1;
+          ce.edge_ac = ed2;
+          found++;
+        }
+      }
+    if (found != 1) throw std::runtime_error("Number of connected edges is not 1");
+    for (int ip = 0; ip < NGLL; ip++) {
+      int ix1, iz1, ix2, iz2;
+      edge_point(ce.edge_el, ip, ix1, iz1);
+      edge_point(ce.edge_ac, ip, ix2, iz2);
+      T n1x, n1z, n2x, n2z;
+      edge_normal(a, pidx<T>(e1, iz1, ix1), ce.edge_el, n1x, n1z);
+      edge_normal(a, pidx<T>(e2, iz2, ix2), ce.edge_ac, n2x, n2z);
+      if ((std::abs(n1x + n2x) > 1.e-4 * (std::abs(n1x - n2x) / 2)) &&
+          (std::abs(n1z + n2z) > 1.e-4 * (std::abs(n1z - n2z) / 2)))
+        throw std::runtime_error("Edge normals need to be opposite in direction");
+      ce.factor[ip] = (ce.edge_el == BOTTOM || ce.edge_el == TOP) ? a.q.w[ix1] : a.q.w[iz1];
+      ce.nx[ip] = n1x;
+      ce.nz[ip] = n1z;
+    }
+    a.coupling.push_back(ce);
+  }
+}
+
 template <class T> Assembly<T> *build_assembly(const MeshIn &m, int chunk, double dt) {
   auto *a = new Assembly<T>();
   a->nspec = m.nspec;
@@ -386,26 +522,47 @@ template <class T> Assembly<T> *build_assembly(const MeshIn &m, int chunk, doubl
   a->chunk = chunk;
   a->q = make_quadrature<T>();
   const int nspec = m.nspec;
-  for (int i = 0; i < m.nmat; i++)
-    if (!(m.cs[i] > 0.0)) throw std::runtime_error("oracle: only elastic P-SV isotropic materials");
+  // Materials with vs = 0 are acoustic
+  // (src/io/mesh/impl/fortran/dim2/read_material_properties.cpp:172-238).
+  std::vector<int> mat_acoustic(m.nmat);
+  for (int i = 0; i < m.nmat; i++) mat_acoustic[i] = m.cs[i] == 0.0 ? 1 : 0;
+  std::vector<int> medium_mesh(nspec);
+  for (int e = 0; e < nspec; e++) medium_mesh[e] = mat_acoustic[m.kmato[e]];
 
-  // Element boundary tags (src/mesh/dim2/tags/tags.cpp) and compute ordering
-  // (src/compute/compute_mesh.cpp:240-279): elastic/isotropic/none first, then
-  // elastic/isotropic/stacey, each in mesh order.
-  std::vector<int> stacey_mesh(nspec, 0);
-  for (int i = 0; i < m.nabs; i++) stacey_mesh[m.abs_ispec[i]] = 1;
+  // Element boundary tags (src/mesh/dim2/tags/tags.cpp:5-47) and compute
+  // ordering (src/compute/compute_mesh.cpp:240-279) by the element types of
+  // include/enumerations/material_definitions.hpp:76-103: elastic none,
+  // elastic Stacey, acoustic none, free surface, Stacey, composite, each in
+  // mesh order.
+  std::vector<int> tag_mesh(nspec, BT_NONE);
+  for (int i = 0; i < m.nabs; i++) tag_mesh[m.abs_ispec[i]] = add_tag(tag_mesh[m.abs_ispec[i]], BT_STACEY);
+  for (int i = 0; i < m.nafs; i++) {
+    const int e = m.afs_ispec[i];
+    if (!medium_mesh[e]) throw std::runtime_error("Acoustic free surface boundary is not an acoustic element");
+    tag_mesh[e] = add_tag(tag_mesh[e], BT_AFS);
+  }
+  for (int e = 0; e < nspec; e++)
+    if (!medium_mesh[e] && tag_mesh[e] != BT_NONE && tag_mesh[e] != BT_STACEY)
+      throw std::runtime_error("elastic element with a free-surface tag");
+  const int group_order[6][2] = {{0, BT_NONE}, {0, BT_STACEY}, {1, BT_NONE}, {1, BT_AFS}, {1, BT_STACEY},
+                                 {1, BT_COMPOSITE}};
   a->compute_to_mesh.clear();
-  for (int pass = 0; pass < 2; pass++)
+  for (auto &grp : group_order)
     for (int e = 0; e < nspec; e++)
-      if (stacey_mesh[e] == pass) a->compute_to_mesh.push_back(e);
+      if (medium_mesh[e] == grp[0] && tag_mesh[e] == grp[1]) a->compute_to_mesh.push_back(e);
   a->mesh_to_compute.assign(nspec, 0);
   a->is_stacey.assign(nspec, 0);
+  a->medium.assign(nspec, 0);
+  a->btag.assign(nspec, 0);
   for (int c = 0; c < nspec; c++) {
-    a->mesh_to_compute[a->compute_to_mesh[c]] = c;
-    a->is_stacey[c] = stacey_mesh[a->compute_to_mesh[c]];
+    const int e = a->compute_to_mesh[c];
+    a->mesh_to_compute[e] = c;
+    a->medium[c] = medium_mesh[e];
+    a->btag[c] = tag_mesh[e];
+    a->is_stacey[c] = (tag_mesh[e] == BT_STACEY || tag_mesh[e] == BT_COMPOSITE) ? 1 : 0;
   }
   a->nspec_none = 0;
-  for (int c = 0; c < nspec; c++) a->nspec_none += a->is_stacey[c] ? 0 : 1;
+  for (int c = 0; c < nspec; c++) a->nspec_none += (a->medium[c] == 0 && !a->is_stacey[c]) ? 1 : 0;
 
   // Control nodes in compute order (compute_mesh.cpp:172-197), as type_real.
   a->ctrl.resize(static_cast<size_t>(nspec) * m.ngnod * 2);
@@ -465,18 +622,23 @@ template <class T> Assembly<T> *build_assembly(const MeshIn &m, int chunk, doubl
       }
 
   // Properties per point (include/medium/dim2/elastic/isotropic/material.hpp:62-77,126).
-  a->kappa.resize(np); a->mu.resize(np); a->rho.resize(np);
+  // Acoustic (include/medium/dim2/acoustic/isotropic/material.hpp:44-101):
+  // kappa = rho cp^2, rho_inverse = 1 / rho.
+  a->kappa.resize(np); a->mu.resize(np); a->rho.resize(np); a->rho_inv.assign(np, 0);
   for (int c = 0; c < nspec; c++) {
     const int mat = m.kmato[a->compute_to_mesh[c]];
     const T density = static_cast<T>(m.rho[mat]);
     const T cs = static_cast<T>(m.cs[mat]);
     const T cp = static_cast<T>(m.cp[mat]);
-    const T mu = density * cs * cs;
-    const T kappa = density * (cp * cp - (4.0 / 3.0) * cs * cs);
+    const bool ac = a->medium[c] == 1;
+    const T mu = ac ? static_cast<T>(0) : density * cs * cs;
+    const T kappa = ac ? density * cp * cp : density * (cp * cp - (4.0 / 3.0) * cs * cs);
+    const T rinv = ac ? static_cast<T>(1.0) / density : static_cast<T>(0);
     for (int k = 0; k < NGLL * NGLL; k++) {
       a->kappa[static_cast<size_t>(c) * NGLL * NGLL + k] = kappa;
       a->mu[static_cast<size_t>(c) * NGLL * NGLL + k] = mu;
       a->rho[static_cast<size_t>(c) * NGLL * NGLL + k] = density;
+      a->rho_inv[static_cast<size_t>(c) * NGLL * NGLL + k] = rinv;
     }
   }
 
@@ -527,6 +689,16 @@ template <class T> Assembly<T> *build_assembly(const MeshIn &m, int chunk, doubl
     }
   }
 
+  // Acoustic free-surface points (acoustic_free_surface.cpp:28-103).
+  a->dir_tag.assign(np, 0);
+  for (int i = 0; i < m.nafs; i++) {
+    const int c = a->mesh_to_compute[m.afs_ispec[i]];
+    for (int iz = 0; iz < NGLL; iz++)
+      for (int ix = 0; ix < NGLL; ix++)
+        if (on_boundary(m.afs_type[i], iz, ix)) a->dir_tag[pidx<T>(c, iz, ix)] = 1;
+  }
+
+  build_coupling(*a, m);
   a->dt = dt;
   compute_mass(*a);
   return a;
@@ -535,19 +707,40 @@ template <class T> Assembly<T> *build_assembly(const MeshIn &m, int chunk, doubl
 // Mass matrix (include/kokkos_kernels/impl/compute_mass_matrix.tpp:80-114,
 // include/medium/dim2/elastic/isotropic/mass_matrix.tpp:5-16, Stacey term
 // stacey.tpp:641-677), then inversion (invert_mass_matrix.tpp:40-47).
+// Acoustic Stacey point term (stacey.tpp:172-213): -w |n| rho_vpinverse
+// times the velocity, rho_vpinverse = sqrt(rho_inverse / kappa)
+// (core/specfem/point/properties.hpp:68-76).
+template <class T> inline void stacey_traction_ac(T nx, T nz, T w, T rho_inv, T kappa, T v, T &t) {
+  const T factor = w * std::sqrt(nx * nx + nz * nz);
+  const T kappa_inverse = static_cast<T>(1.0) / kappa;
+  const T rho_vpinverse = std::sqrt(rho_inv * kappa_inverse);
+  t += static_cast<T>(-1.0) * factor * rho_vpinverse * v;
+}
+
 template <class T> void compute_mass(Assembly<T> &A) {
   Assembly<T> *a = &A;
   const int nspec = a->nspec;
   const double dt = a->dt;
-  const int nglob = a->nglob;
+  const int nglob = a->nglob_el;
   std::vector<T> mass(2 * static_cast<size_t>(nglob), 0);
+  std::vector<T> mass_ac(static_cast<size_t>(a->nglob_ac), 0);
   const T velocity_factor = static_cast<T>(-1.0) * static_cast<T>(dt) * static_cast<T>(0.5);
   for (int c = 0; c < nspec; c++)
     for (int iz = 0; iz < NGLL; iz++)
       for (int ix = 0; ix < NGLL; ix++) {
         size_t p = pidx<T>(c, iz, ix);
-        T m0 = a->rho[p], m1 = a->rho[p];
         const T f = a->q.w[ix] * a->q.w[iz] * a->jac[p];
+        if (a->medium[c] == 1) {
+          // include/medium/dim2/acoustic/isotropic/mass_matrix.tpp: 1 / kappa
+          T m0 = static_cast<T>(1.0) / a->kappa[p];
+          m0 *= f;
+          if (a->bnd_tag[p])
+            stacey_traction_ac<T>(a->bnd_nx[p], a->bnd_nz[p], a->bnd_w[p], a->rho_inv[p], a->kappa[p],
+                                  velocity_factor, m0);
+          mass_ac[a->index_ac[a->ibool[p]]] += m0;
+          continue;
+        }
+        T m0 = a->rho[p], m1 = a->rho[p];
         m0 *= f;
         m1 *= f;
         if (a->bnd_tag[p])
@@ -559,6 +752,8 @@ template <class T> void compute_mass(Assembly<T> &A) {
       }
   a->mass_inverse.resize(2 * static_cast<size_t>(nglob));
   for (size_t i = 0; i < mass.size(); i++) a->mass_inverse[i] = static_cast<T>(1.0) / mass[i];
+  a->mass_inverse_ac.resize(mass_ac.size());
+  for (size_t i = 0; i < mass_ac.size(); i++) a->mass_inverse_ac[i] = static_cast<T>(1.0) / mass_ac[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -663,6 +858,7 @@ struct SourceIn {
   double angle;                 // force
   double mxx, mzz, mxz;         // moment tensor
   double f0, tshift, factor;    // Ricker
+  int stf = 0;                  // 0 Ricker, 1 Dirac
 };
 
 struct ReceiverIn {
@@ -676,6 +872,16 @@ template <class T> T ricker(T t, T f0, T factor) {
   const T d2 = (1.0 - 2.0 * a * t * t) * std::exp(-a * t * t);
   return -1.0 * factor * d2;
 }
+
+// Dirac, src/source_time_function/dirac.cpp:24-48 and utilities.cpp:6-12:
+// -factor times a Gaussian of half duration 1/f0, f0 = 1 / (10 dt).
+template <class T> T dirac(T t, T f0, T factor) {
+  const T pi = static_cast<T>(M_PI);
+  const T a = pi * pi * f0 * f0;
+  const T g = -1.0 * std::exp(-a * t * t) / (2.0 * a);
+  return -1.0 * factor * g;
+}
+template <class T> T dirac_f0(T dt) { return static_cast<T>(1.0 / (10.0 * dt)); }
 
 // One element's weak-form internal force (stiffness kernel body,
 // include/kokkos_kernels/impl/compute_stiffness_interaction.tpp:143-259):
@@ -754,11 +960,14 @@ template <class T> struct Forcing {
   std::vector<T> arr;  // (nsrc, 2, 5, 5)
   std::vector<T> stf;  // (nsrc, nstep, 2)
   std::vector<int> order;
+  // Element type groups of the source kernel launches (domain_kernels.hpp:
+  // 81-95): boundary tags none, free surface, Stacey, composite; a medium's
+  // update applies its own sources only.
   void finish_order(const Assembly<T> &a) {
     order.clear();
-    for (int pass = 0; pass < 2; pass++)
+    for (int pass = 0; pass < 4; pass++)
       for (size_t s = 0; s < ispec.size(); s++)
-        if (a.is_stacey[ispec[s]] == pass) order.push_back(static_cast<int>(s));
+        if (a.btag[ispec[s]] == pass) order.push_back(static_cast<int>(s));
   }
 };
 
@@ -773,7 +982,8 @@ Forcing<T> make_sources(const Assembly<T> &a, int nsrc, const SourceIn *src, int
   T min_t0 = std::numeric_limits<T>::max(), min_tshift = std::numeric_limits<T>::max();
   std::vector<T> src_t0(nsrc);
   for (int s = 0; s < nsrc; s++) {
-    T hdur = 1.0 / static_cast<T>(src[s].f0);
+    const T f0s = src[s].stf == 1 ? dirac_f0<T>(dt) : static_cast<T>(src[s].f0);
+    T hdur = 1.0 / f0s;
     src_t0[s] = -1.2 * hdur + static_cast<T>(src[s].tshift);
     if (src_t0[s] < min_t0) min_t0 = src_t0[s];
     if (static_cast<T>(src[s].tshift) < min_tshift) min_tshift = static_cast<T>(src[s].tshift);
@@ -796,7 +1006,13 @@ Forcing<T> make_sources(const Assembly<T> &a, int nsrc, const SourceIn *src, int
     T hx[NGLL], hpx[NGLL], hg[NGLL], hpg[NGLL];
     lagrange_interpolants<T>(loc.xi, a.q.xi, hx, hpx);
     lagrange_interpolants<T>(loc.gamma, a.q.xi, hg, hpg);
-    if (src[s].type == 0) {
+    if (a.medium[loc.ispec] == 1) {
+      // force source in an acoustic element: the Lagrange product alone
+      // (src/source/force_source.cpp:51-56); moment tensors act on elastic media only
+      if (src[s].type != 0) throw std::invalid_argument("moment-tensor source in an acoustic element");
+      for (int iz = 0; iz < NGLL; iz++)
+        for (int ix = 0; ix < NGLL; ix++) arr[iz * NGLL + ix] = hx[ix] * hg[iz];
+    } else if (src[s].type == 0) {
       // force source, src/source/force_source.cpp:70-78
       const T ang = static_cast<T>(src[s].angle);
       const T pi = static_cast<T>(M_PI);
@@ -832,8 +1048,10 @@ Forcing<T> make_sources(const Assembly<T> &a, int nsrc, const SourceIn *src, int
         }
     }
     const T tshift = user_defined ? static_cast<T>(src[s].tshift) : src_t0[s] - min_t0;
+    const T f0s = src[s].stf == 1 ? dirac_f0<T>(dt) : static_cast<T>(src[s].f0);
     for (int i = 0; i < nstep; i++) {
-      const T v = ricker<T>(t0 + i * dt - tshift, static_cast<T>(src[s].f0), static_cast<T>(src[s].factor));
+      const T v = src[s].stf == 1 ? dirac<T>(t0 + i * dt - tshift, f0s, static_cast<T>(src[s].factor))
+                                  : ricker<T>(t0 + i * dt - tshift, f0s, static_cast<T>(src[s].factor));
       f.stf[(static_cast<size_t>(s) * nstep + i) * 2] = v;
       f.stf[(static_cast<size_t>(s) * nstep + i) * 2 + 1] = v;
     }
@@ -896,15 +1114,15 @@ template <class T> Receivers<T> make_receivers(const Assembly<T> &a, int nrec, c
 // One simulation field (include/compute/fields/impl/field_impl.hpp:30-38):
 // (comp, nglob) on the per-medium index.
 template <class T> struct Field {
-  int nglob = 0;
+  int nglob = 0, ncomp = 2;
   std::vector<T> u, v, acc;
-  explicit Field(int n) : nglob(n), u(2 * static_cast<size_t>(n), 0), v(u), acc(u) {}
+  explicit Field(int n, int nc = 2) : nglob(n), ncomp(nc), u(nc * static_cast<size_t>(n), 0), v(u), acc(u) {}
 };
 
 // Newmark predictor (newmark.tpp:60-116).  The backward field uses
 // (-dt, -dt/2, dt^2/2) (newmark.tpp:225-242).
 template <class T> void predictor(Field<T> &f, T deltat, T deltatover2, T deltasquareover2, bool serial) {
-  const size_t n = 2 * static_cast<size_t>(f.nglob);
+  const size_t n = f.ncomp * static_cast<size_t>(f.nglob);
 #pragma omp parallel for if (!serial) schedule(static)
   for (size_t i = 0; i < n; i++) {
     T add_u = 0;
@@ -920,8 +1138,8 @@ template <class T> void predictor(Field<T> &f, T deltat, T deltatover2, T deltas
 // Mass division (divide_mass_matrix.tpp:42-49) and corrector (newmark.tpp:11-56;
 // -dt/2 for the backward field, :204-222).
 template <class T> void divide_correct(const Assembly<T> &a, Field<T> &f, T deltatover2, bool serial) {
-  const size_t n = 2 * static_cast<size_t>(f.nglob);
-  const T *minv = a.mass_inverse.data();
+  const size_t n = f.ncomp * static_cast<size_t>(f.nglob);
+  const T *minv = f.ncomp == 1 ? a.mass_inverse_ac.data() : a.mass_inverse.data();
 #pragma omp parallel for if (!serial) schedule(static)
   for (size_t i = 0; i < n; i++) {
     f.acc[i] = f.acc[i] * minv[i];
@@ -949,6 +1167,7 @@ void update_forces(const Assembly<T> &a, Field<T> &f, const Forcing<T> &F, int i
   T *ax = f.acc.data(), *az = f.acc.data() + nglob;
   const T *ux = f.u.data(), *uz = f.u.data() + nglob, *vx = f.v.data(), *vz = f.v.data() + nglob;
   for (int s : F.order) {
+    if (a.medium[F.ispec[s]] != 0) continue;
     const T *arr = &F.arr[static_cast<size_t>(s) * 2 * NGLL * NGLL];
     const T sx = F.stf[(static_cast<size_t>(s) * F.nstep + istep) * 2];
     const T sz = F.stf[(static_cast<size_t>(s) * F.nstep + istep) * 2 + 1];
@@ -980,6 +1199,7 @@ void update_forces(const Assembly<T> &a, Field<T> &f, const Forcing<T> &F, int i
   if (serial) {
     T out[NGLL][NGLL][2];
     for (int e = 0; e < a.nspec; e++) {
+      if (a.medium[e] != 0) continue;
       element(e, out);
       for (int iz = 0; iz < NGLL; iz++)
         for (int ix = 0; ix < NGLL; ix++) {
@@ -991,6 +1211,7 @@ void update_forces(const Assembly<T> &a, Field<T> &f, const Forcing<T> &F, int i
   } else {
 #pragma omp parallel for schedule(static)
     for (int e = 0; e < a.nspec; e++) {
+      if (a.medium[e] != 0) continue;
       T out[NGLL][NGLL][2];
       element(e, out);
       for (int iz = 0; iz < NGLL; iz++)
@@ -1037,14 +1258,62 @@ template <class T> void element_pressure(const Assembly<T> &a, const Field<T> &f
     }
 }
 
+// Acoustic receiver values at the points of element e
+// (include/medium/dim2/acoustic/isotropic/wavefield.hpp:14-84): for
+// displacement, velocity and acceleration the gradient of the potential (or
+// its time derivatives) times 1/rho, for pressure -chi_dot_dot.
+template <class T> void element_acoustic_values(const Assembly<T> &a, const Field<T> &ac, int e, int type,
+                                                T out[NGLL][NGLL][2]) {
+  const auto &q = a.q;
+  const std::vector<T> &fld = type == 0 ? ac.u : (type == 1 ? ac.v : ac.acc);
+  T l[NGLL][NGLL];
+  for (int iz = 0; iz < NGLL; iz++)
+    for (int ix = 0; ix < NGLL; ix++) l[iz][ix] = fld[a.index_ac[a.ibool[pidx<T>(e, iz, ix)]]];
+  for (int iz = 0; iz < NGLL; iz++)
+    for (int ix = 0; ix < NGLL; ix++) {
+      if (type == 3) {
+        out[iz][ix][0] = static_cast<T>(-1.0 * l[iz][ix]);
+        out[iz][ix][1] = 0;
+        continue;
+      }
+      T dxi = 0.0, dgam = 0.0;
+      for (int k = 0; k < NGLL; k++) {
+        dxi += q.hprime[ix][k] * l[iz][k];
+        dgam += q.hprime[iz][k] * l[k][ix];
+      }
+      const size_t p = pidx<T>(e, iz, ix);
+      const T du0 = a.xix[p] * dxi + a.gammax[p] * dgam;
+      const T du1 = a.xiz[p] * dxi + a.gammaz[p] * dgam;
+      out[iz][ix][0] = a.rho_inv[p] * du0;
+      out[iz][ix][1] = a.rho_inv[p] * du1;
+    }
+}
+
 // Seismograms (compute_seismogram.tpp:103-135; include/algorithms/interpolate.hpp:176-237).
+// ac: the acoustic field of a coupled mesh (receivers in acoustic elements).
 template <class T>
 void seismograms(const Assembly<T> &a, const Field<T> &f, const Receivers<T> &R, int ntypes, const int *types,
-                 int isig, double *seis_out) {
+                 int isig, double *seis_out, const Field<T> *ac = nullptr) {
   const int nglob = f.nglob, nrec = static_cast<int>(R.ispec.size());
   for (int it = 0; it < ntypes; it++) {
+    for (int r = 0; r < nrec; r++) {
+      if (a.medium[R.ispec[r]] != 1) continue;
+      T w[NGLL][NGLL][2];
+      element_acoustic_values<T>(a, *ac, R.ispec[r], types[it], w);
+      T s0 = 0, s1 = 0;
+      for (int iz = 0; iz < NGLL; iz++)
+        for (int ix = 0; ix < NGLL; ix++) {
+          const T h = R.h[(static_cast<size_t>(r) * NGLL + iz) * NGLL + ix];
+          s0 += h * w[iz][ix][0];
+          s1 += h * w[iz][ix][1];
+        }
+      const size_t o = ((static_cast<size_t>(isig) * ntypes + it) * nrec + r) * 2;
+      seis_out[o] = s0;
+      seis_out[o + 1] = types[it] == 3 ? 0.0 : s1;
+    }
     if (types[it] == 3) {
       for (int r = 0; r < nrec; r++) {
+        if (a.medium[R.ispec[r]] != 0) continue;
         T pr[NGLL][NGLL];
         element_pressure<T>(a, f, R.ispec[r], pr);
         T s0 = 0;
@@ -1059,6 +1328,7 @@ void seismograms(const Assembly<T> &a, const Field<T> &f, const Receivers<T> &R,
     const std::vector<T> &fld = types[it] == 0 ? f.u : (types[it] == 1 ? f.v : f.acc);
     const T *fx = fld.data(), *fz = fld.data() + nglob;
     for (int r = 0; r < nrec; r++) {
+      if (a.medium[R.ispec[r]] != 0) continue;
       T s0 = 0, s1 = 0;
       for (int iz = 0; iz < NGLL; iz++)
         for (int ix = 0; ix < NGLL; ix++) {
@@ -1123,6 +1393,165 @@ void run_forward(const Assembly<T> &a, int nsrc, const SourceIn *src, int nrec, 
     std::copy(f.u.begin(), f.u.end(), final_fields);
     std::copy(f.v.begin(), f.v.end(), final_fields + 2 * static_cast<size_t>(a.nglob));
     std::copy(f.acc.begin(), f.acc.end(), final_fields + 4 * static_cast<size_t>(a.nglob));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Acoustic medium and elastic-acoustic coupling (2-D forward).
+// ---------------------------------------------------------------------------
+
+// Edge iterator of the coupling kernel (include/policies/edge.hpp, struct
+// edge<dim2>): the self and coupled point of edge point ipoint, as (iz, ix).
+inline void self_point(int edge, int ip, int &iz, int &ix) {
+  const int n = NGLL - 1;
+  switch (edge) {
+    case BOTTOM: iz = 0; ix = ip; break;
+    case TOP: iz = n; ix = n - ip; break;
+    case LEFT: iz = ip; ix = 0; break;
+    default: iz = n - ip; ix = n; break;  // RIGHT
+  }
+}
+inline void coupled_point(int edge, int ip, int &iz, int &ix) {
+  const int n = NGLL - 1;
+  switch (edge) {
+    case BOTTOM: iz = 0; ix = n - ip; break;
+    case TOP: iz = n; ix = ip; break;
+    case LEFT: iz = n - ip; ix = n; break;
+    default: iz = ip; ix = 0; break;  // RIGHT
+  }
+}
+
+// Acoustic element force (compute_stiffness_interaction.tpp:143-259 with
+// the acoustic stress T = rho_inverse grad chi,
+// include/medium/dim2/acoustic/isotropic/stress.hpp), Stacey
+// (stacey.tpp:172-213) then the free surface (dirichlet.hpp: the point's
+// whole contribution set to zero).
+template <class T>
+void element_force_ac(const Assembly<T> &a, int e, const Field<T> &ac, T out[NGLL][NGLL]) {
+  const auto &q = a.q;
+  T l[NGLL][NGLL];
+  for (int iz = 0; iz < NGLL; iz++)
+    for (int ix = 0; ix < NGLL; ix++) l[iz][ix] = ac.u[a.index_ac[a.ibool[pidx<T>(e, iz, ix)]]];
+  T F[NGLL][NGLL][2];
+  for (int iz = 0; iz < NGLL; iz++)
+    for (int ix = 0; ix < NGLL; ix++) {
+      T dxi = 0.0, dgam = 0.0;
+      for (int k = 0; k < NGLL; k++) {
+        dxi += q.hprime[ix][k] * l[iz][k];
+        dgam += q.hprime[iz][k] * l[k][ix];
+      }
+      const size_t p = pidx<T>(e, iz, ix);
+      const T xix = a.xix[p], xiz = a.xiz[p], gammax = a.gammax[p], gammaz = a.gammaz[p], J = a.jac[p];
+      const T du0 = xix * dxi + gammax * dgam;
+      const T du1 = xiz * dxi + gammaz * dgam;
+      const T T0 = a.rho_inv[p] * du0, T1 = a.rho_inv[p] * du1;
+      F[iz][ix][0] = J * (T0 * xix + T1 * xiz);
+      F[iz][ix][1] = J * (T0 * gammax + T1 * gammaz);
+    }
+  for (int iz = 0; iz < NGLL; iz++)
+    for (int ix = 0; ix < NGLL; ix++) {
+      T t1 = 0.0, t2 = 0.0;
+      for (int k = 0; k < NGLL; k++) {
+        t1 += F[iz][k][0] * q.hprime_wgll[ix][k];
+        t2 += F[k][ix][1] * q.hprime_wgll[iz][k];
+      }
+      T r = q.w[iz] * t1 + q.w[ix] * t2;
+      r *= static_cast<T>(-1.0);
+      const size_t p = pidx<T>(e, iz, ix);
+      if (a.bnd_tag[p])
+        stacey_traction_ac<T>(a.bnd_nx[p], a.bnd_nz[p], a.bnd_w[p], a.rho_inv[p], a.kappa[p],
+                              ac.v[a.index_ac[a.ibool[p]]], r);
+      if (a.dir_tag[p]) r = 0;
+      out[iz][ix] = r;
+    }
+}
+
+// update_wavefields<acoustic> (domain_kernels.hpp:69-126): coupling from the
+// elastic displacement (coupled_interface.tpp: factor n.u, n the acoustic
+// side's normal, i.e. minus the elastic one), sources (stf * lagrange /
+// kappa, include/medium/dim2/acoustic/isotropic/source.hpp), stiffness.
+template <class T>
+void update_acoustic(const Assembly<T> &a, Field<T> &ac, const Field<T> &el, const Forcing<T> &F, int istep) {
+  T *acc = ac.acc.data();
+  const size_t ne = static_cast<size_t>(el.nglob);
+  for (const auto &ce : a.coupling)
+    for (int ip = 0; ip < NGLL; ip++) {
+      int sz, sx, cz, cx;
+      self_point(ce.edge_ac, ip, sz, sx);
+      coupled_point(ce.edge_el, ip, cz, cx);
+      const int gs = a.index_ac[a.ibool[pidx<T>(ce.e_ac, sz, sx)]];
+      const int gc = a.medium_index[a.ibool[pidx<T>(ce.e_el, cz, cx)]];
+      const T nx = -ce.nx[ip], nz = -ce.nz[ip];
+      acc[gs] += ce.factor[ip] * (nx * el.u[gc] + nz * el.u[ne + gc]);
+    }
+  for (int s : F.order) {
+    const int e = F.ispec[s];
+    if (a.medium[e] != 1) continue;
+    const T *arr = &F.arr[static_cast<size_t>(s) * 2 * NGLL * NGLL];
+    const T sv = F.stf[(static_cast<size_t>(s) * F.nstep + istep) * 2];
+    for (int iz = 0; iz < NGLL; iz++)
+      for (int ix = 0; ix < NGLL; ix++) {
+        const size_t p = pidx<T>(e, iz, ix);
+        acc[a.index_ac[a.ibool[p]]] += sv * arr[iz * NGLL + ix] / a.kappa[p];
+      }
+  }
+  T out[NGLL][NGLL];
+  for (int e = 0; e < a.nspec; e++) {
+    if (a.medium[e] != 1) continue;
+    element_force_ac<T>(a, e, ac, out);
+    for (int iz = 0; iz < NGLL; iz++)
+      for (int ix = 0; ix < NGLL; ix++) acc[a.index_ac[a.ibool[pidx<T>(e, iz, ix)]]] += out[iz][ix];
+  }
+}
+
+// The elastic side of the coupling (coupled_interface.tpp): factor n chi_dot_dot.
+template <class T> void couple_into_elastic(const Assembly<T> &a, Field<T> &el, const Field<T> &ac) {
+  const size_t ne = static_cast<size_t>(el.nglob);
+  for (const auto &ce : a.coupling)
+    for (int ip = 0; ip < NGLL; ip++) {
+      int sz, sx, cz, cx;
+      self_point(ce.edge_el, ip, sz, sx);
+      coupled_point(ce.edge_ac, ip, cz, cx);
+      const int gs = a.medium_index[a.ibool[pidx<T>(ce.e_el, sz, sx)]];
+      const int gc = a.index_ac[a.ibool[pidx<T>(ce.e_ac, cz, cx)]];
+      el.acc[gs] += ce.factor[ip] * ce.nx[ip] * ac.acc[gc];
+      el.acc[ne + gs] += ce.factor[ip] * ce.nz[ip] * ac.acc[gc];
+    }
+}
+
+// Forward loop of a mesh with acoustic elements (time_marching.tpp:33-84):
+// predictors, acoustic update and corrector, elastic update (coupling first)
+// and corrector, seismograms.  Serial order throughout.
+template <class T>
+void run_forward_coupled(const Assembly<T> &a, int nsrc, const SourceIn *src, int nrec, const ReceiverIn *rec,
+                         int nstep, double dt_in, int nsb, int ntypes, const int *types, double *seis_out,
+                         double *t0_out, T *final_fields /* optional: (3, 2, nglob_el) then (3, nglob_ac) */) {
+  const T dt = static_cast<T>(dt_in);
+  const T deltat = dt, deltatover2 = dt / 2.0, deltasquareover2 = dt * dt / 2.0;
+  T t0 = 0;
+  const Forcing<T> S = make_sources<T>(a, nsrc, src, nstep, dt, &t0);
+  if (t0_out) *t0_out = t0;
+  const Receivers<T> R = make_receivers<T>(a, nrec, rec);
+  Field<T> el(a.nglob_el), ac(a.nglob_ac, 1);
+  int isig = 0;
+  for (int istep = 0; istep < nstep; istep++) {
+    predictor<T>(ac, deltat, deltatover2, deltasquareover2, true);
+    predictor<T>(el, deltat, deltatover2, deltasquareover2, true);
+    update_acoustic<T>(a, ac, el, S, istep);
+    divide_correct<T>(a, ac, deltatover2, true);
+    couple_into_elastic<T>(a, el, ac);
+    update_forces<T>(a, el, S, istep, StaceyMode::normal, nullptr, true);
+    divide_correct<T>(a, el, deltatover2, true);
+    if (istep % nsb == 0) seismograms<T>(a, el, R, ntypes, types, isig++, seis_out, &ac);
+  }
+  if (final_fields) {
+    const size_t ne = 2 * static_cast<size_t>(a.nglob_el), na = a.nglob_ac;
+    std::copy(el.u.begin(), el.u.end(), final_fields);
+    std::copy(el.v.begin(), el.v.end(), final_fields + ne);
+    std::copy(el.acc.begin(), el.acc.end(), final_fields + 2 * ne);
+    std::copy(ac.u.begin(), ac.u.end(), final_fields + 3 * ne);
+    std::copy(ac.v.begin(), ac.v.end(), final_fields + 3 * ne + na);
+    std::copy(ac.acc.begin(), ac.acc.end(), final_fields + 3 * ne + 2 * na);
   }
 }
 
@@ -1297,9 +1726,11 @@ const char *orc_last_error() { return g_err.c_str(); }
 
 int orc2d_create(int precision, int nspec, int ngnod, int npgeo, const double *coorg, const int *knods,
                  const int *kmato, int nmat, const double *rho, const double *cp, const double *cs, int nabs,
-                 const int *abs_ispec, const int *abs_type, int chunk, double dt, void **out) {
+                 const int *abs_ispec, const int *abs_type, int nafs, const int *afs_ispec, const int *afs_type,
+                 int nfs, const int *fs_acoustic, const int *fs_elastic, int chunk, double dt, void **out) {
   return guard([&] {
-    MeshIn m{nspec, ngnod, npgeo, coorg, knods, kmato, nmat, rho, cp, cs, nabs, abs_ispec, abs_type};
+    MeshIn m{nspec, ngnod, npgeo, coorg, knods, kmato, nmat, rho, cp, cs, nabs, abs_ispec, abs_type,
+             nafs, afs_ispec, afs_type, nfs, fs_acoustic, fs_elastic};
     auto *h = new Handle{precision, nullptr};
     if (precision == 4)
       h->a = build_assembly<float>(m, chunk, dt);
@@ -1321,6 +1752,26 @@ int orc2d_nglob(void *h) {
   int n = 0;
   DISPATCH(h, n = asm_of<T>(h).nglob);
   return n;
+}
+
+// (nglob of the elastic field, nglob of the acoustic field)
+void orc2d_nglob_media(void *h, int *out2) {
+  DISPATCH(h, {
+    out2[0] = asm_of<T>(h).nglob_el;
+    out2[1] = asm_of<T>(h).nglob_ac;
+  });
+}
+
+// acoustic field index of every iglob (-1 off the acoustic elements), the
+// medium of every compute element and the acoustic mass inverse (nglob_ac)
+void orc2d_acoustic(void *h, int *index_ac, int *medium, double *mass_inverse_ac) {
+  DISPATCH(h, {
+    auto &a = asm_of<T>(h);
+    if (index_ac) std::copy(a.index_ac.begin(), a.index_ac.end(), index_ac);
+    if (medium) std::copy(a.medium.begin(), a.medium.end(), medium);
+    if (mass_inverse_ac)
+      for (size_t i = 0; i < a.mass_inverse_ac.size(); i++) mass_inverse_ac[i] = a.mass_inverse_ac[i];
+  });
 }
 
 // ibool (nspec, 5, 5) compute order; medium index (nglob); compute_to_mesh (nspec)
@@ -1386,8 +1837,9 @@ namespace {
 std::vector<SourceIn> sources_in(int nsrc, const double *src) {
   std::vector<SourceIn> S(nsrc);
   for (int s = 0; s < nsrc; s++) {
-    const double *r = src + 10 * s;
-    S[s] = SourceIn{static_cast<int>(r[0]), r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9]};
+    const double *r = src + 11 * s;
+    S[s] = SourceIn{static_cast<int>(r[0]), r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9],
+                    static_cast<int>(r[10])};
   }
   return S;
 }
@@ -1400,7 +1852,7 @@ std::vector<ReceiverIn> receivers_in(int nrec, const double *rec) {
 
 extern "C" {
 
-// sources: (nsrc, 10) rows [type, x, z, angle, mxx, mzz, mxz, f0, tshift, factor]
+// sources: (nsrc, 11) rows [type, x, z, angle, mxx, mzz, mxz, f0, tshift, factor, stf (0 Ricker, 1 Dirac)]
 // receivers: (nrec, 3) rows [x, z, angle]
 int orc2d_forward(void *h, int nsrc, const double *src, int nrec, const double *rec, int nstep, double dt, int nsb,
                   int ntypes, const int *types, double *seis, double *t0, int nthreads, double *final_fields) {
@@ -1410,9 +1862,16 @@ int orc2d_forward(void *h, int nsrc, const double *src, int nrec, const double *
     DISPATCH(h, {
       auto &a = asm_of<T>(h);
       std::vector<T> ff;
-      if (final_fields) ff.resize(6 * static_cast<size_t>(a.nglob));
-      run_forward<T>(a, nsrc, S.data(), nrec, R.data(), nstep, dt, nsb, ntypes, types, seis, t0, nthreads,
-                     final_fields ? ff.data() : nullptr);
+      if (a.coupled()) {
+        // final fields: elastic (3, 2, nglob_el) then acoustic (3, nglob_ac)
+        if (final_fields) ff.resize(6 * static_cast<size_t>(a.nglob_el) + 3 * static_cast<size_t>(a.nglob_ac));
+        run_forward_coupled<T>(a, nsrc, S.data(), nrec, R.data(), nstep, dt, nsb, ntypes, types, seis, t0,
+                               final_fields ? ff.data() : nullptr);
+      } else {
+        if (final_fields) ff.resize(6 * static_cast<size_t>(a.nglob));
+        run_forward<T>(a, nsrc, S.data(), nrec, R.data(), nstep, dt, nsb, ntypes, types, seis, t0, nthreads,
+                       final_fields ? ff.data() : nullptr);
+      }
       if (final_fields)
         for (size_t i = 0; i < ff.size(); i++) final_fields[i] = ff[i];
     });
@@ -1427,6 +1886,7 @@ int orc2d_sample(void *h, int nrec, const double *rec, int ntypes, const int *ty
     auto R = receivers_in(nrec, rec);
     DISPATCH(h, {
       auto &a = asm_of<T>(h);
+      if (a.coupled()) throw std::invalid_argument("elastic meshes only");
       Field<T> f(a.nglob);
       const size_t n = 2 * static_cast<size_t>(a.nglob);
       for (size_t i = 0; i < n; i++) {
@@ -1496,6 +1956,7 @@ int orc2d_forward_record(void *h, int nsrc, const double *src, int nrec, const d
     auto R = receivers_in(nrec, rec);
     DISPATCH(h, {
       auto &a = asm_of<T>(h);
+      if (a.coupled()) throw std::invalid_argument("elastic meshes only");
       if (!a.is_stacey.empty())
         for (int e = 0; e < a.nspec; e++)
           if ((e >= a.nspec_none) != (a.is_stacey[e] != 0))
@@ -1527,6 +1988,7 @@ int orc2d_combined(void *h, int nsrc, const double *src, int nadj, const double 
     auto R = receivers_in(nrec, rec);
     DISPATCH(h, {
       auto &a = asm_of<T>(h);
+      if (a.coupled()) throw std::invalid_argument("elastic meshes only");
       std::vector<T> ff(final_fields, final_fields + 6 * static_cast<size_t>(a.nglob));
       std::vector<T> rr(record, record + static_cast<size_t>(nstep) * (a.nspec - a.nspec_none) * NGLL * NGLL * 2);
       const size_t npts = static_cast<size_t>(a.nspec) * NGLL * NGLL;

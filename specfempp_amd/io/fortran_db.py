@@ -15,9 +15,13 @@ The record order follows the reference reader:
   element materials . ``read_material_properties.cpp:458-497``
   interfaces count .. ``src/io/mesh/impl/fortran/dim2/mesh.cpp:85``
   absorbing edges ... ``src/io/mesh/impl/fortran/dim2/read_boundaries.cpp:133-204``
+  acoustic surface .. ``read_boundaries.cpp:206-268`` (element, edge or node)
+  coupled interfaces  ``src/io/mesh/impl/fortran/dim2/read_interfaces.cpp:9-30``
+                      (fluid-solid pairs: acoustic element, then elastic)
 
-Only what the elastic P-SV isotropic path needs is interpreted; the rest is
-skipped record by record so that a malformed file still fails loudly.
+Only what the elastic P-SV and acoustic isotropic paths need is interpreted;
+the rest is skipped record by record so that a malformed file still fails
+loudly.
 """
 from __future__ import annotations
 
@@ -30,6 +34,9 @@ import numpy as np
 # (codeabsread1..4 -> bottom, right, top, left;
 # ``read_boundaries.cpp:166-179``).
 EDGE_BOTTOM, EDGE_RIGHT, EDGE_TOP, EDGE_LEFT = 0, 1, 2, 3
+# Corner codes of an acoustic free-surface entry that names one control node
+# (``read_boundaries.cpp:218-232``).
+CORNER_BOTTOM_LEFT, CORNER_BOTTOM_RIGHT, CORNER_TOP_RIGHT, CORNER_TOP_LEFT = 4, 5, 6, 7
 
 
 @dataclass
@@ -60,6 +67,10 @@ class Mesh2D:
     materials: list = field(default_factory=list)
     abs_ispec: np.ndarray = None  # (nelemabs,) int32 mesh element of each edge
     abs_type: np.ndarray = None   # (nelemabs,) int32 EDGE_* code
+    afs_ispec: np.ndarray = None  # (n,) int32 mesh element of each acoustic free-surface entry
+    afs_type: np.ndarray = None   # (n,) int32 EDGE_* or CORNER_* code
+    fs_acoustic: np.ndarray = None  # (n,) int32 acoustic element of each fluid-solid edge
+    fs_elastic: np.ndarray = None   # (n,) int32 elastic element of the same edge
     nstep: int = 0
     dt: float = 0.0
 
@@ -189,14 +200,25 @@ def read_database_2d(path: str) -> Mesh2D:
         abs_ispec[i] = num - 1
         abs_type[i] = [j for j, c in enumerate(codes) if c][0]
 
-    # Remaining sections (acoustic surface, forcing, interfaces, tangential,
-    # axial) are not needed by the elastic path; they are consumed so that a
-    # truncated file is still detected.
-    for _ in range(nelem_acoustic_surface):
-        r.next()
+    afs_ispec = np.zeros(max(nelem_acoustic_surface, 0), dtype=np.int32)
+    afs_type = np.zeros(max(nelem_acoustic_surface, 0), dtype=np.int32)
+    for i in range(max(nelem_acoustic_surface, 0)):
+        e, kind, n1, n2 = r.read("iiii")
+        if e < 1 or e > nspec:
+            raise ValueError("database.bin: acoustic free surface element out of range")
+        afs_ispec[i] = e - 1
+        afs_type[i] = _surface_type(kind, n1 - 1, n2 - 1, knods[:, e - 1])
     for _ in range(nelem_acforcing):
         r.next()
-    for _ in range(n_fs + n_fp + n_sp):
+    fs_acoustic = np.zeros(max(n_fs, 0), dtype=np.int32)
+    fs_elastic = np.zeros(max(n_fs, 0), dtype=np.int32)
+    for i in range(max(n_fs, 0)):
+        ea, ee = r.read("ii")
+        if not (1 <= ea <= nspec and 1 <= ee <= nspec):
+            raise ValueError("database.bin: fluid-solid interface element out of range")
+        fs_acoustic[i] = ea - 1
+        fs_elastic[i] = ee - 1
+    for _ in range(n_fp + n_sp):
         r.next()
     r.next()  # tangential flags
     for _ in range(max(nnodes_tangential, 0)):
@@ -206,7 +228,25 @@ def read_database_2d(path: str) -> Mesh2D:
 
     return Mesh2D(nspec=nspec, npgeo=npgeo, ngnod=ngnod, coorg=coorg,
                   knods=knods, kmato=kmato, materials=materials,
-                  abs_ispec=abs_ispec, abs_type=abs_type, nstep=nstep, dt=dt)
+                  abs_ispec=abs_ispec, abs_type=abs_type, afs_ispec=afs_ispec, afs_type=afs_type,
+                  fs_acoustic=fs_acoustic, fs_elastic=fs_elastic, nstep=nstep, dt=dt)
+
+
+def _surface_type(kind: int, n1: int, n2: int, nodes) -> int:
+    """Edge or corner of an acoustic free-surface entry from its control nodes
+    (0-based), ``read_boundaries.cpp:206-248``: kind 1 names one node (a
+    corner), otherwise two (an edge)."""
+    c = [int(x) for x in nodes[:4]]
+    if kind == 1:
+        for code, node in zip((CORNER_BOTTOM_LEFT, CORNER_BOTTOM_RIGHT, CORNER_TOP_RIGHT, CORNER_TOP_LEFT), c):
+            if n1 == node:
+                return code
+    else:
+        pairs = ((EDGE_BOTTOM, c[0], c[1]), (EDGE_LEFT, c[0], c[3]), (EDGE_RIGHT, c[1], c[2]), (EDGE_TOP, c[2], c[3]))
+        for code, a, b in pairs:
+            if {n1, n2} == {a, b}:
+                return code
+    raise ValueError("database.bin: could not classify an acoustic free surface entry")
 
 
 def read_fortran_array(path: str, dtype) -> np.ndarray:

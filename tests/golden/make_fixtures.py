@@ -27,19 +27,40 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 from specfempp_amd.io.fortran_db import read_fortran_array, read_stations, read_trace  # noqa: E402
 
 UT = "tests/unit-tests"
+NEWMARK = "displacement_tests/Newmark/serial"
+_RICKER4 = [dict(x=x, z=z, angle=0.0, f0=10.0, factor=1e10, tshift=0.0)
+            for (x, z) in [(500.0, 500.0), (3500.0, 500.0), (500.0, 2500.0), (3500.0, 2500.0)]]
 CASES = {
-    # case name: (fixture dir under tests/unit-tests, seismogram types, source positions)
+    # case name: (fixture dir under tests/unit-tests, seismogram kinds (d, v, a,
+    # p = pressure), sources, dt, nstep, source file)
     "elastic_free_80x60": (
-        "displacement_tests/Newmark/serial/HomogeneousElasticDomain",
-        ["d", "v", "a"],
-        [dict(x=2500.0, z=2500.0, angle=0.0, f0=10.0, factor=1e10, tshift=0.0)],
-    ),
+        f"{NEWMARK}/HomogeneousElasticDomain", ["d", "v", "a"],
+        [dict(x=2500.0, z=2500.0, angle=0.0, f0=10.0, factor=1e10, tshift=0.0)], 1.1e-3, 300, "sources.yaml"),
     "elastic_stacey_80x60": (
-        "displacement_tests/Newmark/serial/HomogeneousElasticDomainStaceyDirichletBC",
-        ["d"],
-        [dict(x=x, z=z, angle=0.0, f0=10.0, factor=1e10, tshift=0.0)
-         for (x, z) in [(500.0, 500.0), (3500.0, 500.0), (500.0, 2500.0), (3500.0, 2500.0)]],
-    ),
+        f"{NEWMARK}/HomogeneousElasticDomainStaceyDirichletBC", ["d"], _RICKER4, 1.1e-3, 300, "sources.yaml"),
+    # acoustic medium: free surface on all four sides, a Dirac force source
+    "acoustic_free_80x60": (
+        f"{NEWMARK}/HomogeneousAcousticDomain", ["d", "v", "a", "p"],
+        [dict(x=2500.0, z=2500.0, angle=0.0, factor=1e10, tshift=0.0, stf="dirac")], 1.1e-3, 300,
+        "sources.yaml"),
+    "acoustic_dirichlet_80x60": (
+        f"{NEWMARK}/HomogeneousAcousticDomainDirichletBC", ["d"], _RICKER4, 1.1e-3, 300, "sources.yaml"),
+    # Stacey on bottom, left and right, free surface on top
+    "acoustic_composite_80x60": (
+        f"{NEWMARK}/HomogeneousAcousticDomainCompositeStaceyDirichletBC", ["d"], _RICKER4, 1.1e-3, 300,
+        "source.yaml"),
+    # acoustic medium with Stacey on all four sides (the reference's
+    # HomogeneousElasticDomainStaceyBC case holds an acoustic material, vs = 0)
+    "acoustic_stacey_80x60": (
+        f"{NEWMARK}/HomogeneousElasticDomainStaceyBC", ["d"], _RICKER4, 1.1e-3, 300, "sources.yaml"),
+    # elastic below, acoustic above one horizontal interface (9-node elements)
+    "coupled_acoustic_top": (
+        f"{NEWMARK}/AcousticElasticCoupledDomain", ["d"],
+        [dict(x=1575.0, z=2900.0, angle=0.0, f0=10.0, factor=1e9, tshift=0.0)], 0.85e-3, 600, "sources.yaml"),
+    # acoustic below, elastic above (4-node elements)
+    "coupled_acoustic_bottom": (
+        f"{NEWMARK}/AcousticElasticCoupledDomainVariant2", ["d"],
+        [dict(x=1575.0, z=2900.0, angle=0.0, f0=10.0, factor=1e9, tshift=0.0)], 0.85e-3, 600, "sources.yaml"),
 }
 
 
@@ -57,7 +78,7 @@ def gz_copy(src: str, dst: str) -> None:
 
 def main(ref: str) -> None:
     manifest = {}
-    for name, (rel, kinds, sources) in CASES.items():
+    for name, (rel, kinds, sources, dt, nstep, src_file) in CASES.items():
         src_dir = os.path.join(ref, UT, rel)
         out = os.path.join(HERE, name)
         os.makedirs(out, exist_ok=True)
@@ -66,26 +87,24 @@ def main(ref: str) -> None:
         traces = {}
         time = None
         for k in kinds:
-            arr = np.zeros((len(stations), 2, 0))
             rows = []
             for (sta, net, _x, _z) in stations:
                 comps = []
-                for comp in ("BXX", "BXZ"):
+                for comp in (("PRE",) if k == "p" else ("BXX", "BXZ")):
                     tr = read_trace(os.path.join(src_dir, "traces", f"{net}.{sta}.S2.{comp}.sem{k}"))
                     if time is None:
                         time = tr[:, 0]
                     comps.append(tr[:, 1])
                 rows.append(comps)
-            arr = np.asarray(rows)  # (nrec, 2, nsamples)
-            traces[k] = arr
+            traces[k] = np.asarray(rows)  # (nrec, 2, nsamples); pressure (nrec, 1, nsamples)
         np.savez_compressed(os.path.join(out, "traces.npz"), time=time, **traces)
         # the run inputs as the reference's test holds them (text, unchanged)
-        for fname in ("specfem_config.yaml", "sources.yaml", "STATIONS"):
+        for fname in ("specfem_config.yaml", src_file, "STATIONS"):
             shutil.copyfile(os.path.join(src_dir, fname), os.path.join(out, fname))
         with open(os.path.join(out, "case.json"), "w") as fh:
             json.dump({
                 "source_dir": f"{UT}/{rel}",
-                "dt": 1.1e-3, "nstep": 300, "nstep_between_samples": 1,
+                "dt": dt, "nstep": nstep, "nstep_between_samples": 1,
                 "stations": [dict(name=s, network=n, x=x, z=z) for (s, n, x, z) in stations],
                 "sources": sources, "trace_kinds": kinds, "tolerance": 1e-3,
             }, fh, indent=1)

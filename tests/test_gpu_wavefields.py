@@ -96,6 +96,7 @@ def test_3d_wavefield_snapshots_match_oracle(scatter):
         compare(f"3d_{scatter}_step{k}", ours, ref[0], ref[1])
 
 
+@pytest.mark.timeout(600)
 def test_3d_full_size_64cube_wavefield_matches_threaded_oracle():
     """The full 64^3 configuration (16,974,593 global points, 400 surface
     receivers), 300 steps: engine u, v, a over every point and the 400

@@ -135,10 +135,26 @@ def measured_traffic(workload: str, scatter: str, kernel: str):
 
 def cpu_threads() -> int:
     """Host threads for the CPU baseline: OMP_NUM_THREADS when set, else the
-    host's CPU count, at most 16."""
+    CPUs this process may run on, at most 16 (the host-thread share one GPU
+    is given on these machines)."""
     env = os.environ.get("OMP_NUM_THREADS", "")
-    n = int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+    n = int(env) if env.isdigit() and int(env) > 0 else len(os.sched_getaffinity(0))
     return max(1, min(16, n))
+
+
+def host_info() -> dict:
+    """nproc, the CPUs this process may use, and the CPU model."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "cpu_model": model,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS", "")}
 
 
 def cpu_baseline(w, nsteps: int) -> dict:
@@ -156,7 +172,7 @@ def cpu_baseline(w, nsteps: int) -> dict:
     el1 = time.perf_counter() - t
     one = o.nglob * nsteps / el1
     nt = cpu_threads()
-    out = {"value": one, "unit": "GLL pt-steps/s", "cores": 1, "kind": "port",
+    out = {"value": one, "unit": "GLL pt-steps/s", "cores": 1, "kind": "port", "host": host_info(),
            "sample": f"FP32 oracle (C++ restatement, 1 thread), {nsteps} steps of the same "
                      f"{w.mesh['nx']}^3 workload, {el1:.1f} s timed (+{build_s:.1f} s setup untimed)"}
     if nt > 1:
@@ -168,6 +184,7 @@ def cpu_baseline(w, nsteps: int) -> dict:
         o.forward(w.sources, w.receivers, 2 + extra, w.dt, nthreads=nt)
         tb = time.perf_counter() - t
         out.update(value=o.nglob * extra / (tb - ta), cores=nt, single_thread_value=one,
+                   cores_unused=max(0, (os.cpu_count() or nt) - nt),
                    sample=f"FP32 oracle (C++ restatement, OpenMP element loop, {nt} threads): "
                           f"{2 + extra} minus 2 steps of the same {w.mesh['nx']}^3 workload, "
                           f"{tb - ta:.1f} s; 1 thread: {nsteps} steps, {el1:.1f} s")

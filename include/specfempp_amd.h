@@ -35,9 +35,17 @@ typedef struct sfx_engine sfx_engine;
 #define SFX_EDGE_TOP 2
 #define SFX_EDGE_LEFT 3
 
-/* 2-D mesh: the content of specfem::mesh::mesh<dim2> for an elastic P-SV
- * isotropic domain (include/mesh/dim2/mesh.hpp; read by
- * src/io/mesh/impl/fortran/dim2/mesh.cpp:26-193). */
+/* Corner codes of an acoustic free-surface entry that names one control node
+ * (read_boundaries.cpp:218-232). */
+#define SFX_CORNER_BOTTOM_LEFT 4
+#define SFX_CORNER_BOTTOM_RIGHT 5
+#define SFX_CORNER_TOP_RIGHT 6
+#define SFX_CORNER_TOP_LEFT 7
+
+/* 2-D mesh: the content of specfem::mesh::mesh<dim2> for elastic P-SV and
+ * acoustic isotropic domains (include/mesh/dim2/mesh.hpp; read by
+ * src/io/mesh/impl/fortran/dim2/mesh.cpp:26-193).  A material with vs = 0 is
+ * acoustic (read_material_properties.cpp:172-238). */
 typedef struct {
   int nspec;            /* spectral elements */
   int ngnod;            /* control nodes per element: 4 or 9 */
@@ -50,6 +58,18 @@ typedef struct {
   int nabs;             /* absorbing (Stacey) edges */
   const int *abs_ispec; /* (nabs): mesh element of each edge */
   const int *abs_edge;  /* (nabs): SFX_EDGE_* */
+  /* acoustic free-surface (Dirichlet) entries, specfem::mesh::
+   * acoustic_free_surface (read_boundaries.cpp:206-268): element and
+   * SFX_EDGE_* or SFX_CORNER_* */
+  int nafs;
+  const int *afs_ispec;
+  const int *afs_type;
+  /* fluid-solid interface edges, specfem::mesh::coupled_interfaces
+   * (read_interfaces.cpp:9-30): the acoustic and the elastic mesh element of
+   * each shared edge */
+  int nfs;
+  const int *fs_acoustic;
+  const int *fs_elastic;
 } sfx_mesh2d;
 
 /* 3-D structured hexahedral mesh (NGLL = 5).  Elements are numbered x
@@ -71,10 +91,15 @@ typedef struct {
 #define SFX_SOURCE_FORCE 0
 #define SFX_SOURCE_MOMENT_TENSOR 1
 
-/* One source with a Ricker time function (src/source_time_function/ricker.cpp).
- * 2-D: position (x, z), force angle in degrees, moment tensor
- * (m[0]=Mxx, m[1]=Mzz, m[2]=Mxz).  3-D: position (x, y, z), force vector
- * (fx, fy, fz), moment tensor m = (Mxx, Myy, Mzz, Mxy, Mxz, Myz). */
+/* Source time functions (src/source_time_function/ricker.cpp, dirac.cpp).
+ * The Dirac is a narrow Gaussian of frequency 1 / (10 dt); f0 is unused. */
+#define SFX_STF_RICKER 0
+#define SFX_STF_DIRAC 1
+
+/* One source.  2-D: position (x, z), force angle in degrees, moment tensor
+ * (m[0]=Mxx, m[1]=Mzz, m[2]=Mxz); a force source in an acoustic element is a
+ * pressure source (force_source.cpp:51-56).  3-D: position (x, y, z), force
+ * vector (fx, fy, fz), moment tensor m = (Mxx, Myy, Mzz, Mxy, Mxz, Myz). */
 typedef struct {
   int kind;
   double x, y, z;
@@ -82,6 +107,7 @@ typedef struct {
   double fx, fy, fz;
   double m[6];
   double f0, tshift, factor;
+  int stf; /* SFX_STF_* */
 } sfx_source;
 
 typedef struct {
@@ -157,6 +183,14 @@ void sfx_engine_destroy(sfx_engine *engine);
 
 /* Sizes: dim, nspec, nglob, ngll, nsig (seismogram samples), nrec, nsrc. */
 int sfx_get_sizes(const sfx_engine *engine, int64_t *sizes7);
+/* Per-medium sizes of a 2-D mesh (specfem::compute::simulation_field's
+ * per-medium fields, field_impl.tpp:23-103): elastic elements, acoustic
+ * elements, points of the elastic field, points of the acoustic field.  A mesh
+ * without acoustic elements gives (nspec, 0, nglob, 0). */
+int sfx_get_media_sizes(const sfx_engine *engine, int64_t *sizes4);
+/* Acoustic field index of every global point, -1 where no acoustic element
+ * touches it (field_impl.tpp:39-62 for the acoustic medium).  2-D. */
+int sfx_get_field_index_acoustic(const sfx_engine *engine, int32_t *out);
 /* Simulation start time t0 (src/io/sources.cpp:128-158). */
 int sfx_get_t0(const sfx_engine *engine, double *t0);
 
@@ -244,6 +278,11 @@ int sfx_copy_wavefield(sfx_engine *engine, int dst_wavefield, int src_wavefield)
 int sfx_compute_derivatives(sfx_engine *engine, double dt);
 int sfx_get_wavefield(const sfx_engine *engine, int wavefield, int which, float *out);
 int sfx_set_wavefield(sfx_engine *engine, int wavefield, int which, const float *in);
+/* The forward field of one medium: SFX_MEDIUM_ELASTIC_PSV gives (2, nglob_el)
+ * as sfx_get_field; SFX_MEDIUM_ACOUSTIC gives the potential chi, chi_dot,
+ * chi_dot_dot or the mass inverse, (nglob_ac), on the acoustic field index. */
+int sfx_get_medium_field(const sfx_engine *engine, int medium, int which, float *out);
+int sfx_set_medium_field(sfx_engine *engine, int medium, int which, const float *in);
 
 /* ---- fused fast path --------------------------------------------------------
  * One Newmark step of time_marching<forward>::run (time_marching.tpp:33-109)

@@ -16,7 +16,7 @@ LIB_DIR = os.path.join(HERE, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libspecfempp_amd.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "specfempp_amd.h")
 
-SOURCES = ["engine.hip", "kernels.hip", "block3d.hip", "batch.hip", "assembly.cpp"]
+SOURCES = ["engine.hip", "kernels.hip", "acoustic.hip", "block3d.hip", "batch.hip", "assembly.cpp"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions stay off: some of their forms have given wrong
@@ -82,6 +82,10 @@ def load():
         "sfx_engine_create_host_3d": ([vp, i32, vp, i32, vp, vp, vp], i32),
         "sfx_engine_destroy": ([vp], None),
         "sfx_get_sizes": ([vp, vp], i32),
+        "sfx_get_media_sizes": ([vp, vp], i32),
+        "sfx_get_field_index_acoustic": ([vp, vp], i32),
+        "sfx_get_medium_field": ([vp, i32, i32, vp], i32),
+        "sfx_set_medium_field": ([vp, i32, i32, vp], i32),
         "sfx_get_t0": ([vp, vp], i32),
         "sfx_get_index_mapping": ([vp, vp], i32),
         "sfx_get_field_index": ([vp, vp], i32),

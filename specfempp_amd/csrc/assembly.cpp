@@ -59,24 +59,50 @@ std::vector<uint32_t> sort_by_xz(const std::vector<float> &x, const std::vector<
   return idx;
 }
 
+// Element boundary tags (src/mesh/dim2/tags/tags.cpp:5-47), combined as the
+// tag container's += does (include/enumerations/boundary.hpp:50-90): a Stacey
+// and an acoustic free-surface tag on one element make it composite.
+enum BoundaryTag { TAG_NONE = 0, TAG_FREE_SURFACE = 1, TAG_STACEY = 2, TAG_COMPOSITE = 3 };
+int add_tag(int cur, int t) {
+  if (cur == TAG_NONE) return t;
+  if (cur == t || cur == TAG_COMPOSITE) return cur;
+  return TAG_COMPOSITE;
+}
+
 // Mesh-order element types -> compute order
 // (src/compute/compute_mesh.cpp:240-279 with the ELEMENT_TYPES order of
-// include/enumerations/material_definitions.hpp:76-102: elastic P-SV
-// isotropic without boundary, then with Stacey).
-void compute_ordering(const sfx_mesh2d &m, HostAssembly &a, std::vector<int> &is_stacey_mesh) {
-  is_stacey_mesh.assign(m.nspec, 0);
+// include/enumerations/material_definitions.hpp:76-103: elastic P-SV
+// isotropic without boundary, with Stacey; acoustic isotropic without
+// boundary, with a free surface, with Stacey, composite), each group in mesh
+// order.  acoustic_mesh: 1 for the elements of acoustic materials.
+void compute_ordering(const sfx_mesh2d &m, const std::vector<int> &acoustic_mesh, HostAssembly &a,
+                      std::vector<int> &tag_mesh) {
+  tag_mesh.assign(m.nspec, TAG_NONE);
   for (int i = 0; i < m.nabs; i++) {
     if (m.abs_ispec[i] < 0 || m.abs_ispec[i] >= m.nspec) throw std::invalid_argument("abs_ispec out of range");
     if (m.abs_edge[i] < 0 || m.abs_edge[i] > 3) throw std::invalid_argument("abs_edge out of range");
-    is_stacey_mesh[m.abs_ispec[i]] = 1;
+    tag_mesh[m.abs_ispec[i]] = add_tag(tag_mesh[m.abs_ispec[i]], TAG_STACEY);
   }
+  for (int i = 0; i < m.nafs; i++) {
+    const int e = m.afs_ispec[i];
+    if (e < 0 || e >= m.nspec) throw std::invalid_argument("afs_ispec out of range");
+    if (m.afs_type[i] < 0 || m.afs_type[i] > 7) throw std::invalid_argument("afs_type out of range");
+    if (!acoustic_mesh[e]) throw std::invalid_argument("Acoustic free surface boundary is not an acoustic element");
+    tag_mesh[e] = add_tag(tag_mesh[e], TAG_FREE_SURFACE);
+  }
+  const int groups[6][2] = {{0, TAG_NONE}, {0, TAG_STACEY}, {1, TAG_NONE}, {1, TAG_FREE_SURFACE}, {1, TAG_STACEY},
+                            {1, TAG_COMPOSITE}};
   a.compute_to_mesh.clear();
   a.compute_to_mesh.reserve(m.nspec);
-  for (int pass = 0; pass < 2; pass++)
+  for (const auto &grp : groups)
     for (int e = 0; e < m.nspec; e++)
-      if (is_stacey_mesh[e] == pass) a.compute_to_mesh.push_back(e);
+      if (acoustic_mesh[e] == grp[0] && tag_mesh[e] == grp[1]) a.compute_to_mesh.push_back(e);
   a.nspec_none = 0;
-  for (int e = 0; e < m.nspec; e++) a.nspec_none += is_stacey_mesh[e] ? 0 : 1;
+  a.nspec_el = 0;
+  for (int e = 0; e < m.nspec; e++) {
+    a.nspec_el += acoustic_mesh[e] ? 0 : 1;
+    a.nspec_none += (!acoustic_mesh[e] && tag_mesh[e] == TAG_NONE) ? 1 : 0;
+  }
 }
 
 // Tolerance of the point merge, src/compute/compute_mesh.cpp:19-43: taken on
@@ -143,16 +169,24 @@ void number_points_2d(HostAssembly &a, const std::vector<float> &px, const std::
   if (inum != nglob) throw std::runtime_error("numbering: merged point count mismatch");
   a.nglob = nglob;
   // per-medium field index, include/compute/fields/impl/field_impl.tpp:39-62
-  // (loop order chunk, ix, iz, element)
-  a.field_index.assign(nglob, -1);
-  int count = 0;
-  for (int c0 = 0; c0 < nspec; c0 += chunk)
-    for (int ix = 0; ix < NGLL; ix++)
-      for (int iz = 0; iz < NGLL; iz++)
-        for (int ie = 0; ie < chunk && c0 + ie < nspec; ie++) {
-          const int g = a.index_mapping[static_cast<size_t>(c0 + ie) * npe + iz * NGLL + ix];
-          if (a.field_index[g] < 0) a.field_index[g] = count++;
-        }
+  // (loop order chunk, ix, iz, element; elements of the other medium are
+  // skipped).  Elastic elements are [0, nspec_el) in compute order.
+  auto number_medium = [&](bool acoustic, std::vector<int32_t> &index) {
+    index.assign(nglob, -1);
+    int count = 0;
+    for (int c0 = 0; c0 < nspec; c0 += chunk)
+      for (int ix = 0; ix < NGLL; ix++)
+        for (int iz = 0; iz < NGLL; iz++)
+          for (int ie = 0; ie < chunk && c0 + ie < nspec; ie++) {
+            if ((c0 + ie >= a.nspec_el) != acoustic) continue;
+            const int g = a.index_mapping[static_cast<size_t>(c0 + ie) * npe + iz * NGLL + ix];
+            if (index[g] < 0) index[g] = count++;
+          }
+    return count;
+  };
+  a.nglob_el = number_medium(false, a.field_index);
+  a.nglob_ac = a.nspec_el < nspec ? number_medium(true, a.field_index_ac) : 0;
+  if (a.nspec_el == nspec && a.nglob_el != nglob) throw std::runtime_error("medium numbering: count mismatch");
 }
 
 struct Locator2D {
@@ -395,9 +429,16 @@ void setup_time_function(HostAssembly &a, int nsrc, const sfx_source *src, const
   // (src/io/sources.cpp:128-158, ricker.cpp:15-20).
   float min_t0 = std::numeric_limits<float>::max(), min_tshift = std::numeric_limits<float>::max();
   std::vector<float> t0s(nsrc);
+  const float dtf = static_cast<float>(run.dt);
+  // a Dirac's frequency is 1 / (10 dt) (src/source_time_function/dirac.cpp:24-48)
+  auto f0_of = [&](int s) {
+    return src[s].stf == SFX_STF_DIRAC ? static_cast<float>(1.0 / (10.0 * dtf)) : static_cast<float>(src[s].f0);
+  };
   for (int s = 0; s < nsrc; s++) {
-    if (!(src[s].f0 > 0)) throw std::invalid_argument("source f0 must be positive");
-    const float hdur = 1.0f / static_cast<float>(src[s].f0);
+    if (src[s].stf != SFX_STF_RICKER && src[s].stf != SFX_STF_DIRAC)
+      throw std::invalid_argument("source time function must be SFX_STF_RICKER or SFX_STF_DIRAC");
+    if (src[s].stf == SFX_STF_RICKER && !(src[s].f0 > 0)) throw std::invalid_argument("source f0 must be positive");
+    const float hdur = 1.0f / f0_of(s);
     t0s[s] = -1.2 * hdur + static_cast<float>(src[s].tshift);
     min_t0 = std::min(min_t0, t0s[s]);
     min_tshift = std::min(min_tshift, static_cast<float>(src[s].tshift));
@@ -415,7 +456,9 @@ void setup_time_function(HostAssembly &a, int nsrc, const sfx_source *src, const
     const float shift = user_defined ? static_cast<float>(src[s].tshift) : t0s[s] - min_t0;
     for (int i = 0; i < run.nstep; i++)
       a.stf[static_cast<size_t>(i) * nsrc + s] =
-          ricker(t0 + i * dt - shift, static_cast<float>(src[s].f0), static_cast<float>(src[s].factor));
+          src[s].stf == SFX_STF_DIRAC
+              ? dirac(t0 + i * dt - shift, f0_of(s), static_cast<float>(src[s].factor))
+              : ricker(t0 + i * dt - shift, static_cast<float>(src[s].f0), static_cast<float>(src[s].factor));
   }
 }
 
@@ -445,9 +488,10 @@ void mass_2d(HostAssembly &a, double dt) {
   const int npe = 25;
   std::vector<int> bnd_of(static_cast<size_t>(a.nspec) * npe, -1);
   for (size_t q = 0; q < a.bnd_slot.size(); q++) bnd_of[a.bnd_slot[q]] = static_cast<int>(q);
-  std::vector<float> mass(2 * static_cast<size_t>(a.nglob), 0.0f);
+  const int ng = a.nglob_el;
+  std::vector<float> mass(2 * static_cast<size_t>(ng), 0.0f);
   const float vfac = -1.0f * static_cast<float>(dt) * 0.5f;
-  for (int e = 0; e < a.nspec; e++)
+  for (int e = 0; e < a.nspec_el; e++)
     for (int iz = 0; iz < NGLL; iz++)
       for (int ix = 0; ix < NGLL; ix++) {
         const size_t p = static_cast<size_t>(e) * npe + iz * NGLL + ix;
@@ -459,18 +503,36 @@ void mass_2d(HostAssembly &a, double dt) {
         if (k >= 0) stacey_term<2>(&a.bnd_normal[2 * k], a.bnd_weight[k], a.bnd_rho_vp[k], a.bnd_rho_vs[k], vfac, mm);
         const int gi = a.gather[p];
         mass[gi] += mm[0];
-        mass[a.nglob + gi] += mm[1];
+        mass[ng + gi] += mm[1];
       }
   a.mass_inverse.assign(2 * static_cast<size_t>(a.nglob_pad), 0.0f);
   for (int c = 0; c < 2; c++)
-    for (int g = 0; g < a.nglob; g++)
-      a.mass_inverse[static_cast<size_t>(c) * a.nglob_pad + g] = 1.0f / mass[static_cast<size_t>(c) * a.nglob + g];
+    for (int g = 0; g < ng; g++)
+      a.mass_inverse[static_cast<size_t>(c) * a.nglob_pad + g] = 1.0f / mass[static_cast<size_t>(c) * ng + g];
+  if (!a.coupled()) return;
+  // acoustic: 1 / kappa (include/medium/dim2/acoustic/isotropic/
+  // mass_matrix.tpp) times w_i w_j J, with the Stacey term of a constant
+  // velocity -dt/2 (stacey.tpp:172-213 as the mass matrix uses it)
+  std::vector<float> mass_ac(static_cast<size_t>(a.nglob_ac), 0.0f);
+  for (int e = a.nspec_el; e < a.nspec; e++)
+    for (int iz = 0; iz < NGLL; iz++)
+      for (int ix = 0; ix < NGLL; ix++) {
+        const size_t p = static_cast<size_t>(e) * npe + iz * NGLL + ix;
+        const float f = a.w[ix] * a.w[iz] * a.jac[p];
+        float m0 = 1.0f / a.kappa[p];
+        m0 *= f;
+        if (a.ac_stacey[p] != 0.0f) m0 += -1.0f * a.ac_stacey[p] * vfac;
+        mass_ac[a.gather[p]] += m0;
+      }
+  a.mass_inverse_ac.assign(a.nglob_ac_pad, 0.0f);
+  for (int g = 0; g < a.nglob_ac; g++) a.mass_inverse_ac[g] = 1.0f / mass_ac[g];
 }
 
 // New per-point properties (nspec, 25) in mesh element order: properties,
 // Stacey impedances (stacey.cpp:64-181) and mass matrix are rebuilt.
 void set_properties_2d(HostAssembly &a, const float *rho, const float *kappa, const float *mu, double dt) {
   if (a.dim != 2) throw std::invalid_argument("set_properties is implemented for 2-D meshes");
+  if (a.coupled()) throw std::invalid_argument("set_properties is implemented for elastic meshes");
   if (!rho || !kappa || !mu) throw std::invalid_argument("null property array");
   const int npe = 25;
   for (int e = 0; e < a.nspec; e++)
@@ -491,6 +553,123 @@ void set_properties_2d(HostAssembly &a, const float *rho, const float *kappa, co
   mass_2d(a, dt);
 }
 
+namespace {
+
+// Points of an edge in increasing order, (ix, iz)
+// (include/compute/coupled_interfaces/interface_container.tpp:15-48).
+void edge_point(int edge, int i, int &ix, int &iz) {
+  switch (edge) {
+    case SFX_EDGE_BOTTOM: ix = i; iz = 0; break;
+    case SFX_EDGE_TOP: ix = i; iz = NGLL - 1; break;
+    case SFX_EDGE_LEFT: ix = 0; iz = i; break;
+    default: ix = NGLL - 1; iz = i; break;  // right
+  }
+}
+// The coupling kernel's edge iterator (include/policies/edge.hpp, edge<dim2>):
+// the point of its own element and of the coupled element at edge point ip.
+void self_point(int edge, int ip, int &iz, int &ix) {
+  const int n = NGLL - 1;
+  switch (edge) {
+    case SFX_EDGE_BOTTOM: iz = 0; ix = ip; break;
+    case SFX_EDGE_TOP: iz = n; ix = n - ip; break;
+    case SFX_EDGE_LEFT: iz = ip; ix = 0; break;
+    default: iz = n - ip; ix = n; break;  // right
+  }
+}
+void coupled_point(int edge, int ip, int &iz, int &ix) {
+  const int n = NGLL - 1;
+  switch (edge) {
+    case SFX_EDGE_BOTTOM: iz = 0; ix = n - ip; break;
+    case SFX_EDGE_TOP: iz = n; ix = ip; break;
+    case SFX_EDGE_LEFT: iz = n - ip; ix = n; break;
+    default: iz = ip; ix = 0; break;  // right
+  }
+}
+// Edge normal scaled by the Jacobian (core/specfem/point/partial_derivatives.hpp:371-396).
+void edge_normal(const HostAssembly &a, size_t p, int edge, float &nx, float &nz) {
+  const float J = a.jac[p];
+  switch (edge) {
+    case SFX_EDGE_BOTTOM: nx = -1.0f * a.gammax[p] * J; nz = -1.0f * a.gammaz[p] * J; break;
+    case SFX_EDGE_TOP: nx = a.gammax[p] * J; nz = a.gammaz[p] * J; break;
+    case SFX_EDGE_LEFT: nx = -1.0f * a.xix[p] * J; nz = -1.0f * a.xiz[p] * J; break;
+    default: nx = a.xix[p] * J; nz = a.xiz[p] * J; break;  // right
+  }
+}
+
+}  // namespace
+
+// Elastic-acoustic interface (include/compute/coupled_interfaces/
+// interface_container.tpp:50-365): for every fluid-solid element pair the one
+// pair of edges whose points coincide (distance over the elastic element's
+// edge spacing <= 1e-10), the GLL weight along the edge as the integration
+// factor and the elastic element's edge normals; then the entries of the two
+// coupling terms in the order of the coupling kernel's edge iterator
+// (coupled_interface.tpp): the acoustic side adds factor (n . u) with n the
+// negated elastic normal, the elastic side factor n chi_dot_dot.
+static void build_coupling_2d(HostAssembly &a, const sfx_mesh2d &m, const std::vector<int> &mesh_to_compute,
+                              const std::vector<float> &px, const std::vector<float> &pz) {
+  const int edges[4] = {SFX_EDGE_BOTTOM, SFX_EDGE_TOP, SFX_EDGE_LEFT, SFX_EDGE_RIGHT};
+  auto pidx = [](int e, int iz, int ix) { return static_cast<size_t>(e) * 25 + iz * NGLL + ix; };
+  for (int i = 0; i < m.nfs; i++) {
+    if (m.fs_acoustic[i] < 0 || m.fs_acoustic[i] >= m.nspec || m.fs_elastic[i] < 0 || m.fs_elastic[i] >= m.nspec)
+      throw std::invalid_argument("fluid-solid element out of range");
+    const int e_el = mesh_to_compute[m.fs_elastic[i]], e_ac = mesh_to_compute[m.fs_acoustic[i]];
+    if (!(e_el < a.nspec_el && e_ac >= a.nspec_el))
+      throw std::invalid_argument("Coupled Interfaces: Interface is not between the correct mediums");
+    int edge_el = -1, edge_ac = -1, found = 0;
+    for (int ed1 : edges)
+      for (int ed2 : edges) {
+        if (ed1 == ed2) continue;
+        const float edge_size = (ed1 == SFX_EDGE_BOTTOM || ed1 == SFX_EDGE_TOP)
+                                    ? px[pidx(e_el, 0, 1)] - px[pidx(e_el, 0, 0)]
+                                    : pz[pidx(e_el, 1, 0)] - pz[pidx(e_el, 0, 0)];
+        bool connected = true;
+        for (int ip = 0; ip < NGLL && connected; ip++) {
+          int ix1, iz1, ix2, iz2;
+          edge_point(ed1, ip, ix1, iz1);
+          edge_point(ed2, ip, ix2, iz2);
+          const size_t p1 = pidx(e_el, iz1, ix1), p2 = pidx(e_ac, iz2, ix2);
+          const float dx = px[p1] - px[p2], dz = pz[p1] - pz[p2];
+          if (std::sqrt(dx * dx + dz * dz) / edge_size > 1.e-10f) connected = false;
+        }
+        if (connected) {
+          edge_el = ed1;
+          edge_ac = ed2;
+          found++;
+        }
+      }
+    if (found != 1) throw std::invalid_argument("Number of connected edges is not 1");
+    float factor[NGLL], nx[NGLL], nz[NGLL];
+    for (int ip = 0; ip < NGLL; ip++) {
+      int ix1, iz1, ix2, iz2;
+      edge_point(edge_el, ip, ix1, iz1);
+      edge_point(edge_ac, ip, ix2, iz2);
+      float n1x, n1z, n2x, n2z;
+      edge_normal(a, pidx(e_el, iz1, ix1), edge_el, n1x, n1z);
+      edge_normal(a, pidx(e_ac, iz2, ix2), edge_ac, n2x, n2z);
+      if ((std::abs(n1x + n2x) > 1.e-4f * (std::abs(n1x - n2x) / 2)) &&
+          (std::abs(n1z + n2z) > 1.e-4f * (std::abs(n1z - n2z) / 2)))
+        throw std::invalid_argument("Edge normals need to be opposite in direction");
+      factor[ip] = (edge_el == SFX_EDGE_BOTTOM || edge_el == SFX_EDGE_TOP) ? a.w[ix1] : a.w[iz1];
+      nx[ip] = n1x;
+      nz[ip] = n1z;
+    }
+    for (int ip = 0; ip < NGLL; ip++) {
+      int sz, sx, cz, cx;
+      self_point(edge_ac, ip, sz, sx);
+      coupled_point(edge_el, ip, cz, cx);
+      a.cpl_ac_self.push_back(a.gather[pidx(e_ac, sz, sx)]);
+      a.cpl_ac_other.push_back(a.gather[pidx(e_el, cz, cx)]);
+      a.cpl_ac_coef.insert(a.cpl_ac_coef.end(), {factor[ip], -nx[ip], -nz[ip]});
+      self_point(edge_el, ip, sz, sx);
+      coupled_point(edge_ac, ip, cz, cx);
+      a.cpl_el_self.push_back(a.gather[pidx(e_el, sz, sx)]);
+      a.cpl_el_other.push_back(a.gather[pidx(e_ac, cz, cx)]);
+      a.cpl_el_coef.insert(a.cpl_el_coef.end(), {factor[ip], nx[ip], nz[ip]});
+    }
+  }
+}
+
 HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *src, int nrec,
                                const sfx_receiver *rec, const sfx_run_desc &run) {
   check_run(run);
@@ -501,9 +680,20 @@ HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *
     throw std::invalid_argument("2-D mesh too large: nspec x 25 points exceed 32-bit device indexing");
   if (m.ngnod != 4 && m.ngnod != 9) throw std::invalid_argument("ngnod must be 4 or 9");
   if (!m.coorg || !m.knods || !m.material || !m.rho || !m.vp || !m.vs) throw std::invalid_argument("null mesh array");
+  // vs > 0: elastic P-SV isotropic; vs = 0: acoustic isotropic
+  // (read_material_properties.cpp:172-238)
   for (int i = 0; i < m.nmaterials; i++)
-    if (!(m.vs[i] > 0) || !(m.rho[i] > 0) || !(m.vp[i] > 0))
-      throw std::invalid_argument("only elastic isotropic materials (vs > 0) are supported");
+    if (!(m.vs[i] >= 0) || !(m.rho[i] > 0) || !(m.vp[i] > 0))
+      throw std::invalid_argument("materials need rho > 0, vp > 0 and vs >= 0 (vs = 0: acoustic)");
+  if ((m.nafs && (!m.afs_ispec || !m.afs_type)) || (m.nfs && (!m.fs_acoustic || !m.fs_elastic)) || m.nafs < 0 ||
+      m.nfs < 0)
+    throw std::invalid_argument("null acoustic free-surface or interface array");
+  std::vector<int> acoustic_mesh(m.nspec, 0);
+  for (int e = 0; e < m.nspec; e++) {
+    const int mat = m.material[e];
+    if (mat < 0 || mat >= m.nmaterials) throw std::invalid_argument("material index out of range");
+    acoustic_mesh[e] = m.vs[mat] == 0.0 ? 1 : 0;
+  }
   HostAssembly a;
   a.dim = 2;
   a.ncomp = 2;
@@ -515,9 +705,12 @@ HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *
   for (int i = 0; i < 5; i++)
     for (int j = 0; j < 5; j++) a.hprime[i][j] = g.hprime[i][j];
 
-  std::vector<int> stacey_mesh;
-  compute_ordering(m, a, stacey_mesh);
+  std::vector<int> tag_mesh;
+  compute_ordering(m, acoustic_mesh, a, tag_mesh);
   const int nspec = m.nspec, ngnod = m.ngnod, npe = 25;
+  const bool coupled = a.nspec_el < nspec;
+  if (coupled && (run.scatter == SFX_SCATTER_GATHER))
+    throw std::invalid_argument("meshes with acoustic elements need SFX_SCATTER_COLORED or _ATOMIC");
 
   // control nodes in compute order, as type_real (compute_mesh.cpp:172-197)
   std::vector<float> ctrl(static_cast<size_t>(nspec) * ngnod * 2);
@@ -554,9 +747,13 @@ HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *
   // a field component is addressed as c * nglob_pad + point (c < 2)
   if (2 * ((static_cast<int64_t>(a.nglob) + 63) / 64 * 64) > (int64_t(1) << 31) - 1)
     throw std::invalid_argument("2-D mesh too large: 2 x nglob exceeds 32-bit device indexing");
-  a.nglob_pad = (a.nglob + 63) / 64 * 64;
+  a.nglob_pad = (std::max(a.nglob_el, 1) + 63) / 64 * 64;
+  a.nglob_ac_pad = coupled ? (std::max(a.nglob_ac, 1) + 63) / 64 * 64 : 0;
+  // field index of every local point in its element's medium field
   a.gather.resize(nloc);
-  for (size_t p = 0; p < nloc; p++) a.gather[p] = a.field_index[a.index_mapping[p]];
+  for (size_t p = 0; p < nloc; p++)
+    a.gather[p] = static_cast<int>(p / npe) < a.nspec_el ? a.field_index[a.index_mapping[p]]
+                                                         : a.field_index_ac[a.index_mapping[p]];
 
   // Geometry (compute_partial_derivatives.cpp:46-91; jacobian.cpp:232-257)
   a.xix.resize(nloc);
@@ -613,21 +810,25 @@ HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *
     }
   }
 
-  // Properties (include/medium/dim2/elastic/isotropic/material.hpp:62-77)
+  // Properties (include/medium/dim2/elastic/isotropic/material.hpp:62-77;
+  // acoustic: kappa = rho cp^2, rho_inverse = 1 / rho,
+  // include/medium/dim2/acoustic/isotropic/material.hpp:44-101)
   a.kappa.resize(nloc);
   a.mu.resize(nloc);
   a.rho.resize(nloc);
+  if (coupled) a.rho_inv.assign(nloc, 0.0f);
   for (int e = 0; e < nspec; e++) {
     const int mat = m.material[a.compute_to_mesh[e]];
-    if (mat < 0 || mat >= m.nmaterials) throw std::invalid_argument("material index out of range");
     const float rho = static_cast<float>(m.rho[mat]);
     const float cs = static_cast<float>(m.vs[mat]), cp = static_cast<float>(m.vp[mat]);
-    const float mu = rho * cs * cs;
-    const float kappa = rho * (cp * cp - (4.0 / 3.0) * cs * cs);
+    const bool ac = e >= a.nspec_el;
+    const float mu = ac ? 0.0f : rho * cs * cs;
+    const float kappa = ac ? rho * cp * cp : static_cast<float>(rho * (cp * cp - (4.0 / 3.0) * cs * cs));
     for (int p = 0; p < npe; p++) {
       a.kappa[static_cast<size_t>(e) * npe + p] = kappa;
       a.mu[static_cast<size_t>(e) * npe + p] = mu;
       a.rho[static_cast<size_t>(e) * npe + p] = rho;
+      if (ac) a.rho_inv[static_cast<size_t>(e) * npe + p] = 1.0f / rho;
     }
   }
 
@@ -640,8 +841,49 @@ HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *
   std::map<int, std::vector<int>> edges_of;
   for (int i = 0; i < m.nabs; i++) edges_of[mesh_to_compute[m.abs_ispec[i]]].push_back(i);
   std::vector<int> bnd_of(nloc, -1);  // element point -> Stacey entry
+  if (coupled) a.ac_stacey.assign(nloc, 0.0f);
   for (const auto &kv : edges_of) {
     const int e = kv.first;
+    if (e >= a.nspec_el) {
+      // acoustic Stacey (stacey.tpp:172-213): w |n| rho_vpinverse per point,
+      // rho_vpinverse = sqrt(rho_inverse / kappa) (core/specfem/point/
+      // properties.hpp:68-76); normal and weight of the last listed edge
+      float nxs[25], nzs[25], ws[25];
+      bool on_any[25] = {};
+      for (int i : kv.second) {
+        const int edge = m.abs_edge[i];
+        for (int iz = 0; iz < NGLL; iz++)
+          for (int ix = 0; ix < NGLL; ix++) {
+            const bool on = (edge == SFX_EDGE_TOP && iz == NGLL - 1) || (edge == SFX_EDGE_BOTTOM && iz == 0) ||
+                            (edge == SFX_EDGE_LEFT && ix == 0) || (edge == SFX_EDGE_RIGHT && ix == NGLL - 1);
+            if (!on) continue;
+            const int lp = iz * NGLL + ix;
+            const size_t p = static_cast<size_t>(e) * npe + lp;
+            const float J = a.jac[p];
+            if (edge == SFX_EDGE_LEFT || edge == SFX_EDGE_RIGHT) {
+              const float sg = edge == SFX_EDGE_LEFT ? -1.0f : 1.0f;
+              nxs[lp] = sg * a.xix[p] * J;
+              nzs[lp] = sg * a.xiz[p] * J;
+              ws[lp] = a.w[iz];
+            } else {
+              const float sg = edge == SFX_EDGE_BOTTOM ? -1.0f : 1.0f;
+              nxs[lp] = sg * a.gammax[p] * J;
+              nzs[lp] = sg * a.gammaz[p] * J;
+              ws[lp] = a.w[ix];
+            }
+            on_any[lp] = true;
+          }
+      }
+      for (int lp = 0; lp < 25; lp++) {
+        if (!on_any[lp]) continue;
+        const size_t p = static_cast<size_t>(e) * npe + lp;
+        const float factor = ws[lp] * std::sqrt(nxs[lp] * nxs[lp] + nzs[lp] * nzs[lp]);
+        const float kappa_inverse = 1.0f / a.kappa[p];
+        const float rho_vpinverse = std::sqrt(a.rho_inv[p] * kappa_inverse);
+        a.ac_stacey[p] = factor * rho_vpinverse;
+      }
+      continue;
+    }
     int slot[25];
     std::fill(slot, slot + 25, -1);
     for (int i : kv.second) {
@@ -694,6 +936,27 @@ HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *
     }
   }
 
+  if (coupled) {
+    // acoustic free-surface points (src/compute/boundaries/impl/
+    // acoustic_free_surface.cpp:10-103): the edge's points, or the corner
+    a.ac_dirichlet.assign(nloc, 0);
+    for (int i = 0; i < m.nafs; i++) {
+      const int e = mesh_to_compute[m.afs_ispec[i]], type = m.afs_type[i], n = NGLL - 1;
+      for (int iz = 0; iz < NGLL; iz++)
+        for (int ix = 0; ix < NGLL; ix++) {
+          const bool on = (type == SFX_EDGE_TOP && iz == n) || (type == SFX_EDGE_BOTTOM && iz == 0) ||
+                          (type == SFX_EDGE_LEFT && ix == 0) || (type == SFX_EDGE_RIGHT && ix == n) ||
+                          (type == SFX_CORNER_BOTTOM_RIGHT && iz == 0 && ix == n) ||
+                          (type == SFX_CORNER_BOTTOM_LEFT && iz == 0 && ix == 0) ||
+                          (type == SFX_CORNER_TOP_RIGHT && iz == n && ix == n) ||
+                          (type == SFX_CORNER_TOP_LEFT && iz == n && ix == 0);
+          if (on) a.ac_dirichlet[static_cast<size_t>(e) * npe + iz * NGLL + ix] = 1;
+        }
+    }
+    build_coupling_2d(a, m, mesh_to_compute, px, pz);
+  } else if (m.nafs || m.nfs) {
+    throw std::invalid_argument("free-surface or fluid-solid entries on a mesh without acoustic elements");
+  }
   mass_2d(a, run.dt);
 
   // Sources and receivers (locate_point; force_source.cpp:70-78;
@@ -716,7 +979,15 @@ HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *
     lagrange<float>(xi, a.xi, hx, hpx);
     lagrange<float>(gamma, a.xi, hg, hpg);
     float *arr = &a.src_array[static_cast<size_t>(s) * 2 * npe];
-    if (src[s].kind == SFX_SOURCE_FORCE) {
+    if (e >= a.nspec_el) {
+      // in an acoustic element a force source is the Lagrange product alone
+      // (force_source.cpp:51-56; the acoustic source term divides by kappa,
+      // include/medium/dim2/acoustic/isotropic/source.hpp)
+      if (src[s].kind != SFX_SOURCE_FORCE)
+        throw std::invalid_argument("moment-tensor source in an acoustic element");
+      for (int iz = 0; iz < NGLL; iz++)
+        for (int ix = 0; ix < NGLL; ix++) arr[iz * NGLL + ix] = hx[ix] * hg[iz];
+    } else if (src[s].kind == SFX_SOURCE_FORCE) {
       const float ang = static_cast<float>(src[s].angle);
       for (int iz = 0; iz < NGLL; iz++)
         for (int ix = 0; ix < NGLL; ix++) {
@@ -749,10 +1020,23 @@ HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *
     }
   }
   // application order: element type (no boundary, then Stacey), then source
-  // index (src/compute/compute_sources.cpp:232-270)
+  // index (src/compute/compute_sources.cpp:232-270); on a mesh with acoustic
+  // elements each medium's update applies its own sources, ordered by the
+  // boundary tag of their element (none, free surface, Stacey, composite)
   for (int pass = 0; pass < 2; pass++)
     for (int s = 0; s < nsrc; s++)
       if ((a.src_ispec[s] >= a.nspec_none) == (pass == 1)) a.src_order.push_back(s);
+  a.src_stf_kind.resize(nsrc);
+  for (int s = 0; s < nsrc; s++) a.src_stf_kind[s] = src[s].stf;
+  if (coupled) {
+    std::vector<int> tag_c(nspec);
+    for (int e = 0; e < nspec; e++) tag_c[e] = tag_mesh[a.compute_to_mesh[e]];
+    for (int pass = 0; pass < 4; pass++)
+      for (int s = 0; s < nsrc; s++)
+        if (tag_c[a.src_ispec[s]] == pass) (a.src_ispec[s] < a.nspec_el ? a.src_order_el : a.src_order_ac).push_back(s);
+  } else {
+    a.src_order_el = a.src_order;
+  }
 
   a.nrec = nrec;
   a.rec_ispec.resize(nrec);
@@ -770,6 +1054,7 @@ HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *
     lagrange<float>(gamma, a.xi, hg, nullptr);
     for (int iz = 0; iz < NGLL; iz++)
       for (int ix = 0; ix < NGLL; ix++) a.rec_h[static_cast<size_t>(r) * npe + iz * NGLL + ix] = hx[ix] * hg[iz];
+    (e < a.nspec_el ? a.rec_el : a.rec_ac).push_back(r);
   }
   // kept for locating adjoint sources after the build
   a.ngnod2d = ngnod;
@@ -786,6 +1071,7 @@ HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *
 // kernel's one time function per source applies.
 void place_adjoint_sources_2d(HostAssembly &a, int nadj, const sfx_receiver *pos, const float *stf, int nstep) {
   if (a.dim != 2) throw std::invalid_argument("adjoint sources are implemented for 2-D meshes");
+  if (a.coupled()) throw std::invalid_argument("adjoint runs are implemented for elastic meshes");
   if (nadj < 0 || (nadj && (!pos || !stf))) throw std::invalid_argument("null adjoint source array");
   const int npe = 25;
   Locator2D loc{a, a.ctrl2d, a.px2d, a.pz2d, a.ngnod2d};
@@ -915,6 +1201,8 @@ HostAssembly build_assembly_3d(const sfx_mesh3d &m, int nsrc, const sfx_source *
             }
       }
   a.nglob = inum;
+  a.nglob_el = inum;
+  a.nspec_el = a.nspec;
   if (static_cast<int64_t>(inum) != nlat) throw std::runtime_error("3-D numbering: lattice not fully covered");
   a.field_index.resize(a.nglob);
   for (int i = 0; i < a.nglob; i++) a.field_index[i] = i;
@@ -1181,21 +1469,38 @@ void color_elements(HostAssembly &a, int scatter) {
       std::fill(color.begin(), color.end(), -1);
       ncolors = 0;
       std::vector<int> stamp(64, -1);
+      // elastic and acoustic elements update different fields: each medium is
+      // colored on its own, the acoustic colors numbered after the elastic ones
+      int base = 0;
       for (int e = 0; e < nspec; e++) {
+        if (e == a.nspec_el) base = ncolors;
+        const bool ac = e >= a.nspec_el;
         for (int p = 0; p < npe; p++) {
           const int g = a.index_mapping[static_cast<size_t>(e) * npe + p];
           for (int k = start[g]; k < start[g + 1]; k++) {
             const int c = color[elems[k]];
-            if (c >= 0) stamp[c] = e;
+            if (c >= 0 && (elems[k] >= a.nspec_el) == ac) stamp[c - base] = e;
           }
         }
         int c = 0;
         while (c < 64 && stamp[c] == e) c++;
         if (c == 64) throw std::runtime_error("coloring needs more than 64 colors");
-        color[e] = c;
-        ncolors = std::max(ncolors, c + 1);
+        color[e] = base + c;
+        ncolors = std::max(ncolors, base + c + 1);
       }
+      if (a.coupled() && base == 0 && a.nspec_el > 0) throw std::logic_error("internal: acoustic color base");
     }
+  }
+  a.ncolors_el = ncolors;
+  if (a.coupled()) {
+    if (scatter != SFX_SCATTER_COLORED) {  // one atomic launch per medium
+      for (int e = 0; e < nspec; e++) color[e] = e >= a.nspec_el ? 1 : 0;
+      ncolors = 2;
+    }
+    a.ncolors_el = 0;
+    for (int e = 0; e < a.nspec_el; e++) a.ncolors_el = std::max(a.ncolors_el, color[e] + 1);
+    for (int e = a.nspec_el; e < nspec; e++)
+      if (color[e] < a.ncolors_el) throw std::logic_error("internal: acoustic element in an elastic color");
   }
   // stable counting sort by color
   a.color_offsets.assign(ncolors + 1, 0);

@@ -83,6 +83,29 @@ struct HostAssembly {
   std::vector<int32_t> adj_ispec, adj_order;
   std::vector<float> adj_array;          // (2 nadj, ncomp, npe)
   std::vector<float> adj_stf;            // (nstep, 2 nadj)
+  // 2-D acoustic medium and elastic-acoustic coupling (meshes with vs = 0
+  // materials).  Compute order puts the elastic elements first: [0, nspec_el)
+  // elastic, [nspec_el, nspec) acoustic.  The elastic field has nglob_el
+  // points (nglob_pad is its padded size), the acoustic field nglob_ac.
+  int nspec_el = 0, nglob_el = 0;
+  int nglob_ac = 0, nglob_ac_pad = 0;
+  std::vector<int32_t> field_index_ac;   // iglob -> acoustic field index, or -1
+  // per local point of every compute element (0 at elastic elements):
+  std::vector<float> rho_inv;            // 1 / rho (acoustic properties, material.hpp:44-101)
+  std::vector<float> ac_stacey;          // w |n| rho_vpinverse on acoustic Stacey edges, else 0
+  std::vector<int32_t> ac_dirichlet;     // 1 on an acoustic free surface
+  std::vector<float> mass_inverse_ac;    // (nglob_ac_pad), padding 0
+  // Coupling, one entry per interface edge point.  cpl_ac: the acoustic
+  // side's term factor (nx u_x + nz u_z) with n the acoustic normal, into the
+  // acoustic field point `self` from the elastic field point `other`; cpl_el:
+  // the elastic side's factor n chi_dot_dot into elastic point `self`.
+  std::vector<int32_t> cpl_ac_self, cpl_ac_other, cpl_el_self, cpl_el_other;
+  std::vector<float> cpl_ac_coef, cpl_el_coef;  // (n, 3): factor, nx, nz
+  // sources and receivers by medium, in application order
+  std::vector<int32_t> src_order_el, src_order_ac, rec_el, rec_ac;
+  std::vector<int32_t> src_stf_kind;     // per source: SFX_STF_*
+  int ncolors_el = 0;                    // color_offsets[0..ncolors_el] are the elastic colors
+  bool coupled() const { return dim == 2 && nspec_el < nspec; }
   // Element colors for the scatter: device element slot s holds compute
   // element perm[s]; slots [color_offsets[c], color_offsets[c+1]) form color c,
   // whose elements share no global point.  One color when scatter is atomic.

@@ -190,6 +190,7 @@ struct sfx_batch {
     for (auto *e : engines) {
       if (!e) throw std::invalid_argument("null engine");
       if (engine_host(e).dim != 2) throw std::invalid_argument("batches are implemented for 2-D engines");
+      if (engine_host(e).coupled()) throw std::invalid_argument("batches are implemented for elastic meshes");
     }
     const sfx_run_desc &r0 = engine_run(engines[0]);
     ntypes = r0.nseismo_types;

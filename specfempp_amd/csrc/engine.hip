@@ -125,6 +125,13 @@ struct sfx_engine {
   int nbseg = 0;         // Stacey segments (3-D)
   bool sources_off_boundary = false;  // 3-D gather: no source element has an absorbing face
   DevBuf<float> minv1;                // 3-D gather: M^-1 of one component (nglob_pad)
+  // 2-D acoustic medium: the acoustic field (chi, chi_dot, chi_dot_dot, mass
+  // inverse; nglob_ac_pad each), per-point tables in slot order, the
+  // interface entries, and the sources and receivers of each medium
+  DevBuf<float> ac_fields, rho_inv, ac_stacey;
+  DevBuf<int> ac_dirichlet, cpl_ac_self, cpl_ac_other, cpl_el_self, cpl_el_other, src_order_ac, rec_el, rec_ac;
+  DevBuf<float> cpl_ac_coef, cpl_el_coef;
+  int nsrc_order_el = 0;
   Lattice lat{};
   std::vector<hipEvent_t> ev;
   double timings[4] = {0, 0, 0, 0};
@@ -132,6 +139,11 @@ struct sfx_engine {
   Fields f() const {
     const size_t stride = static_cast<size_t>(h.ncomp) * h.nglob_pad;
     return Fields{fields.p, fields.p + stride, fields.p + 2 * stride, fields.p + 3 * stride, h.nglob_pad};
+  }
+  // the acoustic field of a mesh with acoustic elements
+  Fields fa() const {
+    const size_t n = h.nglob_ac_pad;
+    return Fields{ac_fields.p, ac_fields.p + n, ac_fields.p + 2 * n, ac_fields.p + 3 * n, h.nglob_ac_pad};
   }
 
   ~sfx_engine() {
@@ -167,7 +179,7 @@ struct sfx_engine {
     d_step.ensure(1);
     SFX_HIP(hipMemcpyAsync(d_step.p, &v, sizeof(int), hipMemcpyHostToDevice, stream));
   }
-  bool graph_2d() const { return use_graphs && h.dim == 2; }
+  bool graph_2d() const { return use_graphs && h.dim == 2 && !h.coupled(); }
 
   // A regular 2-D forward step with every step-dependent value read from the
   // device counter; recording adds the Stacey-element record.
@@ -397,7 +409,31 @@ struct sfx_engine {
     bnd_rvp.upload(h.bnd_rho_vp, s);
     bnd_rvs.upload(h.bnd_rho_vs, s);
     if (block_path()) upload_block_boundary(s);
-    src_order.upload(h.src_order, s);
+    if (h.coupled()) {
+      src_order.upload(h.src_order_el, s);
+      nsrc_order_el = static_cast<int>(h.src_order_el.size());
+      src_order_ac.upload(h.src_order_ac, s);
+      rec_el.upload(h.rec_el, s);
+      rec_ac.upload(h.rec_ac, s);
+      rho_inv.upload(permuted(h.rho_inv), s);
+      ac_stacey.upload(permuted(h.ac_stacey), s);
+      ac_dirichlet.upload(permuted(h.ac_dirichlet), s);
+      cpl_ac_self.upload(h.cpl_ac_self, s);
+      cpl_ac_other.upload(h.cpl_ac_other, s);
+      cpl_ac_coef.upload(h.cpl_ac_coef, s);
+      cpl_el_self.upload(h.cpl_el_self, s);
+      cpl_el_other.upload(h.cpl_el_other, s);
+      cpl_el_coef.upload(h.cpl_el_coef, s);
+      const size_t na = h.nglob_ac_pad;
+      if (h.mass_inverse_ac.size() != na) throw std::logic_error("internal: acoustic mass inverse layout");
+      ac_fields.alloc(4 * na);
+      launch_fill(s, ac_fields.p, 3 * na, 0.0f);
+      SFX_HIP(hipMemcpyAsync(ac_fields.p + 3 * na, h.mass_inverse_ac.data(), na * sizeof(float),
+                             hipMemcpyHostToDevice, s));
+    } else {
+      src_order.upload(h.src_order, s);
+      nsrc_order_el = h.nsrc;
+    }
     src_ispec.upload(slots_of(h.src_ispec), s);
     src_array.upload(h.src_array, s);
     stf.upload(h.stf, s);
@@ -477,7 +513,7 @@ struct sfx_engine {
   // One launch per color (plain read-modify-write), or one atomic launch.
   void stiffness(Fields ff) {
     const bool atomic = run.scatter == SFX_SCATTER_ATOMIC;
-    for (size_t c = 0; c + 1 < h.color_offsets.size(); c++) {
+    for (size_t c = 0; c < static_cast<size_t>(h.ncolors_el); c++) {
       const int first = h.color_offsets[c], count = h.color_offsets[c + 1] - first;
       if (h.dim == 2)
         launch_stiffness_2d(stream, first, count, atomic, gather.p, xix.p, xiz.p, gammax.p, gammaz.p, jac.p, kappa.p,
@@ -491,16 +527,54 @@ struct sfx_engine {
     launch_stacey(stream, h.dim, static_cast<int>(h.bnd_gather.size()), bnd_gather.p, bnd_normal.p, bnd_weight.p,
                   bnd_rvp.p, bnd_rvs.p, ff, bnd_rec.p, rec_mode == 1 ? rec_step : nullptr);
   }
+  // the sources of the elastic medium (all sources without acoustic elements)
   void sources(Fields ff, int istep) {
     if (istep < 0 || istep >= run.nstep) throw std::invalid_argument("istep out of range");
-    launch_sources(stream, h.dim, h.npe, h.nsrc, istep, src_order.p, src_ispec.p, gather.p, src_array.p, stf.p, ff);
+    launch_sources(stream, h.dim, h.npe, h.nsrc, istep, src_order.p, src_ispec.p, gather.p, src_array.p, stf.p, ff,
+                   nullptr, nsrc_order_el);
   }
   void seismograms(Fields ff, int isig, int corrected) { seismograms(ff, isig, corrected, nc.dto2); }
+  // Seismograms of the elastic receivers; with acoustic elements also those
+  // of the acoustic receivers, from the acoustic field's current state.
   void seismograms(Fields ff, int isig, int corrected, float dto2) {
     if (isig < 0 || isig >= nsig) throw std::invalid_argument("seismogram step out of range");
+    const bool cpl = h.coupled();
     launch_seismograms(stream, h.dim, h.npe, h.nrec, run.nseismo_types, types.p, isig, rec_ispec.p, gather.p, rec_h.p,
                        ff, dto2, corrected, seismo.p, PointGeom2D{xix.p, xiz.p, gammax.p, gammaz.p, kappa.p, mu.p},
-                       q);
+                       q, nullptr, 1, -1, ForceSum2D{}, cpl ? rec_el.p : nullptr,
+                       cpl ? static_cast<int>(h.rec_el.size()) : -1);
+    if (cpl)
+      launch_acoustic_seismograms(stream, static_cast<int>(h.rec_ac.size()), rec_ac.p, h.nrec, run.nseismo_types,
+                                  types.p, isig, rec_ispec.p, gather.p, rec_h.p, xix.p, xiz.p, gammax.p, gammaz.p,
+                                  rho_inv.p, q, fa(), seismo.p);
+  }
+  // ---- acoustic medium and coupling (meshes with acoustic elements) ----
+  // acoustic colors: color_offsets[ncolors_el ..]
+  void acoustic_stiffness(Fields fac) {
+    const bool atomic = run.scatter == SFX_SCATTER_ATOMIC;
+    for (size_t c = h.ncolors_el; c + 1 < h.color_offsets.size(); c++) {
+      const int first = h.color_offsets[c], count = h.color_offsets[c + 1] - first;
+      launch_acoustic_stiffness_2d(stream, first, count, atomic, gather.p, xix.p, xiz.p, gammax.p, gammaz.p, jac.p,
+                                   rho_inv.p, ac_stacey.p, ac_dirichlet.p, q, fac);
+    }
+  }
+  // update_wavefields<acoustic> (domain_kernels.hpp:69-126): coupling from the
+  // elastic displacement, sources, stiffness with Stacey and free surface;
+  // then the mass division
+  void update_acoustic(int istep, int correct = 0) {
+    if (istep < 0 || istep >= run.nstep) throw std::invalid_argument("istep out of range");
+    Fields fac = fa();
+    launch_coupling_2d(stream, static_cast<int>(h.cpl_ac_self.size()), cpl_ac_self.p, cpl_ac_other.p, cpl_ac_coef.p,
+                       fac, f(), 0);
+    launch_acoustic_sources(stream, static_cast<int>(h.src_order_ac.size()), h.nsrc, istep, src_order_ac.p,
+                            src_ispec.p, gather.p, src_array.p, stf.p, kappa.p, fac);
+    acoustic_stiffness(fac);
+    launch_divide_corrector(stream, 1, fac, nc, 1, correct);
+  }
+  // the elastic side's interface term, before the elastic sources
+  void couple_into_elastic(Fields ff) {
+    launch_coupling_2d(stream, static_cast<int>(h.cpl_el_self.size()), cpl_el_self.p, cpl_el_other.p, cpl_el_coef.p,
+                       ff, fa(), 1);
   }
   void require_init() const {
     if (!initialized) throw std::logic_error("sfx_initialize must be called first");
@@ -576,6 +650,27 @@ struct sfx_engine {
       a_stale = true;
       return;
     }
+    if (h.coupled()) {
+      // time_marching.tpp:33-84 on a mesh with both media: the acoustic
+      // update and corrector, then the elastic update with the interface term
+      // from the corrected chi_dot_dot, seismograms of both media, and the
+      // next step's predictors (the elastic one fused with its corrector)
+      if (recording) throw std::invalid_argument("adjoint runs are implemented for elastic meshes");
+      if (evs) SFX_HIP(hipEventRecord(evs[0], stream));
+      update_acoustic(istep, 1);
+      couple_into_elastic(ff);
+      sources(ff, istep);
+      if (evs) SFX_HIP(hipEventRecord(evs[1], stream));
+      stiffness(ff);
+      if (evs) SFX_HIP(hipEventRecord(evs[2], stream));
+      boundary(ff);
+      if (istep % run.nstep_between_samples == 0) seismograms(ff, istep / run.nstep_between_samples, 1);
+      if (evs) SFX_HIP(hipEventRecord(evs[3], stream));
+      launch_update_fused(stream, h.ncomp, ff, nc);
+      launch_predictor(stream, 1, fa(), nc);
+      if (evs) SFX_HIP(hipEventRecord(evs[4], stream));
+      return;
+    }
     if (recording) {
       rec_mode = 1;
       rec_step = record.p + static_cast<size_t>(istep) * nstacey * h.npe * h.ncomp;
@@ -634,8 +729,27 @@ struct sfx_engine {
     if (medium < SFX_MEDIUM_ELASTIC_PSV || medium > SFX_MEDIUM_ELASTIC_PSV_T)
       throw std::invalid_argument("medium must be one of SFX_MEDIUM_*");
   }
-  int medium_elements(int medium) const { return medium == SFX_MEDIUM_ELASTIC_PSV ? h.nspec : 0; }
-  int medium_dofs(int medium) const { return medium == SFX_MEDIUM_ELASTIC_PSV ? h.nglob * h.ncomp : 0; }
+  int medium_elements(int medium) const {
+    if (medium == SFX_MEDIUM_ELASTIC_PSV) return h.nspec_el;
+    if (medium == SFX_MEDIUM_ACOUSTIC) return h.nspec - h.nspec_el;
+    return 0;
+  }
+  int medium_dofs(int medium) const {
+    if (medium == SFX_MEDIUM_ELASTIC_PSV) return h.nglob_el * h.ncomp;
+    if (medium == SFX_MEDIUM_ACOUSTIC) return h.nglob_ac;
+    return 0;
+  }
+  // the fields of (medium, wavefield): acoustic fields exist for the forward
+  // wavefield of meshes with acoustic elements
+  Fields medium_wavefield(int medium, int w, int &ncomp) {
+    if (medium == SFX_MEDIUM_ACOUSTIC) {
+      if (w != SFX_WAVEFIELD_FORWARD) throw std::invalid_argument("acoustic media have only the forward wavefield");
+      ncomp = 1;
+      return fa();
+    }
+    ncomp = h.ncomp;
+    return wavefield(w);
+  }
   NewmarkConst newmark(int direction) const {
     if (direction == SFX_DIRECTION_FORWARD) return nc;
     if (direction == SFX_DIRECTION_BACKWARD) return NewmarkConst{-nc.dt, -nc.dto2, nc.dt2o2};  // newmark.tpp:204-242
@@ -643,6 +757,7 @@ struct sfx_engine {
   }
   void require_adjoint_capable() const {
     if (h.dim != 2) throw std::invalid_argument("adjoint and backward wavefields are implemented for 2-D meshes");
+    if (h.coupled()) throw std::invalid_argument("adjoint runs are implemented for elastic meshes");
     if (gather2d()) throw std::invalid_argument("adjoint runs need SFX_SCATTER_COLORED or _ATOMIC");
   }
   // update_wavefields of the forward field (domain_kernels.hpp:69-126):
@@ -664,6 +779,7 @@ struct sfx_engine {
       forces_2d(ff, istep, nullptr);
       launch_assemble_update_2d(stream, h.nglob, fs2(), ff, nc, 1);
     } else {
+      if (h.coupled()) couple_into_elastic(ff);
       sources(ff, istep);
       stiffness(ff);
       boundary(ff);
@@ -929,6 +1045,28 @@ int sfx_get_sizes(const sfx_engine *e, int64_t *s) {
   });
 }
 
+int sfx_get_media_sizes(const sfx_engine *e, int64_t *s) {
+  return guarded([&] {
+    check_engine(e);
+    if (!s) throw std::invalid_argument("null output");
+    s[0] = e->h.nspec_el;
+    s[1] = e->h.nspec - e->h.nspec_el;
+    s[2] = e->h.nglob_el;
+    s[3] = e->h.nglob_ac;
+  });
+}
+
+int sfx_get_field_index_acoustic(const sfx_engine *e, int32_t *out) {
+  return guarded([&] {
+    check_engine(e);
+    if (!out) throw std::invalid_argument("null output");
+    if (e->h.field_index_ac.empty())
+      std::fill(out, out + e->h.nglob, -1);
+    else
+      std::memcpy(out, e->h.field_index_ac.data(), e->h.field_index_ac.size() * sizeof(int32_t));
+  });
+}
+
 int sfx_get_t0(const sfx_engine *e, double *t0) {
   return guarded([&] {
     check_engine(e);
@@ -977,6 +1115,7 @@ int sfx_initialize(sfx_engine *e) {
     check_device(e);
     const size_t stride = static_cast<size_t>(e->h.ncomp) * e->h.nglob_pad;
     launch_fill(e->stream, e->fields.p, 3 * stride, 0.0f);
+    if (e->h.coupled()) launch_fill(e->stream, e->ac_fields.p, 3 * static_cast<size_t>(e->h.nglob_ac_pad), 0.0f);
     if (e->seismo.n) launch_fill(e->stream, e->seismo.p, e->seismo.n, 0.0f);
     e->a_stale = false;
     SFX_HIP(hipGetLastError());
@@ -992,7 +1131,7 @@ int sfx_predictor(sfx_engine *e) {
     e->refresh_acceleration();
     launch_predictor(e->stream, e->h.ncomp, e->f(), e->nc);
     SFX_HIP(hipGetLastError());
-    dofs = e->h.nglob * e->h.ncomp;
+    dofs = e->h.nglob_el * e->h.ncomp;
   }, stream_of(e));
   return rc == SFX_OK ? dofs : rc;
 }
@@ -1004,7 +1143,7 @@ int sfx_update_wavefields(sfx_engine *e, int istep) {
     e->require_init();
     e->update_forward(istep);
     SFX_HIP(hipGetLastError());
-    nelem = e->h.nspec;
+    nelem = e->h.nspec_el;
   }, stream_of(e));
   return rc == SFX_OK ? nelem : rc;
 }
@@ -1017,7 +1156,7 @@ int sfx_corrector(sfx_engine *e) {
     e->refresh_acceleration();
     launch_divide_corrector(e->stream, e->h.ncomp, e->f(), e->nc, 0, 1);
     SFX_HIP(hipGetLastError());
-    dofs = e->h.nglob * e->h.ncomp;
+    dofs = e->h.nglob_el * e->h.ncomp;
   }, stream_of(e));
   return rc == SFX_OK ? dofs : rc;
 }
@@ -1057,11 +1196,12 @@ int sfx_predictor_phase(sfx_engine *e, int medium, int wavefield, int direction)
     e->require_init();
     sfx_engine::check_medium(medium);
     const NewmarkConst k = e->newmark(direction);
-    Fields F = e->wavefield(wavefield);
     dofs = e->medium_dofs(medium);
     if (!dofs) return;
+    int ncomp = 0;
+    Fields F = e->medium_wavefield(medium, wavefield, ncomp);
     if (wavefield == SFX_WAVEFIELD_FORWARD) e->refresh_acceleration();
-    launch_predictor(e->stream, e->h.ncomp, F, k);
+    launch_predictor(e->stream, ncomp, F, k);
     SFX_HIP(hipGetLastError());
   }, stream_of(e));
   return rc == SFX_OK ? dofs : rc;
@@ -1074,11 +1214,12 @@ int sfx_corrector_phase(sfx_engine *e, int medium, int wavefield, int direction)
     e->require_init();
     sfx_engine::check_medium(medium);
     const NewmarkConst k = e->newmark(direction);
-    Fields F = e->wavefield(wavefield);
     dofs = e->medium_dofs(medium);
     if (!dofs) return;
+    int ncomp = 0;
+    Fields F = e->medium_wavefield(medium, wavefield, ncomp);
     if (wavefield == SFX_WAVEFIELD_FORWARD) e->refresh_acceleration();
-    launch_divide_corrector(e->stream, e->h.ncomp, F, k, 0, 1);
+    launch_divide_corrector(e->stream, ncomp, F, k, 0, 1);
     SFX_HIP(hipGetLastError());
   }, stream_of(e));
   return rc == SFX_OK ? dofs : rc;
@@ -1092,7 +1233,10 @@ int sfx_update_wavefields_medium(sfx_engine *e, int medium, int wavefield, int i
     sfx_engine::check_medium(medium);
     nelem = e->medium_elements(medium);
     if (!nelem) return;
-    if (wavefield == SFX_WAVEFIELD_FORWARD)
+    if (medium == SFX_MEDIUM_ACOUSTIC) {
+      if (wavefield != SFX_WAVEFIELD_FORWARD) throw std::invalid_argument("acoustic media have only the forward wavefield");
+      e->update_acoustic(istep);
+    } else if (wavefield == SFX_WAVEFIELD_FORWARD)
       e->update_forward(istep);
     else if (wavefield == SFX_WAVEFIELD_ADJOINT)
       e->update_adjoint(istep);
@@ -1281,7 +1425,7 @@ int sfx_get_wavefield(const sfx_engine *ce, int wavefield, int which, float *out
     Fields F = e->wavefield(wavefield);
     const float *arrays[4] = {F.u, F.v, F.a, F.minv};
     SFX_HIP(hipStreamSynchronize(e->stream));
-    const size_t ng = e->h.nglob;
+    const size_t ng = e->h.nglob_el;
     std::vector<float> buf(e->h.nglob_pad);
     for (int c = 0; c < e->h.ncomp; c++) {
       SFX_HIP(hipMemcpy(buf.data(), arrays[which] + static_cast<size_t>(c) * e->h.nglob_pad,
@@ -1307,7 +1451,7 @@ int sfx_set_wavefield(sfx_engine *e, int wavefield, int which, const float *in) 
     Fields F = e->wavefield(wavefield);
     float *arrays[3] = {F.u, F.v, F.a};
     SFX_HIP(hipStreamSynchronize(e->stream));
-    const size_t ng = e->h.nglob;
+    const size_t ng = e->h.nglob_el;
     std::vector<float> buf(e->h.nglob_pad, 0.0f);
     for (int c = 0; c < e->h.ncomp; c++) {
       const float *src = in + static_cast<size_t>(c) * ng;
@@ -1324,6 +1468,36 @@ int sfx_set_wavefield(sfx_engine *e, int wavefield, int which, const float *in) 
 
 int sfx_set_field(sfx_engine *e, int which, const float *in) {
   return sfx_set_wavefield(e, SFX_WAVEFIELD_FORWARD, which, in);
+}
+
+int sfx_get_medium_field(const sfx_engine *ce, int medium, int which, float *out) {
+  if (medium != SFX_MEDIUM_ACOUSTIC) return sfx_get_wavefield(ce, SFX_WAVEFIELD_FORWARD, which, out);
+  return guarded([&] {
+    check_device(ce);
+    if (which < 0 || which > 3) throw std::invalid_argument("which must be 0..3");
+    if (!out) throw std::invalid_argument("null output");
+    if (!ce->h.coupled()) throw std::invalid_argument("the mesh has no acoustic element");
+    const Fields F = ce->fa();
+    const float *arrays[4] = {F.u, F.v, F.a, F.minv};
+    SFX_HIP(hipStreamSynchronize(ce->stream));
+    if (ce->h.nglob_ac)
+      SFX_HIP(hipMemcpy(out, arrays[which], ce->h.nglob_ac * sizeof(float), hipMemcpyDeviceToHost));
+  }, stream_of(ce));
+}
+
+int sfx_set_medium_field(sfx_engine *e, int medium, int which, const float *in) {
+  if (medium != SFX_MEDIUM_ACOUSTIC) return sfx_set_wavefield(e, SFX_WAVEFIELD_FORWARD, which, in);
+  return guarded([&] {
+    check_device(e);
+    if (which < 0 || which > 2) throw std::invalid_argument("which must be 0..2");
+    if (!in) throw std::invalid_argument("null input");
+    if (!e->h.coupled()) throw std::invalid_argument("the mesh has no acoustic element");
+    const Fields F = e->fa();
+    float *arrays[3] = {F.u, F.v, F.a};
+    SFX_HIP(hipStreamSynchronize(e->stream));
+    if (e->h.nglob_ac)
+      SFX_HIP(hipMemcpy(arrays[which], in, e->h.nglob_ac * sizeof(float), hipMemcpyHostToDevice));
+  }, stream_of(e));
 }
 
 int sfx_set_properties(sfx_engine *e, const float *rho, const float *kappa, const float *mu) {
@@ -1351,8 +1525,7 @@ int sfx_forward_record(sfx_engine *e, int snapshot_every) {
   return guarded([&] {
     check_device(e);
     e->require_init();
-    if (e->h.dim != 2) throw std::invalid_argument("adjoint runs are implemented for 2-D meshes");
-    if (e->gather2d()) throw std::invalid_argument("adjoint runs need SFX_SCATTER_COLORED or _ATOMIC");
+    e->require_adjoint_capable();
     if (snapshot_every < 0) throw std::invalid_argument("snapshot_every must be >= 0");
     const size_t need = static_cast<size_t>(e->run.nstep) * e->nstacey * e->h.npe * e->h.ncomp;
     if (e->record.n != need) e->record.alloc(need);

@@ -10,6 +10,8 @@
 //                                (corrector of step n and predictor of n+1 in one pass)
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+
 #include "kernels.hpp"
 #include "lattice.hpp"
 
@@ -798,10 +800,10 @@ __global__ __launch_bounds__(128) void sources_kernel(int npe, int nsrc, int ist
                                                       const int *__restrict__ ispec, const int *__restrict__ gather,
                                                       const float *__restrict__ array,
                                                       const float *__restrict__ stf, Fields f,
-                                                      const int *__restrict__ step) {
+                                                      const int *__restrict__ step, int norder) {
   const int p = threadIdx.x;
   const int k = blockIdx.x;
-  if (p >= npe || k >= nsrc) return;
+  if (p >= npe || k >= norder) return;
   const int istep = step ? *step : istep_arg;
   const int s = order[k];
   const float sv = stf[static_cast<size_t>(istep) * nsrc + s];
@@ -848,7 +850,7 @@ __global__ __launch_bounds__(128) void seismogram_kernel(int npe, int nrec, int 
                                                          Fields f, float dto2, int corrected,
                                                          float *__restrict__ seismo, PointGeom2D pg, GllConst q,
                                                          const int *__restrict__ step, int nsb, int sig_top,
-                                                         ForceSum2D fs) {
+                                                         ForceSum2D fs, const int *__restrict__ rec_list) {
   __shared__ float red[2][D][64];
   __shared__ float su[2][25];
   if (step) {  // sample only on steps istep % nsb == 0 (the same for the whole block)
@@ -856,7 +858,7 @@ __global__ __launch_bounds__(128) void seismogram_kernel(int npe, int nrec, int 
     if (is % nsb != 0) return;
     isig = sig_top >= 0 ? sig_top - is / nsb : is / nsb;  // sig_top: samples counted down in time
   }
-  const int r = blockIdx.x;
+  const int r = rec_list ? rec_list[blockIdx.x] : blockIdx.x;
   const int p = threadIdx.x;
   const int np = f.nglob_pad;
   int g = 0;
@@ -1554,27 +1556,33 @@ void launch_stacey(hipStream_t s, int dim, int nbnd, const int *gather, const fl
 }
 
 void launch_sources(hipStream_t s, int dim, int npe, int nsrc, int istep, const int *order, const int *ispec,
-                    const int *gather, const float *array, const float *stf, Fields f, const int *step) {
-  if (nsrc <= 0) return;
+                    const int *gather, const float *array, const float *stf, Fields f, const int *step, int norder) {
+  if (norder < 0) norder = nsrc;
+  if (norder > nsrc) norder = nsrc;
+  if (norder <= 0) return;
   if (dim == 2)
-    hipLaunchKernelGGL(sources_kernel<2>, dim3(nsrc), dim3(64), 0, s, npe, nsrc, istep, order, ispec, gather, array,
-                       stf, f, step);
+    hipLaunchKernelGGL(sources_kernel<2>, dim3(norder), dim3(64), 0, s, npe, nsrc, istep, order, ispec, gather, array,
+                       stf, f, step, norder);
   else
-    hipLaunchKernelGGL(sources_kernel<3>, dim3(nsrc), dim3(128), 0, s, npe, nsrc, istep, order, ispec, gather, array,
-                       stf, f, step);
+    hipLaunchKernelGGL(sources_kernel<3>, dim3(norder), dim3(128), 0, s, npe, nsrc, istep, order, ispec, gather,
+                       array, stf, f, step, norder);
 }
 
 void launch_seismograms(hipStream_t s, int dim, int npe, int nrec, int ntypes, const int *types, int isig,
                         const int *ispec, const int *gather, const float *h, Fields f, float dto2, int corrected,
                         float *seismo, const PointGeom2D &pg, const GllConst &q, const int *step, int nsb,
-                        int sig_top, ForceSum2D fs) {
-  if (nrec <= 0 || ntypes <= 0) return;
+                        int sig_top, ForceSum2D fs, const int *rec_list, int nlist) {
+  // nlist >= 0: the receivers rec_list[0..nlist) only (an empty list launches
+  // nothing, whatever rec_list holds); -1: every receiver
+  if (nlist > 0 && !rec_list) throw std::invalid_argument("receiver list missing");
+  const int nb = nlist >= 0 ? nlist : nrec;
+  if (nb <= 0 || ntypes <= 0) return;
   if (dim == 2)
-    hipLaunchKernelGGL(seismogram_kernel<2>, dim3(nrec), dim3(128), 0, s, npe, nrec, ntypes, types, isig, ispec,
-                       gather, h, f, dto2, corrected, seismo, pg, q, step, nsb, sig_top, fs);
+    hipLaunchKernelGGL(seismogram_kernel<2>, dim3(nb), dim3(128), 0, s, npe, nrec, ntypes, types, isig, ispec,
+                       gather, h, f, dto2, corrected, seismo, pg, q, step, nsb, sig_top, fs, rec_list);
   else
-    hipLaunchKernelGGL(seismogram_kernel<3>, dim3(nrec), dim3(128), 0, s, npe, nrec, ntypes, types, isig, ispec,
-                       gather, h, f, dto2, corrected, seismo, pg, q, step, nsb, sig_top, ForceSum2D{});
+    hipLaunchKernelGGL(seismogram_kernel<3>, dim3(nb), dim3(128), 0, s, npe, nrec, ntypes, types, isig, ispec,
+                       gather, h, f, dto2, corrected, seismo, pg, q, step, nsb, sig_top, ForceSum2D{}, rec_list);
 }
 
 void launch_elemforce_2d(hipStream_t s, int nspec, const int *gather, const float *xix, const float *xiz,

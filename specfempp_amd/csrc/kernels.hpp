@@ -162,8 +162,11 @@ void launch_seismograms_block(hipStream_t s, Lattice lat, int nrec, int ntypes, 
 void launch_stacey(hipStream_t s, int dim, int nbnd, const int *gather, const float *normal, const float *weight,
                    const float *rho_vp, const float *rho_vs, Fields f, const int *bnd_rec = nullptr,
                    float *rec = nullptr, const int *step = nullptr, long long rec_stride = 0);
+// norder >= 0: only the first norder sources of `order` (nsrc stays the
+// stride of stf); -1: all nsrc.
 void launch_sources(hipStream_t s, int dim, int npe, int nsrc, int istep, const int *order, const int *ispec,
-                    const int *gather, const float *array, const float *stf, Fields f, const int *step = nullptr);
+                    const int *gather, const float *array, const float *stf, Fields f, const int *step = nullptr,
+                    int norder = -1);
 // Per-point 2-D derivatives and properties read by the pressure seismogram.
 struct PointGeom2D {
   const float *xix, *xiz, *gammax, *gammaz, *kappa, *mu;
@@ -183,7 +186,8 @@ struct ForceSum2D {
 void launch_seismograms(hipStream_t s, int dim, int npe, int nrec, int ntypes, const int *types, int isig,
                         const int *ispec, const int *gather, const float *h, Fields f, float dto2, int corrected,
                         float *seismo, const PointGeom2D &pg, const GllConst &q, const int *step = nullptr,
-                        int nsb = 1, int sig_top = -1, ForceSum2D fs = ForceSum2D{});
+                        int nsb = 1, int sig_top = -1, ForceSum2D fs = ForceSum2D{}, const int *rec_list = nullptr,
+                        int nlist = -1);
 void launch_elemforce_2d(hipStream_t s, int nspec, const int *gather, const float *xix, const float *xiz,
                          const float *gammax, const float *gammaz, const float *jac, const float *kappa,
                          const float *mu, const GllConst &q, const float *u, int nglob_pad, const float *geo,
@@ -198,5 +202,28 @@ void launch_update_fused(hipStream_t s, int ncomp, Fields f, NewmarkConst nc);
 void launch_predictor(hipStream_t s, int ncomp, Fields f, NewmarkConst nc);
 void launch_divide_corrector(hipStream_t s, int ncomp, Fields f, NewmarkConst nc, int divide, int correct);
 void launch_fill(hipStream_t s, float *p, size_t n, float value);
+
+// ---- 2-D acoustic medium and elastic-acoustic coupling (acoustic.hip) ----
+// Acoustic stiffness over device element slots [first, first + count) of the
+// acoustic field f: rho^-1 grad chi, the Stacey term stacey[p] * chi_dot
+// (w |n| rho_vpinverse per point, 0 off absorbing edges) and zero at
+// free-surface points (dirichlet[p] = 1).
+void launch_acoustic_stiffness_2d(hipStream_t s, int first, int count, bool atomic, const int *gather,
+                                  const float *xix, const float *xiz, const float *gammax, const float *gammaz,
+                                  const float *jac, const float *rho_inv, const float *stacey, const int *dirichlet,
+                                  const GllConst &q, Fields f);
+// Interface terms: to_elastic = 0 adds factor (nx u_x + nz u_z) of src (the
+// elastic field) to dst (the acoustic field); to_elastic = 1 adds factor n
+// chi_dot_dot of src (acoustic) to dst (elastic).  coef (n, 3): factor, nx, nz.
+void launch_coupling_2d(hipStream_t s, int n, const int *self, const int *other, const float *coef, Fields dst,
+                        Fields src, int to_elastic);
+// Acoustic sources order[0..norder): a += stf * array / kappa.
+void launch_acoustic_sources(hipStream_t s, int norder, int nsrc, int istep, const int *order, const int *ispec,
+                             const int *gather, const float *array, const float *stf, const float *kappa, Fields f);
+// Seismograms of the receivers list[0..nlist) in acoustic elements.
+void launch_acoustic_seismograms(hipStream_t s, int nlist, const int *list, int nrec, int ntypes, const int *types,
+                                 int isig, const int *ispec, const int *gather, const float *h, const float *xix,
+                                 const float *xiz, const float *gammax, const float *gammaz, const float *rho_inv,
+                                 const GllConst &q, Fields f, float *seismo);
 
 }  // namespace sfx

@@ -183,4 +183,14 @@ inline float ricker(float t, float f0, float factor) {
   return -1.0 * factor * d2;
 }
 
+// Dirac: -factor times the Gaussian of utilities.cpp:6-12 (the second
+// integral of a Ricker) at f0 = 1 / (10 dt), src/source_time_function/
+// dirac.cpp:20-48.
+inline float dirac(float t, float f0, float factor) {
+  const float pi = static_cast<float>(M_PI);
+  const float a = pi * pi * f0 * f0;
+  const float g = -1.0 * std::exp(-a * t * t) / (2.0 * a);
+  return -1.0 * factor * g;
+}
+
 }  // namespace sfx

@@ -35,6 +35,8 @@ from . import _native
 from .meshes import material_arrays
 
 DISPLACEMENT, VELOCITY, ACCELERATION = 0, 1, 2
+# source time functions (SFX_STF_*)
+_STF_CODES = {"ricker": 0, "dirac": 1}
 _TYPE_CODES = {"displacement": 0, "velocity": 1, "acceleration": 2, "pressure": 3}
 
 
@@ -46,7 +48,8 @@ class _Mesh2D(C.Structure):
     _fields_ = [("nspec", C.c_int), ("ngnod", C.c_int), ("npgeo", C.c_int), ("coorg", C.c_void_p),
                 ("knods", C.c_void_p), ("material", C.c_void_p), ("nmaterials", C.c_int), ("rho", C.c_void_p),
                 ("vp", C.c_void_p), ("vs", C.c_void_p), ("nabs", C.c_int), ("abs_ispec", C.c_void_p),
-                ("abs_edge", C.c_void_p)]
+                ("abs_edge", C.c_void_p), ("nafs", C.c_int), ("afs_ispec", C.c_void_p), ("afs_type", C.c_void_p),
+                ("nfs", C.c_int), ("fs_acoustic", C.c_void_p), ("fs_elastic", C.c_void_p)]
 
 
 class _Mesh3D(C.Structure):
@@ -59,7 +62,7 @@ class _Mesh3D(C.Structure):
 class _Source(C.Structure):
     _fields_ = [("kind", C.c_int), ("x", C.c_double), ("y", C.c_double), ("z", C.c_double), ("angle", C.c_double),
                 ("fx", C.c_double), ("fy", C.c_double), ("fz", C.c_double), ("m", C.c_double * 6),
-                ("f0", C.c_double), ("tshift", C.c_double), ("factor", C.c_double)]
+                ("f0", C.c_double), ("tshift", C.c_double), ("factor", C.c_double), ("stf", C.c_int)]
 
 
 class _Receiver(C.Structure):
@@ -93,7 +96,8 @@ def _sources(sources, dim):
                  s.get("myz", 0.0)]
         for k in range(6):
             arr[i].m[k] = m[k]
-        arr[i].f0 = s["f0"]
+        arr[i].stf = _STF_CODES.get(s.get("stf") or "ricker", -1)  # the library refuses unknown ones
+        arr[i].f0 = s.get("f0", 0.0) if arr[i].stf == _STF_CODES["dirac"] else s["f0"]
         arr[i].tshift = s.get("tshift", 0.0)
         arr[i].factor = s.get("factor", 1.0)
     return arr
@@ -147,6 +151,7 @@ class _EngineBase:
         t0 = C.c_double()
         self._check(self.lib.sfx_get_t0(self.h, C.byref(t0)))
         self.t0 = t0.value
+        self.field_nglob = self.nglob  # points of the (elastic) field
 
     def close(self):
         if getattr(self, "h", None):
@@ -226,13 +231,13 @@ class _EngineBase:
 
     def wavefield_field(self, wavefield, which):
         codes = {"displacement": 0, "velocity": 1, "acceleration": 2, "mass_inverse": 3}
-        out = np.zeros((self.ncomp, self.nglob), dtype=np.float32)
+        out = np.zeros((self.ncomp, self.field_nglob), dtype=np.float32)
         self._check(self.lib.sfx_get_wavefield(self.h, WAVEFIELDS[wavefield], codes[which], _ptr(out)))
         return out
 
     def set_wavefield_field(self, wavefield, which, values):
         codes = {"displacement": 0, "velocity": 1, "acceleration": 2}
-        arr = np.ascontiguousarray(values, dtype=np.float32).reshape(self.ncomp, self.nglob)
+        arr = np.ascontiguousarray(values, dtype=np.float32).reshape(self.ncomp, self.field_nglob)
         self._check(self.lib.sfx_set_wavefield(self.h, WAVEFIELDS[wavefield], codes[which], _ptr(arr)))
 
     # ---- fused path ----
@@ -268,13 +273,13 @@ class _EngineBase:
 
     def field(self, which):
         codes = {"displacement": 0, "velocity": 1, "acceleration": 2, "mass_inverse": 3}
-        out = np.zeros((self.ncomp, self.nglob), dtype=np.float32)
+        out = np.zeros((self.ncomp, self.field_nglob), dtype=np.float32)
         self._check(self.lib.sfx_get_field(self.h, codes[which], _ptr(out)))
         return out
 
     def set_field(self, which, values):
         codes = {"displacement": 0, "velocity": 1, "acceleration": 2}
-        arr = np.ascontiguousarray(values, dtype=np.float32).reshape(self.ncomp, self.nglob)
+        arr = np.ascontiguousarray(values, dtype=np.float32).reshape(self.ncomp, self.field_nglob)
         self._check(self.lib.sfx_set_field(self.h, codes[which], _ptr(arr)))
 
 
@@ -363,10 +368,19 @@ class Engine2D(_EngineBase):
         rho, vp, vs = material_arrays(mesh.materials)
         abs_ispec = np.ascontiguousarray(mesh.abs_ispec, dtype=np.int32)
         abs_edge = np.ascontiguousarray(mesh.abs_type, dtype=np.int32)
-        self._nstacey = int(np.unique(abs_ispec).size)
-        self._keep = (coorg, knods, kmato, rho, vp, vs, abs_ispec, abs_edge)
+
+        def ints(name):
+            v = getattr(mesh, name, None)
+            return np.ascontiguousarray(v if v is not None else [], dtype=np.int32)
+
+        afs_ispec, afs_type = ints("afs_ispec"), ints("afs_type")
+        fs_ac, fs_el = ints("fs_acoustic"), ints("fs_elastic")
+        acoustic_mat = vs == 0.0
+        self._nstacey = int(np.unique(abs_ispec[~acoustic_mat[kmato[abs_ispec]]]).size) if len(abs_ispec) else 0
+        self._keep = (coorg, knods, kmato, rho, vp, vs, abs_ispec, abs_edge, afs_ispec, afs_type, fs_ac, fs_el)
         m = _Mesh2D(mesh.nspec, mesh.ngnod, mesh.npgeo, _ptr(coorg), _ptr(knods), _ptr(kmato), len(rho), _ptr(rho),
-                    _ptr(vp), _ptr(vs), len(abs_ispec), _ptr(abs_ispec), _ptr(abs_edge))
+                    _ptr(vp), _ptr(vs), len(abs_ispec), _ptr(abs_ispec), _ptr(abs_edge), len(afs_ispec),
+                    _ptr(afs_ispec), _ptr(afs_type), len(fs_ac), _ptr(fs_ac), _ptr(fs_el))
         run = _run_desc(dt, nstep, nstep_between_samples, seismogram_types, chunk_size, scatter, start_time)
         h = C.c_void_p()
         self.h = None
@@ -375,6 +389,25 @@ class Engine2D(_EngineBase):
                            C.byref(run), C.byref(h)))
         self.h = h
         self._finish_init(seismogram_types)
+        sizes = (C.c_int64 * 4)()
+        self._check(self.lib.sfx_get_media_sizes(self.h, sizes))
+        self.nspec_elastic, self.nspec_acoustic, self.nglob_elastic, self.nglob_acoustic = [int(x) for x in sizes]
+        self.field_nglob = self.nglob_elastic
+
+    # ---- acoustic medium ----
+    def field_index_acoustic(self):
+        """Acoustic field index of every global point (-1 off the acoustic elements)."""
+        out = np.zeros(self.nglob, dtype=np.int32)
+        self._check(self.lib.sfx_get_field_index_acoustic(self.h, _ptr(out)))
+        return out
+
+    def acoustic_field(self, which):
+        """chi ("displacement"), chi_dot, chi_dot_dot or the mass inverse of the
+        acoustic field, (nglob_acoustic,) on the acoustic field index."""
+        codes = {"displacement": 0, "velocity": 1, "acceleration": 2, "mass_inverse": 3}
+        out = np.zeros(self.nglob_acoustic, dtype=np.float32)
+        self._check(self.lib.sfx_get_medium_field(self.h, MEDIA["acoustic"], codes[which], _ptr(out)))
+        return out
 
 
 class Engine3D(_EngineBase):

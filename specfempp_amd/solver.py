@@ -167,7 +167,10 @@ class TimeMarching:
     def __init__(self, engine, time_scheme: Newmark, tasks=(), progress=None):
         self.engine, self.time_scheme, self.tasks = engine, time_scheme, list(tasks)
         self.progress = progress
-        self.total_dofs = engine.nglob * engine.ncomp
+        # assembly.get_total_degrees_of_freedom(): every medium's field points
+        # times its components (the acoustic potential has one)
+        self.total_dofs = (getattr(engine, "field_nglob", engine.nglob) * engine.ncomp
+                           + getattr(engine, "nglob_acoustic", 0))
         self.total_elements = engine.nspec
 
     def _check(self, dofs, elements, factor):

@@ -1,0 +1,189 @@
+"""Run the call sequences of the GPU tests on a host (ASan) build of the
+library at small sizes: the forward loop seam by seam (2-D colored and
+gather, 3-D gather), the combined loop seam by seam and fused, the Stacey
+record and point-coordinate getters and setters, wavefield snapshots between
+fused steps (2-D colored and gather, 3-D gather, colored, atomic), a 2-D
+moment-tensor source, storage chunks 2 and 64, and a small batch.  Checks
+against the FP32 oracle where a result exists there.
+Usage: check_paths2d.py LIB.so"""
+import os
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import numpy as np  # noqa: E402
+
+import oracle  # noqa: E402
+import specfempp_amd._native as nat  # noqa: E402
+
+nat._lib = None
+nat.LIB_PATH = sys.argv[1]
+nat.load()
+from conftest import rel_l2  # noqa: E402
+from specfempp_amd import Batch, Engine2D, Engine3D, meshes  # noqa: E402
+from specfempp_amd.solver import Newmark, TimeMarching  # noqa: E402
+
+failures = []
+
+
+def check(name, ok, detail=""):
+    print(f"{'ok  ' if ok else 'FAIL'} {name} {detail}", flush=True)
+    if not ok:
+        failures.append(name)
+
+
+N, H, DT, NSTEP = 6, 50.0, 1e-3, 8
+TYPES = ("displacement", "velocity", "acceleration")
+SRC = [dict(x=110.0, z=160.0, angle=30.0, f0=15.0, factor=1e10)]
+REC = [dict(x=200.0, z=250.0), dict(x=80.0, z=100.0)]
+
+
+def mesh2d(n=N):
+    L = n * H
+    return meshes.quad_mesh(n, n, 0, L, 0, L, materials=[meshes.elastic(2400.0, 3000.0, 1732.0)],
+                            absorbing=("bottom", "right", "left", "top"))
+
+
+mesh = mesh2d()
+ref, _ = oracle.Oracle2D(mesh, 4, 1, DT).forward(SRC, REC, NSTEP, DT, 1, TYPES)
+
+# forward loop seam by seam against the fused run (2-D colored, gather; 3-D gather)
+for scatter in ("colored", "gather"):
+    a = Engine2D(mesh, SRC, REC, DT, NSTEP, 2, TYPES, scatter=scatter)
+    a.initialize()
+    a.run()
+    fused = a.seismograms().astype(np.float64)
+    a.close()
+    b = Engine2D(mesh, SRC, REC, DT, NSTEP, 2, TYPES, scatter=scatter)
+    ts = Newmark(NSTEP, 2, DT, "forward")
+    ts.link_assembly(b)
+    TimeMarching(b, ts).run()
+    seam = b.seismograms().astype(np.float64)
+    b.close()
+    check(f"seams 2-D {scatter}", rel_l2(seam, fused) <= 1e-5 and rel_l2(fused[:, 0], ref[::2, 0]) <= 1e-5,
+          f"{rel_l2(seam, fused):.2e} {rel_l2(fused[:, 0], ref[::2, 0]):.2e}")
+
+w3 = meshes.config3(n=3, nstep=NSTEP, h=100.0, nrec_side=2)
+a = Engine3D(w3.mesh, w3.sources, w3.receivers, w3.dt, NSTEP, 2, TYPES, scatter="gather")
+a.initialize()
+a.run()
+fused3 = a.seismograms().astype(np.float64)
+a.close()
+b = Engine3D(w3.mesh, w3.sources, w3.receivers, w3.dt, NSTEP, 2, TYPES, scatter="gather")
+ts = Newmark(NSTEP, 2, w3.dt, "forward")
+ts.link_assembly(b)
+TimeMarching(b, ts).run()
+check("seams 3-D gather", rel_l2(b.seismograms().astype(np.float64), fused3) <= 1e-5)
+b.close()
+
+# combined loop: fused adjoint run, then seam by seam from the record, the
+# Stacey record and the coordinates through the getters and setters
+rng = np.random.default_rng(7)
+stf = 1e-6 * rng.standard_normal((len(REC), NSTEP, 2))
+fused = Engine2D(mesh, SRC, REC, DT, NSTEP, 1, ("displacement",), scatter="colored")
+fused.initialize()
+fused.forward_record(0)
+record = fused.stacey_record()
+coords = fused.point_coordinates()
+final = [fused.wavefield_field("forward", k) for k in ("displacement", "velocity", "acceleration")]
+fused.set_adjoint_sources(REC, stf)
+fused.adjoint_run()
+k_fused = fused.kernels()
+fused.close()
+eng = Engine2D(mesh, SRC, REC, DT, NSTEP, 1, ("displacement",), scatter="colored")
+eng.initialize()
+eng.set_stacey_record(record)
+for k, v in zip(("displacement", "velocity", "acceleration"), final):
+    eng.set_wavefield_field("buffer", k, v)
+eng.set_adjoint_sources(REC, stf)
+ts = Newmark(NSTEP, 1, DT, "combined")
+ts.link_assembly(eng)
+TimeMarching(eng, ts).run()
+k_seam = eng.kernels()
+eng.close()
+worst = max(rel_l2(k_seam[n].astype(np.float64), k_fused[n].astype(np.float64)) for n in k_fused)
+check("combined seams vs fused (record through the getters)", worst <= 1e-4 and coords.shape == (N * N, 5, 5, 2),
+      f"{worst:.2e}")
+# checkpoints every 3 steps
+eng = Engine2D(mesh, SRC, REC, DT, NSTEP, 1, ("displacement",), scatter="atomic")
+eng.initialize()
+eng.forward_record(3)
+eng.set_adjoint_sources(REC, stf)
+eng.adjoint_run()
+check("adjoint run with checkpoints (atomic)", all(np.isfinite(v).all() for v in eng.kernels().values()))
+eng.close()
+
+# wavefield snapshots between fused steps (the test_gpu_wavefields sequence)
+
+
+def snapshots(eng, steps):
+    eng.initialize()
+    done, out = 0, []
+    for k in steps:
+        if k - 1 > done:
+            eng.run(done, k - 1 - done)
+        eng.update_wavefields(k - 1)
+        eng.apply_corrector_phase()
+        eng.compute_seismograms(k - 1)
+        out.append([eng.field(x).astype(np.float64) for x in TYPES])
+        eng.apply_predictor_phase()
+        done = k
+    return out
+
+
+steps = (3, 5, NSTEP)
+for scatter in ("colored", "gather"):
+    eng = Engine2D(mesh, SRC, REC, DT, NSTEP, 1, ("displacement",), scatter=scatter)
+    snap = snapshots(eng, steps)
+    eng.close()
+    _, _, f = oracle.Oracle2D(mesh, 4, 1, DT).forward(SRC, REC, NSTEP, DT, 1, ("displacement",), return_fields=True)
+    check(f"snapshots 2-D {scatter}", rel_l2(snap[-1][0], f[0]) <= 1e-5, f"{rel_l2(snap[-1][0], f[0]):.2e}")
+for scatter in ("gather", "colored", "atomic"):
+    eng = Engine3D(w3.mesh, w3.sources, w3.receivers, w3.dt, NSTEP, 1, ("displacement",), scatter=scatter)
+    snap = snapshots(eng, steps)
+    eng.close()
+    _, _, f = oracle.Oracle3D(w3.mesh, 4, w3.dt).forward(w3.sources, w3.receivers, NSTEP, w3.dt, return_fields=True)
+    check(f"snapshots 3-D {scatter}", rel_l2(snap[-1][0], f[0]) <= 1e-5, f"{rel_l2(snap[-1][0], f[0]):.2e}")
+
+# 2-D moment tensor; storage chunks 2 and 64
+mt = [dict(type="moment_tensor", x=160.0, z=140.0, mxx=1.0, mzz=-0.5, mxz=0.3, f0=15.0, factor=1e10)]
+r_mt, _ = oracle.Oracle2D(mesh, 4, 1, DT).forward(mt, REC, NSTEP, DT, 1, ("displacement",))
+for scatter in ("colored", "gather"):
+    eng = Engine2D(mesh, mt, REC, DT, NSTEP, 1, ("displacement",), scatter=scatter)
+    eng.initialize()
+    eng.run()
+    e = rel_l2(eng.seismograms().astype(np.float64), r_mt)
+    eng.close()
+    check(f"moment tensor 2-D {scatter}", e <= 1e-5, f"{e:.2e}")
+for chunk in (2, 64):
+    eng = Engine2D(mesh, SRC, REC, DT, NSTEP, 1, ("displacement",), chunk_size=chunk)
+    o = oracle.Oracle2D(mesh, 4, chunk, DT)
+    ibool, midx, c2m = o.numbering()
+    eng.initialize()
+    eng.run()
+    r, _ = o.forward(SRC, REC, NSTEP, DT, 1, ("displacement",))
+    ok = np.array_equal(eng.index_mapping(), ibool) and np.array_equal(eng.field_index(), midx)
+    e = rel_l2(eng.seismograms().astype(np.float64), r)
+    eng.close()
+    check(f"chunk {chunk}", ok and e <= 1e-5, f"{e:.2e}")
+
+# a small batch of unequal meshes and step counts
+engs = [Engine2D(mesh2d(n), SRC, REC, DT, ns, 1, ("displacement",), host_only=True) for n, ns in ((4, 5), (6, 8))]
+b = Batch(engs)
+b.set_config(2, True)
+b.run()
+single = []
+for n, ns in ((4, 5), (6, 8)):
+    e = Engine2D(mesh2d(n), SRC, REC, DT, ns, 1, ("displacement",))
+    e.initialize()
+    e.run()
+    single.append(e.seismograms())
+    e.close()
+ok = all(np.array_equal(b.seismograms(i), single[i]) for i in range(2))
+b.close()
+check("batch vs single runs", ok)
+
+print("failures:", failures, flush=True)
+sys.exit(1 if failures else 0)

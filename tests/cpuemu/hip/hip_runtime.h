@@ -11,6 +11,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <memory>
+#include <stdexcept>
 #include <mutex>
 #include <thread>
 #include <tuple>
@@ -128,21 +130,37 @@ void launch(void (*k)(P...), dim3 grid, dim3 block, size_t, hipStream_t, A &&...
     return;
   }
   shfl_buf.assign(nt, 0.0f);
-  for (unsigned b = 0; b < nb; b++) {
-    std::barrier<> br(nt);
-    bar = &br;
-    std::vector<std::thread> th;
-    th.reserve(nt);
-    for (unsigned t = 0; t < nt; t++)
-      th.emplace_back([&, b, t] {
-        threadIdx.x = t; threadIdx.y = 0; threadIdx.z = 0;
-        blockIdx.x = b; blockIdx.y = 0; blockIdx.z = 0;
-        blockDim = block; gridDim = grid;
+  // nt pool threads run the blocks one after another; each block gets a fresh
+  // barrier for __syncthreads, which a thread leaves when its kernel returns
+  std::barrier<> start(nt + 1), done(nt + 1);
+  std::unique_ptr<std::barrier<>> br;
+  unsigned cur = 0;
+  bool stop = false;
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (unsigned t = 0; t < nt; t++)
+    th.emplace_back([&, t] {
+      threadIdx.x = t; threadIdx.y = 0; threadIdx.z = 0;
+      blockDim = block; gridDim = grid;
+      for (;;) {
+        start.arrive_and_wait();
+        if (stop) return;
+        blockIdx.x = cur; blockIdx.y = 0; blockIdx.z = 0;
         std::apply(k, params);
-        br.arrive_and_drop();
-      });
-    for (auto &x : th) x.join();
+        br->arrive_and_drop();
+        done.arrive_and_wait();
+      }
+    });
+  for (unsigned b = 0; b < nb; b++) {
+    br = std::make_unique<std::barrier<>>(nt);
+    bar = br.get();
+    cur = b;
+    start.arrive_and_wait();
+    done.arrive_and_wait();
   }
+  stop = true;
+  start.arrive_and_wait();
+  for (auto &x : th) x.join();
 }
 }  // namespace emu
 #define hipLaunchKernelGGL(...) emu::launch(__VA_ARGS__)

@@ -41,8 +41,15 @@ def test_2d_run_description_is_checked(kwargs, match):
 
 
 def test_2d_inputs_are_checked():
-    with pytest.raises(SfxError, match="only elastic isotropic materials"):
-        Engine2D(_mesh2d(vs=0.0), SRC2, REC2, 1e-3, 10, host_only=True)
+    with pytest.raises(SfxError, match="vs >= 0"):
+        Engine2D(_mesh2d(vs=-1.0), SRC2, REC2, 1e-3, 10, host_only=True)
+    # vs = 0 is an acoustic medium: a moment tensor cannot act there
+    with pytest.raises(SfxError, match="moment-tensor source in an acoustic element"):
+        Engine2D(_mesh2d(vs=0.0), [dict(SRC2[0], type="moment_tensor", mxx=1.0)], REC2, 1e-3, 10, host_only=True)
+    with pytest.raises(SfxError, match="SFX_SCATTER_COLORED or _ATOMIC"):
+        Engine2D(_mesh2d(vs=0.0), SRC2, REC2, 1e-3, 10, scatter="gather", host_only=True)
+    with pytest.raises(SfxError, match="source time function"):
+        Engine2D(_mesh2d(), [dict(SRC2[0], stf="gaussian")], REC2, 1e-3, 10, host_only=True)
     with pytest.raises(SfxError, match="source f0 must be positive"):
         Engine2D(_mesh2d(), [dict(SRC2[0], f0=0.0)], REC2, 1e-3, 10, host_only=True)
     with pytest.raises(SfxError, match="non-finite source or receiver position"):

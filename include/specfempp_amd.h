@@ -87,6 +87,33 @@ typedef struct {
   int absorbing[6];     /* Stacey on faces xmin, xmax, ymin, ymax, zmin, zmax (1 = yes) */
 } sfx_mesh3d;
 
+/* 3-D mesh database: the elastic content of specfem::mesh::mesh<dim3> as
+ * src/io/mesh/impl/fortran/dim3/mesh.cpp:43-797 reads it (generate_databases
+ * output): numbering, GLL point coordinates, the regular/irregular geometry
+ * split (mesh.cpp:150-151, 208-265), per-point kappa, mu, rho, rho vp, rho vs
+ * and the absorbing faces.  Element arrays are (nspec, 5, 5, 5) with the x
+ * index fastest (the record order of the reference's (nspec, ngllx, nglly,
+ * ngllz) views); indices are 0-based.  Elements not listed as irregular use
+ * xi_x = eta_y = gamma_z = xix_regular and jacobian_regular; the per-point
+ * derivatives are read for the irregular ones. */
+typedef struct {
+  int nspec, nglob;
+  const int *ibool;                     /* (nspec, 125) */
+  const float *x, *y, *z;               /* (nglob) */
+  int nspec_irregular;
+  const int *irregular_element_number;  /* (nspec_irregular) */
+  float xix_regular, jacobian_regular;
+  const float *xix, *xiy, *xiz, *etax, *etay, *etaz, *gammax, *gammay, *gammaz, *jacobian; /* (nspec, 125) */
+  const float *kappa, *mu, *rho;        /* (nspec, 125) */
+  const float *rho_vp, *rho_vs;         /* (nspec, 125): the Stacey impedances */
+  const int *ispec_type;                /* (nspec): 0 acoustic, 1 elastic, 2 poroelastic; elastic served */
+  int nabs;                             /* absorbing faces */
+  const int *abs_ispec;                 /* (nabs) */
+  const int *abs_ijk;                   /* (nabs, 3, 25): i, j, k of every face point, point fastest */
+  const float *abs_jacobian2Dw;         /* (nabs, 25): surface jacobian times the in-face GLL weights */
+  const float *abs_normal;              /* (nabs, 3, 25): unit outward normal, point fastest */
+} sfx_mesh3d_db;
+
 /* Source kinds (src/source/force_source.cpp, moment_tensor_source.cpp). */
 #define SFX_SOURCE_FORCE 0
 #define SFX_SOURCE_MOMENT_TENSOR 1
@@ -169,6 +196,15 @@ int sfx_engine_create_2d(const sfx_mesh2d *mesh, int nsources, const sfx_source 
                          const sfx_receiver *receivers, const sfx_run_desc *run, sfx_engine **out);
 int sfx_engine_create_3d(const sfx_mesh3d *mesh, int nsources, const sfx_source *sources, int nreceivers,
                          const sfx_receiver *receivers, const sfx_run_desc *run, sfx_engine **out);
+/* An engine on a 3-D mesh database (general hexahedra, per-point geometry on
+ * irregular elements): SFX_SCATTER_COLORED (greedy element coloring) or
+ * _ATOMIC.  Sources and receivers are located on the GLL points' isoparametric
+ * map (nearest point, then Newton iterations in the elements that hold it). */
+int sfx_engine_create_3d_db(const sfx_mesh3d_db *mesh, int nsources, const sfx_source *sources, int nreceivers,
+                            const sfx_receiver *receivers, const sfx_run_desc *run, sfx_engine **out);
+int sfx_engine_create_host_3d_db(const sfx_mesh3d_db *mesh, int nsources, const sfx_source *sources,
+                                 int nreceivers, const sfx_receiver *receivers, const sfx_run_desc *run,
+                                 sfx_engine **out);
 /* Host-only engines: the host half of the construction above (the
  * specfem::compute::assembly host views: numbering compute_mesh.cpp:45-170,
  * ordering :240-279, locate_point.cpp, sources and receivers) and no device

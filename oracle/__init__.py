@@ -67,6 +67,10 @@ def load():
     lib.orc3d_last_error.restype = C.c_char_p
     lib.orc3d_create.argtypes = [C.c_int] + [C.c_int] * 3 + [C.c_double] * 6 + [_ip, _dp, _dp, _dp, _ip, C.c_double,
                                                                                  C.POINTER(vp)]
+    fp = C.c_void_p
+    lib.orc3d_create_db.argtypes = ([C.c_int, C.c_int, C.c_int, _ip, fp, fp, fp, C.c_int, _ip, C.c_float, C.c_float,
+                                     C.POINTER(C.c_void_p)] + [fp] * 6 + [C.c_int, _ip, _ip, fp, fp, C.c_double,
+                                                                          C.POINTER(vp)])
     lib.orc3d_destroy.argtypes = [vp]
     lib.orc3d_nglob.argtypes = [vp]
     lib.orc3d_ibool.argtypes = [vp, _ip]
@@ -317,6 +321,37 @@ class Oracle3D:
         self.h = h
         self.nspec = mesh["nx"] * mesh["ny"] * mesh["nz"]
         self.nglob = lib.orc3d_nglob(h)
+
+    @classmethod
+    def from_database(cls, db, precision: int = 8, dt: float = 0.0):
+        """3-D restatement on a mesh database (specfempp_amd.io.database3d
+        layout): general hexahedra with per-point geometry."""
+        lib = load()
+        self = cls.__new__(cls)
+        self.precision = precision
+        f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32).reshape(-1)  # noqa: E731
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32).reshape(-1)  # noqa: E731
+        names = ("xix", "xiy", "xiz", "etax", "etay", "etaz", "gammax", "gammay", "gammaz")
+        d9 = [f32(db.derivs[n]) for n in names]
+        d9p = (C.c_void_p * 9)(*[a.ctypes.data for a in d9])
+        ab = db.absorbing
+        keep = dict(ibool=i32(db.ibool), x=f32(db.x), y=f32(db.y), z=f32(db.z), irr=i32(db.irregular_element_number),
+                    jac=f32(db.derivs["jacobian"]), kappa=f32(db.kappa), mu=f32(db.mu), rho=f32(db.rho),
+                    rvp=f32(db.rho_vp), rvs=f32(db.rho_vs), ai=i32(ab.ispec), aijk=i32(ab.ijk),
+                    ajw=f32(ab.jacobian2Dw), an=f32(ab.normal))
+        h = C.c_void_p()
+        rc = lib.orc3d_create_db(precision, db.nspec, db.nglob, keep["ibool"], _ptr(keep["x"]), _ptr(keep["y"]),
+                                 _ptr(keep["z"]), len(keep["irr"]), keep["irr"], float(db.xix_regular),
+                                 float(db.jacobian_regular), d9p, _ptr(keep["jac"]), _ptr(keep["kappa"]),
+                                 _ptr(keep["mu"]), _ptr(keep["rho"]), _ptr(keep["rvp"]), _ptr(keep["rvs"]),
+                                 len(keep["ai"]), keep["ai"], keep["aijk"], _ptr(keep["ajw"]), _ptr(keep["an"]),
+                                 dt, C.byref(h))
+        if rc != 0:
+            raise RuntimeError(lib.orc3d_last_error().decode())
+        self.h = h
+        self.nspec = db.nspec
+        self.nglob = lib.orc3d_nglob(h)
+        return self
 
     def __del__(self):
         if getattr(self, "h", None):

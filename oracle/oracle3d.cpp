@@ -109,6 +109,10 @@ template <class T> struct Asm {
   };
   std::vector<Bnd> bnd;
   std::vector<T> minv;             // (3, nglob)
+  // mesh databases (general hexahedra): the GLL point coordinates, read by
+  // the general locator
+  bool general = false;
+  std::vector<double> px, py, pz;  // (nglob)
 };
 
 // Trilinear map of a box element evaluated in general form.
@@ -272,6 +276,162 @@ Asm<T> *build(int nx, int ny, int nz, double x0, double y0, double z0, double hx
   return a;
 }
 
+// A 3-D mesh database (src/io/mesh/impl/fortran/dim3/mesh.cpp:43-797; no 3-D
+// solver exists in the reference): numbering as given, per-point derivatives
+// of the irregular elements, xix_regular / jacobian_regular for the others
+// (mesh.cpp:150-151), per-point properties, absorbing faces with unit normal
+// and jacobian2Dw (the traction of stacey3 with |n| = 1), mass as in build().
+template <class T>
+Asm<T> *build_db(int nspec, int nglob, const int *ibool, const float *x, const float *y, const float *z, int nirr,
+                 const int *irr, float xix_reg, float jac_reg, const float *const *d9in, const float *jac,
+                 const float *kappa, const float *mu, const float *rho, const float *rvp, const float *rvs, int nabs,
+                 const int *abs_ispec, const int *abs_ijk, const float *abs_jw, const float *abs_n, double dt) {
+  auto *a = new Asm<T>();
+  a->general = true;
+  a->nx = a->ny = a->nz = 0;
+  a->nspec = nspec;
+  a->nglob = nglob;
+  a->q = quad<T>();
+  const size_t np = static_cast<size_t>(nspec) * 125;
+  a->ibool.assign(ibool, ibool + np);
+  for (int v : a->ibool)
+    if (v < 0 || v >= nglob) throw std::runtime_error("ibool out of range");
+  a->px.assign(x, x + nglob);
+  a->py.assign(y, y + nglob);
+  a->pz.assign(z, z + nglob);
+  std::vector<char> irregular(nspec, 0);
+  for (int i = 0; i < nirr; i++) irregular[irr[i]] = 1;
+  a->dxi.assign(np * 9, 0);
+  a->jac.resize(np);
+  a->kappa.resize(np);
+  a->mu.resize(np);
+  a->rho.resize(np);
+  for (size_t p = 0; p < np; p++) {
+    const int e = static_cast<int>(p / 125);
+    if (irregular[e]) {
+      for (int d = 0; d < 9; d++) a->dxi[p * 9 + d] = static_cast<T>(d9in[d][p]);
+      a->jac[p] = static_cast<T>(jac[p]);
+    } else {
+      a->dxi[p * 9 + 0] = a->dxi[p * 9 + 4] = a->dxi[p * 9 + 8] = static_cast<T>(xix_reg);
+      a->jac[p] = static_cast<T>(jac_reg);
+    }
+    a->kappa[p] = static_cast<T>(kappa[p]);
+    a->mu[p] = static_cast<T>(mu[p]);
+    a->rho[p] = static_cast<T>(rho[p]);
+  }
+  for (int f = 0; f < nabs; f++)
+    for (int qf = 0; qf < 25; qf++) {
+      const int i = abs_ijk[(f * 3 + 0) * 25 + qf], j = abs_ijk[(f * 3 + 1) * 25 + qf],
+                k = abs_ijk[(f * 3 + 2) * 25 + qf];
+      typename Asm<T>::Bnd B;
+      B.e = abs_ispec[f];
+      B.p = (k * 5 + j) * 5 + i;
+      for (int d = 0; d < 3; d++) B.n[d] = static_cast<T>(abs_n[(f * 3 + d) * 25 + qf]);
+      B.w = static_cast<T>(abs_jw[f * 25 + qf]);
+      const size_t pp = static_cast<size_t>(B.e) * 125 + B.p;
+      B.rvp = static_cast<T>(rvp[pp]);
+      B.rvs = static_cast<T>(rvs[pp]);
+      a->bnd.push_back(B);
+    }
+  std::vector<T> mass(3 * static_cast<size_t>(nglob), 0);
+  for (int e = 0; e < nspec; e++)
+    for (int k = 0; k < N; k++)
+      for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) {
+          const size_t p = static_cast<size_t>(e) * 125 + (k * 5 + j) * 5 + i;
+          const T mm = a->rho[p] * (a->q.w[i] * a->q.w[j] * a->q.w[k] * a->jac[p]);
+          for (int c = 0; c < 3; c++) mass[static_cast<size_t>(c) * nglob + a->ibool[p]] += mm;
+        }
+  const T vf = static_cast<T>(-1.0) * static_cast<T>(dt) * static_cast<T>(0.5);
+  for (const auto &B : a->bnd) {
+    T t[3] = {0, 0, 0};
+    const T v[3] = {vf, vf, vf};
+    stacey3(B.n, B.w, B.rvp, B.rvs, v, t);
+    const int g = a->ibool[static_cast<size_t>(B.e) * 125 + B.p];
+    for (int c = 0; c < 3; c++) mass[static_cast<size_t>(c) * nglob + g] += t[c];
+  }
+  a->minv.resize(mass.size());
+  for (size_t i = 0; i < mass.size(); i++) a->minv[i] = static_cast<T>(1.0) / mass[i];
+  return a;
+}
+
+// Point location on a mesh database (the engine's model; the reference has
+// no 3-D locator): the nearest GLL point (double squared distance, first
+// minimum in global order), then in each element holding it (ascending) 100
+// Newton steps on the isoparametric map of the element's GLL points, clamped
+// to [-1.01, 1.01], from that point's reference coordinates; the smallest
+// final distance wins, first on ties.  Computed in double.
+template <class T> void locate_general(const Asm<T> &a, double x, double y, double z, int &ispec, double r_out[3]) {
+  const double xn[N] = {-1.0, -std::sqrt(3.0 / 7.0), 0.0, std::sqrt(3.0 / 7.0), 1.0};
+  int best = -1;
+  double dmin = std::numeric_limits<double>::max();
+  for (int g = 0; g < a.nglob; g++) {
+    const double d = (a.px[g] - x) * (a.px[g] - x) + (a.py[g] - y) * (a.py[g] - y) + (a.pz[g] - z) * (a.pz[g] - z);
+    if (d < dmin) {
+      dmin = d;
+      best = g;
+    }
+  }
+  auto map = [&](int e, const double r[3], double X[3], double D[3][3]) {
+    double h[3][N], hp[3][N];
+    for (int d = 0; d < 3; d++) lagrange<double>(r[d], xn, h[d], hp[d]);
+    for (int c = 0; c < 3; c++) {
+      X[c] = 0;
+      for (int s2 = 0; s2 < 3; s2++) D[c][s2] = 0;
+    }
+    for (int k = 0; k < N; k++)
+      for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) {
+          const int g = a.ibool[static_cast<size_t>(e) * 125 + (k * 5 + j) * 5 + i];
+          const double P[3] = {a.px[g], a.py[g], a.pz[g]};
+          const double w = h[0][i] * h[1][j] * h[2][k];
+          const double dw[3] = {hp[0][i] * h[1][j] * h[2][k], h[0][i] * hp[1][j] * h[2][k], h[0][i] * h[1][j] * hp[2][k]};
+          for (int c = 0; c < 3; c++) {
+            X[c] += w * P[c];
+            for (int s2 = 0; s2 < 3; s2++) D[c][s2] += dw[s2] * P[c];
+          }
+        }
+  };
+  double final_best = std::numeric_limits<double>::max();
+  ispec = -1;
+  for (int e = 0; e < a.nspec; e++) {
+    int pbest = -1;
+    for (int p = 0; p < 125 && pbest < 0; p++)
+      if (a.ibool[static_cast<size_t>(e) * 125 + p] == best) pbest = p;
+    if (pbest < 0) continue;
+    double r[3] = {xn[pbest % 5], xn[(pbest / 5) % 5], xn[pbest / 25]};
+    double X[3], D[3][3];
+    for (int it = 0; it < 100; it++) {
+      map(e, r, X, D);
+      const double res[3] = {x - X[0], y - X[1], z - X[2]};
+      const double det = D[0][0] * (D[1][1] * D[2][2] - D[1][2] * D[2][1]) -
+                         D[0][1] * (D[1][0] * D[2][2] - D[1][2] * D[2][0]) +
+                         D[0][2] * (D[1][0] * D[2][1] - D[1][1] * D[2][0]);
+      if (det == 0.0) break;
+      double inv[3][3];
+      inv[0][0] = (D[1][1] * D[2][2] - D[1][2] * D[2][1]) / det;
+      inv[0][1] = (D[0][2] * D[2][1] - D[0][1] * D[2][2]) / det;
+      inv[0][2] = (D[0][1] * D[1][2] - D[0][2] * D[1][1]) / det;
+      inv[1][0] = (D[1][2] * D[2][0] - D[1][0] * D[2][2]) / det;
+      inv[1][1] = (D[0][0] * D[2][2] - D[0][2] * D[2][0]) / det;
+      inv[1][2] = (D[0][2] * D[1][0] - D[0][0] * D[1][2]) / det;
+      inv[2][0] = (D[1][0] * D[2][1] - D[1][1] * D[2][0]) / det;
+      inv[2][1] = (D[0][1] * D[2][0] - D[0][0] * D[2][1]) / det;
+      inv[2][2] = (D[0][0] * D[1][1] - D[0][1] * D[1][0]) / det;
+      for (int s2 = 0; s2 < 3; s2++)
+        r[s2] = std::min(1.01, std::max(-1.01, r[s2] + inv[s2][0] * res[0] + inv[s2][1] * res[1] + inv[s2][2] * res[2]));
+    }
+    map(e, r, X, D);
+    const double d2 = (x - X[0]) * (x - X[0]) + (y - X[1]) * (y - X[1]) + (z - X[2]) * (z - X[2]);
+    if (d2 < final_best) {
+      final_best = d2;
+      ispec = e;
+      for (int s2 = 0; s2 < 3; s2++) r_out[s2] = r[s2];
+    }
+  }
+  if (ispec < 0) throw std::runtime_error("locate: no element holds the nearest point");
+}
+
 // -K_e u for one element (plus nothing else), general geometry.
 template <class T> void element_force(const Asm<T> &a, int e, const T *u /*(3,nglob)*/, T out[125][3]) {
   const auto &q = a.q;
@@ -343,6 +503,14 @@ void forward(const Asm<T> &a, int nsrc, const double *src /*(nsrc,16)*/, int nre
   if (nsrc == 0) min_t0 = 0;
   *t0_out = min_t0;
   auto locate = [&](double x, double y, double z, int &e, T &xi, T &et, T &ga) {
+    if (a.general) {
+      double r[3];
+      locate_general<T>(a, x, y, z, e, r);
+      xi = static_cast<T>(r[0]);
+      et = static_cast<T>(r[1]);
+      ga = static_cast<T>(r[2]);
+      return;
+    }
     auto one = [](double v, double v0, double h, int n, int &k, double &r) {
       const double s = (v - v0) / h;
       k = std::min(std::max(static_cast<int>(std::floor(s)), 0), n - 1);
@@ -380,9 +548,21 @@ void forward(const Asm<T> &a, int nsrc, const double *src /*(nsrc,16)*/, int nre
             const T h = hx[i] * hy[j] * hz[k];
             for (int c = 0; c < 3; c++) S[s].arr[c][p] = static_cast<T>(r[4 + c]) * h;
           } else {
-            // gradient of the Lagrange product through the element geometry
+            // gradient of the Lagrange product through the element geometry:
+            // on boxes the (constant) centre value; on a mesh database the
+            // derivatives averaged with the source point's Lagrange weights
+            // (the 3-D form of moment_tensor_source.cpp:88-130)
             const size_t pp = static_cast<size_t>(S[s].e) * 125 + 62;  // element centre (constant on boxes)
-            const T *d9 = &a.dxi[pp * 9];
+            T dav[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            if (a.general)
+              for (int kk = 0; kk < N; kk++)
+                for (int jj = 0; jj < N; jj++)
+                  for (int ii = 0; ii < N; ii++) {
+                    const T hl = hx[ii] * hy[jj] * hz[kk];
+                    const size_t q2 = static_cast<size_t>(S[s].e) * 125 + (kk * 5 + jj) * 5 + ii;
+                    for (int d = 0; d < 9; d++) dav[d] += hl * a.dxi[q2 * 9 + d];
+                  }
+            const T *d9 = a.general ? dav : &a.dxi[pp * 9];
             const T dref[3] = {hpx[i] * hy[j] * hz[k], hx[i] * hpy[j] * hz[k], hx[i] * hy[j] * hpz[k]};
             T dx[3];
             for (int rr = 0; rr < 3; rr++) dx[rr] = dref[0] * d9[0 * 3 + rr] + dref[1] * d9[1 * 3 + rr] + dref[2] * d9[2 * 3 + rr];
@@ -544,6 +724,27 @@ int orc3d_create(int precision, int nx, int ny, int nz, double x0, double y0, do
       h->a = orc3::build<float>(nx, ny, nz, x0, y0, z0, hx, hy, hz, material, rho, vp, vs, absorbing, dt);
     else
       h->a = orc3::build<double>(nx, ny, nz, x0, y0, z0, hx, hy, hz, material, rho, vp, vs, absorbing, dt);
+    *out = h;
+    return 0;
+  } catch (const std::exception &e) {
+    g_err3 = e.what();
+    return -1;
+  }
+}
+
+int orc3d_create_db(int precision, int nspec, int nglob, const int *ibool, const float *x, const float *y,
+                    const float *z, int nirr, const int *irr, float xix_reg, float jac_reg, const float *const *d9,
+                    const float *jac, const float *kappa, const float *mu, const float *rho, const float *rvp,
+                    const float *rvs, int nabs, const int *abs_ispec, const int *abs_ijk, const float *abs_jw,
+                    const float *abs_n, double dt, void **out) {
+  try {
+    auto *h = new H3{precision, nullptr};
+    if (precision == 4)
+      h->a = orc3::build_db<float>(nspec, nglob, ibool, x, y, z, nirr, irr, xix_reg, jac_reg, d9, jac, kappa, mu, rho,
+                                   rvp, rvs, nabs, abs_ispec, abs_ijk, abs_jw, abs_n, dt);
+    else
+      h->a = orc3::build_db<double>(nspec, nglob, ibool, x, y, z, nirr, irr, xix_reg, jac_reg, d9, jac, kappa, mu,
+                                    rho, rvp, rvs, nabs, abs_ispec, abs_ijk, abs_jw, abs_n, dt);
     *out = h;
     return 0;
   } catch (const std::exception &e) {

@@ -16,7 +16,8 @@ LIB_DIR = os.path.join(HERE, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libspecfempp_amd.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "specfempp_amd.h")
 
-SOURCES = ["engine.hip", "kernels.hip", "acoustic.hip", "block3d.hip", "batch.hip", "assembly.cpp"]
+SOURCES = ["engine.hip", "kernels.hip", "acoustic.hip", "general3d.hip", "block3d.hip", "batch.hip",
+           "assembly.cpp"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions stay off: some of their forms have given wrong
@@ -80,6 +81,8 @@ def load():
         "sfx_engine_create_3d": ([vp, i32, vp, i32, vp, vp, vp], i32),
         "sfx_engine_create_host_2d": ([vp, i32, vp, i32, vp, vp, vp], i32),
         "sfx_engine_create_host_3d": ([vp, i32, vp, i32, vp, vp, vp], i32),
+        "sfx_engine_create_3d_db": ([vp, i32, vp, i32, vp, vp, vp], i32),
+        "sfx_engine_create_host_3d_db": ([vp, i32, vp, i32, vp, vp, vp], i32),
         "sfx_engine_destroy": ([vp], None),
         "sfx_get_sizes": ([vp, vp], i32),
         "sfx_get_media_sizes": ([vp, vp], i32),

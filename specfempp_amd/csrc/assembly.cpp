@@ -1445,12 +1445,323 @@ HostAssembly build_assembly_3d(const sfx_mesh3d &m, int nsrc, const sfx_source *
 // ---------------------------------------------------------------------------
 // Element coloring
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// 3-D mesh database (general hexahedra)
+// ---------------------------------------------------------------------------
+namespace {
+
+// Element and reference coordinates of a point on a 3-D database mesh: the
+// nearest GLL point (squared distance in double, first minimum in global
+// index order), then, in every element holding that point (ascending element
+// order), Newton iterations on the isoparametric map of the element's 125 GLL
+// points, x(xi) = sum_p l_p(xi) x_p, from the reference coordinates of that
+// point, each step clamped to [-1.01, 1.01]; the candidate with the smallest
+// final distance wins (first on ties).  The same steps as the 2-D
+// locate_point (src/algorithms/locate_point.cpp:120-176) with the element's
+// GLL points as its control nodes.
+struct Locator3D {
+  const sfx_mesh3d_db &m;
+  std::vector<std::vector<int32_t>> owners;  // global point -> elements holding it (ascending)
+  double xi_nodes[5];
+  explicit Locator3D(const sfx_mesh3d_db &mesh, const float *nodes) : m(mesh), owners(mesh.nglob) {
+    for (int e = 0; e < m.nspec; e++)
+      for (int p = 0; p < 125; p++) {
+        auto &o = owners[m.ibool[static_cast<size_t>(e) * 125 + p]];
+        if (o.empty() || o.back() != e) o.push_back(e);
+      }
+    for (int i = 0; i < 5; i++) xi_nodes[i] = nodes[i];
+  }
+  void locate(double x, double y, double z, int &ispec, double ref[3]) const {
+    if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(z))
+      throw std::invalid_argument("non-finite source or receiver position");
+    int best = -1;
+    double dmin = std::numeric_limits<double>::max();
+    for (int g = 0; g < m.nglob; g++) {
+      const double dx = m.x[g] - x, dy = m.y[g] - y, dz = m.z[g] - z;
+      const double d = dx * dx + dy * dy + dz * dz;
+      if (d < dmin) {
+        dmin = d;
+        best = g;
+      }
+    }
+    if (best < 0 || owners[best].empty()) throw std::runtime_error("locate: no element holds the nearest point");
+    double final_best = std::numeric_limits<double>::max();
+    ispec = -1;
+    for (int e : owners[best]) {
+      // start from the reference coordinates of the nearest point in e
+      double r[3] = {0, 0, 0};
+      for (int p = 0; p < 125; p++)
+        if (m.ibool[static_cast<size_t>(e) * 125 + p] == best) {
+          r[0] = xi_nodes[p % 5];
+          r[1] = xi_nodes[(p / 5) % 5];
+          r[2] = xi_nodes[p / 25];
+          break;
+        }
+      double d2 = 0;
+      for (int it = 0; it < 100; it++) {
+        double h[3][5], hp[3][5];
+        for (int a = 0; a < 3; a++) lagrange<double>(r[a], xi_nodes, h[a], hp[a]);
+        double X[3] = {0, 0, 0}, D[3][3] = {{0}};  // D[c][s] = d x_c / d r_s
+        for (int k = 0; k < 5; k++)
+          for (int j = 0; j < 5; j++)
+            for (int i = 0; i < 5; i++) {
+              const int g = m.ibool[static_cast<size_t>(e) * 125 + (k * 5 + j) * 5 + i];
+              const double P[3] = {m.x[g], m.y[g], m.z[g]};
+              const double w = h[0][i] * h[1][j] * h[2][k];
+              const double dw[3] = {hp[0][i] * h[1][j] * h[2][k], h[0][i] * hp[1][j] * h[2][k],
+                                    h[0][i] * h[1][j] * hp[2][k]};
+              for (int c = 0; c < 3; c++) {
+                X[c] += w * P[c];
+                for (int s = 0; s < 3; s++) D[c][s] += dw[s] * P[c];
+              }
+            }
+        const double res[3] = {x - X[0], y - X[1], z - X[2]};
+        d2 = res[0] * res[0] + res[1] * res[1] + res[2] * res[2];
+        const double det = D[0][0] * (D[1][1] * D[2][2] - D[1][2] * D[2][1]) -
+                           D[0][1] * (D[1][0] * D[2][2] - D[1][2] * D[2][0]) +
+                           D[0][2] * (D[1][0] * D[2][1] - D[1][1] * D[2][0]);
+        if (det == 0.0) break;
+        const double inv[3][3] = {
+            {(D[1][1] * D[2][2] - D[1][2] * D[2][1]) / det, (D[0][2] * D[2][1] - D[0][1] * D[2][2]) / det,
+             (D[0][1] * D[1][2] - D[0][2] * D[1][1]) / det},
+            {(D[1][2] * D[2][0] - D[1][0] * D[2][2]) / det, (D[0][0] * D[2][2] - D[0][2] * D[2][0]) / det,
+             (D[0][2] * D[1][0] - D[0][0] * D[1][2]) / det},
+            {(D[1][0] * D[2][1] - D[1][1] * D[2][0]) / det, (D[0][1] * D[2][0] - D[0][0] * D[2][1]) / det,
+             (D[0][0] * D[1][1] - D[0][1] * D[1][0]) / det}};
+        for (int s = 0; s < 3; s++) {
+          const double dr = inv[s][0] * res[0] + inv[s][1] * res[1] + inv[s][2] * res[2];
+          r[s] = std::min(1.01, std::max(-1.01, r[s] + dr));
+        }
+      }
+      // distance at the final reference coordinates
+      {
+        double h[3][5];
+        for (int a = 0; a < 3; a++) lagrange<double>(r[a], xi_nodes, h[a], nullptr);
+        double X[3] = {0, 0, 0};
+        for (int k = 0; k < 5; k++)
+          for (int j = 0; j < 5; j++)
+            for (int i = 0; i < 5; i++) {
+              const int g = m.ibool[static_cast<size_t>(e) * 125 + (k * 5 + j) * 5 + i];
+              const double w = h[0][i] * h[1][j] * h[2][k];
+              X[0] += w * m.x[g];
+              X[1] += w * m.y[g];
+              X[2] += w * m.z[g];
+            }
+        d2 = (x - X[0]) * (x - X[0]) + (y - X[1]) * (y - X[1]) + (z - X[2]) * (z - X[2]);
+      }
+      if (d2 < final_best) {
+        final_best = d2;
+        ispec = e;
+        for (int s = 0; s < 3; s++) ref[s] = r[s];
+      }
+    }
+  }
+};
+
+}  // namespace
+
+HostAssembly build_assembly_3d_db(const sfx_mesh3d_db &m, int nsrc, const sfx_source *src, int nrec,
+                                  const sfx_receiver *rec, const sfx_run_desc &run) {
+  check_run(run);
+  for (int i = 0; i < run.nseismo_types; i++)
+    if (run.seismo_types[i] == SFX_PRESSURE) throw std::invalid_argument("pressure seismograms are 2-D only");
+  if (run.scatter != SFX_SCATTER_COLORED && run.scatter != SFX_SCATTER_ATOMIC)
+    throw std::invalid_argument("3-D database meshes need SFX_SCATTER_COLORED or _ATOMIC");
+  if (m.nspec <= 0 || m.nglob <= 0) throw std::invalid_argument("empty 3-D mesh");
+  if (!m.ibool || !m.x || !m.y || !m.z || !m.kappa || !m.mu || !m.rho || !m.rho_vp || !m.rho_vs || !m.ispec_type ||
+      !m.jacobian || !m.xix || !m.xiy || !m.xiz || !m.etax || !m.etay || !m.etaz || !m.gammax || !m.gammay ||
+      !m.gammaz || (m.nspec_irregular > 0 && !m.irregular_element_number) ||
+      (m.nabs > 0 && (!m.abs_ispec || !m.abs_ijk || !m.abs_jacobian2Dw || !m.abs_normal)))
+    throw std::invalid_argument("null 3-D database array");
+  if (m.nspec_irregular < 0 || m.nspec_irregular > m.nspec || m.nabs < 0)
+    throw std::invalid_argument("bad 3-D database counts");
+  const int64_t pad64 = (static_cast<int64_t>(m.nglob) + 63) / 64 * 64;
+  check_index_range_3d(pad64, m.nspec);
+  const int nspec = m.nspec, npe = 125;
+  const size_t nloc = static_cast<size_t>(nspec) * npe;
+  for (int e = 0; e < nspec; e++)
+    if (m.ispec_type[e] != 1) throw std::invalid_argument("3-D database meshes: elastic elements only");
+  HostAssembly a;
+  a.dim = 3;
+  a.ncomp = 3;
+  a.npe = npe;
+  a.general3d = true;
+  a.nspec = nspec;
+  a.nspec_el = nspec;
+  a.nspec_none = nspec;
+  const Gll g = make_gll();
+  std::copy(g.xi, g.xi + 5, a.xi);
+  std::copy(g.w, g.w + 5, a.w);
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j < 5; j++) a.hprime[i][j] = g.hprime[i][j];
+  // numbering as the database holds it (generate_databases' ibool)
+  a.index_mapping.assign(m.ibool, m.ibool + nloc);
+  for (int32_t v : a.index_mapping)
+    if (v < 0 || v >= m.nglob) throw std::invalid_argument("ibool out of range");
+  a.nglob = a.nglob_el = m.nglob;
+  a.nglob_pad = static_cast<int>(pad64);
+  a.field_index.resize(m.nglob);
+  for (int i = 0; i < m.nglob; i++) a.field_index[i] = i;
+  a.gather = a.index_mapping;
+  // geometry: per point the nine derivatives and the jacobian, (nspec, 10,
+  // 125); regular elements take xix_regular and jacobian_regular
+  // (mesh.cpp:150-151)
+  std::vector<char> irregular(nspec, 0);
+  for (int i = 0; i < m.nspec_irregular; i++) {
+    const int e = m.irregular_element_number[i];
+    if (e < 0 || e >= nspec) throw std::invalid_argument("irregular element number out of range");
+    irregular[e] = 1;
+  }
+  a.geo3.assign(nloc * 10, 0.0f);
+  const float *d9[9] = {m.xix, m.xiy, m.xiz, m.etax, m.etay, m.etaz, m.gammax, m.gammay, m.gammaz};
+  for (int e = 0; e < nspec; e++)
+    for (int p = 0; p < npe; p++) {
+      const size_t q = static_cast<size_t>(e) * npe + p;
+      float *gq = &a.geo3[static_cast<size_t>(e) * 10 * npe + p];
+      if (irregular[e]) {
+        for (int d = 0; d < 9; d++) gq[d * npe] = d9[d][q];
+        gq[9 * npe] = m.jacobian[q];
+      } else {
+        gq[0 * npe] = m.xix_regular;
+        gq[4 * npe] = m.xix_regular;
+        gq[8 * npe] = m.xix_regular;
+        gq[9 * npe] = m.jacobian_regular;
+      }
+      if (!(gq[9 * npe] > 0.0f))
+        throw std::runtime_error("element " + std::to_string(e) + " has a non-positive jacobian");
+    }
+  // properties per point (mesh.cpp:208-209, 265)
+  a.kappa.assign(m.kappa, m.kappa + nloc);
+  a.mu.assign(m.mu, m.mu + nloc);
+  a.rho.assign(m.rho, m.rho + nloc);
+  for (size_t q = 0; q < nloc; q++)
+    if (!(a.rho[q] > 0.0f) || !(a.mu[q] > 0.0f) || !(a.kappa[q] + (4.0f / 3.0f) * a.mu[q] > 0.0f))
+      throw std::invalid_argument("properties must give rho > 0, mu > 0 and lambda + 2 mu > 0");
+  // Stacey: one entry per point of every absorbing face, its unit normal and
+  // jacobian2Dw (the traction t = -[rho vp (v.n) n + rho vs (v - (v.n) n)] jacobian2Dw
+  // is the stacey.tpp:36-72 form with |n| = 1 and weight jacobian2Dw);
+  // entries of one element point are grouped in face order
+  for (int f = 0; f < m.nabs; f++) {
+    const int e = m.abs_ispec[f];
+    if (e < 0 || e >= nspec) throw std::invalid_argument("absorbing face element out of range");
+    for (int qf = 0; qf < 25; qf++) {
+      const int i = m.abs_ijk[(static_cast<size_t>(f) * 3 + 0) * 25 + qf];
+      const int j = m.abs_ijk[(static_cast<size_t>(f) * 3 + 1) * 25 + qf];
+      const int k = m.abs_ijk[(static_cast<size_t>(f) * 3 + 2) * 25 + qf];
+      if (i < 0 || i > 4 || j < 0 || j > 4 || k < 0 || k > 4) throw std::invalid_argument("face ijk out of range");
+      const size_t p = static_cast<size_t>(e) * npe + (k * 5 + j) * 5 + i;
+      a.bnd_gather.push_back(a.gather[p]);
+      a.bnd_slot.push_back(static_cast<int32_t>(p));
+      for (int d = 0; d < 3; d++) a.bnd_normal.push_back(m.abs_normal[(static_cast<size_t>(f) * 3 + d) * 25 + qf]);
+      a.bnd_weight.push_back(m.abs_jacobian2Dw[static_cast<size_t>(f) * 25 + qf]);
+      a.bnd_rho_vp.push_back(m.rho_vp[p]);
+      a.bnd_rho_vs.push_back(m.rho_vs[p]);
+    }
+  }
+  // mass: rho w_i w_j w_k J assembled, plus the Stacey term of a constant
+  // velocity -dt/2
+  {
+    std::vector<float> mass(3 * static_cast<size_t>(a.nglob_pad), 0.0f);
+    for (int e = 0; e < nspec; e++)
+      for (int k = 0; k < 5; k++)
+        for (int j = 0; j < 5; j++)
+          for (int i = 0; i < 5; i++) {
+            const int p = (k * 5 + j) * 5 + i;
+            const size_t q = static_cast<size_t>(e) * npe + p;
+            const float f = a.w[i] * a.w[j] * a.w[k] * a.geo3[static_cast<size_t>(e) * 10 * npe + 9 * npe + p];
+            const float mm = a.rho[q] * f;
+            for (int c = 0; c < 3; c++) mass[static_cast<size_t>(c) * a.nglob_pad + a.gather[q]] += mm;
+          }
+    const float vfac = -1.0f * static_cast<float>(run.dt) * 0.5f;
+    for (size_t q = 0; q < a.bnd_gather.size(); q++) {
+      float t[3] = {0, 0, 0};
+      stacey_term<3>(&a.bnd_normal[3 * q], a.bnd_weight[q], a.bnd_rho_vp[q], a.bnd_rho_vs[q], vfac, t);
+      for (int c = 0; c < 3; c++) mass[static_cast<size_t>(c) * a.nglob_pad + a.bnd_gather[q]] += t[c];
+    }
+    a.mass_inverse.assign(mass.size(), 0.0f);
+    for (size_t i = 0; i < mass.size(); i++)
+      if (mass[i] != 0.0f) a.mass_inverse[i] = 1.0f / mass[i];
+  }
+  // sources and receivers
+  Locator3D loc(m, a.xi);
+  setup_time_function(a, nsrc, src, run);
+  a.nsrc = nsrc;
+  a.src_ispec.resize(nsrc);
+  a.src_array.assign(static_cast<size_t>(nsrc) * 3 * npe, 0.0f);
+  a.src_local.resize(3 * static_cast<size_t>(nsrc));
+  for (int s = 0; s < nsrc; s++) {
+    int e;
+    double ref[3];
+    loc.locate(src[s].x, src[s].y, src[s].z, e, ref);
+    a.src_ispec[s] = e;
+    for (int d = 0; d < 3; d++) a.src_local[3 * s + d] = ref[d];
+    float hx[5], hpx[5], hy[5], hpy[5], hz[5], hpz[5];
+    lagrange<float>(static_cast<float>(ref[0]), a.xi, hx, hpx);
+    lagrange<float>(static_cast<float>(ref[1]), a.xi, hy, hpy);
+    lagrange<float>(static_cast<float>(ref[2]), a.xi, hz, hpz);
+    // element derivatives averaged with the Lagrange weights of the source
+    // point (the 3-D form of moment_tensor_source.cpp:88-130)
+    float dav[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 5; k++)
+      for (int j = 0; j < 5; j++)
+        for (int i = 0; i < 5; i++) {
+          const int p = (k * 5 + j) * 5 + i;
+          const float hl = hx[i] * hy[j] * hz[k];
+          for (int d = 0; d < 9; d++) dav[d] += hl * a.geo3[static_cast<size_t>(e) * 10 * npe + d * npe + p];
+        }
+    float *arr = &a.src_array[static_cast<size_t>(s) * 3 * npe];
+    for (int k = 0; k < 5; k++)
+      for (int j = 0; j < 5; j++)
+        for (int i = 0; i < 5; i++) {
+          const int p = (k * 5 + j) * 5 + i;
+          if (src[s].kind == SFX_SOURCE_FORCE) {
+            const float h = hx[i] * hy[j] * hz[k];
+            arr[p] = static_cast<float>(src[s].fx) * h;
+            arr[npe + p] = static_cast<float>(src[s].fy) * h;
+            arr[2 * npe + p] = static_cast<float>(src[s].fz) * h;
+          } else if (src[s].kind == SFX_SOURCE_MOMENT_TENSOR) {
+            const float dref[3] = {hpx[i] * hy[j] * hz[k], hx[i] * hpy[j] * hz[k], hx[i] * hy[j] * hpz[k]};
+            float dxr[3];
+            for (int r = 0; r < 3; r++) dxr[r] = dref[0] * dav[0 + r] + dref[1] * dav[3 + r] + dref[2] * dav[6 + r];
+            const float M[3][3] = {
+                {static_cast<float>(src[s].m[0]), static_cast<float>(src[s].m[3]), static_cast<float>(src[s].m[4])},
+                {static_cast<float>(src[s].m[3]), static_cast<float>(src[s].m[1]), static_cast<float>(src[s].m[5])},
+                {static_cast<float>(src[s].m[4]), static_cast<float>(src[s].m[5]), static_cast<float>(src[s].m[2])}};
+            for (int c = 0; c < 3; c++) arr[c * npe + p] = M[c][0] * dxr[0] + M[c][1] * dxr[1] + M[c][2] * dxr[2];
+          } else {
+            throw std::invalid_argument("unknown source kind");
+          }
+        }
+  }
+  for (int s = 0; s < nsrc; s++) a.src_order.push_back(s);
+  a.src_order_el = a.src_order;
+  a.nrec = nrec;
+  a.rec_ispec.resize(nrec);
+  a.rec_h.resize(static_cast<size_t>(nrec) * npe);
+  a.rec_local.resize(3 * static_cast<size_t>(nrec));
+  for (int r = 0; r < nrec; r++) {
+    int e;
+    double ref[3];
+    loc.locate(rec[r].x, rec[r].y, rec[r].z, e, ref);
+    a.rec_ispec[r] = e;
+    for (int d = 0; d < 3; d++) a.rec_local[3 * r + d] = ref[d];
+    float hx[5], hy[5], hz[5];
+    lagrange<float>(static_cast<float>(ref[0]), a.xi, hx, nullptr);
+    lagrange<float>(static_cast<float>(ref[1]), a.xi, hy, nullptr);
+    lagrange<float>(static_cast<float>(ref[2]), a.xi, hz, nullptr);
+    for (int k = 0; k < 5; k++)
+      for (int j = 0; j < 5; j++)
+        for (int i = 0; i < 5; i++) a.rec_h[static_cast<size_t>(r) * npe + (k * 5 + j) * 5 + i] = hx[i] * hy[j] * hz[k];
+  }
+  return a;
+}
+
 void color_elements(HostAssembly &a, int scatter) {
   const int nspec = a.nspec, npe = a.npe;
   std::vector<int> color(nspec, 0);
   int ncolors = 1;
   if (scatter == SFX_SCATTER_COLORED) {  // ATOMIC, GATHER and BLOCK keep compute order
-    if (a.dim == 3) {
+    if (a.dim == 3 && !a.general3d) {
       // structured hexahedra: the parity of (ex, ey, ez) is a valid 8-coloring
       // (it is also what first-fit greedy produces in element order)
       for (int e = 0; e < nspec; e++) {

@@ -44,6 +44,11 @@ struct HostAssembly {
   std::vector<float> geo2d;
   // 3-D regular geometry (all elements are boxes of the same size)
   float xix3 = 0, ety3 = 0, gaz3 = 0, jac3 = 0;
+  // 3-D database meshes (general hexahedra): per point the derivatives
+  // xix, xiy, xiz, etax, etay, etaz, gammax, gammay, gammaz and the jacobian,
+  // (nspec, 10, 125)
+  bool general3d = false;
+  std::vector<float> geo3;
   // properties per local point (kappa, mu; rho for mass and Stacey)
   std::vector<float> kappa, mu, rho;
   // Stacey entries: one per (element, local point) on an absorbing edge/face
@@ -133,5 +138,7 @@ HostAssembly build_assembly_2d(const sfx_mesh2d &mesh, int nsrc, const sfx_sourc
 void check_index_range_3d(int64_t nglob_pad, int64_t nspec);
 HostAssembly build_assembly_3d(const sfx_mesh3d &mesh, int nsrc, const sfx_source *src, int nrec,
                                const sfx_receiver *rec, const sfx_run_desc &run);
+HostAssembly build_assembly_3d_db(const sfx_mesh3d_db &mesh, int nsrc, const sfx_source *src, int nrec,
+                                  const sfx_receiver *rec, const sfx_run_desc &run);
 
 }  // namespace sfx

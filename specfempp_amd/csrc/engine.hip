@@ -77,6 +77,7 @@ struct sfx_engine {
   DevBuf<int> gather, bnd_gather, bnd_slot, bnd_seg, src_order, src_ispec, rec_ispec, types;
   DevBuf<float> xix, xiz, gammax, gammaz, jac, kappa, mu;
   DevBuf<float> geo;     // 2-D affine meshes: one geometry record per element (the forward stiffness reads it)
+  DevBuf<float> geo3;    // 3-D database meshes: per-point derivatives and jacobian (nspec, 10, 125)
   // 2-D gather path: element forces, the per-point inverse index, Stacey
   // entries sorted by element point with their segments
   DevBuf<float> re2;
@@ -349,6 +350,7 @@ struct sfx_engine {
     gather.upload(permuted(h.gather), s);
     kappa.upload(permuted(h.kappa), s);
     mu.upload(permuted(h.mu), s);
+    if (h.general3d) geo3.upload(permuted(h.geo3, 10 * static_cast<size_t>(h.npe)), s);
     if (h.dim == 2) {
       xix.upload(permuted(h.xix), s);
       xiz.upload(permuted(h.xiz), s);
@@ -518,6 +520,8 @@ struct sfx_engine {
       if (h.dim == 2)
         launch_stiffness_2d(stream, first, count, atomic, gather.p, xix.p, xiz.p, gammax.p, gammaz.p, jac.p, kappa.p,
                             mu.p, q, ff, rec_mode, rec_index.p, rec_step, nullptr, 0, geo.p);
+      else if (h.general3d)
+        launch_stiffness_3d_general(stream, first, count, atomic, gather.p, geo3.p, kappa.p, mu.p, q, ff);
       else
         launch_stiffness_3d(stream, first, count, atomic, gather.p, kappa.p, mu.p, h.xix3, h.ety3, h.gaz3, h.jac3, q,
                             ff);
@@ -1013,6 +1017,17 @@ int sfx_engine_create_2d(const sfx_mesh2d *mesh, int nsources, const sfx_source 
 int sfx_engine_create_3d(const sfx_mesh3d *mesh, int nsources, const sfx_source *sources, int nreceivers,
                          const sfx_receiver *receivers, const sfx_run_desc *run, sfx_engine **out) {
   return create_engine(mesh, nsources, sources, nreceivers, receivers, run, out, false, build_assembly_3d);
+}
+
+int sfx_engine_create_3d_db(const sfx_mesh3d_db *mesh, int nsources, const sfx_source *sources, int nreceivers,
+                            const sfx_receiver *receivers, const sfx_run_desc *run, sfx_engine **out) {
+  return create_engine(mesh, nsources, sources, nreceivers, receivers, run, out, false, build_assembly_3d_db);
+}
+
+int sfx_engine_create_host_3d_db(const sfx_mesh3d_db *mesh, int nsources, const sfx_source *sources,
+                                 int nreceivers, const sfx_receiver *receivers, const sfx_run_desc *run,
+                                 sfx_engine **out) {
+  return create_engine(mesh, nsources, sources, nreceivers, receivers, run, out, true, build_assembly_3d_db);
 }
 
 int sfx_engine_create_host_2d(const sfx_mesh2d *mesh, int nsources, const sfx_source *sources, int nreceivers,

@@ -203,6 +203,14 @@ void launch_predictor(hipStream_t s, int ncomp, Fields f, NewmarkConst nc);
 void launch_divide_corrector(hipStream_t s, int ncomp, Fields f, NewmarkConst nc, int divide, int correct);
 void launch_fill(hipStream_t s, float *p, size_t n, float value);
 
+// ---- 3-D general hexahedra (general3d.hip) ----
+// Stiffness over device element slots [first, first + count) with per-point
+// geometry geo (nspec, 10, 125): xix, xiy, xiz, etax, etay, etaz, gammax,
+// gammay, gammaz, jacobian.
+void launch_stiffness_3d_general(hipStream_t s, int first, int count, bool atomic, const int *gather,
+                                 const float *geo, const float *kappa, const float *mu, const GllConst &q,
+                                 Fields f);
+
 // ---- 2-D acoustic medium and elastic-acoustic coupling (acoustic.hip) ----
 // Acoustic stiffness over device element slots [first, first + count) of the
 // acoustic field f: rho^-1 grad chi, the Stacey term stacey[p] * chi_dot

@@ -59,6 +59,17 @@ class _Mesh3D(C.Structure):
                 ("vs", C.c_void_p), ("absorbing", C.c_int * 6)]
 
 
+class _Mesh3DDB(C.Structure):
+    _fields_ = [("nspec", C.c_int), ("nglob", C.c_int), ("ibool", C.c_void_p), ("x", C.c_void_p),
+                ("y", C.c_void_p), ("z", C.c_void_p), ("nspec_irregular", C.c_int),
+                ("irregular_element_number", C.c_void_p), ("xix_regular", C.c_float),
+                ("jacobian_regular", C.c_float)] + \
+               [(n, C.c_void_p) for n in ("xix", "xiy", "xiz", "etax", "etay", "etaz", "gammax", "gammay", "gammaz",
+                                          "jacobian", "kappa", "mu", "rho", "rho_vp", "rho_vs", "ispec_type")] + \
+               [("nabs", C.c_int), ("abs_ispec", C.c_void_p), ("abs_ijk", C.c_void_p),
+                ("abs_jacobian2Dw", C.c_void_p), ("abs_normal", C.c_void_p)]
+
+
 class _Source(C.Structure):
     _fields_ = [("kind", C.c_int), ("x", C.c_double), ("y", C.c_double), ("z", C.c_double), ("angle", C.c_double),
                 ("fx", C.c_double), ("fy", C.c_double), ("fz", C.c_double), ("m", C.c_double * 6),
@@ -448,6 +459,46 @@ class Engine3D(_EngineBase):
         h = C.c_void_p()
         self.h = None
         create = self.lib.sfx_engine_create_host_3d if host_only else self.lib.sfx_engine_create_3d
+        self._check(create(C.byref(m), len(sources), _sources(sources, 3), len(receivers), _receivers(receivers),
+                           C.byref(run), C.byref(h)))
+        self.h = h
+        self._finish_init(seismogram_types)
+
+
+class Engine3DDatabase(_EngineBase):
+    """3-D elastic engine on a mesh database (general hexahedra; see
+    ``specfempp_amd.io.database3d``): the reference's 3-D mesh as
+    read_3d_mesh holds it, with per-point geometry on irregular elements.
+    ``scatter`` is "colored" (greedy element coloring) or "atomic"."""
+
+    def __init__(self, db, sources, receivers, dt, nstep, nstep_between_samples=1,
+                 seismogram_types=("displacement",), scatter="colored", host_only=False, start_time=0.0):
+        self.lib = _native.load()
+        self.host_only = host_only
+        self.nstep, self.dt, self.scatter = nstep, dt, scatter
+        nspec = db.nspec
+        f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32).reshape(-1)  # noqa: E731
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32).reshape(-1)  # noqa: E731
+        ab = db.absorbing
+        arrays = dict(ibool=i32(db.ibool), x=f32(db.x), y=f32(db.y), z=f32(db.z),
+                      irregular_element_number=i32(db.irregular_element_number),
+                      kappa=f32(db.kappa), mu=f32(db.mu), rho=f32(db.rho), rho_vp=f32(db.rho_vp),
+                      rho_vs=f32(db.rho_vs), ispec_type=i32(db.ispec_type), abs_ispec=i32(ab.ispec),
+                      abs_ijk=i32(ab.ijk), abs_jacobian2Dw=f32(ab.jacobian2Dw), abs_normal=f32(ab.normal))
+        for n in ("xix", "xiy", "xiz", "etax", "etay", "etaz", "gammax", "gammay", "gammaz", "jacobian"):
+            arrays[n] = f32(db.derivs[n])
+        self._keep = arrays
+        m = _Mesh3DDB()
+        m.nspec, m.nglob = nspec, db.nglob
+        m.nspec_irregular = len(db.irregular_element_number)
+        m.xix_regular, m.jacobian_regular = float(db.xix_regular), float(db.jacobian_regular)
+        m.nabs = len(ab.ispec)
+        for k, v in arrays.items():
+            setattr(m, k, _ptr(v))
+        run = _run_desc(dt, nstep, nstep_between_samples, seismogram_types, 1, scatter, start_time)
+        h = C.c_void_p()
+        self.h = None
+        create = self.lib.sfx_engine_create_host_3d_db if host_only else self.lib.sfx_engine_create_3d_db
         self._check(create(C.byref(m), len(sources), _sources(sources, 3), len(receivers), _receivers(receivers),
                            C.byref(run), C.byref(h)))
         self.h = h

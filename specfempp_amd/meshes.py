@@ -275,3 +275,225 @@ def config3(n: int = 64, nstep: int = 1000, h: float = 100.0, nrec_side: int = 2
 
 def nglob_3d(n: int) -> int:
     return (4 * n + 1) ** 3
+
+
+# GLL points of NGLL = 5 on [-1, 1] and their weights (closed form)
+_GLL_X = np.array([-1.0, -np.sqrt(3.0 / 7.0), 0.0, np.sqrt(3.0 / 7.0), 1.0])
+_GLL_W = np.array([0.1, 49.0 / 90.0, 32.0 / 45.0, 49.0 / 90.0, 0.1])
+
+
+def hex_database(nx, ny, nz, h, material, rho, vp, vs, absorbing=(1, 1, 1, 1, 1, 0), perturb=0.0, seed=0,
+                 dt=1e-3):
+    """A 3-D mesh database (specfempp_amd.io.database3d.Mesh3DDatabase) in
+    the layout generate_databases writes and the reference reads
+    (src/io/mesh/impl/fortran/dim3/mesh.cpp): 8-node hexahedra on an nx x ny
+    x nz grid of spacing h whose interior corner nodes in the lower half along
+    x (node index 1 .. nx // 2) are moved by up to perturb * h along each axis
+    (seeded), so the elements touching them are irregular; the others are
+    cubes, described by xix_regular and jacobian_regular.
+
+    Per GLL point: the trilinear map's position, derivatives (xix ... gammaz)
+    and jacobian, evaluated in double; kappa, mu, rho, rho vp, rho vs of the
+    element's material (vs > 0).  Absorbing faces (xmin, xmax, ymin, ymax,
+    bottom, top flags): every face point's index, unit outward normal and
+    jacobian2Dw (surface jacobian times the two in-face GLL weights); the top
+    faces that are not absorbing form the free surface.  Mass matrix (rho w^3
+    J assembled) and the Stacey mass terms dt/2 jacobian2Dw (rho vp n_c^2 +
+    rho vs (1 - n_c^2)); single-process inner/outer arrays (every element
+    inner), surface flags and element adjacency (elements sharing a point).
+    Numbering: first visit in (element, k, j, i) order, elements x fastest."""
+    from .io.database3d import Faces, Mesh3DDatabase
+
+    material = np.asarray(material, dtype=np.int32).reshape(nz, ny, nx)
+    rho, vp, vs = (np.asarray(v, dtype=np.float64) for v in (rho, vp, vs))
+    if np.any(vs <= 0):
+        raise ValueError("hex_database builds elastic meshes (vs > 0)")
+    rng = np.random.default_rng(seed)
+    # corner nodes, interior ones perturbed
+    X = np.zeros((nz + 1, ny + 1, nx + 1, 3))
+    X[..., 0] = (np.arange(nx + 1) * h)[None, None, :]
+    X[..., 1] = (np.arange(ny + 1) * h)[None, :, None]
+    X[..., 2] = (np.arange(nz + 1) * h)[:, None, None]
+    moved = np.zeros((nz + 1, ny + 1, nx + 1), dtype=bool)
+    if perturb > 0:
+        inner = (slice(1, nz), slice(1, ny), slice(1, max(2, nx // 2 + 1)))
+        X[inner] += rng.uniform(-perturb * h, perturb * h, size=X[inner].shape)
+        moved[inner] = True
+    nspec = nx * ny * nz
+    # numbering: first visit on the GLL lattice
+    LX, LY, LZ = 4 * nx + 1, 4 * ny + 1, 4 * nz + 1
+    first = -np.ones(LX * LY * LZ, dtype=np.int64)
+    ibool = np.zeros((nspec, 5, 5, 5), dtype=np.int32)
+    inum = 0
+    for e in range(nspec):
+        ex, ey, ez = e % nx, (e // nx) % ny, e // (nx * ny)
+        for k in range(5):
+            for j in range(5):
+                for i in range(5):
+                    L = ((4 * ez + k) * LY + (4 * ey + j)) * LX + (4 * ex + i)
+                    if first[L] < 0:
+                        first[L] = inum
+                        inum += 1
+                    ibool[e, k, j, i] = first[L]
+    nglob = inum
+    xi = _GLL_X
+    # trilinear shape functions and derivatives at the 125 points: (8, 125), (3, 8, 125)
+    sgn = np.array([[(c & 1) * 2 - 1, ((c >> 1) & 1) * 2 - 1, ((c >> 2) & 1) * 2 - 1] for c in range(8)], float)
+    K, J_, I = np.meshgrid(np.arange(5), np.arange(5), np.arange(5), indexing="ij")
+    r = np.stack([xi[I.ravel()], xi[J_.ravel()], xi[K.ravel()]])  # (3, 125)
+    f = 1 + sgn[:, :, None] * r[None, :, :]                      # (8, 3, 125)
+    Nsh = f[:, 0] * f[:, 1] * f[:, 2] / 8
+    dN = np.stack([sgn[:, 0, None] * f[:, 1] * f[:, 2], f[:, 0] * sgn[:, 1, None] * f[:, 2],
+                   f[:, 0] * f[:, 1] * sgn[:, 2, None]]) / 8         # (3, 8, 125)
+    pts = np.zeros((nglob, 3))
+    derivs = {n: np.zeros((nspec, 5, 5, 5), np.float32) for n in
+              ("xix", "xiy", "xiz", "etax", "etay", "etaz", "gammax", "gammay", "gammaz", "jacobian")}
+    jac64 = np.zeros((nspec, 125))
+    inv64 = np.zeros((nspec, 125, 3, 3))  # [s][r] = d xi_s / d x_r
+    regular = np.zeros(nspec, dtype=bool)
+    for e in range(nspec):
+        ex, ey, ez = e % nx, (e // nx) % ny, e // (nx * ny)
+        C = np.array([X[ez + ((c >> 2) & 1), ey + ((c >> 1) & 1), ex + (c & 1)] for c in range(8)])  # (8, 3)
+        corner_moved = any(moved[ez + ((c >> 2) & 1), ey + ((c >> 1) & 1), ex + (c & 1)] for c in range(8))
+        regular[e] = not corner_moved
+        P = Nsh.T @ C                                   # (125, 3)
+        pts[ibool[e].ravel()] = P
+        dX = np.einsum("snp,nr->prs", dN, C)            # (125, 3 r, 3 s): d x_r / d xi_s
+        Jd = np.linalg.det(dX)
+        inv = np.linalg.inv(dX)                         # (125, 3 s, 3 r)
+        jac64[e] = Jd
+        inv64[e] = inv
+        for s_, nm in enumerate(("xi", "eta", "gamma")):
+            for r_, ax in enumerate("xyz"):
+                derivs[nm + ax][e] = inv[:, s_, r_].reshape(5, 5, 5)
+        derivs["jacobian"][e] = Jd.reshape(5, 5, 5)
+    mat = material.ravel()
+    rho_p = np.repeat(rho[mat], 125).reshape(nspec, 5, 5, 5).astype(np.float32)
+    mu_p = np.repeat((rho * vs * vs)[mat], 125).reshape(nspec, 5, 5, 5).astype(np.float32)
+    kap_p = np.repeat((rho * (vp * vp - 4.0 / 3.0 * vs * vs))[mat], 125).reshape(nspec, 5, 5, 5).astype(np.float32)
+    rvp = np.repeat((rho * vp)[mat], 125).reshape(nspec, 5, 5, 5).astype(np.float32)
+    rvs = np.repeat((rho * vs)[mat], 125).reshape(nspec, 5, 5, 5).astype(np.float32)
+    w3 = (_GLL_W[I] * _GLL_W[J_] * _GLL_W[K]).ravel()
+    mass = np.zeros(nglob)
+    np.add.at(mass, ibool.reshape(nspec, 125), rho[mat][:, None] * w3[None, :] * jac64)
+
+    # faces: (side, element test, fixed axis, fixed index, outward sign)
+    sides = [("xmin", 0, 0, -1), ("xmax", 0, 4, 1), ("ymin", 1, 0, -1), ("ymax", 1, 4, 1), ("bottom", 2, 0, -1),
+             ("top", 2, 4, 1)]
+
+    def face_points(e, axis, fixed, sign):
+        ijk = np.zeros((3, 25), dtype=np.int32)
+        nrm = np.zeros((3, 25))
+        jw = np.zeros(25)
+        o1, o2 = [a for a in range(3) if a != axis]
+        q = 0
+        for b in range(5):
+            for a in range(5):
+                idx = [0, 0, 0]
+                idx[axis], idx[o1], idx[o2] = fixed, a, b
+                i_, j2, k_ = idx
+                p = (k_ * 5 + j2) * 5 + i_
+                # Nanson: the area vector is J grad(xi_axis) dA_ref
+                n = sign * jac64[e, p] * inv64[e, p, axis, :]
+                area = np.linalg.norm(n)
+                ijk[:, q] = idx
+                nrm[:, q] = n / area
+                jw[q] = area * _GLL_W[a] * _GLL_W[b]
+                q += 1
+        return ijk, nrm, jw
+
+    abs_f, free_f, ibelm = [], [], {}
+    for side, axis, fixed, sign in sides:
+        flag = absorbing[["xmin", "xmax", "ymin", "ymax", "bottom", "top"].index(side)]
+        lst = []
+        for e in range(nspec):
+            ex, ey, ez = e % nx, (e // nx) % ny, e // (nx * ny)
+            pos = (ex, ey, ez)[axis]
+            last = (nx, ny, nz)[axis] - 1
+            if pos != (0 if fixed == 0 else last):
+                continue
+            lst.append(e)
+            if flag:
+                abs_f.append((e,) + face_points(e, axis, fixed, sign))
+            elif side == "top":
+                free_f.append((e,) + face_points(e, axis, fixed, sign))
+        ibelm[side] = np.array(lst if flag else [], dtype=np.int32)
+
+    def faces(lst):
+        if not lst:
+            return Faces.empty()
+        return Faces(np.array([f_[0] for f_ in lst], np.int32), np.array([f_[1] for f_ in lst], np.int32),
+                     np.array([f_[3] for f_ in lst], np.float32), np.array([f_[2] for f_ in lst], np.float32))
+
+    absorbing_faces, free_surface = faces(abs_f), faces(free_f)
+    abs_mass = np.zeros((3, nglob))
+    for e, ijk, nrm, jw in abs_f:
+        for q in range(25):
+            i_, j2, k_ = ijk[:, q]
+            g = ibool[e, k_, j2, i_]
+            m = mat[e]
+            for c in range(3):
+                abs_mass[c, g] += dt / 2 * jw[q] * (rho[m] * vp[m] * nrm[c, q] ** 2 +
+                                                    rho[m] * vs[m] * (1 - nrm[c, q] ** 2))
+    # adjacency: elements sharing a point
+    owners = [[] for _ in range(nglob)]
+    for e in range(nspec):
+        for g in np.unique(ibool[e]):
+            owners[g].append(e)
+    xadj, adj = [0], []
+    for e in range(nspec):
+        nb = sorted({o for g in np.unique(ibool[e]) for o in owners[g]} - {e})
+        adj += nb
+        xadj.append(len(adj))
+    surface_elems = np.zeros(nspec, dtype=bool)
+    for lst in ibelm.values():
+        surface_elems[lst] = True
+    for e in range(nspec):
+        ex, ey, ez = e % nx, (e // nx) % ny, e // (nx * ny)
+        surface_elems[e] = ex in (0, nx - 1) or ey in (0, ny - 1) or ez in (0, nz - 1)
+    lat = np.zeros(nglob, dtype=bool)
+    for e in range(nspec):
+        for k in range(5):
+            for j in range(5):
+                for i in range(5):
+                    ex, ey, ez = e % nx, (e // nx) % ny, e // (nx * ny)
+                    I0, J0, K0 = 4 * ex + i, 4 * ey + j, 4 * ez + k
+                    if I0 in (0, LX - 1) or J0 in (0, LY - 1) or K0 in (0, LZ - 1):
+                        lat[ibool[e, k, j, i]] = True
+    nsurf_faces = 2 * (nx * ny + ny * nz + nx * nz)
+    irregular = np.flatnonzero(~regular).astype(np.int32)
+    params = dict(acoustic_simulation=False, elastic_simulation=True, poroelastic_simulation=False,
+                  anisotropy=False, stacey_abc=bool(any(absorbing)), pml_abc=False, approximate_ocean_load=False,
+                  use_mesh_coloring=False, ndim=3, ngllx=5, nglly=5, ngllz=5, ngllsquare=25, nproc=1,
+                  nspec=nspec, nspec_poro=0, nglob=nglob, nglob_ocean=0,
+                  nspec2D_bottom=len(ibelm["bottom"]), nspec2D_top=len(ibelm["top"]),
+                  nspec2D_xmin=len(ibelm["xmin"]), nspec2D_xmax=len(ibelm["xmax"]),
+                  nspec2D_ymin=len(ibelm["ymin"]), nspec2D_ymax=len(ibelm["ymax"]), nspec_irregular=len(irregular),
+                  num_neighbors=len(adj), nfaces_surface=nsurf_faces,
+                  num_abs_boundary_faces=len(absorbing_faces.ispec),
+                  num_free_surface_faces=len(free_surface.ispec), num_coupling_ac_el_faces=0,
+                  num_coupling_ac_po_faces=0, num_coupling_el_po_faces=0, num_coupling_po_el_faces=0,
+                  num_interfaces_ext_mesh=0, max_nibool_interfaces_ext_mesh=0,
+                  nspec_inner_acoustic=0, nspec_outer_acoustic=0, nspec_inner_elastic=nspec,
+                  nspec_outer_elastic=0, nspec_inner_poroelastic=0, nspec_outer_poroelastic=0,
+                  num_phase_ispec_acoustic=0, num_phase_ispec_elastic=nspec, num_phase_ispec_poroelastic=0,
+                  num_colors_inner_acoustic=0, num_colors_outer_acoustic=0, num_colors_inner_elastic=0,
+                  num_colors_outer_elastic=0)
+    phase = np.zeros((nspec, 2), dtype=np.int32)
+    phase[:, 1] = np.arange(nspec)
+    db = Mesh3DDatabase(params, ibool, pts[:, 0].astype(np.float32), pts[:, 1].astype(np.float32),
+                        pts[:, 2].astype(np.float32), irregular, np.float32(2.0 / h), np.float32((h / 2) ** 3),
+                        derivs, kap_p, mu_p, rho_p, np.ones(nspec, dtype=np.int32))
+    db.mass_elastic = mass.astype(np.float32)
+    db.rho_vp, db.rho_vs = rvp, rvs
+    db.absorbing = absorbing_faces
+    db.abs_mass_elastic = tuple(abs_mass[c].astype(np.float32) for c in range(3))
+    db.ibelm = ibelm
+    db.free_surface = free_surface
+    db.ispec_is_inner = np.ones(nspec, dtype=bool)
+    db.inner_outer = {"elastic": (nspec, 0, phase)}
+    db.ispec_is_surface = surface_elems
+    db.iglob_is_surface = lat
+    db.neighbors_xadj = np.array(xadj, dtype=np.int32)
+    db.neighbors_adjncy = np.array(adj, dtype=np.int32)
+    return db

@@ -13,10 +13,10 @@ C=$R/specfempp_amd/csrc
 OUT=$1
 T=$(mktemp -d)
 FLAGS="-std=c++20 -g -O1 -fPIC -fsanitize=address,bounds -fno-sanitize-recover=bounds -fno-omit-frame-pointer -I$D -I$C -I$R/include"
-for f in engine batch block3d kernels acoustic; do
+for f in engine batch block3d kernels acoustic general3d; do
   g++ $FLAGS -x c++ -c "$C/$f.hip" -o "$T/$f.o" &
 done
 g++ $FLAGS -c "$C/assembly.cpp" -o "$T/assembly.o" &
 wait
-g++ -shared -fsanitize=address,bounds -o "$OUT" "$T"/engine.o "$T"/batch.o "$T"/kernels.o "$T"/acoustic.o "$T"/block3d.o "$T"/assembly.o -lpthread
+g++ -shared -fsanitize=address,bounds -o "$OUT" "$T"/engine.o "$T"/batch.o "$T"/kernels.o "$T"/acoustic.o "$T"/general3d.o "$T"/block3d.o "$T"/assembly.o -lpthread
 rm -rf "$T"

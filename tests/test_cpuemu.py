@@ -6,6 +6,7 @@ oracle:
   fused and seam calls;
 - the 2-D and 3-D call sequences of the seam, combined-loop, checkpoint,
   snapshot, moment-tensor, storage-chunk and batch GPU tests at small sizes;
+- the 3-D mesh-database path (general hexahedra) on a small irregular mesh;
 - the acoustic and coupled elastic-acoustic fixtures for a few steps (the
   regression case of an empty elastic receiver list, which once launched the
   elastic seismogram kernel on acoustic receivers).
@@ -29,7 +30,7 @@ def emu_lib(tmp_path_factory):
 
 
 @pytest.mark.parametrize("check", ["check_gather3d.py", "check_block3d.py", "check_paths2d.py",
-                                   "check_acoustic2d.py"])
+                                   "check_acoustic2d.py", "check_database3d.py"])
 def test_3d_paths_under_asan(emu_lib, check):
     lib = emu_lib
     asan = subprocess.run(["gcc", "-print-file-name=libasan.so"], capture_output=True, text=True,

@@ -1,0 +1,145 @@
+"""The 3-D mesh database (the layout src/io/mesh/impl/fortran/dim3/mesh.cpp
+reads) on the CPU: the writer/reader round trip, and the oracle and the host
+assembly on database meshes.
+
+The reference holds no 3-D database file, so the layout is pinned by the
+round trip through the reference reader's record order and check values, and
+the database path of the oracle by:
+- an undistorted cube written as a database (regular elements, faces with unit
+  normals and jacobian2Dw) gives the structured restatement's seismograms
+  (which are pinned by energy conservation and the closed-form Stacey terms);
+- on an irregular mesh (interior nodes moved) without absorbing faces the
+  central-difference energy is conserved once the source has died out;
+- the mass and Stacey mass terms equal the database's own arrays (rho w^3 J
+  assembled, dt/2 jacobian2Dw (rho vp n^2 + rho vs (1 - n^2))), computed
+  independently in numpy;
+- located points map back onto the target through the element's GLL points.
+"""
+import os
+
+import numpy as np
+
+import oracle
+from specfempp_amd import meshes
+from specfempp_amd.io import database3d as d3
+
+
+def _db(nx=4, ny=3, nz=3, perturb=0.12, absorbing=(1, 1, 1, 1, 1, 0), seed=3, dt=1e-3):
+    mat = np.zeros((nz, ny, nx), np.int32)
+    mat[nz // 2:] = 1
+    return meshes.hex_database(nx, ny, nz, 100.0, mat, [2500.0, 2700.0], [3000.0, 3600.0], [1730.0, 2000.0],
+                               absorbing=absorbing, perturb=perturb, seed=seed, dt=dt)
+
+
+def test_database_round_trip(tmp_path):
+    db = _db(5, 4, 4, perturb=0.1)
+    assert 0 < len(db.irregular_element_number) < db.nspec  # both kinds of element
+    P, D = str(tmp_path / "mesh_parameters.bin"), str(tmp_path / "database.bin")
+    d3.write_database_3d(P, D, db)
+    r = d3.read_database_3d(P, D)
+    assert r.params == db.params
+    for name in ("ibool", "x", "y", "z", "irregular_element_number", "kappa", "mu", "rho", "rho_vp", "rho_vs",
+                 "ispec_type", "mass_elastic", "ispec_is_inner", "ispec_is_surface", "iglob_is_surface",
+                 "neighbors_xadj", "neighbors_adjncy"):
+        assert np.array_equal(getattr(r, name), getattr(db, name)), name
+    for k in db.derivs:
+        assert np.array_equal(r.derivs[k], db.derivs[k]), k
+    for f in ("ispec", "ijk", "jacobian2Dw", "normal"):
+        assert np.array_equal(getattr(r.absorbing, f), getattr(db.absorbing, f)), f
+        assert np.array_equal(getattr(r.free_surface, f), getattr(db.free_surface, f)), f
+    for s in db.ibelm:
+        assert np.array_equal(r.ibelm[s], db.ibelm[s]), s
+    assert all(np.array_equal(a, b) for a, b in zip(r.abs_mass_elastic, db.abs_mass_elastic))
+    assert r.xix_regular == db.xix_regular and r.jacobian_regular == db.jacobian_regular
+    ni, no, ph = r.inner_outer["elastic"]
+    assert (ni, no) == (db.nspec, 0) and np.array_equal(ph, db.inner_outer["elastic"][2])
+    # a corrupted check value is refused, as the reference reader refuses it
+    data = bytearray(open(D, "rb").read())
+    marker = (10000).to_bytes(4, "little")
+    pos = data.find(b"\x04\x00\x00\x00" + marker + b"\x04\x00\x00\x00")
+    data[pos + 4:pos + 8] = (10001).to_bytes(4, "little")
+    open(D, "wb").write(bytes(data))
+    try:
+        d3.read_database_3d(P, D)
+        raise AssertionError("corrupted database read")
+    except ValueError as e:
+        assert "test value (10000)" in str(e)
+
+
+def test_database_cube_equals_structured_oracle():
+    n, h, dt, nstep = 3, 100.0, 1e-3, 120
+    mesh = meshes.hex_mesh(n, n, n, h, np.zeros((n, n, n), np.int32), [2500.0], [3000.0], [1730.0])
+    db = meshes.hex_database(n, n, n, h, np.zeros((n, n, n), np.int32), [2500.0], [3000.0], [1730.0],
+                             absorbing=mesh["absorbing"], perturb=0.0, dt=dt)
+    assert len(db.irregular_element_number) == 0
+    src = [dict(type="moment_tensor", x=140.0, y=160.0, z=130.0, mxx=0.3, myy=-0.5, mzz=0.8, mxy=0.2, mxz=-0.1,
+                myz=0.4, f0=25.0, factor=1e9)]
+    rec = [dict(x=40.0, y=60.0, z=290.0), dict(x=260.0, y=220.0, z=100.0), dict(x=150.0, y=150.0, z=150.0)]
+    types = ("displacement", "velocity")
+    s_struct, t0a = oracle.Oracle3D(mesh, 8, dt).forward(src, rec, nstep, dt, 1, types)
+    o = oracle.Oracle3D.from_database(db, 8, dt)
+    s_db, t0b = o.forward(src, rec, nstep, dt, 1, types)
+    assert t0a == t0b
+    err = np.linalg.norm(s_db - s_struct) / np.linalg.norm(s_struct)
+    # the database holds float32 geometry and coordinates: 1e-6 is FP32 input rounding
+    assert err < 1e-6, err
+    np.testing.assert_allclose(o.mass_inverse(), oracle.Oracle3D(mesh, 8, dt).mass_inverse(), rtol=1e-6)
+
+
+def test_irregular_database_conserves_energy():
+    dt = 5e-4
+    db = _db(3, 3, 3, perturb=0.15, absorbing=(0,) * 6, dt=dt)
+    assert len(db.irregular_element_number) > 0
+    o = oracle.Oracle3D.from_database(db, 8, dt)
+    src = [dict(type="force", x=140.0, y=160.0, z=130.0, fx=0.3, fy=-0.5, fz=1.0, f0=30.0, factor=1e9)]
+    rec = [dict(x=50.0, y=50.0, z=50.0)]
+    minv = o.mass_inverse()
+
+    def energy(nstep):
+        _, _, f = o.forward(src, rec, nstep, dt, 1, return_fields=True)
+        u, v, a = f
+        vh = v + 0.5 * dt * a
+        u1 = u + dt * vh
+        kin = 0.5 * np.sum(vh * vh / minv)
+        pot = 0.5 * np.sum(u1 * -o.apply_stiffness(u))
+        return kin + pot
+
+    e1, e2 = energy(400), energy(700)
+    assert abs(e2 - e1) / e1 < 1e-9, (e1, e2)
+
+
+def test_database_mass_matches_its_own_arrays():
+    dt = 1e-3
+    db = _db(4, 3, 3, perturb=0.12, dt=dt)
+    o = oracle.Oracle3D.from_database(db, 8, dt)
+    M = 1.0 / o.mass_inverse()
+    # rmass plus the Stacey terms, both in the database (float32)
+    ref = np.stack([db.mass_elastic.astype(np.float64) + m.astype(np.float64) for m in db.abs_mass_elastic])
+    # the oracle's mass uses the float32 geometry; the database arrays were
+    # computed from the double geometry and stored as float32
+    np.testing.assert_allclose(M, ref, rtol=2e-6)
+
+
+def test_host_assembly_numbering_and_locations():
+    from specfempp_amd import Engine3DDatabase
+
+    db = _db(5, 4, 3, perturb=0.12)
+    src = [dict(type="force", x=230.0, y=170.0, z=140.0, fx=0.3, fy=-0.5, fz=0.8, f0=12.0, factor=1e10)]
+    rec = [dict(x=100.0, y=100.0, z=300.0), dict(x=333.3, y=222.2, z=111.1), dict(x=450.0, y=350.0, z=250.0)]
+    eng = Engine3DDatabase(db, src, rec, 1e-3, 10, host_only=True)
+    assert np.array_equal(eng.index_mapping(), db.ibool)
+    assert eng.nglob == db.nglob and eng.nspec == db.nspec
+    o = oracle.Oracle3D.from_database(db, 8, 1e-3)
+    xn = np.array([-1.0, -np.sqrt(3 / 7), 0.0, np.sqrt(3 / 7), 1.0])
+
+    def lag(x):
+        return np.array([np.prod([(x - xn[i]) / (xn[d] - xn[i]) for i in range(5) if i != d]) for d in range(5)])
+
+    for which, pts in (("sources", src), ("receivers", rec)):
+        ispec, loc = eng.locations(which)
+        for k, p in enumerate(pts):
+            w = np.einsum("k,j,i->kji", lag(loc[k, 2]), lag(loc[k, 1]), lag(loc[k, 0]))
+            g = db.ibool[ispec[k]]
+            X = np.array([np.sum(w * db.x[g]), np.sum(w * db.y[g]), np.sum(w * db.z[g])])
+            assert np.abs(X - [p["x"], p["y"], p["z"]]).max() < 1e-4 * 100.0
+    eng.close()

@@ -23,6 +23,15 @@ from conftest import GOLDEN, load_case, rel_l1_per_station, rel_l2
 
 pytestmark = pytest.mark.gpu
 
+# The 3-D block path is switched off on the device.  The cause of its one GPU
+# memory fault was found on the host build and fixed (an over-read of the
+# kernel's quadrature-row argument by its idle threads; the ASan + bounds build
+# reports it before the fix and is clean after it, at every shape these tests
+# use), but the fixed kernel has not been run on a GPU, so these tests stay
+# skipped.
+BLOCK_OFF = pytest.mark.skip(reason="3-D block path switched off on the device: fault cause fixed on the host "
+                                    "build, the fixed kernel not yet run on a GPU")
+
 TOL_SEISMO = 1e-5
 
 
@@ -172,8 +181,8 @@ def test_config1_homogeneous_80x80(scatter):
 
 # The Ricker wavelet (f0 = 5 Hz) peaks at step 240 (t0 = -1.2 / f0); windows
 # reach well past it so the traces are signal, not rounding noise.
-@pytest.mark.parametrize("n,nstep,scatter", [(6, 600, "block"),
-                                            (10, 500, "block"), (6, 600, "gather"),
+@pytest.mark.parametrize("n,nstep,scatter", [pytest.param(6, 600, "block", marks=BLOCK_OFF),
+                                            pytest.param(10, 500, "block", marks=BLOCK_OFF), (6, 600, "gather"),
                                             (10, 500, "gather"), (6, 600, "colored"), (6, 600, "atomic")])
 def test_3d_layered_matches_oracle(n, nstep, scatter):
     from specfempp_amd import Engine3D, meshes
@@ -199,7 +208,7 @@ def test_3d_layered_matches_oracle(n, nstep, scatter):
             assert e32 <= TOL_SEISMO
 
 
-@pytest.mark.parametrize("n,scatter", [(5, "gather"), (6, "block")])
+@pytest.mark.parametrize("n,scatter", [(5, "gather"), pytest.param(6, "block", marks=BLOCK_OFF)])
 def test_3d_seam_path_and_fields_match_fused_path(n, scatter):
     """Seam calls (predictor / update_wavefields / corrector / seismograms) on
     the 3-D gather and block paths equal the fused step, and the wavefield read
@@ -260,6 +269,7 @@ def _box_workload():
     return mesh, src, rec, 1e-3, 400
 
 
+@BLOCK_OFF
 def test_3d_block_box_all_absorbing_matches_oracle():
     from specfempp_amd import Engine3D
 
@@ -287,6 +297,7 @@ def test_3d_block_box_all_absorbing_matches_oracle():
             assert e32 <= TOL_SEISMO
 
 
+@BLOCK_OFF
 def test_3d_block_step_calls_equal_run_bitwise():
     """Stepping one call per step stores the full acceleration after every
     step; the displacement and velocity updates are the same instructions, so

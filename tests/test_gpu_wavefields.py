@@ -21,7 +21,7 @@ import pytest
 
 import oracle
 from conftest import load_case, rel_l2
-from test_gpu_parity import _check, _report
+from test_gpu_parity import BLOCK_OFF, _check, _report
 
 pytestmark = pytest.mark.gpu
 
@@ -77,7 +77,7 @@ def test_2d_wavefield_snapshots_match_oracle(scatter):
         compare(f"2d_{scatter}_step{k}", ours, ref[0], ref[1])
 
 
-@pytest.mark.parametrize("scatter", ["gather", "colored", "atomic", "block"])
+@pytest.mark.parametrize("scatter", ["gather", "colored", "atomic", pytest.param("block", marks=BLOCK_OFF)])
 def test_3d_wavefield_snapshots_match_oracle(scatter):
     from specfempp_amd import Engine3D, meshes
 

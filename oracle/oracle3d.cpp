@@ -394,6 +394,11 @@ template <class T> void locate_general(const Asm<T> &a, double x, double y, doub
   };
   double final_best = std::numeric_limits<double>::max();
   ispec = -1;
+  struct Cand {
+    int e;
+    double d, r[3];
+  };
+  std::vector<Cand> cand;
   for (int e = 0; e < a.nspec; e++) {
     int pbest = -1;
     for (int p = 0; p < 125 && pbest < 0; p++)
@@ -422,13 +427,29 @@ template <class T> void locate_general(const Asm<T> &a, double x, double y, doub
         r[s2] = std::min(1.01, std::max(-1.01, r[s2] + inv[s2][0] * res[0] + inv[s2][1] * res[1] + inv[s2][2] * res[2]));
     }
     map(e, r, X, D);
-    const double d2 = (x - X[0]) * (x - X[0]) + (y - X[1]) * (y - X[1]) + (z - X[2]) * (z - X[2]);
-    if (d2 < final_best) {
-      final_best = d2;
-      ispec = e;
-      for (int s2 = 0; s2 < 3; s2++) r_out[s2] = r[s2];
-    }
+    const double d = std::sqrt((x - X[0]) * (x - X[0]) + (y - X[1]) * (y - X[1]) + (z - X[2]) * (z - X[2]));
+    cand.push_back({e, d, {r[0], r[1], r[2]}});
+    final_best = std::min(final_best, d);
   }
+  // ties (a point on a shared face, edge or corner: distances equal up to
+  // rounding, within 1e-9 of the mesh's bounding-box diagonal) go to the
+  // highest-numbered element, as in the structured location rule below
+  double lo[3] = {a.px[0], a.py[0], a.pz[0]}, hi[3] = {a.px[0], a.py[0], a.pz[0]};
+  for (int g = 0; g < a.nglob; g++) {
+    lo[0] = std::min(lo[0], static_cast<double>(a.px[g]));
+    lo[1] = std::min(lo[1], static_cast<double>(a.py[g]));
+    lo[2] = std::min(lo[2], static_cast<double>(a.pz[g]));
+    hi[0] = std::max(hi[0], static_cast<double>(a.px[g]));
+    hi[1] = std::max(hi[1], static_cast<double>(a.py[g]));
+    hi[2] = std::max(hi[2], static_cast<double>(a.pz[g]));
+  }
+  const double tie = 1e-9 * std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
+                                      (hi[2] - lo[2]) * (hi[2] - lo[2]));
+  for (const auto &c : cand)
+    if (c.d <= final_best + tie && c.e > ispec) {
+      ispec = c.e;
+      for (int s2 = 0; s2 < 3; s2++) r_out[s2] = c.r[s2];
+    }
   if (ispec < 0) throw std::runtime_error("locate: no element holds the nearest point");
 }
 

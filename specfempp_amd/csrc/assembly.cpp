@@ -1463,6 +1463,7 @@ struct Locator3D {
   const sfx_mesh3d_db &m;
   std::vector<std::vector<int32_t>> owners;  // global point -> elements holding it (ascending)
   double xi_nodes[5];
+  double tie = 0;  // final distances within this of the smallest are ties
   explicit Locator3D(const sfx_mesh3d_db &mesh, const float *nodes) : m(mesh), owners(mesh.nglob) {
     for (int e = 0; e < m.nspec; e++)
       for (int p = 0; p < 125; p++) {
@@ -1470,6 +1471,16 @@ struct Locator3D {
         if (o.empty() || o.back() != e) o.push_back(e);
       }
     for (int i = 0; i < 5; i++) xi_nodes[i] = nodes[i];
+    double lo[3] = {m.x[0], m.y[0], m.z[0]}, hi[3] = {m.x[0], m.y[0], m.z[0]};
+    for (int g = 0; g < m.nglob; g++) {
+      const double P[3] = {m.x[g], m.y[g], m.z[g]};
+      for (int c = 0; c < 3; c++) {
+        lo[c] = std::min(lo[c], P[c]);
+        hi[c] = std::max(hi[c], P[c]);
+      }
+    }
+    tie = 1e-9 * std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
+                           (hi[2] - lo[2]) * (hi[2] - lo[2]));
   }
   void locate(double x, double y, double z, int &ispec, double ref[3]) const {
     if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(z))
@@ -1487,6 +1498,11 @@ struct Locator3D {
     if (best < 0 || owners[best].empty()) throw std::runtime_error("locate: no element holds the nearest point");
     double final_best = std::numeric_limits<double>::max();
     ispec = -1;
+    struct Cand {
+      int e;
+      double d, r[3];
+    };
+    std::vector<Cand> cand;
     for (int e : owners[best]) {
       // start from the reference coordinates of the nearest point in e
       double r[3] = {0, 0, 0};
@@ -1549,12 +1565,19 @@ struct Locator3D {
             }
         d2 = (x - X[0]) * (x - X[0]) + (y - X[1]) * (y - X[1]) + (z - X[2]) * (z - X[2]);
       }
-      if (d2 < final_best) {
-        final_best = d2;
-        ispec = e;
-        for (int s = 0; s < 3; s++) ref[s] = r[s];
-      }
+      cand.push_back({e, std::sqrt(d2), {r[0], r[1], r[2]}});
+      final_best = std::min(final_best, std::sqrt(d2));
     }
+    // A point on a face, edge or corner shared by several elements is at the
+    // same distance from each of them up to rounding: among the elements
+    // within `tie` of the closest, the highest-numbered one holds it (on a
+    // structured lattice, the element above the shared face in every
+    // direction, as the structured mesh's location rule has it).
+    for (const auto &c : cand)
+      if (c.d <= final_best + tie && c.e > ispec) {
+        ispec = c.e;
+        for (int s = 0; s < 3; s++) ref[s] = c.r[s];
+      }
   }
 };
 

@@ -72,9 +72,15 @@ def test_database_cube_equals_structured_oracle():
     db = meshes.hex_database(n, n, n, h, np.zeros((n, n, n), np.int32), [2500.0], [3000.0], [1730.0],
                              absorbing=mesh["absorbing"], perturb=0.0, dt=dt)
     assert len(db.irregular_element_number) == 0
+    # the second source sits on an element edge (x = y = 200): four elements
+    # hold it, and the moment tensor's discrete form depends on which one;
+    # both location rules give it to the highest-numbered element
     src = [dict(type="moment_tensor", x=140.0, y=160.0, z=130.0, mxx=0.3, myy=-0.5, mzz=0.8, mxy=0.2, mxz=-0.1,
-                myz=0.4, f0=25.0, factor=1e9)]
-    rec = [dict(x=40.0, y=60.0, z=290.0), dict(x=260.0, y=220.0, z=100.0), dict(x=150.0, y=150.0, z=150.0)]
+                myz=0.4, f0=25.0, factor=1e9),
+           dict(type="moment_tensor", x=200.0, y=200.0, z=170.0, mxx=-0.2, myy=0.4, mzz=0.1, mxy=0.3, mxz=0.2,
+                myz=-0.5, f0=25.0, factor=1e9)]
+    rec = [dict(x=40.0, y=60.0, z=290.0), dict(x=260.0, y=220.0, z=100.0), dict(x=150.0, y=150.0, z=150.0),
+           dict(x=100.0, y=200.0, z=100.0)]
     types = ("displacement", "velocity")
     s_struct, t0a = oracle.Oracle3D(mesh, 8, dt).forward(src, rec, nstep, dt, 1, types)
     o = oracle.Oracle3D.from_database(db, 8, dt)
@@ -84,6 +90,39 @@ def test_database_cube_equals_structured_oracle():
     # the database holds float32 geometry and coordinates: 1e-6 is FP32 input rounding
     assert err < 1e-6, err
     np.testing.assert_allclose(o.mass_inverse(), oracle.Oracle3D(mesh, 8, dt).mass_inverse(), rtol=1e-6)
+
+
+def test_location_ties_go_to_the_highest_numbered_element():
+    """A point on a face, edge or corner shared by several elements: the
+    database locator and the structured one give it to the same element (the
+    highest-numbered of those holding it) at reference coordinates -1 on the
+    shared axes."""
+    from specfempp_amd import Engine3D, Engine3DDatabase
+
+    n, h = 4, 100.0
+    mat = np.zeros((n, n, n), np.int32)
+    mesh = meshes.hex_mesh(n, n, n, h, mat, [2500.0], [3000.0], [1730.0])
+    db = meshes.hex_database(n, n, n, h, mat, [2500.0], [3000.0], [1730.0], absorbing=mesh["absorbing"],
+                             perturb=0.0)
+    pts = [dict(x=200.0, y=150.0, z=150.0), dict(x=200.0, y=100.0, z=150.0), dict(x=100.0, y=300.0, z=200.0),
+           dict(x=400.0, y=0.0, z=250.0), dict(x=130.0, y=170.0, z=310.0)]
+    src = [dict(type="force", x=p["x"], y=p["y"], z=p["z"], fx=1.0, fy=0.0, fz=0.0, f0=10.0, factor=1.0) for p in pts]
+    a = Engine3D(mesh, src, pts, 1e-3, 4, host_only=True)
+    b = Engine3DDatabase(db, src, pts, 1e-3, 4, host_only=True)
+    for which in ("sources", "receivers"):
+        ea, ra = a.locations(which)
+        eb, rb = b.locations(which)
+        assert np.array_equal(ea, eb), (which, ea, eb)
+        np.testing.assert_allclose(rb, ra, atol=1e-6)
+    # known answers: (2, 1, 1) with xi = -1; (2, 1, 1) with xi = eta = -1;
+    # (1, 3, 2) with xi = eta = gamma = -1; the mesh's upper x face: (3, 0, 2)
+    # with xi = +1, eta = -1; an interior point
+    assert list(ea) == [(1 * n + 1) * n + 2, (1 * n + 1) * n + 2, (2 * n + 3) * n + 1, (2 * n + 0) * n + 3,
+                        (3 * n + 1) * n + 1]
+    np.testing.assert_allclose(ra[1], [-1.0, -1.0, 0.0], atol=1e-12)
+    np.testing.assert_allclose(ra[3], [1.0, -1.0, 0.0], atol=1e-12)
+    a.close()
+    b.close()
 
 
 def test_irregular_database_conserves_energy():

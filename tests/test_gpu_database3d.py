@@ -11,7 +11,11 @@ src/io/mesh/impl/fortran/dim3/mesh.cpp) through the C ABI.
 - The database read back from its files runs as the one written.
 - An undistorted cube written as a database gives the structured engine's
   seismograms (regular elements; Stacey from unit normals and jacobian2Dw
-  against the structured J grad xi form) within 1e-5.
+  against the structured J grad xi form; the moment tensor on an element edge,
+  located in the same element by both location rules): displacement within
+  1e-5, velocity and acceleration within twice the FP32 rounding floor (the
+  structured FP32 oracle's distance to the FP64 one; two FP32 runs that
+  round differently are each about that far from the exact result).
 """
 import numpy as np
 import pytest
@@ -101,7 +105,10 @@ def test_database_cube_equals_structured_engine():
         eng.run()
         runs.append(eng.seismograms().astype(np.float64))
         eng.close()
+    o32, _ = oracle.Oracle3D(mesh, 4, dt).forward(w.sources, w.receivers, nstep, dt, 1, TYPES)
+    o64, _ = oracle.Oracle3D(mesh, 8, dt).forward(w.sources, w.receivers, nstep, dt, 1, TYPES)
     for it, t in enumerate(TYPES):
         e = rel_l2(runs[1][:, it], runs[0][:, it])
-        _report(f"db3d_cube_vs_structured/{t}", rel_l2=e)
-        assert e <= (1e-5 if t == "displacement" else 5e-5)
+        floor = rel_l2(o32[:, it], o64[:, it])
+        _report(f"db3d_cube_vs_structured/{t}", rel_l2=e, fp32_floor=floor)
+        assert e <= (1e-5 if t == "displacement" else max(1e-5, 2.0 * floor))

@@ -419,25 +419,34 @@ int sfx_get_point_weights(const sfx_engine *engine, float *out);
  * The reference runs one simulation per process (time_marching<forward>::run,
  * time_marching.tpp:9-124).  A batch advances many independent 2-D
  * simulations (meshes of any size, own sources, receivers, dt and step count)
- * in the same launches: one launch per kernel family per step covers every
- * simulation still running, and runs of steps between two simulation ends
- * replay one captured HIP graph.  Engines are only read at creation (the batch
- * copies their data); they must record the same seismogram types.
- * sfx_batch_tune: times elements per workgroup (2, 5, 10) with and without
- *   graphs on the batch itself and keeps the fastest; choice3 = (elements per
- *   workgroup, graphs on, colors).  sfx_batch_run starts from a zero state.
+ * in the same launches.  Engines are only read at creation (the batch copies
+ * their data); they must record the same seismogram types.  Scatters:
+ *   SFX_SCATTER_GATHER: one launch per time step covers every simulation still
+ *     running (double-buffered state, no grid-wide synchronisation); each
+ *     simulation gives its engine's SFX_SCATTER_GATHER results bit for bit;
+ *   SFX_SCATTER_COLORED / _ATOMIC: one launch per kernel family per step
+ *     (sources, stiffness per color, Stacey, seismograms, update).
+ * Runs of steps between two simulation ends replay one captured HIP graph
+ * when graphs are on.
+ * sfx_batch_tune: times every scatter with 2, 5 and 10 elements per workgroup
+ *   (50, 125 or 250 threads, one per element point), with and without graphs,
+ *   on the batch itself and keeps the fastest; choice4 = (elements per
+ *   workgroup, graphs on, colors, scatter).  sfx_batch_run starts from a zero
+ *   state.
+ * sfx_batch_set_config: the same choice made by the caller.
  * sfx_batch_get_seismograms: simulation i (creation order), laid out as
  *   sfx_get_seismograms of its engine.
  * sfx_batch_get_info: last run's device time (ms), elements per workgroup,
- *   graphs on, colors, padded points, total point-steps. */
+ *   graphs on, colors, padded points, total point-steps, scatter, launches
+ *   per step. */
 typedef struct sfx_batch sfx_batch;
 int sfx_batch_create(int n, sfx_engine *const *engines, sfx_batch **out);
 void sfx_batch_destroy(sfx_batch *batch);
-int sfx_batch_tune(sfx_batch *batch, int steps_per_candidate, int *choice3);
-int sfx_batch_set_config(sfx_batch *batch, int elems_per_block, int use_graph);
+int sfx_batch_tune(sfx_batch *batch, int steps_per_candidate, int *choice4);
+int sfx_batch_set_config(sfx_batch *batch, int scatter, int elems_per_block, int use_graph);
 int sfx_batch_run(sfx_batch *batch);
 int sfx_batch_get_seismograms(const sfx_batch *batch, int i, float *out);
-int sfx_batch_get_info(const sfx_batch *batch, double *out6);
+int sfx_batch_get_info(const sfx_batch *batch, double *out8);
 
 /* ---- measurement -------------------------------------------------------------
  * Average device time (ms) of each kernel family over the last sfx_run, taken

@@ -128,7 +128,7 @@ def load():
         "sfx_batch_create": ([i32, vp, vp], i32),
         "sfx_batch_destroy": ([vp], None),
         "sfx_batch_tune": ([vp, i32, vp], i32),
-        "sfx_batch_set_config": ([vp, i32, i32], i32),
+        "sfx_batch_set_config": ([vp, i32, i32, i32], i32),
         "sfx_batch_run": ([vp], i32),
         "sfx_batch_get_seismograms": ([vp, i32, vp], i32),
         "sfx_batch_get_info": ([vp, vp], i32),

@@ -1780,6 +1780,15 @@ HostAssembly build_assembly_3d_db(const sfx_mesh3d_db &m, int nsrc, const sfx_so
 }
 
 void color_elements(HostAssembly &a, int scatter) {
+  ElementColoring c = element_coloring(a, scatter);
+  a.perm = std::move(c.perm);
+  a.inv_perm = std::move(c.inv_perm);
+  a.color_offsets = std::move(c.color_offsets);
+  a.ncolors_el = c.ncolors_el;
+}
+
+ElementColoring element_coloring(const HostAssembly &a, int scatter) {
+  ElementColoring out;
   const int nspec = a.nspec, npe = a.npe;
   std::vector<int> color(nspec, 0);
   int ncolors = 1;
@@ -1825,29 +1834,30 @@ void color_elements(HostAssembly &a, int scatter) {
       if (a.coupled() && base == 0 && a.nspec_el > 0) throw std::logic_error("internal: acoustic color base");
     }
   }
-  a.ncolors_el = ncolors;
+  out.ncolors_el = ncolors;
   if (a.coupled()) {
     if (scatter != SFX_SCATTER_COLORED) {  // one atomic launch per medium
       for (int e = 0; e < nspec; e++) color[e] = e >= a.nspec_el ? 1 : 0;
       ncolors = 2;
     }
-    a.ncolors_el = 0;
-    for (int e = 0; e < a.nspec_el; e++) a.ncolors_el = std::max(a.ncolors_el, color[e] + 1);
+    out.ncolors_el = 0;
+    for (int e = 0; e < a.nspec_el; e++) out.ncolors_el = std::max(out.ncolors_el, color[e] + 1);
     for (int e = a.nspec_el; e < nspec; e++)
-      if (color[e] < a.ncolors_el) throw std::logic_error("internal: acoustic element in an elastic color");
+      if (color[e] < out.ncolors_el) throw std::logic_error("internal: acoustic element in an elastic color");
   }
   // stable counting sort by color
-  a.color_offsets.assign(ncolors + 1, 0);
-  for (int e = 0; e < nspec; e++) a.color_offsets[color[e] + 1]++;
-  for (int c = 0; c < ncolors; c++) a.color_offsets[c + 1] += a.color_offsets[c];
-  std::vector<int> pos(a.color_offsets.begin(), a.color_offsets.end() - 1);
-  a.perm.resize(nspec);
-  a.inv_perm.resize(nspec);
+  out.color_offsets.assign(ncolors + 1, 0);
+  for (int e = 0; e < nspec; e++) out.color_offsets[color[e] + 1]++;
+  for (int c = 0; c < ncolors; c++) out.color_offsets[c + 1] += out.color_offsets[c];
+  std::vector<int> pos(out.color_offsets.begin(), out.color_offsets.end() - 1);
+  out.perm.resize(nspec);
+  out.inv_perm.resize(nspec);
   for (int e = 0; e < nspec; e++) {
     const int s = pos[color[e]]++;
-    a.perm[s] = e;
-    a.inv_perm[e] = s;
+    out.perm[s] = e;
+    out.inv_perm[e] = s;
   }
+  return out;
 }
 
 }  // namespace sfx

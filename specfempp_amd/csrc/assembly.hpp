@@ -121,6 +121,12 @@ struct HostAssembly {
 // and fill perm / inv_perm / color_offsets.  scatter = SFX_SCATTER_ATOMIC keeps
 // compute order in a single color.
 void color_elements(HostAssembly &a, int scatter);
+// The same coloring, returned instead of stored.
+struct ElementColoring {
+  std::vector<int32_t> perm, inv_perm, color_offsets;
+  int ncolors_el = 0;
+};
+ElementColoring element_coloring(const HostAssembly &a, int scatter);
 
 // Replace the per-point properties ((nspec, 25), mesh element order) and
 // rebuild the Stacey impedances and the mass matrix (2-D).

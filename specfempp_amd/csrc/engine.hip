@@ -1151,7 +1151,7 @@ int sfx_predictor(sfx_engine *e) {
     check_device(e);
     e->require_init();
     e->refresh_acceleration();
-    launch_predictor(e->stream, e->h.ncomp, e->f(), e->nc);
+    launch_predictor(e->stream, e->h.ncomp, e->f(), e->nc, e->gather2d());
     SFX_HIP(hipGetLastError());
     dofs = e->h.nglob_el * e->h.ncomp;
   }, stream_of(e));
@@ -1176,7 +1176,7 @@ int sfx_corrector(sfx_engine *e) {
     check_device(e);
     e->require_init();
     e->refresh_acceleration();
-    launch_divide_corrector(e->stream, e->h.ncomp, e->f(), e->nc, 0, 1);
+    launch_divide_corrector(e->stream, e->h.ncomp, e->f(), e->nc, 0, 1, e->gather2d());
     SFX_HIP(hipGetLastError());
     dofs = e->h.nglob_el * e->h.ncomp;
   }, stream_of(e));
@@ -1223,7 +1223,7 @@ int sfx_predictor_phase(sfx_engine *e, int medium, int wavefield, int direction)
     int ncomp = 0;
     Fields F = e->medium_wavefield(medium, wavefield, ncomp);
     if (wavefield == SFX_WAVEFIELD_FORWARD) e->refresh_acceleration();
-    launch_predictor(e->stream, ncomp, F, k);
+    launch_predictor(e->stream, ncomp, F, k, e->gather2d());
     SFX_HIP(hipGetLastError());
   }, stream_of(e));
   return rc == SFX_OK ? dofs : rc;
@@ -1241,7 +1241,7 @@ int sfx_corrector_phase(sfx_engine *e, int medium, int wavefield, int direction)
     int ncomp = 0;
     Fields F = e->medium_wavefield(medium, wavefield, ncomp);
     if (wavefield == SFX_WAVEFIELD_FORWARD) e->refresh_acceleration();
-    launch_divide_corrector(e->stream, ncomp, F, k, 0, 1);
+    launch_divide_corrector(e->stream, ncomp, F, k, 0, 1, e->gather2d());
     SFX_HIP(hipGetLastError());
   }, stream_of(e));
   return rc == SFX_OK ? dofs : rc;

@@ -174,6 +174,9 @@ __global__ __launch_bounds__(128) void elemforce_2d_kernel(
     const float *__restrict__ gammax_, const float *__restrict__ gammaz_, const float *__restrict__ jac_,
     const float *__restrict__ kappa_, const float *__restrict__ mu_, GllConst q, const float *__restrict__ uf,
     int np, const float *__restrict__ geo, float *__restrict__ re2) {
+  // every product and sum rounded on its own: the one-launch batch step
+  // performs the same operations and must round them the same way
+#pragma clang fp contract(off)
   __shared__ float su[E2][2][25];
   __shared__ float sF[E2][4][25];  // F(c, d) as [2c + d]
   __shared__ __align__(16) float st[E2 * RE2_ELEM];
@@ -277,6 +280,7 @@ __global__ __launch_bounds__(256) void stacey_re2_kernel(int nseg, const int *__
                                                          const float *__restrict__ rho_vp,
                                                          const float *__restrict__ rho_vs, Fields f,
                                                          float *__restrict__ re2) {
+#pragma clang fp contract(off)
   const int sidx = blockIdx.x * blockDim.x + threadIdx.x;
   if (sidx >= nseg) return;
   const int q0 = seg[sidx], q1 = seg[sidx + 1];
@@ -309,6 +313,7 @@ __global__ __launch_bounds__(64) void sources_re2_kernel(int nsrc, int istep_arg
                                                          const float *__restrict__ array,
                                                          const float *__restrict__ stf, float *__restrict__ re2,
                                                          const int *__restrict__ step) {
+#pragma clang fp contract(off)
   const int p = threadIdx.x;
   if (p >= 25) return;
   const int istep = step ? *step : istep_arg;
@@ -327,6 +332,7 @@ __global__ __launch_bounds__(64) void sources_re2_kernel(int nsrc, int istep_arg
 // the step's acceleration.
 __global__ __launch_bounds__(256) void assemble_update_2d_kernel(int nglob, ForceSum2D fs, Fields f,
                                                                  NewmarkConst nc, int divide_only) {
+#pragma clang fp contract(off)
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nglob) return;
   float sum[2] = {0.0f, 0.0f};
@@ -897,7 +903,9 @@ __global__ __launch_bounds__(128) void seismogram_kernel(int npe, int nrec, int 
           if (corrected) acc = acc * f.minv[c * np + g];
           if (type == 1) {
             x = f.v[c * np + g];
-            if (corrected) x = x + dto2 * acc;
+            // the gather path rounds v + dt/2 a as its update does (no fused
+            // multiply-add)
+            if (corrected) x = fs.re2 ? plus_product_rounded(x, dto2, acc) : x + dto2 * acc;
           } else {
             x = acc;
           }
@@ -953,19 +961,29 @@ __global__ __launch_bounds__(256) void update_fused_kernel(int n4, int ncomp, Fi
 }
 
 // Predictor alone (newmark.tpp:60-116).
+// ROUNDED: every product and sum rounded on its own (no fused multiply-add),
+// as the 2-D gather path's fused update forms them.
+template <bool ROUNDED>
 __global__ __launch_bounds__(256) void predictor_kernel(int n, int ncomp, Fields f, NewmarkConst nc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   for (int c = 0; c < ncomp; c++) {
     const size_t o = static_cast<size_t>(c) * f.nglob_pad + i;
     const float a = f.a[o], v = f.v[o];
-    f.u[o] = f.u[o] + (nc.dt * v + nc.dt2o2 * a);
-    f.v[o] = v + nc.dto2 * a;
+    if constexpr (ROUNDED) {
+#pragma clang fp contract(off)
+      f.u[o] = f.u[o] + (nc.dt * v + nc.dt2o2 * a);
+      f.v[o] = v + nc.dto2 * a;
+    } else {
+      f.u[o] = f.u[o] + (nc.dt * v + nc.dt2o2 * a);
+      f.v[o] = v + nc.dto2 * a;
+    }
     f.a[o] = 0.0f;
   }
 }
 
 // Mass division (divide_mass_matrix.tpp:42-49) and/or corrector (newmark.tpp:11-56).
+template <bool ROUNDED>
 __global__ __launch_bounds__(256) void divide_corrector_kernel(int n, int ncomp, Fields f, NewmarkConst nc,
                                                                int divide, int correct) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -977,7 +995,14 @@ __global__ __launch_bounds__(256) void divide_corrector_kernel(int n, int ncomp,
       a = a * f.minv[o];
       f.a[o] = a;
     }
-    if (correct) f.v[o] = f.v[o] + nc.dto2 * a;
+    if (correct) {
+      if constexpr (ROUNDED) {
+#pragma clang fp contract(off)
+        f.v[o] = f.v[o] + nc.dto2 * a;
+      } else {
+        f.v[o] = f.v[o] + nc.dto2 * a;
+      }
+    }
   }
 }
 
@@ -1619,13 +1644,23 @@ void launch_update_fused(hipStream_t s, int ncomp, Fields f, NewmarkConst nc) {
   hipLaunchKernelGGL(update_fused_kernel, dim3(blocks(n4, 256)), dim3(256), 0, s, n4, ncomp, f, nc);
 }
 
-void launch_predictor(hipStream_t s, int ncomp, Fields f, NewmarkConst nc) {
-  hipLaunchKernelGGL(predictor_kernel, dim3(blocks(f.nglob_pad, 256)), dim3(256), 0, s, f.nglob_pad, ncomp, f, nc);
+void launch_predictor(hipStream_t s, int ncomp, Fields f, NewmarkConst nc, bool rounded) {
+  if (rounded)
+    hipLaunchKernelGGL(predictor_kernel<true>, dim3(blocks(f.nglob_pad, 256)), dim3(256), 0, s, f.nglob_pad, ncomp,
+                       f, nc);
+  else
+    hipLaunchKernelGGL(predictor_kernel<false>, dim3(blocks(f.nglob_pad, 256)), dim3(256), 0, s, f.nglob_pad, ncomp,
+                       f, nc);
 }
 
-void launch_divide_corrector(hipStream_t s, int ncomp, Fields f, NewmarkConst nc, int divide, int correct) {
-  hipLaunchKernelGGL(divide_corrector_kernel, dim3(blocks(f.nglob_pad, 256)), dim3(256), 0, s, f.nglob_pad, ncomp, f,
-                     nc, divide, correct);
+void launch_divide_corrector(hipStream_t s, int ncomp, Fields f, NewmarkConst nc, int divide, int correct,
+                             bool rounded) {
+  if (rounded)
+    hipLaunchKernelGGL(divide_corrector_kernel<true>, dim3(blocks(f.nglob_pad, 256)), dim3(256), 0, s, f.nglob_pad,
+                       ncomp, f, nc, divide, correct);
+  else
+    hipLaunchKernelGGL(divide_corrector_kernel<false>, dim3(blocks(f.nglob_pad, 256)), dim3(256), 0, s, f.nglob_pad,
+                       ncomp, f, nc, divide, correct);
 }
 
 void launch_fill(hipStream_t s, float *p, size_t n, float value) {

@@ -28,6 +28,13 @@ struct Gll3 {
 };
 Gll3 scaled_gll3(const GllConst &q, float xix, float ety, float gaz, float jac);
 
+// x + s * y with the product rounded before the sum (never one fused
+// multiply-add, whatever contraction the caller's code allows).
+__device__ __forceinline__ float plus_product_rounded(float x, float s, float y) {
+#pragma clang fp contract(off)
+  return x + s * y;
+}
+
 struct Fields {
   float *u, *v, *a;
   const float *minv;
@@ -199,8 +206,11 @@ void launch_sources_re2(hipStream_t s, int nsrc, int istep, const int *order, co
                         const float *stf, float *re2, const int *step = nullptr);
 void launch_assemble_update_2d(hipStream_t s, int nglob, ForceSum2D fs, Fields f, NewmarkConst nc, int divide_only);
 void launch_update_fused(hipStream_t s, int ncomp, Fields f, NewmarkConst nc);
-void launch_predictor(hipStream_t s, int ncomp, Fields f, NewmarkConst nc);
-void launch_divide_corrector(hipStream_t s, int ncomp, Fields f, NewmarkConst nc, int divide, int correct);
+// rounded: every product and sum rounded on its own (no fused multiply-add),
+// as the 2-D gather path's fused update rounds them; its seams pass true.
+void launch_predictor(hipStream_t s, int ncomp, Fields f, NewmarkConst nc, bool rounded = false);
+void launch_divide_corrector(hipStream_t s, int ncomp, Fields f, NewmarkConst nc, int divide, int correct,
+                             bool rounded = false);
 void launch_fill(hipStream_t s, float *p, size_t n, float value);
 
 // ---- 3-D general hexahedra (general3d.hip) ----

@@ -527,13 +527,22 @@ class Batch:
             raise SfxError(self.lib.sfx_last_error().decode())
         return rc
 
-    def tune(self, steps_per_candidate=10):
-        out = (C.c_int * 3)()
-        self._check(self.lib.sfx_batch_tune(self.h, steps_per_candidate, out))
-        return {"elems_per_block": out[0], "graphs": bool(out[1]), "colors": out[2]}
+    _SCATTERS = {"gather": 2, "colored": 0, "atomic": 1}
 
-    def set_config(self, elems_per_block=5, graphs=True):
-        self._check(self.lib.sfx_batch_set_config(self.h, elems_per_block, int(graphs)))
+    def tune(self, steps_per_candidate=10):
+        """Time every scatter, elements per workgroup and graph use on this
+        batch and keep the fastest."""
+        out = (C.c_int * 4)()
+        self._check(self.lib.sfx_batch_tune(self.h, steps_per_candidate, out))
+        names = {v: k for k, v in self._SCATTERS.items()}
+        return {"elems_per_block": out[0], "graphs": bool(out[1]), "colors": out[2], "scatter": names[out[3]]}
+
+    def set_config(self, elems_per_block=5, graphs=True, scatter="gather"):
+        """scatter "gather": one launch per time step; "colored" / "atomic": one
+        launch per kernel family per step."""
+        if scatter not in self._SCATTERS:
+            raise ValueError(f"unknown batch scatter {scatter!r}")
+        self._check(self.lib.sfx_batch_set_config(self.h, self._SCATTERS[scatter], elems_per_block, int(graphs)))
 
     def run(self):
         self._check(self.lib.sfx_batch_run(self.h))
@@ -545,10 +554,12 @@ class Batch:
         return out
 
     def info(self):
-        out = (C.c_double * 6)()
+        out = (C.c_double * 8)()
         self._check(self.lib.sfx_batch_get_info(self.h, out))
+        names = {v: k for k, v in self._SCATTERS.items()}
         return {"run_ms": out[0], "elems_per_block": int(out[1]), "graphs": bool(out[2]), "colors": int(out[3]),
-                "padded_points": int(out[4]), "point_steps": out[5]}
+                "padded_points": int(out[4]), "point_steps": out[5], "scatter": names[int(out[6])],
+                "launches_per_step": int(out[7])}
 
     def close(self):
         if getattr(self, "h", None):

@@ -169,21 +169,66 @@ for chunk in (2, 64):
     eng.close()
     check(f"chunk {chunk}", ok and e <= 1e-5, f"{e:.2e}")
 
-# a small batch of unequal meshes and step counts
-engs = [Engine2D(mesh2d(n), SRC, REC, DT, ns, 1, ("displacement",), host_only=True) for n, ns in ((4, 5), (6, 8))]
+# a small batch of unequal meshes and step counts, every scatter: the
+# one-launch-per-step scatter against the engines' gather path bit for bit,
+# the colored one against their colored path, the atomic one within 1e-6
+SIZES = ((4, 5), (6, 8), (3, 8))
+single = {}
+for scatter in ("gather", "colored"):
+    single[scatter] = []
+    for n, ns in SIZES:
+        e = Engine2D(mesh2d(n), SRC, REC, DT, ns, 1, TYPES, scatter=scatter)
+        e.initialize()
+        e.run()
+        single[scatter].append(e.seismograms())
+        e.close()
+for scatter, epb, graphs in (("gather", 2, True), ("gather", 5, False), ("gather", 10, True), ("gather", 5, True),
+                             ("colored", 2, True), ("atomic", 5, False)):
+    engs = [Engine2D(mesh2d(n), SRC, REC, DT, ns, 1, TYPES, host_only=True) for n, ns in SIZES]
+    b = Batch(engs)
+    for e in engs:
+        e.close()
+    b.set_config(epb, graphs, scatter)
+    b.run()
+    ref = single["colored" if scatter == "atomic" else scatter]
+    got = [b.seismograms(i) for i in range(len(SIZES))]
+    if scatter == "atomic":
+        ok = all(rel_l2(got[i].astype(np.float64), ref[i].astype(np.float64)) <= 1e-6 for i in range(len(SIZES)))
+    else:
+        ok = all(np.array_equal(got[i], ref[i]) for i in range(len(SIZES)))
+    info = b.info()
+    b.close()
+    check(f"batch {scatter} E={epb} graphs={graphs} vs single runs", ok and info["scatter"] == scatter,
+          f"launches/step {info['launches_per_step']}")
+
+# a colored batch of engines built with the atomic scatter (one "color" in
+# compute order): the batch colors the meshes itself
+engs = [Engine2D(mesh2d(n), SRC, REC, DT, ns, 1, TYPES, scatter="atomic", host_only=True) for n, ns in SIZES]
 b = Batch(engs)
-b.set_config(2, True)
+for e in engs:
+    e.close()
+b.set_config(5, False, "colored")
 b.run()
-single = []
-for n, ns in ((4, 5), (6, 8)):
-    e = Engine2D(mesh2d(n), SRC, REC, DT, ns, 1, ("displacement",))
+check("colored batch of atomic-scatter engines", all(np.array_equal(b.seismograms(i), single["colored"][i])
+                                                     for i in range(len(SIZES))))
+b.close()
+
+# samples every third step, simulations ending at different steps
+ref3 = []
+for n, ns in SIZES:
+    e = Engine2D(mesh2d(n), SRC, REC, DT, ns, 3, TYPES, scatter="gather")
     e.initialize()
     e.run()
-    single.append(e.seismograms())
+    ref3.append(e.seismograms())
     e.close()
-ok = all(np.array_equal(b.seismograms(i), single[i]) for i in range(2))
+engs = [Engine2D(mesh2d(n), SRC, REC, DT, ns, 3, TYPES, host_only=True) for n, ns in SIZES]
+b = Batch(engs)
+for e in engs:
+    e.close()
+b.set_config(5, True, "gather")
+b.run()
+check("batch gather, every third step sampled", all(np.array_equal(b.seismograms(i), ref3[i]) for i in range(len(SIZES))))
 b.close()
-check("batch vs single runs", ok)
 
 print("failures:", failures, flush=True)
 sys.exit(1 if failures else 0)

@@ -1,10 +1,17 @@
 """GPU parity of batched 2-D runs.
 
 Each simulation of a batch must give the seismograms of its own engine run
-alone (same kernels and coloring; only the order of float atomics on Stacey
-and source points may differ, so the bar is relative L2 <= 1e-6) and those of
-the FP32 oracle (<= 1e-5, the bar of every forward parity test), whatever
-launch geometry the autotuner picks, with and without graph capture.
+alone and those of the FP32 oracle (<= 1e-5, the bar of every forward parity
+test), whatever scatter, launch geometry and graph use the autotuner picks:
+- the one-launch-per-step scatter ("gather") performs its engine's gather-path
+  operations in the same order with the same rounding of each product and
+  sum (no contraction into fused multiply-adds in either), so its
+  seismograms equal those of the engine run alone with scatter="gather" bit
+  for bit;
+- the colored and atomic scatters use the engine's colored kernels; only the
+  order of float atomics on source points (and, atomic, on every shared
+  point) may differ, so the bar is relative L2 <= 1e-6 against the engine's
+  colored run.
 """
 import numpy as np
 import pytest
@@ -22,44 +29,84 @@ def _workloads():
     return meshes.config4(nsim=6, seed=11, nmin=8, nmax=24, nstep_range=(150, 400), nrec=4)
 
 
-def _engines(ws):
+def _engines(ws, scatter="colored", types=TYPES):
     from specfempp_amd import Engine2D
 
-    return [Engine2D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, 1, TYPES) for w in ws]
+    return [Engine2D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, 1, types, scatter=scatter) for w in ws]
 
 
 @pytest.fixture(scope="module")
 def alone():
     ws = _workloads()
-    out = []
-    for e in _engines(ws):
-        e.initialize()
-        e.run()
-        out.append(e.seismograms().astype(np.float64))
-        e.close()
+    out = {}
+    for scatter in ("colored", "gather"):
+        out[scatter] = []
+        for e in _engines(ws, scatter):
+            e.initialize()
+            e.run()
+            out[scatter].append(e.seismograms().astype(np.float64))
+            e.close()
     return ws, out
 
 
-@pytest.mark.parametrize("elems_per_block,graphs", [(5, True), (2, False), (10, True)])
-def test_batch_matches_single_runs(alone, elems_per_block, graphs):
+@pytest.mark.parametrize("scatter,elems_per_block,graphs",
+                         [("gather", 5, True), ("gather", 2, False), ("gather", 10, True), ("colored", 5, True),
+                          ("colored", 2, False), ("colored", 10, True), ("atomic", 5, True)])
+def test_batch_matches_single_runs(alone, scatter, elems_per_block, graphs):
     from specfempp_amd import Batch
 
-    ws, ref = alone
+    ws, ref_all = alone
+    ref = ref_all["gather" if scatter == "gather" else "colored"]
     engines = _engines(ws)
     b = Batch(engines)
     for e in engines:
         e.close()  # the batch holds its own copy
-    b.set_config(elems_per_block, graphs)
+    b.set_config(elems_per_block, graphs, scatter)
+    print(f"[case] batch of {len(ws)}: scatter={scatter} E={elems_per_block} graphs={graphs}", flush=True)
     b.run()
+    info = b.info()
+    assert info["scatter"] == scatter
+    assert info["launches_per_step"] == (1 if scatter == "gather" else info["colors"] + 5)
     for i, w in enumerate(ws):
         got = b.seismograms(i).astype(np.float64)
         assert got.shape == ref[i].shape
         err = rel_l2(got, ref[i])
-        print(f"[batch] E={elems_per_block} graphs={graphs} sim {i} ({w.name}, {w.nstep} steps): {err:.2e}",
-              flush=True)
-        assert err <= 1e-6
-    info = b.info()
+        print(f"[batch] {scatter} E={elems_per_block} graphs={graphs} sim {i} ({w.name}, {w.nstep} steps): "
+              f"{err:.2e}", flush=True)
+        if scatter == "gather":
+            assert np.array_equal(got, ref[i])
+        else:
+            assert err <= 1e-6
     assert info["point_steps"] == sum(((4 * w.notes["n"] + 1) ** 2) * w.nstep for w in ws)
+    b.close()
+
+
+def test_one_launch_batch_samples_every_type_and_step_interval():
+    """Displacement, velocity and acceleration sampled every third step, with
+    simulations ending at different steps: the one-launch-per-step batch
+    equals each engine's gather run bit for bit."""
+    from specfempp_amd import Batch, Engine2D
+
+    ws = meshes.config4(nsim=4, seed=23, nmin=6, nmax=14, nstep_range=(40, 120), nrec=5)
+    types = ("displacement", "velocity", "acceleration")
+    ref = []
+    for w in ws:
+        e = Engine2D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, 3, types, scatter="gather")
+        e.initialize()
+        e.run()
+        ref.append(e.seismograms())
+        e.close()
+    engines = [Engine2D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, 3, types, host_only=True) for w in ws]
+    b = Batch(engines)
+    for e in engines:
+        e.close()
+    for graphs in (True, False):
+        b.set_config(5, graphs, "gather")
+        b.run()
+        for i in range(len(ws)):
+            err = rel_l2(b.seismograms(i).astype(np.float64), ref[i].astype(np.float64))
+            print(f"[batch] every third step, graphs={graphs}, sim {i}: {err:.2e}", flush=True)
+            assert err == 0.0, (graphs, i)
     b.close()
 
 
@@ -70,7 +117,9 @@ def test_batch_matches_oracle_and_tuning_keeps_results(alone):
     b = Batch(_engines(ws))
     choice = b.tune(3)
     assert choice["elems_per_block"] in (2, 5, 10)
+    assert choice["scatter"] in ("gather", "colored", "atomic")
     b.run()
+    ref = ref["gather" if choice["scatter"] == "gather" else "colored"]
     for i in (0, len(ws) - 1):
         w = ws[i]
         s32, _ = oracle.Oracle2D(w.mesh, 4, 1, w.dt).forward(w.sources, w.receivers, w.nstep, w.dt, 1, TYPES)
@@ -83,19 +132,22 @@ def test_batch_matches_oracle_and_tuning_keeps_results(alone):
 
 
 def test_batch_of_host_only_engines(alone):
-    """A batch built from host-only engines (no device memory of their own)
-    gives the same seismograms."""
+    """A batch built from host-only engines (no device memory of their own),
+    made with the atomic scatter (one "color" in compute order: the batch
+    colors the meshes itself), gives the same seismograms."""
     from specfempp_amd import Batch, Engine2D
 
     ws, ref = alone
-    engines = [Engine2D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, 1, TYPES, host_only=True) for w in ws]
+    engines = [Engine2D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, 1, TYPES, scatter="atomic", host_only=True)
+               for w in ws]
     b = Batch(engines)
     for e in engines:
         e.close()
-    b.set_config(5, True)
-    b.run()
-    for i in range(len(ws)):
-        assert rel_l2(b.seismograms(i).astype(np.float64), ref[i]) <= 1e-6
+    for scatter in ("gather", "colored"):
+        b.set_config(5, True, scatter)
+        b.run()
+        for i in range(len(ws)):
+            assert rel_l2(b.seismograms(i).astype(np.float64), ref[scatter][i]) <= 1e-6
     b.close()
 
 
@@ -115,17 +167,17 @@ def test_batch_at_the_configuration_extremes_matches_oracle():
     print("[case] batch extremes: " + ", ".join(f"{w.name} x {w.nstep}" for w in ws), flush=True)
     with jobs.pool(len(ws)) as ex:
         fut = [ex.submit(jobs.forward_2d, w.mesh, 4, w.sources, w.receivers, w.nstep, w.dt, TYPES) for w in ws]
-        alone = []
-        for e in _engines(ws):
-            e.initialize()
-            e.run()
-            alone.append(e.seismograms().astype(np.float64))
-            e.close()
         b = Batch(_engines(ws))
         choice = b.tune(3)
         b.run()
         got = [b.seismograms(i).astype(np.float64) for i in range(len(ws))]
         b.close()
+        alone = []
+        for e in _engines(ws, "gather" if choice["scatter"] == "gather" else "colored"):
+            e.initialize()
+            e.run()
+            alone.append(e.seismograms().astype(np.float64))
+            e.close()
         ref = [f.result() for f in fut]
     for i, w in enumerate(ws):
         e32 = rel_l2(got[i][:, 0], ref[i][:, 0])

@@ -2,15 +2,17 @@
 forward Newmark loop (64^3 layered cube) on one MI355X, with the roofline of
 the dominant kernel and the CPU oracle timed beside it; then the batched
 multi-event leg (64 2-D simulations in one batch vs the same 64 run one after
-another) and the adjoint leg (200^2 2-D gradient vs a plain forward run).
+another), the adjoint leg (200^2 2-D gradient vs a plain forward run) and the
+config-2 leg (2-D Gaussian-random-field medium, 4000^2 quads of 2.5 m, 256 M
+points, with the roofline of its dominant kernel).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 64] [--no-cpu]
-                    [--no-batch] [--no-adjoint] [--scatter auto|gather|colored|atomic]
+                    [--no-batch] [--no-adjoint] [--no-config2] [--c2-steps K]
+                    [--scatter auto|gather|colored|atomic]
     python bench.py --config2 [--c2-n 4000] [--steps K] [--warmup W]
 
-Prints one JSON line.  --config2 measures instead the 2-D heterogeneous
-workload (Gaussian-random-field medium, 4000^2 quads of 2.5 m, 256 M points)
-with the roofline of its stiffness kernel.  A "step" is one Newmark step of the whole hot path:
+Prints one JSON line.  --config2 measures only the config-2 workload.
+A "step" is one Newmark step of the whole hot path:
 sources, stiffness (gradient, stress, divergence, assembly), Stacey,
 seismograms, fused mass division + corrector + predictor.
 """
@@ -420,6 +422,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-config2", action="store_true",
+                    help="skip the config-2 leg (2-D 4000^2 heterogeneous, ~30 s of host setup)")
+    ap.add_argument("--c2-steps", type=int, default=40, help="timed steps of the config-2 leg")
     ap.add_argument("--batch-sims", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (scale reference for HBM byte counters)")
@@ -535,6 +540,9 @@ def main() -> int:
         out["batched"] = batched_leg(args.batch_sims, 5)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg()
+    if not args.no_config2:
+        # the 2-D heterogeneous configuration at full size (256 M points), gather path
+        out["config2"] = config2_leg(args.c2_n, args.c2_steps, 3, scatter=args.c2_scatter)
     print(json.dumps(out))
     return 0
 

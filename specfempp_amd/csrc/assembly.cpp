@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <numeric>
 #include <cstring>
 #include <limits>
 #include <map>
@@ -670,6 +671,44 @@ static void build_coupling_2d(HostAssembly &a, const sfx_mesh2d &m, const std::v
   }
 }
 
+// 2-D gather path: device points in row order of their coordinates (z, then
+// x; z taken on a grid of 1e-6 of the mesh height, so the points of one
+// lattice row of a structured mesh stay together).  The reference's point
+// numbering follows the elements, so an element's rows of points lie far
+// apart in it; in row order, the elements of one row of the mesh read and
+// write a few contiguous runs of points.  gather, the Stacey point list and
+// M^-1 move to device indices; lattice_of[f] is the device index of field
+// point f (the API keeps the reference numbering).
+static void order_points_by_rows(HostAssembly &a) {
+  const int ng = a.nglob_el;
+  const size_t nloc = static_cast<size_t>(a.nspec) * a.npe;
+  std::vector<double> cx(ng, 0.0), cz(ng, 0.0);
+  double zlo = 0.0, zhi = 0.0;
+  for (size_t p = 0; p < nloc; p++) {
+    cx[a.gather[p]] = a.px2d[p];
+    cz[a.gather[p]] = a.pz2d[p];
+    zlo = p ? std::min(zlo, static_cast<double>(a.pz2d[p])) : a.pz2d[p];
+    zhi = p ? std::max(zhi, static_cast<double>(a.pz2d[p])) : a.pz2d[p];
+  }
+  const double grid = std::max(1e-6 * (zhi - zlo), 1e-30);
+  std::vector<int64_t> row(ng);
+  for (int g = 0; g < ng; g++) row[g] = std::llround((cz[g] - zlo) / grid);
+  std::vector<int32_t> order(ng);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int32_t u, int32_t v) {
+    return row[u] != row[v] ? row[u] < row[v] : cx[u] < cx[v];
+  });
+  a.lattice_of.assign(ng, -1);
+  for (int d = 0; d < ng; d++) a.lattice_of[order[d]] = d;
+  for (auto &g : a.gather) g = a.lattice_of[g];
+  for (auto &g : a.bnd_gather) g = a.lattice_of[g];
+  std::vector<float> m(a.mass_inverse.size(), 0.0f);
+  for (int c = 0; c < a.ncomp; c++)
+    for (int g = 0; g < ng; g++)
+      m[static_cast<size_t>(c) * a.nglob_pad + a.lattice_of[g]] = a.mass_inverse[static_cast<size_t>(c) * a.nglob_pad + g];
+  a.mass_inverse = std::move(m);
+}
+
 HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *src, int nrec,
                                const sfx_receiver *rec, const sfx_run_desc &run) {
   check_run(run);
@@ -1061,6 +1100,7 @@ HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *
   a.ctrl2d = std::move(ctrl);
   a.px2d = std::move(px);
   a.pz2d = std::move(pz);
+  if (run.scatter == SFX_SCATTER_GATHER && !coupled) order_points_by_rows(a);
   return a;
 }
 

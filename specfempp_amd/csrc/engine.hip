@@ -510,6 +510,10 @@ struct sfx_engine {
       launch_assemble_3d(stream, lat, re_p, f(), 1);
       a_stale = false;
     }
+    if (gather2d() && a_stale) {  // re2 still holds the last step's forces
+      launch_assemble_update_2d(stream, h.nglob, fs2(), f(), nc, 1);
+      a_stale = false;
+    }
   }
 
   // One launch per color (plain read-modify-write), or one atomic launch.
@@ -624,6 +628,7 @@ struct sfx_engine {
       if (evs) SFX_HIP(hipEventRecord(evs[3], stream));
       launch_assemble_update_2d(stream, h.nglob, fs2(), ff, nc, 0);
       if (evs) SFX_HIP(hipEventRecord(evs[4], stream));
+      a_stale = true;
       return;
     }
     if (gather_path()) {
@@ -782,6 +787,7 @@ struct sfx_engine {
       if (istep < 0 || istep >= run.nstep) throw std::invalid_argument("istep out of range");
       forces_2d(ff, istep, nullptr);
       launch_assemble_update_2d(stream, h.nglob, fs2(), ff, nc, 1);
+      a_stale = false;
     } else {
       if (h.coupled()) couple_into_elastic(ff);
       sources(ff, istep);
@@ -1341,6 +1347,7 @@ int sfx_run(sfx_engine *e, int first, int count) {
         throttle.tick();
       }
       SFX_HIP(hipEventRecord(e->ev[1], e->stream));
+      if (e->gather2d()) e->a_stale = true;
       SFX_HIP(hipStreamSynchronize(e->stream));
       float all = 0.0f;
       SFX_HIP(hipEventElapsedTime(&all, e->ev[0], e->ev[1]));

@@ -327,9 +327,10 @@ __global__ __launch_bounds__(64) void sources_re2_kernel(int nsrc, int istep_arg
 }
 
 // One thread per field point: the assembled force (its element contributions
-// in a fixed order), a = M^-1 f, then (unless divide_only) corrector and
-// predictor: v += dt/2 a; u += dt v + dt^2/2 a; v += dt/2 a.  The stored a is
-// the step's acceleration.
+// in a fixed order), a = M^-1 f; divide_only stores a, otherwise corrector and
+// predictor follow: v += dt/2 a; u += dt v + dt^2/2 a; v += dt/2 a, and a is
+// not stored (nothing in the fused loop reads it; the engine recomputes it
+// from re2 with divide_only when a caller asks for it).
 __global__ __launch_bounds__(256) void assemble_update_2d_kernel(int nglob, ForceSum2D fs, Fields f,
                                                                  NewmarkConst nc, int divide_only) {
 #pragma clang fp contract(off)
@@ -346,8 +347,10 @@ __global__ __launch_bounds__(256) void assemble_update_2d_kernel(int nglob, Forc
   for (int c = 0; c < 2; c++) {
     const size_t o = static_cast<size_t>(c) * np + g;
     const float a = sum[c] * f.minv[o];
-    f.a[o] = a;
-    if (divide_only) continue;
+    if (divide_only) {
+      f.a[o] = a;
+      continue;
+    }
     const float v1 = f.v[o] + nc.dto2 * a;
     f.u[o] = f.u[o] + (nc.dt * v1 + nc.dt2o2 * a);
     f.v[o] = v1 + nc.dto2 * a;

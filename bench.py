@@ -326,14 +326,15 @@ def algorithmic_bytes_2d(nspec: int, nglob: int, per_element_geometry: bool, sca
     point-step (25/16 local points per point); model_per_pt_step keeps it.
     The gather path instead stores the element forces (8 B per local point)
     and its update reads them once with the inverse index (4 B per local
-    point and 4 B per point), and u, v, M^-1 (24 B), writes u, v, a (24 B):
+    point and 4 B per point), and u, v, M^-1 (24 B), writes u, v (16 B; the
+    acceleration is recomputed only when a caller asks for it):
       elemforce_2d: gather index, kappa, mu per local point, geometry, the
                     displacement (8 B per point), the forces written
       assemble_update_2d: index and forces read, the update as above."""
     geo = nspec * 32 if per_element_geometry else nspec * 25 * 20
     if scatter == "gather":
         stiff = nspec * 25 * (12 + 8) + geo + nglob * 8
-        upd = nspec * 25 * (8 + 4) + nglob * (4 + 24 + 24)
+        upd = nspec * 25 * (8 + 4) + nglob * (4 + 24 + 16)
     else:
         stiff = nspec * 25 * 12 + geo + nglob * 24
         upd = nglob * 56

@@ -73,9 +73,11 @@ def minv_bytes_3d(nx: int, ny: int, nz: int, absorbing) -> int:
     return total
 
 
-def algorithmic_bytes_3d(nspec: int, nglob: int, minv_bytes: int = None) -> dict:
+def algorithmic_bytes_3d(nspec: int, nglob: int, minv_bytes: int = None, element_properties: bool = False) -> dict:
     """Bytes each kernel of the 3-D gather path must move per launch (FP32):
-      elemforce_3d:       kappa + mu per local point (8 B x 125 / element), the
+      elemforce_3d:       kappa + mu per local point (8 B x 125 / element), or
+                          one pair per element (8 B) when the engine reads them
+                          so (every element holds one value of each), the
                           displacement once per global point (12 B), the element
                           forces stored once (12 B x 125 / element)
       assemble_update_3d: the element forces loaded once (12 B x 125 / element),
@@ -87,7 +89,8 @@ def algorithmic_bytes_3d(nspec: int, nglob: int, minv_bytes: int = None) -> dict
     read-modify-write, fused update read u v a M^-1 / write u v a)."""
     if minv_bytes is None:
         minv_bytes = nglob * 12
-    elem = nspec * 125 * 8 + nglob * 12 + nspec * 125 * 12
+    props = nspec * 8 if element_properties else nspec * 125 * 8
+    elem = props + nglob * 12 + nspec * 125 * 12
     upd = nspec * 125 * 12 + nglob * 48 + minv_bytes
     survey = nspec * 125 * 12 + nglob * 36 + nglob * 84
     return {"elemforce": elem, "update": upd, "per_pt_step": (elem + upd) / nglob,
@@ -313,12 +316,18 @@ def adjoint_leg(snapshot_every: int = 55) -> dict:
             "misfit": misfit, "snapshot_every": snapshot_every, "kernels_finite": finite}
 
 
-def algorithmic_bytes_2d(nspec: int, nglob: int, per_element_geometry: bool, scatter: str = "colored") -> dict:
+GEOMETRY_BYTES_PER_ELEMENT_2D = {"point": 25 * 20, "element": 32, "control_nodes_4": 4 * 8, "control_nodes_9": 9 * 8}
+
+
+def algorithmic_bytes_2d(nspec: int, nglob: int, geometry: str = "point", element_properties: bool = False,
+                         scatter: str = "colored") -> dict:
     """Bytes each kernel of the 2-D path must move per step (FP32):
       stiffness_2d (all colors): gather index, kappa, mu per local point
-                   (12 B x 25 / element) and the geometry: xix, xiz, gammax,
-                   gammaz, J per local point (20 B x 25 / element), or one
-                   32-byte record per element on affine meshes; the
+                   (12 B x 25 / element) and the geometry (the engine's
+                   layout): xix, xiz, gammax, gammaz, J per local point
+                   (20 B x 25 / element), one 32-byte record per element
+                   (affine elements), or the element's 4 or 9 control nodes
+                   (8 B each) from which the gather path recomputes it; the
                    displacement gathered once per point (8 B), the
                    acceleration read-modify-written once per point (16 B)
       update_fused: a, M^-1, u, v read (32 B) and u, v, a written (24 B) per point
@@ -328,15 +337,18 @@ def algorithmic_bytes_2d(nspec: int, nglob: int, per_element_geometry: bool, sca
     and its update reads them once with the inverse index (4 B per local
     point and 4 B per point), and u, v, M^-1 (24 B), writes u, v (16 B; the
     acceleration is recomputed only when a caller asks for it):
-      elemforce_2d: gather index, kappa, mu per local point, geometry, the
-                    displacement (8 B per point), the forces written
+      elemforce_2d: gather index per local point, kappa and mu per local
+                    point or (element_properties) one pair per element, the
+                    geometry, the displacement (8 B per point), the forces
+                    written
       assemble_update_2d: index and forces read, the update as above."""
-    geo = nspec * 32 if per_element_geometry else nspec * 25 * 20
+    geo = nspec * GEOMETRY_BYTES_PER_ELEMENT_2D[geometry]
+    props = nspec * 8 if element_properties else nspec * 25 * 8
     if scatter == "gather":
-        stiff = nspec * 25 * (12 + 8) + geo + nglob * 8
+        stiff = nspec * 25 * (4 + 8) + props + geo + nglob * 8
         upd = nspec * 25 * (8 + 4) + nglob * (4 + 24 + 16)
     else:
-        stiff = nspec * 25 * 12 + geo + nglob * 24
+        stiff = nspec * 25 * 4 + props + geo + nglob * 24
         upd = nglob * 56
     model = nspec * 25 * 32 + nglob * 24 + nglob * 56  # per-point geometry, colored scatter, fused update
     return {"stiffness_2d": stiff, "update_fused": upd, "per_pt_step": (stiff + upd) / nglob,
@@ -376,11 +388,12 @@ def config2_leg(n: int, steps: int, warmup: int, sample_steps: int = 5, scatter:
         for k, v in eng.timings().items():
             acc[k] += v / sample_steps
     finite = bool(np.isfinite(eng.seismograms()).all())
+    lay = eng.layout()
     eng.close()
     value = nglob * steps / elapsed
-    # the workload's elements are rectangles: the engine reads one geometry
-    # record per element
-    ab = algorithmic_bytes_2d(nspec, nglob, per_element_geometry=True, scatter=scatter)
+    # the bytes of the layout the engine chose (geometry source, properties
+    # per point or per element)
+    ab = algorithmic_bytes_2d(nspec, nglob, lay["geometry"], lay["element_properties"], scatter=scatter)
     if scatter == "gather":
         kern = {"elemforce_2d_kernel": (ab["stiffness_2d"], acc["stiffness_ms"]),
                 "assemble_update_2d_kernel": (ab["update_fused"], acc["update_ms"])}
@@ -408,7 +421,7 @@ def config2_leg(n: int, steps: int, warmup: int, sample_steps: int = 5, scatter:
         "algorithmic_bytes_per_pt_step": ab["per_pt_step"],
         "model_bytes_per_pt_step": ab["model_per_pt_step"],
         "model_step_hbm_frac": ab["model_per_pt_step"] * value / 1e9 / HBM_PEAK_GBS,
-        "geometry": "one record per element (affine elements)", "scatter": scatter, "seismograms_finite": finite,
+        "layout": lay, "scatter": scatter, "seismograms_finite": finite,
     }
 
 
@@ -463,6 +476,7 @@ def main() -> int:
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     tm = eng.timings()
+    lay = eng.layout()
     seis = eng.seismograms()
     finite = bool(np.isfinite(seis).all())
     nglob, nspec = eng.nglob, eng.nspec
@@ -478,7 +492,7 @@ def main() -> int:
         # the GATHER update reads one M^-1 component off the absorbing faces; the
         # colored and atomic updates read all three
         minv = minv_bytes_3d(args.n, args.n, args.n, w.mesh["absorbing"]) if scatter == "gather" else None
-        ab = algorithmic_bytes_3d(nspec, nglob, minv)
+        ab = algorithmic_bytes_3d(nspec, nglob, minv, element_properties=lay["element_properties"])
         kern = {"elemforce_3d_kernel": (ab["elemforce"], tm["stiffness_ms"]),
                 "assemble_update_3d_kernel": (ab["update"], tm["update_ms"])}
     dominant = max(kern, key=lambda k: kern[k][1])
@@ -527,6 +541,7 @@ def main() -> int:
         "algorithmic_bytes_per_pt_step": ab["per_pt_step"],
         "survey_bytes_per_pt_step": ab["survey_per_pt_step"],
         "survey_step_hbm_frac": survey_gbs / HBM_PEAK_GBS,
+        "layout": lay,
         "seismograms_finite": finite,
     }
     # the practical HBM ceiling beside the nominal peak: a plain 1 GiB device copy

@@ -219,6 +219,16 @@ void sfx_engine_destroy(sfx_engine *engine);
 
 /* Sizes: dim, nspec, nglob, ngll, nsig (seismogram samples), nrec, nsrc. */
 int sfx_get_sizes(const sfx_engine *engine, int64_t *sizes7);
+/* Storage layout the engine chose (no counterpart in the reference; for
+ * byte models and tests): [0] geometry of the element forces: 0 per-point
+ * arrays, 1 one record per element (every element's geometry is the same at
+ * all of its points), 4 or 9 recomputed at every point from the element's 4
+ * or 9 control nodes (2-D gather path; the same bits as the per-point
+ * arrays); [1] 1 if the 2-D gather path reads kappa and mu once per element
+ * (both the same at all points of every element), else 0; [2] the scatter
+ * mode; [3] 1 if the device numbers the points in coordinate row order (2-D
+ * gather path), else 0. */
+int sfx_get_layout(const sfx_engine *engine, int64_t *layout4);
 /* Per-medium sizes of a 2-D mesh (specfem::compute::simulation_field's
  * per-medium fields, field_impl.tpp:23-103): elastic elements, acoustic
  * elements, points of the elastic field, points of the acoustic field.  A mesh

@@ -86,6 +86,7 @@ def load():
         "sfx_engine_destroy": ([vp], None),
         "sfx_get_sizes": ([vp, vp], i32),
         "sfx_get_media_sizes": ([vp, vp], i32),
+        "sfx_get_layout": ([vp, vp], i32),
         "sfx_get_field_index_acoustic": ([vp, vp], i32),
         "sfx_get_medium_field": ([vp, i32, i32, vp], i32),
         "sfx_set_medium_field": ([vp, i32, i32, vp], i32),

@@ -803,6 +803,13 @@ HostAssembly build_assembly_2d(const sfx_mesh2d &m, int nsrc, const sfx_source *
   float dsh0[5][5][9], dsh1[5][5][9];
   for (int iz = 0; iz < NGLL; iz++)
     for (int ix = 0; ix < NGLL; ix++) dshape2d(a.xi[ix], a.xi[iz], ngnod, dsh0[iz][ix], dsh1[iz][ix]);
+  a.dsh2d.assign(static_cast<size_t>(npe) * 2 * ngnod, 0.0f);
+  for (int iz = 0; iz < NGLL; iz++)
+    for (int ix = 0; ix < NGLL; ix++)
+      for (int n = 0; n < ngnod; n++) {
+        a.dsh2d[(static_cast<size_t>(iz * NGLL + ix) * 2) * ngnod + n] = dsh0[iz][ix][n];
+        a.dsh2d[(static_cast<size_t>(iz * NGLL + ix) * 2 + 1) * ngnod + n] = dsh1[iz][ix][n];
+      }
   for (int e = 0; e < nspec; e++)
     for (int iz = 0; iz < NGLL; iz++)
       for (int ix = 0; ix < NGLL; ix++) {

@@ -83,6 +83,9 @@ struct HostAssembly {
   // 2-D locator data (control nodes (nspec, ngnod, 2), GLL point coordinates)
   int ngnod2d = 0;
   std::vector<float> ctrl2d, px2d, pz2d;
+  // 2-D: the shape-function derivatives the geometry above was computed from,
+  // per GLL point (iz * 5 + ix) ngnod values in xi, then ngnod in gamma
+  std::vector<float> dsh2d;
   // adjoint sources as single-component sources (see place_adjoint_sources_2d)
   int nadj = 0;
   std::vector<int32_t> adj_ispec, adj_order;

@@ -81,6 +81,12 @@ struct sfx_engine {
   // 2-D gather path: element forces, the per-point inverse index, Stacey
   // entries sorted by element point with their segments
   DevBuf<float> re2;
+  // 2-D gather path on meshes without constant element geometry: the control
+  // nodes per element and the shape derivatives at the GLL points, from which
+  // the element forces recompute each point's geometry; and (2-D and 3-D
+  // gather paths) (kappa, mu) per element when both are the same at all
+  // points of every element
+  DevBuf<float> ctrl, dsh, kmu_e;
   DevBuf<int> inv_off, inv_ent, g2_seg, g2_gather, g2_slot;
   DevBuf<float> g2_normal, g2_weight, g2_rvp, g2_rvs;
   int g2_nseg = 0;
@@ -230,6 +236,50 @@ struct sfx_engine {
   bool gather_path() const { return h.dim == 3 && run.scatter == SFX_SCATTER_GATHER; }
   bool gather2d() const { return h.dim == 2 && run.scatter == SFX_SCATTER_GATHER; }
   ForceSum2D fs2() const { return ForceSum2D{inv_off.p, inv_ent.p, re2.p}; }
+  // where the 2-D gather path's element forces take each point's geometry from
+  int geom2d_form() const {
+    if (h.dim != 2) return GEOM2D_POINT;
+    if (!h.geo2d.empty()) return GEOM2D_RECORD;
+    if (gather2d() && (h.ngnod2d == 4 || h.ngnod2d == 9)) return h.ngnod2d;
+    return GEOM2D_POINT;
+  }
+  Geom2D geom2d() const {
+    Geom2D gm;
+    gm.xix = xix.p;
+    gm.xiz = xiz.p;
+    gm.gammax = gammax.p;
+    gm.gammaz = gammaz.p;
+    gm.jac = jac.p;
+    gm.mode = geom2d_form();
+    gm.record = geo.p;
+    gm.ctrl = ctrl.p;
+    gm.dsh = dsh.p;
+    return gm;
+  }
+  // (kappa, mu) per element slot for the 2-D and 3-D gather paths when every
+  // element holds one value of each at all of its points (bit for bit); empty
+  // otherwise
+  std::vector<float> element_properties() const {
+    if (!gather2d() && !gather_path()) return {};
+    const size_t npe = h.npe;
+    std::vector<float> kmu(2 * static_cast<size_t>(h.nspec));
+    auto same = [](float a, float b) { return std::memcmp(&a, &b, sizeof(float)) == 0; };
+    for (int slot = 0; slot < h.nspec; slot++) {
+      const size_t b = static_cast<size_t>(h.perm[slot]) * npe;
+      for (size_t p = 1; p < npe; p++)
+        if (!same(h.kappa[b + p], h.kappa[b]) || !same(h.mu[b + p], h.mu[b])) return {};
+      kmu[2 * static_cast<size_t>(slot)] = h.kappa[b];
+      kmu[2 * static_cast<size_t>(slot) + 1] = h.mu[b];
+    }
+    return kmu;
+  }
+  void upload_element_properties(hipStream_t s) {
+    const std::vector<float> kmu = element_properties();
+    if (kmu.empty())
+      kmu_e.alloc(0);
+    else
+      kmu_e.upload(kmu, s);
+  }
 
   // 2-D gather path: per field point the re2 offsets of its element
   // contributions (ascending element slot, then point), and the Stacey entries
@@ -283,8 +333,7 @@ struct sfx_engine {
   }
   // element forces, Stacey and sources into re2 (the step's force, unassembled)
   void forces_2d(Fields ff, int istep, const int *step) {
-    launch_elemforce_2d(stream, h.nspec, gather.p, xix.p, xiz.p, gammax.p, gammaz.p, jac.p, kappa.p, mu.p, q, ff.u,
-                        h.nglob_pad, geo.p, re2.p);
+    launch_elemforce_2d(stream, h.nspec, gather.p, geom2d(), kappa.p, mu.p, kmu_e.p, q, ff.u, h.nglob_pad, re2.p);
     launch_stacey_re2(stream, g2_nseg, g2_seg.p, g2_gather.p, g2_slot.p, g2_normal.p, g2_weight.p, g2_rvp.p, g2_rvs.p,
                       ff, re2.p);
     launch_sources_re2(stream, h.nsrc, istep, src_order.p, src_ispec.p, src_array.p, stf.p, re2.p, step);
@@ -350,6 +399,7 @@ struct sfx_engine {
     gather.upload(permuted(h.gather), s);
     kappa.upload(permuted(h.kappa), s);
     mu.upload(permuted(h.mu), s);
+    upload_element_properties(s);
     if (h.general3d) geo3.upload(permuted(h.geo3, 10 * static_cast<size_t>(h.npe)), s);
     if (h.dim == 2) {
       xix.upload(permuted(h.xix), s);
@@ -358,6 +408,10 @@ struct sfx_engine {
       gammaz.upload(permuted(h.gammaz), s);
       jac.upload(permuted(h.jac), s);
       if (!h.geo2d.empty()) geo.upload(permuted(h.geo2d, 8), s);
+      if (geom2d_form() == GEOM2D_CTRL4 || geom2d_form() == GEOM2D_CTRL9) {
+        ctrl.upload(permuted(h.ctrl2d, 2 * static_cast<size_t>(h.ngnod2d)), s);
+        dsh.upload(h.dsh2d, s);
+      }
       rho.upload(permuted(h.rho), s);
       // Stacey elements are the last ones in compute order
       nstacey = h.nspec - h.nspec_none;
@@ -481,7 +535,8 @@ struct sfx_engine {
   // Gather path: element forces into re, then sources and Stacey into re.
   void element_forces(Fields ff, int istep) {
     if (istep < 0 || istep >= run.nstep) throw std::invalid_argument("istep out of range");
-    launch_elemforce_3d(stream, lat, kappa.p, mu.p, h.xix3, h.ety3, h.gaz3, h.jac3, q, ff.u, h.nglob_pad, re_p);
+    launch_elemforce_3d(stream, lat, kappa.p, mu.p, kmu_e.p, h.xix3, h.ety3, h.gaz3, h.jac3, q, ff.u, h.nglob_pad,
+                        re_p);
   }
   void element_boundary_sources(Fields ff, int istep) {
     if (sources_off_boundary) {
@@ -1073,6 +1128,17 @@ int sfx_get_sizes(const sfx_engine *e, int64_t *s) {
   });
 }
 
+int sfx_get_layout(const sfx_engine *e, int64_t *s) {
+  return guarded([&] {
+    check_engine(e);
+    if (!s) throw std::invalid_argument("null output");
+    s[0] = e->h.dim == 2 ? e->geom2d_form() : (e->h.general3d ? GEOM2D_POINT : GEOM2D_RECORD);
+    s[1] = e->element_properties().empty() ? 0 : 1;
+    s[2] = e->run.scatter;
+    s[3] = e->gather2d() ? 1 : 0;
+  });
+}
+
 int sfx_get_media_sizes(const sfx_engine *e, int64_t *s) {
   return guarded([&] {
     check_engine(e);
@@ -1537,6 +1603,7 @@ int sfx_set_properties(sfx_engine *e, const float *rho, const float *kappa, cons
     hipStream_t s = e->stream;
     e->kappa.upload(e->permuted(e->h.kappa), s);
     e->mu.upload(e->permuted(e->h.mu), s);
+    e->upload_element_properties(s);
     e->rho.upload(e->permuted(e->h.rho), s);
     e->bnd_rvp.upload(e->h.bnd_rho_vp, s);
     e->bnd_rvs.upload(e->h.bnd_rho_vs, s);

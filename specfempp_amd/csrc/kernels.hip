@@ -169,11 +169,54 @@ __global__ __launch_bounds__(threads_for(E)) void stiffness_2d_kernel(
 // order (ascending element, then point) and applies the Newmark update, with
 // no scattered read-modify-write of the acceleration.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(128) void elemforce_2d_kernel(
-    int nspec, const int *__restrict__ gather, const float *__restrict__ xix_, const float *__restrict__ xiz_,
-    const float *__restrict__ gammax_, const float *__restrict__ gammaz_, const float *__restrict__ jac_,
-    const float *__restrict__ kappa_, const float *__restrict__ mu_, GllConst q, const float *__restrict__ uf,
-    int np, const float *__restrict__ geo, float *__restrict__ re2) {
+// Geometry of local point p (index idx) of element e from the source GEOM
+// names.  The control-node forms sum the shape derivatives times the nodes in
+// node order, then form J and the four quotients exactly as the host does.
+template <int GEOM>
+__device__ __forceinline__ void point_geometry_2d(const Geom2D &gm, int e, int p, size_t idx, float &xix,
+                                                  float &xiz, float &gammax, float &gammaz, float &J) {
+#pragma clang fp contract(off)
+  if constexpr (GEOM == GEOM2D_POINT) {
+    xix = gm.xix[idx];
+    xiz = gm.xiz[idx];
+    gammax = gm.gammax[idx];
+    gammaz = gm.gammaz[idx];
+    J = gm.jac[idx];
+  } else if constexpr (GEOM == GEOM2D_RECORD) {
+    const float *ge = gm.record + static_cast<size_t>(e) * 8;
+    xix = ge[0];
+    xiz = ge[1];
+    gammax = ge[2];
+    gammaz = ge[3];
+    J = ge[4];
+  } else {
+    constexpr int NG = GEOM;
+    const float *c = gm.ctrl + static_cast<size_t>(e) * (2 * NG);
+    const float *d = gm.dsh + p * (2 * NG);
+    float xxi = 0.0f, zxi = 0.0f, xgamma = 0.0f, zgamma = 0.0f;
+#pragma unroll
+    for (int n = 0; n < NG; n++) {
+      const float cx = c[2 * n], cz = c[2 * n + 1];
+      xxi += d[n] * cx;
+      zxi += d[n] * cz;
+      xgamma += d[NG + n] * cx;
+      zgamma += d[NG + n] * cz;
+    }
+    J = xxi * zgamma - xgamma * zxi;
+    xix = zgamma / J;
+    gammax = -zxi / J;
+    xiz = -xgamma / J;
+    gammaz = xxi / J;
+  }
+}
+
+template <int GEOM, bool EPROP>
+__global__ __launch_bounds__(128) void elemforce_2d_kernel(int nspec, const int *__restrict__ gather, Geom2D gm,
+                                                           const float *__restrict__ kappa_,
+                                                           const float *__restrict__ mu_,
+                                                           const float *__restrict__ kmu, GllConst q,
+                                                           const float *__restrict__ uf, int np,
+                                                           float *__restrict__ re2) {
   // every product and sum rounded on its own: the one-launch batch step
   // performs the same operations and must round them the same way
 #pragma clang fp contract(off)
@@ -191,25 +234,17 @@ __global__ __launch_bounds__(128) void elemforce_2d_kernel(
 
   float kappa = 0, mu = 0, xix = 0, xiz = 0, gammax = 0, gammaz = 0, J = 0;
   if (active) {
-    kappa = kappa_[idx];
-    mu = mu_[idx];
     const int g = gather[idx];
     su[el][0][p] = uf[g];
     su[el][1][p] = uf[np + g];
-    if (geo) {
-      const float *ge = geo + static_cast<size_t>(e) * 8;
-      xix = ge[0];
-      xiz = ge[1];
-      gammax = ge[2];
-      gammaz = ge[3];
-      J = ge[4];
+    if constexpr (EPROP) {
+      kappa = kmu[2 * static_cast<size_t>(e)];
+      mu = kmu[2 * static_cast<size_t>(e) + 1];
     } else {
-      xix = xix_[idx];
-      xiz = xiz_[idx];
-      gammax = gammax_[idx];
-      gammaz = gammaz_[idx];
-      J = jac_[idx];
+      kappa = kappa_[idx];
+      mu = mu_[idx];
     }
+    point_geometry_2d<GEOM>(gm, e, p, idx, xix, xiz, gammax, gammaz, J);
   }
   __syncthreads();
   float hpx[5], hpz[5], hwx[5], hwz[5];
@@ -1019,8 +1054,12 @@ __global__ __launch_bounds__(256) void divide_corrector_kernel(int n, int ncomp,
 // workgroups round-robin, so each XCD is given one contiguous slab of
 // elements and its L2 sees the shared faces of neighbouring elements.
 // ---------------------------------------------------------------------------
+// EPROP: kappa and mu are read once per element from kmu (kappa, mu pairs)
+// instead of at each of its 125 points; the values are the same bits.
+template <bool EPROP>
 __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const float *__restrict__ kappa_,
-                                                           const float *__restrict__ mu_, Gll3 q,
+                                                           const float *__restrict__ mu_,
+                                                           const float *__restrict__ kmu, Gll3 q,
                                                            const float *__restrict__ uf, int np,
                                                            double inv_nx, double inv_nxy,
                                                            float *__restrict__ re) {
@@ -1058,10 +1097,19 @@ __global__ __launch_bounds__(128) void elemforce_3d_kernel(Lattice lat, const fl
   // material first, so its latency overlaps the displacement loads
   float kap[5], muv[5];
   if (active) {
+    if constexpr (EPROP) {
+      const float ke = kmu[2 * static_cast<size_t>(e)], me = kmu[2 * static_cast<size_t>(e) + 1];
 #pragma unroll
-    for (int k = 0; k < 5; k++) {
-      kap[k] = kappa_[base + k * 25];
-      muv[k] = mu_[base + k * 25];
+      for (int k = 0; k < 5; k++) {
+        kap[k] = ke;
+        muv[k] = me;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+        kap[k] = kappa_[base + k * 25];
+        muv[k] = mu_[base + k * 25];
+      }
     }
   }
   float u[5][3];
@@ -1613,13 +1661,39 @@ void launch_seismograms(hipStream_t s, int dim, int npe, int nrec, int ntypes, c
                        gather, h, f, dto2, corrected, seismo, pg, q, step, nsb, sig_top, ForceSum2D{}, rec_list);
 }
 
-void launch_elemforce_2d(hipStream_t s, int nspec, const int *gather, const float *xix, const float *xiz,
-                         const float *gammax, const float *gammaz, const float *jac, const float *kappa,
-                         const float *mu, const GllConst &q, const float *u, int nglob_pad, const float *geo,
+template <int GEOM>
+static void launch_elemforce_2d_geom(hipStream_t s, int nspec, const int *gather, const Geom2D &gm,
+                                     const float *kappa, const float *mu, const float *kmu, const GllConst &q,
+                                     const float *u, int nglob_pad, float *re2) {
+  const dim3 grid(blocks(nspec, E2));
+  if (kmu)
+    hipLaunchKernelGGL((elemforce_2d_kernel<GEOM, true>), grid, dim3(128), 0, s, nspec, gather, gm, kappa, mu, kmu, q,
+                       u, nglob_pad, re2);
+  else
+    hipLaunchKernelGGL((elemforce_2d_kernel<GEOM, false>), grid, dim3(128), 0, s, nspec, gather, gm, kappa, mu, kmu,
+                       q, u, nglob_pad, re2);
+}
+
+void launch_elemforce_2d(hipStream_t s, int nspec, const int *gather, const Geom2D &gm, const float *kappa,
+                         const float *mu, const float *kmu, const GllConst &q, const float *u, int nglob_pad,
                          float *re2) {
   if (nspec <= 0) return;
-  hipLaunchKernelGGL(elemforce_2d_kernel, dim3(blocks(nspec, E2)), dim3(128), 0, s, nspec, gather, xix, xiz, gammax,
-                     gammaz, jac, kappa, mu, q, u, nglob_pad, geo, re2);
+  switch (gm.mode) {
+    case GEOM2D_POINT:
+      launch_elemforce_2d_geom<GEOM2D_POINT>(s, nspec, gather, gm, kappa, mu, kmu, q, u, nglob_pad, re2);
+      break;
+    case GEOM2D_RECORD:
+      launch_elemforce_2d_geom<GEOM2D_RECORD>(s, nspec, gather, gm, kappa, mu, kmu, q, u, nglob_pad, re2);
+      break;
+    case GEOM2D_CTRL4:
+      launch_elemforce_2d_geom<GEOM2D_CTRL4>(s, nspec, gather, gm, kappa, mu, kmu, q, u, nglob_pad, re2);
+      break;
+    case GEOM2D_CTRL9:
+      launch_elemforce_2d_geom<GEOM2D_CTRL9>(s, nspec, gather, gm, kappa, mu, kmu, q, u, nglob_pad, re2);
+      break;
+    default:
+      throw std::invalid_argument("elemforce_2d: unknown geometry form");
+  }
 }
 
 void launch_stacey_re2(hipStream_t s, int nseg, const int *seg, const int *gather, const int *slot,
@@ -1687,14 +1761,19 @@ Gll3 scaled_gll3(const GllConst &q, float xix, float ety, float gaz, float jac) 
   return g;
 }
 
-void launch_elemforce_3d(hipStream_t s, Lattice lat, const float *kappa, const float *mu, float xix, float ety,
-                         float gaz, float jac, const GllConst &q, const float *u, int nglob_pad, float *re) {
+void launch_elemforce_3d(hipStream_t s, Lattice lat, const float *kappa, const float *mu, const float *kmu,
+                         float xix, float ety, float gaz, float jac, const GllConst &q, const float *u,
+                         int nglob_pad, float *re) {
   const int nspec = lat.nx * lat.ny * lat.nz;
   if (nspec <= 0) return;
   const Gll3 g = scaled_gll3(q, xix, ety, gaz, jac);
   const double inv_nx = 1.0 / lat.nx, inv_nxy = 1.0 / (static_cast<double>(lat.nx) * lat.ny);
-  hipLaunchKernelGGL(elemforce_3d_kernel, dim3(blocks(nspec, E3)), dim3(128), 0, s, lat, kappa, mu, g, u, nglob_pad,
-                     inv_nx, inv_nxy, re);
+  if (kmu)
+    hipLaunchKernelGGL(elemforce_3d_kernel<true>, dim3(blocks(nspec, E3)), dim3(128), 0, s, lat, kappa, mu, kmu, g,
+                       u, nglob_pad, inv_nx, inv_nxy, re);
+  else
+    hipLaunchKernelGGL(elemforce_3d_kernel<false>, dim3(blocks(nspec, E3)), dim3(128), 0, s, lat, kappa, mu, kmu, g,
+                       u, nglob_pad, inv_nx, inv_nxy, re);
 }
 
 void launch_sources_re(hipStream_t s, int nsrc, int istep, const int *ispec, const float *array, const float *stf,

@@ -98,9 +98,11 @@ struct Lattice {
   int nx, ny, nz, TX, TY, TZ;
   int nint_pad;  // block layout only: slots of the block-interior points (lattice.hpp)
 };
-// Element forces -K_e u_e of every element into re (plain coalesced stores).
-void launch_elemforce_3d(hipStream_t s, Lattice lat, const float *kappa, const float *mu, float xix, float ety,
-                         float gaz, float jac, const GllConst &q, const float *u, int nglob_pad, float *re);
+// Element forces -K_e u_e of every element into re (plain coalesced stores);
+// kappa and mu per point, or (kmu given) one (kappa, mu) pair per element.
+void launch_elemforce_3d(hipStream_t s, Lattice lat, const float *kappa, const float *mu, const float *kmu,
+                         float xix, float ety, float gaz, float jac, const GllConst &q, const float *u,
+                         int nglob_pad, float *re);
 // re += source contributions (stf(istep) * source_array) of each source element.
 void launch_sources_re(hipStream_t s, int nsrc, int istep, const int *ispec, const float *array, const float *stf,
                        float *re);
@@ -195,9 +197,22 @@ void launch_seismograms(hipStream_t s, int dim, int npe, int nrec, int ntypes, c
                         float *seismo, const PointGeom2D &pg, const GllConst &q, const int *step = nullptr,
                         int nsb = 1, int sig_top = -1, ForceSum2D fs = ForceSum2D{}, const int *rec_list = nullptr,
                         int nlist = -1);
-void launch_elemforce_2d(hipStream_t s, int nspec, const int *gather, const float *xix, const float *xiz,
-                         const float *gammax, const float *gammaz, const float *jac, const float *kappa,
-                         const float *mu, const GllConst &q, const float *u, int nglob_pad, const float *geo,
+// Where elemforce_2d takes an element point's geometry from.  Every form gives
+// the same bits: the control-node forms repeat the host's float operations in
+// the host's order (compute_partial_derivatives.cpp:46-91, no fused
+// multiply-add, correctly rounded division), so they trade 20 bytes per point
+// for 8 or 18 floats per element.
+enum Geom2DMode : int { GEOM2D_POINT = 0, GEOM2D_RECORD = 1, GEOM2D_CTRL4 = 4, GEOM2D_CTRL9 = 9 };
+struct Geom2D {
+  int mode = GEOM2D_POINT;
+  const float *xix = nullptr, *xiz = nullptr, *gammax = nullptr, *gammaz = nullptr, *jac = nullptr;  // per point
+  const float *record = nullptr;  // per element: xix, xiz, gammax, gammaz, J, 3 unused
+  const float *ctrl = nullptr;    // per element: its ngnod control nodes (x, z)
+  const float *dsh = nullptr;     // (25, 2, ngnod): shape derivatives in xi, then gamma, at each GLL point
+};
+// kappa and mu per local point, or (kmu given) one (kappa, mu) pair per element
+void launch_elemforce_2d(hipStream_t s, int nspec, const int *gather, const Geom2D &gm, const float *kappa,
+                         const float *mu, const float *kmu, const GllConst &q, const float *u, int nglob_pad,
                          float *re2);
 void launch_stacey_re2(hipStream_t s, int nseg, const int *seg, const int *gather, const int *slot,
                        const float *normal, const float *weight, const float *rho_vp, const float *rho_vs, Fields f,

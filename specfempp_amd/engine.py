@@ -271,6 +271,18 @@ class _EngineBase:
     def synchronize(self):
         self._check(self.lib.sfx_synchronize(self.h))
 
+    def layout(self):
+        """The storage the engine chose: ``geometry`` "point", "element" or
+        "control_nodes_4" / "control_nodes_9" (recomputed per point from the
+        element's control nodes, the same bits as the per-point arrays),
+        ``element_properties`` (kappa and mu read once per element),
+        ``scatter`` and ``row_order`` (points in coordinate row order)."""
+        out = (C.c_int64 * 4)()
+        self._check(self.lib.sfx_get_layout(self.h, out))
+        geom = {0: "point", 1: "element", 4: "control_nodes_4", 9: "control_nodes_9"}[int(out[0])]
+        scat = {0: "colored", 1: "atomic", 2: "gather", 3: "block"}.get(int(out[2]), str(int(out[2])))
+        return {"geometry": geom, "element_properties": bool(out[1]), "scatter": scat, "row_order": bool(out[3])}
+
     def timings(self):
         out = (C.c_double * 4)()
         self._check(self.lib.sfx_get_timings(self.h, out))

@@ -19,6 +19,8 @@ from specfempp_amd import Engine3D, meshes  # noqa: E402
 types = ("displacement", "velocity", "acceleration")
 w = meshes.config3(n=3, nstep=12, h=100.0, nrec_side=2)
 eng = Engine3D(w.mesh, w.sources, w.receivers, w.dt, w.nstep, 1, types, scatter="gather")
+# a layered medium: one (kappa, mu) per element
+assert eng.layout()["element_properties"], eng.layout()
 eng.initialize()
 eng.run()
 ours = eng.seismograms().astype(np.float64)

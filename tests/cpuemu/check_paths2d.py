@@ -157,6 +157,21 @@ for scatter in ("colored", "gather"):
     e = rel_l2(eng.seismograms().astype(np.float64), r_mt)
     eng.close()
     check(f"moment tensor 2-D {scatter}", e <= 1e-5, f"{e:.2e}")
+# the reference's 9-node 80 x 60 fixture on the gather path: each point's
+# geometry recomputed from the element's 9 control nodes, kappa and mu read
+# once per element; the displacement after a few steps against the oracle's
+from conftest import load_case  # noqa: E402
+
+m9, cfg9 = load_case("elastic_stacey_80x60")
+eng = Engine2D(m9, cfg9["sources"], cfg9["stations"], cfg9["dt"], 6, 1, ("displacement",), scatter="gather")
+lay = eng.layout()
+d9 = snapshots(eng, (6,))[-1][0]
+eng.close()
+_, _, f9 = oracle.Oracle2D(m9, 4, 1, cfg9["dt"]).forward(cfg9["sources"], cfg9["stations"], 6, cfg9["dt"], 1,
+                                                          ("displacement",), return_fields=True)
+check("9-node mesh, gather (geometry from control nodes)",
+      lay["geometry"] == "control_nodes_9" and lay["element_properties"] and rel_l2(d9, f9[0]) <= 1e-5
+      and np.abs(d9).max() > 0, f"{lay} {rel_l2(d9, f9[0]):.2e}")
 for chunk in (2, 64):
     eng = Engine2D(mesh, SRC, REC, DT, NSTEP, 1, ("displacement",), chunk_size=chunk)
     o = oracle.Oracle2D(mesh, 4, chunk, DT)

@@ -33,6 +33,31 @@ def test_algorithmic_bytes_3d_closed_form():
     assert abs(ab["per_pt_step"] - (ab["elemforce"] + ab["update"]) / NGLOB) < 1e-9
 
 
+def test_algorithmic_bytes_3d_element_properties():
+    """kappa and mu read once per element: 8 B per element instead of 1000."""
+    per_point = bench.algorithmic_bytes_3d(NSPEC, NGLOB)
+    per_elem = bench.algorithmic_bytes_3d(NSPEC, NGLOB, element_properties=True)
+    assert per_point["elemforce"] - per_elem["elemforce"] == NSPEC * (125 * 8 - 8)
+    assert per_elem["update"] == per_point["update"]
+
+
+def test_algorithmic_bytes_2d_layouts():
+    """The 2-D element-force bytes for each geometry source and property layout,
+    written out per element (25 local points) and per point."""
+    nspec, nglob = 4000 ** 2, 16001 ** 2
+    for geometry, gbytes in (("point", 500), ("element", 32), ("control_nodes_4", 32), ("control_nodes_9", 72)):
+        for eprops, pbytes in ((False, 200), (True, 8)):
+            ab = bench.algorithmic_bytes_2d(nspec, nglob, geometry, eprops, scatter="gather")
+            assert ab["stiffness_2d"] == nspec * (25 * 12 + pbytes + gbytes) + nglob * 8
+            assert ab["update_fused"] == nspec * 25 * 12 + nglob * 44
+            col = bench.algorithmic_bytes_2d(nspec, nglob, geometry, eprops, scatter="colored")
+            assert col["stiffness_2d"] == nspec * (25 * 4 + pbytes + gbytes) + nglob * 24
+            assert col["update_fused"] == nglob * 56
+    # the data model: 130 B per point-step with per-point geometry (25/16 local points per point)
+    model = bench.algorithmic_bytes_2d(nspec, nglob)["model_per_pt_step"]
+    assert abs(model - (25 * 32 * nspec + 80 * nglob) / nglob) < 1e-9
+
+
 def _minv_bytes_brute(nx, ny, nz, absorbing):
     total = 0
     for tz in range(nz + 1):

@@ -76,6 +76,11 @@ def test_forward_seismograms_match_oracle_and_specfem2d(case, scatter):
     kinds = cfg["trace_kinds"]
     types = [{"d": "displacement", "v": "velocity", "a": "acceleration"}[k] for k in kinds]
     eng = _engine2d(mesh, cfg, types, scatter=scatter)
+    if scatter == "gather":
+        # 9-node elements with curved-capable geometry: recomputed per point
+        # from the control nodes; one material: kappa, mu once per element
+        lay = eng.layout()
+        assert lay["geometry"] == "control_nodes_9" and lay["element_properties"], lay
     eng.initialize()
     eng.run()
     ours = eng.seismograms().astype(np.float64)  # (nsig, ntypes, nrec, 2)
@@ -105,10 +110,12 @@ def test_forward_seismograms_match_oracle_and_specfem2d(case, scatter):
         assert ref <= cfg["tolerance"]
 
 
-def test_non_affine_mesh_matches_oracle():
+@pytest.mark.parametrize("scatter", ["colored", "gather"])
+def test_non_affine_mesh_matches_oracle(scatter):
     """Interior nodes moved by up to a quarter element: the geometry varies over
-    each element, so the stiffness reads it per point (affine meshes use one
-    record per element)."""
+    each element, so the colored stiffness reads it per point (affine meshes use
+    one record per element) and the gather path recomputes it at every point
+    from the element's 4 control nodes, giving the oracle's bits."""
     from specfempp_amd import Engine2D, meshes
 
     n, h = 24, 10.0
@@ -123,7 +130,8 @@ def test_non_affine_mesh_matches_oracle():
     src = [dict(x=117.3, z=131.9, angle=20.0, f0=60.0, factor=1e10, tshift=0.0)]
     rec = [dict(x=x, z=205.0) for x in (30.5, 90.2, 150.7, 210.1)]
     dt, nstep = meshes.stable_dt(0.75 * h, 3000.0), 700
-    e = Engine2D(mesh, src, rec, dt, nstep, 1, ("displacement",))
+    e = Engine2D(mesh, src, rec, dt, nstep, 1, ("displacement",), scatter=scatter)
+    lay = e.layout()
     e.initialize()
     e.run()
     ours = e.seismograms().astype(np.float64)
@@ -131,8 +139,57 @@ def test_non_affine_mesh_matches_oracle():
     s32, _ = oracle.Oracle2D(mesh, 4, 1, dt).forward(src, rec, nstep, dt, 1, ("displacement",))
     s64, _ = oracle.Oracle2D(mesh, 8, 1, dt).forward(src, rec, nstep, dt, 1, ("displacement",))
     e32, e64, floor = rel_l2(ours, s32), rel_l2(ours, s64), rel_l2(s32, s64)
-    _report("non_affine_24x24", vs_oracle_fp32=e32, vs_oracle_fp64=e64, fp32_floor=floor)
+    _report(f"non_affine_24x24/{scatter}", vs_oracle_fp32=e32, vs_oracle_fp64=e64, fp32_floor=floor)
     assert e32 <= TOL_SEISMO
+    if scatter == "gather":
+        # the one-launch batch step reads the stored per-point geometry and
+        # kappa, mu; the engine recomputes the geometry from the control nodes
+        # and reads one (kappa, mu) per element: the same bits
+        from specfempp_amd import Batch
+
+        assert lay["geometry"] == "control_nodes_4" and lay["element_properties"], lay
+        host = Engine2D(mesh, src, rec, dt, nstep, 1, ("displacement",), host_only=True)
+        b = Batch([host])
+        host.close()
+        b.set_config(5, True, "gather")
+        b.run()
+        assert np.array_equal(b.seismograms(0), ours.astype(np.float32))
+        b.close()
+
+
+def test_gather_properties_per_point_and_per_element():
+    """sfx_set_properties on the 2-D gather path: values that vary within an
+    element make the element forces read kappa and mu per point, values
+    constant over each element one pair per element; both runs match the FP32
+    oracle with the same properties."""
+    from specfempp_amd import Engine2D, meshes
+
+    n, h = 16, 20.0
+    mesh = meshes.quad_mesh(n, n, 0.0, n * h, 0.0, n * h, materials=[meshes.elastic(2000.0, 3000.0, 1700.0)],
+                            absorbing=("bottom", "right", "left"))
+    src = [dict(x=150.3, z=170.9, angle=10.0, f0=40.0, factor=1e10, tshift=0.0)]
+    rec = [dict(x=x, z=250.0) for x in (40.5, 120.2, 200.7)]
+    dt, nstep = meshes.stable_dt(h, 3300.0), 400
+    rho0, kap0, mu0, _ = oracle.Oracle2D(mesh, 4, 1, dt).properties()
+    rng = np.random.default_rng(5)
+    per_point = 1.0 + 0.05 * rng.uniform(-1.0, 1.0, size=rho0.shape)
+    per_elem = 1.0 + 0.05 * rng.uniform(-1.0, 1.0, size=(mesh.nspec, 1, 1)) * np.ones(rho0.shape)
+    for scale, element_props in ((per_point, False), (per_elem, True)):
+        props = [(np.asarray(x, dtype=np.float64) * scale).astype(np.float32) for x in (rho0, kap0, mu0)]
+        e = Engine2D(mesh, src, rec, dt, nstep, 1, ("displacement",), scatter="gather")
+        e.set_properties(*props)
+        lay = e.layout()
+        e.initialize()
+        e.run()
+        ours = e.seismograms().astype(np.float64)
+        e.close()
+        o = oracle.Oracle2D(mesh, 4, 1, dt)
+        o.set_properties(*props)
+        s32, _ = o.forward(src, rec, nstep, dt, 1, ("displacement",))
+        _report(f"gather_properties/{'element' if element_props else 'point'}", vs_oracle_fp32=rel_l2(ours, s32))
+        assert lay["element_properties"] == element_props, lay
+        assert np.abs(s32).max() > 0
+        assert rel_l2(ours, s32) <= TOL_SEISMO
 
 
 @pytest.mark.parametrize("scatter", ["colored", "gather"])

@@ -210,7 +210,12 @@ __device__ __forceinline__ void point_geometry_2d(const Geom2D &gm, int e, int p
   }
 }
 
-template <int GEOM, bool EPROP>
+// A workgroup of 128 threads computes G groups of E2 elements: thread t holds
+// local point t % 25 of element t / 25 of every group, and the loads of all
+// groups are issued before the first is used, so a wave keeps G times the
+// bytes in flight (the kernel is bound by the latency of its two dependent
+// loads, the gather index and the displacement it names).
+template <int GEOM, bool EPROP, int G>
 __global__ __launch_bounds__(128) void elemforce_2d_kernel(int nspec, const int *__restrict__ gather, Geom2D gm,
                                                            const float *__restrict__ kappa_,
                                                            const float *__restrict__ mu_,
@@ -220,33 +225,54 @@ __global__ __launch_bounds__(128) void elemforce_2d_kernel(int nspec, const int 
   // every product and sum rounded on its own: the one-launch batch step
   // performs the same operations and must round them the same way
 #pragma clang fp contract(off)
-  __shared__ float su[E2][2][25];
-  __shared__ float sF[E2][4][25];  // F(c, d) as [2c + d]
-  __shared__ __align__(16) float st[E2 * RE2_ELEM];
+  constexpr int NE = G * E2;  // elements per workgroup
+  __shared__ float su[NE][2][25];
+  __shared__ float sF[NE][4][25];  // F(c, d) as [2c + d]
+  __shared__ __align__(16) float st[NE * RE2_ELEM];
 
   const int t = threadIdx.x;
   const int el = t / 25;
   const int p = t - el * 25;
   const int iz = p / 5, ix = p - (p / 5) * 5;
-  const int e = blockIdx.x * E2 + el;
-  const bool active = (el < E2) && (e < nspec);
-  const size_t idx = static_cast<size_t>(e) * 25 + p;
-
-  float kappa = 0, mu = 0, xix = 0, xiz = 0, gammax = 0, gammaz = 0, J = 0;
-  if (active) {
-    const int g = gather[idx];
-    su[el][0][p] = uf[g];
-    su[el][1][p] = uf[np + g];
-    if constexpr (EPROP) {
-      kappa = kmu[2 * static_cast<size_t>(e)];
-      mu = kmu[2 * static_cast<size_t>(e) + 1];
-    } else {
-      kappa = kappa_[idx];
-      mu = mu_[idx];
-    }
-    point_geometry_2d<GEOM>(gm, e, p, idx, xix, xiz, gammax, gammaz, J);
+  const int first = blockIdx.x * NE;
+  // Loads are issued for every lane, inactive ones re-reading the last
+  // element (all addresses stay in bounds): with no branch around them the
+  // loads of all groups are in flight together instead of one after another.
+  int e[G];
+  bool active[G];
+  size_t idx[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    const int eg = first + g * E2 + el;
+    active[g] = (el < E2) && (eg < nspec);
+    e[g] = eg < nspec ? eg : nspec - 1;
+    idx[g] = static_cast<size_t>(e[g]) * 25 + p;
   }
-  __syncthreads();
+
+  // the gather indices, then the displacement they name, of every group
+  int gi[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) gi[g] = gather[idx[g]];
+  float ux[G], uz[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    ux[g] = uf[gi[g]];
+    uz[g] = uf[np + gi[g]];
+  }
+  float kappa[G], mu[G], xix[G], xiz[G], gammax[G], gammaz[G], J[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    if constexpr (EPROP) {
+      kappa[g] = kmu[2 * static_cast<size_t>(e[g])];
+      mu[g] = kmu[2 * static_cast<size_t>(e[g]) + 1];
+    } else {
+      kappa[g] = kappa_[idx[g]];
+      mu[g] = mu_[idx[g]];
+    }
+    point_geometry_2d<GEOM>(gm, e[g], p, idx[g], xix[g], xiz[g], gammax[g], gammaz[g], J[g]);
+  }
+  // the derivative rows of this point (lane-indexed, so vector loads; before
+  // the barrier, where their latency overlaps the loads above)
   float hpx[5], hpz[5], hwx[5], hwz[5];
 #pragma unroll
   for (int l = 0; l < 5; l++) {
@@ -256,50 +282,62 @@ __global__ __launch_bounds__(128) void elemforce_2d_kernel(int nspec, const int 
     hwz[l] = q.hpw[iz][l];
   }
   const float wx = q.w[ix], wz = q.w[iz];
-  if (active) {
+#pragma unroll
+  for (int g = 0; g < G; g++)
+    if (active[g]) {
+      su[g * E2 + el][0][p] = ux[g];
+      su[g * E2 + el][1][p] = uz[g];
+    }
+  __syncthreads();
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    if (!active[g]) continue;
+    const int le = g * E2 + el;
     // the gradient, stress and stress integrand of stiffness_2d_kernel
     float dxi[2] = {0.0f, 0.0f}, dga[2] = {0.0f, 0.0f};
 #pragma unroll
     for (int l = 0; l < 5; l++) {
 #pragma unroll
       for (int c = 0; c < 2; c++) {
-        dxi[c] += hpx[l] * su[el][c][iz * 5 + l];
-        dga[c] += hpz[l] * su[el][c][l * 5 + ix];
+        dxi[c] += hpx[l] * su[le][c][iz * 5 + l];
+        dga[c] += hpz[l] * su[le][c][l * 5 + ix];
       }
     }
     float du[2][2];
 #pragma unroll
     for (int c = 0; c < 2; c++) {
-      du[c][0] = xix * dxi[c] + gammax * dga[c];
-      du[c][1] = xiz * dxi[c] + gammaz * dga[c];
+      du[c][0] = xix[g] * dxi[c] + gammax[g] * dga[c];
+      du[c][1] = xiz[g] * dxi[c] + gammaz[g] * dga[c];
     }
-    const float lp2m = kappa + (4.0f / 3.0f) * mu;
-    const float lambda = lp2m - 2.0f * mu;
+    const float lp2m = kappa[g] + (4.0f / 3.0f) * mu[g];
+    const float lambda = lp2m - 2.0f * mu[g];
     const float sxx = lp2m * du[0][0] + lambda * du[1][1];
     const float szz = lp2m * du[1][1] + lambda * du[0][0];
-    const float sxz = mu * (du[0][1] + du[1][0]);
-    sF[el][0][p] = J * (sxx * xix + sxz * xiz);
-    sF[el][1][p] = J * (sxx * gammax + sxz * gammaz);
-    sF[el][2][p] = J * (sxz * xix + szz * xiz);
-    sF[el][3][p] = J * (sxz * gammax + szz * gammaz);
+    const float sxz = mu[g] * (du[0][1] + du[1][0]);
+    sF[le][0][p] = J[g] * (sxx * xix[g] + sxz * xiz[g]);
+    sF[le][1][p] = J[g] * (sxx * gammax[g] + sxz * gammaz[g]);
+    sF[le][2][p] = J[g] * (sxz * xix[g] + szz * xiz[g]);
+    sF[le][3][p] = J[g] * (sxz * gammax[g] + szz * gammaz[g]);
   }
   __syncthreads();
-  if (active) {
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    if (!active[g]) continue;
+    const int le = g * E2 + el;
     float t1[2] = {0.0f, 0.0f}, t2[2] = {0.0f, 0.0f};
 #pragma unroll
     for (int l = 0; l < 5; l++) {
 #pragma unroll
       for (int c = 0; c < 2; c++) {
-        t1[c] += sF[el][2 * c][iz * 5 + l] * hwx[l];
-        t2[c] += sF[el][2 * c + 1][l * 5 + ix] * hwz[l];
+        t1[c] += sF[le][2 * c][iz * 5 + l] * hwx[l];
+        t2[c] += sF[le][2 * c + 1][l * 5 + ix] * hwz[l];
       }
     }
 #pragma unroll
-    for (int c = 0; c < 2; c++) st[el * RE2_ELEM + 2 * p + c] = -(wz * t1[c] + wx * t2[c]);
+    for (int c = 0; c < 2; c++) st[le * RE2_ELEM + 2 * p + c] = -(wz * t1[c] + wx * t2[c]);
   }
   __syncthreads();
-  const int first = blockIdx.x * E2;
-  const int nel = nspec - first < E2 ? nspec - first : E2;
+  const int nel = nspec - first < NE ? nspec - first : NE;
   float4 *dst = reinterpret_cast<float4 *>(re2 + static_cast<size_t>(first) * RE2_ELEM);
   const float4 *from = reinterpret_cast<const float4 *>(st);
   for (int qv = t; qv < nel * (RE2_ELEM / 4); qv += 128) dst[qv] = from[qv];
@@ -366,28 +404,60 @@ __global__ __launch_bounds__(64) void sources_re2_kernel(int nsrc, int istep_arg
 // predictor follow: v += dt/2 a; u += dt v + dt^2/2 a; v += dt/2 a, and a is
 // not stored (nothing in the fused loop reads it; the engine recomputes it
 // from re2 with divide_only when a caller asks for it).
+// The first AU2_BATCH contributions of a point are loaded together (entry
+// indices clamped to the point's last one, so every address is valid) and
+// summed in ascending order, then any further ones one by one: the same sum
+// as a plain loop, with two dependent round trips instead of one per entry.
+// M^-1, u and v do not depend on the sum and are loaded first.
+constexpr int AU2_BATCH = 4;
+
 __global__ __launch_bounds__(256) void assemble_update_2d_kernel(int nglob, ForceSum2D fs, Fields f,
                                                                  NewmarkConst nc, int divide_only) {
 #pragma clang fp contract(off)
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nglob) return;
-  float sum[2] = {0.0f, 0.0f};
-  for (int k = fs.off[g]; k < fs.off[g + 1]; k++) {
-    const float2 r = *reinterpret_cast<const float2 *>(fs.re2 + fs.ent[k]);
-    sum[0] += r.x;
-    sum[1] += r.y;
-  }
   const int np = f.nglob_pad;
+  float minv[2], u[2] = {0.0f, 0.0f}, v[2] = {0.0f, 0.0f};
 #pragma unroll
   for (int c = 0; c < 2; c++) {
     const size_t o = static_cast<size_t>(c) * np + g;
-    const float a = sum[c] * f.minv[o];
+    minv[c] = f.minv[o];
+    if (!divide_only) {
+      u[c] = f.u[o];
+      v[c] = f.v[o];
+    }
+  }
+  const int k0 = fs.off[g], k1 = fs.off[g + 1];
+  float sum[2] = {0.0f, 0.0f};
+  if (k1 > k0) {
+    int en[AU2_BATCH];
+#pragma unroll
+    for (int j = 0; j < AU2_BATCH; j++) en[j] = fs.ent[k0 + j < k1 ? k0 + j : k1 - 1];
+    float2 r[AU2_BATCH];
+#pragma unroll
+    for (int j = 0; j < AU2_BATCH; j++) r[j] = *reinterpret_cast<const float2 *>(fs.re2 + en[j]);
+#pragma unroll
+    for (int j = 0; j < AU2_BATCH; j++)
+      if (k0 + j < k1) {
+        sum[0] += r[j].x;
+        sum[1] += r[j].y;
+      }
+    for (int k = k0 + AU2_BATCH; k < k1; k++) {
+      const float2 rr = *reinterpret_cast<const float2 *>(fs.re2 + fs.ent[k]);
+      sum[0] += rr.x;
+      sum[1] += rr.y;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const size_t o = static_cast<size_t>(c) * np + g;
+    const float a = sum[c] * minv[c];
     if (divide_only) {
       f.a[o] = a;
       continue;
     }
-    const float v1 = f.v[o] + nc.dto2 * a;
-    f.u[o] = f.u[o] + (nc.dt * v1 + nc.dt2o2 * a);
+    const float v1 = v[c] + nc.dto2 * a;
+    f.u[o] = u[c] + (nc.dt * v1 + nc.dt2o2 * a);
     f.v[o] = v1 + nc.dto2 * a;
   }
 }
@@ -1661,17 +1731,20 @@ void launch_seismograms(hipStream_t s, int dim, int npe, int nrec, int ntypes, c
                        gather, h, f, dto2, corrected, seismo, pg, q, step, nsb, sig_top, ForceSum2D{}, rec_list);
 }
 
+// element groups per workgroup of elemforce_2d
+constexpr int EF2_GROUPS = 2;
+
 template <int GEOM>
 static void launch_elemforce_2d_geom(hipStream_t s, int nspec, const int *gather, const Geom2D &gm,
                                      const float *kappa, const float *mu, const float *kmu, const GllConst &q,
                                      const float *u, int nglob_pad, float *re2) {
-  const dim3 grid(blocks(nspec, E2));
+  const dim3 grid(blocks(nspec, EF2_GROUPS * E2));
   if (kmu)
-    hipLaunchKernelGGL((elemforce_2d_kernel<GEOM, true>), grid, dim3(128), 0, s, nspec, gather, gm, kappa, mu, kmu, q,
-                       u, nglob_pad, re2);
+    hipLaunchKernelGGL((elemforce_2d_kernel<GEOM, true, EF2_GROUPS>), grid, dim3(128), 0, s, nspec, gather, gm, kappa,
+                       mu, kmu, q, u, nglob_pad, re2);
   else
-    hipLaunchKernelGGL((elemforce_2d_kernel<GEOM, false>), grid, dim3(128), 0, s, nspec, gather, gm, kappa, mu, kmu,
-                       q, u, nglob_pad, re2);
+    hipLaunchKernelGGL((elemforce_2d_kernel<GEOM, false, EF2_GROUPS>), grid, dim3(128), 0, s, nspec, gather, gm,
+                       kappa, mu, kmu, q, u, nglob_pad, re2);
 }
 
 void launch_elemforce_2d(hipStream_t s, int nspec, const int *gather, const Geom2D &gm, const float *kappa,

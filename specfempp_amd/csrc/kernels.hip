@@ -61,17 +61,23 @@ __global__ __launch_bounds__(threads_for(E)) void stiffness_2d_kernel(
   const int iz = p / 5, ix = p - (p / 5) * 5;
   const int el_idx = blockIdx.x * E + el;
   const bool active = (el < E) && (el_idx < count);
-  const int e = first + el_idx;
+  // Every lane issues its loads, inactive ones re-reading the last element of
+  // the range (all addresses valid), and all before the barrier: with no
+  // branch around them they are in flight together.
+  const int e = first + (el_idx < count ? el_idx : count - 1);
   const size_t idx = static_cast<size_t>(e) * 25 + p;
 
-  int g = 0;
-  if (active) {
-    g = gather[idx];
-    su[el][0][p] = f.u[g];
-    su[el][1][p] = f.u[f.nglob_pad + g];
-  }
-  __syncthreads();
-
+  const int g = gather[idx];
+  const float u0 = f.u[g], u1 = f.u[f.nglob_pad + g];
+  // geometry: one record per element when every element's geometry is
+  // constant over its points, else the per-point arrays
+  const size_t gr = static_cast<size_t>(e) * 8;
+  const float xix = geo ? geo[gr] : xix_[idx];
+  const float xiz = geo ? geo[gr + 1] : xiz_[idx];
+  const float gammax = geo ? geo[gr + 2] : gammax_[idx];
+  const float gammaz = geo ? geo[gr + 3] : gammaz_[idx];
+  const float J = geo ? geo[gr + 4] : jac_[idx];
+  const float kappa = kappa_[idx], mu = mu_[idx];
   // Per-thread rows of the derivative matrices, loaded once (the row index is
   // lane-dependent, so they are held in VGPRs rather than re-read).
   float hpx[5], hpz[5], hwx[5], hwz[5];
@@ -83,24 +89,13 @@ __global__ __launch_bounds__(threads_for(E)) void stiffness_2d_kernel(
     hwz[l] = q.hpw[iz][l];
   }
   const float wx = q.w[ix], wz = q.w[iz];
-  float xix = 0, xiz = 0, gammax = 0, gammaz = 0, J = 0, kappa = 0, mu = 0;
   if (active) {
-    if (geo) {  // every element's geometry is constant over its points: one record per element
-      const float *ge = geo + static_cast<size_t>(e) * 8;
-      xix = ge[0];
-      xiz = ge[1];
-      gammax = ge[2];
-      gammaz = ge[3];
-      J = ge[4];
-    } else {
-      xix = xix_[idx];
-      xiz = xiz_[idx];
-      gammax = gammax_[idx];
-      gammaz = gammaz_[idx];
-      J = jac_[idx];
-    }
-    kappa = kappa_[idx];
-    mu = mu_[idx];
+    su[el][0][p] = u0;
+    su[el][1][p] = u1;
+  }
+  __syncthreads();
+
+  if (active) {
     float dxi[2] = {0.0f, 0.0f}, dga[2] = {0.0f, 0.0f};
 #pragma unroll
     for (int l = 0; l < 5; l++) {

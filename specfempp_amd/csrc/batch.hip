@@ -203,6 +203,8 @@ struct StepState {
 };
 
 template <int E> constexpr int step_threads() { return (E * 25 + 63) / 64 * 64; }
+// contributions of a point loaded together before the sum (the rest one by one)
+constexpr int STEP_BATCH = 4;
 
 template <int E>
 __global__ __launch_bounds__(step_threads<E>()) void batch_step_kernel(int par, int nel, StepTables T, StepState S,
@@ -228,33 +230,78 @@ __global__ __launch_bounds__(step_threads<E>()) void batch_step_kernel(int par, 
   const float *UA = S.u[par], *VA = S.v[par], *reA = S.re[par];
   float *UB = S.u[par ^ 1], *VB = S.v[par ^ 1], *reB = S.re[par ^ 1];
 
+  // Every lane issues all of its loads here, before the first barrier and
+  // without branches around them (lanes past the last element re-read it; all
+  // addresses are valid): the element's simulation and the point's gather
+  // index, then the point's contribution bounds, state and M^-1, the
+  // simulation's constants, the first STEP_BATCH contributions and the
+  // element's properties and geometry, so their latencies overlap.
+  const int ec = e < nel ? e : nel - 1;
+  const size_t idx = static_cast<size_t>(ec) * 25 + p;
+  const int sim_l = T.elem_sim[ec];
+  const int g = T.gather[idx];
+  const int ns = T.nstep[sim_l], nsb = T.nsb[sim_l];
+  const float dt = T.ncs[3 * sim_l], dto2 = T.ncs[3 * sim_l + 1], dt2o2 = T.ncs[3 * sim_l + 2];
+  const int k0 = T.off[g], k1 = T.off[g + 1];
+  float minv[2], ua[2], va[2];
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const size_t o = static_cast<size_t>(c) * np + g;
+    minv[c] = T.minv[o];
+    ua[c] = UA[o];
+    va[c] = VA[o];
+  }
+  int en[STEP_BATCH];
+#pragma unroll
+  for (int j = 0; j < STEP_BATCH; j++) en[j] = T.ent[k0 + j < k1 ? k0 + j : k1 - 1];
+  float2 rj[STEP_BATCH];
+#pragma unroll
+  for (int j = 0; j < STEP_BATCH; j++) rj[j] = *reinterpret_cast<const float2 *>(reA + en[j]);
+  const float kappa = T.kappa[idx], mu = T.mu[idx];
+  const size_t gr = static_cast<size_t>(ec) * 8;
+  const float xix = T.geo ? T.geo[gr] : T.xix[idx];
+  const float xiz = T.geo ? T.geo[gr + 1] : T.xiz[idx];
+  const float gammax = T.geo ? T.geo[gr + 2] : T.gammax[idx];
+  const float gammaz = T.geo ? T.geo[gr + 3] : T.gammaz[idx];
+  const float J = T.geo ? T.geo[gr + 4] : T.jac[idx];
+  float hpx[5], hpz[5], hwx[5], hwz[5];
+#pragma unroll
+  for (int l = 0; l < 5; l++) {
+    hpx[l] = q.hp[ix][l];
+    hpz[l] = q.hp[iz][l];
+    hwx[l] = q.hpw[ix][l];
+    hwz[l] = q.hpw[iz][l];
+  }
+  const float wx = q.w[ix], wz = q.w[iz];
+
   int sim = 0;
   bool forces = false, sample = false;
   if (inside) {
-    sim = T.elem_sim[e];
-    const int ns = T.nstep[sim];
+    sim = sim_l;
     forces = n < ns;
-    sample = n >= 1 && n - 1 < ns && (n - 1) % T.nsb[sim] == 0;
+    sample = n >= 1 && n - 1 < ns && (n - 1) % nsb == 0;
   }
-  const size_t idx = static_cast<size_t>(e) * 25 + p;
   float un[2] = {0.0f, 0.0f}, vn[2] = {0.0f, 0.0f};
   if (forces || sample) {
-    const float dt = T.ncs[3 * sim], dto2 = T.ncs[3 * sim + 1], dt2o2 = T.ncs[3 * sim + 2];
-    const int g = T.gather[idx];
-    const int k0 = T.off[g], k1 = T.off[g + 1];
     float sum[2] = {0.0f, 0.0f};
-    for (int k = k0; k < k1; k++) {
+#pragma unroll
+    for (int j = 0; j < STEP_BATCH; j++)
+      if (k0 + j < k1) {
+        sum[0] += rj[j].x;
+        sum[1] += rj[j].y;
+      }
+    for (int k = k0 + STEP_BATCH; k < k1; k++) {
       const float2 r = *reinterpret_cast<const float2 *>(reA + T.ent[k]);
       sum[0] += r.x;
       sum[1] += r.y;
     }
-    const bool owner = forces && T.ent[k0] == e * RE2_ELEM + 2 * p;
+    const bool owner = forces && en[0] == e * RE2_ELEM + 2 * p;
 #pragma unroll
     for (int c = 0; c < 2; c++) {
       const size_t o = static_cast<size_t>(c) * np + g;
-      const float a = sum[c] * T.minv[o];
-      const float u0 = UA[o];
-      const float v1 = VA[o] + dto2 * a;
+      const float a = sum[c] * minv[c];
+      const float u0 = ua[c];
+      const float v1 = va[c] + dto2 * a;
       un[c] = u0 + (dt * v1 + dt2o2 * a);
       vn[c] = v1 + dto2 * a;
       if (sample) {
@@ -299,32 +346,7 @@ __global__ __launch_bounds__(step_threads<E>()) void batch_step_kernel(int par, 
     }
   }
 
-  float hpx[5], hpz[5], hwx[5], hwz[5];
-#pragma unroll
-  for (int l = 0; l < 5; l++) {
-    hpx[l] = q.hp[ix][l];
-    hpz[l] = q.hp[iz][l];
-    hwx[l] = q.hpw[ix][l];
-    hwz[l] = q.hpw[iz][l];
-  }
-  const float wx = q.w[ix], wz = q.w[iz];
-  float xix = 0, xiz = 0, gammax = 0, gammaz = 0, J = 0;
   if (forces) {
-    const float kappa = T.kappa[idx], mu = T.mu[idx];
-    if (T.geo) {
-      const float *ge = T.geo + static_cast<size_t>(e) * 8;
-      xix = ge[0];
-      xiz = ge[1];
-      gammax = ge[2];
-      gammaz = ge[3];
-      J = ge[4];
-    } else {
-      xix = T.xix[idx];
-      xiz = T.xiz[idx];
-      gammax = T.gammax[idx];
-      gammaz = T.gammaz[idx];
-      J = T.jac[idx];
-    }
     float dxi[2] = {0.0f, 0.0f}, dga[2] = {0.0f, 0.0f};
 #pragma unroll
     for (int l = 0; l < 5; l++) {

@@ -482,21 +482,22 @@ __global__ __launch_bounds__(128) void stiffness_pair_2d_kernel(
   const int iz = p / 5, ix = p - (p / 5) * 5;
   const int el_idx = blockIdx.x * E2 + el;
   const bool active = (el < E2) && (el_idx < count);
-  const int e = first + el_idx;
+  // every lane loads (inactive ones re-read the range's last element; all
+  // addresses valid), all before the barrier and without branches, so the
+  // loads are in flight together
+  const int e = first + (el_idx < count ? el_idx : count - 1);
   const size_t idx = static_cast<size_t>(e) * 25 + p;
   const int np = adj.nglob_pad;
 
-  int g = 0;
-  if (active) {
-    g = gather[idx];
+  const int g = gather[idx];
+  float ua[2], ub[2];
 #pragma unroll
-    for (int c = 0; c < 2; c++) {
-      su[0][el][c][p] = adj.u[c * np + g];
-      su[1][el][c][p] = bwd.u[c * np + g];
-    }
+  for (int c = 0; c < 2; c++) {
+    ua[c] = adj.u[c * np + g];
+    ub[c] = bwd.u[c * np + g];
   }
-  __syncthreads();
-
+  const float xix = xix_[idx], xiz = xiz_[idx], gammax = gammax_[idx], gammaz = gammaz_[idx], J = jac_[idx];
+  const float kappa = kappa_[idx], mu = mu_[idx];
   float hpx[5], hpz[5], hwx[5], hwz[5];
 #pragma unroll
   for (int l = 0; l < 5; l++) {
@@ -507,8 +508,15 @@ __global__ __launch_bounds__(128) void stiffness_pair_2d_kernel(
   }
   const float wx = q.w[ix], wz = q.w[iz];
   if (active) {
-    const float xix = xix_[idx], xiz = xiz_[idx], gammax = gammax_[idx], gammaz = gammaz_[idx], J = jac_[idx];
-    const float kappa = kappa_[idx], mu = mu_[idx];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      su[0][el][c][p] = ua[c];
+      su[1][el][c][p] = ub[c];
+    }
+  }
+  __syncthreads();
+
+  if (active) {
     const float lp2m = kappa + (4.0f / 3.0f) * mu;
     const float lambda = lp2m - 2.0f * mu;
     float du[2][2][2];  // [field][c][j]
